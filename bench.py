@@ -1,0 +1,136 @@
+#!/usr/bin/env python3
+"""Throughput of the SPH step pipeline on one MI355X.
+
+Workload: coupled thermo-mechanical Johnson-Cook cylinder compression,
+R 15 mm, H 30 mm, dx 0.3 mm (759,132 particles from the reference's
+AddCylinderLength), aluminium-like JC with FP64 Newton return map,
+Taylor-Quinney 0.9 plastic heating with conduction, top layers at -2 m/s.
+A step is one full pass of the Fraser step (density rate + update, rate
+tensors, pressure + stress + return map + energy, acceleration,
+BC + integration, conduction + leapfrog, neighbour rebuild every 5 steps
+once yielding), all inputs resident in HBM.
+
+Prints ONE JSON line: value = particle-updates/s (particles x steps / s),
+plus the HBM roofline of the dominant kernel (HIP-event durations on the
+library's stream x algorithmic bytes) and the FP64 CPU oracle timed on the
+host cores on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-updates/s: mixed+surrogate vs plain FP64 @ stated error; HBM % peak"
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per particle-update of each kernel (FP64 plain path):
+# compulsory SoA reads + writes, neighbour data counted once, list excluded
+ALG_BYTES_FP64 = {"dens": 80, "rates": 144, "stress": 384, "accel": 144, "integrate": 172,
+                  "thermal": 120}
+
+
+def cpu_baseline(sc, warm=1, steps=2):
+    """FP64 oracle (OpenMP over particles) on the same workload."""
+    import numpy as np  # noqa: F401
+    from oracle import pyoracle as po
+    st = po.State(sc.n)
+    for k, v in sc.fields.items():
+        st[k] = v
+    st["xs"] = sc.fields["x"]
+    st.nsearch(sc.cfg)
+    st.step(sc.cfg, sc.bcs, warm)
+    t0 = time.perf_counter()
+    st.step(sc.cfg, sc.bcs, steps)
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    return {"value": sc.n * steps / dt, "unit": "particle-updates/s", "cores": cores,
+            "kind": "port",
+            "sample": f"{steps} full steps of the same {sc.n}-particle workload after "
+                      f"{warm} warm-up step(s), FP64 oracle, OpenMP"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--timing-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--scale", type=float, default=1.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path is "
+              "single-GPU in this build)", file=sys.stderr)
+        sys.exit(2)
+
+    from weldformgpu_amd import scenarios
+    from weldformgpu_amd.domain import Domain
+
+    sc = scenarios.c3_johnson_cook(scale=args.scale)
+    n = sc.n
+    print(f"[bench] workload {sc.name}: {n} particles, dt={sc.cfg.dt:.4e}", file=sys.stderr,
+          flush=True)
+    dom = Domain(sc.cfg)
+    scenarios.load_domain(dom, sc)
+    dom.step(args.warmup)
+    dom.sync()
+    t0 = time.perf_counter()
+    dom.step(args.steps)
+    dom.sync()
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    rate = n * args.steps / (t1 - t0)
+    print(f"[bench] timed {args.steps} steps: {ms_step:.3f} ms/step", file=sys.stderr, flush=True)
+
+    # per-kernel HIP-event durations on the library stream
+    dom.set_timing(True)
+    dom.step(args.timing_steps)
+    kern = {}
+    for k, b in ALG_BYTES_FP64.items():
+        ms, cnt = dom.kernel_time(k)
+        if cnt:
+            avg = ms / cnt
+            kern[k] = {"avg_ms": avg, "launches": cnt, "alg_bytes": b * n,
+                       "GBs": b * n / (avg * 1e-3) / 1e9}
+    ms_ns, cnt_ns = dom.kernel_time("nsearch")
+    if cnt_ns:
+        kern["nsearch"] = {"avg_ms": ms_ns / cnt_ns, "launches": cnt_ns}
+    dom.set_timing(False)
+    stats = dom.stats()
+    dom.close()
+    dom_k = max((k for k in kern if k in ALG_BYTES_FP64), key=lambda k: kern[k]["avg_ms"])
+    dk = kern[dom_k]
+    roof = {"kernel": dom_k, "bound": "hbm", "achieved": dk["GBs"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": dk["GBs"] / HBM_PEAK_GBS, "traffic": None,
+            "alg_bytes_per_particle": ALG_BYTES_FP64[dom_k], "avg_ms": dk["avg_ms"]}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(sc, steps=args.cpu_steps)
+
+    out = {
+        "metric": METRIC, "value": rate, "unit": "particle-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{sc.name}: JC thermo-mechanical cylinder R15mm H30mm "
+                               f"dx0.3mm, {n} particles, FP64 pair loops, FP64 Newton return "
+                               "map, conduction + Taylor-Quinney 0.9, top at -2 m/s",
+                   "particles": n, "precision": "fp64", "surrogate": False,
+                   "parallelism": "single-gpu"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernels": kern,
+        "run": {"newton_calls": stats["newton_calls"], "newton_iters": stats["newton_iters"],
+                "ns_builds": stats["ns_builds"], "is_yielding": stats["is_yielding"]},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,276 @@
+/*
+ * wf_abi.h — C-ABI of libwfhip, the gfx950 HIP implementation of the
+ * WeldFormGPU per-step SPH particle pipeline (neighbour search, pair loops,
+ * stress/plasticity update, integration + velocity BCs, thermal conduction,
+ * internal energy).
+ *
+ * The entry points replace the reference's Domain_d device call surface
+ * (src/cuda/Domain_d.cuh:101-426) one for one; each declaration cites the
+ * reference kernel or method it stands in for.  The ABI is plain C: opaque
+ * handle, plain pointers and sizes, int status codes, no C++ types and no
+ * exceptions across it.
+ *
+ * Conventions
+ *  - All per-particle arrays crossing the ABI are in the ORIGINAL particle
+ *    order (the order the caller generated them in).  The library reorders
+ *    particles internally along a Morton curve of the neighbour-search cells
+ *    and keeps the permutation; wf_get_field undoes it.
+ *  - Vectors are interleaved xyz (double[3*n]) like the reference's double3
+ *    arrays; symmetric and antisymmetric tensors are 6-component flat
+ *    [xx,yy,zz,xy,yz,xz] (reference ToFlatSymPtr, src/cuda/tensor3.cu:95-100).
+ *  - All calls enqueue on the handle's own HIP stream and return; wf_sync
+ *    waits.  Field transfers and queries are synchronous.
+ *  - A handle is not thread-safe (like the reference's Domain_d).
+ *  - Return value 0 = WF_OK, negative = WF_E_*; wf_last_error() gives the
+ *    message of the last failure on the calling thread.
+ */
+#ifndef WF_ABI_H
+#define WF_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WF_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+  WF_OK = 0,
+  WF_E_ARG = -1,         /* bad argument (null, size mismatch, unknown enum) */
+  WF_E_HIP = -2,         /* HIP runtime error                                */
+  WF_E_STATE = -3,       /* call not valid in the current state              */
+  WF_E_NBCAP = -4,       /* a particle has more than WF_MAX_NEIGHBORS        */
+  WF_E_NONFINITE = -5,   /* NaN/Inf detected in a checked field              */
+  WF_E_NOMEM = -6,       /* device allocation failed                         */
+  WF_E_UNSUPPORTED = -7  /* option not built / not supported                 */
+};
+
+/* cap on neighbours per particle: cuNSearchKernels.cuh:6 (CUDA_MAX_NEIGHBORS) */
+#define WF_MAX_NEIGHBORS 70
+
+/* ---- material models: src/cuda/Material.cuh:6-8 ------------------------- */
+enum { WF_MAT_BILINEAR = 0, WF_MAT_HOLLOMON = 1, WF_MAT_JOHNSON_COOK = 2 };
+
+/* ---- arithmetic of the pair loops -------------------------------------- */
+enum {
+  WF_FP64 = 0,  /* everything in FP64 (the reference's arithmetic)          */
+  WF_MIXED = 1  /* FP32 pair loops with compensated sums and cell-relative
+                   coordinates; FP64 integration, return map and reductions */
+};
+
+/* ---- plastic return map ------------------------------------------------- */
+enum {
+  WF_RETURN_REFERENCE = 0, /* scale back with old sigma_y, explicit re-eval,
+                              dep=(s_trial-s_y)/(3G): Mechanical.cu:504-535 */
+  WF_RETURN_NEWTON = 1     /* FP64 Newton radial return on
+                              s_trial - 3G*dg - s_y(ep+dg) = 0 (extension)  */
+};
+
+/* ---- reference quirks reproduced by default (bit flags) ---------------- */
+#define WF_Q_ROT_XZ_SIGN   0x1 /* W.xz negated: InteractionAlt.cu:362           */
+#define WF_Q_TENSOR_PRODUCT 0x2 /* tensor3 operator* takes A*B in the first column
+                                  and A*B^T elsewhere: tensor3.cu:241-255       */
+#define WF_Q_STALE_NS_POS  0x4 /* rebuild from the last output snapshot x_h:
+                                  SolverFraser.cu:171-175                       */
+#define WF_Q_ALL (WF_Q_ROT_XZ_SIGN | WF_Q_TENSOR_PRODUCT | WF_Q_STALE_NS_POS)
+
+/* ---- thermal boundary codes (per particle BC_T) ------------------------- */
+enum {
+  WF_BCT_NONE = 0,
+  WF_BCT_CONVECTION = 1, /* CalcConvHeatKernel, Thermal.cu:36-64             */
+  WF_BCT_FIXED = 2       /* Dirichlet: temperature held (extension)         */
+};
+
+/* ---- material record: src/cuda/Material.cuh:24-93; the JSON "const" array
+ *      order is WeldFormGPU.cu:753-792 ------------------------------------- */
+typedef struct wf_material_params {
+  int model;       /* WF_MAT_*                                              */
+  int _pad0;
+  double E, nu;    /* youngsModulus, poissonsRatio                          */
+  double sy0;      /* yieldStress0                                          */
+  double Et;       /* bilinear tangent modulus, const[0]                    */
+  double K_h, m_h; /* Hollomon K, m                                         */
+  double A, B, n, C, eps0, m, Tm, Tt; /* Johnson-Cook const[0..7]            */
+} wf_material_params;
+
+typedef struct wf_config {
+  int abi_version;       /* WF_ABI_VERSION                                  */
+  int n_particles;       /* particle_count                                  */
+  int solid_count;       /* solid_part_count (== n_particles, no contact)   */
+  int precision;         /* WF_FP64 / WF_MIXED                              */
+  double dt;             /* deltat                                          */
+  double alpha, beta;    /* artifViscAlpha / artifViscBeta                  */
+  double h_glob;         /* search radius is 2*h_glob (SolverFraser.cu:72)  */
+  wf_material_params mat;
+  int return_map;        /* WF_RETURN_*                                     */
+  int quirks;            /* WF_Q_* bit set                                  */
+  int mechanics;         /* 1: run the mechanical pipeline in wf_step       */
+  int thermal;           /* 1: conduction + leapfrog (ThermalCalcs)         */
+  double taylor_quinney; /* fraction of plastic work to heat (0 = off)      */
+  double T_inf, h_conv;  /* convection BC parameters (Thermal.cu:36-64)    */
+  int ns_interval;       /* rebuild every k steps once yielding (5)         */
+  int surrogate;         /* 1: MLP initial guess for the Newton return map  */
+  double ns_min_plastic; /* yielding threshold on max eps_p (1e-6)          */
+  int newton_max_iter;   /* Newton iteration cap (default 50)               */
+  int mlp_train_interval;/* online training cadence in steps (200)          */
+  double newton_tol;     /* |residual| <= newton_tol * s_trial              */
+} wf_config;
+
+/* Fill *cfg with the reference's defaults: alpha=1, beta=0
+ * (Domain_d.cu:191-192), ns_interval=5, ns_min_plastic=1e-6
+ * (SolverFraser.cu:17,91), all quirks on, FP64, reference return map. */
+void wf_config_default(wf_config* cfg);
+
+/* ---- fields ------------------------------------------------------------- */
+enum {
+  /* double[3*n] */
+  WF_F_X = 0, WF_F_V, WF_F_A, WF_F_U, WF_F_VA, WF_F_X_SNAPSHOT,
+  /* double[n] */
+  WF_F_RHO = 16, WF_F_RHO0, WF_F_DRHO, WF_F_M, WF_F_H, WF_F_P, WF_F_CS, WF_F_G,
+  WF_F_SIGMA_Y, WF_F_PL_STRAIN, WF_F_SIGMA_EQ, WF_F_EFF_STRAIN_RATE,
+  WF_F_INT_ENERGY, WF_F_T, WF_F_TA, WF_F_TB, WF_F_DTDT, WF_F_K_T, WF_F_CP_T,
+  WF_F_Q_PLASTIC,
+  /* double[6*n] */
+  WF_F_SIGMA = 48, WF_F_SHEARSTRESS, WF_F_STRAIN, WF_F_STRRATE, WF_F_ROTRATE,
+  /* int32[n] */
+  WF_F_ID = 64, WF_F_BC_T,
+  WF_F_COUNT_
+};
+
+/* Components per particle and element byte size of a field; 0 if unknown. */
+int wf_field_shape(int field, int* components, int* elem_bytes);
+
+/* ---- lifecycle: SetDimension (Domain_d.cu:19-234) ----------------------- */
+int wf_create(const wf_config* cfg, struct wf_domain** out);
+int wf_destroy(struct wf_domain* d);
+const char* wf_last_error(void);
+int wf_sync(struct wf_domain* d);
+
+/* Upload / download a whole field (original order).  bytes must equal
+ * n_particles * components * elem_bytes.  Replaces the Set* setters
+ * (Domain_d.cu:307-477) and the cudaMemcpy's of WeldFormGPU.cu:694-841. */
+int wf_set_field(struct wf_domain* d, int field, const void* host, size_t bytes);
+int wf_get_field(struct wf_domain* d, int field, void* host, size_t bytes);
+/* Fill a double field with a constant (SetDensity, Set_h, SetShearModulus,
+ * SetSigmay ... Domain_d.cu:307-477). */
+int wf_fill_field(struct wf_domain* d, int field, double value);
+
+/* Copy x into the neighbour-search snapshot x_h, as the reference does at
+ * output times (SolverFraser.cu:323-325). */
+int wf_snapshot_positions(struct wf_domain* d);
+
+/* Velocity boundary conditions applied by wf_step (UserAcc,
+ * WeldFormGPU.cu:78-140; boundaryCondition, Boundary_Condition.cuh:5-14).
+ * At most WF_MAX_BCS; applied in order, so the last matching zone wins. */
+#define WF_MAX_BCS 16
+int wf_set_bcs(struct wf_domain* d, int count, const int* zone_ids,
+               const double* values /* 3*count */);
+
+/* ---- neighbour search: cuNSearch::NeighborhoodSearch::find_neighbors
+ *      (src/cuNSearch/src/cuNSearch.cu:111-137).  Radius 2*h_glob, FP32
+ *      criterion 0 < |xj-xi|^2 < r^2 (cuNSearchKernels.cu:163-164), cap 70.
+ *      Positions come from the snapshot (WF_Q_STALE_NS_POS) or from x. ---- */
+int wf_nsearch_build(struct wf_domain* d);
+/* Total number of stored neighbour entries and particles over the cap. */
+int wf_nsearch_stats(struct wf_domain* d, int64_t* total, int* over_cap);
+/* CSR of the current lists in ORIGINAL indices, each row sorted ascending.
+ * counts[n], offsets[n+1], list[list_cap]. */
+int wf_get_neighbors(struct wf_domain* d, int* counts, int* offsets, int* list,
+                     int64_t list_cap);
+
+/* ---- the per-step entry points (reference kernels) ---------------------- */
+/* CalcDensIncKernel, InteractionAlt.cu:153-246                             */
+int wf_calc_dens_inc(struct wf_domain* d);
+/* UpdateDensityKernel, Mechanical.cu:194-203                               */
+int wf_update_density(struct wf_domain* d, double dt);
+/* CalcRateTensorsKernel, InteractionAlt.cu:251-383                         */
+int wf_calc_rate_tensors(struct wf_domain* d);
+/* PressureKernelExt, Mechanical.cu:289-298 (EOS case 0, Functions.cu:211)  */
+int wf_pressure(struct wf_domain* d);
+/* StressStrainKickDriftKernel -> StressStrainOne, Mechanical.cu:458-565,
+ * plus the per-particle internal-energy increment of CalcIntEnergy
+ * (Energy.cu:5-19, per-particle indexing corrected).                       */
+int wf_stress_strain(struct wf_domain* d);
+/* CalcAccelKernel, InteractionAlt.cu:5-150 (KernelType 0, XSPH 0)          */
+int wf_calc_accel(struct wf_domain* d);
+/* ApplyBCVelKernel, Mechanical.cu:572-589                                  */
+int wf_apply_bc_vel(struct wf_domain* d, int zone_id, double vx, double vy,
+                    double vz);
+/* UpdatePosFraserKernel, Mechanical.cu:179-191                             */
+int wf_update_pos_fraser(struct wf_domain* d, double dt);
+/* UpdateVelKernel, Mechanical.cu:142-152                                   */
+int wf_update_vel(struct wf_domain* d, double dt);
+/* ThermalSolveKernel, Thermal.cu:101-142 (the GPU form of
+ * Domain::CalcTempInc, Thermal_ONLY4REF_NOTUSE.cpp:10-78), using the
+ * mechanical neighbour lists; then CalcConvHeatKernel (Thermal.cu:36-64)
+ * and the plastic-work heat source.                                         */
+int wf_calc_temp_inc(struct wf_domain* d);
+/* Domain_d::ThermalCalcs, Thermal.cu:68-99: wf_calc_temp_inc followed by
+ * TempCalcLeapfrogFirst / TempCalcLeapfrog (Thermal.cu:16-31).              */
+int wf_thermal_calcs(struct wf_domain* d, double dt);
+/* FP64 deterministic sum of the per-particle internal energy
+ * (Energy.cu:17 int_energy_sum, without its race).                         */
+int wf_int_energy(struct wf_domain* d, double* sum);
+
+/* ---- fused step: Domain_d::MechFraserSolve loop body, SolverFraser.cu:164-399:
+ *      rebuild policy, dens-inc + density update, rate tensors, pressure +
+ *      stress, accel, BC, position/velocity update, BC, thermal, energy. */
+int wf_step(struct wf_domain* d, int n_steps);
+
+/* ---- run statistics ------------------------------------------------------ */
+typedef struct wf_stats {
+  int64_t step;            /* steps taken by wf_step                        */
+  double time;             /* simulated time                                */
+  int is_yielding;         /* max eps_p has exceeded ns_min_plastic         */
+  int ns_builds;           /* neighbour rebuilds                            */
+  int64_t newton_iters;    /* total Newton iterations (all particles)       */
+  int64_t newton_calls;    /* particles that ran the Newton return map      */
+  int64_t newton_iters_wave; /* sum over wavefronts of their max iterations */
+  int mlp_trainings;       /* online training rounds                        */
+  int nb_over_cap;         /* particles whose candidates exceeded the cap   */
+  int _pad;
+} wf_stats;
+int wf_get_stats(struct wf_domain* d, wf_stats* out);
+
+/* ---- per-kernel timing with HIP events on the handle's stream ----------- */
+/* Enable (1) / disable (0) event timing; resets the accumulators.           */
+int wf_set_timing(struct wf_domain* d, int on);
+/* Kernel ids for wf_kernel_time. */
+enum {
+  WF_K_DENS = 0,    /* density-rate pair loop + density update      */
+  WF_K_RATES,       /* strain/rotation-rate pair loop               */
+  WF_K_STRESS,      /* pressure + stress/plasticity + energy        */
+  WF_K_ACCEL,       /* acceleration pair loop                       */
+  WF_K_INTEGRATE,   /* BC + position/velocity update + BC           */
+  WF_K_THERMAL,     /* conduction pair loop + leapfrog              */
+  WF_K_NSEARCH,     /* neighbour rebuild (all its kernels)          */
+  WF_K_MLP,         /* surrogate training                           */
+  WF_K_COUNT_
+};
+/* Accumulated milliseconds and launch count of kernel id k. */
+int wf_kernel_time(struct wf_domain* d, int k, double* total_ms, int64_t* launches);
+
+/* ---- host-side input builders (no device needed) ---------------------------
+ * Lattice generators and zone assignment of the host domain build, in the
+ * reference's generation order and floating-point arithmetic, so particle
+ * counts and positions match a reference run of the same deck.
+ * Each returns the particle count; xyz (3*count doubles) is filled when
+ * non-null and cap is large enough; *mass_out gets the per-particle mass. */
+/* Domain::AddBoxLength, cubic packing (Domain.cpp:194-329)                  */
+int wf_gen_box(const double start[3], double Lx, double Ly, double Lz, double r,
+               double density, double* xyz, int cap, double* mass_out);
+/* Domain::AddCylinderLength (Domain.cpp:490-574, calcHalfPartCount :473-488) */
+int wf_gen_cylinder(const double start[3], double R, double Lz, double r,
+                    double density, double* xyz, int cap, double* mass_out);
+/* Domain::AssignZone (Domain.cpp:989-1005): ID[a] = id for particles inside
+ * the closed box [start,end]; returns how many.                             */
+int wf_assign_zone(const double* xyz, int n, const double start[3],
+                   const double end[3], int id, int* ID);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WF_ABI_H */

@@ -1,0 +1,114 @@
+/*
+ * oracle.h — FP64 CPU restatement of the WeldFormGPU per-step SPH pipeline.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker the HIP product is
+ * compared against; only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product (libwfhip) never links or calls
+ * it.
+ *
+ * It restates, in source order and FP64 without FMA contraction (the
+ * reference builds with --fmad=false --prec-div=true --prec-sqrt=true,
+ * CMakeLists.txt:29), the arithmetic of:
+ *   src/Domain.cpp:194-398, 473-574, 989-1005   lattice generators, zones
+ *   src/cuNSearch/src/cuNSearchKernels.cu:125-233 FP32 neighbour criterion
+ *   src/cuda/Functions.cu:27-283                  cubic-spline kernel, EOS
+ *   src/cuda/InteractionAlt.cu:5-383              the three pair loops
+ *   src/cuda/tensor3.cu:62-312                    tensor helpers
+ *   src/cuda/Mechanical.cu:142-203,289-298,458-589 per-particle kernels
+ *   src/cuda/Material.cuh:192-242                 Hollomon / Johnson-Cook
+ *   src/cuda/Thermal.cu:16-142                    conduction, convection, leapfrog
+ *   src/cuda/Energy.cu:5-19                       internal energy
+ *   src/cuda/SolverFraser.cu:30-409               step order and rebuild policy
+ * All arrays are in the caller's (original) particle order.
+ */
+#ifndef WF_ORACLE_H
+#define WF_ORACLE_H
+
+#include <stdint.h>
+#include "../include/wf_abi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct or_state {
+  int n, solid;
+  double *x, *v, *a, *u, *va, *xs;                 /* 3n; xs = snapshot x_h */
+  double *rho, *rho0, *drho, *m, *h, *p, *Cs, *G;  /* n */
+  double *sigma_y, *pl_strain, *sigma_eq, *eff_strain_rate, *int_energy;
+  double *T, *Ta, *Tb, *dTdt, *k_T, *cp_T, *q_plastic;
+  double *sigma, *shearstress, *strain, *strrate, *rotrate; /* 6n */
+  int *ID, *BC_T;
+  int *nb_count, *nb_offset; /* n, n+1 */
+  int *nb_list;
+  int64_t nb_cap;
+  /* Newton statistics of the last or_stress_strain call */
+  int64_t newton_iters, newton_calls;
+} or_state;
+
+typedef struct or_run {
+  double time;
+  int64_t step;
+  int ts_i;
+  int is_yielding;
+  int isfirst_step;
+  int ns_builds;
+  double max_pl_strain;
+  int nb_over_cap;
+} or_run;
+
+/* geometry (Domain.cpp) ----------------------------------------------------*/
+int or_gen_box(const double V[3], double Lx, double Ly, double Lz, double r,
+               double* xyz, int cap, double density, double* mass_out);
+int or_gen_cylinder(const double V[3], double Rxy, double Lz, double r,
+                    double* xyz, int cap, double density, double* mass_out);
+int or_half_part_count(double r, double R, int xinc);
+int or_assign_zone(const double* xyz, int n, const double start[3],
+                   const double end[3], int id, int* ID);
+
+/* neighbour search (FP32 criterion); rows sorted ascending -----------------*/
+int64_t or_neighbors(const double* xyz, int n, double radius, int* counts,
+                     int* offsets, int* list, int64_t cap, int* over_cap);
+int64_t or_neighbors_bruteforce(const double* xyz, int n, double radius,
+                                int* counts, int* offsets, int* list,
+                                int64_t cap);
+
+/* math helpers ------------------------------------------------------------*/
+double or_kernel(double q, double h);
+double or_grad_kernel(double q, double h);
+double or_jc_yield(const wf_material_params* mp, double ep, double eps_rate,
+                   double T);
+double or_jc_hardening(const wf_material_params* mp, double ep,
+                       double eps_rate, double T);
+double or_hollomon_yield(const wf_material_params* mp, double ep);
+/* Newton radial return on s_trial - 3G dg - s_y(ep+dg) = 0 for JC. */
+double or_jc_newton(const wf_material_params* mp, double s_trial, double G,
+                    double ep, double eps_rate, double T, double g0,
+                    double tol, int max_iter, int* iters);
+
+/* per-step kernels --------------------------------------------------------*/
+void or_calc_dens_inc(or_state* s, const wf_config* c);
+void or_update_density(or_state* s, double dt);
+void or_calc_rate_tensors(or_state* s, const wf_config* c);
+void or_pressure(or_state* s);
+void or_stress_strain(or_state* s, const wf_config* c);
+void or_calc_accel(or_state* s, const wf_config* c);
+void or_apply_bc_vel(or_state* s, int zone, double vx, double vy, double vz);
+void or_update_pos_fraser(or_state* s, double dt);
+void or_update_vel(or_state* s, double dt);
+void or_calc_temp_inc(or_state* s, const wf_config* c);
+void or_thermal_calcs(or_state* s, const wf_config* c, int* isfirst_step);
+void or_int_energy(or_state* s, const wf_config* c);
+double or_int_energy_sum(const or_state* s);
+
+/* neighbour rebuild from the snapshot (or x) into s->nb_* */
+int or_nsearch_build(or_state* s, const wf_config* c, int* over_cap);
+
+/* one Fraser step (SolverFraser.cu:164-399) */
+int or_step(or_state* s, const wf_config* c, or_run* run, int nbc,
+            const int* bc_zone, const double* bc_val);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,247 @@
+"""ctypes wrapper of oracle/liboracle.so — the FP64 CPU restatement.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as the checker.  The product never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from weldformgpu_amd._abi import wf_config, wf_material_params  # noqa: F401  (types only)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_DBL3 = ["x", "v", "a", "u", "va", "xs"]
+_DBL1 = ["rho", "rho0", "drho", "m", "h", "p", "Cs", "G", "sigma_y", "pl_strain", "sigma_eq",
+         "eff_strain_rate", "int_energy", "T", "Ta", "Tb", "dTdt", "k_T", "cp_T", "q_plastic"]
+_DBL6 = ["sigma", "shearstress", "strain", "strrate", "rotrate"]
+_INT1 = ["ID", "BC_T"]
+
+
+class or_state(C.Structure):
+    _fields_ = ([("n", C.c_int), ("solid", C.c_int)]
+                + [(k, C.c_void_p) for k in _DBL3 + _DBL1 + _DBL6 + _INT1]
+                + [("nb_count", C.c_void_p), ("nb_offset", C.c_void_p), ("nb_list", C.c_void_p),
+                   ("nb_cap", C.c_int64), ("newton_iters", C.c_int64),
+                   ("newton_calls", C.c_int64)])
+
+
+class or_run(C.Structure):
+    _fields_ = [("time", C.c_double), ("step", C.c_int64), ("ts_i", C.c_int),
+                ("is_yielding", C.c_int), ("isfirst_step", C.c_int), ("ns_builds", C.c_int),
+                ("max_pl_strain", C.c_double), ("nb_over_cap", C.c_int)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P, I, D, I64 = C.c_void_p, C.c_int, C.c_double, C.c_int64
+        S, CF, R = C.POINTER(or_state), C.POINTER(wf_config), C.POINTER(or_run)
+        sig = {
+            "or_gen_box": (I, [P, D, D, D, D, P, I, D, P]),
+            "or_gen_cylinder": (I, [P, D, D, D, P, I, D, P]),
+            "or_half_part_count": (I, [D, D, I]),
+            "or_assign_zone": (I, [P, I, P, P, I, P]),
+            "or_neighbors": (I64, [P, I, D, P, P, P, I64, P]),
+            "or_neighbors_bruteforce": (I64, [P, I, D, P, P, P, I64]),
+            "or_kernel": (D, [D, D]),
+            "or_grad_kernel": (D, [D, D]),
+            "or_jc_yield": (D, [C.POINTER(wf_material_params), D, D, D]),
+            "or_jc_hardening": (D, [C.POINTER(wf_material_params), D, D, D]),
+            "or_hollomon_yield": (D, [C.POINTER(wf_material_params), D]),
+            "or_jc_newton": (D, [C.POINTER(wf_material_params), D, D, D, D, D, D, D, I,
+                                 C.POINTER(I)]),
+            "or_calc_dens_inc": (None, [S, CF]),
+            "or_update_density": (None, [S, D]),
+            "or_calc_rate_tensors": (None, [S, CF]),
+            "or_pressure": (None, [S]),
+            "or_stress_strain": (None, [S, CF]),
+            "or_calc_accel": (None, [S, CF]),
+            "or_apply_bc_vel": (None, [S, I, D, D, D]),
+            "or_update_pos_fraser": (None, [S, D]),
+            "or_update_vel": (None, [S, D]),
+            "or_calc_temp_inc": (None, [S, CF]),
+            "or_thermal_calcs": (None, [S, CF, C.POINTER(I)]),
+            "or_int_energy": (None, [S, CF]),
+            "or_int_energy_sum": (D, [S]),
+            "or_nsearch_build": (I, [S, CF, C.POINTER(I)]),
+            "or_step": (I, [S, CF, R, I, P, P]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+# ---- geometry -------------------------------------------------------------
+def gen_box(V, L, r, density=1.0):
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    n = lib().or_gen_box(_ptr(V), L[0], L[1], L[2], r, None, 0, density, None)
+    xyz = np.zeros((n, 3))
+    mass = C.c_double(0.0)
+    lib().or_gen_box(_ptr(V), L[0], L[1], L[2], r, _ptr(xyz), n, density, C.byref(mass))
+    return xyz, mass.value
+
+
+def gen_cylinder(V, R, Lz, r, density=1.0):
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    n = lib().or_gen_cylinder(_ptr(V), R, Lz, r, None, 0, density, None)
+    xyz = np.zeros((n, 3))
+    mass = C.c_double(0.0)
+    lib().or_gen_cylinder(_ptr(V), R, Lz, r, _ptr(xyz), n, density, C.byref(mass))
+    return xyz, mass.value
+
+
+def assign_zone(xyz, ID, start, end, zid):
+    s = np.ascontiguousarray(start, dtype=np.float64)
+    e = np.ascontiguousarray(end, dtype=np.float64)
+    return lib().or_assign_zone(_ptr(xyz), len(xyz), _ptr(s), _ptr(e), zid, _ptr(ID))
+
+
+# ---- neighbours -----------------------------------------------------------
+def neighbors(xyz, radius, brute=False):
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    n = len(xyz)
+    counts = np.zeros(n, np.int32)
+    offs = np.zeros(n + 1, np.int32)
+    cap = max(64 * n, 16)
+    while True:
+        lst = np.zeros(cap, np.int32)
+        oc = C.c_int(0)
+        if brute:
+            tot = lib().or_neighbors_bruteforce(_ptr(xyz), n, radius, _ptr(counts), _ptr(offs),
+                                                _ptr(lst), cap)
+        else:
+            tot = lib().or_neighbors(_ptr(xyz), n, radius, _ptr(counts), _ptr(offs), _ptr(lst),
+                                     cap, C.byref(oc))
+        if tot <= cap:
+            return counts, offs, lst[:tot].copy(), oc.value
+        cap = int(tot)
+
+
+# ---- state ----------------------------------------------------------------
+class State:
+    """Owns numpy arrays for one oracle domain (original particle order)."""
+
+    def __init__(self, n, solid=None, nb_cap=None):
+        self.n = n
+        self.arrays = {}
+        for k in _DBL3:
+            self.arrays[k] = np.zeros((n, 3))
+        for k in _DBL1:
+            self.arrays[k] = np.zeros(n)
+        for k in _DBL6:
+            self.arrays[k] = np.zeros((n, 6))
+        for k in _INT1:
+            self.arrays[k] = np.zeros(n, np.int32)
+        self.nb_cap = nb_cap or 80 * n + 16
+        self.arrays["nb_count"] = np.zeros(n, np.int32)
+        self.arrays["nb_offset"] = np.zeros(n + 1, np.int32)
+        self.arrays["nb_list"] = np.zeros(self.nb_cap, np.int32)
+        self.st = or_state()
+        self.st.n = n
+        self.st.solid = n if solid is None else solid
+        for k, a in self.arrays.items():
+            setattr(self.st, k, a.ctypes.data)
+        self.st.nb_cap = self.nb_cap
+        self.run = or_run()
+        self.run.isfirst_step = 1
+
+    def __getitem__(self, k):
+        return self.arrays[k]
+
+    def __setitem__(self, k, v):
+        self.arrays[k][...] = v
+
+    def ref(self):
+        return C.byref(self.st)
+
+    def nsearch(self, cfg):
+        oc = C.c_int(0)
+        rc = lib().or_nsearch_build(self.ref(), C.byref(cfg), C.byref(oc))
+        if rc != 0:
+            raise RuntimeError(f"oracle nsearch failed {rc}")
+        self.run.nb_over_cap = oc.value
+        return oc.value
+
+    def neighbor_rows(self):
+        c, o, l = self["nb_count"], self["nb_offset"], self["nb_list"]
+        return c.copy(), o.copy(), l[: o[self.n]].copy()
+
+    def step(self, cfg, bcs=(), nsteps=1):
+        zones = np.array([b[0] for b in bcs] or [0], np.int32)
+        vals = np.array([list(b[1]) for b in bcs] or [[0, 0, 0]], np.float64)
+        for _ in range(nsteps):
+            rc = lib().or_step(self.ref(), C.byref(cfg), C.byref(self.run), len(bcs),
+                               _ptr(zones), _ptr(vals))
+            if rc != 0:
+                raise RuntimeError(f"oracle step failed {rc}")
+
+    # single kernels
+    def calc_dens_inc(self, cfg):
+        lib().or_calc_dens_inc(self.ref(), C.byref(cfg))
+
+    def update_density(self, dt):
+        lib().or_update_density(self.ref(), dt)
+
+    def calc_rate_tensors(self, cfg):
+        lib().or_calc_rate_tensors(self.ref(), C.byref(cfg))
+
+    def pressure(self):
+        lib().or_pressure(self.ref())
+
+    def stress_strain(self, cfg):
+        lib().or_stress_strain(self.ref(), C.byref(cfg))
+
+    def calc_accel(self, cfg):
+        lib().or_calc_accel(self.ref(), C.byref(cfg))
+
+    def apply_bc_vel(self, zone, v):
+        lib().or_apply_bc_vel(self.ref(), zone, v[0], v[1], v[2])
+
+    def update_pos_fraser(self, dt):
+        lib().or_update_pos_fraser(self.ref(), dt)
+
+    def update_vel(self, dt):
+        lib().or_update_vel(self.ref(), dt)
+
+    def calc_temp_inc(self, cfg):
+        lib().or_calc_temp_inc(self.ref(), C.byref(cfg))
+
+    def thermal_calcs(self, cfg):
+        f = C.c_int(self.run.isfirst_step)
+        lib().or_thermal_calcs(self.ref(), C.byref(cfg), C.byref(f))
+        self.run.isfirst_step = f.value
+
+    def int_energy(self, cfg):
+        lib().or_int_energy(self.ref(), C.byref(cfg))
+
+    def int_energy_sum(self):
+        return lib().or_int_energy_sum(self.ref())
+
+
+def kernel(q, h):
+    return lib().or_kernel(q, h)
+
+
+def grad_kernel(q, h):
+    return lib().or_grad_kernel(q, h)

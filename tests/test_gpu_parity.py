@@ -1,0 +1,196 @@
+"""GPU parity: the HIP path (through the C-ABI) against the FP64 CPU oracle on
+identical seeded inputs.
+
+Tolerances (FP64 path; the GPU contracts a*b+c into FMAs and sums neighbours
+in a different order, so results agree to rounding, not bit for bit):
+  - neighbour sets: bit-exact after sorting rows;
+  - one kernel call: relative L2 <= 1e-12 per field;
+  - tens of steps: relative L2 <= 1e-9 per field.
+"""
+import numpy as np
+import pytest
+
+from weldformgpu_amd import _abi, scenarios
+
+from harness import max_rel, randomize, rel_l2, small_c1, to_gpu, to_oracle
+
+pytestmark = pytest.mark.gpu
+
+KERNEL_TOL = 1e-12
+STEP_TOL = 1e-9
+
+
+def _print_case(name, sc):
+    print(f"[case] {name}: n={sc.n}", flush=True)
+
+
+def test_neighbour_sets_bit_exact_lattice(oracle, wflib):
+    sc = small_c1(0.5)
+    _print_case("nb-lattice", sc)
+    st = to_oracle(oracle, sc)
+    st.nsearch(sc.cfg)
+    oc, oo, ol = st.neighbor_rows()
+    with to_gpu(sc, wflib) as d:
+        d.nsearch()
+        gc, go, gl = d.neighbors()
+    assert np.array_equal(gc, oc)
+    assert np.array_equal(go, oo)
+    assert np.array_equal(gl, ol)
+
+
+def test_neighbour_sets_bit_exact_perturbed(oracle, wflib):
+    sc = small_c1(0.5)
+    rng = np.random.default_rng(7)
+    r = sc.meta["r"]
+    sc.fields["x"] = sc.fields["x"] + rng.uniform(-0.6 * r, 0.6 * r, sc.fields["x"].shape)
+    _print_case("nb-perturbed", sc)
+    st = to_oracle(oracle, sc)
+    st.nsearch(sc.cfg)
+    oc, oo, ol = st.neighbor_rows()
+    assert oc.max() <= _abi.WF_MAX_NEIGHBORS
+    with to_gpu(sc, wflib) as d:
+        d.nsearch()
+        gc, go, gl = d.neighbors()
+        tot, over = d.nsearch_stats()
+    assert over == 0
+    assert np.array_equal(gc, oc)
+    assert np.array_equal(gl, ol)
+
+
+def test_neighbour_cap_reported(oracle, wflib):
+    # a dense random cloud: many particles have more than 70 candidates
+    rng = np.random.default_rng(3)
+    sc = small_c1(0.3)
+    n = sc.n
+    sc.fields["x"] = rng.uniform(0, 1.2e-3, (n, 3))
+    _print_case("nb-cap", sc)
+    cnt, off, lst, over_o = oracle.neighbors(sc.fields["x"], 2 * sc.cfg.h_glob)
+    assert over_o > 0
+    with to_gpu(sc, wflib) as d:
+        d.nsearch()
+        gc, go, gl = d.neighbors()
+        _, over_g = d.nsearch_stats()
+    assert over_g == over_o
+    assert np.array_equal(gc, np.minimum(cnt, 70))
+    # every stored neighbour is a true neighbour
+    rows_o = [set(lst[off[i]:off[i] + cnt[i]]) for i in range(n)]
+    for i in range(n):
+        assert set(gl[go[i]:go[i] + gc[i]]) <= rows_o[i]
+
+
+@pytest.mark.parametrize("quirks", [_abi.WF_Q_ALL, 0])
+@pytest.mark.parametrize("return_map", [_abi.WF_RETURN_REFERENCE, _abi.WF_RETURN_NEWTON])
+def test_single_kernels_bilinear(oracle, wflib, quirks, return_map):
+    sc = randomize(small_c1(0.4), seed=11)
+    sc.cfg.quirks = quirks
+    sc.cfg.return_map = return_map
+    _print_case(f"kernels q={quirks} rm={return_map}", sc)
+    st = to_oracle(oracle, sc)
+    cfg = sc.cfg
+    st.nsearch(cfg)
+    with to_gpu(sc, wflib) as d:
+        d.nsearch()
+        st.calc_dens_inc(cfg)
+        d.CalcDensInc()
+        assert rel_l2(d.get("drho"), st["drho"]) < KERNEL_TOL
+        st.update_density(cfg.dt)
+        d.UpdateDensity(cfg.dt)
+        assert rel_l2(d.get("rho"), st["rho"]) < KERNEL_TOL
+        st.calc_rate_tensors(cfg)
+        d.CalcRateTensors()
+        assert rel_l2(d.get("strrate"), st["strrate"]) < KERNEL_TOL
+        assert rel_l2(d.get("rotrate"), st["rotrate"]) < KERNEL_TOL
+        st.pressure()
+        d.Pressure()
+        assert rel_l2(d.get("p"), st["p"]) < KERNEL_TOL
+        st.stress_strain(cfg)
+        st.int_energy(cfg)
+        d.StressStrain()
+        for k in ["sigma", "shearstress", "sigma_y", "pl_strain", "sigma_eq",
+                  "eff_strain_rate", "strain", "int_energy"]:
+            assert rel_l2(d.get(k), st[k]) < KERNEL_TOL, k
+        assert st["pl_strain"].max() > 0  # the random stress makes particles yield
+        st.calc_accel(cfg)
+        d.CalcAccel()
+        assert rel_l2(d.get("a"), st["a"]) < KERNEL_TOL
+        for z, v in sc.bcs:
+            st.apply_bc_vel(z, v)
+            d.ApplyBCVel(z, v)
+        st.update_pos_fraser(cfg.dt)
+        d.UpdatePosFraser(cfg.dt)
+        st.update_vel(cfg.dt)
+        d.UpdateVel(cfg.dt)
+        for k in ["x", "u", "v", "a"]:
+            assert rel_l2(d.get(k), st[k]) < KERNEL_TOL, k
+        e = d.CalcIntEnergy()
+        assert abs(e - st.int_energy_sum()) <= 1e-12 * abs(st.int_energy_sum())
+
+
+def test_steps_c1_bilinear(oracle, wflib):
+    sc = randomize(small_c1(0.4), seed=5, v_scale=0.5, s_scale=250e6)
+    _print_case("steps-c1", sc)
+    st = to_oracle(oracle, sc)
+    st.nsearch(sc.cfg)
+    nsteps = 25
+    st.step(sc.cfg, sc.bcs, nsteps)
+    with to_gpu(sc, wflib) as d:
+        d.step(nsteps)
+        s = d.stats()
+        assert s["step"] == nsteps
+        assert s["is_yielding"] == st.run.is_yielding
+        assert s["ns_builds"] == 1 + st.run.ns_builds
+        for k in ["u", "v", "rho", "sigma", "sigma_eq", "pl_strain", "int_energy"]:
+            assert rel_l2(d.get(k), st[k]) < STEP_TOL, k
+
+
+def test_steps_c3_jc_newton_thermal(oracle, wflib):
+    sc = randomize(scenarios.c3_johnson_cook(scale=0.2), seed=9, v_scale=0.5, s_scale=200e6)
+    _print_case("steps-c3", sc)
+    st = to_oracle(oracle, sc)
+    st.nsearch(sc.cfg)
+    nsteps = 12
+    st.step(sc.cfg, sc.bcs, nsteps)
+    with to_gpu(sc, wflib) as d:
+        d.step(nsteps)
+        s = d.stats()
+        assert s["newton_calls"] > 0
+        for k in ["u", "v", "rho", "sigma", "sigma_eq", "pl_strain", "T", "int_energy"]:
+            assert rel_l2(d.get(k), st[k]) < STEP_TOL, k
+        assert d.get("T").max() > 300.0  # plastic work heats
+
+
+def test_steps_c2_thermal(oracle, wflib):
+    sc = scenarios.c2_thermal_cube(L=14e-3)
+    _print_case("steps-c2", sc)
+    st = to_oracle(oracle, sc)
+    st.nsearch(sc.cfg)
+    nsteps = 40
+    st.step(sc.cfg, sc.bcs, nsteps)
+    with to_gpu(sc, wflib) as d:
+        d.step(nsteps)
+        T = d.get("T")
+        assert rel_l2(T - 300.0, st["T"] - 300.0) < STEP_TOL
+        assert max_rel(T, st["T"], 1.0) < 1e-12
+        fixed = sc.fields["BC_T"] == _abi.WF_BCT_FIXED
+        assert np.all(T[fixed] == 300.0)
+
+
+def test_full_size_c1_properties(oracle, wflib):
+    """BASELINE C1 geometry at full size (18,792 particles): neighbour sets
+    match the oracle, the rows are symmetric, and 20 steps stay finite with
+    the driven layers at their prescribed velocity."""
+    sc = scenarios.c1_bilinear()
+    _print_case("full-c1", sc)
+    assert sc.n == 18792
+    cnt, off, lst, _ = oracle.neighbors(sc.fields["x"], 2 * sc.cfg.h_glob)
+    with to_gpu(sc, wflib) as d:
+        d.nsearch()
+        gc, go, gl = d.neighbors()
+        assert np.array_equal(gc, cnt) and np.array_equal(gl, lst)
+        pairs = set(zip(np.repeat(np.arange(sc.n), gc).tolist(), gl.tolist()))
+        assert all((j, i) in pairs for i, j in list(pairs)[:20000])
+        d.step(20)
+        v = d.get("v")
+        assert np.isfinite(v).all() and np.isfinite(d.get("sigma")).all()
+        top = sc.fields["ID"] == 3
+        assert np.all(v[top] == np.array([0.0, 0.0, -1.0]))

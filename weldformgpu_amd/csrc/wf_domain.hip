@@ -1,0 +1,755 @@
+// wf_domain.hip — the C-ABI of libwfhip (include/wf_abi.h): the device domain
+// handle, field transfers, neighbour rebuilds with physical reordering, the
+// per-kernel entry points and the fused Fraser step.
+//
+// One handle owns one HIP stream (default priority) and plain hipMalloc'd
+// planes; no managed memory (the reference keeps Domain_d in managed memory,
+// WeldFormGPU.cu:170).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "wf_internal.hpp"
+
+using namespace wf;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPC(expr)                                                                      \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(WF_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));         \
+  } while (0)
+
+// ---- per-field plane access -------------------------------------------------
+struct PlaneRef {
+  double** d = nullptr;
+  int** i = nullptr;
+};
+
+}  // namespace
+
+struct wf_domain {
+  wf_config cfg;
+  int n = 0, solid = 0;
+  int64_t npad = 0;
+  hipStream_t stream = nullptr;
+  Fields f{};
+  std::vector<void*> owned;  // every allocation, freed in wf_destroy
+  int* perm = nullptr;       // sorted slot -> original index
+  int* perm_tmp = nullptr;
+  int* gather = nullptr;
+  int* nb_count = nullptr;
+  int* nb_list = nullptr;
+  double* spare_d = nullptr;  // one plane, reorder scratch
+  int* spare_i = nullptr;
+  void* stage = nullptr;      // transfer staging (6*n doubles)
+  Counters* ctr = nullptr;
+  Counters* ctr_host = nullptr;  // pinned
+  double* sum_part = nullptr;
+  double* sum_out = nullptr;
+  NsWork* ns = nullptr;
+  bool nb_built = false;
+  bool snapshot_set = false;
+  BcTable bcs{};
+  // run state (SolverFraser.cu:30-409)
+  double time = 0.0;
+  int64_t step = 0;
+  int ts_i = 0;
+  bool is_yielding = false;
+  bool isfirst_step = true;
+  int ns_builds = 0;
+  int mlp_trainings = 0;
+  // timing
+  bool timing = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double kt_ms[WF_K_COUNT_] = {0};
+  int64_t kt_n[WF_K_COUNT_] = {0};
+
+  StepParams params() const;
+  PlaneRef plane(int field, int c);
+  std::vector<PlaneRef> all_planes();
+  int alloc_plane_d(double** p);
+  int alloc_plane_i(int** p);
+};
+
+namespace {
+
+int field_info(int field, int* comps, int* elem) {
+  if (field >= WF_F_X && field <= WF_F_X_SNAPSHOT) {
+    *comps = 3;
+    *elem = 8;
+  } else if (field >= WF_F_RHO && field <= WF_F_Q_PLASTIC) {
+    *comps = 1;
+    *elem = 8;
+  } else if (field >= WF_F_SIGMA && field <= WF_F_ROTRATE) {
+    *comps = 6;
+    *elem = 8;
+  } else if (field == WF_F_ID || field == WF_F_BC_T) {
+    *comps = 1;
+    *elem = 4;
+  } else {
+    return WF_E_ARG;
+  }
+  return WF_OK;
+}
+
+// interleaved host order (original index) <-> planes (sorted order)
+struct Planes6 {
+  double* p[6];
+};
+
+__global__ void k_upload_f64(Planes6 planes, int comps, const double* stage, const int* perm,
+                             int n) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int o = perm[s];
+  for (int c = 0; c < comps; c++) planes.p[c][s] = stage[(int64_t)o * comps + c];
+}
+__global__ void k_download_f64(Planes6 planes, int comps, double* stage, const int* perm, int n) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int o = perm[s];
+  for (int c = 0; c < comps; c++) stage[(int64_t)o * comps + c] = planes.p[c][s];
+}
+__global__ void k_upload_i32(int* plane, const int* stage, const int* perm, int n) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n) plane[s] = stage[perm[s]];
+}
+__global__ void k_download_i32(const int* plane, int* stage, const int* perm, int n) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n) stage[perm[s]] = plane[s];
+}
+__global__ void k_iota(int* a, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = i;
+}
+__global__ void k_copy_f64(double* dst, const double* src, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+__global__ void k_zero_bytes(unsigned char* p, int64_t nbytes) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nbytes) p[i] = 0;
+}
+
+int gridn(int64_t n) { return (int)((n + 255) / 256); }
+
+}  // namespace
+
+StepParams wf_domain::params() const {
+  StepParams p{};
+  p.n = n;
+  p.solid = solid;
+  p.dt = cfg.dt;
+  p.alpha = cfg.alpha;
+  p.beta = cfg.beta;
+  p.quirks = cfg.quirks;
+  p.return_map = cfg.return_map;
+  p.taylor_quinney = cfg.taylor_quinney;
+  p.T_inf = cfg.T_inf;
+  p.h_conv = cfg.h_conv;
+  p.newton_tol = cfg.newton_tol;
+  p.newton_max_iter = cfg.newton_max_iter;
+  const wf_material_params& m = cfg.mat;
+  p.mat.model = m.model;
+  p.mat.E = m.E;
+  p.mat.Et = m.Et;
+  // Ep = E*Et/(E-Et) (WeldFormGPU.cu:755), E if negative (Mechanical.cu:527)
+  double Ep = (m.E != m.Et) ? m.E * m.Et / (m.E - m.Et) : m.E;
+  if (Ep < 0) Ep = 1.0 * m.E;
+  p.mat.Ep = Ep;
+  p.mat.sy0 = m.sy0;
+  p.mat.K_h = m.K_h;
+  p.mat.m_h = m.m_h;
+  p.mat.eps0_h = (m.E != 0) ? m.sy0 / m.E : 0.0;  // Material.cuh:57-59
+  p.mat.eps1_h = (m.K_h > 0 && m.m_h != 0) ? std::pow(m.sy0 / m.K_h, 1. / m.m_h) : 0.0;
+  p.mat.A = m.A;
+  p.mat.B = m.B;
+  p.mat.n = m.n;
+  p.mat.C = m.C;
+  p.mat.eps0 = m.eps0;
+  p.mat.m = m.m;
+  p.mat.Tm = m.Tm;
+  p.mat.Tt = m.Tt;
+  return p;
+}
+
+PlaneRef wf_domain::plane(int field, int c) {
+  PlaneRef r;
+  switch (field) {
+    case WF_F_X: r.d = &f.x[c]; break;
+    case WF_F_V: r.d = &f.v[c]; break;
+    case WF_F_A: r.d = &f.a[c]; break;
+    case WF_F_U: r.d = &f.u[c]; break;
+    case WF_F_VA: r.d = &f.va[c]; break;
+    case WF_F_X_SNAPSHOT: r.d = &f.xs[c]; break;
+    case WF_F_RHO: r.d = &f.rho; break;
+    case WF_F_RHO0: r.d = &f.rho0; break;
+    case WF_F_DRHO: r.d = &f.drho; break;
+    case WF_F_M: r.d = &f.m; break;
+    case WF_F_H: r.d = &f.h; break;
+    case WF_F_P: r.d = &f.p; break;
+    case WF_F_CS: r.d = &f.Cs; break;
+    case WF_F_G: r.d = &f.G; break;
+    case WF_F_SIGMA_Y: r.d = &f.sigma_y; break;
+    case WF_F_PL_STRAIN: r.d = &f.pl_strain; break;
+    case WF_F_SIGMA_EQ: r.d = &f.sigma_eq; break;
+    case WF_F_EFF_STRAIN_RATE: r.d = &f.esr; break;
+    case WF_F_INT_ENERGY: r.d = &f.eint; break;
+    case WF_F_T: r.d = &f.T; break;
+    case WF_F_TA: r.d = &f.Ta; break;
+    case WF_F_TB: r.d = &f.Tb; break;
+    case WF_F_DTDT: r.d = &f.dTdt; break;
+    case WF_F_K_T: r.d = &f.k; break;
+    case WF_F_CP_T: r.d = &f.cp; break;
+    case WF_F_Q_PLASTIC: r.d = &f.qpl; break;
+    case WF_F_SIGMA: r.d = &f.sigma[c]; break;
+    case WF_F_SHEARSTRESS: r.d = &f.ss[c]; break;
+    case WF_F_STRAIN: r.d = &f.strain[c]; break;
+    case WF_F_STRRATE: r.d = &f.strrate[c]; break;
+    case WF_F_ROTRATE: r.d = &f.rotrate[c]; break;
+    case WF_F_ID: r.i = &f.ID; break;
+    case WF_F_BC_T: r.i = &f.BC_T; break;
+    default: break;
+  }
+  return r;
+}
+
+std::vector<PlaneRef> wf_domain::all_planes() {
+  std::vector<PlaneRef> v;
+  for (int fld = 0; fld < WF_F_COUNT_; fld++) {
+    int comps, elem;
+    if (field_info(fld, &comps, &elem) != WF_OK) continue;
+    for (int c = 0; c < comps; c++) {
+      PlaneRef r = plane(fld, c);
+      if (r.d || r.i) v.push_back(r);
+    }
+  }
+  return v;
+}
+
+int wf_domain::alloc_plane_d(double** p) {
+  void* q = nullptr;
+  if (hipMalloc(&q, sizeof(double) * npad) != hipSuccess) return WF_E_NOMEM;
+  owned.push_back(q);
+  *p = (double*)q;
+  hipLaunchKernelGGL(k_zero_bytes, dim3(gridn(sizeof(double) * npad)), dim3(256), 0, stream,
+                     (unsigned char*)q, (int64_t)(sizeof(double) * npad));
+  return WF_OK;
+}
+int wf_domain::alloc_plane_i(int** p) {
+  void* q = nullptr;
+  if (hipMalloc(&q, sizeof(int) * npad) != hipSuccess) return WF_E_NOMEM;
+  owned.push_back(q);
+  *p = (int*)q;
+  hipLaunchKernelGGL(k_zero_bytes, dim3(gridn(sizeof(int) * npad)), dim3(256), 0, stream,
+                     (unsigned char*)q, (int64_t)(sizeof(int) * npad));
+  return WF_OK;
+}
+
+// ============================================================================
+extern "C" {
+
+void wf_config_default(wf_config* c) {
+  if (!c) return;
+  std::memset(c, 0, sizeof(*c));
+  c->abi_version = WF_ABI_VERSION;
+  c->precision = WF_FP64;
+  c->alpha = 1.0;  // Domain_d.cu:191-192
+  c->beta = 0.0;
+  c->return_map = WF_RETURN_REFERENCE;
+  c->quirks = WF_Q_ALL;
+  c->mechanics = 1;
+  c->thermal = 0;
+  c->T_inf = 500.0;  // Thermal.cu:203-204
+  c->h_conv = 100.0;
+  c->ns_interval = 5;        // SolverFraser.cu:91
+  c->ns_min_plastic = 1e-6;  // SolverFraser.cu:17
+  c->newton_max_iter = 50;
+  c->newton_tol = 1e-11;
+  c->mlp_train_interval = 200;
+}
+
+int wf_field_shape(int field, int* components, int* elem_bytes) {
+  int c = 0, e = 0;
+  if (field_info(field, &c, &e) != WF_OK) return fail(WF_E_ARG, "unknown field");
+  if (components) *components = c;
+  if (elem_bytes) *elem_bytes = e;
+  return WF_OK;
+}
+
+const char* wf_last_error(void) { return g_err.c_str(); }
+
+int wf_create(const wf_config* cfg, wf_domain** out) {
+  if (!cfg || !out) return fail(WF_E_ARG, "null argument");
+  if (cfg->abi_version != WF_ABI_VERSION) return fail(WF_E_ARG, "abi_version mismatch");
+  if (cfg->n_particles <= 0) return fail(WF_E_ARG, "n_particles must be positive");
+  if (cfg->solid_count < 0 || cfg->solid_count > cfg->n_particles)
+    return fail(WF_E_ARG, "solid_count out of range");
+  if (cfg->precision != WF_FP64)
+    return fail(WF_E_UNSUPPORTED, "precision mode not available in this build");
+  if (cfg->surrogate) return fail(WF_E_UNSUPPORTED, "surrogate not available in this build");
+  if (cfg->ns_interval <= 0) return fail(WF_E_ARG, "ns_interval must be positive");
+  int dev_count = 0;
+  if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0)
+    return fail(WF_E_HIP, "no HIP device");
+  wf_domain* d = new wf_domain();
+  d->cfg = *cfg;
+  d->n = cfg->n_particles;
+  d->solid = cfg->solid_count ? cfg->solid_count : cfg->n_particles;
+  d->npad = ((int64_t)d->n + 63) / 64 * 64;
+  if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete d;
+    return fail(WF_E_HIP, "hipStreamCreate failed");
+  }
+  int rc = WF_OK;
+  for (PlaneRef r : d->all_planes()) {
+    rc = r.d ? d->alloc_plane_d(r.d) : d->alloc_plane_i(r.i);
+    if (rc) break;
+  }
+  if (!rc) rc = d->alloc_plane_d(&d->f.rho_alt);
+  if (!rc) rc = d->alloc_plane_d(&d->f.T_alt);
+  if (!rc) rc = d->alloc_plane_d(&d->spare_d);
+  if (!rc) rc = d->alloc_plane_i(&d->spare_i);
+  if (!rc) rc = d->alloc_plane_i(&d->perm);
+  if (!rc) rc = d->alloc_plane_i(&d->perm_tmp);
+  if (!rc) rc = d->alloc_plane_i(&d->gather);
+  if (!rc) rc = d->alloc_plane_i(&d->nb_count);
+  if (!rc) {
+    void* q = nullptr;
+    if (hipMalloc(&q, sizeof(int) * d->npad * kMaxNb) != hipSuccess) rc = WF_E_NOMEM;
+    else {
+      d->owned.push_back(q);
+      d->nb_list = (int*)q;
+    }
+  }
+  if (!rc) {
+    void* q = nullptr;
+    if (hipMalloc(&q, sizeof(double) * 6 * (int64_t)d->n) != hipSuccess) rc = WF_E_NOMEM;
+    else {
+      d->owned.push_back(q);
+      d->stage = q;
+    }
+  }
+  if (!rc) {
+    void* q = nullptr;
+    if (hipMalloc(&q, sizeof(Counters)) != hipSuccess) rc = WF_E_NOMEM;
+    else {
+      d->owned.push_back(q);
+      d->ctr = (Counters*)q;
+      hipLaunchKernelGGL(k_zero_bytes, dim3(1), dim3(256), 0, d->stream, (unsigned char*)q,
+                         (int64_t)sizeof(Counters));
+    }
+  }
+  if (!rc && hipHostMalloc(&d->ctr_host, sizeof(Counters)) != hipSuccess) rc = WF_E_NOMEM;
+  if (!rc) {
+    void* q = nullptr;
+    if (hipMalloc(&q, sizeof(double) * 1025) != hipSuccess) rc = WF_E_NOMEM;
+    else {
+      d->owned.push_back(q);
+      d->sum_part = (double*)q;
+      d->sum_out = d->sum_part + 1024;
+    }
+  }
+  if (!rc && (hipEventCreate(&d->ev0) != hipSuccess || hipEventCreate(&d->ev1) != hipSuccess))
+    rc = WF_E_HIP;
+  if (rc) {
+    wf_destroy(d);
+    return fail(rc, "device allocation failed");
+  }
+  hipLaunchKernelGGL(k_iota, dim3(gridn(d->n)), dim3(256), 0, d->stream, d->perm, d->n);
+  // thermal defaults so an unused T never poisons the JC law: T = 0
+  if (hipStreamSynchronize(d->stream) != hipSuccess) {
+    wf_destroy(d);
+    return fail(WF_E_HIP, "initialisation failed");
+  }
+  *out = d;
+  return WF_OK;
+}
+
+int wf_destroy(wf_domain* d) {
+  if (!d) return WF_OK;
+  if (d->stream) hipStreamSynchronize(d->stream);
+  for (void* p : d->owned) hipFree(p);
+  if (d->ctr_host) hipHostFree(d->ctr_host);
+  ns_free(d->ns);
+  if (d->ev0) hipEventDestroy(d->ev0);
+  if (d->ev1) hipEventDestroy(d->ev1);
+  if (d->stream) hipStreamDestroy(d->stream);
+  delete d;
+  return WF_OK;
+}
+
+int wf_sync(wf_domain* d) {
+  if (!d) return fail(WF_E_ARG, "null domain");
+  HIPC(hipStreamSynchronize(d->stream));
+  return WF_OK;
+}
+
+int wf_set_field(wf_domain* d, int field, const void* host, size_t bytes) {
+  if (!d || !host) return fail(WF_E_ARG, "null argument");
+  int comps, elem;
+  if (field_info(field, &comps, &elem)) return fail(WF_E_ARG, "unknown field");
+  if (bytes != (size_t)d->n * comps * elem) return fail(WF_E_ARG, "size mismatch");
+  HIPC(hipMemcpyAsync(d->stage, host, bytes, hipMemcpyHostToDevice, d->stream));
+  if (elem == 8) {
+    Planes6 planes{};
+    for (int c = 0; c < comps; c++) planes.p[c] = *d->plane(field, c).d;
+    hipLaunchKernelGGL(k_upload_f64, dim3(gridn(d->n)), dim3(256), 0, d->stream, planes, comps,
+                       (const double*)d->stage, d->perm, d->n);
+  } else {
+    hipLaunchKernelGGL(k_upload_i32, dim3(gridn(d->n)), dim3(256), 0, d->stream,
+                       *d->plane(field, 0).i, (const int*)d->stage, d->perm, d->n);
+  }
+  // x also initialises the search snapshot x_h (SolverFraser.cu:56)
+  if (field == WF_F_X) {
+    for (int c = 0; c < 3; c++)
+      hipLaunchKernelGGL(k_copy_f64, dim3(gridn(d->n)), dim3(256), 0, d->stream, d->f.xs[c],
+                         d->f.x[c], d->n);
+    d->snapshot_set = true;
+  }
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(d->stream));
+  return WF_OK;
+}
+
+int wf_get_field(wf_domain* d, int field, void* host, size_t bytes) {
+  if (!d || !host) return fail(WF_E_ARG, "null argument");
+  int comps, elem;
+  if (field_info(field, &comps, &elem)) return fail(WF_E_ARG, "unknown field");
+  if (bytes != (size_t)d->n * comps * elem) return fail(WF_E_ARG, "size mismatch");
+  if (elem == 8) {
+    Planes6 planes{};
+    for (int c = 0; c < comps; c++) planes.p[c] = *d->plane(field, c).d;
+    hipLaunchKernelGGL(k_download_f64, dim3(gridn(d->n)), dim3(256), 0, d->stream, planes, comps,
+                       (double*)d->stage, d->perm, d->n);
+  } else {
+    hipLaunchKernelGGL(k_download_i32, dim3(gridn(d->n)), dim3(256), 0, d->stream,
+                       *d->plane(field, 0).i, (int*)d->stage, d->perm, d->n);
+  }
+  HIPC(hipGetLastError());
+  HIPC(hipMemcpyAsync(host, d->stage, bytes, hipMemcpyDeviceToHost, d->stream));
+  HIPC(hipStreamSynchronize(d->stream));
+  return WF_OK;
+}
+
+int wf_fill_field(wf_domain* d, int field, double value) {
+  if (!d) return fail(WF_E_ARG, "null domain");
+  int comps, elem;
+  if (field_info(field, &comps, &elem) || elem != 8) return fail(WF_E_ARG, "not a double field");
+  for (int c = 0; c < comps; c++) launch_fill_f64(d->stream, *d->plane(field, c).d, value, d->n);
+  if (field == WF_F_X)
+    for (int c = 0; c < 3; c++) launch_fill_f64(d->stream, d->f.xs[c], value, d->n);
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(d->stream));
+  return WF_OK;
+}
+
+int wf_snapshot_positions(wf_domain* d) {
+  if (!d) return fail(WF_E_ARG, "null domain");
+  for (int c = 0; c < 3; c++)
+    hipLaunchKernelGGL(k_copy_f64, dim3(gridn(d->n)), dim3(256), 0, d->stream, d->f.xs[c],
+                       d->f.x[c], d->n);
+  d->snapshot_set = true;
+  HIPC(hipGetLastError());
+  return WF_OK;
+}
+
+int wf_set_bcs(wf_domain* d, int count, const int* zone_ids, const double* values) {
+  if (!d) return fail(WF_E_ARG, "null domain");
+  if (count < 0 || count > WF_MAX_BCS) return fail(WF_E_ARG, "too many boundary conditions");
+  if (count > 0 && (!zone_ids || !values)) return fail(WF_E_ARG, "null argument");
+  d->bcs.count = count;
+  for (int b = 0; b < count; b++) {
+    d->bcs.zone[b] = zone_ids[b];
+    for (int c = 0; c < 3; c++) d->bcs.val[b][c] = values[3 * b + c];
+  }
+  return WF_OK;
+}
+
+static void timer_begin(wf_domain* d) {
+  if (d->timing) hipEventRecord(d->ev0, d->stream);
+}
+static int timer_end(wf_domain* d, int k) {
+  if (!d->timing) return WF_OK;
+  HIPC(hipEventRecord(d->ev1, d->stream));
+  HIPC(hipEventSynchronize(d->ev1));
+  float ms = 0.f;
+  HIPC(hipEventElapsedTime(&ms, d->ev0, d->ev1));
+  d->kt_ms[k] += ms;
+  d->kt_n[k] += 1;
+  return WF_OK;
+}
+
+int wf_nsearch_build(wf_domain* d) {
+  if (!d) return fail(WF_E_ARG, "null domain");
+  timer_begin(d);
+  const bool stale = (d->cfg.quirks & WF_Q_STALE_NS_POS) != 0;
+  const double* src[3];
+  for (int c = 0; c < 3; c++) src[c] = stale ? d->f.xs[c] : d->f.x[c];
+  int over = 0;
+  int rc = ns_build(d->stream, &d->ns, src, d->n, d->npad, 2.0 * d->cfg.h_glob, d->gather,
+                    d->nb_count, d->nb_list, d->ctr, &over, d->perm, d->perm_tmp);
+  if (rc) return fail(rc, rc == WF_E_NONFINITE ? "non-finite particle position"
+                                               : "neighbour search failed");
+  // physically reorder every field into the new order
+  for (PlaneRef r : d->all_planes()) {
+    if (r.d) {
+      launch_gather_f64(d->stream, d->spare_d, *r.d, d->gather, d->n);
+      std::swap(*r.d, d->spare_d);
+    } else {
+      launch_gather_i32(d->stream, d->spare_i, *r.i, d->gather, d->n);
+      std::swap(*r.i, d->spare_i);
+    }
+  }
+  std::swap(d->perm, d->perm_tmp);
+  HIPC(hipGetLastError());
+  d->nb_built = true;
+  d->ns_builds++;
+  return timer_end(d, WF_K_NSEARCH);
+}
+
+int wf_nsearch_stats(wf_domain* d, int64_t* total, int* over_cap) {
+  if (!d) return fail(WF_E_ARG, "null domain");
+  if (!d->nb_built) return fail(WF_E_STATE, "neighbour lists not built");
+  std::vector<int> cnt(d->n);
+  HIPC(hipMemcpyAsync(cnt.data(), d->nb_count, sizeof(int) * d->n, hipMemcpyDeviceToHost,
+                      d->stream));
+  HIPC(hipMemcpyAsync(d->ctr_host, d->ctr, sizeof(Counters), hipMemcpyDeviceToHost, d->stream));
+  HIPC(hipStreamSynchronize(d->stream));
+  int64_t t = 0;
+  for (int c : cnt) t += c;
+  if (total) *total = t;
+  if (over_cap) *over_cap = d->ctr_host->nb_over_cap;
+  return WF_OK;
+}
+
+int wf_get_neighbors(wf_domain* d, int* counts, int* offsets, int* list, int64_t list_cap) {
+  if (!d || !counts || !offsets) return fail(WF_E_ARG, "null argument");
+  if (!d->nb_built) return fail(WF_E_STATE, "neighbour lists not built");
+  int n = d->n;
+  std::vector<int> cnt(n), perm(n), lst((size_t)d->npad * kMaxNb);
+  HIPC(hipMemcpyAsync(cnt.data(), d->nb_count, sizeof(int) * n, hipMemcpyDeviceToHost, d->stream));
+  HIPC(hipMemcpyAsync(perm.data(), d->perm, sizeof(int) * n, hipMemcpyDeviceToHost, d->stream));
+  HIPC(hipMemcpyAsync(lst.data(), d->nb_list, sizeof(int) * lst.size(), hipMemcpyDeviceToHost,
+                      d->stream));
+  HIPC(hipStreamSynchronize(d->stream));
+  std::vector<std::vector<int>> rows(n);
+  for (int s = 0; s < n; s++) {
+    auto& row = rows[perm[s]];
+    for (int k = 0; k < cnt[s]; k++) row.push_back(perm[lst[(size_t)k * d->npad + s]]);
+    std::sort(row.begin(), row.end());
+  }
+  int64_t t = 0;
+  for (int o = 0; o < n; o++) {
+    offsets[o] = (int)t;
+    counts[o] = (int)rows[o].size();
+    for (int j : rows[o]) {
+      if (list && t < list_cap) list[t] = j;
+      t++;
+    }
+  }
+  offsets[n] = (int)t;
+  if (list && t > list_cap) return fail(WF_E_ARG, "list_cap too small");
+  return WF_OK;
+}
+
+static NbLists lists(const wf_domain* d) { return NbLists{d->nb_count, d->nb_list, d->npad}; }
+
+#define NEED_NB(d) \
+  if (!(d)->nb_built) return fail(WF_E_STATE, "neighbour lists not built (call wf_nsearch_build)")
+
+int wf_calc_dens_inc(wf_domain* d) {
+  if (!d) return fail(WF_E_ARG, "null domain");
+  NEED_NB(d);
+  launch_dens_inc(d->stream, d->f, lists(d), d->params(), false);
+  HIPC(hipGetLastError());
+  return WF_OK;
+}
+int wf_update_density(wf_domain* d, double dt) {
+  if (!d) return fail(WF_E_ARG, "null domain");
+  launch_update_density(d->stream, d->f, d->params(), dt);
+  HIPC(hipGetLastError());
+  return WF_OK;
+}
+int wf_calc_rate_tensors(wf_domain* d) {
+  if (!d) return fail(WF_E_ARG, "null domain");
+  NEED_NB(d);
+  launch_rate_tensors(d->stream, d->f, lists(d), d->params());
+  HIPC(hipGetLastError());
+  return WF_OK;
+}
+int wf_pressure(wf_domain* d) {
+  if (!d) return fail(WF_E_ARG, "null domain");
+  launch_pressure(d->stream, d->f, d->params());
+  HIPC(hipGetLastError());
+  return WF_OK;
+}
+int wf_stress_strain(wf_domain* d) {
+  if (!d) return fail(WF_E_ARG, "null domain");
+  launch_stress(d->stream, d->f, d->params(), d->ctr, false, true);
+  HIPC(hipGetLastError());
+  return WF_OK;
+}
+int wf_calc_accel(wf_domain* d) {
+  if (!d) return fail(WF_E_ARG, "null domain");
+  NEED_NB(d);
+  launch_accel(d->stream, d->f, lists(d), d->params());
+  HIPC(hipGetLastError());
+  return WF_OK;
+}
+int wf_apply_bc_vel(wf_domain* d, int zone, double vx, double vy, double vz) {
+  if (!d) return fail(WF_E_ARG, "null domain");
+  launch_apply_bc(d->stream, d->f, d->n, zone, vx, vy, vz);
+  HIPC(hipGetLastError());
+  return WF_OK;
+}
+int wf_update_pos_fraser(wf_domain* d, double dt) {
+  if (!d) return fail(WF_E_ARG, "null domain");
+  launch_update_pos(d->stream, d->f, d->n, dt);
+  HIPC(hipGetLastError());
+  return WF_OK;
+}
+int wf_update_vel(wf_domain* d, double dt) {
+  if (!d) return fail(WF_E_ARG, "null domain");
+  launch_update_vel(d->stream, d->f, d->n, dt);
+  HIPC(hipGetLastError());
+  return WF_OK;
+}
+int wf_calc_temp_inc(wf_domain* d) {
+  if (!d) return fail(WF_E_ARG, "null domain");
+  NEED_NB(d);
+  launch_temp_inc(d->stream, d->f, lists(d), d->params());
+  HIPC(hipGetLastError());
+  return WF_OK;
+}
+int wf_thermal_calcs(wf_domain* d, double dt) {
+  if (!d) return fail(WF_E_ARG, "null domain");
+  NEED_NB(d);
+  StepParams p = d->params();
+  p.dt = dt;
+  launch_temp_inc(d->stream, d->f, lists(d), p);
+  launch_leapfrog(d->stream, d->f, d->n, dt, d->isfirst_step);
+  d->isfirst_step = false;
+  HIPC(hipGetLastError());
+  return WF_OK;
+}
+int wf_int_energy(wf_domain* d, double* sum) {
+  if (!d || !sum) return fail(WF_E_ARG, "null argument");
+  int nb = std::min(1024, std::max(1, gridn(d->n)));
+  launch_sum(d->stream, d->f.eint, d->n, d->sum_part, nb, d->sum_out);
+  HIPC(hipGetLastError());
+  HIPC(hipMemcpyAsync(sum, d->sum_out, sizeof(double), hipMemcpyDeviceToHost, d->stream));
+  HIPC(hipStreamSynchronize(d->stream));
+  return WF_OK;
+}
+
+// Domain_d::MechFraserSolve loop body, SolverFraser.cu:164-399
+int wf_step(wf_domain* d, int n_steps) {
+  if (!d) return fail(WF_E_ARG, "null domain");
+  if (n_steps < 0) return fail(WF_E_ARG, "negative step count");
+  int rc;
+  if (!d->nb_built) {  // initial search, SolverFraser.cu:109
+    rc = wf_nsearch_build(d);
+    if (rc) return rc;
+  }
+  const NbLists nbl = lists(d);
+  for (int s = 0; s < n_steps; s++) {
+    StepParams p = d->params();
+    if (d->cfg.mechanics) {
+      if (d->ts_i == 0 && d->is_yielding) {
+        rc = wf_nsearch_build(d);
+        if (rc) return rc;
+      }
+      timer_begin(d);
+      launch_dens_inc(d->stream, d->f, nbl, p, true);
+      std::swap(d->f.rho, d->f.rho_alt);
+      if ((rc = timer_end(d, WF_K_DENS))) return rc;
+      timer_begin(d);
+      launch_rate_tensors(d->stream, d->f, nbl, p);
+      if ((rc = timer_end(d, WF_K_RATES))) return rc;
+      timer_begin(d);
+      launch_stress(d->stream, d->f, p, d->ctr, true, true);
+      if ((rc = timer_end(d, WF_K_STRESS))) return rc;
+      timer_begin(d);
+      launch_accel(d->stream, d->f, nbl, p);
+      if ((rc = timer_end(d, WF_K_ACCEL))) return rc;
+      if (!d->is_yielding) {  // SolverFraser.cu:260-271, on the device-side max
+        HIPC(hipMemcpyAsync(d->ctr_host, d->ctr, sizeof(Counters), hipMemcpyDeviceToHost,
+                            d->stream));
+        HIPC(hipStreamSynchronize(d->stream));
+        double mx;
+        unsigned long long b = d->ctr_host->max_pl_strain_bits;
+        std::memcpy(&mx, &b, sizeof(mx));
+        if (mx > d->cfg.ns_min_plastic) d->is_yielding = true;
+      }
+      timer_begin(d);
+      launch_integrate(d->stream, d->f, d->n, d->cfg.dt, d->bcs);
+      if ((rc = timer_end(d, WF_K_INTEGRATE))) return rc;
+    }
+    if (d->cfg.thermal) {
+      timer_begin(d);
+      launch_thermal_fused(d->stream, d->f, nbl, p, d->isfirst_step);
+      std::swap(d->f.T, d->f.T_alt);
+      if ((rc = timer_end(d, WF_K_THERMAL))) return rc;
+    }
+    d->isfirst_step = false;
+    d->time += d->cfg.dt;
+    d->step++;
+    d->ts_i++;
+    if (d->ts_i > d->cfg.ns_interval - 1) d->ts_i = 0;
+  }
+  HIPC(hipGetLastError());
+  return WF_OK;
+}
+
+int wf_get_stats(wf_domain* d, wf_stats* out) {
+  if (!d || !out) return fail(WF_E_ARG, "null argument");
+  HIPC(hipMemcpyAsync(d->ctr_host, d->ctr, sizeof(Counters), hipMemcpyDeviceToHost, d->stream));
+  HIPC(hipStreamSynchronize(d->stream));
+  std::memset(out, 0, sizeof(*out));
+  out->step = d->step;
+  out->time = d->time;
+  out->is_yielding = d->is_yielding;
+  out->ns_builds = d->ns_builds;
+  out->newton_iters = (int64_t)d->ctr_host->newton_iters;
+  out->newton_calls = (int64_t)d->ctr_host->newton_calls;
+  out->newton_iters_wave = (int64_t)d->ctr_host->newton_iters_wave;
+  out->mlp_trainings = d->mlp_trainings;
+  out->nb_over_cap = d->ctr_host->nb_over_cap;
+  return WF_OK;
+}
+
+int wf_set_timing(wf_domain* d, int on) {
+  if (!d) return fail(WF_E_ARG, "null domain");
+  HIPC(hipStreamSynchronize(d->stream));
+  d->timing = on != 0;
+  for (int k = 0; k < WF_K_COUNT_; k++) {
+    d->kt_ms[k] = 0.0;
+    d->kt_n[k] = 0;
+  }
+  return WF_OK;
+}
+
+int wf_kernel_time(wf_domain* d, int k, double* total_ms, int64_t* launches) {
+  if (!d || k < 0 || k >= WF_K_COUNT_) return fail(WF_E_ARG, "bad argument");
+  if (total_ms) *total_ms = d->kt_ms[k];
+  if (launches) *launches = d->kt_n[k];
+  return WF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,117 @@
+// wf_geometry.cpp — host-side lattice builders of the C-ABI (no device code).
+// The loops keep the reference's generation order and its floating-point
+// steps (positions by formula for the box, by accumulation for the cylinder)
+// so that counts and coordinates agree bit for bit with a reference run.
+#include <cmath>
+
+#include "../../include/wf_abi.h"
+
+namespace {
+
+// number of particles on a half row at row offset xinc from the axis:
+// calcHalfPartCount, Domain.cpp:473-488
+int half_row(double r, double R, int xinc) {
+  if (xinc <= 0) return -1;
+  int cnt = 1;
+  double yp = r;
+  const double xp = r + (double)(xinc - 1) * 2. * r;
+  double rad = std::sqrt(yp * yp + xp * xp);
+  while (rad <= R - r) {
+    yp += 2. * r;
+    rad = std::sqrt(yp * yp + xp * xp);
+    cnt++;
+  }
+  return cnt - 1;
+}
+
+inline void put(double* xyz, int cap, int k, double x, double y, double z) {
+  if (xyz && k < cap) {
+    xyz[3 * k] = x;
+    xyz[3 * k + 1] = y;
+    xyz[3 * k + 2] = z;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int wf_gen_box(const double V[3], double Lx, double Ly, double Lz, double r, double density,
+               double* xyz, int cap, double* mass_out) {
+  int count = 0;
+  double mx = V[0], my = V[1], mz = V[2];
+  for (size_t k = 0;; k++) {
+    // the first layer is always emitted: the loop test starts from V itself
+    double zp = (k == 0) ? V[2] : V[2] + (2.0 * k + 1) * r;
+    if (!(zp <= (V[2] + Lz - r))) break;
+    for (size_t j = 0;; j++) {
+      double yp = (j == 0) ? V[1] : V[1] + (2.0 * j + 1) * r;
+      if (!(yp <= (V[1] + Ly - r))) break;
+      for (size_t i = 0;; i++) {
+        double xp = (i == 0) ? V[0] : V[0] + (2 * i + 1) * r;
+        if (!(xp <= (V[0] + Lx - r))) break;
+        double x = V[0] + (2.0 * i + 1) * r;
+        double y = V[1] + (2.0 * j + 1) * r;
+        double z = V[2] + (2.0 * k + 1) * r;
+        put(xyz, cap, count, x, y, z);
+        if (x > mx) mx = x;
+        if (y > my) my = y;
+        if (z > mz) mz = z;
+        count++;
+      }
+    }
+  }
+  if (mass_out && count > 0) {
+    mx += r;
+    my += r;
+    mz += r;
+    *mass_out = (mx - V[0]) * (my - V[1]) * (mz - V[2]) * density / count;
+  }
+  return count;
+}
+
+int wf_gen_cylinder(const double V[3], double R, double Lz, double r, double density,
+                    double* xyz, int cap, double* mass_out) {
+  int count = 0;
+  const int nxy = half_row(r, R, 1);
+  for (double zp = V[2] + r; zp <= (V[2] + Lz - r); zp += 2. * r) {
+    double yp = V[1] - r - (2. * r * (nxy - 1));
+    int row = nxy, step = -1;
+    for (int j = 0; j < 2 * nxy; j++) {
+      const int nx = half_row(r, R, row);
+      double xp = V[0] - r - (2. * r * (nx - 1));
+      for (int i = 0; i < 2 * nx; i++) {
+        put(xyz, cap, count, xp, yp, zp);
+        count++;
+        xp += 2. * r;
+      }
+      yp += 2. * r;
+      row += step;
+      if (row < 1) {  // crossed the axis: rows grow again
+        row = 1;
+        step = 1;
+      }
+    }
+  }
+  if (mass_out && count > 0) *mass_out = M_PI * R * R * Lz * density / count;
+  return count;
+}
+
+int wf_assign_zone(const double* xyz, int n, const double start[3], const double end[3], int id,
+                   int* ID) {
+  if (!xyz || !ID || !start || !end || n < 0) return WF_E_ARG;
+  int hits = 0;
+  for (int a = 0; a < n; a++) {
+    const double* p = xyz + 3 * a;
+    bool in = true;
+    for (int c = 0; c < 3; c++)
+      if (p[c] < start[c] || p[c] > end[c]) in = false;
+    if (in) {
+      ID[a] = id;
+      hits++;
+    }
+  }
+  return hits;
+}
+
+}  // extern "C"

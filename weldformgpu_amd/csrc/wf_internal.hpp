@@ -1,0 +1,119 @@
+// wf_internal.hpp — device state layout and kernel launchers of libwfhip.
+//
+// Particle data lives in HBM as structure-of-arrays in Morton-sorted order
+// (see wf_nsearch.hip): one contiguous block per field, components of a
+// vector/tensor field stored as separate planes of npad elements, so a
+// wavefront's 64 consecutive particles read 64 consecutive elements.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/wf_abi.h"
+
+namespace wf {
+
+constexpr int kBlock = 256;       // threads per block of the per-particle kernels
+constexpr int kMaxNb = WF_MAX_NEIGHBORS;
+
+// Plane pointers of every particle field (device addresses, sorted order).
+// Passed by value to kernels.
+struct Fields {
+  double *x[3], *v[3], *a[3], *u[3], *va[3], *xs[3];
+  double *rho, *rho_alt, *rho0, *drho, *m, *h, *p, *Cs, *G;
+  double *sigma_y, *pl_strain, *sigma_eq, *esr, *eint;
+  double *T, *T_alt, *Ta, *Tb, *dTdt, *k, *cp, *qpl;
+  double *sigma[6], *ss[6], *strain[6], *strrate[6], *rotrate[6];
+  int *ID, *BC_T;
+};
+
+// Neighbour lists: interleaved, entry k of particle i at list[k*npad + i]
+// (a wavefront reads 64 consecutive ints per k), count[i] <= kMaxNb.
+struct NbLists {
+  const int* count;
+  const int* list;
+  int64_t npad;
+};
+
+struct MatParams {
+  int model;
+  double E, Et, Ep, sy0, K_h, m_h, eps0_h, eps1_h;
+  double A, B, n, C, eps0, m, Tm, Tt;
+};
+
+struct StepParams {
+  int n, solid;
+  double dt;
+  double alpha, beta;
+  int quirks;
+  int return_map;
+  double taylor_quinney;
+  double T_inf, h_conv;
+  double newton_tol;
+  int newton_max_iter;
+  MatParams mat;
+};
+
+struct BcTable {
+  int count;
+  int zone[WF_MAX_BCS];
+  double val[WF_MAX_BCS][3];
+};
+
+// device-side counters written by the kernels (one allocation)
+struct Counters {
+  unsigned long long max_pl_strain_bits;  // max eps_p as ordered bits (eps_p >= 0)
+  unsigned long long newton_iters;
+  unsigned long long newton_calls;
+  unsigned long long newton_iters_wave;
+  int nb_over_cap;
+  int nonfinite;
+};
+
+// ---- launchers (wf_kernels.hip) ------------------------------------------
+void launch_dens_inc(hipStream_t s, const Fields& f, const NbLists& nb, const StepParams& p,
+                     bool fuse_update);
+void launch_update_density(hipStream_t s, const Fields& f, const StepParams& p, double dt);
+void launch_rate_tensors(hipStream_t s, const Fields& f, const NbLists& nb, const StepParams& p);
+void launch_pressure(hipStream_t s, const Fields& f, const StepParams& p);
+void launch_stress(hipStream_t s, const Fields& f, const StepParams& p, Counters* ctr,
+                   bool fuse_pressure, bool energy);
+void launch_accel(hipStream_t s, const Fields& f, const NbLists& nb, const StepParams& p);
+void launch_apply_bc(hipStream_t s, const Fields& f, int n, int zone, double vx, double vy,
+                     double vz);
+void launch_update_pos(hipStream_t s, const Fields& f, int n, double dt);
+void launch_update_vel(hipStream_t s, const Fields& f, int n, double dt);
+void launch_integrate(hipStream_t s, const Fields& f, int n, double dt, const BcTable& bc);
+void launch_temp_inc(hipStream_t s, const Fields& f, const NbLists& nb, const StepParams& p);
+void launch_leapfrog(hipStream_t s, const Fields& f, int n, double dt, bool first);
+void launch_thermal_fused(hipStream_t s, const Fields& f, const NbLists& nb, const StepParams& p,
+                          bool first);
+void launch_int_energy(hipStream_t s, const Fields& f, const StepParams& p);
+void launch_sum(hipStream_t s, const double* v, int n, double* partial, int nblocks,
+                double* out);
+
+// ---- neighbour search (wf_nsearch.hip) -----------------------------------
+struct NsWork;  // opaque scratch of the search
+struct NsResult {
+  int over_cap;
+};
+// Build order + lists.  pos_src: 3 planes (sorted order) the search reads.
+// Outputs: new_from_old gather map (slot -> old sorted index) in `gather`,
+// lists in list/count (new order).  Returns WF_OK or an error code.
+// perm: sorted slot -> original index of the current order; perm_tmp receives
+// the new one.
+int ns_build(hipStream_t s, NsWork** work, const double* const pos_src[3], int n, int64_t npad,
+             double radius, int* gather, int* count, int* list, Counters* ctr, int* over_cap,
+             int* perm, int* perm_tmp);
+void ns_free(NsWork* w);
+
+// generic gathers used to reorder fields: dst[i] = src[idx[i]]
+void launch_gather_f64(hipStream_t s, double* dst, const double* src, const int* idx, int n);
+void launch_gather_i32(hipStream_t s, int* dst, const int* src, const int* idx, int n);
+// scatter for downloads: dst[idx[i]] = src[i]
+void launch_scatter_f64(hipStream_t s, double* dst, const double* src, const int* idx, int n);
+void launch_scatter_i32(hipStream_t s, int* dst, const int* src, const int* idx, int n);
+// interleave components: dst[i*comps + c] = planes[c][i] and back
+void launch_fill_f64(hipStream_t s, double* dst, double v, int n);
+
+}  // namespace wf

@@ -1,0 +1,656 @@
+// wf_kernels.hip — FP64 per-step kernels of the SPH pipeline (gfx950).
+//
+// Pair loops run one thread per particle over Morton-sorted SoA data: the
+// 64 lanes of a wavefront are 64 spatially adjacent particles, so their
+// neighbour gathers hit the same L1/L2 lines, and the interleaved neighbour
+// list gives one coalesced 256-byte load per list slot.  Every sum keeps the
+// reference's per-pair arithmetic (InteractionAlt.cu, Mechanical.cu,
+// Thermal.cu); only the neighbour order differs.
+#include "wf_device.hpp"
+
+namespace wf {
+
+static inline int grid_for(int n) { return (n + kBlock - 1) / kBlock; }
+
+// ---------------------------------------------------------------------------
+// A: density rate, CalcDensInc (InteractionAlt.cu:166-246).  With FUSE the
+// density update (Mechanical.cu:198-203) is applied into the second density
+// buffer, so every thread still reads the old rho_j.
+template <bool FUSE>
+__global__ __launch_bounds__(kBlock) void k_dens_inc(Fields f, NbLists nb, int n, int solid,
+                                                     double dt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double rhoi = f.rho[i];
+  if (i >= solid) {
+    if (FUSE) f.rho_alt[i] = rhoi;
+    return;
+  }
+  const double xi = f.x[0][i], yi = f.x[1][i], zi = f.x[2][i];
+  const double vxi = f.v[0][i], vyi = f.v[1][i], vzi = f.v[2][i];
+  const double hi = f.h[i];
+  const int cnt = nb.count[i];
+  double acc = 0.0;
+  for (int k = 0; k < cnt; k++) {
+    const int j = nb.list[(int64_t)k * nb.npad + i];
+    double xx = xi - f.x[0][j], xy = yi - f.x[1][j], xz = zi - f.x[2][j];
+    double rij = sqrt(xx * xx + xy * xy + xz * xz);
+    double dj = f.rho[j], mj = f.m[j];
+    double vx = vxi - f.v[0][j], vy = vyi - f.v[1][j], vz = vzi - f.v[2][j];
+    double h_ = (hi + f.h[j]) / 2.0;
+    double GK = grad_kernel(rij / h_, h_);
+    double temp1 = vx * (GK * xx) + vy * (GK * xy) + vz * (GK * xz);
+    acc += mj * (rhoi / dj) * temp1;
+  }
+  f.drho[i] = acc;
+  if (FUSE) f.rho_alt[i] = rhoi + dt * acc;
+}
+
+__global__ __launch_bounds__(kBlock) void k_update_density(Fields f, int solid, double dt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < solid) f.rho[i] += dt * f.drho[i];
+}
+
+// ---------------------------------------------------------------------------
+// B: strain-rate and rotation-rate tensors, CalcRateTensors
+// (InteractionAlt.cu:262-383), stored [xx,yy,zz,xy,yz,xz].  With the xz quirk
+// the rotation-rate xz is negated (InteractionAlt.cu:362).
+__global__ __launch_bounds__(kBlock) void k_rate_tensors(Fields f, NbLists nb, int solid,
+                                                         int quirks) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= solid) return;
+  const double xi = f.x[0][i], yi = f.x[1][i], zi = f.x[2][i];
+  const double vxi = f.v[0][i], vyi = f.v[1][i], vzi = f.v[2][i];
+  const double hi = f.h[i];
+  const double xz_sign = (quirks & WF_Q_ROT_XZ_SIGN) ? -1.0 : 1.0;
+  const int cnt = nb.count[i];
+  double sxx = 0, syy = 0, szz = 0, sxy = 0, syz = 0, sxz = 0;
+  double wxy = 0, wyz = 0, wxz = 0;
+  for (int k = 0; k < cnt; k++) {
+    const int j = nb.list[(int64_t)k * nb.npad + i];
+    double xx = xi - f.x[0][j], xy = yi - f.x[1][j], xz = zi - f.x[2][j];
+    double rij = sqrt(xx * xx + xy * xy + xz * xz);
+    double dj = f.rho[j], mj = f.m[j];
+    double vx = vxi - f.v[0][j], vy = vyi - f.v[1][j], vz = vzi - f.v[2][j];
+    double h_ = (hi + f.h[j]) / 2.0;
+    double GK = grad_kernel(rij / h_, h_);
+    double fs = (-0.5) * GK;
+    double mj_dj = mj / dj;
+    sxx += (2.0 * vx * xx) * fs * mj_dj;
+    syy += (2.0 * vy * xy) * fs * mj_dj;
+    szz += (2.0 * vz * xz) * fs * mj_dj;
+    sxy += (vx * xy + vy * xx) * fs * mj_dj;
+    syz += (vy * xz + vz * xy) * fs * mj_dj;
+    sxz += (vx * xz + vz * xx) * fs * mj_dj;
+    wxy += (vx * xy - vy * xx) * fs * mj_dj;
+    wyz += (vy * xz - vz * xy) * fs * mj_dj;
+    wxz += (xz_sign * (vx * xz - vz * xx)) * fs * mj_dj;
+  }
+  f.strrate[0][i] = sxx;
+  f.strrate[1][i] = syy;
+  f.strrate[2][i] = szz;
+  f.strrate[3][i] = sxy;
+  f.strrate[4][i] = syz;
+  f.strrate[5][i] = sxz;
+  f.rotrate[0][i] = 0.0;
+  f.rotrate[1][i] = 0.0;
+  f.rotrate[2][i] = 0.0;
+  f.rotrate[3][i] = wxy;
+  f.rotrate[4][i] = wyz;
+  f.rotrate[5][i] = wxz;
+}
+
+// ---------------------------------------------------------------------------
+// Pressure, PressureKernelExt + EOS case 0 (Mechanical.cu:289-298,
+// Functions.cu:211-213) with PresEq = P0 = 0.
+__global__ __launch_bounds__(kBlock) void k_pressure(Fields f, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) f.p[i] = 0.0 + (f.Cs[i] * f.Cs[i]) * (f.rho[i] - f.rho0[i]);
+}
+
+struct T9 {
+  double xx, xy, xz, yx, yy, yz, zx, zy, zz;
+};
+
+// tensor3 operator*, tensor3.cu:241-255 as written (first column A*B, the
+// others A*B^T) or the true product.
+__device__ __forceinline__ T9 tmul(const T9& a, const T9& b, bool quirk) {
+  T9 r;
+  r.xx = a.xx * b.xx + a.xy * b.yx + a.xz * b.zx;
+  r.yx = a.yx * b.xx + a.yy * b.yx + a.yz * b.zx;
+  r.zx = a.zx * b.xx + a.zy * b.yx + a.zz * b.zx;
+  if (quirk) {
+    r.xy = a.xx * b.yx + a.xy * b.yy + a.xz * b.yz;
+    r.xz = a.xx * b.zx + a.xy * b.zy + a.xz * b.zz;
+    r.yy = a.yx * b.yx + a.yy * b.yy + a.yz * b.yz;
+    r.yz = a.yx * b.zx + a.yy * b.zy + a.yz * b.zz;
+    r.zy = a.zx * b.yx + a.zy * b.yy + a.zz * b.yz;
+    r.zz = a.zx * b.zx + a.zy * b.zy + a.zz * b.zz;
+  } else {
+    r.xy = a.xx * b.xy + a.xy * b.yy + a.xz * b.zy;
+    r.xz = a.xx * b.xz + a.xy * b.yz + a.xz * b.zz;
+    r.yy = a.yx * b.xy + a.yy * b.yy + a.yz * b.zy;
+    r.yz = a.yx * b.xz + a.yy * b.yz + a.yz * b.zz;
+    r.zy = a.zx * b.xy + a.zy * b.yy + a.zz * b.zy;
+    r.zz = a.zx * b.xz + a.zy * b.yz + a.zz * b.zz;
+  }
+  return r;
+}
+
+// Safeguarded Newton for the Johnson-Cook radial return,
+// r(g) = s_trial - 3G g - s_y(ep+g) = 0 on [0, (s_trial - s_y(ep))/(3G)].
+__device__ double jc_newton(const MatParams& mp, double s_trial, double G, double ep,
+                            double rate_f, double therm_f, double g0, double tol, int max_iter,
+                            int* iters) {
+  double sy0 = jc_yield(mp, ep, rate_f, therm_f);
+  double lo = 0.0, hi = (s_trial - sy0) / (3.0 * G);
+  if (!(hi > 0.0)) {
+    *iters = 0;
+    return 0.0;
+  }
+  double g = g0;
+  if (!(g > lo) || !(g < hi)) g = (g0 >= hi) ? hi : lo;
+  int it = 0;
+  for (; it < max_iter; it++) {
+    double sy = jc_yield(mp, ep + g, rate_f, therm_f);
+    double r = s_trial - 3.0 * G * g - sy;
+    if (fabs(r) <= tol * s_trial) break;
+    if (r > 0.0) lo = g;
+    else hi = g;
+    double H = jc_hardening(mp, ep + g, rate_f, therm_f);
+    double gn = g + r / (3.0 * G + H);
+    if (!(gn > lo && gn < hi)) gn = 0.5 * (lo + hi);
+    g = gn;
+  }
+  *iters = it;
+  return g;
+}
+
+// ---------------------------------------------------------------------------
+// C: Jaumann stress update + return map, StressStrainOne (Mechanical.cu:458-554),
+// optionally with the pressure (FUSE_P) and the per-particle internal-energy
+// increment (ENERGY, Energy.cu:5-19 with per-particle indexing).
+template <bool FUSE_P, bool ENERGY>
+__global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Counters* ctr) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool active = i < p.solid;
+  unsigned long long ep_bits = 0ull, nit = 0ull, ncall = 0ull;
+  if (active) {
+    const MatParams& mp = p.mat;
+    const bool qp = (p.quirks & WF_Q_TENSOR_PRODUCT) != 0;
+    const double dt = p.dt;
+    if (FUSE_P) f.p[i] = 0.0 + (f.Cs[i] * f.Cs[i]) * (f.rho[i] - f.rho0[i]);
+    T9 S, D, W;
+    S.xx = f.ss[0][i]; S.yy = f.ss[1][i]; S.zz = f.ss[2][i];
+    S.xy = S.yx = f.ss[3][i]; S.yz = S.zy = f.ss[4][i]; S.xz = S.zx = f.ss[5][i];
+    D.xx = f.strrate[0][i]; D.yy = f.strrate[1][i]; D.zz = f.strrate[2][i];
+    D.xy = D.yx = f.strrate[3][i]; D.yz = D.zy = f.strrate[4][i];
+    D.xz = D.zx = f.strrate[5][i];
+    W.xx = f.rotrate[0][i]; W.yy = f.rotrate[1][i]; W.zz = f.rotrate[2][i];
+    W.xy = f.rotrate[3][i]; W.yz = f.rotrate[4][i]; W.xz = f.rotrate[5][i];
+    W.yx = -W.xy; W.zy = -W.yz; W.zx = -W.xz;
+    T9 WT = {W.xx, W.yx, W.zx, W.xy, W.yy, W.zy, W.xz, W.yz, W.zz};
+    T9 SRT = tmul(S, WT, qp);
+    T9 RS = tmul(W, S, qp);
+    const double G = f.G[i];
+    const double tr3 = 1.0 / 3.0 * (D.xx + D.yy + D.zz);
+    const double g2 = 2.0 * G;
+    // S = dt*(2G(D - tr/3 I) + SRT + RS) + S   (Mechanical.cu:490)
+    S.xx = ((D.xx - tr3) * g2 + SRT.xx + RS.xx) * dt + S.xx;
+    S.yy = ((D.yy - tr3) * g2 + SRT.yy + RS.yy) * dt + S.yy;
+    S.zz = ((D.zz - tr3) * g2 + SRT.zz + RS.zz) * dt + S.zz;
+    S.xy = (D.xy * g2 + SRT.xy + RS.xy) * dt + S.xy;
+    S.yx = (D.yx * g2 + SRT.yx + RS.yx) * dt + S.yx;
+    S.xz = (D.xz * g2 + SRT.xz + RS.xz) * dt + S.xz;
+    S.zx = (D.zx * g2 + SRT.zx + RS.zx) * dt + S.zx;
+    S.yz = (D.yz * g2 + SRT.yz + RS.yz) * dt + S.yz;
+    S.zy = (D.zy * g2 + SRT.zy + RS.zy) * dt + S.zy;
+
+    double esr = sqrt(0.5 * ((D.xx - D.yy) * (D.xx - D.yy) + (D.yy - D.zz) * (D.yy - D.zz) +
+                             (D.zz - D.xx) * (D.zz - D.xx)) +
+                      3.0 * (D.xy * D.xy + D.yz * D.yz + D.zx * D.zx));
+    f.esr[i] = esr;
+    double J2 = 0.5 * (S.xx * S.xx + 2.0 * S.xy * S.yx + 2.0 * S.xz * S.zx + S.yy * S.yy +
+                       2.0 * S.yz * S.zy + S.zz * S.zz);
+    double sig_trial = sqrt(3.0 * J2);
+    double T = f.T ? f.T[i] : 0.0;
+    double sigy = f.sigma_y[i];
+    double ep = f.pl_strain[i];
+    double sigeq = sig_trial;
+    double dep = 0.0;
+    double scale = 1.0;
+    if (p.return_map == WF_RETURN_REFERENCE) {
+      if (sigy < sig_trial) scale = sigy / sig_trial;
+      if (mp.model == WF_MAT_HOLLOMON) sigy = hollomon_yield(mp, ep);
+      else if (mp.model == WF_MAT_JOHNSON_COOK)
+        sigy = jc_yield(mp, ep, jc_rate_factor(mp, esr), 1.0 - pow(jc_theta(mp, T), mp.m));
+      if (sig_trial > sigy) {
+        dep = (sig_trial - sigy) / (3. * G);
+        ep += dep;
+        sigeq = sigy;
+        if (mp.model == WF_MAT_BILINEAR) sigy += dep * mp.Ep;
+      }
+    } else {
+      double rate_f = 1.0, therm_f = 1.0;
+      if (mp.model == WF_MAT_HOLLOMON) sigy = hollomon_yield(mp, ep);
+      else if (mp.model == WF_MAT_JOHNSON_COOK) {
+        rate_f = jc_rate_factor(mp, esr);
+        therm_f = 1.0 - pow(jc_theta(mp, T), mp.m);
+        sigy = jc_yield(mp, ep, rate_f, therm_f);
+      }
+      if (sig_trial > sigy) {
+        double g;
+        if (mp.model == WF_MAT_BILINEAR) {
+          g = (sig_trial - sigy) / (3.0 * G + mp.Ep);
+          sigy = sigy + mp.Ep * g;
+        } else if (mp.model == WF_MAT_JOHNSON_COOK) {
+          int it = 0;
+          g = jc_newton(mp, sig_trial, G, ep, rate_f, therm_f, 0.0, p.newton_tol,
+                        p.newton_max_iter, &it);
+          nit = (unsigned long long)it;
+          ncall = 1ull;
+          sigy = jc_yield(mp, ep + g, rate_f, therm_f);
+        } else {
+          double lo = 0.0, hi = (sig_trial - sigy) / (3.0 * G);
+          for (int it = 0; it < 200; it++) {
+            double mid = 0.5 * (lo + hi);
+            double r = sig_trial - 3.0 * G * mid - hollomon_yield(mp, ep + mid);
+            if (r > 0) lo = mid;
+            else hi = mid;
+          }
+          g = 0.5 * (lo + hi);
+          sigy = hollomon_yield(mp, ep + g);
+        }
+        scale = (sig_trial - 3.0 * G * g) / sig_trial;
+        dep = g;
+        ep += g;
+        sigeq = sig_trial - 3.0 * G * g;
+      }
+    }
+    if (scale != 1.0) {
+      S.xx *= scale; S.yy *= scale; S.zz *= scale;
+      S.xy *= scale; S.yz *= scale; S.xz *= scale;
+    }
+    f.sigma_y[i] = sigy;
+    f.pl_strain[i] = ep;
+    f.sigma_eq[i] = sigeq;
+    if (f.qpl) f.qpl[i] = p.taylor_quinney * sigeq * dep / dt;
+    const double pi = f.p[i];
+    // sigma = -p I + S (Mechanical.cu:540); store upper triangle
+    f.sigma[0][i] = -pi + S.xx;
+    f.sigma[1][i] = -pi + S.yy;
+    f.sigma[2][i] = -pi + S.zz;
+    f.sigma[3][i] = S.xy;
+    f.sigma[4][i] = S.yz;
+    f.sigma[5][i] = S.xz;
+    f.ss[0][i] = S.xx; f.ss[1][i] = S.yy; f.ss[2][i] = S.zz;
+    f.ss[3][i] = S.xy; f.ss[4][i] = S.yz; f.ss[5][i] = S.xz;
+    f.strain[0][i] += D.xx * dt; f.strain[1][i] += D.yy * dt; f.strain[2][i] += D.zz * dt;
+    f.strain[3][i] += D.xy * dt; f.strain[4][i] += D.yz * dt; f.strain[5][i] += D.xz * dt;
+    if (ENERGY) {
+      double de = f.m[i] / f.rho[i] *
+                  (S.xx * D.xx + S.yy * D.yy + S.zz * D.zz +
+                   2.0 * (S.xy * D.xy + S.yz * D.yz + S.xz * D.xz));
+      f.eint[i] += de * dt;
+    }
+    ep_bits = (unsigned long long)__double_as_longlong(ep > 0.0 ? ep : 0.0);
+  }
+  // wavefront reductions, one atomic per wave
+  unsigned long long mx = wave_max_u64(ep_bits);
+  unsigned long long sit = wave_sum_u64(nit);
+  unsigned long long sc = wave_sum_u64(ncall);
+  unsigned long long wmax = wave_max_u64(nit);
+  if (lane_id() == 0) {
+    if (mx) atomicMax(&ctr->max_pl_strain_bits, mx);
+    if (sc) {
+      atomicAdd(&ctr->newton_iters, sit);
+      atomicAdd(&ctr->newton_calls, sc);
+      atomicAdd(&ctr->newton_iters_wave, wmax);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// D: acceleration, CalcAccel (InteractionAlt.cu:18-150): Monaghan viscosity
+// Pi_ij = (alpha C mu + beta mu^2)/rho_bar when v_ij.x_ij < 0,
+// a_i = sum m_j (sigma_i/rho_i^2 + sigma_j/rho_j^2 + Pi_ij I) . (GK x_ij)
+__global__ __launch_bounds__(kBlock) void k_accel(Fields f, NbLists nb, int solid, double alpha,
+                                                  double beta) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= solid) return;
+  const double xi = f.x[0][i], yi = f.x[1][i], zi = f.x[2][i];
+  const double vxi = f.v[0][i], vyi = f.v[1][i], vzi = f.v[2][i];
+  const double hi = f.h[i], di = f.rho[i], Ci = f.Cs[i];
+  const double s0 = f.sigma[0][i], s1 = f.sigma[1][i], s2 = f.sigma[2][i];
+  const double s3 = f.sigma[3][i], s4 = f.sigma[4][i], s5 = f.sigma[5][i];
+  const double idi = 1.0 / (di * di);
+  const bool visc = (alpha != 0.0 || beta != 0.0);
+  const int cnt = nb.count[i];
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  for (int k = 0; k < cnt; k++) {
+    const int j = nb.list[(int64_t)k * nb.npad + i];
+    double xx = xi - f.x[0][j], xy = yi - f.x[1][j], xz = zi - f.x[2][j];
+    double rij = sqrt(xx * xx + xy * xy + xz * xz);
+    double dj = f.rho[j], mj = f.m[j];
+    double vx = vxi - f.v[0][j], vy = vyi - f.v[1][j], vz = vzi - f.v[2][j];
+    double h_ = (hi + f.h[j]) / 2.0;
+    double GK = grad_kernel(rij / h_, h_);
+    double PI = 0.0;
+    if (visc) {
+      double vdx = vx * xx + vy * xy + vz * xz;
+      double MUij = h_ * vdx / (rij * rij + 0.01 * h_ * h_);
+      double Cij = 0.5 * (Ci + f.Cs[j]);
+      if (vdx < 0) PI = (alpha * Cij * MUij + beta * MUij * MUij) / (0.5 * (di + dj));
+    }
+    double idj = 1.0 / (dj * dj);
+    double mxx = (s0 * idi + f.sigma[0][j] * idj + PI) * mj;
+    double myy = (s1 * idi + f.sigma[1][j] * idj + PI) * mj;
+    double mzz = (s2 * idi + f.sigma[2][j] * idj + PI) * mj;
+    double mxy = (s3 * idi + f.sigma[3][j] * idj) * mj;
+    double myz = (s4 * idi + f.sigma[4][j] * idj) * mj;
+    double mxz = (s5 * idi + f.sigma[5][j] * idj) * mj;
+    double gx = GK * xx, gy = GK * xy, gz = GK * xz;
+    ax += mxx * gx + mxy * gy + mxz * gz;
+    ay += mxy * gx + myy * gy + myz * gz;
+    az += mxz * gx + myz * gy + mzz * gz;
+  }
+  f.a[0][i] = ax;
+  f.a[1][i] = ay;
+  f.a[2][i] = az;
+}
+
+// ---------------------------------------------------------------------------
+// F: velocity BCs + UpdatePosFraser + UpdateVel + BCs in one pass
+// (SolverFraser.cu:273-281; Mechanical.cu:142-191,572-584)
+__global__ __launch_bounds__(kBlock) void k_integrate(Fields f, int n, double dt, BcTable bc) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int id = f.ID[i];
+  double v0 = f.v[0][i], v1 = f.v[1][i], v2 = f.v[2][i];
+  double a0 = f.a[0][i], a1 = f.a[1][i], a2 = f.a[2][i];
+  int hit = -1;
+  for (int b = 0; b < bc.count; b++)
+    if (id == bc.zone[b]) hit = b;
+  if (hit >= 0) {
+    a0 = a1 = a2 = 0.0;
+    v0 = bc.val[hit][0];
+    v1 = bc.val[hit][1];
+    v2 = bc.val[hit][2];
+  }
+  double u0 = v0 * dt + ((a0 * 0.5) * dt) * dt;
+  double u1 = v1 * dt + ((a1 * 0.5) * dt) * dt;
+  double u2 = v2 * dt + ((a2 * 0.5) * dt) * dt;
+  f.x[0][i] += u0;
+  f.x[1][i] += u1;
+  f.x[2][i] += u2;
+  f.u[0][i] += u0;
+  f.u[1][i] += u1;
+  f.u[2][i] += u2;
+  v0 += a0 * dt;
+  v1 += a1 * dt;
+  v2 += a2 * dt;
+  if (hit >= 0) {
+    a0 = a1 = a2 = 0.0;
+    v0 = bc.val[hit][0];
+    v1 = bc.val[hit][1];
+    v2 = bc.val[hit][2];
+    f.va[0][i] = v0;
+    f.va[1][i] = v1;
+    f.va[2][i] = v2;
+  }
+  f.v[0][i] = v0;
+  f.v[1][i] = v1;
+  f.v[2][i] = v2;
+  f.a[0][i] = a0;
+  f.a[1][i] = a1;
+  f.a[2][i] = a2;
+}
+
+__global__ __launch_bounds__(kBlock) void k_apply_bc(Fields f, int n, int zone, double vx,
+                                                     double vy, double vz) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || f.ID[i] != zone) return;
+  f.a[0][i] = 0.0; f.a[1][i] = 0.0; f.a[2][i] = 0.0;
+  f.v[0][i] = vx; f.v[1][i] = vy; f.v[2][i] = vz;
+  f.va[0][i] = vx; f.va[1][i] = vy; f.va[2][i] = vz;
+}
+
+__global__ __launch_bounds__(kBlock) void k_update_pos(Fields f, int n, double dt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int c = 0; c < 3; c++) {
+    double u_ = f.v[c][i] * dt + ((f.a[c][i] * 0.5) * dt) * dt;
+    f.x[c][i] += u_;
+    f.u[c][i] += u_;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_update_vel(Fields f, int n, double dt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int c = 0; c < 3; c++) f.v[c][i] += f.a[c][i] * dt;
+}
+
+// ---------------------------------------------------------------------------
+// G: conduction (Thermal.cu:101-142, Fraser eq. 3.99) + convection
+// (Thermal.cu:36-64) + plastic-work source; FIXED BC holds T.
+// LEAP: fuse the leapfrog update (Thermal.cu:16-31), writing T into T_alt so
+// that neighbours keep reading the old T.
+template <bool LEAP>
+__global__ __launch_bounds__(kBlock) void k_thermal(Fields f, NbLists nb, int n, double T_inf,
+                                                    double h_conv, double dt, int first) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double xi = f.x[0][i], yi = f.x[1][i], zi = f.x[2][i];
+  const double hi = f.h[i], ki = f.k[i], Ti = f.T[i];
+  const int cnt = nb.count[i];
+  double acc = 0.0;
+  for (int k = 0; k < cnt; k++) {
+    const int j = nb.list[(int64_t)k * nb.npad + i];
+    double xx = xi - f.x[0][j], xy = yi - f.x[1][j], xz = zi - f.x[2][j];
+    double h_ = (hi + f.h[j]) / 2.0;
+    double r2 = xx * xx + xy * xy + xz * xz;
+    double nxij = sqrt(r2);
+    double GK = grad_kernel(nxij / h_, h_);
+    double kj = f.k[j];
+    acc += f.m[j] / f.rho[j] * (4.0 * ki * kj / (ki + kj) * (Ti - f.T[j])) *
+           (xx * (GK * xx) + xy * (GK * xy) + xz * (GK * xz)) / (nxij * nxij);
+  }
+  const double rhoi = f.rho[i], cpi = f.cp[i];
+  acc *= 1. / (rhoi * cpi);
+  const int bct = f.BC_T ? f.BC_T[i] : 0;
+  if (bct == WF_BCT_CONVECTION) {
+    double mi = f.m[i];
+    double dS2 = pow(mi / rhoi, 0.666666666);
+    double q_conv = h_conv * dS2 * (T_inf - Ti) / mi;
+    acc += 1. / cpi * q_conv;
+  }
+  if (f.qpl) {
+    double q = f.qpl[i];
+    if (q != 0.0) acc += q * (1. / (rhoi * cpi));
+  }
+  if (bct == WF_BCT_FIXED) acc = 0.0;
+  f.dTdt[i] = acc;
+  if (LEAP) {
+    if (first) {
+      f.Ta[i] = Ti - dt / 2.0 * acc;
+      f.T_alt[i] = Ti;
+    } else {
+      double ta = f.Ta[i];
+      f.Tb[i] = ta;
+      double tan = ta + acc * dt;
+      f.Ta[i] = tan;
+      f.T_alt[i] = (tan + ta) / 2.;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_leapfrog(Fields f, int n, double dt, int first) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (first) {
+    f.Ta[i] = f.T[i] - dt / 2.0 * f.dTdt[i];
+  } else {
+    f.Tb[i] = f.Ta[i];
+    f.Ta[i] += f.dTdt[i] * dt;
+    f.T[i] = (f.Ta[i] + f.Tb[i]) / 2.;
+  }
+}
+
+// CalcIntEnergy per particle (Energy.cu:5-19, corrected indexing)
+__global__ __launch_bounds__(kBlock) void k_int_energy(Fields f, int n, double dt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double de = f.m[i] / f.rho[i] *
+              (f.ss[0][i] * f.strrate[0][i] + f.ss[1][i] * f.strrate[1][i] +
+               f.ss[2][i] * f.strrate[2][i] +
+               2.0 * (f.ss[3][i] * f.strrate[3][i] + f.ss[4][i] * f.strrate[4][i] +
+                      f.ss[5][i] * f.strrate[5][i]));
+  f.eint[i] += de * dt;
+}
+
+// Deterministic FP64 sum: fixed-shape block partials, then one block.
+__global__ __launch_bounds__(kBlock) void k_sum_partial(const double* v, int n, double* partial) {
+  __shared__ double sh[kBlock / 64];
+  double acc = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) acc += v[i];
+  acc = wave_sum_f64(acc);
+  if (lane_id() == 0) sh[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kBlock / 64; w++) t += sh[w];
+    partial[blockIdx.x] = t;
+  }
+}
+__global__ void k_sum_final(const double* partial, int nb, double* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double t = 0.0;
+    for (int b = 0; b < nb; b++) t += partial[b];
+    *out = t;
+  }
+}
+
+// ---- generic reorder helpers ----------------------------------------------
+__global__ void k_gather_f64(double* dst, const double* src, const int* idx, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+__global__ void k_gather_i32(int* dst, const int* src, const int* idx, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+__global__ void k_scatter_f64(double* dst, const double* src, const int* idx, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[idx[i]] = src[i];
+}
+__global__ void k_scatter_i32(int* dst, const int* src, const int* idx, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[idx[i]] = src[i];
+}
+__global__ void k_fill_f64(double* dst, double v, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = v;
+}
+
+// ---- launchers --------------------------------------------------------------
+void launch_dens_inc(hipStream_t s, const Fields& f, const NbLists& nb, const StepParams& p,
+                     bool fuse) {
+  if (p.n <= 0) return;
+  if (fuse)
+    hipLaunchKernelGGL(k_dens_inc<true>, dim3(grid_for(p.n)), dim3(kBlock), 0, s, f, nb, p.n,
+                       p.solid, p.dt);
+  else
+    hipLaunchKernelGGL(k_dens_inc<false>, dim3(grid_for(p.n)), dim3(kBlock), 0, s, f, nb, p.n,
+                       p.solid, p.dt);
+}
+void launch_update_density(hipStream_t s, const Fields& f, const StepParams& p, double dt) {
+  if (p.solid <= 0) return;
+  hipLaunchKernelGGL(k_update_density, dim3(grid_for(p.solid)), dim3(kBlock), 0, s, f, p.solid,
+                     dt);
+}
+void launch_rate_tensors(hipStream_t s, const Fields& f, const NbLists& nb, const StepParams& p) {
+  if (p.solid <= 0) return;
+  hipLaunchKernelGGL(k_rate_tensors, dim3(grid_for(p.solid)), dim3(kBlock), 0, s, f, nb,
+                     p.solid, p.quirks);
+}
+void launch_pressure(hipStream_t s, const Fields& f, const StepParams& p) {
+  if (p.n <= 0) return;
+  hipLaunchKernelGGL(k_pressure, dim3(grid_for(p.n)), dim3(kBlock), 0, s, f, p.n);
+}
+void launch_stress(hipStream_t s, const Fields& f, const StepParams& p, Counters* ctr,
+                   bool fuse_pressure, bool energy) {
+  if (p.solid <= 0) return;
+  dim3 g(grid_for(p.solid)), b(kBlock);
+  if (fuse_pressure && energy)
+    hipLaunchKernelGGL((k_stress<true, true>), g, b, 0, s, f, p, ctr);
+  else if (fuse_pressure)
+    hipLaunchKernelGGL((k_stress<true, false>), g, b, 0, s, f, p, ctr);
+  else if (energy)
+    hipLaunchKernelGGL((k_stress<false, true>), g, b, 0, s, f, p, ctr);
+  else
+    hipLaunchKernelGGL((k_stress<false, false>), g, b, 0, s, f, p, ctr);
+}
+void launch_accel(hipStream_t s, const Fields& f, const NbLists& nb, const StepParams& p) {
+  if (p.solid <= 0) return;
+  hipLaunchKernelGGL(k_accel, dim3(grid_for(p.solid)), dim3(kBlock), 0, s, f, nb, p.solid,
+                     p.alpha, p.beta);
+}
+void launch_apply_bc(hipStream_t s, const Fields& f, int n, int zone, double vx, double vy,
+                     double vz) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_apply_bc, dim3(grid_for(n)), dim3(kBlock), 0, s, f, n, zone, vx, vy, vz);
+}
+void launch_update_pos(hipStream_t s, const Fields& f, int n, double dt) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_update_pos, dim3(grid_for(n)), dim3(kBlock), 0, s, f, n, dt);
+}
+void launch_update_vel(hipStream_t s, const Fields& f, int n, double dt) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_update_vel, dim3(grid_for(n)), dim3(kBlock), 0, s, f, n, dt);
+}
+void launch_integrate(hipStream_t s, const Fields& f, int n, double dt, const BcTable& bc) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_integrate, dim3(grid_for(n)), dim3(kBlock), 0, s, f, n, dt, bc);
+}
+void launch_temp_inc(hipStream_t s, const Fields& f, const NbLists& nb, const StepParams& p) {
+  if (p.n <= 0) return;
+  hipLaunchKernelGGL(k_thermal<false>, dim3(grid_for(p.n)), dim3(kBlock), 0, s, f, nb, p.n,
+                     p.T_inf, p.h_conv, p.dt, 0);
+}
+void launch_thermal_fused(hipStream_t s, const Fields& f, const NbLists& nb, const StepParams& p,
+                          bool first) {
+  if (p.n <= 0) return;
+  hipLaunchKernelGGL(k_thermal<true>, dim3(grid_for(p.n)), dim3(kBlock), 0, s, f, nb, p.n,
+                     p.T_inf, p.h_conv, p.dt, first ? 1 : 0);
+}
+void launch_leapfrog(hipStream_t s, const Fields& f, int n, double dt, bool first) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_leapfrog, dim3(grid_for(n)), dim3(kBlock), 0, s, f, n, dt, first ? 1 : 0);
+}
+void launch_int_energy(hipStream_t s, const Fields& f, const StepParams& p) {
+  if (p.n <= 0) return;
+  hipLaunchKernelGGL(k_int_energy, dim3(grid_for(p.n)), dim3(kBlock), 0, s, f, p.n, p.dt);
+}
+void launch_sum(hipStream_t s, const double* v, int n, double* partial, int nblocks,
+                double* out) {
+  hipLaunchKernelGGL(k_sum_partial, dim3(nblocks), dim3(kBlock), 0, s, v, n, partial);
+  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(64), 0, s, partial, nblocks, out);
+}
+void launch_gather_f64(hipStream_t s, double* dst, const double* src, const int* idx, int n) {
+  if (n > 0) hipLaunchKernelGGL(k_gather_f64, dim3(grid_for(n)), dim3(kBlock), 0, s, dst, src, idx, n);
+}
+void launch_gather_i32(hipStream_t s, int* dst, const int* src, const int* idx, int n) {
+  if (n > 0) hipLaunchKernelGGL(k_gather_i32, dim3(grid_for(n)), dim3(kBlock), 0, s, dst, src, idx, n);
+}
+void launch_scatter_f64(hipStream_t s, double* dst, const double* src, const int* idx, int n) {
+  if (n > 0) hipLaunchKernelGGL(k_scatter_f64, dim3(grid_for(n)), dim3(kBlock), 0, s, dst, src, idx, n);
+}
+void launch_scatter_i32(hipStream_t s, int* dst, const int* src, const int* idx, int n) {
+  if (n > 0) hipLaunchKernelGGL(k_scatter_i32, dim3(grid_for(n)), dim3(kBlock), 0, s, dst, src, idx, n);
+}
+void launch_fill_f64(hipStream_t s, double* dst, double v, int n) {
+  if (n > 0) hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(n)), dim3(kBlock), 0, s, dst, v, n);
+}
+
+}  // namespace wf

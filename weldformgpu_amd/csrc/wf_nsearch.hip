@@ -1,0 +1,385 @@
+// wf_nsearch.hip — fixed-radius neighbour search with Morton-sorted
+// particle storage (replaces cuNSearch, src/cuNSearch/src/cuNSearch.cu:111-137).
+//
+//  1. positions -> float (the reference searches in FP32: Common.h:11-15),
+//     bounding box by a two-level reduction (one 24-byte D2H per rebuild);
+//  2. uniform grid of cell size slightly above the radius, padded by a cell;
+//     cell key = 8^3 meta-block index * 512 + Morton code inside the block,
+//     so consecutive keys are spatially compact (the same layout family as
+//     cuNSearch's ToCellIndex_MortonMetaGrid, cuNSearchKernels.cu:10-21);
+//  3. counting sort by key (atomic histogram + exclusive scan), then each
+//     cell's particles ordered by original index so the permutation is
+//     deterministic (the reference's atomic ranks are not,
+//     cuNSearchKernels.cu:104);
+//  4. the caller gathers every field into the new order;
+//  5. one thread per particle walks the 27 neighbour cells and appends j
+//     with 0 < d2 < r2 in float, d2 = dx*dx + dy*dy + dz*dz without
+//     contraction (cuNSearchKernels.cu:160-165), cap 70 (:169).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "wf_device.hpp"
+
+namespace wf {
+
+struct NsWork {
+  int n_cap = 0;
+  int64_t key_cap = 0;
+  float* pf[3] = {nullptr, nullptr, nullptr};   // float positions, old order
+  float* pfn[3] = {nullptr, nullptr, nullptr};  // float positions, new order
+  int* key = nullptr;
+  int* rank = nullptr;
+  int* slot_orig = nullptr;  // key-sorted: original index (for ordering)
+  int* slot_old = nullptr;   // key-sorted: old sorted index
+  int* cell_count = nullptr;
+  int* cell_start = nullptr;  // key_cap + 1
+  int* scan_part = nullptr;
+  float* mm_part = nullptr;  // per block 6 floats
+  float* mm = nullptr;       // 6 floats
+  float* mm_host = nullptr;  // pinned
+};
+
+constexpr int kScanBlock = 1024;
+constexpr int kMmBlocks = 512;
+
+__global__ void k_ns_to_float(const double* __restrict__ x0, const double* __restrict__ x1,
+                              const double* __restrict__ x2, float* p0, float* p1, float* p2,
+                              int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    p0[i] = (float)x0[i];
+    p1[i] = (float)x1[i];
+    p2[i] = (float)x2[i];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_ns_minmax_part(const float* p0, const float* p1,
+                                                        const float* p2, int n, float* part) {
+  __shared__ float sh[6][256];
+  float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    float v[3] = {p0[i], p1[i], p2[i]};
+    for (int c = 0; c < 3; c++) {
+      mn[c] = fminf(mn[c], v[c]);
+      mx[c] = fmaxf(mx[c], v[c]);
+    }
+  }
+  for (int c = 0; c < 3; c++) {
+    sh[c][threadIdx.x] = mn[c];
+    sh[3 + c][threadIdx.x] = mx[c];
+  }
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int c = 0; c < 3; c++) {
+        sh[c][threadIdx.x] = fminf(sh[c][threadIdx.x], sh[c][threadIdx.x + s]);
+        sh[3 + c][threadIdx.x] = fmaxf(sh[3 + c][threadIdx.x], sh[3 + c][threadIdx.x + s]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ void k_ns_minmax_final(const float* part, int nb, float* mm) {
+  int c = threadIdx.x;
+  if (c >= 6) return;
+  float v = c < 3 ? INFINITY : -INFINITY;
+  for (int b = 0; b < nb; b++) v = c < 3 ? fminf(v, part[b * 6 + c]) : fmaxf(v, part[b * 6 + c]);
+  mm[c] = v;
+}
+
+struct Grid {
+  double org[3];
+  double inv_cs;
+  int dim[3];
+  int mdim[3];
+};
+
+__device__ __forceinline__ int morton3(int x, int y, int z) {
+  int m = 0;
+#pragma unroll
+  for (int b = 0; b < 3; b++)
+    m |= (((x >> b) & 1) << (3 * b)) | (((y >> b) & 1) << (3 * b + 1)) | (((z >> b) & 1) << (3 * b + 2));
+  return m;
+}
+__device__ __forceinline__ int cell_key(const Grid& g, int cx, int cy, int cz) {
+  int meta = ((cz >> 3) * g.mdim[1] + (cy >> 3)) * g.mdim[0] + (cx >> 3);
+  return meta * 512 + morton3(cx & 7, cy & 7, cz & 7);
+}
+__device__ __forceinline__ void cell_of(const Grid& g, float px, float py, float pz, int& cx,
+                                        int& cy, int& cz) {
+  cx = (int)floor(((double)px - g.org[0]) * g.inv_cs);
+  cy = (int)floor(((double)py - g.org[1]) * g.inv_cs);
+  cz = (int)floor(((double)pz - g.org[2]) * g.inv_cs);
+  cx = min(max(cx, 0), g.dim[0] - 1);
+  cy = min(max(cy, 0), g.dim[1] - 1);
+  cz = min(max(cz, 0), g.dim[2] - 1);
+}
+
+__global__ void k_ns_fill_i32(int* a, int v, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+__global__ void k_ns_keys(const float* p0, const float* p1, const float* p2, int n, Grid g,
+                          int* key, int* rank, int* cell_count) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int cx, cy, cz;
+  cell_of(g, p0[i], p1[i], p2[i], cx, cy, cz);
+  int k = cell_key(g, cx, cy, cz);
+  key[i] = k;
+  rank[i] = atomicAdd(&cell_count[k], 1);
+}
+
+// exclusive scan over int (length m), three phases
+__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const int* in, int64_t m, int* part) {
+  __shared__ int sh[kScanBlock];
+  int64_t i = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
+  sh[threadIdx.x] = i < m ? in[i] : 0;
+  __syncthreads();
+  for (int s = kScanBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+__global__ void k_scan_parts(int* part, int nb) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    int run = 0;
+    for (int b = 0; b < nb; b++) {
+      int t = part[b];
+      part[b] = run;
+      run += t;
+    }
+  }
+}
+__global__ __launch_bounds__(kScanBlock) void k_scan_apply(const int* in, int64_t m,
+                                                           const int* part, int* out) {
+  __shared__ int sh[kScanBlock];
+  int64_t i = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
+  int v = i < m ? in[i] : 0;
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  // Hillis-Steele inclusive scan
+  for (int off = 1; off < kScanBlock; off <<= 1) {
+    int t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += t;
+    __syncthreads();
+  }
+  if (i < m) out[i] = part[blockIdx.x] + sh[threadIdx.x] - v;
+  if (i == m - 1) out[m] = part[blockIdx.x] + sh[threadIdx.x];
+}
+
+__global__ void k_ns_place(const int* key, const int* rank, const int* cell_start,
+                           const int* perm, int n, int* slot_orig, int* slot_old) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int s = cell_start[key[i]] + rank[i];
+  slot_orig[s] = perm[i];
+  slot_old[s] = i;
+}
+
+// order each cell's particles by original index (one thread per cell)
+__global__ void k_ns_cell_sort(const int* cell_start, int64_t ncell, int* slot_orig,
+                               int* slot_old) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  int b = cell_start[c], e = cell_start[c + 1];
+  for (int s = b + 1; s < e; s++) {
+    int ko = slot_orig[s], kd = slot_old[s];
+    int t = s - 1;
+    while (t >= b && slot_orig[t] > ko) {
+      slot_orig[t + 1] = slot_orig[t];
+      slot_old[t + 1] = slot_old[t];
+      t--;
+    }
+    slot_orig[t + 1] = ko;
+    slot_old[t + 1] = kd;
+  }
+}
+
+__global__ void k_ns_gather_f32(float* dst, const float* src, const int* idx, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+
+__global__ __launch_bounds__(256) void k_ns_lists(const float* __restrict__ p0,
+                                                  const float* __restrict__ p1,
+                                                  const float* __restrict__ p2, int n, Grid g,
+                                                  const int* __restrict__ cell_start, float r2,
+                                                  int64_t npad, int* __restrict__ count,
+                                                  int* __restrict__ list, Counters* ctr) {
+#pragma clang fp contract(off)
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int over = 0;
+  if (i < n) {
+    const float xi = p0[i], yi = p1[i], zi = p2[i];
+    int cx, cy, cz;
+    cell_of(g, xi, yi, zi, cx, cy, cz);
+    int c = 0;
+    for (int dz = -1; dz <= 1; dz++) {
+      int z = cz + dz;
+      if (z < 0 || z >= g.dim[2]) continue;
+      for (int dy = -1; dy <= 1; dy++) {
+        int y = cy + dy;
+        if (y < 0 || y >= g.dim[1]) continue;
+        for (int dx = -1; dx <= 1; dx++) {
+          int x = cx + dx;
+          if (x < 0 || x >= g.dim[0]) continue;
+          int k = cell_key(g, x, y, z);
+          int b = cell_start[k], e = cell_start[k + 1];
+          for (int j = b; j < e; j++) {
+            float ddx = p0[j] - xi, ddy = p1[j] - yi, ddz = p2[j] - zi;
+            float d2 = __fadd_rn(__fadd_rn(__fmul_rn(ddx, ddx), __fmul_rn(ddy, ddy)),
+                                 __fmul_rn(ddz, ddz));
+            if (d2 < r2 && d2 > 0.0f) {
+              if (c < kMaxNb) list[(int64_t)c * npad + i] = j;
+              else over = 1;
+              c++;
+            }
+          }
+        }
+      }
+    }
+    count[i] = c < kMaxNb ? c : kMaxNb;
+  }
+  unsigned long long o = wave_sum_u64((unsigned long long)over);
+  if (lane_id() == 0 && o) atomicAdd(&ctr->nb_over_cap, (int)o);
+}
+
+static int grid1(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
+static int ensure(NsWork* w, int n, int64_t nkeys) {
+  if (n > w->n_cap) {
+    for (int c = 0; c < 3; c++) {
+      hipFree(w->pf[c]);
+      hipFree(w->pfn[c]);
+    }
+    hipFree(w->key);
+    hipFree(w->rank);
+    hipFree(w->slot_orig);
+    hipFree(w->slot_old);
+    int cap = std::max(n, 64);
+    for (int c = 0; c < 3; c++) {
+      if (hipMalloc(&w->pf[c], sizeof(float) * cap) != hipSuccess) return WF_E_NOMEM;
+      if (hipMalloc(&w->pfn[c], sizeof(float) * cap) != hipSuccess) return WF_E_NOMEM;
+    }
+    if (hipMalloc(&w->key, sizeof(int) * cap) != hipSuccess) return WF_E_NOMEM;
+    if (hipMalloc(&w->rank, sizeof(int) * cap) != hipSuccess) return WF_E_NOMEM;
+    if (hipMalloc(&w->slot_orig, sizeof(int) * cap) != hipSuccess) return WF_E_NOMEM;
+    if (hipMalloc(&w->slot_old, sizeof(int) * cap) != hipSuccess) return WF_E_NOMEM;
+    w->n_cap = cap;
+  }
+  if (nkeys > w->key_cap) {
+    hipFree(w->cell_count);
+    hipFree(w->cell_start);
+    hipFree(w->scan_part);
+    int64_t cap = nkeys + nkeys / 4 + 1024;
+    if (hipMalloc(&w->cell_count, sizeof(int) * cap) != hipSuccess) return WF_E_NOMEM;
+    if (hipMalloc(&w->cell_start, sizeof(int) * (cap + 1)) != hipSuccess) return WF_E_NOMEM;
+    if (hipMalloc(&w->scan_part, sizeof(int) * (grid1(cap, kScanBlock) + 1)) != hipSuccess)
+      return WF_E_NOMEM;
+    w->key_cap = cap;
+  }
+  if (!w->mm_part) {
+    if (hipMalloc(&w->mm_part, sizeof(float) * 6 * kMmBlocks) != hipSuccess) return WF_E_NOMEM;
+    if (hipMalloc(&w->mm, sizeof(float) * 6) != hipSuccess) return WF_E_NOMEM;
+    if (hipHostMalloc(&w->mm_host, sizeof(float) * 6) != hipSuccess) return WF_E_NOMEM;
+  }
+  return WF_OK;
+}
+
+void ns_free(NsWork* w) {
+  if (!w) return;
+  for (int c = 0; c < 3; c++) {
+    hipFree(w->pf[c]);
+    hipFree(w->pfn[c]);
+  }
+  hipFree(w->key);
+  hipFree(w->rank);
+  hipFree(w->slot_orig);
+  hipFree(w->slot_old);
+  hipFree(w->cell_count);
+  hipFree(w->cell_start);
+  hipFree(w->scan_part);
+  hipFree(w->mm_part);
+  hipFree(w->mm);
+  if (w->mm_host) hipHostFree(w->mm_host);
+  delete w;
+}
+
+// gather: out slot -> old sorted index; perm (in/out): sorted slot -> original
+int ns_build(hipStream_t s, NsWork** work, const double* const pos_src[3], int n, int64_t npad,
+             double radius, int* gather, int* count, int* list, Counters* ctr, int* over_cap,
+             int* perm, int* perm_tmp) {
+  if (!*work) *work = new NsWork();
+  NsWork* w = *work;
+  int rc = ensure(w, n, 0);
+  if (rc) return rc;
+  const float rf = (float)radius;
+  const float r2 = rf * rf;
+  hipLaunchKernelGGL(k_ns_to_float, dim3(grid1(n, 256)), dim3(256), 0, s, pos_src[0], pos_src[1],
+                     pos_src[2], w->pf[0], w->pf[1], w->pf[2], n);
+  int nbm = std::min(kMmBlocks, std::max(1, grid1(n, 256)));
+  hipLaunchKernelGGL(k_ns_minmax_part, dim3(nbm), dim3(256), 0, s, w->pf[0], w->pf[1], w->pf[2],
+                     n, w->mm_part);
+  hipLaunchKernelGGL(k_ns_minmax_final, dim3(1), dim3(64), 0, s, w->mm_part, nbm, w->mm);
+  if (hipMemcpyAsync(w->mm_host, w->mm, 6 * sizeof(float), hipMemcpyDeviceToHost, s) != hipSuccess)
+    return WF_E_HIP;
+  if (hipStreamSynchronize(s) != hipSuccess) return WF_E_HIP;
+  Grid g;
+  double cs = (double)rf * 1.01;
+  g.inv_cs = 1.0 / cs;
+  int64_t nkeys = 1;
+  for (int c = 0; c < 3; c++) {
+    double lo = w->mm_host[c], hi = w->mm_host[3 + c];
+    if (!std::isfinite(lo) || !std::isfinite(hi)) return WF_E_NONFINITE;
+    g.org[c] = lo - cs;
+    double ext = (hi - g.org[c]) / cs;
+    if (ext > 1.0e6) return WF_E_ARG;  // domain too sparse for a uniform grid
+    g.dim[c] = (int)std::floor(ext) + 2;
+    g.mdim[c] = (g.dim[c] + 7) / 8;
+    nkeys *= g.mdim[c];
+  }
+  nkeys *= 512;
+  if (nkeys > (int64_t)1 << 30) return WF_E_ARG;
+  rc = ensure(w, n, nkeys);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_ns_fill_i32, dim3(grid1(nkeys + 1, 256)), dim3(256), 0, s, w->cell_count, 0,
+                     nkeys + 1);
+  hipLaunchKernelGGL(k_ns_keys, dim3(grid1(n, 256)), dim3(256), 0, s, w->pf[0], w->pf[1],
+                     w->pf[2], n, g, w->key, w->rank, w->cell_count);
+  int nsb = grid1(nkeys, kScanBlock);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nsb), dim3(kScanBlock), 0, s, w->cell_count, nkeys,
+                     w->scan_part);
+  hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(64), 0, s, w->scan_part, nsb);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(kScanBlock), 0, s, w->cell_count, nkeys,
+                     w->scan_part, w->cell_start);
+  hipLaunchKernelGGL(k_ns_place, dim3(grid1(n, 256)), dim3(256), 0, s, w->key, w->rank,
+                     w->cell_start, perm, n, w->slot_orig, w->slot_old);
+  hipLaunchKernelGGL(k_ns_cell_sort, dim3(grid1(nkeys, 256)), dim3(256), 0, s, w->cell_start,
+                     nkeys, w->slot_orig, w->slot_old);
+  // new permutation and gather map
+  if (hipMemcpyAsync(perm_tmp, w->slot_orig, sizeof(int) * n, hipMemcpyDeviceToDevice, s) !=
+      hipSuccess)
+    return WF_E_HIP;
+  if (hipMemcpyAsync(gather, w->slot_old, sizeof(int) * n, hipMemcpyDeviceToDevice, s) !=
+      hipSuccess)
+    return WF_E_HIP;
+  for (int c = 0; c < 3; c++)
+    hipLaunchKernelGGL(k_ns_gather_f32, dim3(grid1(n, 256)), dim3(256), 0, s, w->pfn[c], w->pf[c],
+                       gather, n);
+  hipLaunchKernelGGL(k_ns_fill_i32, dim3(1), dim3(64), 0, s, &ctr->nb_over_cap, 0, (int64_t)1);
+  hipLaunchKernelGGL(k_ns_lists, dim3(grid1(n, 256)), dim3(256), 0, s, w->pfn[0], w->pfn[1],
+                     w->pfn[2], n, g, w->cell_start, r2, npad, count, list, ctr);
+  if (hipGetLastError() != hipSuccess) return WF_E_HIP;
+  (void)over_cap;
+  return WF_OK;
+}
+
+}  // namespace wf

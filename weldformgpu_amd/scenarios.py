@@ -1,0 +1,193 @@
+"""Synthetic input decks for the step pipeline (the workloads of the tests
+and the benchmark), built the way WeldFormGPU.cu's main builds a domain
+(WeldFormGPU.cu:199-906):
+
+  K = E/(3(1-2nu)), G = E/(2(1+nu)), dx = 2r, h = hFactor*dx, Cs = sqrt(K/rho)
+  (WeldFormGPU.cu:253-267); dt = 0.4 h/(Cs + VMAX), VMAX = 10
+  (WeldFormGPU.cu:62,904-906); lattice from AddCylinderLength/AddBoxLength;
+  zones by AssignZone; velocity BCs per zone.
+
+The lattices come from libwfhip's host builders (wf_gen_cylinder /
+wf_gen_box), which need no GPU.
+"""
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi
+
+
+@dataclass
+class Scenario:
+    name: str
+    cfg: _abi.wf_config
+    fields: dict
+    bcs: list = field(default_factory=list)
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n(self):
+        return self.cfg.n_particles
+
+
+def _lib():
+    lib = C.CDLL(_abi.LIB_PATH)
+    P, I, D = C.c_void_p, C.c_int, C.c_double
+    lib.wf_gen_box.restype = I
+    lib.wf_gen_box.argtypes = [P, D, D, D, D, D, P, I, P]
+    lib.wf_gen_cylinder.restype = I
+    lib.wf_gen_cylinder.argtypes = [P, D, D, D, D, P, I, P]
+    lib.wf_assign_zone.restype = I
+    lib.wf_assign_zone.argtypes = [P, I, P, P, I, P]
+    return lib
+
+
+def gen_cylinder(start, R, Lz, r, density):
+    lib = _lib()
+    V = np.ascontiguousarray(start, np.float64)
+    n = lib.wf_gen_cylinder(V.ctypes.data, R, Lz, r, density, None, 0, None)
+    xyz = np.zeros((n, 3))
+    m = C.c_double()
+    lib.wf_gen_cylinder(V.ctypes.data, R, Lz, r, density, xyz.ctypes.data, n, C.byref(m))
+    return xyz, m.value
+
+
+def gen_box(start, L, r, density):
+    lib = _lib()
+    V = np.ascontiguousarray(start, np.float64)
+    n = lib.wf_gen_box(V.ctypes.data, L[0], L[1], L[2], r, density, None, 0, None)
+    xyz = np.zeros((n, 3))
+    m = C.c_double()
+    lib.wf_gen_box(V.ctypes.data, L[0], L[1], L[2], r, density, xyz.ctypes.data, n, C.byref(m))
+    return xyz, m.value
+
+
+def assign_zone(xyz, ID, start, end, zid):
+    lib = _lib()
+    s = np.ascontiguousarray(start, np.float64)
+    e = np.ascontiguousarray(end, np.float64)
+    return lib.wf_assign_zone(xyz.ctypes.data, len(xyz), s.ctypes.data, e.ctypes.data, zid,
+                              ID.ctypes.data)
+
+
+def _mech_fields(xyz, mass, rho, h, Cs, G, sy0):
+    n = len(xyz)
+    return {
+        "x": xyz.copy(), "v": np.zeros((n, 3)), "rho": np.full(n, rho),
+        "rho0": np.full(n, rho), "m": np.full(n, mass), "h": np.full(n, h),
+        "Cs": np.full(n, Cs), "G": np.full(n, G), "sigma_y": np.full(n, sy0),
+        "ID": np.zeros(n, np.int32),
+    }
+
+
+def cylinder_compression(R, H, r, *, rho, E, nu, material, v_top, hfac=1.2, thermal=None,
+                         name="cylinder", layers=2):
+    """Cylinder on the z axis, bottom `layers` layers held at v=0 (zone 2),
+    top `layers` layers driven at (0,0,v_top) (zone 3)."""
+    xyz, mass = gen_cylinder([0.0, 0.0, 0.0], R, H, r, rho)
+    n = len(xyz)
+    dx = 2.0 * r
+    h = dx * hfac
+    K = E / (3. * (1. - 2 * nu))
+    G = E / (2. * (1. + nu))
+    Cs = np.sqrt(K / rho)
+    cfg = _abi.default_config()
+    cfg.n_particles = n
+    cfg.solid_count = n
+    cfg.h_glob = h
+    cfg.dt = 0.4 * h / (Cs + 10.0)
+    mp = cfg.mat
+    mp.E, mp.nu = E, nu
+    for k, v in material.items():
+        setattr(mp, k, v)
+    sy0 = material.get("sy0", 0.0)
+    f = _mech_fields(xyz, mass, rho, h, Cs, G, sy0)
+    zmin, zmax = xyz[:, 2].min(), xyz[:, 2].max()
+    big = 10 * R
+    lay = (layers - 0.5) * dx
+    assign_zone(xyz, f["ID"], [-big, -big, -big], [big, big, zmin + lay], 2)
+    assign_zone(xyz, f["ID"], [-big, -big, zmax - lay], [big, big, big], 3)
+    bcs = [(2, (0.0, 0.0, 0.0)), (3, (0.0, 0.0, v_top))]
+    if thermal:
+        cfg.thermal = 1
+        f["T"] = np.full(n, thermal["T0"])
+        f["k_T"] = np.full(n, thermal["k"])
+        f["cp_T"] = np.full(n, thermal["cp"])
+        cfg.taylor_quinney = thermal.get("taylor_quinney", 0.0)
+    return Scenario(name, cfg, f, bcs, {"R": R, "H": H, "r": r, "dx": dx, "h": h, "Cs": Cs,
+                                        "mass": mass})
+
+
+def c1_bilinear(scale=1.0):
+    """Elastic-plastic steel cylinder, bilinear hardening: R 7.5 mm, H 15 mm,
+    dx 0.5 mm (scale multiplies R and H), top at -1 m/s."""
+    E = 200e9
+    return cylinder_compression(7.5e-3 * scale, 15e-3 * scale, 0.25e-3, rho=7850.0, E=E, nu=0.3,
+                                material={"model": _abi.WF_MAT_BILINEAR, "sy0": 300e6,
+                                          "Et": 1e9},
+                                v_top=-1.0, name="c1_bilinear_cylinder")
+
+
+JC_AL = {"model": _abi.WF_MAT_JOHNSON_COOK, "A": 324e6, "B": 114e6, "n": 0.42, "C": 0.002,
+         "eps0": 1.0, "m": 1.34, "Tm": 925.0, "Tt": 300.0}
+
+
+def c3_johnson_cook(scale=1.0, r=0.15e-3):
+    """Coupled thermo-mechanical Johnson-Cook compression: R 15 mm, H 30 mm,
+    dx 0.3 mm, aluminium-like JC, Taylor-Quinney 0.9, top at -2 m/s,
+    FP64 Newton return map."""
+    T0 = 300.0
+    sc = cylinder_compression(15e-3 * scale, 30e-3 * scale, r, rho=2700.0, E=68.9e9, nu=0.3,
+                              material=dict(JC_AL, sy0=JC_AL["A"]), v_top=-2.0,
+                              thermal={"T0": T0, "k": 200.0, "cp": 900.0,
+                                       "taylor_quinney": 0.9},
+                              name="c3_jc_thermomech_cylinder")
+    sc.cfg.return_map = _abi.WF_RETURN_NEWTON
+    # initial flow stress at eps_p = 0, rate 0 (-> 1e-5), T0: Material.cuh:209-219
+    mp = sc.cfg.mat
+    th = min(max((T0 - mp.Tt) / (mp.Tm - mp.Tt), 0.0), 1.0)
+    sy = mp.A * (1.0 + mp.C * np.log(1e-5 / mp.eps0)) * (1.0 - th ** mp.m)
+    sc.fields["sigma_y"][:] = sy
+    return sc
+
+
+def c2_thermal_cube(L=46e-3, r=0.5e-3, seed=42):
+    """Conduction only: cube of edge L, dx = 2r, rho 2700, cp 900, k 200,
+    T = 300 + 500 exp(-|x-c|^2/(2 sigma^2)) + U(-1,1), sigma 5 mm; the first
+    and last lattice layer on each axis held at 300 K; dt = 0.3 h^2 rho cp/k
+    (Domain_d.cu:509)."""
+    rho, cp, k = 2700.0, 900.0, 200.0
+    xyz, mass = gen_box([0.0, 0.0, 0.0], [L, L, L], r, rho)
+    n = len(xyz)
+    dx = 2.0 * r
+    h = 1.2 * dx
+    cfg = _abi.default_config()
+    cfg.n_particles = n
+    cfg.solid_count = n
+    cfg.h_glob = h
+    cfg.mechanics = 0
+    cfg.thermal = 1
+    cfg.dt = 0.3 * h * h * rho * cp / k
+    c = np.array([L / 2, L / 2, L / 2])
+    sig = 5e-3
+    rng = np.random.default_rng(seed)
+    d2 = ((xyz - c) ** 2).sum(1)
+    T = 300.0 + 500.0 * np.exp(-d2 / (2 * sig * sig)) + rng.uniform(-1.0, 1.0, n)
+    lo, hi = xyz.min(0), xyz.max(0)
+    face = np.zeros(n, bool)
+    for a in range(3):
+        face |= np.isclose(xyz[:, a], lo[a]) | np.isclose(xyz[:, a], hi[a])
+    T[face] = 300.0
+    bct = np.where(face, _abi.WF_BCT_FIXED, _abi.WF_BCT_NONE).astype(np.int32)
+    f = {"x": xyz.copy(), "rho": np.full(n, rho), "rho0": np.full(n, rho),
+         "m": np.full(n, mass), "h": np.full(n, h), "T": T, "k_T": np.full(n, k),
+         "cp_T": np.full(n, cp), "BC_T": bct, "ID": np.zeros(n, np.int32)}
+    return Scenario("c2_thermal_cube", cfg, f, [], {"L": L, "r": r, "dx": dx, "h": h})
+
+
+def load_domain(dom, sc):
+    """Upload a scenario's fields and BCs into a Domain."""
+    for k, v in sc.fields.items():
+        dom.set(k, v)
+    dom.set_bcs(sc.bcs)
