@@ -62,8 +62,14 @@ struct wf_domain {
   double* sum_part = nullptr;
   double* sum_out = nullptr;
   NsWork* ns = nullptr;
+  TileLists tl{};
   bool nb_built = false;
   bool snapshot_set = false;
+  // the search reads the snapshot x_h under WF_Q_STALE_NS_POS: a rebuild
+  // from an unchanged snapshot reproduces the current order and lists, so
+  // it is skipped (snap_gen counts snapshot writes, built_gen is the one the
+  // lists were built from)
+  int64_t snap_gen = 0, built_gen = -1;
   BcTable bcs{};
   // run state (SolverFraser.cu:30-409)
   double time = 0.0;
@@ -415,12 +421,14 @@ int wf_set_field(wf_domain* d, int field, const void* host, size_t bytes) {
     hipLaunchKernelGGL(k_upload_i32, dim3(gridn(d->n)), dim3(256), 0, d->stream,
                        *d->plane(field, 0).i, (const int*)d->stage, d->perm, d->n);
   }
+  if (field == WF_F_X_SNAPSHOT) d->snap_gen++;
   // x also initialises the search snapshot x_h (SolverFraser.cu:56)
   if (field == WF_F_X) {
     for (int c = 0; c < 3; c++)
       hipLaunchKernelGGL(k_copy_f64, dim3(gridn(d->n)), dim3(256), 0, d->stream, d->f.xs[c],
                          d->f.x[c], d->n);
     d->snapshot_set = true;
+    d->snap_gen++;
   }
   HIPC(hipGetLastError());
   HIPC(hipStreamSynchronize(d->stream));
@@ -452,8 +460,11 @@ int wf_fill_field(wf_domain* d, int field, double value) {
   int comps, elem;
   if (field_info(field, &comps, &elem) || elem != 8) return fail(WF_E_ARG, "not a double field");
   for (int c = 0; c < comps; c++) launch_fill_f64(d->stream, *d->plane(field, c).d, value, d->n);
-  if (field == WF_F_X)
+  if (field == WF_F_X) {
     for (int c = 0; c < 3; c++) launch_fill_f64(d->stream, d->f.xs[c], value, d->n);
+    d->snap_gen++;
+  }
+  if (field == WF_F_X_SNAPSHOT) d->snap_gen++;
   HIPC(hipGetLastError());
   HIPC(hipStreamSynchronize(d->stream));
   return WF_OK;
@@ -465,6 +476,7 @@ int wf_snapshot_positions(wf_domain* d) {
     hipLaunchKernelGGL(k_copy_f64, dim3(gridn(d->n)), dim3(256), 0, d->stream, d->f.xs[c],
                        d->f.x[c], d->n);
   d->snapshot_set = true;
+  d->snap_gen++;
   HIPC(hipGetLastError());
   return WF_OK;
 }
@@ -497,15 +509,21 @@ static int timer_end(wf_domain* d, int k) {
 
 int wf_nsearch_build(wf_domain* d) {
   if (!d) return fail(WF_E_ARG, "null domain");
-  timer_begin(d);
   const bool stale = (d->cfg.quirks & WF_Q_STALE_NS_POS) != 0;
+  if (stale && d->nb_built && d->built_gen == d->snap_gen) {
+    d->ns_builds++;
+    return WF_OK;
+  }
+  timer_begin(d);
   const double* src[3];
   for (int c = 0; c < 3; c++) src[c] = stale ? d->f.xs[c] : d->f.x[c];
   int over = 0;
   int rc = ns_build(d->stream, &d->ns, src, d->n, d->npad, 2.0 * d->cfg.h_glob, d->gather,
-                    d->nb_count, d->nb_list, d->ctr, &over, d->perm, d->perm_tmp);
-  if (rc) return fail(rc, rc == WF_E_NONFINITE ? "non-finite particle position"
-                                               : "neighbour search failed");
+                    d->nb_count, d->nb_list, d->ctr, &over, d->perm, d->perm_tmp, &d->tl);
+  if (rc)
+    return fail(rc, rc == WF_E_NONFINITE ? "non-finite particle position"
+                    : rc == WF_E_NBCAP   ? "a neighbour tile exceeds its cell/slot budget"
+                                         : "neighbour search failed");
   // physically reorder every field into the new order
   for (PlaneRef r : d->all_planes()) {
     if (r.d) {
@@ -519,6 +537,7 @@ int wf_nsearch_build(wf_domain* d) {
   std::swap(d->perm, d->perm_tmp);
   HIPC(hipGetLastError());
   d->nb_built = true;
+  d->built_gen = d->snap_gen;
   d->ns_builds++;
   return timer_end(d, WF_K_NSEARCH);
 }
@@ -568,7 +587,6 @@ int wf_get_neighbors(wf_domain* d, int* counts, int* offsets, int* list, int64_t
   return WF_OK;
 }
 
-static NbLists lists(const wf_domain* d) { return NbLists{d->nb_count, d->nb_list, d->npad}; }
 
 #define NEED_NB(d) \
   if (!(d)->nb_built) return fail(WF_E_STATE, "neighbour lists not built (call wf_nsearch_build)")
@@ -576,7 +594,7 @@ static NbLists lists(const wf_domain* d) { return NbLists{d->nb_count, d->nb_lis
 int wf_calc_dens_inc(wf_domain* d) {
   if (!d) return fail(WF_E_ARG, "null domain");
   NEED_NB(d);
-  launch_dens_inc(d->stream, d->f, lists(d), d->params(), false);
+  launch_dens_tiled(d->stream, d->f, d->tl, d->n, d->solid, d->cfg.dt, false);
   HIPC(hipGetLastError());
   return WF_OK;
 }
@@ -589,7 +607,7 @@ int wf_update_density(wf_domain* d, double dt) {
 int wf_calc_rate_tensors(wf_domain* d) {
   if (!d) return fail(WF_E_ARG, "null domain");
   NEED_NB(d);
-  launch_rate_tensors(d->stream, d->f, lists(d), d->params());
+  launch_rates_tiled(d->stream, d->f, d->tl, d->solid, d->cfg.quirks);
   HIPC(hipGetLastError());
   return WF_OK;
 }
@@ -608,7 +626,7 @@ int wf_stress_strain(wf_domain* d) {
 int wf_calc_accel(wf_domain* d) {
   if (!d) return fail(WF_E_ARG, "null domain");
   NEED_NB(d);
-  launch_accel(d->stream, d->f, lists(d), d->params());
+  launch_accel_tiled(d->stream, d->f, d->tl, d->solid, d->cfg.alpha, d->cfg.beta);
   HIPC(hipGetLastError());
   return WF_OK;
 }
@@ -633,16 +651,15 @@ int wf_update_vel(wf_domain* d, double dt) {
 int wf_calc_temp_inc(wf_domain* d) {
   if (!d) return fail(WF_E_ARG, "null domain");
   NEED_NB(d);
-  launch_temp_inc(d->stream, d->f, lists(d), d->params());
+  launch_cond_tiled(d->stream, d->f, d->tl, d->n, d->cfg.T_inf, d->cfg.h_conv, d->cfg.dt, false,
+                    false);
   HIPC(hipGetLastError());
   return WF_OK;
 }
 int wf_thermal_calcs(wf_domain* d, double dt) {
   if (!d) return fail(WF_E_ARG, "null domain");
   NEED_NB(d);
-  StepParams p = d->params();
-  p.dt = dt;
-  launch_temp_inc(d->stream, d->f, lists(d), p);
+  launch_cond_tiled(d->stream, d->f, d->tl, d->n, d->cfg.T_inf, d->cfg.h_conv, dt, false, false);
   launch_leapfrog(d->stream, d->f, d->n, dt, d->isfirst_step);
   d->isfirst_step = false;
   HIPC(hipGetLastError());
@@ -667,7 +684,6 @@ int wf_step(wf_domain* d, int n_steps) {
     rc = wf_nsearch_build(d);
     if (rc) return rc;
   }
-  const NbLists nbl = lists(d);
   for (int s = 0; s < n_steps; s++) {
     StepParams p = d->params();
     if (d->cfg.mechanics) {
@@ -676,17 +692,17 @@ int wf_step(wf_domain* d, int n_steps) {
         if (rc) return rc;
       }
       timer_begin(d);
-      launch_dens_inc(d->stream, d->f, nbl, p, true);
+      launch_dens_tiled(d->stream, d->f, d->tl, d->n, d->solid, p.dt, true);
       std::swap(d->f.rho, d->f.rho_alt);
       if ((rc = timer_end(d, WF_K_DENS))) return rc;
       timer_begin(d);
-      launch_rate_tensors(d->stream, d->f, nbl, p);
+      launch_rates_tiled(d->stream, d->f, d->tl, d->solid, p.quirks);
       if ((rc = timer_end(d, WF_K_RATES))) return rc;
       timer_begin(d);
       launch_stress(d->stream, d->f, p, d->ctr, true, true);
       if ((rc = timer_end(d, WF_K_STRESS))) return rc;
       timer_begin(d);
-      launch_accel(d->stream, d->f, nbl, p);
+      launch_accel_tiled(d->stream, d->f, d->tl, d->solid, p.alpha, p.beta);
       if ((rc = timer_end(d, WF_K_ACCEL))) return rc;
       if (!d->is_yielding) {  // SolverFraser.cu:260-271, on the device-side max
         HIPC(hipMemcpyAsync(d->ctr_host, d->ctr, sizeof(Counters), hipMemcpyDeviceToHost,
@@ -703,7 +719,8 @@ int wf_step(wf_domain* d, int n_steps) {
     }
     if (d->cfg.thermal) {
       timer_begin(d);
-      launch_thermal_fused(d->stream, d->f, nbl, p, d->isfirst_step);
+      launch_cond_tiled(d->stream, d->f, d->tl, d->n, p.T_inf, p.h_conv, p.dt, true,
+                        d->isfirst_step);
       std::swap(d->f.T, d->f.T_alt);
       if ((rc = timer_end(d, WF_K_THERMAL))) return rc;
     }

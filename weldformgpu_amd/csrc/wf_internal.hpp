@@ -10,6 +10,7 @@
 #include <cstdint>
 
 #include "../../include/wf_abi.h"
+#include "wf_tiles.hpp"
 
 namespace wf {
 
@@ -70,24 +71,27 @@ struct Counters {
   int nonfinite;
 };
 
-// ---- launchers (wf_kernels.hip) ------------------------------------------
-void launch_dens_inc(hipStream_t s, const Fields& f, const NbLists& nb, const StepParams& p,
-                     bool fuse_update);
+// ---- LDS-tiled pair loops (wf_pair.hip) ----------------------------------
+void launch_dens_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int n, int solid,
+                       double dt, bool fuse);
+void launch_rates_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
+                        int quirks);
+void launch_accel_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
+                        double alpha, double beta);
+void launch_cond_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int n, double T_inf,
+                       double h_conv, double dt, bool leap, bool first);
+
+// ---- per-particle kernels (wf_kernels.hip) --------------------------------
 void launch_update_density(hipStream_t s, const Fields& f, const StepParams& p, double dt);
-void launch_rate_tensors(hipStream_t s, const Fields& f, const NbLists& nb, const StepParams& p);
 void launch_pressure(hipStream_t s, const Fields& f, const StepParams& p);
 void launch_stress(hipStream_t s, const Fields& f, const StepParams& p, Counters* ctr,
                    bool fuse_pressure, bool energy);
-void launch_accel(hipStream_t s, const Fields& f, const NbLists& nb, const StepParams& p);
 void launch_apply_bc(hipStream_t s, const Fields& f, int n, int zone, double vx, double vy,
                      double vz);
 void launch_update_pos(hipStream_t s, const Fields& f, int n, double dt);
 void launch_update_vel(hipStream_t s, const Fields& f, int n, double dt);
 void launch_integrate(hipStream_t s, const Fields& f, int n, double dt, const BcTable& bc);
-void launch_temp_inc(hipStream_t s, const Fields& f, const NbLists& nb, const StepParams& p);
 void launch_leapfrog(hipStream_t s, const Fields& f, int n, double dt, bool first);
-void launch_thermal_fused(hipStream_t s, const Fields& f, const NbLists& nb, const StepParams& p,
-                          bool first);
 void launch_int_energy(hipStream_t s, const Fields& f, const StepParams& p);
 void launch_sum(hipStream_t s, const double* v, int n, double* partial, int nblocks,
                 double* out);
@@ -104,7 +108,7 @@ struct NsResult {
 // the new one.
 int ns_build(hipStream_t s, NsWork** work, const double* const pos_src[3], int n, int64_t npad,
              double radius, int* gather, int* count, int* list, Counters* ctr, int* over_cap,
-             int* perm, int* perm_tmp);
+             int* perm, int* perm_tmp, TileLists* tiles);
 void ns_free(NsWork* w);
 
 // generic gathers used to reorder fields: dst[i] = src[idx[i]]

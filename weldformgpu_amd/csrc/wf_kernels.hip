@@ -1,103 +1,17 @@
-// wf_kernels.hip — FP64 per-step kernels of the SPH pipeline (gfx950).
-//
-// Pair loops run one thread per particle over Morton-sorted SoA data: the
-// 64 lanes of a wavefront are 64 spatially adjacent particles, so their
-// neighbour gathers hit the same L1/L2 lines, and the interleaved neighbour
-// list gives one coalesced 256-byte load per list slot.  Every sum keeps the
-// reference's per-pair arithmetic (InteractionAlt.cu, Mechanical.cu,
-// Thermal.cu); only the neighbour order differs.
+// wf_kernels.hip — FP64 per-particle kernels of the SPH pipeline (gfx950):
+// pressure + Jaumann stress + return map + energy, BC + integration,
+// leapfrog, reductions and reorder helpers.  One thread per particle over
+// Morton-sorted SoA planes, so every load and store is coalesced.  The pair
+// loops are in wf_pair.hip.
 #include "wf_device.hpp"
 
 namespace wf {
 
 static inline int grid_for(int n) { return (n + kBlock - 1) / kBlock; }
 
-// ---------------------------------------------------------------------------
-// A: density rate, CalcDensInc (InteractionAlt.cu:166-246).  With FUSE the
-// density update (Mechanical.cu:198-203) is applied into the second density
-// buffer, so every thread still reads the old rho_j.
-template <bool FUSE>
-__global__ __launch_bounds__(kBlock) void k_dens_inc(Fields f, NbLists nb, int n, int solid,
-                                                     double dt) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double rhoi = f.rho[i];
-  if (i >= solid) {
-    if (FUSE) f.rho_alt[i] = rhoi;
-    return;
-  }
-  const double xi = f.x[0][i], yi = f.x[1][i], zi = f.x[2][i];
-  const double vxi = f.v[0][i], vyi = f.v[1][i], vzi = f.v[2][i];
-  const double hi = f.h[i];
-  const int cnt = nb.count[i];
-  double acc = 0.0;
-  for (int k = 0; k < cnt; k++) {
-    const int j = nb.list[(int64_t)k * nb.npad + i];
-    double xx = xi - f.x[0][j], xy = yi - f.x[1][j], xz = zi - f.x[2][j];
-    double rij = sqrt(xx * xx + xy * xy + xz * xz);
-    double dj = f.rho[j], mj = f.m[j];
-    double vx = vxi - f.v[0][j], vy = vyi - f.v[1][j], vz = vzi - f.v[2][j];
-    double h_ = (hi + f.h[j]) / 2.0;
-    double GK = grad_kernel(rij / h_, h_);
-    double temp1 = vx * (GK * xx) + vy * (GK * xy) + vz * (GK * xz);
-    acc += mj * (rhoi / dj) * temp1;
-  }
-  f.drho[i] = acc;
-  if (FUSE) f.rho_alt[i] = rhoi + dt * acc;
-}
-
 __global__ __launch_bounds__(kBlock) void k_update_density(Fields f, int solid, double dt) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < solid) f.rho[i] += dt * f.drho[i];
-}
-
-// ---------------------------------------------------------------------------
-// B: strain-rate and rotation-rate tensors, CalcRateTensors
-// (InteractionAlt.cu:262-383), stored [xx,yy,zz,xy,yz,xz].  With the xz quirk
-// the rotation-rate xz is negated (InteractionAlt.cu:362).
-__global__ __launch_bounds__(kBlock) void k_rate_tensors(Fields f, NbLists nb, int solid,
-                                                         int quirks) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= solid) return;
-  const double xi = f.x[0][i], yi = f.x[1][i], zi = f.x[2][i];
-  const double vxi = f.v[0][i], vyi = f.v[1][i], vzi = f.v[2][i];
-  const double hi = f.h[i];
-  const double xz_sign = (quirks & WF_Q_ROT_XZ_SIGN) ? -1.0 : 1.0;
-  const int cnt = nb.count[i];
-  double sxx = 0, syy = 0, szz = 0, sxy = 0, syz = 0, sxz = 0;
-  double wxy = 0, wyz = 0, wxz = 0;
-  for (int k = 0; k < cnt; k++) {
-    const int j = nb.list[(int64_t)k * nb.npad + i];
-    double xx = xi - f.x[0][j], xy = yi - f.x[1][j], xz = zi - f.x[2][j];
-    double rij = sqrt(xx * xx + xy * xy + xz * xz);
-    double dj = f.rho[j], mj = f.m[j];
-    double vx = vxi - f.v[0][j], vy = vyi - f.v[1][j], vz = vzi - f.v[2][j];
-    double h_ = (hi + f.h[j]) / 2.0;
-    double GK = grad_kernel(rij / h_, h_);
-    double fs = (-0.5) * GK;
-    double mj_dj = mj / dj;
-    sxx += (2.0 * vx * xx) * fs * mj_dj;
-    syy += (2.0 * vy * xy) * fs * mj_dj;
-    szz += (2.0 * vz * xz) * fs * mj_dj;
-    sxy += (vx * xy + vy * xx) * fs * mj_dj;
-    syz += (vy * xz + vz * xy) * fs * mj_dj;
-    sxz += (vx * xz + vz * xx) * fs * mj_dj;
-    wxy += (vx * xy - vy * xx) * fs * mj_dj;
-    wyz += (vy * xz - vz * xy) * fs * mj_dj;
-    wxz += (xz_sign * (vx * xz - vz * xx)) * fs * mj_dj;
-  }
-  f.strrate[0][i] = sxx;
-  f.strrate[1][i] = syy;
-  f.strrate[2][i] = szz;
-  f.strrate[3][i] = sxy;
-  f.strrate[4][i] = syz;
-  f.strrate[5][i] = sxz;
-  f.rotrate[0][i] = 0.0;
-  f.rotrate[1][i] = 0.0;
-  f.rotrate[2][i] = 0.0;
-  f.rotrate[3][i] = wxy;
-  f.rotrate[4][i] = wyz;
-  f.rotrate[5][i] = wxz;
 }
 
 // ---------------------------------------------------------------------------
@@ -151,13 +65,16 @@ __device__ double jc_newton(const MatParams& mp, double s_trial, double G, doubl
   double g = g0;
   if (!(g > lo) || !(g < hi)) g = (g0 >= hi) ? hi : lo;
   int it = 0;
+  const double rt = rate_f * therm_f;
   for (; it < max_iter; it++) {
-    double sy = jc_yield(mp, ep + g, rate_f, therm_f);
-    double r = s_trial - 3.0 * G * g - sy;
+    // one pow per iterate: s_y = (A + B x^n) rt, H = n B x^n / x rt
+    const double x = ep + g;
+    const double xn = pow(x, mp.n);
+    double r = s_trial - 3.0 * G * g - (mp.A + mp.B * xn) * rt;
     if (fabs(r) <= tol * s_trial) break;
     if (r > 0.0) lo = g;
     else hi = g;
-    double H = jc_hardening(mp, ep + g, rate_f, therm_f);
+    double H = x > 0.0 ? mp.n * mp.B * (xn / x) * rt : HUGE_VAL;
     double gn = g + r / (3.0 * G + H);
     if (!(gn > lo && gn < hi)) gn = 0.5 * (lo + hi);
     g = gn;
@@ -311,55 +228,6 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Count
 }
 
 // ---------------------------------------------------------------------------
-// D: acceleration, CalcAccel (InteractionAlt.cu:18-150): Monaghan viscosity
-// Pi_ij = (alpha C mu + beta mu^2)/rho_bar when v_ij.x_ij < 0,
-// a_i = sum m_j (sigma_i/rho_i^2 + sigma_j/rho_j^2 + Pi_ij I) . (GK x_ij)
-__global__ __launch_bounds__(kBlock) void k_accel(Fields f, NbLists nb, int solid, double alpha,
-                                                  double beta) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= solid) return;
-  const double xi = f.x[0][i], yi = f.x[1][i], zi = f.x[2][i];
-  const double vxi = f.v[0][i], vyi = f.v[1][i], vzi = f.v[2][i];
-  const double hi = f.h[i], di = f.rho[i], Ci = f.Cs[i];
-  const double s0 = f.sigma[0][i], s1 = f.sigma[1][i], s2 = f.sigma[2][i];
-  const double s3 = f.sigma[3][i], s4 = f.sigma[4][i], s5 = f.sigma[5][i];
-  const double idi = 1.0 / (di * di);
-  const bool visc = (alpha != 0.0 || beta != 0.0);
-  const int cnt = nb.count[i];
-  double ax = 0.0, ay = 0.0, az = 0.0;
-  for (int k = 0; k < cnt; k++) {
-    const int j = nb.list[(int64_t)k * nb.npad + i];
-    double xx = xi - f.x[0][j], xy = yi - f.x[1][j], xz = zi - f.x[2][j];
-    double rij = sqrt(xx * xx + xy * xy + xz * xz);
-    double dj = f.rho[j], mj = f.m[j];
-    double vx = vxi - f.v[0][j], vy = vyi - f.v[1][j], vz = vzi - f.v[2][j];
-    double h_ = (hi + f.h[j]) / 2.0;
-    double GK = grad_kernel(rij / h_, h_);
-    double PI = 0.0;
-    if (visc) {
-      double vdx = vx * xx + vy * xy + vz * xz;
-      double MUij = h_ * vdx / (rij * rij + 0.01 * h_ * h_);
-      double Cij = 0.5 * (Ci + f.Cs[j]);
-      if (vdx < 0) PI = (alpha * Cij * MUij + beta * MUij * MUij) / (0.5 * (di + dj));
-    }
-    double idj = 1.0 / (dj * dj);
-    double mxx = (s0 * idi + f.sigma[0][j] * idj + PI) * mj;
-    double myy = (s1 * idi + f.sigma[1][j] * idj + PI) * mj;
-    double mzz = (s2 * idi + f.sigma[2][j] * idj + PI) * mj;
-    double mxy = (s3 * idi + f.sigma[3][j] * idj) * mj;
-    double myz = (s4 * idi + f.sigma[4][j] * idj) * mj;
-    double mxz = (s5 * idi + f.sigma[5][j] * idj) * mj;
-    double gx = GK * xx, gy = GK * xy, gz = GK * xz;
-    ax += mxx * gx + mxy * gy + mxz * gz;
-    ay += mxy * gx + myy * gy + myz * gz;
-    az += mxz * gx + myz * gy + mzz * gz;
-  }
-  f.a[0][i] = ax;
-  f.a[1][i] = ay;
-  f.a[2][i] = az;
-}
-
-// ---------------------------------------------------------------------------
 // F: velocity BCs + UpdatePosFraser + UpdateVel + BCs in one pass
 // (SolverFraser.cu:273-281; Mechanical.cu:142-191,572-584)
 __global__ __launch_bounds__(kBlock) void k_integrate(Fields f, int n, double dt, BcTable bc) {
@@ -431,60 +299,6 @@ __global__ __launch_bounds__(kBlock) void k_update_vel(Fields f, int n, double d
   for (int c = 0; c < 3; c++) f.v[c][i] += f.a[c][i] * dt;
 }
 
-// ---------------------------------------------------------------------------
-// G: conduction (Thermal.cu:101-142, Fraser eq. 3.99) + convection
-// (Thermal.cu:36-64) + plastic-work source; FIXED BC holds T.
-// LEAP: fuse the leapfrog update (Thermal.cu:16-31), writing T into T_alt so
-// that neighbours keep reading the old T.
-template <bool LEAP>
-__global__ __launch_bounds__(kBlock) void k_thermal(Fields f, NbLists nb, int n, double T_inf,
-                                                    double h_conv, double dt, int first) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double xi = f.x[0][i], yi = f.x[1][i], zi = f.x[2][i];
-  const double hi = f.h[i], ki = f.k[i], Ti = f.T[i];
-  const int cnt = nb.count[i];
-  double acc = 0.0;
-  for (int k = 0; k < cnt; k++) {
-    const int j = nb.list[(int64_t)k * nb.npad + i];
-    double xx = xi - f.x[0][j], xy = yi - f.x[1][j], xz = zi - f.x[2][j];
-    double h_ = (hi + f.h[j]) / 2.0;
-    double r2 = xx * xx + xy * xy + xz * xz;
-    double nxij = sqrt(r2);
-    double GK = grad_kernel(nxij / h_, h_);
-    double kj = f.k[j];
-    acc += f.m[j] / f.rho[j] * (4.0 * ki * kj / (ki + kj) * (Ti - f.T[j])) *
-           (xx * (GK * xx) + xy * (GK * xy) + xz * (GK * xz)) / (nxij * nxij);
-  }
-  const double rhoi = f.rho[i], cpi = f.cp[i];
-  acc *= 1. / (rhoi * cpi);
-  const int bct = f.BC_T ? f.BC_T[i] : 0;
-  if (bct == WF_BCT_CONVECTION) {
-    double mi = f.m[i];
-    double dS2 = pow(mi / rhoi, 0.666666666);
-    double q_conv = h_conv * dS2 * (T_inf - Ti) / mi;
-    acc += 1. / cpi * q_conv;
-  }
-  if (f.qpl) {
-    double q = f.qpl[i];
-    if (q != 0.0) acc += q * (1. / (rhoi * cpi));
-  }
-  if (bct == WF_BCT_FIXED) acc = 0.0;
-  f.dTdt[i] = acc;
-  if (LEAP) {
-    if (first) {
-      f.Ta[i] = Ti - dt / 2.0 * acc;
-      f.T_alt[i] = Ti;
-    } else {
-      double ta = f.Ta[i];
-      f.Tb[i] = ta;
-      double tan = ta + acc * dt;
-      f.Ta[i] = tan;
-      f.T_alt[i] = (tan + ta) / 2.;
-    }
-  }
-}
-
 __global__ __launch_bounds__(kBlock) void k_leapfrog(Fields f, int n, double dt, int first) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -554,25 +368,10 @@ __global__ void k_fill_f64(double* dst, double v, int n) {
 }
 
 // ---- launchers --------------------------------------------------------------
-void launch_dens_inc(hipStream_t s, const Fields& f, const NbLists& nb, const StepParams& p,
-                     bool fuse) {
-  if (p.n <= 0) return;
-  if (fuse)
-    hipLaunchKernelGGL(k_dens_inc<true>, dim3(grid_for(p.n)), dim3(kBlock), 0, s, f, nb, p.n,
-                       p.solid, p.dt);
-  else
-    hipLaunchKernelGGL(k_dens_inc<false>, dim3(grid_for(p.n)), dim3(kBlock), 0, s, f, nb, p.n,
-                       p.solid, p.dt);
-}
 void launch_update_density(hipStream_t s, const Fields& f, const StepParams& p, double dt) {
   if (p.solid <= 0) return;
   hipLaunchKernelGGL(k_update_density, dim3(grid_for(p.solid)), dim3(kBlock), 0, s, f, p.solid,
                      dt);
-}
-void launch_rate_tensors(hipStream_t s, const Fields& f, const NbLists& nb, const StepParams& p) {
-  if (p.solid <= 0) return;
-  hipLaunchKernelGGL(k_rate_tensors, dim3(grid_for(p.solid)), dim3(kBlock), 0, s, f, nb,
-                     p.solid, p.quirks);
 }
 void launch_pressure(hipStream_t s, const Fields& f, const StepParams& p) {
   if (p.n <= 0) return;
@@ -591,11 +390,6 @@ void launch_stress(hipStream_t s, const Fields& f, const StepParams& p, Counters
   else
     hipLaunchKernelGGL((k_stress<false, false>), g, b, 0, s, f, p, ctr);
 }
-void launch_accel(hipStream_t s, const Fields& f, const NbLists& nb, const StepParams& p) {
-  if (p.solid <= 0) return;
-  hipLaunchKernelGGL(k_accel, dim3(grid_for(p.solid)), dim3(kBlock), 0, s, f, nb, p.solid,
-                     p.alpha, p.beta);
-}
 void launch_apply_bc(hipStream_t s, const Fields& f, int n, int zone, double vx, double vy,
                      double vz) {
   if (n <= 0) return;
@@ -612,17 +406,6 @@ void launch_update_vel(hipStream_t s, const Fields& f, int n, double dt) {
 void launch_integrate(hipStream_t s, const Fields& f, int n, double dt, const BcTable& bc) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_integrate, dim3(grid_for(n)), dim3(kBlock), 0, s, f, n, dt, bc);
-}
-void launch_temp_inc(hipStream_t s, const Fields& f, const NbLists& nb, const StepParams& p) {
-  if (p.n <= 0) return;
-  hipLaunchKernelGGL(k_thermal<false>, dim3(grid_for(p.n)), dim3(kBlock), 0, s, f, nb, p.n,
-                     p.T_inf, p.h_conv, p.dt, 0);
-}
-void launch_thermal_fused(hipStream_t s, const Fields& f, const NbLists& nb, const StepParams& p,
-                          bool first) {
-  if (p.n <= 0) return;
-  hipLaunchKernelGGL(k_thermal<true>, dim3(grid_for(p.n)), dim3(kBlock), 0, s, f, nb, p.n,
-                     p.T_inf, p.h_conv, p.dt, first ? 1 : 0);
 }
 void launch_leapfrog(hipStream_t s, const Fields& f, int n, double dt, bool first) {
   if (n <= 0) return;

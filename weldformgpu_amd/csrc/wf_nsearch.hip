@@ -12,9 +12,10 @@
 //     deterministic (the reference's atomic ranks are not,
 //     cuNSearchKernels.cu:104);
 //  4. the caller gathers every field into the new order;
-//  5. one thread per particle walks the 27 neighbour cells and appends j
-//     with 0 < d2 < r2 in float, d2 = dx*dx + dy*dy + dz*dz without
-//     contraction (cuNSearchKernels.cu:160-165), cap 70 (:169).
+//  5. per tile of the new order, the occupied neighbour cells, the staged
+//     set and the neighbour lists (wf_tiles.hip): j is a neighbour of i iff
+//     0 < d2 < r2 in float, d2 = dx*dx + dy*dy + dz*dz without contraction
+//     (cuNSearchKernels.cu:160-165), at most 70 kept (:169).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -22,6 +23,7 @@
 #include <vector>
 
 #include "wf_device.hpp"
+#include "wf_tiles.hpp"
 
 namespace wf {
 
@@ -40,6 +42,18 @@ struct NsWork {
   float* mm_part = nullptr;  // per block 6 floats
   float* mm = nullptr;       // 6 floats
   float* mm_host = nullptr;  // pinned
+  // tiles
+  int nt_cap = 0;
+  int* tile_keys = nullptr;
+  int* tile_nc = nullptr;
+  int* tile_ucnt = nullptr;
+  int* tile_uoff = nullptr;
+  int* tile_scan = nullptr;
+  int* tile_ovf = nullptr;
+  int* tile_host = nullptr;  // pinned: [total staged, overflow]
+  int64_t tj_cap = 0;
+  int* tile_j = nullptr;
+  unsigned short* slot = nullptr;
 };
 
 constexpr int kScanBlock = 1024;
@@ -91,12 +105,7 @@ __global__ void k_ns_minmax_final(const float* part, int nb, float* mm) {
   mm[c] = v;
 }
 
-struct Grid {
-  double org[3];
-  double inv_cs;
-  int dim[3];
-  int mdim[3];
-};
+using Grid = GridDesc;
 
 __device__ __forceinline__ int morton3(int x, int y, int z) {
   int m = 0;
@@ -208,50 +217,6 @@ __global__ void k_ns_gather_f32(float* dst, const float* src, const int* idx, in
   if (i < n) dst[i] = src[idx[i]];
 }
 
-__global__ __launch_bounds__(256) void k_ns_lists(const float* __restrict__ p0,
-                                                  const float* __restrict__ p1,
-                                                  const float* __restrict__ p2, int n, Grid g,
-                                                  const int* __restrict__ cell_start, float r2,
-                                                  int64_t npad, int* __restrict__ count,
-                                                  int* __restrict__ list, Counters* ctr) {
-#pragma clang fp contract(off)
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int over = 0;
-  if (i < n) {
-    const float xi = p0[i], yi = p1[i], zi = p2[i];
-    int cx, cy, cz;
-    cell_of(g, xi, yi, zi, cx, cy, cz);
-    int c = 0;
-    for (int dz = -1; dz <= 1; dz++) {
-      int z = cz + dz;
-      if (z < 0 || z >= g.dim[2]) continue;
-      for (int dy = -1; dy <= 1; dy++) {
-        int y = cy + dy;
-        if (y < 0 || y >= g.dim[1]) continue;
-        for (int dx = -1; dx <= 1; dx++) {
-          int x = cx + dx;
-          if (x < 0 || x >= g.dim[0]) continue;
-          int k = cell_key(g, x, y, z);
-          int b = cell_start[k], e = cell_start[k + 1];
-          for (int j = b; j < e; j++) {
-            float ddx = p0[j] - xi, ddy = p1[j] - yi, ddz = p2[j] - zi;
-            float d2 = __fadd_rn(__fadd_rn(__fmul_rn(ddx, ddx), __fmul_rn(ddy, ddy)),
-                                 __fmul_rn(ddz, ddz));
-            if (d2 < r2 && d2 > 0.0f) {
-              if (c < kMaxNb) list[(int64_t)c * npad + i] = j;
-              else over = 1;
-              c++;
-            }
-          }
-        }
-      }
-    }
-    count[i] = c < kMaxNb ? c : kMaxNb;
-  }
-  unsigned long long o = wave_sum_u64((unsigned long long)over);
-  if (lane_id() == 0 && o) atomicAdd(&ctr->nb_over_cap, (int)o);
-}
-
 static int grid1(int64_t n, int b) { return (int)((n + b - 1) / b); }
 
 static int ensure(NsWork* w, int n, int64_t nkeys) {
@@ -286,6 +251,31 @@ static int ensure(NsWork* w, int n, int64_t nkeys) {
       return WF_E_NOMEM;
     w->key_cap = cap;
   }
+  int nt = (n + kTileP - 1) / kTileP;
+  if (nt > w->nt_cap) {
+    hipFree(w->tile_keys);
+    hipFree(w->tile_nc);
+    hipFree(w->tile_ucnt);
+    hipFree(w->tile_uoff);
+    hipFree(w->tile_scan);
+    hipFree(w->slot);
+    int cap = std::max(nt, 1);
+    int64_t npad = ((int64_t)cap * kTileP);
+    if (hipMalloc(&w->tile_keys, sizeof(int) * (int64_t)cap * kTileMaxCells) != hipSuccess)
+      return WF_E_NOMEM;
+    if (hipMalloc(&w->tile_nc, sizeof(int) * cap) != hipSuccess) return WF_E_NOMEM;
+    if (hipMalloc(&w->tile_ucnt, sizeof(int) * cap) != hipSuccess) return WF_E_NOMEM;
+    if (hipMalloc(&w->tile_uoff, sizeof(int) * (cap + 1)) != hipSuccess) return WF_E_NOMEM;
+    if (hipMalloc(&w->tile_scan, sizeof(int) * (grid1(cap, kScanBlock) + 1)) != hipSuccess)
+      return WF_E_NOMEM;
+    if (hipMalloc(&w->slot, sizeof(unsigned short) * 4 * kSlotQuads * npad) != hipSuccess)
+      return WF_E_NOMEM;
+    w->nt_cap = cap;
+  }
+  if (!w->tile_ovf) {
+    if (hipMalloc(&w->tile_ovf, sizeof(int)) != hipSuccess) return WF_E_NOMEM;
+    if (hipHostMalloc(&w->tile_host, sizeof(int) * 2) != hipSuccess) return WF_E_NOMEM;
+  }
   if (!w->mm_part) {
     if (hipMalloc(&w->mm_part, sizeof(float) * 6 * kMmBlocks) != hipSuccess) return WF_E_NOMEM;
     if (hipMalloc(&w->mm, sizeof(float) * 6) != hipSuccess) return WF_E_NOMEM;
@@ -310,13 +300,22 @@ void ns_free(NsWork* w) {
   hipFree(w->mm_part);
   hipFree(w->mm);
   if (w->mm_host) hipHostFree(w->mm_host);
+  hipFree(w->tile_keys);
+  hipFree(w->tile_nc);
+  hipFree(w->tile_ucnt);
+  hipFree(w->tile_uoff);
+  hipFree(w->tile_scan);
+  hipFree(w->tile_ovf);
+  if (w->tile_host) hipHostFree(w->tile_host);
+  hipFree(w->tile_j);
+  hipFree(w->slot);
   delete w;
 }
 
 // gather: out slot -> old sorted index; perm (in/out): sorted slot -> original
 int ns_build(hipStream_t s, NsWork** work, const double* const pos_src[3], int n, int64_t npad,
              double radius, int* gather, int* count, int* list, Counters* ctr, int* over_cap,
-             int* perm, int* perm_tmp) {
+             int* perm, int* perm_tmp, TileLists* tiles) {
   if (!*work) *work = new NsWork();
   NsWork* w = *work;
   int rc = ensure(w, n, 0);
@@ -375,9 +374,41 @@ int ns_build(hipStream_t s, NsWork** work, const double* const pos_src[3], int n
     hipLaunchKernelGGL(k_ns_gather_f32, dim3(grid1(n, 256)), dim3(256), 0, s, w->pfn[c], w->pf[c],
                        gather, n);
   hipLaunchKernelGGL(k_ns_fill_i32, dim3(1), dim3(64), 0, s, &ctr->nb_over_cap, 0, (int64_t)1);
-  hipLaunchKernelGGL(k_ns_lists, dim3(grid1(n, 256)), dim3(256), 0, s, w->pfn[0], w->pfn[1],
-                     w->pfn[2], n, g, w->cell_start, r2, npad, count, list, ctr);
+  hipLaunchKernelGGL(k_ns_fill_i32, dim3(1), dim3(64), 0, s, w->tile_ovf, 0, (int64_t)1);
+  // tiles of the new order: occupied neighbour cells and staged counts
+  const int nt = (n + kTileP - 1) / kTileP;
+  const float* pfn[3] = {w->pfn[0], w->pfn[1], w->pfn[2]};
+  launch_tile_cells(s, pfn, n, g, w->cell_start, w->tile_keys, w->tile_nc, w->tile_ucnt,
+                    w->tile_ovf);
+  int ntb = grid1(nt, kScanBlock);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(ntb), dim3(kScanBlock), 0, s, w->tile_ucnt,
+                     (int64_t)nt, w->tile_scan);
+  hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(64), 0, s, w->tile_scan, ntb);
+  hipLaunchKernelGGL(k_scan_apply, dim3(ntb), dim3(kScanBlock), 0, s, w->tile_ucnt, (int64_t)nt,
+                     w->tile_scan, w->tile_uoff);
+  if (hipMemcpyAsync(w->tile_host, w->tile_uoff + nt, sizeof(int), hipMemcpyDeviceToHost, s) !=
+      hipSuccess)
+    return WF_E_HIP;
+  if (hipMemcpyAsync(w->tile_host + 1, w->tile_ovf, sizeof(int), hipMemcpyDeviceToHost, s) !=
+      hipSuccess)
+    return WF_E_HIP;
+  if (hipStreamSynchronize(s) != hipSuccess) return WF_E_HIP;
+  if (w->tile_host[1] != 0) return WF_E_NBCAP;  // tile outgrew its cell or slot budget
+  const int64_t staged = w->tile_host[0];
+  if (staged > w->tj_cap) {
+    hipFree(w->tile_j);
+    int64_t cap = staged + staged / 4 + 1024;
+    if (hipMalloc(&w->tile_j, sizeof(int) * cap) != hipSuccess) return WF_E_NOMEM;
+    w->tj_cap = cap;
+  }
+  launch_tile_lists(s, pfn, n, g, w->cell_start, w->tile_keys, w->tile_nc, w->tile_uoff, r2,
+                    npad, count, list, w->slot, w->tile_j, &ctr->nb_over_cap);
   if (hipGetLastError() != hipSuccess) return WF_E_HIP;
+  tiles->count = count;
+  tiles->slotq = (const uint2*)w->slot;
+  tiles->uoff = w->tile_uoff;
+  tiles->tj = w->tile_j;
+  tiles->npad = npad;
   (void)over_cap;
   return WF_OK;
 }

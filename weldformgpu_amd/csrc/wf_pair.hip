@@ -1,0 +1,403 @@
+// wf_pair.hip — LDS-staged FP64 pair loops (gfx950).
+//
+// One workgroup per tile of kTileP Morton-consecutive particles, one thread
+// per particle.  The tile's staged set (the particles of every occupied cell
+// adjacent to the tile) streams through LDS in chunks of kTileChunk with
+// coalesced global loads; each thread walks its neighbour list, whose
+// entries are staged positions in ascending order, so each chunk consumes a
+// prefix of the remaining list.  The per-pair physics is the reference's
+// (InteractionAlt.cu, Thermal.cu), with per-particle factors (m/rho,
+// sigma/rho^2, 1/h) formed once instead of once per pair; the driver below
+// is shared, the policies (DensK, RatesK, AccelK, CondK) say what is staged
+// and what a pair adds.
+#include <climits>
+
+#include "wf_device.hpp"
+#include "wf_tiles.hpp"
+
+namespace wf {
+
+// Neighbour-slot reader over the quad layout: four 16-bit slots per 8 bytes,
+// the next quad prefetched one quad ahead.
+struct SlotReader {
+  const uint2* q;
+  int64_t npad;
+  int i, cnt, k;
+  uint2 cur, nxt;
+  __device__ __forceinline__ void init(const TileLists& tl, int i_, int cnt_) {
+    q = tl.slotq;
+    npad = tl.npad;
+    i = i_;
+    cnt = cnt_;
+    k = 0;
+    cur = cnt > 0 ? q[i] : make_uint2(0u, 0u);
+    nxt = cnt > 4 ? q[npad + i] : make_uint2(0u, 0u);
+  }
+  __device__ __forceinline__ int slot() const {
+    unsigned w = (k & 2) ? cur.y : cur.x;
+    return (int)((w >> ((k & 1) * 16)) & 0xFFFFu);
+  }
+  __device__ __forceinline__ void advance() {
+    k++;
+    if ((k & 3) == 0) {
+      cur = nxt;
+      if (k + 4 < cnt) nxt = q[(int64_t)((k >> 2) + 1) * npad + i];
+    }
+  }
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs (XCD = block % 8).
+// Give each XCD one contiguous run of tiles so that tiles sharing staged
+// cells share that XCD's L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile(int b, int nt) {
+  const int q = nt / kXcds, r = nt % kXcds;
+  const int x = b % kXcds, k = b / kXcds;
+  return x * q + min(x, r) + k;
+}
+
+template <class K>
+__global__ __launch_bounds__(kTileP) void k_pair_tiled(Fields f, TileLists tl,
+                                                       typename K::Prm prm) {
+  __shared__ double lds[K::NF][kTileChunk];
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int tid = threadIdx.x;
+  const int i = t * kTileP + tid;
+  const bool act = i < prm.limit;
+  typename K::Own own;
+  typename K::Acc acc;
+  K::init(f, i, act, own, acc);
+  const int ub = tl.uoff[t], ucnt = tl.uoff[t + 1] - ub;
+  SlotReader sr;
+  sr.init(tl, i, act ? tl.count[i] : 0);
+  for (int c0 = 0; c0 < ucnt; c0 += kTileChunk) {
+    const int m = min(kTileChunk, ucnt - c0);
+    __syncthreads();
+    for (int s = tid; s < m; s += kTileP) K::stage(f, tl.tj[ub + c0 + s], lds, s);
+    __syncthreads();
+    const int lim = c0 + m;
+    while (sr.k < sr.cnt) {
+      const int s = sr.slot();
+      if (s >= lim) break;
+      K::pair(lds, s - c0, own, acc, prm);
+      sr.advance();
+    }
+  }
+  K::finish(f, i, act, own, acc, prm);
+}
+
+// ---------------------------------------------------------------------------
+// Per-pair cubic-spline gradient GK = (1/r) dW/dr (Functions.cu:76-88) from
+// r^2: one refined reciprocal square root instead of sqrt + three IEEE
+// divisions.  With hj == hi (the solver sets one h for all particles,
+// Domain_d::Set_h) the thread's own 1/h and 1/(pi h^4) are reused, so the
+// common case needs no division at all.  Differences from the reference's
+// expression order are a few ulp.
+struct HConst {
+  double h, inv_h, C;
+};
+__device__ __forceinline__ HConst hconst(double h) {
+  return HConst{h, 1.0 / h, 1.0 / (h * h * h * h * kPi)};
+}
+__device__ __forceinline__ double rsqrt_f64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double hx = 0.5 * x;
+  y = y * (1.5 - hx * y * y);
+  y = y * (1.5 - hx * y * y);
+  return y;
+}
+__device__ __forceinline__ double gk_pair(double r2, double hj, const HConst& hc, double& rinv,
+                                          double& h_) {
+  // coincident particles (never in a list built with d2 > 0, but possible
+  // after motion) contribute nothing instead of 0 * inf
+  rinv = r2 > 0.0 ? rsqrt_f64(r2) : 0.0;
+  const double rij = r2 * rinv;
+  double inv_h = hc.inv_h, C = hc.C;
+  h_ = hc.h;
+  if (hj != hc.h) {
+    h_ = (hc.h + hj) / 2.0;
+    inv_h = 1.0 / h_;
+    C = 1.0 / (h_ * h_ * h_ * h_ * kPi);
+  }
+  const double q = rij * inv_h;
+  const double t = 2.0 - q;
+  const double poly = q < 1.0 ? (-3.0 * q + (9.0 / 4.0) * q * q)
+                              : (q < 2.0 ? (-3.0 / 4.0) * t * t : 0.0);
+  return C * rinv * poly;
+}
+
+// ---------------------------------------------------------------------------
+// A: density rate drho_i = rho_i sum_j (m_j/rho_j) v_ij . (GK x_ij)
+// (InteractionAlt.cu:166-246) [+ update rho_i + dt drho_i into rho_alt]
+struct DensK {
+  static constexpr int NF = 8;  // x y z vx vy vz h m/rho
+  struct Prm {
+    int limit, n;
+    double dt;
+    int fuse;
+  };
+  struct Own {
+    double x, y, z, vx, vy, vz, rho;
+    HConst hc;
+  };
+  struct Acc {
+    double s;
+  };
+  __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a) {
+    a.s = 0.0;
+    if (!act) return;
+    o.x = f.x[0][i]; o.y = f.x[1][i]; o.z = f.x[2][i];
+    o.vx = f.v[0][i]; o.vy = f.v[1][i]; o.vz = f.v[2][i];
+    o.rho = f.rho[i];
+    o.hc = hconst(f.h[i]);
+  }
+  __device__ static void stage(const Fields& f, int j, double (*L)[kTileChunk], int s) {
+    L[0][s] = f.x[0][j]; L[1][s] = f.x[1][j]; L[2][s] = f.x[2][j];
+    L[3][s] = f.v[0][j]; L[4][s] = f.v[1][j]; L[5][s] = f.v[2][j];
+    L[6][s] = f.h[j]; L[7][s] = f.m[j] / f.rho[j];
+  }
+  __device__ static void pair(double (*L)[kTileChunk], int s, const Own& o, Acc& a, const Prm&) {
+    const double xx = o.x - L[0][s], xy = o.y - L[1][s], xz = o.z - L[2][s];
+    const double r2 = xx * xx + xy * xy + xz * xz;
+    double rinv, h_;
+    const double GK = gk_pair(r2, L[6][s], o.hc, rinv, h_);
+    const double vx = o.vx - L[3][s], vy = o.vy - L[4][s], vz = o.vz - L[5][s];
+    a.s += L[7][s] * (GK * (vx * xx + vy * xy + vz * xz));
+  }
+  __device__ static void finish(const Fields& f, int i, bool act, const Own& o, const Acc& a,
+                                const Prm& p) {
+    if (act) {
+      const double drho = o.rho * a.s;
+      f.drho[i] = drho;
+      if (p.fuse) f.rho_alt[i] = o.rho + p.dt * drho;
+    } else if (p.fuse && i < p.n) {
+      f.rho_alt[i] = f.rho[i];
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// B: strain rate -1/2 sum (m_j/rho_j) GK (v_ij (x) x_ij + x_ij (x) v_ij) and
+// rotation rate -1/2 sum (m_j/rho_j) GK (v_ij (x) x_ij - x_ij (x) v_ij),
+// upper triangle, xz optionally negated (InteractionAlt.cu:262-383)
+struct RatesK {
+  static constexpr int NF = 8;
+  struct Prm {
+    int limit;
+    double xz_sign;
+  };
+  struct Own {
+    double x, y, z, vx, vy, vz;
+    HConst hc;
+  };
+  struct Acc {
+    double dxx, dyy, dzz, dxy, dyz, dxz, wxy, wyz, wxz;
+  };
+  __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a) {
+    a = Acc{0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (!act) return;
+    o.x = f.x[0][i]; o.y = f.x[1][i]; o.z = f.x[2][i];
+    o.vx = f.v[0][i]; o.vy = f.v[1][i]; o.vz = f.v[2][i];
+    o.hc = hconst(f.h[i]);
+  }
+  __device__ static void stage(const Fields& f, int j, double (*L)[kTileChunk], int s) {
+    DensK::stage(f, j, L, s);
+  }
+  __device__ static void pair(double (*L)[kTileChunk], int s, const Own& o, Acc& a, const Prm&) {
+    const double xx = o.x - L[0][s], xy = o.y - L[1][s], xz = o.z - L[2][s];
+    const double r2 = xx * xx + xy * xy + xz * xz;
+    double rinv, h_;
+    const double w = gk_pair(r2, L[6][s], o.hc, rinv, h_) * L[7][s];
+    const double vx = o.vx - L[3][s], vy = o.vy - L[4][s], vz = o.vz - L[5][s];
+    const double ax = w * vx, ay = w * vy, az = w * vz;
+    a.dxx += ax * xx;
+    a.dyy += ay * xy;
+    a.dzz += az * xz;
+    a.dxy += ax * xy + ay * xx;
+    a.dyz += ay * xz + az * xy;
+    a.dxz += ax * xz + az * xx;
+    a.wxy += ax * xy - ay * xx;
+    a.wyz += ay * xz - az * xy;
+    a.wxz += ax * xz - az * xx;
+  }
+  __device__ static void finish(const Fields& f, int i, bool act, const Own&, const Acc& a,
+                                const Prm& p) {
+    if (!act) return;
+    f.strrate[0][i] = -a.dxx; f.strrate[1][i] = -a.dyy; f.strrate[2][i] = -a.dzz;
+    f.strrate[3][i] = -0.5 * a.dxy; f.strrate[4][i] = -0.5 * a.dyz;
+    f.strrate[5][i] = -0.5 * a.dxz;
+    f.rotrate[0][i] = 0.0; f.rotrate[1][i] = 0.0; f.rotrate[2][i] = 0.0;
+    f.rotrate[3][i] = -0.5 * a.wxy; f.rotrate[4][i] = -0.5 * a.wyz;
+    f.rotrate[5][i] = -0.5 * p.xz_sign * a.wxz;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// D: a_i = sum_j m_j (sigma_i/rho_i^2 + sigma_j/rho_j^2 + Pi_ij I) . (GK x_ij),
+// Pi_ij = (alpha C mu + beta mu^2)/rho_bar for approaching pairs,
+// mu = h v.x/(r^2 + 0.01 h^2) (InteractionAlt.cu:18-150).  sigma/rho^2 is
+// formed once per staged particle.
+struct AccelK {
+  static constexpr int NF = 16;  // x y z vx vy vz h m rho Cs (sigma/rho^2)[6]
+  struct Prm {
+    int limit;
+    double alpha, beta;
+  };
+  struct Own {
+    double x, y, z, vx, vy, vz, di, Ci, s0, s1, s2, s3, s4, s5;
+    HConst hc;
+  };
+  struct Acc {
+    double ax, ay, az;
+  };
+  __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a) {
+    a = Acc{0, 0, 0};
+    if (!act) return;
+    o.x = f.x[0][i]; o.y = f.x[1][i]; o.z = f.x[2][i];
+    o.vx = f.v[0][i]; o.vy = f.v[1][i]; o.vz = f.v[2][i];
+    o.di = f.rho[i]; o.Ci = f.Cs[i];
+    o.hc = hconst(f.h[i]);
+    const double idi = 1.0 / (o.di * o.di);
+    o.s0 = f.sigma[0][i] * idi; o.s1 = f.sigma[1][i] * idi; o.s2 = f.sigma[2][i] * idi;
+    o.s3 = f.sigma[3][i] * idi; o.s4 = f.sigma[4][i] * idi; o.s5 = f.sigma[5][i] * idi;
+  }
+  __device__ static void stage(const Fields& f, int j, double (*L)[kTileChunk], int s) {
+    L[0][s] = f.x[0][j]; L[1][s] = f.x[1][j]; L[2][s] = f.x[2][j];
+    L[3][s] = f.v[0][j]; L[4][s] = f.v[1][j]; L[5][s] = f.v[2][j];
+    const double rj = f.rho[j];
+    L[6][s] = f.h[j]; L[7][s] = f.m[j]; L[8][s] = rj; L[9][s] = f.Cs[j];
+    const double idj = 1.0 / (rj * rj);
+    L[10][s] = f.sigma[0][j] * idj; L[11][s] = f.sigma[1][j] * idj;
+    L[12][s] = f.sigma[2][j] * idj; L[13][s] = f.sigma[3][j] * idj;
+    L[14][s] = f.sigma[4][j] * idj; L[15][s] = f.sigma[5][j] * idj;
+  }
+  __device__ static void pair(double (*L)[kTileChunk], int s, const Own& o, Acc& a,
+                              const Prm& p) {
+    const double xx = o.x - L[0][s], xy = o.y - L[1][s], xz = o.z - L[2][s];
+    const double r2 = xx * xx + xy * xy + xz * xz;
+    double rinv, h_;
+    const double GK = gk_pair(r2, L[6][s], o.hc, rinv, h_);
+    const double vx = o.vx - L[3][s], vy = o.vy - L[4][s], vz = o.vz - L[5][s];
+    const double vdx = vx * xx + vy * xy + vz * xz;
+    double PI = 0.0;
+    if (vdx < 0.0 && (p.alpha != 0.0 || p.beta != 0.0)) {
+      const double MU = h_ * vdx / (r2 + 0.01 * h_ * h_);
+      const double Cij = 0.5 * (o.Ci + L[9][s]);
+      PI = (p.alpha * Cij * MU + p.beta * MU * MU) / (0.5 * (o.di + L[8][s]));
+    }
+    const double g = GK * L[7][s];
+    const double gx = g * xx, gy = g * xy, gz = g * xz;
+    const double mxx = o.s0 + L[10][s] + PI, myy = o.s1 + L[11][s] + PI;
+    const double mzz = o.s2 + L[12][s] + PI;
+    const double mxy = o.s3 + L[13][s], myz = o.s4 + L[14][s], mxz = o.s5 + L[15][s];
+    a.ax += mxx * gx + mxy * gy + mxz * gz;
+    a.ay += mxy * gx + myy * gy + myz * gz;
+    a.az += mxz * gx + myz * gy + mzz * gz;
+  }
+  __device__ static void finish(const Fields& f, int i, bool act, const Own&, const Acc& a,
+                                const Prm&) {
+    if (!act) return;
+    f.a[0][i] = a.ax;
+    f.a[1][i] = a.ay;
+    f.a[2][i] = a.az;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// G: conduction dT_i/dt = 1/(rho_i cp_i) sum_j (m_j/rho_j) 4 k_i k_j/(k_i+k_j)
+// (T_i - T_j) GK  (Thermal.cu:101-142, where x_ij.(GK x_ij)/r^2 = GK), then
+// convection (Thermal.cu:36-64), plastic-work source and FIXED BC; optional
+// fused leapfrog (Thermal.cu:16-31) writing T into T_alt.
+struct CondK {
+  static constexpr int NF = 7;  // x y z h m/rho T k
+  struct Prm {
+    int limit;
+    double T_inf, h_conv, dt;
+    int leap, first;
+  };
+  struct Own {
+    double x, y, z, k, T;
+    HConst hc;
+  };
+  struct Acc {
+    double q;
+  };
+  __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a) {
+    a.q = 0.0;
+    if (!act) return;
+    o.x = f.x[0][i]; o.y = f.x[1][i]; o.z = f.x[2][i];
+    o.k = f.k[i]; o.T = f.T[i];
+    o.hc = hconst(f.h[i]);
+  }
+  __device__ static void stage(const Fields& f, int j, double (*L)[kTileChunk], int s) {
+    L[0][s] = f.x[0][j]; L[1][s] = f.x[1][j]; L[2][s] = f.x[2][j];
+    L[3][s] = f.h[j]; L[4][s] = f.m[j] / f.rho[j]; L[5][s] = f.T[j]; L[6][s] = f.k[j];
+  }
+  __device__ static void pair(double (*L)[kTileChunk], int s, const Own& o, Acc& a, const Prm&) {
+    const double xx = o.x - L[0][s], xy = o.y - L[1][s], xz = o.z - L[2][s];
+    const double r2 = xx * xx + xy * xy + xz * xz;
+    double rinv, h_;
+    const double GK = gk_pair(r2, L[3][s], o.hc, rinv, h_);
+    const double kj = L[6][s];
+    const double kh = (kj == o.k) ? 2.0 * o.k : 4.0 * o.k * kj / (o.k + kj);
+    a.q += L[4][s] * kh * (o.T - L[5][s]) * GK;
+  }
+  __device__ static void finish(const Fields& f, int i, bool act, const Own& o, const Acc& a,
+                                const Prm& p) {
+    if (!act) return;
+    const double rhoi = f.rho[i], cpi = f.cp[i];
+    double acc = a.q * (1. / (rhoi * cpi));
+    const int bct = f.BC_T[i];
+    if (bct == WF_BCT_CONVECTION) {
+      double mi = f.m[i];
+      double dS2 = pow(mi / rhoi, 0.666666666);
+      double q_conv = p.h_conv * dS2 * (p.T_inf - o.T) / mi;
+      acc += 1. / cpi * q_conv;
+    }
+    double qp = f.qpl[i];
+    if (qp != 0.0) acc += qp * (1. / (rhoi * cpi));
+    if (bct == WF_BCT_FIXED) acc = 0.0;
+    f.dTdt[i] = acc;
+    if (p.leap) {
+      if (p.first) {
+        f.Ta[i] = o.T - p.dt / 2.0 * acc;
+        f.T_alt[i] = o.T;
+      } else {
+        double ta = f.Ta[i];
+        f.Tb[i] = ta;
+        double tan = ta + acc * p.dt;
+        f.Ta[i] = tan;
+        f.T_alt[i] = (tan + ta) / 2.;
+      }
+    }
+  }
+};
+
+static inline int ntiles(int n) { return (n + kTileP - 1) / kTileP; }
+
+void launch_dens_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int n, int solid,
+                       double dt, bool fuse) {
+  if (n <= 0) return;
+  DensK::Prm p{solid, n, dt, fuse ? 1 : 0};
+  hipLaunchKernelGGL(k_pair_tiled<DensK>, dim3(ntiles(n)), dim3(kTileP), 0, s, f, tl, p);
+}
+void launch_rates_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
+                        int quirks) {
+  if (solid <= 0) return;
+  RatesK::Prm p{solid, (quirks & WF_Q_ROT_XZ_SIGN) ? -1.0 : 1.0};
+  hipLaunchKernelGGL(k_pair_tiled<RatesK>, dim3(ntiles(solid)), dim3(kTileP), 0, s, f, tl, p);
+}
+void launch_accel_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
+                        double alpha, double beta) {
+  if (solid <= 0) return;
+  AccelK::Prm p{solid, alpha, beta};
+  hipLaunchKernelGGL(k_pair_tiled<AccelK>, dim3(ntiles(solid)), dim3(kTileP), 0, s, f, tl, p);
+}
+void launch_cond_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int n, double T_inf,
+                       double h_conv, double dt, bool leap, bool first) {
+  if (n <= 0) return;
+  CondK::Prm p{n, T_inf, h_conv, dt, leap ? 1 : 0, first ? 1 : 0};
+  hipLaunchKernelGGL(k_pair_tiled<CondK>, dim3(ntiles(n)), dim3(kTileP), 0, s, f, tl, p);
+}
+
+}  // namespace wf

@@ -1,0 +1,243 @@
+// wf_tiles.hip — LDS-staged neighbour tiles.
+//
+// A tile is kTileP consecutive particles of the Morton order (one workgroup
+// of the pair kernels).  At each rebuild the tile gets:
+//   - the sorted set of occupied grid cells adjacent to any of its particles
+//     (tile_keys), and the concatenation of those cells' particle ranges
+//     (tile_j, ascending particle index) — the tile's staged set;
+//   - for each of its particles, the neighbour list in ascending index order,
+//     both as global indices (int32) and as 16-bit positions in the staged
+//     set, interleaved [k*npad + i] so a wavefront reads 128 contiguous bytes
+//     per list slot.
+// The pair kernels stream the staged set through LDS in chunks of
+// kTileChunk particles with coalesced loads; every neighbour read inside the
+// pair loop is an LDS read.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+
+#include "wf_device.hpp"
+#include "wf_tiles.hpp"
+
+namespace wf {
+
+__device__ __forceinline__ int morton3_t(int x, int y, int z) {
+  int m = 0;
+#pragma unroll
+  for (int b = 0; b < 3; b++)
+    m |= (((x >> b) & 1) << (3 * b)) | (((y >> b) & 1) << (3 * b + 1)) | (((z >> b) & 1) << (3 * b + 2));
+  return m;
+}
+__device__ __forceinline__ int key_of(const GridDesc& g, int cx, int cy, int cz) {
+  int meta = ((cz >> 3) * g.mdim[1] + (cy >> 3)) * g.mdim[0] + (cx >> 3);
+  return meta * 512 + morton3_t(cx & 7, cy & 7, cz & 7);
+}
+__device__ __forceinline__ void coords_of(const GridDesc& g, int key, int& cx, int& cy, int& cz) {
+  int meta = key >> 9, m = key & 511;
+  int x = 0, y = 0, z = 0;
+#pragma unroll
+  for (int b = 0; b < 3; b++) {
+    x |= ((m >> (3 * b)) & 1) << b;
+    y |= ((m >> (3 * b + 1)) & 1) << b;
+    z |= ((m >> (3 * b + 2)) & 1) << b;
+  }
+  int mx = meta % g.mdim[0];
+  int my = (meta / g.mdim[0]) % g.mdim[1];
+  int mz = meta / (g.mdim[0] * g.mdim[1]);
+  cx = mx * 8 + x;
+  cy = my * 8 + y;
+  cz = mz * 8 + z;
+}
+__device__ __forceinline__ void cell_of_t(const GridDesc& g, float px, float py, float pz, int& cx,
+                                          int& cy, int& cz) {
+  cx = (int)floor(((double)px - g.org[0]) * g.inv_cs);
+  cy = (int)floor(((double)py - g.org[1]) * g.inv_cs);
+  cz = (int)floor(((double)pz - g.org[2]) * g.inv_cs);
+  cx = min(max(cx, 0), g.dim[0] - 1);
+  cy = min(max(cy, 0), g.dim[1] - 1);
+  cz = min(max(cz, 0), g.dim[2] - 1);
+}
+
+constexpr int kSortN = 4096;  // >= kTileP * 27, power of two
+static_assert(kTileP * 27 <= kSortN, "sort buffer too small");
+
+// Per tile: sorted unique occupied neighbour cells and the staged count.
+__global__ __launch_bounds__(kTileP) void k_tile_cells(const float* __restrict__ p0,
+                                                       const float* __restrict__ p1,
+                                                       const float* __restrict__ p2, int n,
+                                                       GridDesc g,
+                                                       const int* __restrict__ cell_start,
+                                                       int* tile_keys, int* tile_nc,
+                                                       int* tile_ucnt, int* overflow) {
+  __shared__ int keys[kSortN];
+  __shared__ int segcnt[kTileP];
+  const int t = blockIdx.x;
+  const int tid = threadIdx.x;
+  for (int s = tid; s < kSortN; s += kTileP) keys[s] = INT_MAX;
+  __syncthreads();
+  const int i = t * kTileP + tid;
+  if (i < n) {
+    int cx, cy, cz;
+    cell_of_t(g, p0[i], p1[i], p2[i], cx, cy, cz);
+    int q = 0;
+    for (int dz = -1; dz <= 1; dz++)
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++, q++) {
+          int x = cx + dx, y = cy + dy, z = cz + dz;
+          if (x < 0 || y < 0 || z < 0 || x >= g.dim[0] || y >= g.dim[1] || z >= g.dim[2]) continue;
+          int k = key_of(g, x, y, z);
+          if (cell_start[k + 1] > cell_start[k]) keys[tid * 27 + q] = k;
+        }
+  }
+  __syncthreads();
+  // bitonic sort of kSortN keys
+  for (int size = 2; size <= kSortN; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int s = tid; s < kSortN / 2; s += kTileP) {
+        int lo = 2 * s - (s & (stride - 1));
+        int hi = lo + stride;
+        bool up = ((lo & size) == 0);
+        int a = keys[lo], b = keys[hi];
+        if ((a > b) == up) {
+          keys[lo] = b;
+          keys[hi] = a;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // unique: each thread owns a segment of kSortN/kTileP entries
+  constexpr int seg = kSortN / kTileP;
+  int c = 0;
+  for (int s = tid * seg; s < tid * seg + seg; s++) {
+    int k = keys[s];
+    if (k != INT_MAX && (s == 0 || keys[s - 1] != k)) c++;
+  }
+  segcnt[tid] = c;
+  __syncthreads();
+  if (tid == 0) {
+    int run = 0;
+    for (int u = 0; u < kTileP; u++) {
+      int v = segcnt[u];
+      segcnt[u] = run;
+      run += v;
+    }
+    tile_nc[t] = run;
+    if (run > kTileMaxCells) atomicAdd(overflow, 1);
+  }
+  __syncthreads();
+  int pos = segcnt[tid];
+  int* out = tile_keys + (int64_t)t * kTileMaxCells;
+  for (int s = tid * seg; s < tid * seg + seg; s++) {
+    int k = keys[s];
+    if (k != INT_MAX && (s == 0 || keys[s - 1] != k)) {
+      if (pos < kTileMaxCells) out[pos] = k;
+      pos++;
+    }
+  }
+  __syncthreads();
+  // staged particle count
+  int nc = min(tile_nc[t], kTileMaxCells);
+  int cnt = 0;
+  for (int p = tid; p < nc; p += kTileP) {
+    int k = out[p];
+    cnt += cell_start[k + 1] - cell_start[k];
+  }
+  segcnt[tid] = cnt;
+  __syncthreads();
+  if (tid == 0) {
+    int tot = 0;
+    for (int u = 0; u < kTileP; u++) tot += segcnt[u];
+    tile_ucnt[t] = tot;
+    if (tot > 65535) atomicAdd(overflow, 1);
+  }
+}
+
+// Neighbour lists in staged order (global index + 16-bit slot), and the
+// staged set tile_j.  Same float criterion as the reference search:
+// 0 < d2 < r2, d2 = dx*dx + dy*dy + dz*dz without contraction, cap 70.
+__global__ __launch_bounds__(kTileP) void k_tile_lists(
+    const float* __restrict__ p0, const float* __restrict__ p1, const float* __restrict__ p2,
+    int n, GridDesc g, const int* __restrict__ cell_start, const int* __restrict__ tile_keys,
+    const int* __restrict__ tile_nc, const int* __restrict__ tile_uoff, float r2, int64_t npad,
+    int* __restrict__ count, int* __restrict__ list, unsigned short* __restrict__ slot,
+    int* __restrict__ tile_j, int* over_cap) {
+#pragma clang fp contract(off)
+  __shared__ int sb[kTileMaxCells], sn[kTileMaxCells], spre[kTileMaxCells];
+  __shared__ int scx[kTileMaxCells], scy[kTileMaxCells], scz[kTileMaxCells];
+  const int t = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int nc = min(tile_nc[t], kTileMaxCells);
+  const int* keys = tile_keys + (int64_t)t * kTileMaxCells;
+  for (int p = tid; p < nc; p += kTileP) {
+    int k = keys[p];
+    sb[p] = cell_start[k];
+    sn[p] = cell_start[k + 1] - cell_start[k];
+    int cx, cy, cz;
+    coords_of(g, k, cx, cy, cz);
+    scx[p] = cx;
+    scy[p] = cy;
+    scz[p] = cz;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int run = 0;
+    for (int p = 0; p < nc; p++) {
+      spre[p] = run;
+      run += sn[p];
+    }
+  }
+  __syncthreads();
+  const int uoff = tile_uoff[t];
+  for (int p = 0; p < nc; p++)
+    for (int r = tid; r < sn[p]; r += kTileP) tile_j[uoff + spre[p] + r] = sb[p] + r;
+  const int i = t * kTileP + tid;
+  int over = 0;
+  if (i < n) {
+    const float xi = p0[i], yi = p1[i], zi = p2[i];
+    int cx, cy, cz;
+    cell_of_t(g, xi, yi, zi, cx, cy, cz);
+    int c = 0;
+    for (int p = 0; p < nc; p++) {
+      if (abs(scx[p] - cx) > 1 || abs(scy[p] - cy) > 1 || abs(scz[p] - cz) > 1) continue;
+      const int b = sb[p], e = b + sn[p], base = spre[p];
+      for (int j = b; j < e; j++) {
+        float ddx = p0[j] - xi, ddy = p1[j] - yi, ddz = p2[j] - zi;
+        float d2 = __fadd_rn(__fadd_rn(__fmul_rn(ddx, ddx), __fmul_rn(ddy, ddy)),
+                             __fmul_rn(ddz, ddz));
+        if (d2 < r2 && d2 > 0.0f) {
+          if (c < kMaxNb) {
+            list[(int64_t)c * npad + i] = j;
+            slot[(((int64_t)(c >> 2)) * npad + i) * 4 + (c & 3)] = (unsigned short)(base + (j - b));
+          } else {
+            over = 1;
+          }
+          c++;
+        }
+      }
+    }
+    count[i] = c < kMaxNb ? c : kMaxNb;
+  }
+  unsigned long long o = wave_sum_u64((unsigned long long)over);
+  if (lane_id() == 0 && o) atomicAdd(over_cap, (int)o);
+}
+
+void launch_tile_cells(hipStream_t s, const float* const pf[3], int n, const GridDesc& g,
+                       const int* cell_start, int* tile_keys, int* tile_nc, int* tile_ucnt,
+                       int* overflow) {
+  int nt = (n + kTileP - 1) / kTileP;
+  hipLaunchKernelGGL(k_tile_cells, dim3(nt), dim3(kTileP), 0, s, pf[0], pf[1], pf[2], n, g,
+                     cell_start, tile_keys, tile_nc, tile_ucnt, overflow);
+}
+
+void launch_tile_lists(hipStream_t s, const float* const pf[3], int n, const GridDesc& g,
+                       const int* cell_start, const int* tile_keys, const int* tile_nc,
+                       const int* tile_uoff, float r2, int64_t npad, int* count, int* list,
+                       unsigned short* slot, int* tile_j, int* over_cap) {
+  int nt = (n + kTileP - 1) / kTileP;
+  hipLaunchKernelGGL(k_tile_lists, dim3(nt), dim3(kTileP), 0, s, pf[0], pf[1], pf[2], n, g,
+                     cell_start, tile_keys, tile_nc, tile_uoff, r2, npad, count, list, slot,
+                     tile_j, over_cap);
+}
+
+}  // namespace wf
