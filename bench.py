@@ -33,6 +33,9 @@ HBM_PEAK_GBS = 8000.0
 # compulsory SoA reads + writes, neighbour data counted once, list excluded
 ALG_BYTES_FP64 = {"dens": 80, "rates": 144, "stress": 384, "accel": 144, "integrate": 172,
                   "thermal": 120}
+# mixed (FP32 pair-loop inputs and outputs), same accounting
+ALG_BYTES_MIXED = {"dens": 40, "rates": 72, "stress": 204, "accel": 72, "integrate": 184,
+                   "thermal": 60}
 
 
 def cpu_baseline(sc, warm=1, steps=2):
@@ -70,51 +73,69 @@ def main():
               "single-GPU in this build)", file=sys.stderr)
         sys.exit(2)
 
-    from weldformgpu_amd import scenarios
+    from weldformgpu_amd import _abi, scenarios
     from weldformgpu_amd.domain import Domain
 
-    sc = scenarios.c3_johnson_cook(scale=args.scale)
-    n = sc.n
-    print(f"[bench] workload {sc.name}: {n} particles, dt={sc.cfg.dt:.4e}", file=sys.stderr,
-          flush=True)
-    dom = Domain(sc.cfg)
-    scenarios.load_domain(dom, sc)
-    dom.step(args.warmup)
-    dom.sync()
-    t0 = time.perf_counter()
-    dom.step(args.steps)
-    dom.sync()
-    t1 = time.perf_counter()
-    ms_step = (t1 - t0) * 1e3 / args.steps
-    rate = n * args.steps / (t1 - t0)
-    print(f"[bench] timed {args.steps} steps: {ms_step:.3f} ms/step", file=sys.stderr, flush=True)
+    def run(precision, timing_steps):
+        sc = scenarios.c3_johnson_cook(scale=args.scale)
+        sc.cfg.precision = precision
+        dom = Domain(sc.cfg)
+        scenarios.load_domain(dom, sc)
+        dom.step(args.warmup)
+        dom.sync()
+        t0 = time.perf_counter()
+        dom.step(args.steps)
+        dom.sync()
+        t1 = time.perf_counter()
+        res = {"sc": sc, "secs": t1 - t0,
+               "fields": {k: dom.get(k) for k in ["u", "sigma", "sigma_eq", "pl_strain", "T"]}}
+        # per-kernel HIP-event durations on the library stream
+        dom.set_timing(True)
+        dom.step(timing_steps)
+        kern = {}
+        alg = ALG_BYTES_FP64 if precision == _abi.WF_FP64 else ALG_BYTES_MIXED
+        for k, b in alg.items():
+            ms, cnt = dom.kernel_time(k)
+            if cnt:
+                avg = ms / cnt
+                kern[k] = {"avg_ms": avg, "launches": cnt, "alg_bytes": b * sc.n,
+                           "GBs": b * sc.n / (avg * 1e-3) / 1e9}
+        dom.set_timing(False)
+        # known-size copy kernel for HBM-counter calibration: the x -> x_h
+        # snapshot copies 3 planes of n doubles (24 n bytes read, 24 n written)
+        for _ in range(3):
+            dom.snapshot_positions()
+        dom.sync()
+        res["kern"] = kern
+        res["stats"] = dom.stats()
+        dom.close()
+        return res
 
-    # per-kernel HIP-event durations on the library stream
-    dom.set_timing(True)
-    dom.step(args.timing_steps)
-    kern = {}
-    for k, b in ALG_BYTES_FP64.items():
-        ms, cnt = dom.kernel_time(k)
-        if cnt:
-            avg = ms / cnt
-            kern[k] = {"avg_ms": avg, "launches": cnt, "alg_bytes": b * n,
-                       "GBs": b * n / (avg * 1e-3) / 1e9}
-    ms_ns, cnt_ns = dom.kernel_time("nsearch")
-    if cnt_ns:
-        kern["nsearch"] = {"avg_ms": ms_ns / cnt_ns, "launches": cnt_ns}
-    dom.set_timing(False)
-    # known-size copy kernel for HBM-counter calibration: the x -> x_h
-    # snapshot copies 3 planes of n doubles (24 n bytes read, 24 n written)
-    for _ in range(3):
-        dom.snapshot_positions()
-    dom.sync()
-    stats = dom.stats()
-    dom.close()
-    dom_k = max((k for k in kern if k in ALG_BYTES_FP64), key=lambda k: kern[k]["avg_ms"])
+    r64 = run(_abi.WF_FP64, args.timing_steps)
+    sc = r64["sc"]
+    n = sc.n
+    print(f"[bench] workload {sc.name}: {n} particles, dt={sc.cfg.dt:.4e}; fp64 "
+          f"{r64['secs'] * 1e3 / args.steps:.3f} ms/step", file=sys.stderr, flush=True)
+    rmx = run(_abi.WF_MIXED, args.timing_steps)
+    ms_step = rmx["secs"] * 1e3 / args.steps
+    rate = n * args.steps / rmx["secs"]
+    rate64 = n * args.steps / r64["secs"]
+    print(f"[bench] mixed {ms_step:.3f} ms/step", file=sys.stderr, flush=True)
+
+    def rl2(a, b):
+        import numpy as np
+        d = np.linalg.norm(b)
+        return float(np.linalg.norm(a - b) / d) if d > 0 else float(np.linalg.norm(a - b))
+    err = {k: rl2(rmx["fields"][k], r64["fields"][k]) for k in ["u", "sigma", "sigma_eq",
+                                                                  "pl_strain"]}
+    err["T_rise"] = rl2(rmx["fields"]["T"] - 300.0, r64["fields"]["T"] - 300.0)
+    kern = rmx["kern"]
+    stats = rmx["stats"]
+    dom_k = max((k for k in kern), key=lambda k: kern[k]["avg_ms"])
     dk = kern[dom_k]
     roof = {"kernel": dom_k, "bound": "hbm", "achieved": dk["GBs"], "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": dk["GBs"] / HBM_PEAK_GBS, "traffic": None,
-            "alg_bytes_per_particle": ALG_BYTES_FP64[dom_k], "avg_ms": dk["avg_ms"]}
+            "alg_bytes_per_particle": ALG_BYTES_MIXED[dom_k], "avg_ms": dk["avg_ms"]}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -124,13 +145,18 @@ def main():
     out = {
         "metric": METRIC, "value": rate, "unit": "particle-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": f"{sc.name}: JC thermo-mechanical cylinder R15mm H30mm "
-                               f"dx0.3mm, {n} particles, FP64 pair loops, FP64 Newton return "
-                               "map, conduction + Taylor-Quinney 0.9, top at -2 m/s",
-                   "particles": n, "precision": "fp64", "surrogate": False,
+                               f"dx0.3mm, {n} particles, mixed precision (FP32 compensated "
+                               "pair loops, FP64 stress/Newton return map/integration), "
+                               "conduction + Taylor-Quinney 0.9, top at -2 m/s",
+                   "particles": n, "precision": "mixed", "surrogate": False,
                    "parallelism": "single-gpu"},
+        "fp64": {"value": rate64, "ms_per_step": r64["secs"] * 1e3 / args.steps,
+                 "kernels": r64["kern"]},
+        "speedup_vs_fp64": rate / rate64,
+        "error_vs_fp64": err,
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": kern,

@@ -1,0 +1,95 @@
+"""Mixed precision (FP32 pair loops with compensated sums on cell-relative
+FP32 coordinates; FP64 stress, return map, integration) against the FP64
+oracle.
+
+Stated tolerances:
+  - one pair-kernel call: relative L2 <= 2e-5 per output field;
+  - coupled runs: relative L2 <= 1e-4 in displacement u, stress sigma,
+    von Mises sigma_eq, plastic strain and temperature rise (T - T0).
+"""
+import numpy as np
+import pytest
+
+from weldformgpu_amd import _abi, scenarios
+
+from harness import randomize, rel_l2, small_c1, to_gpu, to_oracle
+
+pytestmark = pytest.mark.gpu
+
+KERNEL_TOL = 2e-5
+RUN_TOL = 1e-4
+
+
+def _mixed(sc):
+    sc.cfg.precision = _abi.WF_MIXED
+    return sc
+
+
+def test_mixed_single_kernels(oracle, wflib):
+    sc = _mixed(randomize(small_c1(0.4), seed=21))
+    print(f"[case] mixed kernels n={sc.n}", flush=True)
+    st = to_oracle(oracle, sc)
+    cfg = sc.cfg
+    st.nsearch(cfg)
+    with to_gpu(sc, wflib) as d:
+        d.nsearch()
+        st.calc_dens_inc(cfg)
+        d.CalcDensInc()
+        assert rel_l2(d.get("drho"), st["drho"]) < KERNEL_TOL
+        st.calc_rate_tensors(cfg)
+        d.CalcRateTensors()
+        assert rel_l2(d.get("strrate"), st["strrate"]) < KERNEL_TOL
+        assert rel_l2(d.get("rotrate"), st["rotrate"]) < KERNEL_TOL
+        st.calc_accel(cfg)
+        d.CalcAccel()
+        assert rel_l2(d.get("a"), st["a"]) < KERNEL_TOL
+
+
+def test_mixed_steps_c3_vs_fp64_oracle(oracle, wflib):
+    sc = _mixed(randomize(scenarios.c3_johnson_cook(scale=0.2), seed=9, v_scale=0.5,
+                          s_scale=200e6))
+    print(f"[case] mixed c3 n={sc.n}", flush=True)
+    st = to_oracle(oracle, sc)
+    cfg64 = scenarios.c3_johnson_cook(scale=0.2).cfg
+    st.nsearch(cfg64)
+    nsteps = 40
+    st.step(cfg64, sc.bcs, nsteps)
+    with to_gpu(sc, wflib) as d:
+        d.step(nsteps)
+        assert d.stats()["newton_calls"] > 0
+        for k in ["u", "sigma", "sigma_eq", "pl_strain"]:
+            e = rel_l2(d.get(k), st[k])
+            print(f"  {k}: rel L2 {e:.2e}")
+            assert e < RUN_TOL, k
+        e = rel_l2(d.get("T") - 300.0, st["T"] - 300.0)
+        print(f"  T-T0: rel L2 {e:.2e}")
+        assert e < RUN_TOL
+
+
+def test_mixed_steps_c2_thermal(oracle, wflib):
+    sc = _mixed(scenarios.c2_thermal_cube(L=14e-3))
+    print(f"[case] mixed c2 n={sc.n}", flush=True)
+    st = to_oracle(oracle, sc)
+    st.nsearch(sc.cfg)
+    nsteps = 40
+    st.step(sc.cfg, sc.bcs, nsteps)
+    with to_gpu(sc, wflib) as d:
+        d.step(nsteps)
+        e = rel_l2(d.get("T") - 300.0, st["T"] - 300.0)
+        print(f"  T-300: rel L2 {e:.2e}")
+        assert e < RUN_TOL
+
+
+def test_mixed_matches_fp64_gpu_path(wflib):
+    """The two GPU paths on the same bilinear compression, 60 steps."""
+    base = randomize(small_c1(0.5), seed=3, v_scale=0.3, s_scale=250e6)
+    out = {}
+    for prec in (_abi.WF_FP64, _abi.WF_MIXED):
+        sc = randomize(small_c1(0.5), seed=3, v_scale=0.3, s_scale=250e6)
+        sc.cfg.precision = prec
+        with to_gpu(sc, wflib) as d:
+            d.step(60)
+            out[prec] = {k: d.get(k) for k in ["u", "sigma_eq", "pl_strain", "rho"]}
+    assert base.n == len(out[_abi.WF_FP64]["rho"])
+    for k in ["u", "sigma_eq", "pl_strain", "rho"]:
+        assert rel_l2(out[_abi.WF_MIXED][k], out[_abi.WF_FP64][k]) < RUN_TOL, k

@@ -46,6 +46,16 @@ __device__ __forceinline__ double hollomon_yield(const MatParams& mp, double ep)
   return mp.sy0;
 }
 
+// packed integer cell of the FP32 frame: cx | cy << 11 | cz << 22
+__device__ __forceinline__ int pack_cell(int cx, int cy, int cz) {
+  return cx | (cy << 11) | (cz << 22);
+}
+__device__ __forceinline__ void unpack_cell(int c, int& cx, int& cy, int& cz) {
+  cx = c & 2047;
+  cy = (c >> 11) & 2047;
+  cz = (c >> 22) & 1023;
+}
+
 // ---- wavefront (64-lane) reductions ---------------------------------------
 __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
 #pragma unroll

@@ -63,6 +63,10 @@ struct wf_domain {
   double* sum_out = nullptr;
   NsWork* ns = nullptr;
   TileLists tl{};
+  GridDesc grid{};
+  Frame32 frame{};
+  bool mixed = false;
+  bool mirror_dirty = true;  // FP32 mirrors out of date with the FP64 state
   bool nb_built = false;
   bool snapshot_set = false;
   // the search reads the snapshot x_h under WF_Q_STALE_NS_POS: a rebuild
@@ -90,6 +94,8 @@ struct wf_domain {
   std::vector<PlaneRef> all_planes();
   int alloc_plane_d(double** p);
   int alloc_plane_i(int** p);
+  int alloc_plane_f(float** p);
+  void refresh_mirrors();
 };
 
 namespace {
@@ -257,6 +263,22 @@ int wf_domain::alloc_plane_d(double** p) {
                      (unsigned char*)q, (int64_t)(sizeof(double) * npad));
   return WF_OK;
 }
+int wf_domain::alloc_plane_f(float** p) {
+  void* q = nullptr;
+  if (hipMalloc(&q, sizeof(float) * npad) != hipSuccess) return WF_E_NOMEM;
+  owned.push_back(q);
+  *p = (float*)q;
+  hipLaunchKernelGGL(k_zero_bytes, dim3(gridn(sizeof(float) * npad)), dim3(256), 0, stream,
+                     (unsigned char*)q, (int64_t)(sizeof(float) * npad));
+  return WF_OK;
+}
+// mixed precision: rebuild every FP32 mirror when the FP64 state changed
+// outside the step's own producers (uploads, rebuilds, single-kernel calls)
+void wf_domain::refresh_mirrors() {
+  if (!mixed || !mirror_dirty || !nb_built) return;
+  launch_mirror32(stream, f, n, frame);
+  mirror_dirty = false;
+}
 int wf_domain::alloc_plane_i(int** p) {
   void* q = nullptr;
   if (hipMalloc(&q, sizeof(int) * npad) != hipSuccess) return WF_E_NOMEM;
@@ -306,8 +328,8 @@ int wf_create(const wf_config* cfg, wf_domain** out) {
   if (cfg->n_particles <= 0) return fail(WF_E_ARG, "n_particles must be positive");
   if (cfg->solid_count < 0 || cfg->solid_count > cfg->n_particles)
     return fail(WF_E_ARG, "solid_count out of range");
-  if (cfg->precision != WF_FP64)
-    return fail(WF_E_UNSUPPORTED, "precision mode not available in this build");
+  if (cfg->precision != WF_FP64 && cfg->precision != WF_MIXED)
+    return fail(WF_E_ARG, "unknown precision mode");
   if (cfg->surrogate) return fail(WF_E_UNSUPPORTED, "surrogate not available in this build");
   if (cfg->ns_interval <= 0) return fail(WF_E_ARG, "ns_interval must be positive");
   int dev_count = 0;
@@ -326,6 +348,17 @@ int wf_create(const wf_config* cfg, wf_domain** out) {
   for (PlaneRef r : d->all_planes()) {
     rc = r.d ? d->alloc_plane_d(r.d) : d->alloc_plane_i(r.i);
     if (rc) break;
+  }
+  d->mixed = cfg->precision == WF_MIXED;
+  {
+    float** f32[] = {&d->f.xr[0], &d->f.xr[1], &d->f.xr[2], &d->f.v32[0], &d->f.v32[1],
+                     &d->f.v32[2], &d->f.h32, &d->f.m32, &d->f.mdj32, &d->f.mdj32_alt,
+                     &d->f.rho32, &d->f.cs32, &d->f.sr32[0], &d->f.sr32[1], &d->f.sr32[2],
+                     &d->f.sr32[3], &d->f.sr32[4], &d->f.sr32[5], &d->f.T32, &d->f.T32_alt,
+                     &d->f.k32};
+    for (float** q : f32)
+      if (!rc && d->mixed) rc = d->alloc_plane_f(q);
+    if (!rc && d->mixed) rc = d->alloc_plane_i(&d->f.cell);
   }
   if (!rc) rc = d->alloc_plane_d(&d->f.rho_alt);
   if (!rc) rc = d->alloc_plane_d(&d->f.T_alt);
@@ -430,6 +463,7 @@ int wf_set_field(wf_domain* d, int field, const void* host, size_t bytes) {
     d->snapshot_set = true;
     d->snap_gen++;
   }
+  d->mirror_dirty = true;
   HIPC(hipGetLastError());
   HIPC(hipStreamSynchronize(d->stream));
   return WF_OK;
@@ -465,6 +499,7 @@ int wf_fill_field(wf_domain* d, int field, double value) {
     d->snap_gen++;
   }
   if (field == WF_F_X_SNAPSHOT) d->snap_gen++;
+  d->mirror_dirty = true;
   HIPC(hipGetLastError());
   HIPC(hipStreamSynchronize(d->stream));
   return WF_OK;
@@ -519,7 +554,8 @@ int wf_nsearch_build(wf_domain* d) {
   for (int c = 0; c < 3; c++) src[c] = stale ? d->f.xs[c] : d->f.x[c];
   int over = 0;
   int rc = ns_build(d->stream, &d->ns, src, d->n, d->npad, 2.0 * d->cfg.h_glob, d->gather,
-                    d->nb_count, d->nb_list, d->ctr, &over, d->perm, d->perm_tmp, &d->tl);
+                    d->nb_count, d->nb_list, d->ctr, &over, d->perm, d->perm_tmp, &d->tl,
+                    &d->grid);
   if (rc)
     return fail(rc, rc == WF_E_NONFINITE ? "non-finite particle position"
                     : rc == WF_E_NBCAP   ? "a neighbour tile exceeds its cell/slot budget"
@@ -538,6 +574,10 @@ int wf_nsearch_build(wf_domain* d) {
   HIPC(hipGetLastError());
   d->nb_built = true;
   d->built_gen = d->snap_gen;
+  for (int c = 0; c < 3; c++) d->frame.org[c] = d->grid.org[c];
+  d->frame.cs = 1.0 / d->grid.inv_cs;
+  d->frame.cs32 = (float)d->frame.cs;
+  d->mirror_dirty = true;
   d->ns_builds++;
   return timer_end(d, WF_K_NSEARCH);
 }
@@ -594,20 +634,31 @@ int wf_get_neighbors(wf_domain* d, int* counts, int* offsets, int* list, int64_t
 int wf_calc_dens_inc(wf_domain* d) {
   if (!d) return fail(WF_E_ARG, "null domain");
   NEED_NB(d);
-  launch_dens_tiled(d->stream, d->f, d->tl, d->n, d->solid, d->cfg.dt, false);
+  if (d->mixed) {
+    d->refresh_mirrors();
+    launch_dens_mixed(d->stream, d->f, d->tl, d->n, d->solid, d->cfg.dt, false, d->frame.cs32);
+  } else {
+    launch_dens_tiled(d->stream, d->f, d->tl, d->n, d->solid, d->cfg.dt, false);
+  }
   HIPC(hipGetLastError());
   return WF_OK;
 }
 int wf_update_density(wf_domain* d, double dt) {
   if (!d) return fail(WF_E_ARG, "null domain");
   launch_update_density(d->stream, d->f, d->params(), dt);
+  d->mirror_dirty = true;
   HIPC(hipGetLastError());
   return WF_OK;
 }
 int wf_calc_rate_tensors(wf_domain* d) {
   if (!d) return fail(WF_E_ARG, "null domain");
   NEED_NB(d);
-  launch_rates_tiled(d->stream, d->f, d->tl, d->solid, d->cfg.quirks);
+  if (d->mixed) {
+    d->refresh_mirrors();
+    launch_rates_mixed(d->stream, d->f, d->tl, d->solid, d->cfg.quirks, d->frame.cs32);
+  } else {
+    launch_rates_tiled(d->stream, d->f, d->tl, d->solid, d->cfg.quirks);
+  }
   HIPC(hipGetLastError());
   return WF_OK;
 }
@@ -619,48 +670,72 @@ int wf_pressure(wf_domain* d) {
 }
 int wf_stress_strain(wf_domain* d) {
   if (!d) return fail(WF_E_ARG, "null domain");
-  launch_stress(d->stream, d->f, d->params(), d->ctr, false, true);
+  launch_stress(d->stream, d->f, d->params(), d->ctr, false, true, false);
+  d->mirror_dirty = true;
   HIPC(hipGetLastError());
   return WF_OK;
 }
 int wf_calc_accel(wf_domain* d) {
   if (!d) return fail(WF_E_ARG, "null domain");
   NEED_NB(d);
-  launch_accel_tiled(d->stream, d->f, d->tl, d->solid, d->cfg.alpha, d->cfg.beta);
+  if (d->mixed) {
+    d->refresh_mirrors();
+    launch_accel_mixed(d->stream, d->f, d->tl, d->solid, d->cfg.alpha, d->cfg.beta,
+                       d->frame.cs32);
+  } else {
+    launch_accel_tiled(d->stream, d->f, d->tl, d->solid, d->cfg.alpha, d->cfg.beta);
+  }
   HIPC(hipGetLastError());
   return WF_OK;
 }
 int wf_apply_bc_vel(wf_domain* d, int zone, double vx, double vy, double vz) {
   if (!d) return fail(WF_E_ARG, "null domain");
   launch_apply_bc(d->stream, d->f, d->n, zone, vx, vy, vz);
+  d->mirror_dirty = true;
   HIPC(hipGetLastError());
   return WF_OK;
 }
 int wf_update_pos_fraser(wf_domain* d, double dt) {
   if (!d) return fail(WF_E_ARG, "null domain");
   launch_update_pos(d->stream, d->f, d->n, dt);
+  d->mirror_dirty = true;
   HIPC(hipGetLastError());
   return WF_OK;
 }
 int wf_update_vel(wf_domain* d, double dt) {
   if (!d) return fail(WF_E_ARG, "null domain");
   launch_update_vel(d->stream, d->f, d->n, dt);
+  d->mirror_dirty = true;
   HIPC(hipGetLastError());
   return WF_OK;
 }
 int wf_calc_temp_inc(wf_domain* d) {
   if (!d) return fail(WF_E_ARG, "null domain");
   NEED_NB(d);
-  launch_cond_tiled(d->stream, d->f, d->tl, d->n, d->cfg.T_inf, d->cfg.h_conv, d->cfg.dt, false,
-                    false);
+  if (d->mixed) {
+    d->refresh_mirrors();
+    launch_cond_mixed(d->stream, d->f, d->tl, d->n, d->cfg.T_inf, d->cfg.h_conv, d->cfg.dt,
+                      false, false, d->frame.cs32);
+  } else {
+    launch_cond_tiled(d->stream, d->f, d->tl, d->n, d->cfg.T_inf, d->cfg.h_conv, d->cfg.dt, false,
+                      false);
+  }
   HIPC(hipGetLastError());
   return WF_OK;
 }
 int wf_thermal_calcs(wf_domain* d, double dt) {
   if (!d) return fail(WF_E_ARG, "null domain");
   NEED_NB(d);
-  launch_cond_tiled(d->stream, d->f, d->tl, d->n, d->cfg.T_inf, d->cfg.h_conv, dt, false, false);
+  if (d->mixed) {
+    d->refresh_mirrors();
+    launch_cond_mixed(d->stream, d->f, d->tl, d->n, d->cfg.T_inf, d->cfg.h_conv, dt, false,
+                      false, d->frame.cs32);
+  } else {
+    launch_cond_tiled(d->stream, d->f, d->tl, d->n, d->cfg.T_inf, d->cfg.h_conv, dt, false,
+                      false);
+  }
   launch_leapfrog(d->stream, d->f, d->n, dt, d->isfirst_step);
+  d->mirror_dirty = true;
   d->isfirst_step = false;
   HIPC(hipGetLastError());
   return WF_OK;
@@ -691,36 +766,53 @@ int wf_step(wf_domain* d, int n_steps) {
         rc = wf_nsearch_build(d);
         if (rc) return rc;
       }
+      d->refresh_mirrors();
+      const bool mx = d->mixed;
+      const float cs32 = d->frame.cs32;
       timer_begin(d);
-      launch_dens_tiled(d->stream, d->f, d->tl, d->n, d->solid, p.dt, true);
+      if (mx) {
+        launch_dens_mixed(d->stream, d->f, d->tl, d->n, d->solid, p.dt, true, cs32);
+        std::swap(d->f.mdj32, d->f.mdj32_alt);
+      } else {
+        launch_dens_tiled(d->stream, d->f, d->tl, d->n, d->solid, p.dt, true);
+      }
       std::swap(d->f.rho, d->f.rho_alt);
       if ((rc = timer_end(d, WF_K_DENS))) return rc;
       timer_begin(d);
-      launch_rates_tiled(d->stream, d->f, d->tl, d->solid, p.quirks);
+      if (mx) launch_rates_mixed(d->stream, d->f, d->tl, d->solid, p.quirks, cs32);
+      else launch_rates_tiled(d->stream, d->f, d->tl, d->solid, p.quirks);
       if ((rc = timer_end(d, WF_K_RATES))) return rc;
       timer_begin(d);
-      launch_stress(d->stream, d->f, p, d->ctr, true, true);
+      launch_stress(d->stream, d->f, p, d->ctr, true, true, mx);
       if ((rc = timer_end(d, WF_K_STRESS))) return rc;
       timer_begin(d);
-      launch_accel_tiled(d->stream, d->f, d->tl, d->solid, p.alpha, p.beta);
+      if (mx) launch_accel_mixed(d->stream, d->f, d->tl, d->solid, p.alpha, p.beta, cs32);
+      else launch_accel_tiled(d->stream, d->f, d->tl, d->solid, p.alpha, p.beta);
       if ((rc = timer_end(d, WF_K_ACCEL))) return rc;
       if (!d->is_yielding) {  // SolverFraser.cu:260-271, on the device-side max
         HIPC(hipMemcpyAsync(d->ctr_host, d->ctr, sizeof(Counters), hipMemcpyDeviceToHost,
                             d->stream));
         HIPC(hipStreamSynchronize(d->stream));
-        double mx;
+        double mxp;
         unsigned long long b = d->ctr_host->max_pl_strain_bits;
-        std::memcpy(&mx, &b, sizeof(mx));
-        if (mx > d->cfg.ns_min_plastic) d->is_yielding = true;
+        std::memcpy(&mxp, &b, sizeof(mxp));
+        if (mxp > d->cfg.ns_min_plastic) d->is_yielding = true;
       }
       timer_begin(d);
-      launch_integrate(d->stream, d->f, d->n, d->cfg.dt, d->bcs);
+      launch_integrate(d->stream, d->f, d->n, d->cfg.dt, d->bcs, mx ? &d->frame : nullptr);
       if ((rc = timer_end(d, WF_K_INTEGRATE))) return rc;
     }
     if (d->cfg.thermal) {
+      if (d->mixed) d->refresh_mirrors();
       timer_begin(d);
-      launch_cond_tiled(d->stream, d->f, d->tl, d->n, p.T_inf, p.h_conv, p.dt, true,
-                        d->isfirst_step);
+      if (d->mixed) {
+        launch_cond_mixed(d->stream, d->f, d->tl, d->n, p.T_inf, p.h_conv, p.dt, true,
+                          d->isfirst_step, d->frame.cs32);
+        std::swap(d->f.T32, d->f.T32_alt);
+      } else {
+        launch_cond_tiled(d->stream, d->f, d->tl, d->n, p.T_inf, p.h_conv, p.dt, true,
+                          d->isfirst_step);
+      }
       std::swap(d->f.T, d->f.T_alt);
       if ((rc = timer_end(d, WF_K_THERMAL))) return rc;
     }

@@ -26,6 +26,20 @@ struct Fields {
   double *T, *T_alt, *Ta, *Tb, *dTdt, *k, *cp, *qpl;
   double *sigma[6], *ss[6], *strain[6], *strrate[6], *rotrate[6];
   int *ID, *BC_T;
+  // FP32 mirrors read by the mixed-precision pair loops (WF_MIXED): position
+  // relative to the origin of an integer grid cell (cell packed as
+  // cx | cy << 11 | cz << 22), velocity, h, m, m/rho, rho, Cs, sigma/rho^2,
+  // T, k.  mdj32 and T32 are double-buffered like rho and T.
+  float *xr[3], *v32[3], *h32, *m32, *mdj32, *mdj32_alt, *rho32, *cs32, *sr32[6];
+  float *T32, *T32_alt, *k32;
+  int* cell;
+};
+
+// frame of the FP32 cell-relative coordinates
+struct Frame32 {
+  double org[3];
+  double cs;      // cell edge (FP64) used to form the relative coordinates
+  float cs32;     // the same edge in FP32, used inside the pair loops
 };
 
 // Neighbour lists: interleaved, entry k of particle i at list[k*npad + i]
@@ -81,16 +95,29 @@ void launch_accel_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int
 void launch_cond_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int n, double T_inf,
                        double h_conv, double dt, bool leap, bool first);
 
+// mixed precision (FP32 pair loops, compensated sums)
+void launch_dens_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int n, int solid,
+                       double dt, bool fuse, float cs32);
+void launch_rates_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
+                        int quirks, float cs32);
+void launch_accel_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
+                        double alpha, double beta, float cs32);
+void launch_cond_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int n, double T_inf,
+                       double h_conv, double dt, bool leap, bool first, float cs32);
+// refresh every FP32 mirror from the FP64 state (after uploads and rebuilds)
+void launch_mirror32(hipStream_t s, const Fields& f, int n, const Frame32& fr);
+
 // ---- per-particle kernels (wf_kernels.hip) --------------------------------
 void launch_update_density(hipStream_t s, const Fields& f, const StepParams& p, double dt);
 void launch_pressure(hipStream_t s, const Fields& f, const StepParams& p);
 void launch_stress(hipStream_t s, const Fields& f, const StepParams& p, Counters* ctr,
-                   bool fuse_pressure, bool energy);
+                   bool fuse_pressure, bool energy, bool mirror32);
 void launch_apply_bc(hipStream_t s, const Fields& f, int n, int zone, double vx, double vy,
                      double vz);
 void launch_update_pos(hipStream_t s, const Fields& f, int n, double dt);
 void launch_update_vel(hipStream_t s, const Fields& f, int n, double dt);
-void launch_integrate(hipStream_t s, const Fields& f, int n, double dt, const BcTable& bc);
+void launch_integrate(hipStream_t s, const Fields& f, int n, double dt, const BcTable& bc,
+                      const Frame32* fr /* non-null: also refresh xr, v32 */);
 void launch_leapfrog(hipStream_t s, const Fields& f, int n, double dt, bool first);
 void launch_int_energy(hipStream_t s, const Fields& f, const StepParams& p);
 void launch_sum(hipStream_t s, const double* v, int n, double* partial, int nblocks,
@@ -108,7 +135,7 @@ struct NsResult {
 // the new one.
 int ns_build(hipStream_t s, NsWork** work, const double* const pos_src[3], int n, int64_t npad,
              double radius, int* gather, int* count, int* list, Counters* ctr, int* over_cap,
-             int* perm, int* perm_tmp, TileLists* tiles);
+             int* perm, int* perm_tmp, TileLists* tiles, GridDesc* grid_out);
 void ns_free(NsWork* w);
 
 // generic gathers used to reorder fields: dst[i] = src[idx[i]]

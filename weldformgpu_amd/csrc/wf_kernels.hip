@@ -87,7 +87,7 @@ __device__ double jc_newton(const MatParams& mp, double s_trial, double G, doubl
 // C: Jaumann stress update + return map, StressStrainOne (Mechanical.cu:458-554),
 // optionally with the pressure (FUSE_P) and the per-particle internal-energy
 // increment (ENERGY, Energy.cu:5-19 with per-particle indexing).
-template <bool FUSE_P, bool ENERGY>
+template <bool FUSE_P, bool ENERGY, bool MIR>
 __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Counters* ctr) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   bool active = i < p.solid;
@@ -200,6 +200,15 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Count
     f.sigma[3][i] = S.xy;
     f.sigma[4][i] = S.yz;
     f.sigma[5][i] = S.xz;
+    if (MIR) {  // sigma/rho^2 for the FP32 acceleration loop
+      const double ir2 = 1.0 / (f.rho[i] * f.rho[i]);
+      f.sr32[0][i] = (float)((-pi + S.xx) * ir2);
+      f.sr32[1][i] = (float)((-pi + S.yy) * ir2);
+      f.sr32[2][i] = (float)((-pi + S.zz) * ir2);
+      f.sr32[3][i] = (float)(S.xy * ir2);
+      f.sr32[4][i] = (float)(S.yz * ir2);
+      f.sr32[5][i] = (float)(S.xz * ir2);
+    }
     f.ss[0][i] = S.xx; f.ss[1][i] = S.yy; f.ss[2][i] = S.zz;
     f.ss[3][i] = S.xy; f.ss[4][i] = S.yz; f.ss[5][i] = S.xz;
     f.strain[0][i] += D.xx * dt; f.strain[1][i] += D.yy * dt; f.strain[2][i] += D.zz * dt;
@@ -230,7 +239,9 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Count
 // ---------------------------------------------------------------------------
 // F: velocity BCs + UpdatePosFraser + UpdateVel + BCs in one pass
 // (SolverFraser.cu:273-281; Mechanical.cu:142-191,572-584)
-__global__ __launch_bounds__(kBlock) void k_integrate(Fields f, int n, double dt, BcTable bc) {
+template <bool MIR>
+__global__ __launch_bounds__(kBlock) void k_integrate(Fields f, int n, double dt, BcTable bc,
+                                                     Frame32 fr) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int id = f.ID[i];
@@ -272,6 +283,45 @@ __global__ __launch_bounds__(kBlock) void k_integrate(Fields f, int n, double dt
   f.a[0][i] = a0;
   f.a[1][i] = a1;
   f.a[2][i] = a2;
+  if (MIR) {
+    int cx, cy, cz;
+    unpack_cell(f.cell[i], cx, cy, cz);
+    f.xr[0][i] = (float)(f.x[0][i] - (fr.org[0] + cx * fr.cs));
+    f.xr[1][i] = (float)(f.x[1][i] - (fr.org[1] + cy * fr.cs));
+    f.xr[2][i] = (float)(f.x[2][i] - (fr.org[2] + cz * fr.cs));
+    f.v32[0][i] = (float)v0;
+    f.v32[1][i] = (float)v1;
+    f.v32[2][i] = (float)v2;
+  }
+}
+
+// every FP32 mirror from the FP64 state; the integer cell of each particle is
+// (re)chosen here and kept until the next refresh
+__global__ __launch_bounds__(kBlock) void k_mirror32(Fields f, int n, Frame32 fr) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int c[3];
+  for (int a = 0; a < 3; a++) {
+    double u = floor((f.x[a][i] - fr.org[a]) / fr.cs);
+    int lim = a < 2 ? 2047 : 1023;
+    c[a] = (int)fmin(fmax(u, 0.0), (double)lim);
+    f.xr[a][i] = (float)(f.x[a][i] - (fr.org[a] + c[a] * fr.cs));
+    f.v32[a][i] = (float)f.v[a][i];
+  }
+  f.cell[i] = pack_cell(c[0], c[1], c[2]);
+  const double rho = f.rho[i];
+  f.h32[i] = (float)f.h[i];
+  f.m32[i] = (float)f.m[i];
+  const float mdj = (float)(f.m[i] / rho);
+  f.mdj32[i] = mdj;
+  f.mdj32_alt[i] = mdj;
+  f.rho32[i] = (float)rho;
+  f.cs32[i] = (float)f.Cs[i];
+  const double ir2 = 1.0 / (rho * rho);
+  for (int k = 0; k < 6; k++) f.sr32[k][i] = (float)(f.sigma[k][i] * ir2);
+  f.T32[i] = (float)f.T[i];
+  f.T32_alt[i] = (float)f.T[i];
+  f.k32[i] = (float)f.k[i];
 }
 
 __global__ __launch_bounds__(kBlock) void k_apply_bc(Fields f, int n, int zone, double vx,
@@ -378,17 +428,19 @@ void launch_pressure(hipStream_t s, const Fields& f, const StepParams& p) {
   hipLaunchKernelGGL(k_pressure, dim3(grid_for(p.n)), dim3(kBlock), 0, s, f, p.n);
 }
 void launch_stress(hipStream_t s, const Fields& f, const StepParams& p, Counters* ctr,
-                   bool fuse_pressure, bool energy) {
+                   bool fuse_pressure, bool energy, bool mirror32) {
   if (p.solid <= 0) return;
   dim3 g(grid_for(p.solid)), b(kBlock);
-  if (fuse_pressure && energy)
-    hipLaunchKernelGGL((k_stress<true, true>), g, b, 0, s, f, p, ctr);
+  if (mirror32)
+    hipLaunchKernelGGL((k_stress<true, true, true>), g, b, 0, s, f, p, ctr);
+  else if (fuse_pressure && energy)
+    hipLaunchKernelGGL((k_stress<true, true, false>), g, b, 0, s, f, p, ctr);
   else if (fuse_pressure)
-    hipLaunchKernelGGL((k_stress<true, false>), g, b, 0, s, f, p, ctr);
+    hipLaunchKernelGGL((k_stress<true, false, false>), g, b, 0, s, f, p, ctr);
   else if (energy)
-    hipLaunchKernelGGL((k_stress<false, true>), g, b, 0, s, f, p, ctr);
+    hipLaunchKernelGGL((k_stress<false, true, false>), g, b, 0, s, f, p, ctr);
   else
-    hipLaunchKernelGGL((k_stress<false, false>), g, b, 0, s, f, p, ctr);
+    hipLaunchKernelGGL((k_stress<false, false, false>), g, b, 0, s, f, p, ctr);
 }
 void launch_apply_bc(hipStream_t s, const Fields& f, int n, int zone, double vx, double vy,
                      double vz) {
@@ -403,9 +455,19 @@ void launch_update_vel(hipStream_t s, const Fields& f, int n, double dt) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_update_vel, dim3(grid_for(n)), dim3(kBlock), 0, s, f, n, dt);
 }
-void launch_integrate(hipStream_t s, const Fields& f, int n, double dt, const BcTable& bc) {
+void launch_integrate(hipStream_t s, const Fields& f, int n, double dt, const BcTable& bc,
+                      const Frame32* fr) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_integrate, dim3(grid_for(n)), dim3(kBlock), 0, s, f, n, dt, bc);
+  if (fr)
+    hipLaunchKernelGGL(k_integrate<true>, dim3(grid_for(n)), dim3(kBlock), 0, s, f, n, dt, bc,
+                       *fr);
+  else
+    hipLaunchKernelGGL(k_integrate<false>, dim3(grid_for(n)), dim3(kBlock), 0, s, f, n, dt, bc,
+                       Frame32{});
+}
+void launch_mirror32(hipStream_t s, const Fields& f, int n, const Frame32& fr) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_mirror32, dim3(grid_for(n)), dim3(kBlock), 0, s, f, n, fr);
 }
 void launch_leapfrog(hipStream_t s, const Fields& f, int n, double dt, bool first) {
   if (n <= 0) return;

@@ -315,7 +315,7 @@ void ns_free(NsWork* w) {
 // gather: out slot -> old sorted index; perm (in/out): sorted slot -> original
 int ns_build(hipStream_t s, NsWork** work, const double* const pos_src[3], int n, int64_t npad,
              double radius, int* gather, int* count, int* list, Counters* ctr, int* over_cap,
-             int* perm, int* perm_tmp, TileLists* tiles) {
+             int* perm, int* perm_tmp, TileLists* tiles, GridDesc* grid_out) {
   if (!*work) *work = new NsWork();
   NsWork* w = *work;
   int rc = ensure(w, n, 0);
@@ -404,6 +404,7 @@ int ns_build(hipStream_t s, NsWork** work, const double* const pos_src[3], int n
   launch_tile_lists(s, pfn, n, g, w->cell_start, w->tile_keys, w->tile_nc, w->tile_uoff, r2,
                     npad, count, list, w->slot, w->tile_j, &ctr->nb_over_cap);
   if (hipGetLastError() != hipSuccess) return WF_E_HIP;
+  *grid_out = g;
   tiles->count = count;
   tiles->slotq = (const uint2*)w->slot;
   tiles->uoff = w->tile_uoff;

@@ -59,21 +59,22 @@ __device__ __forceinline__ int xcd_tile(int b, int nt) {
 template <class K>
 __global__ __launch_bounds__(kTileP) void k_pair_tiled(Fields f, TileLists tl,
                                                        typename K::Prm prm) {
-  __shared__ double lds[K::NF][kTileChunk];
+  __shared__ typename K::LT lds[K::NF][kTileChunk];
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
   const int i = t * kTileP + tid;
   const bool act = i < prm.limit;
+  const typename K::Ctx cx = K::ctx(f, t, prm);
   typename K::Own own;
   typename K::Acc acc;
-  K::init(f, i, act, own, acc);
+  K::init(f, i, act, own, acc, cx);
   const int ub = tl.uoff[t], ucnt = tl.uoff[t + 1] - ub;
   SlotReader sr;
   sr.init(tl, i, act ? tl.count[i] : 0);
   for (int c0 = 0; c0 < ucnt; c0 += kTileChunk) {
     const int m = min(kTileChunk, ucnt - c0);
     __syncthreads();
-    for (int s = tid; s < m; s += kTileP) K::stage(f, tl.tj[ub + c0 + s], lds, s);
+    for (int s = tid; s < m; s += kTileP) K::stage(f, tl.tj[ub + c0 + s], lds, s, cx);
     __syncthreads();
     const int lim = c0 + m;
     while (sr.k < sr.cnt) {
@@ -85,6 +86,13 @@ __global__ __launch_bounds__(kTileP) void k_pair_tiled(Fields f, TileLists tl,
   }
   K::finish(f, i, act, own, acc, prm);
 }
+
+// FP64 policies need no per-tile context
+struct NoCtx {};
+#define WF_FP64_POLICY                                                           \
+  typedef double LT;                                                             \
+  typedef NoCtx Ctx;                                                             \
+  __device__ static Ctx ctx(const Fields&, int, const Prm&) { return Ctx{}; }
 
 // ---------------------------------------------------------------------------
 // Per-pair cubic-spline gradient GK = (1/r) dW/dr (Functions.cu:76-88) from
@@ -136,6 +144,7 @@ struct DensK {
     double dt;
     int fuse;
   };
+  WF_FP64_POLICY
   struct Own {
     double x, y, z, vx, vy, vz, rho;
     HConst hc;
@@ -143,7 +152,7 @@ struct DensK {
   struct Acc {
     double s;
   };
-  __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a) {
+  __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx&) {
     a.s = 0.0;
     if (!act) return;
     o.x = f.x[0][i]; o.y = f.x[1][i]; o.z = f.x[2][i];
@@ -151,7 +160,8 @@ struct DensK {
     o.rho = f.rho[i];
     o.hc = hconst(f.h[i]);
   }
-  __device__ static void stage(const Fields& f, int j, double (*L)[kTileChunk], int s) {
+  __device__ static void stage(const Fields& f, int j, double (*L)[kTileChunk], int s,
+                               const Ctx&) {
     L[0][s] = f.x[0][j]; L[1][s] = f.x[1][j]; L[2][s] = f.x[2][j];
     L[3][s] = f.v[0][j]; L[4][s] = f.v[1][j]; L[5][s] = f.v[2][j];
     L[6][s] = f.h[j]; L[7][s] = f.m[j] / f.rho[j];
@@ -186,6 +196,7 @@ struct RatesK {
     int limit;
     double xz_sign;
   };
+  WF_FP64_POLICY
   struct Own {
     double x, y, z, vx, vy, vz;
     HConst hc;
@@ -193,15 +204,16 @@ struct RatesK {
   struct Acc {
     double dxx, dyy, dzz, dxy, dyz, dxz, wxy, wyz, wxz;
   };
-  __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a) {
+  __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx&) {
     a = Acc{0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (!act) return;
     o.x = f.x[0][i]; o.y = f.x[1][i]; o.z = f.x[2][i];
     o.vx = f.v[0][i]; o.vy = f.v[1][i]; o.vz = f.v[2][i];
     o.hc = hconst(f.h[i]);
   }
-  __device__ static void stage(const Fields& f, int j, double (*L)[kTileChunk], int s) {
-    DensK::stage(f, j, L, s);
+  __device__ static void stage(const Fields& f, int j, double (*L)[kTileChunk], int s,
+                               const Ctx&) {
+    DensK::stage(f, j, L, s, NoCtx{});
   }
   __device__ static void pair(double (*L)[kTileChunk], int s, const Own& o, Acc& a, const Prm&) {
     const double xx = o.x - L[0][s], xy = o.y - L[1][s], xz = o.z - L[2][s];
@@ -243,6 +255,7 @@ struct AccelK {
     int limit;
     double alpha, beta;
   };
+  WF_FP64_POLICY
   struct Own {
     double x, y, z, vx, vy, vz, di, Ci, s0, s1, s2, s3, s4, s5;
     HConst hc;
@@ -250,7 +263,7 @@ struct AccelK {
   struct Acc {
     double ax, ay, az;
   };
-  __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a) {
+  __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx&) {
     a = Acc{0, 0, 0};
     if (!act) return;
     o.x = f.x[0][i]; o.y = f.x[1][i]; o.z = f.x[2][i];
@@ -261,7 +274,8 @@ struct AccelK {
     o.s0 = f.sigma[0][i] * idi; o.s1 = f.sigma[1][i] * idi; o.s2 = f.sigma[2][i] * idi;
     o.s3 = f.sigma[3][i] * idi; o.s4 = f.sigma[4][i] * idi; o.s5 = f.sigma[5][i] * idi;
   }
-  __device__ static void stage(const Fields& f, int j, double (*L)[kTileChunk], int s) {
+  __device__ static void stage(const Fields& f, int j, double (*L)[kTileChunk], int s,
+                               const Ctx&) {
     L[0][s] = f.x[0][j]; L[1][s] = f.x[1][j]; L[2][s] = f.x[2][j];
     L[3][s] = f.v[0][j]; L[4][s] = f.v[1][j]; L[5][s] = f.v[2][j];
     const double rj = f.rho[j];
@@ -315,6 +329,7 @@ struct CondK {
     double T_inf, h_conv, dt;
     int leap, first;
   };
+  WF_FP64_POLICY
   struct Own {
     double x, y, z, k, T;
     HConst hc;
@@ -322,14 +337,15 @@ struct CondK {
   struct Acc {
     double q;
   };
-  __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a) {
+  __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx&) {
     a.q = 0.0;
     if (!act) return;
     o.x = f.x[0][i]; o.y = f.x[1][i]; o.z = f.x[2][i];
     o.k = f.k[i]; o.T = f.T[i];
     o.hc = hconst(f.h[i]);
   }
-  __device__ static void stage(const Fields& f, int j, double (*L)[kTileChunk], int s) {
+  __device__ static void stage(const Fields& f, int j, double (*L)[kTileChunk], int s,
+                               const Ctx&) {
     L[0][s] = f.x[0][j]; L[1][s] = f.x[1][j]; L[2][s] = f.x[2][j];
     L[3][s] = f.h[j]; L[4][s] = f.m[j] / f.rho[j]; L[5][s] = f.T[j]; L[6][s] = f.k[j];
   }
@@ -373,6 +389,323 @@ struct CondK {
   }
 };
 
+// ===========================================================================
+// Mixed precision (WF_MIXED): the same pair physics in FP32, on FP32 mirrors,
+// with coordinates relative to the tile's base cell (x = xr + (c - c_tile) cs,
+// c an integer grid cell), compensated (Kahan-Neumaier) FP32 sums and FP64
+// results.  Per-particle terms stay FP64.
+
+// compensated FP32 accumulator
+struct KSum {
+  float s, c;
+  __device__ __forceinline__ void zero() { s = 0.f; c = 0.f; }
+  __device__ __forceinline__ void add(float y) {
+#pragma clang fp contract(off)
+    const float t = s + y;
+    c += (fabsf(s) >= fabsf(y)) ? ((s - t) + y) : ((y - t) + s);
+    s = t;
+  }
+  __device__ __forceinline__ double val() const { return (double)s + (double)c; }
+};
+
+struct HConstF {
+  float h, inv_h, C;
+};
+__device__ __forceinline__ HConstF hconst_f(float h) {
+  const float hh = h * h;
+  return HConstF{h, __builtin_amdgcn_rcpf(h), __builtin_amdgcn_rcpf(hh * hh * (float)kPi)};
+}
+// FP32 cubic-spline gradient (1/r) dW/dr from r^2 (Functions.cu:76-88)
+__device__ __forceinline__ float gk_pair_f(float r2, float hj, const HConstF& hc, float& h_) {
+  const float rinv = r2 > 0.f ? __builtin_amdgcn_rsqf(r2) : 0.f;
+  const float rij = r2 * rinv;
+  float inv_h = hc.inv_h, C = hc.C;
+  h_ = hc.h;
+  if (hj != hc.h) {
+    h_ = 0.5f * (hc.h + hj);
+    inv_h = __builtin_amdgcn_rcpf(h_);
+    C = __builtin_amdgcn_rcpf(h_ * h_ * h_ * h_ * (float)kPi);
+  }
+  const float q = rij * inv_h;
+  const float t = 2.f - q;
+  const float poly = q < 1.f ? (-3.f * q + 2.25f * q * q) : (q < 2.f ? -0.75f * t * t : 0.f);
+  return C * rinv * poly;
+}
+
+struct Ctx32 {
+  int bx, by, bz;
+  float cs;
+};
+__device__ __forceinline__ Ctx32 ctx32(const Fields& f, int t, float cs) {
+  int bx, by, bz;
+  unpack_cell(f.cell[t * kTileP], bx, by, bz);
+  return Ctx32{bx, by, bz, cs};
+}
+// position of particle j in the tile frame
+__device__ __forceinline__ void pos32(const Fields& f, int j, const Ctx32& c, float& x, float& y,
+                                      float& z) {
+  int cx, cy, cz;
+  unpack_cell(f.cell[j], cx, cy, cz);
+  x = f.xr[0][j] + (float)(cx - c.bx) * c.cs;
+  y = f.xr[1][j] + (float)(cy - c.by) * c.cs;
+  z = f.xr[2][j] + (float)(cz - c.bz) * c.cs;
+}
+
+struct DensM {
+  static constexpr int NF = 8;  // x y z vx vy vz h m/rho
+  typedef float LT;
+  typedef Ctx32 Ctx;
+  struct Prm {
+    int limit, n;
+    double dt;
+    int fuse;
+    float cs;
+  };
+  __device__ static Ctx ctx(const Fields& f, int t, const Prm& p) { return ctx32(f, t, p.cs); }
+  struct Own {
+    float x, y, z, vx, vy, vz;
+    HConstF hc;
+  };
+  typedef KSum Acc;
+  __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx& c) {
+    a.zero();
+    if (!act) return;
+    pos32(f, i, c, o.x, o.y, o.z);
+    o.vx = f.v32[0][i]; o.vy = f.v32[1][i]; o.vz = f.v32[2][i];
+    o.hc = hconst_f(f.h32[i]);
+  }
+  __device__ static void stage(const Fields& f, int j, float (*L)[kTileChunk], int s,
+                               const Ctx& c) {
+    pos32(f, j, c, L[0][s], L[1][s], L[2][s]);
+    L[3][s] = f.v32[0][j]; L[4][s] = f.v32[1][j]; L[5][s] = f.v32[2][j];
+    L[6][s] = f.h32[j]; L[7][s] = f.mdj32[j];
+  }
+  __device__ static void pair(float (*L)[kTileChunk], int s, const Own& o, Acc& a, const Prm&) {
+    const float xx = o.x - L[0][s], xy = o.y - L[1][s], xz = o.z - L[2][s];
+    const float r2 = xx * xx + xy * xy + xz * xz;
+    float h_;
+    const float GK = gk_pair_f(r2, L[6][s], o.hc, h_);
+    const float vx = o.vx - L[3][s], vy = o.vy - L[4][s], vz = o.vz - L[5][s];
+    a.add(L[7][s] * (GK * (vx * xx + vy * xy + vz * xz)));
+  }
+  __device__ static void finish(const Fields& f, int i, bool act, const Own&, const Acc& a,
+                                const Prm& p) {
+    if (act) {
+      const double rho = f.rho[i];
+      const double drho = rho * a.val();
+      f.drho[i] = drho;
+      if (p.fuse) {
+        const double rn = rho + p.dt * drho;
+        f.rho_alt[i] = rn;
+        f.rho32[i] = (float)rn;
+        f.mdj32_alt[i] = (float)(f.m[i] / rn);
+      }
+    } else if (p.fuse && i < p.n) {
+      f.rho_alt[i] = f.rho[i];
+      f.mdj32_alt[i] = f.mdj32[i];
+    }
+  }
+};
+
+struct RatesM {
+  static constexpr int NF = 8;
+  typedef float LT;
+  typedef Ctx32 Ctx;
+  struct Prm {
+    int limit;
+    double xz_sign;
+    float cs;
+  };
+  __device__ static Ctx ctx(const Fields& f, int t, const Prm& p) { return ctx32(f, t, p.cs); }
+  struct Own {
+    float x, y, z, vx, vy, vz;
+    HConstF hc;
+  };
+  struct Acc {
+    KSum dxx, dyy, dzz, dxy, dyz, dxz, wxy, wyz, wxz;
+  };
+  __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx& c) {
+    a.dxx.zero(); a.dyy.zero(); a.dzz.zero(); a.dxy.zero(); a.dyz.zero(); a.dxz.zero();
+    a.wxy.zero(); a.wyz.zero(); a.wxz.zero();
+    if (!act) return;
+    pos32(f, i, c, o.x, o.y, o.z);
+    o.vx = f.v32[0][i]; o.vy = f.v32[1][i]; o.vz = f.v32[2][i];
+    o.hc = hconst_f(f.h32[i]);
+  }
+  __device__ static void stage(const Fields& f, int j, float (*L)[kTileChunk], int s,
+                               const Ctx& c) {
+    DensM::stage(f, j, L, s, c);
+  }
+  __device__ static void pair(float (*L)[kTileChunk], int s, const Own& o, Acc& a, const Prm&) {
+    const float xx = o.x - L[0][s], xy = o.y - L[1][s], xz = o.z - L[2][s];
+    const float r2 = xx * xx + xy * xy + xz * xz;
+    float h_;
+    const float w = gk_pair_f(r2, L[6][s], o.hc, h_) * L[7][s];
+    const float vx = o.vx - L[3][s], vy = o.vy - L[4][s], vz = o.vz - L[5][s];
+    const float ax = w * vx, ay = w * vy, az = w * vz;
+    a.dxx.add(ax * xx);
+    a.dyy.add(ay * xy);
+    a.dzz.add(az * xz);
+    a.dxy.add(ax * xy + ay * xx);
+    a.dyz.add(ay * xz + az * xy);
+    a.dxz.add(ax * xz + az * xx);
+    a.wxy.add(ax * xy - ay * xx);
+    a.wyz.add(ay * xz - az * xy);
+    a.wxz.add(ax * xz - az * xx);
+  }
+  __device__ static void finish(const Fields& f, int i, bool act, const Own&, const Acc& a,
+                                const Prm& p) {
+    if (!act) return;
+    f.strrate[0][i] = -a.dxx.val(); f.strrate[1][i] = -a.dyy.val();
+    f.strrate[2][i] = -a.dzz.val();
+    f.strrate[3][i] = -0.5 * a.dxy.val(); f.strrate[4][i] = -0.5 * a.dyz.val();
+    f.strrate[5][i] = -0.5 * a.dxz.val();
+    f.rotrate[0][i] = 0.0; f.rotrate[1][i] = 0.0; f.rotrate[2][i] = 0.0;
+    f.rotrate[3][i] = -0.5 * a.wxy.val(); f.rotrate[4][i] = -0.5 * a.wyz.val();
+    f.rotrate[5][i] = -0.5 * p.xz_sign * a.wxz.val();
+  }
+};
+
+struct AccelM {
+  static constexpr int NF = 16;  // x y z vx vy vz h m rho Cs (sigma/rho^2)[6]
+  typedef float LT;
+  typedef Ctx32 Ctx;
+  struct Prm {
+    int limit;
+    double alpha, beta;
+    float cs;
+  };
+  __device__ static Ctx ctx(const Fields& f, int t, const Prm& p) { return ctx32(f, t, p.cs); }
+  struct Own {
+    float x, y, z, vx, vy, vz, di, Ci, s0, s1, s2, s3, s4, s5;
+    HConstF hc;
+  };
+  struct Acc {
+    KSum ax, ay, az;
+  };
+  __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx& c) {
+    a.ax.zero(); a.ay.zero(); a.az.zero();
+    if (!act) return;
+    pos32(f, i, c, o.x, o.y, o.z);
+    o.vx = f.v32[0][i]; o.vy = f.v32[1][i]; o.vz = f.v32[2][i];
+    o.di = f.rho32[i]; o.Ci = f.cs32[i];
+    o.hc = hconst_f(f.h32[i]);
+    o.s0 = f.sr32[0][i]; o.s1 = f.sr32[1][i]; o.s2 = f.sr32[2][i];
+    o.s3 = f.sr32[3][i]; o.s4 = f.sr32[4][i]; o.s5 = f.sr32[5][i];
+  }
+  __device__ static void stage(const Fields& f, int j, float (*L)[kTileChunk], int s,
+                               const Ctx& c) {
+    pos32(f, j, c, L[0][s], L[1][s], L[2][s]);
+    L[3][s] = f.v32[0][j]; L[4][s] = f.v32[1][j]; L[5][s] = f.v32[2][j];
+    L[6][s] = f.h32[j]; L[7][s] = f.m32[j]; L[8][s] = f.rho32[j]; L[9][s] = f.cs32[j];
+    L[10][s] = f.sr32[0][j]; L[11][s] = f.sr32[1][j]; L[12][s] = f.sr32[2][j];
+    L[13][s] = f.sr32[3][j]; L[14][s] = f.sr32[4][j]; L[15][s] = f.sr32[5][j];
+  }
+  __device__ static void pair(float (*L)[kTileChunk], int s, const Own& o, Acc& a,
+                              const Prm& p) {
+    const float xx = o.x - L[0][s], xy = o.y - L[1][s], xz = o.z - L[2][s];
+    const float r2 = xx * xx + xy * xy + xz * xz;
+    float h_;
+    const float GK = gk_pair_f(r2, L[6][s], o.hc, h_);
+    const float vx = o.vx - L[3][s], vy = o.vy - L[4][s], vz = o.vz - L[5][s];
+    const float vdx = vx * xx + vy * xy + vz * xz;
+    float PI = 0.f;
+    if (vdx < 0.f && (p.alpha != 0.0 || p.beta != 0.0)) {
+      const float MU = h_ * vdx * __builtin_amdgcn_rcpf(r2 + 0.01f * h_ * h_);
+      const float Cij = 0.5f * (o.Ci + L[9][s]);
+      PI = ((float)p.alpha * Cij * MU + (float)p.beta * MU * MU) *
+           __builtin_amdgcn_rcpf(0.5f * (o.di + L[8][s]));
+    }
+    const float g = GK * L[7][s];
+    const float gx = g * xx, gy = g * xy, gz = g * xz;
+    const float mxx = o.s0 + L[10][s] + PI, myy = o.s1 + L[11][s] + PI;
+    const float mzz = o.s2 + L[12][s] + PI;
+    const float mxy = o.s3 + L[13][s], myz = o.s4 + L[14][s], mxz = o.s5 + L[15][s];
+    a.ax.add(mxx * gx + mxy * gy + mxz * gz);
+    a.ay.add(mxy * gx + myy * gy + myz * gz);
+    a.az.add(mxz * gx + myz * gy + mzz * gz);
+  }
+  __device__ static void finish(const Fields& f, int i, bool act, const Own&, const Acc& a,
+                                const Prm&) {
+    if (!act) return;
+    f.a[0][i] = a.ax.val();
+    f.a[1][i] = a.ay.val();
+    f.a[2][i] = a.az.val();
+  }
+};
+
+struct CondM {
+  static constexpr int NF = 7;  // x y z h m/rho T k
+  typedef float LT;
+  typedef Ctx32 Ctx;
+  struct Prm {
+    int limit;
+    double T_inf, h_conv, dt;
+    int leap, first;
+    float cs;
+  };
+  __device__ static Ctx ctx(const Fields& f, int t, const Prm& p) { return ctx32(f, t, p.cs); }
+  struct Own {
+    float x, y, z, k, T;
+    HConstF hc;
+  };
+  typedef KSum Acc;
+  __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx& c) {
+    a.zero();
+    if (!act) return;
+    pos32(f, i, c, o.x, o.y, o.z);
+    o.k = f.k32[i];
+    o.T = f.T32[i];
+    o.hc = hconst_f(f.h32[i]);
+  }
+  __device__ static void stage(const Fields& f, int j, float (*L)[kTileChunk], int s,
+                               const Ctx& c) {
+    pos32(f, j, c, L[0][s], L[1][s], L[2][s]);
+    L[3][s] = f.h32[j]; L[4][s] = f.mdj32[j]; L[5][s] = f.T32[j]; L[6][s] = f.k32[j];
+  }
+  __device__ static void pair(float (*L)[kTileChunk], int s, const Own& o, Acc& a, const Prm&) {
+    const float xx = o.x - L[0][s], xy = o.y - L[1][s], xz = o.z - L[2][s];
+    const float r2 = xx * xx + xy * xy + xz * xz;
+    float h_;
+    const float GK = gk_pair_f(r2, L[3][s], o.hc, h_);
+    const float kj = L[6][s];
+    const float kh = (kj == o.k) ? 2.f * o.k : 4.f * o.k * kj * __builtin_amdgcn_rcpf(o.k + kj);
+    a.add(L[4][s] * kh * (o.T - L[5][s]) * GK);
+  }
+  __device__ static void finish(const Fields& f, int i, bool act, const Own&, const Acc& a,
+                                const Prm& p) {
+    if (!act) return;
+    const double rhoi = f.rho[i], cpi = f.cp[i], Ti = f.T[i];
+    double acc = a.val() * (1. / (rhoi * cpi));
+    const int bct = f.BC_T[i];
+    if (bct == WF_BCT_CONVECTION) {
+      double mi = f.m[i];
+      double dS2 = pow(mi / rhoi, 0.666666666);
+      double q_conv = p.h_conv * dS2 * (p.T_inf - Ti) / mi;
+      acc += 1. / cpi * q_conv;
+    }
+    double qp = f.qpl[i];
+    if (qp != 0.0) acc += qp * (1. / (rhoi * cpi));
+    if (bct == WF_BCT_FIXED) acc = 0.0;
+    f.dTdt[i] = acc;
+    if (p.leap) {
+      double Tn;
+      if (p.first) {
+        f.Ta[i] = Ti - p.dt / 2.0 * acc;
+        Tn = Ti;
+      } else {
+        double ta = f.Ta[i];
+        f.Tb[i] = ta;
+        double tan = ta + acc * p.dt;
+        f.Ta[i] = tan;
+        Tn = (tan + ta) / 2.;
+      }
+      f.T_alt[i] = Tn;
+      f.T32_alt[i] = (float)Tn;
+    }
+  }
+};
+
 static inline int ntiles(int n) { return (n + kTileP - 1) / kTileP; }
 
 void launch_dens_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int n, int solid,
@@ -400,4 +733,31 @@ void launch_cond_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int 
   hipLaunchKernelGGL(k_pair_tiled<CondK>, dim3(ntiles(n)), dim3(kTileP), 0, s, f, tl, p);
 }
 
+}  // namespace wf
+
+namespace wf {
+void launch_dens_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int n, int solid,
+                       double dt, bool fuse, float cs32) {
+  if (n <= 0) return;
+  DensM::Prm p{solid, n, dt, fuse ? 1 : 0, cs32};
+  hipLaunchKernelGGL(k_pair_tiled<DensM>, dim3(ntiles(n)), dim3(kTileP), 0, s, f, tl, p);
+}
+void launch_rates_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
+                        int quirks, float cs32) {
+  if (solid <= 0) return;
+  RatesM::Prm p{solid, (quirks & WF_Q_ROT_XZ_SIGN) ? -1.0 : 1.0, cs32};
+  hipLaunchKernelGGL(k_pair_tiled<RatesM>, dim3(ntiles(solid)), dim3(kTileP), 0, s, f, tl, p);
+}
+void launch_accel_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
+                        double alpha, double beta, float cs32) {
+  if (solid <= 0) return;
+  AccelM::Prm p{solid, alpha, beta, cs32};
+  hipLaunchKernelGGL(k_pair_tiled<AccelM>, dim3(ntiles(solid)), dim3(kTileP), 0, s, f, tl, p);
+}
+void launch_cond_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int n, double T_inf,
+                       double h_conv, double dt, bool leap, bool first, float cs32) {
+  if (n <= 0) return;
+  CondM::Prm p{n, T_inf, h_conv, dt, leap ? 1 : 0, first ? 1 : 0, cs32};
+  hipLaunchKernelGGL(k_pair_tiled<CondM>, dim3(ntiles(n)), dim3(kTileP), 0, s, f, tl, p);
+}
 }  // namespace wf
