@@ -232,6 +232,8 @@ typedef struct wf_stats {
   int mlp_trainings;       /* online training rounds                        */
   int nb_over_cap;         /* particles whose candidates exceeded the cap   */
   int _pad;
+  double mlp_loss;         /* training loss of the last surrogate round     */
+  int64_t mlp_samples;     /* logged samples available at that round        */
 } wf_stats;
 int wf_get_stats(struct wf_domain* d, wf_stats* out);
 
@@ -242,16 +244,46 @@ int wf_set_timing(struct wf_domain* d, int on);
 enum {
   WF_K_DENS = 0,    /* density-rate pair loop + density update      */
   WF_K_RATES,       /* strain/rotation-rate pair loop               */
-  WF_K_STRESS,      /* pressure + stress/plasticity + energy        */
+  WF_K_STRESS,      /* pressure + stress/plasticity + energy; with the
+                       Newton Johnson-Cook return map, the elastic
+                       predictor and the queue of yielding particles */
   WF_K_ACCEL,       /* acceleration pair loop                       */
   WF_K_INTEGRATE,   /* BC + position/velocity update + BC           */
   WF_K_THERMAL,     /* conduction pair loop + leapfrog              */
   WF_K_NSEARCH,     /* neighbour rebuild (all its kernels)          */
   WF_K_MLP,         /* surrogate training                           */
+  WF_K_RETURN,      /* batched Johnson-Cook return map (+ surrogate) */
   WF_K_COUNT_
 };
 /* Accumulated milliseconds and launch count of kernel id k. */
 int wf_kernel_time(struct wf_domain* d, int k, double* total_ms, int64_t* launches);
+
+/* ---- return-map surrogate ----------------------------------------------------
+ * No counterpart in the reference (an extension of the Johnson-Cook return
+ * map, Mechanical.cu:507-535): a 4 -> 32 -> 32 -> 1 ReLU MLP gives the FP64
+ * Newton its initial plastic increment.  Features per yielding particle:
+ * s_trial/A, 10 eps_p, log10(max(rate,1e-3))/3, homologous temperature;
+ * target: dg / dg_max with dg_max = (s_trial - s_y(eps_p))/(3G).  Inference
+ * runs on BF16 MFMA with FP32 accumulation inside the return-map kernel;
+ * with cfg.surrogate it is trained online every mlp_train_interval steps on
+ * the samples the solver logs (Adam, FP32, deterministic).  Newton always
+ * finishes to newton_tol, so the surrogate changes iteration counts only.
+ * Parameter order: W1[4][32], b1[32], W2[32][32], b2[32], W3[32], b3. */
+#define WF_MLP_PARAMS 1249
+int wf_mlp_get_params(struct wf_domain* d, float* out, int n);
+int wf_mlp_set_params(struct wf_domain* d, const float* in, int n);
+/* zero the Adam moments and step counter */
+int wf_mlp_reset_adam(struct wf_domain* d);
+/* BF16-MFMA inference on n feature rows X[n][4] -> y[n] */
+int wf_mlp_infer(struct wf_domain* d, const float* X, int n, float* y);
+/* gradient of mean((y - Y)^2) over the rows (FP32, deterministic) */
+int wf_mlp_grad(struct wf_domain* d, const float* X, const float* Y, int n, float* grad,
+                float* loss);
+/* iters full-batch Adam steps (lr; beta1 0.9, beta2 0.999, eps 1e-8) */
+int wf_mlp_fit(struct wf_domain* d, const float* X, const float* Y, int n, int iters,
+               float lr, float* loss);
+/* one online round on the logged samples */
+int wf_mlp_train(struct wf_domain* d, int iters, float lr, int64_t* samples, float* loss);
 
 /* ---- host-side input builders (no device needed) ---------------------------
  * Lattice generators and zone assignment of the host domain build, in the

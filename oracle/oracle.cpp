@@ -743,6 +743,59 @@ double or_int_energy_sum(const or_state* s) {
   return sum;
 }
 
+// ---------------------------------------------------------------------------
+// surrogate MLP in FP64: y = W3 . relu(W2^T relu(W1^T x + b1) + b2) + b3
+static void mlp_fwd(const double* w, const double* x, double* h1, double* h2, double* y) {
+  for (int u = 0; u < 32; u++) {
+    double a = w[128 + u];
+    for (int k = 0; k < 4; k++) a += w[k * 32 + u] * x[k];
+    h1[u] = a > 0.0 ? a : 0.0;
+  }
+  double out = w[1248];
+  for (int u = 0; u < 32; u++) {
+    double a = w[1184 + u];
+    for (int k = 0; k < 32; k++) a += w[160 + k * 32 + u] * h1[k];
+    h2[u] = a > 0.0 ? a : 0.0;
+    out += w[1216 + u] * h2[u];
+  }
+  *y = out;
+}
+
+double or_mlp_forward(const double* w, const double* x) {
+  double h1[32], h2[32], y;
+  mlp_fwd(w, x, h1, h2, &y);
+  return y;
+}
+
+double or_mlp_loss_grad(const double* w, const double* X, const double* Y, int n, double* g) {
+  for (int p = 0; p < 1249; p++) g[p] = 0.0;
+  double loss = 0.0;
+  for (int s = 0; s < n; s++) {
+    const double* x = X + 4 * s;
+    double h1[32], h2[32], y;
+    mlp_fwd(w, x, h1, h2, &y);
+    const double e = y - Y[s];
+    loss += e * e / n;
+    const double dy = 2.0 * e / n;
+    double d2[32], d1[32];
+    for (int u = 0; u < 32; u++) d2[u] = h2[u] > 0.0 ? dy * w[1216 + u] : 0.0;
+    for (int k = 0; k < 32; k++) {
+      double a = 0.0;
+      for (int u = 0; u < 32; u++) a += w[160 + k * 32 + u] * d2[u];
+      d1[k] = h1[k] > 0.0 ? a : 0.0;
+    }
+    for (int k = 0; k < 4; k++)
+      for (int u = 0; u < 32; u++) g[k * 32 + u] += d1[u] * x[k];
+    for (int u = 0; u < 32; u++) g[128 + u] += d1[u];
+    for (int k = 0; k < 32; k++)
+      for (int u = 0; u < 32; u++) g[160 + k * 32 + u] += d2[u] * h1[k];
+    for (int u = 0; u < 32; u++) g[1184 + u] += d2[u];
+    for (int u = 0; u < 32; u++) g[1216 + u] += dy * h2[u];
+    g[1248] += dy;
+  }
+  return loss;
+}
+
 int or_nsearch_build(or_state* s, const wf_config* c, int* over_cap) {
   const double* src = (c->quirks & WF_Q_STALE_NS_POS) ? s->xs : s->x;
   int64_t tot = or_neighbors(src, s->n, 2.0 * c->h_glob, s->nb_count, s->nb_offset, s->nb_list,

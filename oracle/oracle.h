@@ -104,6 +104,12 @@ double or_int_energy_sum(const or_state* s);
 /* neighbour rebuild from the snapshot (or x) into s->nb_* */
 int or_nsearch_build(or_state* s, const wf_config* c, int* over_cap);
 
+/* Return-map surrogate network (no reference counterpart): FP64 forward and
+ * mean-squared-error gradient of the 4 -> 32 -> 32 -> 1 ReLU MLP, parameter
+ * order W1[4][32], b1[32], W2[32][32], b2[32], W3[32], b3 (1249 values). */
+double or_mlp_forward(const double* w, const double* x);
+double or_mlp_loss_grad(const double* w, const double* X, const double* Y, int n, double* grad);
+
 /* one Fraser step (SolverFraser.cu:164-399) */
 int or_step(or_state* s, const wf_config* c, or_run* run, int nbc,
             const int* bc_zone, const double* bc_val);

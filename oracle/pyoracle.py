@@ -79,6 +79,8 @@ def lib():
             "or_int_energy_sum": (D, [S]),
             "or_nsearch_build": (I, [S, CF, C.POINTER(I)]),
             "or_step": (I, [S, CF, R, I, P, P]),
+            "or_mlp_forward": (D, [P, P]),
+            "or_mlp_loss_grad": (D, [P, P, P, I, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -245,3 +247,18 @@ def kernel(q, h):
 
 def grad_kernel(q, h):
     return lib().or_grad_kernel(q, h)
+
+
+def mlp_forward(w, x):
+    w = np.ascontiguousarray(w, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    return lib().or_mlp_forward(_ptr(w), _ptr(x))
+
+
+def mlp_loss_grad(w, X, Y):
+    w = np.ascontiguousarray(w, np.float64)
+    X = np.ascontiguousarray(X, np.float64)
+    Y = np.ascontiguousarray(Y, np.float64)
+    g = np.zeros(1249)
+    loss = lib().or_mlp_loss_grad(_ptr(w), _ptr(X), _ptr(Y), len(Y), _ptr(g))
+    return g, loss

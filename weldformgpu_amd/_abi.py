@@ -43,7 +43,8 @@ for _k, _n in enumerate(["sigma", "shearstress", "strain", "strrate", "rotrate"]
 for _k, _n in enumerate(["ID", "BC_T"]):
     FIELDS[_n] = (64 + _k, 1, "i4")
 
-KERNELS = ["dens", "rates", "stress", "accel", "integrate", "thermal", "nsearch", "mlp"]
+KERNELS = ["dens", "rates", "stress", "accel", "integrate", "thermal", "nsearch", "mlp", "return"]
+WF_MLP_PARAMS = 1249
 
 
 class wf_material_params(C.Structure):
@@ -73,7 +74,8 @@ class wf_stats(C.Structure):
     _fields_ = [("step", C.c_int64), ("time", C.c_double), ("is_yielding", C.c_int),
                 ("ns_builds", C.c_int), ("newton_iters", C.c_int64),
                 ("newton_calls", C.c_int64), ("newton_iters_wave", C.c_int64),
-                ("mlp_trainings", C.c_int), ("nb_over_cap", C.c_int), ("_pad", C.c_int)]
+                ("mlp_trainings", C.c_int), ("nb_over_cap", C.c_int), ("_pad", C.c_int),
+                ("mlp_loss", C.c_double), ("mlp_samples", C.c_int64)]
 
 
 def default_config():
@@ -133,6 +135,16 @@ def _proto(lib):
         "wf_get_stats": (I, [P, C.POINTER(wf_stats)]),
         "wf_set_timing": (I, [P, I]),
         "wf_kernel_time": (I, [P, I, C.POINTER(D), C.POINTER(I64)]),
+        "wf_mlp_get_params": (I, [P, VP, I]),
+        "wf_mlp_set_params": (I, [P, VP, I]),
+        "wf_mlp_reset_adam": (I, [P]),
+        "wf_mlp_infer": (I, [P, VP, I, VP]),
+        "wf_mlp_grad": (I, [P, VP, VP, I, VP, C.POINTER(C.c_float)]),
+        "wf_mlp_fit": (I, [P, VP, VP, I, I, C.c_float, C.POINTER(C.c_float)]),
+        "wf_mlp_train": (I, [P, I, C.c_float, C.POINTER(I64), C.POINTER(C.c_float)]),
+        "wf_gen_box": (I, [VP, D, D, D, D, D, VP, I, VP]),
+        "wf_gen_cylinder": (I, [VP, D, D, D, D, VP, I, VP]),
+        "wf_assign_zone": (I, [VP, I, VP, VP, I, VP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -150,7 +162,9 @@ def declared_symbols():
             "wf_calc_rate_tensors", "wf_pressure", "wf_stress_strain", "wf_calc_accel",
             "wf_apply_bc_vel", "wf_update_pos_fraser", "wf_update_vel", "wf_calc_temp_inc",
             "wf_thermal_calcs", "wf_int_energy", "wf_step", "wf_get_stats", "wf_set_timing",
-            "wf_kernel_time"]
+            "wf_kernel_time", "wf_mlp_get_params", "wf_mlp_set_params", "wf_mlp_reset_adam",
+            "wf_mlp_infer", "wf_mlp_grad", "wf_mlp_fit", "wf_mlp_train", "wf_gen_box",
+            "wf_gen_cylinder", "wf_assign_zone"]
 
 
 def load(path=None):

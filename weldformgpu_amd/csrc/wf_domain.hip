@@ -65,6 +65,22 @@ struct wf_domain {
   TileLists tl{};
   GridDesc grid{};
   Frame32 frame{};
+  // return-map queue and surrogate
+  RetQueue rq{};
+  BlockStats* bstats = nullptr;
+  int nblk = 0;  // blocks of the per-particle stress kernel
+  int64_t mlp_cap = 0;
+  bool use_queue = false;
+  float *mlp_w = nullptr, *mlp_m = nullptr, *mlp_v = nullptr, *mlp_grad = nullptr;
+  float *mlp_part = nullptr, *mlp_loss_part = nullptr, *mlp_loss = nullptr;
+  float *mlp_X = nullptr, *mlp_Y = nullptr;
+  int *mlp_flags = nullptr, *mlp_scan = nullptr, *mlp_scan_part = nullptr, *mlp_count = nullptr;
+  int* host_i = nullptr;    // pinned scratch
+  float* host_f = nullptr;  // pinned scratch
+  int adam_t = 0;
+  bool mlp_trained = false;
+  double mlp_last_loss = 0.0;
+  int64_t mlp_last_samples = 0;
   bool mixed = false;
   bool mirror_dirty = true;  // FP32 mirrors out of date with the FP64 state
   bool nb_built = false;
@@ -96,7 +112,16 @@ struct wf_domain {
   int alloc_plane_i(int** p);
   int alloc_plane_f(float** p);
   void refresh_mirrors();
+  int mlp_train_round(int iters, float lr);
+  int mlp_fit(const float* X, const float* Y, int n, int stride, int iters, float lr);
+  int dev_alloc(void** p, size_t bytes);
 };
+
+int wf_domain::dev_alloc(void** p, size_t bytes) {
+  if (hipMalloc(p, bytes) != hipSuccess) return WF_E_NOMEM;
+  owned.push_back(*p);
+  return WF_OK;
+}
 
 namespace {
 
@@ -254,6 +279,24 @@ std::vector<PlaneRef> wf_domain::all_planes() {
   return v;
 }
 
+namespace {
+__global__ void k_fill_i32v(int* a, int v, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+__global__ void k_gather_f32(float* dst, const float* src, const int* idx, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+// splitmix64, for the deterministic initial weights
+uint64_t splitmix64(uint64_t& x) {
+  uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+}  // namespace
+
 int wf_domain::alloc_plane_d(double** p) {
   void* q = nullptr;
   if (hipMalloc(&q, sizeof(double) * npad) != hipSuccess) return WF_E_NOMEM;
@@ -286,6 +329,39 @@ int wf_domain::alloc_plane_i(int** p) {
   *p = (int*)q;
   hipLaunchKernelGGL(k_zero_bytes, dim3(gridn(sizeof(int) * npad)), dim3(256), 0, stream,
                      (unsigned char*)q, (int64_t)(sizeof(int) * npad));
+  return WF_OK;
+}
+
+// one online training round on the logged samples (ascending particle index,
+// at most 16384 taken with a fixed stride)
+int wf_domain::mlp_train_round(int iters, float lr) {
+  launch_mlp_collect(stream, f, n, mlp_X, mlp_Y, mlp_flags, mlp_scan, mlp_scan_part, mlp_count);
+  HIPC(hipMemcpyAsync(host_i, mlp_count, sizeof(int), hipMemcpyDeviceToHost, stream));
+  HIPC(hipStreamSynchronize(stream));
+  const int count = host_i[0];
+  mlp_last_samples = count;
+  if (count < 64) return WF_OK;
+  const int ntrain = std::min(count, 16384);
+  const int stride = count / ntrain;
+  int rc = mlp_fit(mlp_X, mlp_Y, ntrain, stride, iters, lr);
+  if (rc) return rc;
+  mlp_trained = true;
+  mlp_trainings++;
+  return WF_OK;
+}
+
+// Adam iterations on device-resident samples X[s*stride], Y[s*stride]
+int wf_domain::mlp_fit(const float* X, const float* Y, int nsamp, int stride, int iters,
+                       float lr) {
+  const int nblocks = std::max(1, std::min(64, (nsamp + 255) / 256));
+  for (int it = 0; it < iters; it++) {
+    launch_mlp_grad(stream, mlp_w, X, Y, nsamp, stride, mlp_part, mlp_loss_part, nblocks,
+                    mlp_grad, mlp_loss);
+    launch_mlp_adam(stream, mlp_w, mlp_m, mlp_v, mlp_grad, ++adam_t, lr);
+  }
+  HIPC(hipMemcpyAsync(host_f, mlp_loss, sizeof(float), hipMemcpyDeviceToHost, stream));
+  HIPC(hipStreamSynchronize(stream));
+  mlp_last_loss = host_f[0];
   return WF_OK;
 }
 
@@ -330,7 +406,6 @@ int wf_create(const wf_config* cfg, wf_domain** out) {
     return fail(WF_E_ARG, "solid_count out of range");
   if (cfg->precision != WF_FP64 && cfg->precision != WF_MIXED)
     return fail(WF_E_ARG, "unknown precision mode");
-  if (cfg->surrogate) return fail(WF_E_UNSUPPORTED, "surrogate not available in this build");
   if (cfg->ns_interval <= 0) return fail(WF_E_ARG, "ns_interval must be positive");
   int dev_count = 0;
   if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0)
@@ -359,6 +434,55 @@ int wf_create(const wf_config* cfg, wf_domain** out) {
     for (float** q : f32)
       if (!rc && d->mixed) rc = d->alloc_plane_f(q);
     if (!rc && d->mixed) rc = d->alloc_plane_i(&d->f.cell);
+  }
+  // return-map queue, surrogate samples and weights
+  d->use_queue = cfg->return_map == WF_RETURN_NEWTON && cfg->mat.model == WF_MAT_JOHNSON_COOK;
+  if (!rc) rc = d->dev_alloc((void**)&d->rq.idx, sizeof(int) * d->npad);
+  if (!rc) rc = d->dev_alloc((void**)&d->rq.strial, sizeof(double) * d->npad);
+  if (!rc) rc = d->dev_alloc((void**)&d->rq.rt, sizeof(double) * d->npad);
+  if (!rc) rc = d->dev_alloc((void**)&d->rq.gmax, sizeof(double) * d->npad);
+  d->nblk = (int)((d->npad + kBlock - 1) / kBlock);
+  if (!rc) rc = d->dev_alloc((void**)&d->rq.count, sizeof(int) * (d->nblk + 1));
+  if (!rc) rc = d->dev_alloc((void**)&d->bstats, sizeof(BlockStats) * (d->nblk + 1));
+  for (int k = 0; k < 4 && !rc; k++) rc = d->alloc_plane_f(&d->f.mlx[k]);
+  if (!rc) rc = d->alloc_plane_f(&d->f.mly);
+  if (!rc) rc = d->alloc_plane_i(&d->f.mlstep);
+  if (!rc) {
+    const size_t P = sizeof(float) * kMlpParams;
+    rc = d->dev_alloc((void**)&d->mlp_w, P);
+    if (!rc) rc = d->dev_alloc((void**)&d->mlp_m, P);
+    if (!rc) rc = d->dev_alloc((void**)&d->mlp_v, P);
+    if (!rc) rc = d->dev_alloc((void**)&d->mlp_grad, P);
+    if (!rc) rc = d->dev_alloc((void**)&d->mlp_part, P * 64);
+    if (!rc) rc = d->dev_alloc((void**)&d->mlp_loss_part, sizeof(float) * 64);
+    if (!rc) rc = d->dev_alloc((void**)&d->mlp_loss, sizeof(float) * 4);
+    d->mlp_cap = std::max<int64_t>(d->npad, 65536);
+    if (!rc) rc = d->dev_alloc((void**)&d->mlp_X, sizeof(float) * 4 * d->mlp_cap);
+    if (!rc) rc = d->dev_alloc((void**)&d->mlp_Y, sizeof(float) * d->mlp_cap);
+    if (!rc) rc = d->dev_alloc((void**)&d->mlp_flags, sizeof(int) * d->npad);
+    if (!rc) rc = d->dev_alloc((void**)&d->mlp_scan, sizeof(int) * (d->npad + 1));
+    if (!rc) rc = d->dev_alloc((void**)&d->mlp_scan_part, sizeof(int) * (d->npad / 1024 + 4));
+    if (!rc) rc = d->dev_alloc((void**)&d->mlp_count, sizeof(int) * 4);
+  }
+  if (!rc && hipHostMalloc(&d->host_i, sizeof(int) * 16) != hipSuccess) rc = WF_E_NOMEM;
+  if (!rc && hipHostMalloc(&d->host_f, sizeof(float) * 16) != hipSuccess) rc = WF_E_NOMEM;
+  if (!rc) {
+    // Xavier-uniform weights from a fixed seed, zero biases, zero Adam moments
+    std::vector<float> w(kMlpParams, 0.f);
+    uint64_t seed = 0x5EEDF00DULL;
+    auto uni = [&](float lim) {
+      return (float)(((splitmix64(seed) >> 11) * (1.0 / 9007199254740992.0)) * 2.0 - 1.0) * lim;
+    };
+    for (int k = 0; k < 128; k++) w[kMlpOffW1 + k] = uni(std::sqrt(6.f / (4 + 32)));
+    for (int k = 0; k < 1024; k++) w[kMlpOffW2 + k] = uni(std::sqrt(6.f / (32 + 32)));
+    for (int k = 0; k < 32; k++) w[kMlpOffW3 + k] = uni(std::sqrt(6.f / (32 + 1)));
+    std::vector<float> z(kMlpParams, 0.f);
+    if (hipMemcpy(d->mlp_w, w.data(), sizeof(float) * kMlpParams, hipMemcpyHostToDevice) ||
+        hipMemcpy(d->mlp_m, z.data(), sizeof(float) * kMlpParams, hipMemcpyHostToDevice) ||
+        hipMemcpy(d->mlp_v, z.data(), sizeof(float) * kMlpParams, hipMemcpyHostToDevice))
+      rc = WF_E_HIP;
+    hipLaunchKernelGGL(k_fill_i32v, dim3(gridn(d->npad)), dim3(256), 0, d->stream, d->f.mlstep, -1,
+                       (int)d->npad);
   }
   if (!rc) rc = d->alloc_plane_d(&d->f.rho_alt);
   if (!rc) rc = d->alloc_plane_d(&d->f.T_alt);
@@ -425,6 +549,8 @@ int wf_destroy(wf_domain* d) {
   if (d->stream) hipStreamSynchronize(d->stream);
   for (void* p : d->owned) hipFree(p);
   if (d->ctr_host) hipHostFree(d->ctr_host);
+  if (d->host_i) hipHostFree(d->host_i);
+  if (d->host_f) hipHostFree(d->host_f);
   ns_free(d->ns);
   if (d->ev0) hipEventDestroy(d->ev0);
   if (d->ev1) hipEventDestroy(d->ev1);
@@ -570,6 +696,15 @@ int wf_nsearch_build(wf_domain* d) {
       std::swap(*r.i, d->spare_i);
     }
   }
+  for (int k = 0; k < 5; k++) {
+    float*& pl = k < 4 ? d->f.mlx[k] : d->f.mly;
+    float* tmp = (float*)d->spare_d;  // a double plane holds a float plane
+    hipLaunchKernelGGL(k_gather_f32, dim3(gridn(d->n)), dim3(256), 0, d->stream, tmp, pl,
+                       d->gather, d->n);
+    HIPC(hipMemcpyAsync(pl, tmp, sizeof(float) * d->n, hipMemcpyDeviceToDevice, d->stream));
+  }
+  launch_gather_i32(d->stream, d->spare_i, d->f.mlstep, d->gather, d->n);
+  std::swap(d->f.mlstep, d->spare_i);
   std::swap(d->perm, d->perm_tmp);
   HIPC(hipGetLastError());
   d->nb_built = true;
@@ -670,7 +805,13 @@ int wf_pressure(wf_domain* d) {
 }
 int wf_stress_strain(wf_domain* d) {
   if (!d) return fail(WF_E_ARG, "null domain");
-  launch_stress(d->stream, d->f, d->params(), d->ctr, false, true, false);
+  const StepParams p = d->params();
+  launch_stress(d->stream, d->f, p, d->bstats, false, true, false,
+                d->use_queue ? &d->rq : nullptr);
+  if (d->use_queue)
+    launch_return(d->stream, d->f, p, d->bstats, d->rq, d->mlp_w,
+                  d->cfg.surrogate && d->mlp_trained, false, d->step);
+  launch_fold_stats(d->stream, d->bstats, (p.solid + kBlock - 1) / kBlock, d->ctr);
   d->mirror_dirty = true;
   HIPC(hipGetLastError());
   return WF_OK;
@@ -783,8 +924,16 @@ int wf_step(wf_domain* d, int n_steps) {
       else launch_rates_tiled(d->stream, d->f, d->tl, d->solid, p.quirks);
       if ((rc = timer_end(d, WF_K_RATES))) return rc;
       timer_begin(d);
-      launch_stress(d->stream, d->f, p, d->ctr, true, true, mx);
+      launch_stress(d->stream, d->f, p, d->bstats, true, true, mx,
+                    d->use_queue ? &d->rq : nullptr);
       if ((rc = timer_end(d, WF_K_STRESS))) return rc;
+      if (d->use_queue) {
+        timer_begin(d);
+        launch_return(d->stream, d->f, p, d->bstats, d->rq, d->mlp_w,
+                      d->cfg.surrogate && d->mlp_trained, mx, d->step);
+        if ((rc = timer_end(d, WF_K_RETURN))) return rc;
+      }
+      launch_fold_stats(d->stream, d->bstats, (p.solid + kBlock - 1) / kBlock, d->ctr);
       timer_begin(d);
       if (mx) launch_accel_mixed(d->stream, d->f, d->tl, d->solid, p.alpha, p.beta, cs32);
       else launch_accel_tiled(d->stream, d->f, d->tl, d->solid, p.alpha, p.beta);
@@ -816,6 +965,13 @@ int wf_step(wf_domain* d, int n_steps) {
       std::swap(d->f.T, d->f.T_alt);
       if ((rc = timer_end(d, WF_K_THERMAL))) return rc;
     }
+    if (d->cfg.surrogate && d->use_queue && d->is_yielding && d->cfg.mlp_train_interval > 0 &&
+        (d->step + 1) % d->cfg.mlp_train_interval == 0) {
+      timer_begin(d);
+      rc = d->mlp_train_round(250, 2e-3f);
+      if (rc) return rc;
+      if ((rc = timer_end(d, WF_K_MLP))) return rc;
+    }
     d->isfirst_step = false;
     d->time += d->cfg.dt;
     d->step++;
@@ -840,6 +996,87 @@ int wf_get_stats(wf_domain* d, wf_stats* out) {
   out->newton_iters_wave = (int64_t)d->ctr_host->newton_iters_wave;
   out->mlp_trainings = d->mlp_trainings;
   out->nb_over_cap = d->ctr_host->nb_over_cap;
+  out->mlp_loss = d->mlp_last_loss;
+  out->mlp_samples = d->mlp_last_samples;
+  return WF_OK;
+}
+
+int wf_mlp_get_params(wf_domain* d, float* out, int n) {
+  if (!d || !out || n != kMlpParams) return fail(WF_E_ARG, "need WF_MLP_PARAMS floats");
+  HIPC(hipMemcpyAsync(out, d->mlp_w, sizeof(float) * n, hipMemcpyDeviceToHost, d->stream));
+  HIPC(hipStreamSynchronize(d->stream));
+  return WF_OK;
+}
+
+int wf_mlp_set_params(wf_domain* d, const float* in, int n) {
+  if (!d || !in || n != kMlpParams) return fail(WF_E_ARG, "need WF_MLP_PARAMS floats");
+  HIPC(hipMemcpyAsync(d->mlp_w, in, sizeof(float) * n, hipMemcpyHostToDevice, d->stream));
+  HIPC(hipStreamSynchronize(d->stream));
+  d->mlp_trained = true;
+  return WF_OK;
+}
+
+int wf_mlp_reset_adam(wf_domain* d) {
+  if (!d) return fail(WF_E_ARG, "null domain");
+  std::vector<float> z(kMlpParams, 0.f);
+  HIPC(hipMemcpy(d->mlp_m, z.data(), sizeof(float) * kMlpParams, hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(d->mlp_v, z.data(), sizeof(float) * kMlpParams, hipMemcpyHostToDevice));
+  d->adam_t = 0;
+  return WF_OK;
+}
+
+static int upload_samples(wf_domain* d, const float* X, const float* Y, int n) {
+  if (n <= 0 || n > d->mlp_cap) return fail(WF_E_ARG, "sample count out of range");
+  HIPC(hipMemcpyAsync(d->mlp_X, X, sizeof(float) * 4 * n, hipMemcpyHostToDevice, d->stream));
+  if (Y) HIPC(hipMemcpyAsync(d->mlp_Y, Y, sizeof(float) * n, hipMemcpyHostToDevice, d->stream));
+  return WF_OK;
+}
+
+int wf_mlp_infer(wf_domain* d, const float* X, int n, float* y) {
+  if (!d || !X || !y) return fail(WF_E_ARG, "null argument");
+  int rc = upload_samples(d, X, nullptr, n);
+  if (rc) return rc;
+  launch_mlp_infer(d->stream, d->mlp_w, d->mlp_X, n, d->mlp_Y);
+  HIPC(hipGetLastError());
+  HIPC(hipMemcpyAsync(y, d->mlp_Y, sizeof(float) * n, hipMemcpyDeviceToHost, d->stream));
+  HIPC(hipStreamSynchronize(d->stream));
+  return WF_OK;
+}
+
+int wf_mlp_grad(wf_domain* d, const float* X, const float* Y, int n, float* grad, float* loss) {
+  if (!d || !X || !Y || !grad) return fail(WF_E_ARG, "null argument");
+  int rc = upload_samples(d, X, Y, n);
+  if (rc) return rc;
+  const int nblocks = std::max(1, std::min(64, (n + 255) / 256));
+  launch_mlp_grad(d->stream, d->mlp_w, d->mlp_X, d->mlp_Y, n, 1, d->mlp_part, d->mlp_loss_part,
+                  nblocks, d->mlp_grad, d->mlp_loss);
+  HIPC(hipGetLastError());
+  HIPC(hipMemcpyAsync(grad, d->mlp_grad, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost,
+                      d->stream));
+  HIPC(hipMemcpyAsync(d->host_f, d->mlp_loss, sizeof(float), hipMemcpyDeviceToHost, d->stream));
+  HIPC(hipStreamSynchronize(d->stream));
+  if (loss) *loss = d->host_f[0];
+  return WF_OK;
+}
+
+int wf_mlp_fit(wf_domain* d, const float* X, const float* Y, int n, int iters, float lr,
+               float* loss) {
+  if (!d || !X || !Y || iters < 0) return fail(WF_E_ARG, "bad argument");
+  int rc = upload_samples(d, X, Y, n);
+  if (rc) return rc;
+  rc = d->mlp_fit(d->mlp_X, d->mlp_Y, n, 1, iters, lr);
+  if (rc) return rc;
+  d->mlp_trained = true;
+  if (loss) *loss = (float)d->mlp_last_loss;
+  return WF_OK;
+}
+
+int wf_mlp_train(wf_domain* d, int iters, float lr, int64_t* samples, float* loss) {
+  if (!d || iters < 0) return fail(WF_E_ARG, "bad argument");
+  int rc = d->mlp_train_round(iters, lr);
+  if (rc) return rc;
+  if (samples) *samples = d->mlp_last_samples;
+  if (loss) *loss = (float)d->mlp_last_loss;
   return WF_OK;
 }
 

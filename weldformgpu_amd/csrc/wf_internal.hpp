@@ -7,6 +7,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "../../include/wf_abi.h"
@@ -33,7 +34,34 @@ struct Fields {
   float *xr[3], *v32[3], *h32, *m32, *mdj32, *mdj32_alt, *rho32, *cs32, *sr32[6];
   float *T32, *T32_alt, *k32;
   int* cell;
+  // return-map surrogate training samples, one slot per particle: features
+  // (s_trial/A, 10 eps_p, log10(max(rate,1e-3))/3, theta), target dg/dg_max,
+  // step of the sample (-1: none)
+  float *mlx[4], *mly;
+  int* mlstep;
 };
+
+// Queue of yielding Johnson-Cook particles whose radial return is solved by
+// k_return (wf_return.hip): particle index, trial von Mises stress, rate x
+// thermal factor of the flow stress, and the bracket end dg_max.  The queue
+// is segmented by stress-kernel block: block b owns slots [b*kBlock,
+// b*kBlock + count[b]), compacted in particle order (no atomics).
+struct RetQueue {
+  int* idx;
+  double *strial, *rt, *gmax;
+  int* count;  // per block
+};
+
+// per-block statistics of the stress/return kernels, folded into Counters
+// once per step: max eps_p bits, Newton iterations, Newton calls, sum over
+// wavefronts of the wave's largest iteration count
+struct BlockStats {
+  unsigned long long ep_bits, iters, calls, wave_iters;
+};
+
+constexpr int kMlpParams = 1249;  // 4x32 + 32 + 32x32 + 32 + 32 + 1
+constexpr int kMlpOffW1 = 0, kMlpOffB1 = 128, kMlpOffW2 = 160, kMlpOffB2 = 1184,
+              kMlpOffW3 = 1216, kMlpOffB3 = 1248;
 
 // frame of the FP32 cell-relative coordinates
 struct Frame32 {
@@ -110,8 +138,25 @@ void launch_mirror32(hipStream_t s, const Fields& f, int n, const Frame32& fr);
 // ---- per-particle kernels (wf_kernels.hip) --------------------------------
 void launch_update_density(hipStream_t s, const Fields& f, const StepParams& p, double dt);
 void launch_pressure(hipStream_t s, const Fields& f, const StepParams& p);
-void launch_stress(hipStream_t s, const Fields& f, const StepParams& p, Counters* ctr,
-                   bool fuse_pressure, bool energy, bool mirror32);
+void launch_stress(hipStream_t s, const Fields& f, const StepParams& p, BlockStats* bs,
+                   bool fuse_pressure, bool energy, bool mirror32, const RetQueue* q);
+// fold the per-block statistics of the last stress (+ return) pass into ctr
+void launch_fold_stats(hipStream_t s, const BlockStats* bs, int nblocks, Counters* ctr);
+// ---- return map and surrogate (wf_return.hip, wf_mlp.hip) -----------------
+void launch_return(hipStream_t s, const Fields& f, const StepParams& p, BlockStats* bs,
+                   const RetQueue& q, const float* mlp_w, bool use_mlp, bool mirror32,
+                   int64_t step);
+// MFMA inference of the surrogate on given feature rows (test hook)
+void launch_mlp_infer(hipStream_t s, const float* w, const float* X, int n, float* y);
+// one gradient of the mean squared error over n samples (deterministic)
+void launch_mlp_grad(hipStream_t s, const float* w, const float* X, const float* Y, int n,
+                     int stride, float* part, float* loss_part, int nblocks, float* grad,
+                     float* loss);
+void launch_mlp_adam(hipStream_t s, float* w, float* m, float* v, const float* grad, int t,
+                     float lr);
+// gather the logged samples (ascending particle index) into X (n x 4), Y
+void launch_mlp_collect(hipStream_t s, const Fields& f, int n, float* X, float* Y, int* flags,
+                        int* scan, int* scan_part, int* count);
 void launch_apply_bc(hipStream_t s, const Fields& f, int n, int zone, double vx, double vy,
                      double vz);
 void launch_update_pos(hipStream_t s, const Fields& f, int n, double dt);
@@ -137,6 +182,9 @@ int ns_build(hipStream_t s, NsWork** work, const double* const pos_src[3], int n
              double radius, int* gather, int* count, int* list, Counters* ctr, int* over_cap,
              int* perm, int* perm_tmp, TileLists* tiles, GridDesc* grid_out);
 void ns_free(NsWork* w);
+// exclusive scan of m ints into out[0..m] (out[m] = total); part needs
+// ceil(m/1024)+1 ints
+void launch_scan_exclusive(hipStream_t s, const int* in, int64_t m, int* part, int* out);
 
 // generic gathers used to reorder fields: dst[i] = src[idx[i]]
 void launch_gather_f64(hipStream_t s, double* dst, const double* src, const int* idx, int n);

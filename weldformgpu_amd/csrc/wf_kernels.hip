@@ -85,12 +85,18 @@ __device__ double jc_newton(const MatParams& mp, double s_trial, double G, doubl
 
 // ---------------------------------------------------------------------------
 // C: Jaumann stress update + return map, StressStrainOne (Mechanical.cu:458-554),
-// optionally with the pressure (FUSE_P) and the per-particle internal-energy
-// increment (ENERGY, Energy.cu:5-19 with per-particle indexing).
+// optionally with the pressure (FUSE_P), the per-particle internal-energy
+// increment (ENERGY, Energy.cu:5-19 with per-particle indexing) and the FP32
+// sigma/rho^2 mirror (MIR).  With a queue, yielding Johnson-Cook particles
+// under the Newton return map keep their trial state and are appended to the
+// queue; k_return finishes them.
 template <bool FUSE_P, bool ENERGY, bool MIR>
-__global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Counters* ctr) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool active = i < p.solid;
+__global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, BlockStats* bs,
+                                                   RetQueue q, int use_queue) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = i < p.solid;
+  bool queued = false;
+  double q_strial = 0.0, q_rt = 0.0, q_gmax = 0.0;
   unsigned long long ep_bits = 0ull, nit = 0ull, ncall = 0ull;
   if (active) {
     const MatParams& mp = p.mat;
@@ -123,14 +129,14 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Count
     S.yz = (D.yz * g2 + SRT.yz + RS.yz) * dt + S.yz;
     S.zy = (D.zy * g2 + SRT.zy + RS.zy) * dt + S.zy;
 
-    double esr = sqrt(0.5 * ((D.xx - D.yy) * (D.xx - D.yy) + (D.yy - D.zz) * (D.yy - D.zz) +
-                             (D.zz - D.xx) * (D.zz - D.xx)) +
-                      3.0 * (D.xy * D.xy + D.yz * D.yz + D.zx * D.zx));
+    const double esr = sqrt(0.5 * ((D.xx - D.yy) * (D.xx - D.yy) + (D.yy - D.zz) * (D.yy - D.zz) +
+                                   (D.zz - D.xx) * (D.zz - D.xx)) +
+                            3.0 * (D.xy * D.xy + D.yz * D.yz + D.zx * D.zx));
     f.esr[i] = esr;
-    double J2 = 0.5 * (S.xx * S.xx + 2.0 * S.xy * S.yx + 2.0 * S.xz * S.zx + S.yy * S.yy +
-                       2.0 * S.yz * S.zy + S.zz * S.zz);
-    double sig_trial = sqrt(3.0 * J2);
-    double T = f.T ? f.T[i] : 0.0;
+    const double J2 = 0.5 * (S.xx * S.xx + 2.0 * S.xy * S.yx + 2.0 * S.xz * S.zx +
+                             S.yy * S.yy + 2.0 * S.yz * S.zy + S.zz * S.zz);
+    const double sig_trial = sqrt(3.0 * J2);
+    const double T = f.T[i];
     double sigy = f.sigma_y[i];
     double ep = f.pl_strain[i];
     double sigeq = sig_trial;
@@ -156,10 +162,15 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Count
         sigy = jc_yield(mp, ep, rate_f, therm_f);
       }
       if (sig_trial > sigy) {
-        double g;
+        double g = 0.0;
         if (mp.model == WF_MAT_BILINEAR) {
           g = (sig_trial - sigy) / (3.0 * G + mp.Ep);
           sigy = sigy + mp.Ep * g;
+        } else if (mp.model == WF_MAT_JOHNSON_COOK && use_queue) {
+          queued = true;
+          q_strial = sig_trial;
+          q_rt = rate_f * therm_f;
+          q_gmax = (sig_trial - sigy) / (3.0 * G);
         } else if (mp.model == WF_MAT_JOHNSON_COOK) {
           int it = 0;
           g = jc_newton(mp, sig_trial, G, ep, rate_f, therm_f, 0.0, p.newton_tol,
@@ -178,61 +189,123 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Count
           g = 0.5 * (lo + hi);
           sigy = hollomon_yield(mp, ep + g);
         }
-        scale = (sig_trial - 3.0 * G * g) / sig_trial;
-        dep = g;
-        ep += g;
-        sigeq = sig_trial - 3.0 * G * g;
+        if (!queued) {
+          scale = (sig_trial - 3.0 * G * g) / sig_trial;
+          dep = g;
+          ep += g;
+          sigeq = sig_trial - 3.0 * G * g;
+        }
       }
     }
     if (scale != 1.0) {
       S.xx *= scale; S.yy *= scale; S.zz *= scale;
       S.xy *= scale; S.yz *= scale; S.xz *= scale;
     }
-    f.sigma_y[i] = sigy;
-    f.pl_strain[i] = ep;
-    f.sigma_eq[i] = sigeq;
-    if (f.qpl) f.qpl[i] = p.taylor_quinney * sigeq * dep / dt;
-    const double pi = f.p[i];
-    // sigma = -p I + S (Mechanical.cu:540); store upper triangle
-    f.sigma[0][i] = -pi + S.xx;
-    f.sigma[1][i] = -pi + S.yy;
-    f.sigma[2][i] = -pi + S.zz;
-    f.sigma[3][i] = S.xy;
-    f.sigma[4][i] = S.yz;
-    f.sigma[5][i] = S.xz;
-    if (MIR) {  // sigma/rho^2 for the FP32 acceleration loop
-      const double ir2 = 1.0 / (f.rho[i] * f.rho[i]);
-      f.sr32[0][i] = (float)((-pi + S.xx) * ir2);
-      f.sr32[1][i] = (float)((-pi + S.yy) * ir2);
-      f.sr32[2][i] = (float)((-pi + S.zz) * ir2);
-      f.sr32[3][i] = (float)(S.xy * ir2);
-      f.sr32[4][i] = (float)(S.yz * ir2);
-      f.sr32[5][i] = (float)(S.xz * ir2);
-    }
     f.ss[0][i] = S.xx; f.ss[1][i] = S.yy; f.ss[2][i] = S.zz;
     f.ss[3][i] = S.xy; f.ss[4][i] = S.yz; f.ss[5][i] = S.xz;
     f.strain[0][i] += D.xx * dt; f.strain[1][i] += D.yy * dt; f.strain[2][i] += D.zz * dt;
     f.strain[3][i] += D.xy * dt; f.strain[4][i] += D.yz * dt; f.strain[5][i] += D.xz * dt;
-    if (ENERGY) {
-      double de = f.m[i] / f.rho[i] *
-                  (S.xx * D.xx + S.yy * D.yy + S.zz * D.zz +
-                   2.0 * (S.xy * D.xy + S.yz * D.yz + S.xz * D.xz));
-      f.eint[i] += de * dt;
+    if (!queued) {
+      f.sigma_y[i] = sigy;
+      f.pl_strain[i] = ep;
+      f.sigma_eq[i] = sigeq;
+      f.qpl[i] = p.taylor_quinney * sigeq * dep / dt;
+      const double pi = f.p[i];
+      // sigma = -p I + S (Mechanical.cu:540); store upper triangle
+      f.sigma[0][i] = -pi + S.xx;
+      f.sigma[1][i] = -pi + S.yy;
+      f.sigma[2][i] = -pi + S.zz;
+      f.sigma[3][i] = S.xy;
+      f.sigma[4][i] = S.yz;
+      f.sigma[5][i] = S.xz;
+      if (MIR) {  // sigma/rho^2 for the FP32 acceleration loop
+        const double ir2 = 1.0 / (f.rho[i] * f.rho[i]);
+        f.sr32[0][i] = (float)((-pi + S.xx) * ir2);
+        f.sr32[1][i] = (float)((-pi + S.yy) * ir2);
+        f.sr32[2][i] = (float)((-pi + S.zz) * ir2);
+        f.sr32[3][i] = (float)(S.xy * ir2);
+        f.sr32[4][i] = (float)(S.yz * ir2);
+        f.sr32[5][i] = (float)(S.xz * ir2);
+      }
+      if (ENERGY) {
+        double de = f.m[i] / f.rho[i] *
+                    (S.xx * D.xx + S.yy * D.yy + S.zz * D.zz +
+                     2.0 * (S.xy * D.xy + S.yz * D.yz + S.xz * D.xz));
+        f.eint[i] += de * dt;
+      }
+      ep_bits = (unsigned long long)__double_as_longlong(ep > 0.0 ? ep : 0.0);
     }
-    ep_bits = (unsigned long long)__double_as_longlong(ep > 0.0 ? ep : 0.0);
   }
-  // wavefront reductions, one atomic per wave
-  unsigned long long mx = wave_max_u64(ep_bits);
-  unsigned long long sit = wave_sum_u64(nit);
-  unsigned long long sc = wave_sum_u64(ncall);
-  unsigned long long wmax = wave_max_u64(nit);
-  if (lane_id() == 0) {
-    if (mx) atomicMax(&ctr->max_pl_strain_bits, mx);
-    if (sc) {
-      atomicAdd(&ctr->newton_iters, sit);
-      atomicAdd(&ctr->newton_calls, sc);
-      atomicAdd(&ctr->newton_iters_wave, wmax);
+  // block-segmented queue append (particle order, no atomics) and block stats
+  __shared__ int wcnt[kBlock / 64];
+  __shared__ unsigned long long wst[kBlock / 64][4];
+  const int lane = lane_id(), w = threadIdx.x >> 6;
+  const unsigned long long bal = __ballot(queued);
+  const unsigned long long mx = wave_max_u64(ep_bits);
+  const unsigned long long sit = wave_sum_u64(nit);
+  const unsigned long long sc = wave_sum_u64(ncall);
+  const unsigned long long wmax = wave_max_u64(nit);
+  if (lane == 0) {
+    wcnt[w] = __popcll(bal);
+    wst[w][0] = mx;
+    wst[w][1] = sit;
+    wst[w][2] = sc;
+    wst[w][3] = wmax;
+  }
+  __syncthreads();
+  int base = 0;
+  for (int u = 0; u < w; u++) base += wcnt[u];
+  if (queued) {
+    const int slot = blockIdx.x * kBlock + base + __popcll(bal & ((1ull << lane) - 1ull));
+    q.idx[slot] = i;
+    q.strial[slot] = q_strial;
+    q.rt[slot] = q_rt;
+    q.gmax[slot] = q_gmax;
+  }
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    BlockStats b{0ull, 0ull, 0ull, 0ull};
+    for (int u = 0; u < kBlock / 64; u++) {
+      tot += wcnt[u];
+      b.ep_bits = wst[u][0] > b.ep_bits ? wst[u][0] : b.ep_bits;
+      b.iters += wst[u][1];
+      b.calls += wst[u][2];
+      b.wave_iters += wst[u][3];
     }
+    if (use_queue) q.count[blockIdx.x] = tot;
+    bs[blockIdx.x] = b;
+  }
+}
+
+// one block folds the per-block statistics into the run counters
+__global__ __launch_bounds__(256) void k_fold_stats(const BlockStats* bs, int nb, Counters* ctr) {
+  __shared__ unsigned long long sh[4][256];
+  unsigned long long m = 0ull, it = 0ull, ca = 0ull, wi = 0ull;
+  for (int b = threadIdx.x; b < nb; b += 256) {
+    const BlockStats x = bs[b];
+    m = x.ep_bits > m ? x.ep_bits : m;
+    it += x.iters;
+    ca += x.calls;
+    wi += x.wave_iters;
+  }
+  sh[0][threadIdx.x] = m;
+  sh[1][threadIdx.x] = it;
+  sh[2][threadIdx.x] = ca;
+  sh[3][threadIdx.x] = wi;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) {
+      sh[0][threadIdx.x] = sh[0][threadIdx.x + st] > sh[0][threadIdx.x] ? sh[0][threadIdx.x + st]
+                                                                        : sh[0][threadIdx.x];
+      for (int k = 1; k < 4; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (sh[0][0] > ctr->max_pl_strain_bits) ctr->max_pl_strain_bits = sh[0][0];
+    ctr->newton_iters += sh[1][0];
+    ctr->newton_calls += sh[2][0];
+    ctr->newton_iters_wave += sh[3][0];
   }
 }
 
@@ -427,20 +500,26 @@ void launch_pressure(hipStream_t s, const Fields& f, const StepParams& p) {
   if (p.n <= 0) return;
   hipLaunchKernelGGL(k_pressure, dim3(grid_for(p.n)), dim3(kBlock), 0, s, f, p.n);
 }
-void launch_stress(hipStream_t s, const Fields& f, const StepParams& p, Counters* ctr,
-                   bool fuse_pressure, bool energy, bool mirror32) {
+void launch_stress(hipStream_t s, const Fields& f, const StepParams& p, BlockStats* bs,
+                   bool fuse_pressure, bool energy, bool mirror32, const RetQueue* q) {
   if (p.solid <= 0) return;
   dim3 g(grid_for(p.solid)), b(kBlock);
+  RetQueue qq = q ? *q : RetQueue{};
+  int uq = q ? 1 : 0;
   if (mirror32)
-    hipLaunchKernelGGL((k_stress<true, true, true>), g, b, 0, s, f, p, ctr);
+    hipLaunchKernelGGL((k_stress<true, true, true>), g, b, 0, s, f, p, bs, qq, uq);
   else if (fuse_pressure && energy)
-    hipLaunchKernelGGL((k_stress<true, true, false>), g, b, 0, s, f, p, ctr);
+    hipLaunchKernelGGL((k_stress<true, true, false>), g, b, 0, s, f, p, bs, qq, uq);
   else if (fuse_pressure)
-    hipLaunchKernelGGL((k_stress<true, false, false>), g, b, 0, s, f, p, ctr);
+    hipLaunchKernelGGL((k_stress<true, false, false>), g, b, 0, s, f, p, bs, qq, uq);
   else if (energy)
-    hipLaunchKernelGGL((k_stress<false, true, false>), g, b, 0, s, f, p, ctr);
+    hipLaunchKernelGGL((k_stress<false, true, false>), g, b, 0, s, f, p, bs, qq, uq);
   else
-    hipLaunchKernelGGL((k_stress<false, false, false>), g, b, 0, s, f, p, ctr);
+    hipLaunchKernelGGL((k_stress<false, false, false>), g, b, 0, s, f, p, bs, qq, uq);
+}
+void launch_fold_stats(hipStream_t s, const BlockStats* bs, int nblocks, Counters* ctr) {
+  if (nblocks <= 0) return;
+  hipLaunchKernelGGL(k_fold_stats, dim3(1), dim3(256), 0, s, bs, nblocks, ctr);
 }
 void launch_apply_bc(hipStream_t s, const Fields& f, int n, int zone, double vx, double vy,
                      double vz) {

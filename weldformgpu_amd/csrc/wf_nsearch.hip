@@ -219,6 +219,14 @@ __global__ void k_ns_gather_f32(float* dst, const float* src, const int* idx, in
 
 static int grid1(int64_t n, int b) { return (int)((n + b - 1) / b); }
 
+void launch_scan_exclusive(hipStream_t s, const int* in, int64_t m, int* part, int* out) {
+  const int nb = grid1(m, kScanBlock);
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanBlock), 0, s, in, m, part);
+  hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(64), 0, s, part, nb);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kScanBlock), 0, s, in, m, part, out);
+}
+
 static int ensure(NsWork* w, int n, int64_t nkeys) {
   if (n > w->n_cap) {
     for (int c = 0; c < 3; c++) {

@@ -158,3 +158,45 @@ class Domain:
         ms, n = C.c_double(), C.c_int64()
         _check(self.lib, self.lib.wf_kernel_time(self.h, k, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    # -- return-map surrogate ------------------------------------------------
+    def mlp_params(self):
+        w = np.zeros(_abi.WF_MLP_PARAMS, np.float32)
+        _check(self.lib, self.lib.wf_mlp_get_params(self.h, w.ctypes.data, w.size))
+        return w
+
+    def set_mlp_params(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        _check(self.lib, self.lib.wf_mlp_set_params(self.h, w.ctypes.data, w.size))
+
+    def mlp_reset_adam(self):
+        _check(self.lib, self.lib.wf_mlp_reset_adam(self.h))
+
+    def mlp_infer(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.zeros(len(X), np.float32)
+        _check(self.lib, self.lib.wf_mlp_infer(self.h, X.ctypes.data, len(X), y.ctypes.data))
+        return y
+
+    def mlp_grad(self, X, Y):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        g = np.zeros(_abi.WF_MLP_PARAMS, np.float32)
+        loss = C.c_float()
+        _check(self.lib, self.lib.wf_mlp_grad(self.h, X.ctypes.data, Y.ctypes.data, len(X),
+                                              g.ctypes.data, C.byref(loss)))
+        return g, loss.value
+
+    def mlp_fit(self, X, Y, iters, lr):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        loss = C.c_float()
+        _check(self.lib, self.lib.wf_mlp_fit(self.h, X.ctypes.data, Y.ctypes.data, len(X),
+                                             int(iters), float(lr), C.byref(loss)))
+        return loss.value
+
+    def mlp_train(self, iters=250, lr=2e-3):
+        ns, loss = C.c_int64(), C.c_float()
+        _check(self.lib, self.lib.wf_mlp_train(self.h, int(iters), float(lr), C.byref(ns),
+                                               C.byref(loss)))
+        return ns.value, loss.value
