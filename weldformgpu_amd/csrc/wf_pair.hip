@@ -2,7 +2,7 @@
 //
 // One workgroup per tile of kTileP Morton-consecutive particles, one thread
 // per particle.  The tile's staged set (the particles of every occupied cell
-// adjacent to the tile) streams through LDS in chunks of kTileChunk with
+// adjacent to the tile) streams through LDS in chunks (kChunk64 / kChunk32) with
 // coalesced global loads; each thread walks its neighbour list, whose
 // entries are staged positions in ascending order, so each chunk consumes a
 // prefix of the remaining list.  The per-pair physics is the reference's
@@ -59,7 +59,8 @@ __device__ __forceinline__ int xcd_tile(int b, int nt) {
 template <class K>
 __global__ __launch_bounds__(kTileP) void k_pair_tiled(Fields f, TileLists tl,
                                                        typename K::Prm prm) {
-  __shared__ typename K::LT lds[K::NF][kTileChunk];
+  constexpr int CH = K::CHUNK;
+  __shared__ typename K::LT lds[K::NF][CH];
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
   const int i = t * kTileP + tid;
@@ -71,26 +72,43 @@ __global__ __launch_bounds__(kTileP) void k_pair_tiled(Fields f, TileLists tl,
   const int ub = tl.uoff[t], ucnt = tl.uoff[t + 1] - ub;
   SlotReader sr;
   sr.init(tl, i, act ? tl.count[i] : 0);
-  for (int c0 = 0; c0 < ucnt; c0 += kTileChunk) {
-    const int m = min(kTileChunk, ucnt - c0);
+  for (int c0 = 0; c0 < ucnt; c0 += CH) {
+    const int m = min(CH, ucnt - c0);
     __syncthreads();
     for (int s = tid; s < m; s += kTileP) K::stage(f, tl.tj[ub + c0 + s], lds, s, cx);
     __syncthreads();
     const int lim = c0 + m;
+    // with K::UNROLL, two neighbours per iteration while both fall in this
+    // chunk, so the LDS reads of the second overlap the arithmetic of the first
     while (sr.k < sr.cnt) {
-      const int s = sr.slot();
-      if (s >= lim) break;
-      K::pair(lds, s - c0, own, acc, prm);
+      const int s0 = sr.slot();
+      if (s0 >= lim) break;
       sr.advance();
+      if (K::UNROLL && sr.k < sr.cnt) {
+        const int s1 = sr.slot();
+        if (s1 < lim) {
+          sr.advance();
+          K::pair(lds, s0 - c0, own, acc, prm);
+          K::pair(lds, s1 - c0, own, acc, prm);
+          continue;
+        }
+      }
+      K::pair(lds, s0 - c0, own, acc, prm);
     }
   }
   K::finish(f, i, act, own, acc, prm);
 }
 
+// staged particles per LDS chunk: FP64 and FP32 policies
+constexpr int kChunk64 = 256;
+constexpr int kChunk32 = 512;
+
 // FP64 policies need no per-tile context
 struct NoCtx {};
 #define WF_FP64_POLICY                                                           \
   typedef double LT;                                                             \
+  static constexpr int CHUNK = kChunk64;                                         \
+  static constexpr bool UNROLL = false;                                          \
   typedef NoCtx Ctx;                                                             \
   __device__ static Ctx ctx(const Fields&, int, const Prm&) { return Ctx{}; }
 
@@ -160,13 +178,13 @@ struct DensK {
     o.rho = f.rho[i];
     o.hc = hconst(f.h[i]);
   }
-  __device__ static void stage(const Fields& f, int j, double (*L)[kTileChunk], int s,
+  __device__ static void stage(const Fields& f, int j, double (*L)[kChunk64], int s,
                                const Ctx&) {
     L[0][s] = f.x[0][j]; L[1][s] = f.x[1][j]; L[2][s] = f.x[2][j];
     L[3][s] = f.v[0][j]; L[4][s] = f.v[1][j]; L[5][s] = f.v[2][j];
     L[6][s] = f.h[j]; L[7][s] = f.m[j] / f.rho[j];
   }
-  __device__ static void pair(double (*L)[kTileChunk], int s, const Own& o, Acc& a, const Prm&) {
+  __device__ static void pair(double (*L)[kChunk64], int s, const Own& o, Acc& a, const Prm&) {
     const double xx = o.x - L[0][s], xy = o.y - L[1][s], xz = o.z - L[2][s];
     const double r2 = xx * xx + xy * xy + xz * xz;
     double rinv, h_;
@@ -211,11 +229,11 @@ struct RatesK {
     o.vx = f.v[0][i]; o.vy = f.v[1][i]; o.vz = f.v[2][i];
     o.hc = hconst(f.h[i]);
   }
-  __device__ static void stage(const Fields& f, int j, double (*L)[kTileChunk], int s,
+  __device__ static void stage(const Fields& f, int j, double (*L)[kChunk64], int s,
                                const Ctx&) {
     DensK::stage(f, j, L, s, NoCtx{});
   }
-  __device__ static void pair(double (*L)[kTileChunk], int s, const Own& o, Acc& a, const Prm&) {
+  __device__ static void pair(double (*L)[kChunk64], int s, const Own& o, Acc& a, const Prm&) {
     const double xx = o.x - L[0][s], xy = o.y - L[1][s], xz = o.z - L[2][s];
     const double r2 = xx * xx + xy * xy + xz * xz;
     double rinv, h_;
@@ -274,7 +292,7 @@ struct AccelK {
     o.s0 = f.sigma[0][i] * idi; o.s1 = f.sigma[1][i] * idi; o.s2 = f.sigma[2][i] * idi;
     o.s3 = f.sigma[3][i] * idi; o.s4 = f.sigma[4][i] * idi; o.s5 = f.sigma[5][i] * idi;
   }
-  __device__ static void stage(const Fields& f, int j, double (*L)[kTileChunk], int s,
+  __device__ static void stage(const Fields& f, int j, double (*L)[kChunk64], int s,
                                const Ctx&) {
     L[0][s] = f.x[0][j]; L[1][s] = f.x[1][j]; L[2][s] = f.x[2][j];
     L[3][s] = f.v[0][j]; L[4][s] = f.v[1][j]; L[5][s] = f.v[2][j];
@@ -285,7 +303,7 @@ struct AccelK {
     L[12][s] = f.sigma[2][j] * idj; L[13][s] = f.sigma[3][j] * idj;
     L[14][s] = f.sigma[4][j] * idj; L[15][s] = f.sigma[5][j] * idj;
   }
-  __device__ static void pair(double (*L)[kTileChunk], int s, const Own& o, Acc& a,
+  __device__ static void pair(double (*L)[kChunk64], int s, const Own& o, Acc& a,
                               const Prm& p) {
     const double xx = o.x - L[0][s], xy = o.y - L[1][s], xz = o.z - L[2][s];
     const double r2 = xx * xx + xy * xy + xz * xz;
@@ -344,12 +362,12 @@ struct CondK {
     o.k = f.k[i]; o.T = f.T[i];
     o.hc = hconst(f.h[i]);
   }
-  __device__ static void stage(const Fields& f, int j, double (*L)[kTileChunk], int s,
+  __device__ static void stage(const Fields& f, int j, double (*L)[kChunk64], int s,
                                const Ctx&) {
     L[0][s] = f.x[0][j]; L[1][s] = f.x[1][j]; L[2][s] = f.x[2][j];
     L[3][s] = f.h[j]; L[4][s] = f.m[j] / f.rho[j]; L[5][s] = f.T[j]; L[6][s] = f.k[j];
   }
-  __device__ static void pair(double (*L)[kTileChunk], int s, const Own& o, Acc& a, const Prm&) {
+  __device__ static void pair(double (*L)[kChunk64], int s, const Own& o, Acc& a, const Prm&) {
     const double xx = o.x - L[0][s], xy = o.y - L[1][s], xz = o.z - L[2][s];
     const double r2 = xx * xx + xy * xy + xz * xz;
     double rinv, h_;
@@ -395,17 +413,19 @@ struct CondK {
 // c an integer grid cell), compensated (Kahan-Neumaier) FP32 sums and FP64
 // results.  Per-particle terms stay FP64.
 
-// compensated FP32 accumulator
+// compensated FP32 accumulator (Kahan 1965): c carries the low-order part
+// lost by each addition
 struct KSum {
   float s, c;
   __device__ __forceinline__ void zero() { s = 0.f; c = 0.f; }
   __device__ __forceinline__ void add(float y) {
 #pragma clang fp contract(off)
-    const float t = s + y;
-    c += (fabsf(s) >= fabsf(y)) ? ((s - t) + y) : ((y - t) + s);
+    const float yc = y - c;
+    const float t = s + yc;
+    c = (t - s) - yc;
     s = t;
   }
-  __device__ __forceinline__ double val() const { return (double)s + (double)c; }
+  __device__ __forceinline__ double val() const { return (double)s - (double)c; }
 };
 
 struct HConstF {
@@ -452,6 +472,8 @@ __device__ __forceinline__ void pos32(const Fields& f, int j, const Ctx32& c, fl
 }
 
 struct DensM {
+  static constexpr int CHUNK = kChunk32;
+  static constexpr bool UNROLL = true;
   static constexpr int NF = 8;  // x y z vx vy vz h m/rho
   typedef float LT;
   typedef Ctx32 Ctx;
@@ -474,13 +496,13 @@ struct DensM {
     o.vx = f.v32[0][i]; o.vy = f.v32[1][i]; o.vz = f.v32[2][i];
     o.hc = hconst_f(f.h32[i]);
   }
-  __device__ static void stage(const Fields& f, int j, float (*L)[kTileChunk], int s,
+  __device__ static void stage(const Fields& f, int j, float (*L)[CHUNK], int s,
                                const Ctx& c) {
     pos32(f, j, c, L[0][s], L[1][s], L[2][s]);
     L[3][s] = f.v32[0][j]; L[4][s] = f.v32[1][j]; L[5][s] = f.v32[2][j];
     L[6][s] = f.h32[j]; L[7][s] = f.mdj32[j];
   }
-  __device__ static void pair(float (*L)[kTileChunk], int s, const Own& o, Acc& a, const Prm&) {
+  __device__ static void pair(float (*L)[CHUNK], int s, const Own& o, Acc& a, const Prm&) {
     const float xx = o.x - L[0][s], xy = o.y - L[1][s], xz = o.z - L[2][s];
     const float r2 = xx * xx + xy * xy + xz * xz;
     float h_;
@@ -508,6 +530,8 @@ struct DensM {
 };
 
 struct RatesM {
+  static constexpr int CHUNK = kChunk32;
+  static constexpr bool UNROLL = true;
   static constexpr int NF = 8;
   typedef float LT;
   typedef Ctx32 Ctx;
@@ -532,11 +556,11 @@ struct RatesM {
     o.vx = f.v32[0][i]; o.vy = f.v32[1][i]; o.vz = f.v32[2][i];
     o.hc = hconst_f(f.h32[i]);
   }
-  __device__ static void stage(const Fields& f, int j, float (*L)[kTileChunk], int s,
+  __device__ static void stage(const Fields& f, int j, float (*L)[CHUNK], int s,
                                const Ctx& c) {
     DensM::stage(f, j, L, s, c);
   }
-  __device__ static void pair(float (*L)[kTileChunk], int s, const Own& o, Acc& a, const Prm&) {
+  __device__ static void pair(float (*L)[CHUNK], int s, const Own& o, Acc& a, const Prm&) {
     const float xx = o.x - L[0][s], xy = o.y - L[1][s], xz = o.z - L[2][s];
     const float r2 = xx * xx + xy * xy + xz * xz;
     float h_;
@@ -567,6 +591,8 @@ struct RatesM {
 };
 
 struct AccelM {
+  static constexpr int CHUNK = kChunk64;
+  static constexpr bool UNROLL = false;
   static constexpr int NF = 16;  // x y z vx vy vz h m rho Cs (sigma/rho^2)[6]
   typedef float LT;
   typedef Ctx32 Ctx;
@@ -593,7 +619,7 @@ struct AccelM {
     o.s0 = f.sr32[0][i]; o.s1 = f.sr32[1][i]; o.s2 = f.sr32[2][i];
     o.s3 = f.sr32[3][i]; o.s4 = f.sr32[4][i]; o.s5 = f.sr32[5][i];
   }
-  __device__ static void stage(const Fields& f, int j, float (*L)[kTileChunk], int s,
+  __device__ static void stage(const Fields& f, int j, float (*L)[CHUNK], int s,
                                const Ctx& c) {
     pos32(f, j, c, L[0][s], L[1][s], L[2][s]);
     L[3][s] = f.v32[0][j]; L[4][s] = f.v32[1][j]; L[5][s] = f.v32[2][j];
@@ -601,7 +627,7 @@ struct AccelM {
     L[10][s] = f.sr32[0][j]; L[11][s] = f.sr32[1][j]; L[12][s] = f.sr32[2][j];
     L[13][s] = f.sr32[3][j]; L[14][s] = f.sr32[4][j]; L[15][s] = f.sr32[5][j];
   }
-  __device__ static void pair(float (*L)[kTileChunk], int s, const Own& o, Acc& a,
+  __device__ static void pair(float (*L)[CHUNK], int s, const Own& o, Acc& a,
                               const Prm& p) {
     const float xx = o.x - L[0][s], xy = o.y - L[1][s], xz = o.z - L[2][s];
     const float r2 = xx * xx + xy * xy + xz * xz;
@@ -635,6 +661,8 @@ struct AccelM {
 };
 
 struct CondM {
+  static constexpr int CHUNK = kChunk32;
+  static constexpr bool UNROLL = true;
   static constexpr int NF = 7;  // x y z h m/rho T k
   typedef float LT;
   typedef Ctx32 Ctx;
@@ -658,12 +686,12 @@ struct CondM {
     o.T = f.T32[i];
     o.hc = hconst_f(f.h32[i]);
   }
-  __device__ static void stage(const Fields& f, int j, float (*L)[kTileChunk], int s,
+  __device__ static void stage(const Fields& f, int j, float (*L)[CHUNK], int s,
                                const Ctx& c) {
     pos32(f, j, c, L[0][s], L[1][s], L[2][s]);
     L[3][s] = f.h32[j]; L[4][s] = f.mdj32[j]; L[5][s] = f.T32[j]; L[6][s] = f.k32[j];
   }
-  __device__ static void pair(float (*L)[kTileChunk], int s, const Own& o, Acc& a, const Prm&) {
+  __device__ static void pair(float (*L)[CHUNK], int s, const Own& o, Acc& a, const Prm&) {
     const float xx = o.x - L[0][s], xy = o.y - L[1][s], xz = o.z - L[2][s];
     const float r2 = xx * xx + xy * xy + xz * xz;
     float h_;

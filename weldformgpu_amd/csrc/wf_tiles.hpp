@@ -8,7 +8,6 @@
 namespace wf {
 
 constexpr int kTileP = 128;         // particles per tile = threads per pair workgroup
-constexpr int kTileChunk = 256;     // staged particles per LDS chunk
 constexpr int kTileMaxCells = 1024; // occupied neighbour cells per tile
 constexpr int kSlotQuads = (70 + 3) / 4;  // quads of 16-bit slots per particle
 
