@@ -2,7 +2,7 @@
  * wf_abi.h — C-ABI of libwfhip, the gfx950 HIP implementation of the
  * WeldFormGPU per-step SPH particle pipeline (neighbour search, pair loops,
  * stress/plasticity update, integration + velocity BCs, thermal conduction,
- * internal energy).
+ * rigid triangle-mesh penalty contact, internal energy).
  *
  * The entry points replace the reference's Domain_d device call surface
  * (src/cuda/Domain_d.cuh:101-426) one for one; each declaration cites the
@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define WF_ABI_VERSION 1
+#define WF_ABI_VERSION 2
 
 /* ---- status codes ------------------------------------------------------ */
 enum {
@@ -84,6 +84,15 @@ enum {
   WF_BCT_FIXED = 2       /* Dirichlet: temperature held (extension)         */
 };
 
+/* ---- contact penalty law ------------------------------------------------ */
+enum {
+  WF_CONTACT_REFERENCE = 0, /* F = 0.4 (2 m_i/dt^2) (h_i - d) n, friction
+                               stiffness 2 m_i/dt^2: Contact.cu:247,266  */
+  WF_CONTACT_STIFFNESS = 1  /* F = kappa E dx (h_i - d) n (a stiffness
+                               kappa E/dx over a particle face dx^2),
+                               friction stiffness kappa E dx (extension) */
+};
+
 /* ---- material record: src/cuda/Material.cuh:24-93; the JSON "const" array
  *      order is WeldFormGPU.cu:753-792 ------------------------------------- */
 typedef struct wf_material_params {
@@ -117,17 +126,30 @@ typedef struct wf_config {
   int newton_max_iter;   /* Newton iteration cap (default 50)               */
   int mlp_train_interval;/* online training cadence in steps (200)          */
   double newton_tol;     /* |residual| <= newton_tol * s_trial              */
+  /* rigid-mesh contact (SolverFraser.cu:233-257, WeldFormGPU.cu:495-636)   */
+  int contact;           /* 1: surface + contact forces + mesh motion       */
+  int contact_law;       /* WF_CONTACT_*                                    */
+  double contact_kappa;  /* WF_CONTACT_STIFFNESS factor (0.1)               */
+  double contact_dx;     /* WF_CONTACT_STIFFNESS lattice spacing dx          */
+  double fric_sta;       /* Contact[0] fricCoeffStatic (0 = no friction)    */
+  double fric_dyn;       /* Contact[0] fricCoeffDynamic                      */
+  int id_free_surf;      /* ID given to free-surface particles (1,
+                            SolverFraser.cu:120)                            */
+  int _pad1;
 } wf_config;
 
 /* Fill *cfg with the reference's defaults: alpha=1, beta=0
  * (Domain_d.cu:191-192), ns_interval=5, ns_min_plastic=1e-6
- * (SolverFraser.cu:17,91), all quirks on, FP64, reference return map. */
+ * (SolverFraser.cu:17,91), all quirks on, FP64, reference return map,
+ * contact off with the reference penalty law, id_free_surf=1. */
 void wf_config_default(wf_config* cfg);
 
 /* ---- fields ------------------------------------------------------------- */
 enum {
   /* double[3*n] */
   WF_F_X = 0, WF_F_V, WF_F_A, WF_F_U, WF_F_VA, WF_F_X_SNAPSHOT,
+  WF_F_NORMAL,    /* surface normal of CalculateSurface (Geometry.cu:72)   */
+  WF_F_CONTFORCE, /* contact force of the last contact call (Contact.cu:247) */
   /* double[n] */
   WF_F_RHO = 16, WF_F_RHO0, WF_F_DRHO, WF_F_M, WF_F_H, WF_F_P, WF_F_CS, WF_F_G,
   WF_F_SIGMA_Y, WF_F_PL_STRAIN, WF_F_SIGMA_EQ, WF_F_EFF_STRAIN_RATE,
@@ -137,6 +159,8 @@ enum {
   WF_F_SIGMA = 48, WF_F_SHEARSTRESS, WF_F_STRAIN, WF_F_STRRATE, WF_F_ROTRATE,
   /* int32[n] */
   WF_F_ID = 64, WF_F_BC_T,
+  /* int32[n], read-only: neighbour count, contact-neighbour count          */
+  WF_F_NB_COUNT, WF_F_CONT_NB_COUNT,
   WF_F_COUNT_
 };
 
@@ -231,7 +255,7 @@ typedef struct wf_stats {
   int64_t newton_iters_wave; /* sum over wavefronts of their max iterations */
   int mlp_trainings;       /* online training rounds                        */
   int nb_over_cap;         /* particles whose candidates exceeded the cap   */
-  int _pad;
+  int cont_over_cap;       /* contact candidates beyond WF_MAX_CONT_NB      */
   double mlp_loss;         /* training loss of the last surrogate round     */
   int64_t mlp_samples;     /* logged samples available at that round        */
 } wf_stats;
@@ -253,6 +277,7 @@ enum {
   WF_K_NSEARCH,     /* neighbour rebuild (all its kernels)          */
   WF_K_MLP,         /* surrogate training                           */
   WF_K_RETURN,      /* batched Johnson-Cook return map (+ surrogate) */
+  WF_K_CONTACT,     /* surface, contact neighbours, forces, mesh move */
   WF_K_COUNT_
 };
 /* Accumulated milliseconds and launch count of kernel id k. */
@@ -284,6 +309,66 @@ int wf_mlp_fit(struct wf_domain* d, const float* X, const float* Y, int n, int i
                float lr, float* loss);
 /* one online round on the logged samples */
 int wf_mlp_train(struct wf_domain* d, int iters, float lr, int64_t* samples, float* loss);
+
+/* ---- rigid triangle meshes and penalty contact ---------------------------
+ * The reference stores each rigid mesh as a device TriMesh_d (Mesh.cuh:19-54)
+ * and adds one zero-mass "particle" per triangle centroid to the SPH set
+ * (Domain::AddTrimeshParticles, Domain.cpp:967-983) so that contact pairs
+ * come out of the SPH neighbour search.  Here meshes are kept apart from the
+ * particles: nodes, connectivity, centroids, unit normals and plane
+ * constants live in their own device arrays, and contact candidates come
+ * from a grid over triangle centroids.  Element indices are global over all
+ * meshes, in the order the meshes were added. */
+#define WF_MAX_MESHES 8
+#define WF_MAX_CONT_NB 20 /* MAX_NB_COUNT, Contact.cu:7 */
+
+/* TriMesh_d::AxisPlaneMesh(2, false, p1, p2, dens), Mesh.cu:48-194: a plate
+ * of (dens+1)^2 nodes spaced (p2.x-p1.x)/dens at z = p1.z, two triangles per
+ * quad with normal -z.  Host only.  Fills nodes (3*(dens+1)^2) and elnode
+ * (3*2*dens^2) when the caps allow; returns the element count.             */
+int wf_gen_plane_mesh(const double p1[3], const double p2[3], int dens, double* nodes,
+                      int node_cap, int* elnode, int elem_cap, int* n_nodes);
+/* Upload a rigid mesh (WeldFormGPU.cu:495-575): nodes xyz interleaved,
+ * elnode 3 per triangle (0-based, into this mesh's nodes).  zone_id is the
+ * mesh's ID (contact_surf_id).  Centroids, normals (n0n1 x n0n2, unit) and
+ * plane constants are computed on the device.  *mesh gets its index.       */
+int wf_add_mesh(struct wf_domain* d, const double* nodes, int n_nodes, const int* elnode,
+                int n_elems, int zone_id, int* mesh);
+/* Rigid motion of a mesh: node velocity v + w x (x - center) (SetMeshVel,
+ * Domain_d.cuh:553-556; TriMesh_d::Move uses w x x, i.e. center 0).       */
+int wf_set_mesh_motion(struct wf_domain* d, int mesh, const double v[3], const double w[3],
+                       const double center[3]);
+/* Counts of mesh m (nodes, elements, zone id); m = -1: totals over meshes. */
+int wf_mesh_info(struct wf_domain* d, int mesh, int* n_nodes, int* n_elems, int* zone_id);
+/* Download mesh m: nodes/node_v (3*n_nodes), centroids/normals (3*n_elems),
+ * plane constants (n_elems); any pointer may be null.                      */
+int wf_get_mesh(struct wf_domain* d, int mesh, double* nodes, double* node_v,
+                double* centroids, double* normals, double* pplane);
+/* CalculateSurfaceKernel, Geometry.cu:13-88: normal_i = sum_j m_j x_ij *
+ * N/(M nb_i) with N = particles + triangles and M the total mass; particles
+ * with |normal_i| >= 0.25 h_i and nb_i <= 46 get ID = id_free_surf unless
+ * their ID was non-zero at the first call (not_write_surf_ID,
+ * WeldFormGPU.cu:845-858).                                                 */
+int wf_calc_surface(struct wf_domain* d);
+/* CalcContactNbKernel, Contact.cu:15-62: for each particle with ID ==
+ * id_free_surf, the triangles whose centroid lies within 2 h_i, ascending
+ * element index, at most WF_MAX_CONT_NB.                                   */
+int wf_calc_contact_nb(struct wf_domain* d);
+/* CalcContactForcesKernel -> CalcContactForcesWang, Contact.cu:114-321:
+ * predicted position, plane distance, inside-triangle test, penalty normal
+ * force and static/dynamic Coulomb friction added to a; contact force kept
+ * in WF_F_CONTFORCE.                                                       */
+int wf_calc_contact_forces(struct wf_domain* d);
+/* MeshUpdateKernel (Domain_d.cuh:545-551: Move, CalcCentroids, CalcNormals,
+ * UpdatePlaneCoeff) for every mesh, then UpdateContactParticlesKernel
+ * (Contact.cu:64-112): element velocity = mean of its node velocities.     */
+int wf_mesh_update(struct wf_domain* d, double dt);
+/* Contact candidate lists in ORIGINAL particle order: counts[n] and
+ * list[n * WF_MAX_CONT_NB] (global element indices, -1 past the count).    */
+int wf_get_contact_neighbors(struct wf_domain* d, int* counts, int* list);
+/* Sum of |contact force| over particles with ID == id_free_surf (the
+ * "Contact Force Sum" line, SolverFraser.cu:355-360), FP64, deterministic. */
+int wf_contact_force_sum(struct wf_domain* d, double* sum);
 
 /* ---- host-side input builders (no device needed) ---------------------------
  * Lattice generators and zone assignment of the host domain build, in the

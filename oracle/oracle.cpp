@@ -806,6 +806,191 @@ int or_nsearch_build(or_state* s, const wf_config* c, int* over_cap) {
 
 // One pass of the MechFraserSolve loop body, SolverFraser.cu:164-399
 // (contact and AxiSymm branches excluded).
+// ---- rigid-mesh contact ----------------------------------------------------
+namespace {
+// vector_math.h:811-814
+inline V3 cross(V3 a, V3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+}  // namespace
+
+// TriMesh_d::AxisPlaneMesh(2, false, p1, p2, dens), Mesh.cu:48-194: nodes row
+// by row with the spacing dl = (p2-p1).x/dens accumulated in x and y,
+// triangles [n0,n2,n1], [n1,n2,n3] per quad
+int or_gen_plane_mesh(const double p1[3], const double p2[3], int dens, double* nodes,
+                      int* elnode) {
+  const double dl = (p2[0] - p1[0]) / dens;
+  if (nodes) {
+    int vi = 0;
+    double x2 = p1[1];
+    for (int j = 0; j < dens + 1; j++) {
+      double x1 = p1[0];
+      for (int i = 0; i < dens + 1; i++) {
+        nodes[3 * vi] = x1;
+        nodes[3 * vi + 1] = x2;
+        nodes[3 * vi + 2] = p1[2];
+        vi++;
+        x1 += dl;
+      }
+      x2 += dl;
+    }
+  }
+  if (elnode) {
+    int el = 0;
+    for (int j = 0; j < dens; j++)
+      for (int i = 0; i < dens; i++) {
+        int n0 = (dens + 1) * j + i, n1 = n0 + 1, n2 = (dens + 1) * (j + 1) + i, n3 = n2 + 1;
+        int con[2][3] = {{n0, n2, n1}, {n1, n2, n3}};
+        for (int e = 0; e < 2; e++, el++)
+          for (int k = 0; k < 3; k++) elnode[3 * el + k] = con[e][k];
+      }
+  }
+  return 2 * dens * dens;
+}
+
+// CalcCentroids, CalcNormals, UpdatePlaneCoeff (Mesh.cu:223-258); the plane
+// constant from the element's first node (the reference reads node
+// elnode[3e]+2, which lies in the same plane on a plate)
+void or_mesh_geom(or_mesh* m) {
+  for (int e = 0; e < m->ne; e++) {
+    V3 a = ld3(m->x, m->el[3 * e]), b = ld3(m->x, m->el[3 * e + 1]),
+       c = ld3(m->x, m->el[3 * e + 2]);
+    V3 sum = add(add(a, b), c);  // (n0 + n1 + n2) / 3.
+    st3(m->c, e, V3{sum.x / 3., sum.y / 3., sum.z / 3.});
+    V3 w = cross(sub(b, a), sub(c, a));
+    double l = length(w);
+    V3 n{w.x / l, w.y / l, w.z / l};
+    st3(m->nrm, e, n);
+    m->pp[e] = dot(a, n);
+  }
+}
+
+// TriMesh_d::Move (Mesh.cu:260-276) with the rotation about a centre
+void or_mesh_move(or_mesh* m, double dt) {
+  for (int q = 0; q < m->nn; q++) {
+    const double* mo = m->motion + 9 * m->node_mesh[q];
+    V3 vel{mo[0], mo[1], mo[2]}, w{mo[3], mo[4], mo[5]}, ctr{mo[6], mo[7], mo[8]};
+    V3 v = add(vel, cross(w, sub(ld3(m->x, q), ctr)));
+    st3(m->v, q, v);
+    st3(m->x, q, add(ld3(m->x, q), scl(v, dt)));
+  }
+  or_mesh_geom(m);
+}
+
+// CalculateSurface, Geometry.cu:26-88: normal_i = sum_j m_j x_ij, times
+// N/(M nb_i) with N = particles + rigid triangles and M the total mass;
+// |normal| >= 0.25 h and nb <= 46 marks the free surface unless the
+// particle's ID was non-zero at the first pass (WeldFormGPU.cu:845-858)
+void or_calc_surface(or_state* s, const wf_config* c) {
+  or_contact* ct = s->contact;
+  if (!ct->surf_init) {
+    double tm = 0.0;
+    for (int i = 0; i < s->n; i++) tm += s->m[i];  // WeldFormGPU.cu:808-814
+    ct->totmass = tm;
+    for (int i = 0; i < s->n; i++) ct->idlock[i] = s->ID[i] != 0;
+    ct->surf_init = 1;
+  }
+  const double ntot = (double)s->n + (double)(ct->mesh ? ct->mesh->ne : 0);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < s->solid; i++) {
+    V3 n{0.0, 0.0, 0.0};
+    V3 xi = ld3(s->x, i);
+    const int cnt = s->nb_count[i];
+    for (int k = 0; k < cnt; k++) {
+      int j = s->nb_list[s->nb_offset[i] + k];
+      n = add(n, scl(sub(xi, ld3(s->x, j)), s->m[j]));
+    }
+    n = scl(n, ntot / (ct->totmass * (double)cnt));
+    st3(ct->normal, i, n);
+    if (length(n) >= 0.25 * s->h[i] && cnt <= 46 && !ct->idlock[i]) s->ID[i] = c->id_free_surf;
+  }
+}
+
+// CalcContactNb, Contact.cu:29-62: free-surface particles against every
+// triangle, centroid closer than 2 h_i, ascending element index, at most 20
+void or_contact_nb(or_state* s, const wf_config* c) {
+  or_contact* ct = s->contact;
+  const or_mesh* m = ct->mesh;
+  int over = 0;
+  for (int i = 0; i < s->solid; i++) {
+    int cnt = 0;
+    if (s->ID[i] == c->id_free_surf) {
+      V3 xi = ld3(s->x, i);
+      bool ov = false;
+      for (int e = 0; e < m->ne; e++) {
+        if (!(length(sub(xi, ld3(m->c, e))) < 2. * s->h[i])) continue;
+        if (cnt < WF_MAX_CONT_NB) ct->cnb_list[WF_MAX_CONT_NB * i + cnt++] = e;
+        else ov = true;
+      }
+      over += ov;
+    }
+    ct->cnb_count[i] = cnt;
+  }
+  ct->cont_over_cap += over;
+}
+
+// CalcContactForcesWang, Contact.cu:134-321
+void or_contact_forces(or_state* s, const wf_config* c) {
+  or_contact* ct = s->contact;
+  const or_mesh* m = ct->mesh;
+  const double dt = c->dt;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < s->solid; i++) {
+    V3 cf{0.0, 0.0, 0.0};
+    const V3 xi = ld3(s->x, i), vi = ld3(s->v, i);
+    V3 ai = ld3(s->a, i);
+    const double mi = s->m[i], hi = s->h[i];
+    for (int k = 0; k < ct->cnb_count[i]; k++) {
+      const int e = ct->cnb_list[WF_MAX_CONT_NB * i + k];
+      // x + v dt + a dt dt / 2. with the acceleration of earlier contacts
+      const V3 t = scl(scl(ai, dt), dt);
+      const V3 x_pred = add(add(xi, scl(vi, dt)), V3{t.x / 2., t.y / 2., t.z / 2.});
+      const V3 n = ld3(m->nrm, e);
+      const double dist = dot(n, x_pred) - m->pp[e];
+      if (!(dist < hi)) continue;
+      const V3 Qj = sub(xi, scl(n, dist));
+      bool inside = true;
+      for (int l = 0; l < 3 && inside; l++) {
+        int nn = l + 1;
+        if (nn > 2) nn = 0;
+        V3 pl = ld3(m->x, m->el[3 * e + l]), pn = ld3(m->x, m->el[3 * e + nn]);
+        if (dot(cross(sub(pn, pl), sub(Qj, pl)), n) < 0.0) inside = false;
+      }
+      if (!inside) continue;
+      const double delta = hi - dist;
+      double kij, fac;
+      if (c->contact_law == WF_CONTACT_REFERENCE) {
+        kij = 2.0 * mi / (dt * dt);
+        fac = 0.4;
+      } else {
+        kij = c->contact_kappa * c->mat.E * c->contact_dx;
+        fac = 1.0;
+      }
+      cf = scl(n, fac * kij * delta);
+      ai = add(ai, V3{cf.x / mi, cf.y / mi, cf.z / mi});
+      if (c->fric_sta > 0.) {
+        V3 vv{0.0, 0.0, 0.0};
+        for (int en = 0; en < 3; en++) vv = add(vv, ld3(m->v, m->el[3 * e + en]));
+        V3 ve{vv.x / 3., vv.y / 3., vv.z / 3.};
+        V3 du = sub(sub(x_pred, xi), scl(ve, dt));
+        V3 dtg = sub(du, scl(n, dot(du, n)));
+        V3 tg = scl(dtg, kij);
+        const double ltg = length(tg), lcf = length(cf);
+        if (ltg < c->fric_sta * lcf) {
+          ai = sub(ai, V3{tg.x / mi, tg.y / mi, tg.z / mi});
+        } else {
+          V3 fd{c->fric_dyn * lcf * tg.x / ltg, c->fric_dyn * lcf * tg.y / ltg,
+                c->fric_dyn * lcf * tg.z / ltg};
+          cf = sub(cf, fd);
+          ai = sub(ai, V3{fd.x / mi, fd.y / mi, fd.z / mi});
+        }
+      }
+    }
+    st3(s->a, i, ai);
+    st3(ct->cf, i, cf);
+  }
+}
+
 int or_step(or_state* s, const wf_config* c, or_run* run, int nbc, const int* bc_zone,
             const double* bc_val) {
   if (c->mechanics) {
@@ -822,6 +1007,13 @@ int or_step(or_state* s, const wf_config* c, or_run* run, int nbc, const int* bc
     or_pressure(s);
     or_stress_strain(s, c);
     or_calc_accel(s, c);
+    if (c->contact && s->contact) {  // SolverFraser.cu:233-257
+      or_calc_surface(s, c);
+      if (s->contact->mesh && s->contact->mesh->ne > 0) {
+        if (run->ts_i == 0) or_contact_nb(s, c);
+        or_contact_forces(s, c);
+      }
+    }
     if (!run->is_yielding) {
       for (int i = 0; i < s->n; i++)
         if (s->pl_strain[i] > run->max_pl_strain) run->max_pl_strain = s->pl_strain[i];
@@ -836,6 +1028,8 @@ int or_step(or_state* s, const wf_config* c, or_run* run, int nbc, const int* bc
   }
   if (c->thermal) or_thermal_calcs(s, c, &run->isfirst_step);
   if (c->mechanics) or_int_energy(s, c);
+  if (c->mechanics && c->contact && s->contact && s->contact->mesh)  // SolverFraser.cu:290-318
+    or_mesh_move(s->contact->mesh, c->dt);
   run->isfirst_step = 0;
   run->time += c->dt;
   run->step++;

@@ -18,6 +18,9 @@
  *   src/cuda/Material.cuh:192-242                 Hollomon / Johnson-Cook
  *   src/cuda/Thermal.cu:16-142                    conduction, convection, leapfrog
  *   src/cuda/Energy.cu:5-19                       internal energy
+ *   src/cuda/Geometry.cu:26-88                    free-surface normals and flag
+ *   src/cuda/Contact.cu:29-62,85-112,134-321      contact candidates and forces
+ *   src/cuda/Mesh.cu:48-276                       plate mesh, rigid motion, normals
  *   src/cuda/SolverFraser.cu:30-409               step order and rebuild policy
  * All arrays are in the caller's (original) particle order.
  */
@@ -30,6 +33,8 @@
 #ifdef __cplusplus
 extern "C" {
 #endif
+
+struct or_contact;
 
 typedef struct or_state {
   int n, solid;
@@ -44,7 +49,28 @@ typedef struct or_state {
   int64_t nb_cap;
   /* Newton statistics of the last or_stress_strain call */
   int64_t newton_iters, newton_calls;
+  struct or_contact* contact; /* null: no contact data */
 } or_state;
+
+/* rigid triangle meshes, all meshes concatenated (TriMesh_d, Mesh.cuh:19-54) */
+typedef struct or_mesh {
+  int nn, ne, n_meshes;
+  double *x, *v;          /* 3*nn interleaved: node positions, velocities  */
+  int* node_mesh;         /* nn: owning mesh                                */
+  int* el;                /* 3*ne global node indices                       */
+  double *c, *nrm, *pp;   /* 3*ne centroids, 3*ne unit normals, ne planes   */
+  double* motion;         /* 9 per mesh: velocity, angular velocity, centre */
+} or_mesh;
+
+typedef struct or_contact {
+  or_mesh* mesh;
+  double *normal, *cf;    /* 3n: surface normal, contact force              */
+  int* idlock;            /* n: ID != 0 at the first surface pass           */
+  int *cnb_count, *cnb_list; /* n, 20n: candidates, ascending element index */
+  int surf_init;
+  int cont_over_cap;
+  double totmass;
+} or_contact;
 
 typedef struct or_run {
   double time;
@@ -109,6 +135,16 @@ int or_nsearch_build(or_state* s, const wf_config* c, int* over_cap);
  * order W1[4][32], b1[32], W2[32][32], b2[32], W3[32], b3 (1249 values). */
 double or_mlp_forward(const double* w, const double* x);
 double or_mlp_loss_grad(const double* w, const double* X, const double* Y, int n, double* grad);
+
+/* contact ------------------------------------------------------------------*/
+/* AxisPlaneMesh(2, false, p1, p2, dens), Mesh.cu:48-194; returns elements */
+int or_gen_plane_mesh(const double p1[3], const double p2[3], int dens, double* nodes,
+                      int* elnode);
+void or_mesh_geom(or_mesh* m);           /* CalcCentroids/CalcNormals/UpdatePlaneCoeff */
+void or_mesh_move(or_mesh* m, double dt); /* TriMesh_d::Move                         */
+void or_calc_surface(or_state* s, const wf_config* c);  /* CalculateSurface          */
+void or_contact_nb(or_state* s, const wf_config* c);    /* CalcContactNb (brute force) */
+void or_contact_forces(or_state* s, const wf_config* c);/* CalcContactForcesWang     */
 
 /* one Fraser step (SolverFraser.cu:164-399) */
 int or_step(or_state* s, const wf_config* c, or_run* run, int nbc,

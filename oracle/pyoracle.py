@@ -26,7 +26,18 @@ class or_state(C.Structure):
                 + [(k, C.c_void_p) for k in _DBL3 + _DBL1 + _DBL6 + _INT1]
                 + [("nb_count", C.c_void_p), ("nb_offset", C.c_void_p), ("nb_list", C.c_void_p),
                    ("nb_cap", C.c_int64), ("newton_iters", C.c_int64),
-                   ("newton_calls", C.c_int64)])
+                   ("newton_calls", C.c_int64), ("contact", C.c_void_p)])
+
+
+class or_mesh(C.Structure):
+    _fields_ = [("nn", C.c_int), ("ne", C.c_int), ("n_meshes", C.c_int)] + [
+        (k, C.c_void_p) for k in ["x", "v", "node_mesh", "el", "c", "nrm", "pp", "motion"]]
+
+
+class or_contact(C.Structure):
+    _fields_ = [("mesh", C.c_void_p)] + [
+        (k, C.c_void_p) for k in ["normal", "cf", "idlock", "cnb_count", "cnb_list"]] + [
+        ("surf_init", C.c_int), ("cont_over_cap", C.c_int), ("totmass", C.c_double)]
 
 
 class or_run(C.Structure):
@@ -79,6 +90,12 @@ def lib():
             "or_int_energy_sum": (D, [S]),
             "or_nsearch_build": (I, [S, CF, C.POINTER(I)]),
             "or_step": (I, [S, CF, R, I, P, P]),
+            "or_gen_plane_mesh": (I, [P, P, I, P, P]),
+            "or_mesh_geom": (None, [P]),
+            "or_mesh_move": (None, [P, D]),
+            "or_calc_surface": (None, [S, CF]),
+            "or_contact_nb": (None, [S, CF]),
+            "or_contact_forces": (None, [S, CF]),
             "or_mlp_forward": (D, [P, P]),
             "or_mlp_loss_grad": (D, [P, P, P, I, P]),
         }
@@ -117,6 +134,16 @@ def assign_zone(xyz, ID, start, end, zid):
     s = np.ascontiguousarray(start, dtype=np.float64)
     e = np.ascontiguousarray(end, dtype=np.float64)
     return lib().or_assign_zone(_ptr(xyz), len(xyz), _ptr(s), _ptr(e), zid, _ptr(ID))
+
+
+def gen_plane_mesh(p1, p2, dens):
+    """AxisPlaneMesh(2, false, p1, p2, dens): (nodes (nn,3), elnode (ne,3))."""
+    p1 = np.ascontiguousarray(p1, np.float64)
+    p2 = np.ascontiguousarray(p2, np.float64)
+    nodes = np.zeros(((dens + 1) ** 2, 3))
+    el = np.zeros((2 * dens * dens, 3), np.int32)
+    lib().or_gen_plane_mesh(_ptr(p1), _ptr(p2), dens, _ptr(nodes), _ptr(el))
+    return nodes, el
 
 
 # ---- neighbours -----------------------------------------------------------
@@ -188,6 +215,60 @@ class State:
     def neighbor_rows(self):
         c, o, l = self["nb_count"], self["nb_offset"], self["nb_list"]
         return c.copy(), o.copy(), l[: o[self.n]].copy()
+
+    # -- rigid meshes and contact -------------------------------------------
+    def add_contact(self, meshes):
+        """meshes: list of dicts with nodes (nn,3), elnode (ne,3) and optional
+        vel, omega, center (3 each).  Concatenates them like the library."""
+        xs, els, nm, mot = [], [], [], []
+        off = 0
+        for k, m in enumerate(meshes):
+            nodes = np.asarray(m["nodes"], np.float64).reshape(-1, 3)
+            xs.append(nodes)
+            els.append(np.asarray(m["elnode"], np.int32).reshape(-1, 3) + off)
+            nm.append(np.full(len(nodes), k, np.int32))
+            mot.append(np.concatenate([np.asarray(m.get(q, (0, 0, 0)), np.float64)
+                                       for q in ("vel", "omega", "center")]))
+            off += len(nodes)
+        nn, ne = off, sum(len(e) for e in els)
+        cat = (lambda a, shape, dt: np.ascontiguousarray(np.concatenate(a)) if a
+               else np.zeros(shape, dt))
+        M = self.mesh_arrays = {
+            "x": cat(xs, (0, 3), np.float64), "v": np.zeros((nn, 3)),
+            "node_mesh": cat(nm, 0, np.int32), "el": cat(els, (0, 3), np.int32),
+            "c": np.zeros((ne, 3)), "nrm": np.zeros((ne, 3)), "pp": np.zeros(ne),
+            "motion": cat(mot, 0, np.float64)}
+        self.mesh = or_mesh()
+        self.mesh.nn, self.mesh.ne, self.mesh.n_meshes = nn, ne, len(meshes)
+        for k, a in M.items():
+            setattr(self.mesh, k, a.ctypes.data)
+        n = self.n
+        self.contact_arrays = {"normal": np.zeros((n, 3)), "cf": np.zeros((n, 3)),
+                               "idlock": np.zeros(n, np.int32),
+                               "cnb_count": np.zeros(n, np.int32),
+                               "cnb_list": np.full((n, 20), -1, np.int32)}
+        self.ct = or_contact()
+        self.ct.mesh = C.cast(C.pointer(self.mesh), C.c_void_p)
+        for k, a in self.contact_arrays.items():
+            setattr(self.ct, k, a.ctypes.data)
+        self.st.contact = C.cast(C.pointer(self.ct), C.c_void_p)
+        lib().or_mesh_geom(C.byref(self.mesh))
+
+    def set_mesh_motion(self, mesh, vel=(0, 0, 0), omega=(0, 0, 0), center=(0, 0, 0)):
+        self.mesh_arrays["motion"][9 * mesh:9 * mesh + 9] = np.concatenate(
+            [np.asarray(q, np.float64) for q in (vel, omega, center)])
+
+    def calc_surface(self, cfg):
+        lib().or_calc_surface(self.ref(), C.byref(cfg))
+
+    def contact_nb(self, cfg):
+        lib().or_contact_nb(self.ref(), C.byref(cfg))
+
+    def contact_forces(self, cfg):
+        lib().or_contact_forces(self.ref(), C.byref(cfg))
+
+    def mesh_move(self, dt):
+        lib().or_mesh_move(C.byref(self.mesh), dt)
 
     def step(self, cfg, bcs=(), nsteps=1):
         zones = np.array([b[0] for b in bcs] or [0], np.int32)

@@ -16,6 +16,8 @@ def to_oracle(po, sc):
     for k, v in sc.fields.items():
         st[ORACLE_NAME.get(k, k)] = v
     st["xs"] = sc.fields["x"]
+    if sc.meshes:
+        st.add_contact(sc.meshes)
     return st
 
 

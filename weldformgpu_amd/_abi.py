@@ -7,7 +7,7 @@ product can silently run without the native code.
 import ctypes as C
 import os
 
-WF_ABI_VERSION = 1
+WF_ABI_VERSION = 2
 
 WF_OK = 0
 WF_E_ARG = -1
@@ -29,10 +29,13 @@ WF_Q_TENSOR_PRODUCT = 0x2
 WF_Q_STALE_NS_POS = 0x4
 WF_Q_ALL = 0x7
 WF_BCT_NONE, WF_BCT_CONVECTION, WF_BCT_FIXED = 0, 1, 2
+WF_CONTACT_REFERENCE, WF_CONTACT_STIFFNESS = 0, 1
+WF_MAX_MESHES = 8
+WF_MAX_CONT_NB = 20
 
 # fields: name -> (id, components, numpy dtype)
 FIELDS = {}
-for _k, _n in enumerate(["x", "v", "a", "u", "va", "x_snapshot"]):
+for _k, _n in enumerate(["x", "v", "a", "u", "va", "x_snapshot", "normal", "contforce"]):
     FIELDS[_n] = (_k, 3, "f8")
 for _k, _n in enumerate(["rho", "rho0", "drho", "m", "h", "p", "Cs", "G", "sigma_y",
                          "pl_strain", "sigma_eq", "eff_strain_rate", "int_energy", "T",
@@ -40,10 +43,11 @@ for _k, _n in enumerate(["rho", "rho0", "drho", "m", "h", "p", "Cs", "G", "sigma
     FIELDS[_n] = (16 + _k, 1, "f8")
 for _k, _n in enumerate(["sigma", "shearstress", "strain", "strrate", "rotrate"]):
     FIELDS[_n] = (48 + _k, 6, "f8")
-for _k, _n in enumerate(["ID", "BC_T"]):
+for _k, _n in enumerate(["ID", "BC_T", "nb_count", "cont_nb_count"]):
     FIELDS[_n] = (64 + _k, 1, "i4")
 
-KERNELS = ["dens", "rates", "stress", "accel", "integrate", "thermal", "nsearch", "mlp", "return"]
+KERNELS = ["dens", "rates", "stress", "accel", "integrate", "thermal", "nsearch", "mlp", "return",
+           "contact"]
 WF_MLP_PARAMS = 1249
 
 
@@ -67,14 +71,18 @@ class wf_config(C.Structure):
                 ("h_conv", C.c_double), ("ns_interval", C.c_int),
                 ("surrogate", C.c_int), ("ns_min_plastic", C.c_double),
                 ("newton_max_iter", C.c_int), ("mlp_train_interval", C.c_int),
-                ("newton_tol", C.c_double)]
+                ("newton_tol", C.c_double),
+                ("contact", C.c_int), ("contact_law", C.c_int),
+                ("contact_kappa", C.c_double), ("contact_dx", C.c_double),
+                ("fric_sta", C.c_double), ("fric_dyn", C.c_double),
+                ("id_free_surf", C.c_int), ("_pad1", C.c_int)]
 
 
 class wf_stats(C.Structure):
     _fields_ = [("step", C.c_int64), ("time", C.c_double), ("is_yielding", C.c_int),
                 ("ns_builds", C.c_int), ("newton_iters", C.c_int64),
                 ("newton_calls", C.c_int64), ("newton_iters_wave", C.c_int64),
-                ("mlp_trainings", C.c_int), ("nb_over_cap", C.c_int), ("_pad", C.c_int),
+                ("mlp_trainings", C.c_int), ("nb_over_cap", C.c_int), ("cont_over_cap", C.c_int),
                 ("mlp_loss", C.c_double), ("mlp_samples", C.c_int64)]
 
 
@@ -95,6 +103,10 @@ def default_config():
     c.newton_tol = 1e-11
     c.surrogate = 0
     c.mlp_train_interval = 200
+    c.contact = 0
+    c.contact_law = WF_CONTACT_REFERENCE
+    c.contact_kappa = 0.1
+    c.id_free_surf = 1
     return c
 
 
@@ -142,6 +154,17 @@ def _proto(lib):
         "wf_mlp_grad": (I, [P, VP, VP, I, VP, C.POINTER(C.c_float)]),
         "wf_mlp_fit": (I, [P, VP, VP, I, I, C.c_float, C.POINTER(C.c_float)]),
         "wf_mlp_train": (I, [P, I, C.c_float, C.POINTER(I64), C.POINTER(C.c_float)]),
+        "wf_gen_plane_mesh": (I, [VP, VP, I, VP, I, VP, I, C.POINTER(I)]),
+        "wf_add_mesh": (I, [P, VP, I, VP, I, I, C.POINTER(I)]),
+        "wf_set_mesh_motion": (I, [P, I, VP, VP, VP]),
+        "wf_mesh_info": (I, [P, I, C.POINTER(I), C.POINTER(I), C.POINTER(I)]),
+        "wf_get_mesh": (I, [P, I, VP, VP, VP, VP, VP]),
+        "wf_calc_surface": (I, [P]),
+        "wf_calc_contact_nb": (I, [P]),
+        "wf_calc_contact_forces": (I, [P]),
+        "wf_mesh_update": (I, [P, D]),
+        "wf_get_contact_neighbors": (I, [P, VP, VP]),
+        "wf_contact_force_sum": (I, [P, C.POINTER(D)]),
         "wf_gen_box": (I, [VP, D, D, D, D, D, VP, I, VP]),
         "wf_gen_cylinder": (I, [VP, D, D, D, D, VP, I, VP]),
         "wf_assign_zone": (I, [VP, I, VP, VP, I, VP]),
@@ -163,7 +186,10 @@ def declared_symbols():
             "wf_apply_bc_vel", "wf_update_pos_fraser", "wf_update_vel", "wf_calc_temp_inc",
             "wf_thermal_calcs", "wf_int_energy", "wf_step", "wf_get_stats", "wf_set_timing",
             "wf_kernel_time", "wf_mlp_get_params", "wf_mlp_set_params", "wf_mlp_reset_adam",
-            "wf_mlp_infer", "wf_mlp_grad", "wf_mlp_fit", "wf_mlp_train", "wf_gen_box",
+            "wf_mlp_infer", "wf_mlp_grad", "wf_mlp_fit", "wf_mlp_train",
+            "wf_gen_plane_mesh", "wf_add_mesh", "wf_set_mesh_motion", "wf_mesh_info",
+            "wf_get_mesh", "wf_calc_surface", "wf_calc_contact_nb", "wf_calc_contact_forces",
+            "wf_mesh_update", "wf_get_contact_neighbors", "wf_contact_force_sum", "wf_gen_box",
             "wf_gen_cylinder", "wf_assign_zone"]
 
 

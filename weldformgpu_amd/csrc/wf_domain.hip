@@ -91,6 +91,22 @@ struct wf_domain {
   // lists were built from)
   int64_t snap_gen = 0, built_gen = -1;
   BcTable bcs{};
+  // rigid meshes (all concatenated) and contact state
+  MeshDev mesh{};
+  MeshMotions motions{};
+  int n_meshes = 0;
+  int mesh_node_off[kMaxMeshes + 1] = {0}, mesh_el_off[kMaxMeshes + 1] = {0};
+  int mesh_zone[kMaxMeshes] = {0};
+  std::vector<void*> mesh_owned;
+  ContactLists cl{};
+  bool cont_built = false;
+  bool surf_init = false;  // not_write_surf_ID and total mass taken
+  double totmass = 0.0;
+  int *tri_cnt = nullptr, *tri_start = nullptr, *tri_rank = nullptr, *tri_sorted = nullptr;
+  int* tri_scan_part = nullptr;
+  int64_t tri_cells_cap = 0;
+  double* cscratch = nullptr;   // device: centroid bbox + max h
+  double* cscratch_h = nullptr; // pinned
   // run state (SolverFraser.cu:30-409)
   double time = 0.0;
   int64_t step = 0;
@@ -115,6 +131,9 @@ struct wf_domain {
   int mlp_train_round(int iters, float lr);
   int mlp_fit(const float* X, const float* Y, int n, int stride, int iters, float lr);
   int dev_alloc(void** p, size_t bytes);
+  int surface_init();
+  int contact_nb();
+  ContactParams contact_params() const;
 };
 
 int wf_domain::dev_alloc(void** p, size_t bytes) {
@@ -126,7 +145,7 @@ int wf_domain::dev_alloc(void** p, size_t bytes) {
 namespace {
 
 int field_info(int field, int* comps, int* elem) {
-  if (field >= WF_F_X && field <= WF_F_X_SNAPSHOT) {
+  if (field >= WF_F_X && field <= WF_F_CONTFORCE) {
     *comps = 3;
     *elem = 8;
   } else if (field >= WF_F_RHO && field <= WF_F_Q_PLASTIC) {
@@ -135,7 +154,7 @@ int field_info(int field, int* comps, int* elem) {
   } else if (field >= WF_F_SIGMA && field <= WF_F_ROTRATE) {
     *comps = 6;
     *elem = 8;
-  } else if (field == WF_F_ID || field == WF_F_BC_T) {
+  } else if (field >= WF_F_ID && field <= WF_F_CONT_NB_COUNT) {
     *comps = 1;
     *elem = 4;
   } else {
@@ -234,6 +253,8 @@ PlaneRef wf_domain::plane(int field, int c) {
     case WF_F_U: r.d = &f.u[c]; break;
     case WF_F_VA: r.d = &f.va[c]; break;
     case WF_F_X_SNAPSHOT: r.d = &f.xs[c]; break;
+    case WF_F_NORMAL: r.d = &f.nrm[c]; break;
+    case WF_F_CONTFORCE: r.d = &f.cf[c]; break;
     case WF_F_RHO: r.d = &f.rho; break;
     case WF_F_RHO0: r.d = &f.rho0; break;
     case WF_F_DRHO: r.d = &f.drho; break;
@@ -261,6 +282,8 @@ PlaneRef wf_domain::plane(int field, int c) {
     case WF_F_ROTRATE: r.d = &f.rotrate[c]; break;
     case WF_F_ID: r.i = &f.ID; break;
     case WF_F_BC_T: r.i = &f.BC_T; break;
+    case WF_F_NB_COUNT: r.i = &nb_count; break;
+    case WF_F_CONT_NB_COUNT: r.i = &cl.count; break;
     default: break;
   }
   return r;
@@ -269,6 +292,7 @@ PlaneRef wf_domain::plane(int field, int c) {
 std::vector<PlaneRef> wf_domain::all_planes() {
   std::vector<PlaneRef> v;
   for (int fld = 0; fld < WF_F_COUNT_; fld++) {
+    if (fld == WF_F_NB_COUNT || fld == WF_F_CONT_NB_COUNT) continue;  // derived, read-only
     int comps, elem;
     if (field_info(fld, &comps, &elem) != WF_OK) continue;
     for (int c = 0; c < comps; c++) {
@@ -365,6 +389,86 @@ int wf_domain::mlp_fit(const float* X, const float* Y, int nsamp, int stride, in
   return WF_OK;
 }
 
+// not_write_surf_ID (ID != 0) and the total mass, taken at the first surface
+// pass after IDs or masses were set (WeldFormGPU.cu:808-858)
+int wf_domain::surface_init() {
+  if (surf_init) return WF_OK;
+  launch_surface_lock(stream, f, n);
+  const int nb = std::min(1024, std::max(1, gridn(n)));
+  launch_sum(stream, f.m, n, sum_part, nb, sum_out);
+  HIPC(hipGetLastError());
+  HIPC(hipMemcpyAsync(cscratch_h, sum_out, sizeof(double), hipMemcpyDeviceToHost, stream));
+  HIPC(hipStreamSynchronize(stream));
+  totmass = cscratch_h[0];
+  surf_init = true;
+  return WF_OK;
+}
+
+// contact candidates: bin the triangle centroids on a grid of edge >= 2 max h,
+// then one thread per particle scans its 3x3x3 block of cells
+int wf_domain::contact_nb() {
+  constexpr int64_t kMaxTriCells = 1 << 22;
+  launch_tri_bbox(stream, mesh, cscratch);
+  launch_max_f64(stream, f.h, solid, cscratch + 6);
+  HIPC(hipGetLastError());
+  HIPC(hipMemcpyAsync(cscratch_h, cscratch, sizeof(double) * 7, hipMemcpyDeviceToHost, stream));
+  HIPC(hipStreamSynchronize(stream));
+  const double* b = cscratch_h;
+  const double hmax = b[6];
+  if (!(hmax > 0.0) || !std::isfinite(hmax)) return fail(WF_E_ARG, "smoothing length h must be positive");
+  for (int c = 0; c < 6; c++)
+    if (!std::isfinite(b[c])) return fail(WF_E_NONFINITE, "non-finite rigid mesh node");
+  TriGrid g{};
+  double cs = 2.0 * hmax * (1.0 + 1e-6);
+  int64_t cells = 0;
+  for (;;) {
+    cells = 1;
+    for (int c = 0; c < 3; c++) {
+      const double ext = b[3 + c] - b[c];
+      const double k = std::floor(ext / cs) + 1.0;
+      g.dim[c] = k > 1e6 ? 1000001 : (int)k;
+      cells *= g.dim[c];
+    }
+    if (cells <= kMaxTriCells) break;
+    cs *= 1.26;
+  }
+  for (int c = 0; c < 3; c++) g.org[c] = b[c];
+  g.inv_cs = 1.0 / cs;
+  if (cells + 1 > tri_cells_cap) {
+    const int64_t cap = std::max<int64_t>(cells + 1, 4096);
+    int *cnt = nullptr, *start = nullptr, *part = nullptr;
+    if (hipMalloc(&cnt, sizeof(int) * cap) != hipSuccess ||
+        hipMalloc(&start, sizeof(int) * (cap + 1)) != hipSuccess ||
+        hipMalloc(&part, sizeof(int) * (cap / 1024 + 4)) != hipSuccess) {
+      hipFree(cnt);
+      hipFree(start);
+      return fail(WF_E_NOMEM, "contact grid allocation failed");
+    }
+    for (void* q : {(void*)cnt, (void*)start, (void*)part}) mesh_owned.push_back(q);
+    tri_cnt = cnt;
+    tri_start = start;
+    tri_scan_part = part;
+    tri_cells_cap = cap;
+  }
+  hipLaunchKernelGGL(k_fill_i32v, dim3(gridn(cells)), dim3(256), 0, stream, tri_cnt, 0,
+                     (int)cells);
+  launch_tri_bin(stream, mesh, g, tri_cnt, tri_rank, tri_start, tri_sorted, tri_scan_part);
+  launch_contact_nb(stream, f, solid, mesh, g, tri_start, tri_sorted, cl, cfg.id_free_surf, ctr);
+  HIPC(hipGetLastError());
+  cont_built = true;
+  return WF_OK;
+}
+
+ContactParams wf_domain::contact_params() const {
+  ContactParams cp{};
+  cp.dt = cfg.dt;
+  cp.kappa_E_dx = cfg.contact_kappa * cfg.mat.E * cfg.contact_dx;
+  cp.mu_s = cfg.fric_sta;
+  cp.mu_d = cfg.fric_dyn;
+  cp.law = cfg.contact_law;
+  return cp;
+}
+
 // ============================================================================
 extern "C" {
 
@@ -386,6 +490,10 @@ void wf_config_default(wf_config* c) {
   c->newton_max_iter = 50;
   c->newton_tol = 1e-11;
   c->mlp_train_interval = 200;
+  c->contact = 0;
+  c->contact_law = WF_CONTACT_REFERENCE;
+  c->contact_kappa = 0.1;
+  c->id_free_surf = 1;  // SolverFraser.cu:120
 }
 
 int wf_field_shape(int field, int* components, int* elem_bytes) {
@@ -407,6 +515,11 @@ int wf_create(const wf_config* cfg, wf_domain** out) {
   if (cfg->precision != WF_FP64 && cfg->precision != WF_MIXED)
     return fail(WF_E_ARG, "unknown precision mode");
   if (cfg->ns_interval <= 0) return fail(WF_E_ARG, "ns_interval must be positive");
+  if (cfg->contact_law != WF_CONTACT_REFERENCE && cfg->contact_law != WF_CONTACT_STIFFNESS)
+    return fail(WF_E_ARG, "unknown contact law");
+  if (cfg->contact && cfg->contact_law == WF_CONTACT_STIFFNESS &&
+      !(cfg->contact_kappa > 0.0 && cfg->contact_dx > 0.0))
+    return fail(WF_E_ARG, "the stiffness contact law needs contact_kappa > 0 and contact_dx > 0");
   int dev_count = 0;
   if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0)
     return fail(WF_E_HIP, "no HIP device");
@@ -492,6 +605,9 @@ int wf_create(const wf_config* cfg, wf_domain** out) {
   if (!rc) rc = d->alloc_plane_i(&d->perm_tmp);
   if (!rc) rc = d->alloc_plane_i(&d->gather);
   if (!rc) rc = d->alloc_plane_i(&d->nb_count);
+  if (!rc) rc = d->alloc_plane_i(&d->f.idlock);
+  if (!rc) rc = d->dev_alloc((void**)&d->cscratch, sizeof(double) * 8);
+  if (!rc && hipHostMalloc(&d->cscratch_h, sizeof(double) * 8) != hipSuccess) rc = WF_E_NOMEM;
   if (!rc) {
     void* q = nullptr;
     if (hipMalloc(&q, sizeof(int) * d->npad * kMaxNb) != hipSuccess) rc = WF_E_NOMEM;
@@ -551,6 +667,8 @@ int wf_destroy(wf_domain* d) {
   if (d->ctr_host) hipHostFree(d->ctr_host);
   if (d->host_i) hipHostFree(d->host_i);
   if (d->host_f) hipHostFree(d->host_f);
+  if (d->cscratch_h) hipHostFree(d->cscratch_h);
+  for (void* p : d->mesh_owned) hipFree(p);
   ns_free(d->ns);
   if (d->ev0) hipEventDestroy(d->ev0);
   if (d->ev1) hipEventDestroy(d->ev1);
@@ -570,6 +688,9 @@ int wf_set_field(wf_domain* d, int field, const void* host, size_t bytes) {
   int comps, elem;
   if (field_info(field, &comps, &elem)) return fail(WF_E_ARG, "unknown field");
   if (bytes != (size_t)d->n * comps * elem) return fail(WF_E_ARG, "size mismatch");
+  if (field == WF_F_NB_COUNT || field == WF_F_CONT_NB_COUNT)
+    return fail(WF_E_ARG, "read-only field");
+  if (field == WF_F_ID || field == WF_F_M) d->surf_init = false;
   HIPC(hipMemcpyAsync(d->stage, host, bytes, hipMemcpyHostToDevice, d->stream));
   if (elem == 8) {
     Planes6 planes{};
@@ -600,6 +721,10 @@ int wf_get_field(wf_domain* d, int field, void* host, size_t bytes) {
   int comps, elem;
   if (field_info(field, &comps, &elem)) return fail(WF_E_ARG, "unknown field");
   if (bytes != (size_t)d->n * comps * elem) return fail(WF_E_ARG, "size mismatch");
+  if (elem == 4 && !*d->plane(field, 0).i) {  // contact lists not allocated: none
+    std::memset(host, 0, bytes);
+    return WF_OK;
+  }
   if (elem == 8) {
     Planes6 planes{};
     for (int c = 0; c < comps; c++) planes.p[c] = *d->plane(field, c).d;
@@ -620,6 +745,7 @@ int wf_fill_field(wf_domain* d, int field, double value) {
   int comps, elem;
   if (field_info(field, &comps, &elem) || elem != 8) return fail(WF_E_ARG, "not a double field");
   for (int c = 0; c < comps; c++) launch_fill_f64(d->stream, *d->plane(field, c).d, value, d->n);
+  if (field == WF_F_M) d->surf_init = false;
   if (field == WF_F_X) {
     for (int c = 0; c < 3; c++) launch_fill_f64(d->stream, d->f.xs[c], value, d->n);
     d->snap_gen++;
@@ -705,6 +831,18 @@ int wf_nsearch_build(wf_domain* d) {
   }
   launch_gather_i32(d->stream, d->spare_i, d->f.mlstep, d->gather, d->n);
   std::swap(d->f.mlstep, d->spare_i);
+  launch_gather_i32(d->stream, d->spare_i, d->f.idlock, d->gather, d->n);
+  std::swap(d->f.idlock, d->spare_i);
+  if (d->cl.count) {  // contact candidates follow their particles
+    launch_gather_i32(d->stream, d->spare_i, d->cl.count, d->gather, d->n);
+    std::swap(d->cl.count, d->spare_i);
+    for (int k = 0; k < kMaxContNb; k++) {
+      int* row = d->cl.list + (int64_t)k * d->npad;
+      launch_gather_i32(d->stream, d->spare_i, row, d->gather, d->n);
+      HIPC(hipMemcpyAsync(row, d->spare_i, sizeof(int) * d->n, hipMemcpyDeviceToDevice,
+                          d->stream));
+    }
+  }
   std::swap(d->perm, d->perm_tmp);
   HIPC(hipGetLastError());
   d->nb_built = true;
@@ -891,6 +1029,222 @@ int wf_int_energy(wf_domain* d, double* sum) {
   return WF_OK;
 }
 
+// ---- rigid meshes and contact ---------------------------------------------
+int wf_add_mesh(wf_domain* d, const double* nodes, int nn, const int* el, int ne, int zone,
+                int* mesh_out) {
+  if (!d || !nodes || !el) return fail(WF_E_ARG, "null argument");
+  if (nn <= 0 || ne <= 0) return fail(WF_E_ARG, "empty mesh");
+  if (d->n_meshes >= kMaxMeshes) return fail(WF_E_ARG, "too many meshes");
+  for (int k = 0; k < 3 * ne; k++)
+    if (el[k] < 0 || el[k] >= nn) return fail(WF_E_ARG, "element node index out of range");
+  for (int k = 0; k < 3 * nn; k++)
+    if (!std::isfinite(nodes[k])) return fail(WF_E_ARG, "non-finite node coordinate");
+  const int nn0 = d->mesh.nn, ne0 = d->mesh.ne;
+  if ((int64_t)nn0 + nn > INT32_MAX / 4 || (int64_t)ne0 + ne > INT32_MAX / 4)
+    return fail(WF_E_ARG, "mesh too large");
+  HIPC(hipStreamSynchronize(d->stream));  // mesh moves queued on the handle's stream
+  const int NN = nn0 + nn, NE = ne0 + ne;
+  // current device state of the meshes already added, plus the new one
+  std::vector<double> hx[3], hv[3];
+  for (int c = 0; c < 3; c++) {
+    hx[c].resize(NN);
+    hv[c].assign(NN, 0.0);
+    if (nn0) {
+      HIPC(hipMemcpy(hx[c].data(), d->mesh.x[c], sizeof(double) * nn0, hipMemcpyDeviceToHost));
+      HIPC(hipMemcpy(hv[c].data(), d->mesh.v[c], sizeof(double) * nn0, hipMemcpyDeviceToHost));
+    }
+    for (int q = 0; q < nn; q++) hx[c][nn0 + q] = nodes[3 * q + c];
+  }
+  std::vector<int> nmesh(NN), hel(3 * (size_t)NE);
+  if (nn0) {
+    HIPC(hipMemcpy(nmesh.data(), d->mesh.node_mesh, sizeof(int) * nn0, hipMemcpyDeviceToHost));
+    HIPC(hipMemcpy(hel.data(), d->mesh.el, sizeof(int) * 3 * ne0, hipMemcpyDeviceToHost));
+  }
+  const int mi = d->n_meshes;
+  for (int q = 0; q < nn; q++) nmesh[nn0 + q] = mi;
+  for (int k = 0; k < 3 * ne; k++) hel[3 * (size_t)ne0 + k] = el[k] + nn0;
+  // fresh device arrays
+  std::vector<void*> fresh;
+  auto grab = [&](void** p, size_t bytes) {
+    if (hipMalloc(p, bytes) != hipSuccess) return false;
+    fresh.push_back(*p);
+    return true;
+  };
+  MeshDev m{};
+  bool ok = true;
+  for (int c = 0; c < 3 && ok; c++)
+    ok = grab((void**)&m.x[c], sizeof(double) * NN) && grab((void**)&m.v[c], sizeof(double) * NN) &&
+         grab((void**)&m.c[c], sizeof(double) * NE) && grab((void**)&m.nrm[c], sizeof(double) * NE);
+  int *rank = nullptr, *sorted = nullptr;
+  ok = ok && grab((void**)&m.node_mesh, sizeof(int) * NN) &&
+       grab((void**)&m.el, sizeof(int) * 3 * NE) && grab((void**)&m.pp, sizeof(double) * NE) &&
+       grab((void**)&m.cell, sizeof(int) * NE) && grab((void**)&rank, sizeof(int) * NE) &&
+       grab((void**)&sorted, sizeof(int) * NE);
+  if (!ok) {
+    for (void* q : fresh) hipFree(q);
+    return fail(WF_E_NOMEM, "mesh allocation failed");
+  }
+  m.nn = NN;
+  m.ne = NE;
+  for (int c = 0; c < 3; c++) {
+    HIPC(hipMemcpy(m.x[c], hx[c].data(), sizeof(double) * NN, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(m.v[c], hv[c].data(), sizeof(double) * NN, hipMemcpyHostToDevice));
+  }
+  HIPC(hipMemcpy(m.node_mesh, nmesh.data(), sizeof(int) * NN, hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(m.el, hel.data(), sizeof(int) * 3 * NE, hipMemcpyHostToDevice));
+  // contact lists on first use
+  if (!d->cl.count) {
+    int rc = d->alloc_plane_i(&d->cl.count);
+    if (!rc) rc = d->dev_alloc((void**)&d->cl.list, sizeof(int) * d->npad * kMaxContNb);
+    if (rc) {
+      for (void* q : fresh) hipFree(q);
+      return fail(rc, "contact list allocation failed");
+    }
+    d->cl.npad = d->npad;
+  }
+  // swap in; the old arrays (and the grid scratch) go
+  HIPC(hipStreamSynchronize(d->stream));
+  for (void* q : d->mesh_owned) hipFree(q);
+  d->mesh_owned = fresh;
+  d->tri_cnt = d->tri_start = d->tri_scan_part = nullptr;
+  d->tri_cells_cap = 0;
+  d->tri_rank = rank;
+  d->tri_sorted = sorted;
+  d->mesh = m;
+  d->mesh_node_off[mi + 1] = NN;
+  d->mesh_el_off[mi + 1] = NE;
+  d->mesh_zone[mi] = zone;
+  d->motions.m[mi] = MeshMotion{};
+  d->n_meshes++;
+  d->cont_built = false;
+  launch_mesh_geom(d->stream, d->mesh);
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(d->stream));
+  if (mesh_out) *mesh_out = mi;
+  return WF_OK;
+}
+
+int wf_set_mesh_motion(wf_domain* d, int mesh, const double v[3], const double w[3],
+                       const double center[3]) {
+  if (!d) return fail(WF_E_ARG, "null domain");
+  if (mesh < 0 || mesh >= d->n_meshes) return fail(WF_E_ARG, "no such mesh");
+  MeshMotion& M = d->motions.m[mesh];
+  for (int c = 0; c < 3; c++) {
+    M.vel[c] = v ? v[c] : 0.0;
+    M.omega[c] = w ? w[c] : 0.0;
+    M.center[c] = center ? center[c] : 0.0;
+  }
+  return WF_OK;
+}
+
+int wf_mesh_info(wf_domain* d, int mesh, int* n_nodes, int* n_elems, int* zone_id) {
+  if (!d) return fail(WF_E_ARG, "null domain");
+  if (mesh == -1) {
+    if (n_nodes) *n_nodes = d->mesh.nn;
+    if (n_elems) *n_elems = d->mesh.ne;
+    if (zone_id) *zone_id = d->n_meshes;
+    return WF_OK;
+  }
+  if (mesh < 0 || mesh >= d->n_meshes) return fail(WF_E_ARG, "no such mesh");
+  if (n_nodes) *n_nodes = d->mesh_node_off[mesh + 1] - d->mesh_node_off[mesh];
+  if (n_elems) *n_elems = d->mesh_el_off[mesh + 1] - d->mesh_el_off[mesh];
+  if (zone_id) *zone_id = d->mesh_zone[mesh];
+  return WF_OK;
+}
+
+int wf_get_mesh(wf_domain* d, int mesh, double* nodes, double* node_v, double* centroids,
+                double* normals, double* pplane) {
+  if (!d) return fail(WF_E_ARG, "null domain");
+  if (mesh < 0 || mesh >= d->n_meshes) return fail(WF_E_ARG, "no such mesh");
+  HIPC(hipStreamSynchronize(d->stream));
+  const int n0 = d->mesh_node_off[mesh], nn = d->mesh_node_off[mesh + 1] - n0;
+  const int e0 = d->mesh_el_off[mesh], ne = d->mesh_el_off[mesh + 1] - e0;
+  std::vector<double> buf(std::max(nn, ne));
+  auto get3 = [&](double* out, double* const* planes, int off, int cnt) -> int {
+    if (!out) return WF_OK;
+    for (int c = 0; c < 3; c++) {
+      HIPC(hipMemcpy(buf.data(), planes[c] + off, sizeof(double) * cnt, hipMemcpyDeviceToHost));
+      for (int k = 0; k < cnt; k++) out[3 * k + c] = buf[k];
+    }
+    return WF_OK;
+  };
+  int rc = get3(nodes, d->mesh.x, n0, nn);
+  if (!rc) rc = get3(node_v, d->mesh.v, n0, nn);
+  if (!rc) rc = get3(centroids, d->mesh.c, e0, ne);
+  if (!rc) rc = get3(normals, d->mesh.nrm, e0, ne);
+  if (rc) return rc;
+  if (pplane) HIPC(hipMemcpy(pplane, d->mesh.pp + e0, sizeof(double) * ne, hipMemcpyDeviceToHost));
+  return WF_OK;
+}
+
+int wf_calc_surface(wf_domain* d) {
+  if (!d) return fail(WF_E_ARG, "null domain");
+  NEED_NB(d);
+  int rc = d->surface_init();
+  if (rc) return rc;
+  launch_surface_tiled(d->stream, d->f, d->tl, d->solid, (double)d->n + (double)d->mesh.ne,
+                       d->totmass, d->cfg.id_free_surf);
+  HIPC(hipGetLastError());
+  return WF_OK;
+}
+
+int wf_calc_contact_nb(wf_domain* d) {
+  if (!d) return fail(WF_E_ARG, "null domain");
+  if (d->n_meshes == 0) return fail(WF_E_STATE, "no rigid mesh (call wf_add_mesh)");
+  return d->contact_nb();
+}
+
+int wf_calc_contact_forces(wf_domain* d) {
+  if (!d) return fail(WF_E_ARG, "null domain");
+  if (!d->cont_built) return fail(WF_E_STATE, "contact neighbours not built");
+  launch_contact_force(d->stream, d->f, d->solid, d->mesh, d->cl, d->contact_params());
+  HIPC(hipGetLastError());
+  return WF_OK;
+}
+
+int wf_mesh_update(wf_domain* d, double dt) {
+  if (!d) return fail(WF_E_ARG, "null domain");
+  launch_mesh_move(d->stream, d->mesh, d->motions, dt);
+  HIPC(hipGetLastError());
+  return WF_OK;
+}
+
+int wf_get_contact_neighbors(wf_domain* d, int* counts, int* list) {
+  if (!d || !counts) return fail(WF_E_ARG, "null argument");
+  const int n = d->n;
+  if (!d->cl.count || !d->cont_built) {
+    std::memset(counts, 0, sizeof(int) * n);
+    if (list)
+      for (int64_t k = 0; k < (int64_t)n * kMaxContNb; k++) list[k] = -1;
+    return WF_OK;
+  }
+  std::vector<int> cnt(n), perm(n), lst((size_t)d->npad * kMaxContNb);
+  HIPC(hipMemcpyAsync(cnt.data(), d->cl.count, sizeof(int) * n, hipMemcpyDeviceToHost, d->stream));
+  HIPC(hipMemcpyAsync(perm.data(), d->perm, sizeof(int) * n, hipMemcpyDeviceToHost, d->stream));
+  HIPC(hipMemcpyAsync(lst.data(), d->cl.list, sizeof(int) * lst.size(), hipMemcpyDeviceToHost,
+                      d->stream));
+  HIPC(hipStreamSynchronize(d->stream));
+  for (int s = 0; s < n; s++) {
+    const int o = perm[s];
+    counts[o] = cnt[s];
+    if (list)
+      for (int k = 0; k < kMaxContNb; k++)
+        list[(int64_t)o * kMaxContNb + k] = k < cnt[s] ? lst[(size_t)k * d->npad + s] : -1;
+  }
+  return WF_OK;
+}
+
+int wf_contact_force_sum(wf_domain* d, double* sum) {
+  if (!d || !sum) return fail(WF_E_ARG, "null argument");
+  launch_contact_norm(d->stream, d->f, d->n, d->cfg.id_free_surf, d->spare_d);
+  const int nb = std::min(1024, std::max(1, gridn(d->n)));
+  launch_sum(d->stream, d->spare_d, d->n, d->sum_part, nb, d->sum_out);
+  HIPC(hipGetLastError());
+  HIPC(hipMemcpyAsync(sum, d->sum_out, sizeof(double), hipMemcpyDeviceToHost, d->stream));
+  HIPC(hipStreamSynchronize(d->stream));
+  return WF_OK;
+}
+
 // Domain_d::MechFraserSolve loop body, SolverFraser.cu:164-399
 int wf_step(wf_domain* d, int n_steps) {
   if (!d) return fail(WF_E_ARG, "null domain");
@@ -938,6 +1292,17 @@ int wf_step(wf_domain* d, int n_steps) {
       if (mx) launch_accel_mixed(d->stream, d->f, d->tl, d->solid, p.alpha, p.beta, cs32);
       else launch_accel_tiled(d->stream, d->f, d->tl, d->solid, p.alpha, p.beta);
       if ((rc = timer_end(d, WF_K_ACCEL))) return rc;
+      if (d->cfg.contact) {  // SolverFraser.cu:233-257
+        timer_begin(d);
+        if ((rc = d->surface_init())) return rc;
+        launch_surface_tiled(d->stream, d->f, d->tl, d->solid, (double)d->n + (double)d->mesh.ne,
+                             d->totmass, d->cfg.id_free_surf);
+        if (d->n_meshes > 0) {
+          if (d->ts_i == 0 && (rc = d->contact_nb())) return rc;
+          launch_contact_force(d->stream, d->f, d->solid, d->mesh, d->cl, d->contact_params());
+        }
+        if ((rc = timer_end(d, WF_K_CONTACT))) return rc;
+      }
       if (!d->is_yielding) {  // SolverFraser.cu:260-271, on the device-side max
         HIPC(hipMemcpyAsync(d->ctr_host, d->ctr, sizeof(Counters), hipMemcpyDeviceToHost,
                             d->stream));
@@ -964,6 +1329,11 @@ int wf_step(wf_domain* d, int n_steps) {
       }
       std::swap(d->f.T, d->f.T_alt);
       if ((rc = timer_end(d, WF_K_THERMAL))) return rc;
+    }
+    if (d->cfg.mechanics && d->cfg.contact && d->n_meshes > 0) {  // SolverFraser.cu:290-318
+      timer_begin(d);
+      launch_mesh_move(d->stream, d->mesh, d->motions, p.dt);
+      if ((rc = timer_end(d, WF_K_CONTACT))) return rc;
     }
     if (d->cfg.surrogate && d->use_queue && d->is_yielding && d->cfg.mlp_train_interval > 0 &&
         (d->step + 1) % d->cfg.mlp_train_interval == 0) {
@@ -996,6 +1366,7 @@ int wf_get_stats(wf_domain* d, wf_stats* out) {
   out->newton_iters_wave = (int64_t)d->ctr_host->newton_iters_wave;
   out->mlp_trainings = d->mlp_trainings;
   out->nb_over_cap = d->ctr_host->nb_over_cap;
+  out->cont_over_cap = d->ctr_host->cont_over_cap;
   out->mlp_loss = d->mlp_last_loss;
   out->mlp_samples = d->mlp_last_samples;
   return WF_OK;

@@ -1,4 +1,5 @@
-// wf_geometry.cpp — host-side lattice builders of the C-ABI (no device code).
+// wf_geometry.cpp — host-side lattice and plate-mesh builders of the C-ABI
+// (no device code).
 // The loops keep the reference's generation order and its floating-point
 // steps (positions by formula for the box, by accumulation for the cylinder)
 // so that counts and coordinates agree bit for bit with a reference run.
@@ -112,6 +113,44 @@ int wf_assign_zone(const double* xyz, int n, const double start[3], const double
     }
   }
   return hits;
+}
+
+// z-normal plate: nodes row by row (x fastest) with the spacing accumulated,
+// two triangles per quad in the clockwise order that gives normal -z
+// (TriMesh_d::AxisPlaneMesh(2, false, ...), Mesh.cu:48-194)
+int wf_gen_plane_mesh(const double p1[3], const double p2[3], int dens, double* nodes,
+                      int node_cap, int* elnode, int elem_cap, int* n_nodes) {
+  if (!p1 || !p2 || dens <= 0 || dens > 4096) return WF_E_ARG;
+  const int nn = (dens + 1) * (dens + 1), ne = 2 * dens * dens;
+  if (n_nodes) *n_nodes = nn;
+  const double dl = (p2[0] - p1[0]) / dens;
+  if (nodes && node_cap >= nn) {
+    int k = 0;
+    double x2 = p1[1];
+    for (int j = 0; j <= dens; j++) {
+      double x1 = p1[0];
+      for (int i = 0; i <= dens; i++) {
+        nodes[3 * k] = x1;
+        nodes[3 * k + 1] = x2;
+        nodes[3 * k + 2] = p1[2];
+        k++;
+        x1 += dl;
+      }
+      x2 += dl;
+    }
+  }
+  if (elnode && elem_cap >= ne) {
+    int e = 0;
+    for (int j = 0; j < dens; j++)
+      for (int i = 0; i < dens; i++) {
+        const int n0 = (dens + 1) * j + i, n1 = n0 + 1;
+        const int n2 = (dens + 1) * (j + 1) + i, n3 = n2 + 1;
+        const int tri[2][3] = {{n0, n2, n1}, {n1, n2, n3}};
+        for (int t = 0; t < 2; t++, e++)
+          for (int c = 0; c < 3; c++) elnode[3 * e + c] = tri[t][c];
+      }
+  }
+  return ne;
 }
 
 }  // extern "C"

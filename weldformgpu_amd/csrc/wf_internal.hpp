@@ -39,7 +39,68 @@ struct Fields {
   // step of the sample (-1: none)
   float *mlx[4], *mly;
   int* mlstep;
+  // contact: surface normal, contact force, not_write_surf_ID (ID != 0 at
+  // the first surface pass)
+  double *nrm[3], *cf[3];
+  int* idlock;
 };
+
+// ---- rigid triangle meshes (wf_contact.hip) --------------------------------
+struct Counters;
+constexpr int kMaxContNb = WF_MAX_CONT_NB;
+constexpr int kMaxMeshes = WF_MAX_MESHES;
+
+// all meshes concatenated: nodes (position, velocity, owning mesh),
+// connectivity with global node indices, per-element centroid, unit normal
+// and plane constant n . x_node0
+struct MeshDev {
+  double *x[3], *v[3];
+  int* node_mesh;
+  int* el;
+  double *c[3], *nrm[3], *pp;
+  int* cell;  // centroid grid cell of each element (contact candidate build)
+  int nn, ne;
+};
+struct MeshMotion {
+  double vel[3], omega[3], center[3];
+};
+struct MeshMotions {
+  MeshMotion m[kMaxMeshes];
+};
+// uniform grid over triangle centroids, cell edge >= 2 max h
+struct TriGrid {
+  double org[3];
+  double inv_cs;
+  int dim[3];
+};
+// contact candidates: entry k of particle i at list[k*npad + i]
+struct ContactLists {
+  int* count;
+  int* list;
+  int64_t npad;
+};
+struct ContactParams {
+  double dt, kappa_E_dx, mu_s, mu_d;
+  int law;
+};
+
+void launch_mesh_geom(hipStream_t s, const MeshDev& m);
+void launch_mesh_move(hipStream_t s, const MeshDev& m, const MeshMotions& mo, double dt);
+// centroid bounding box (6 doubles: min xyz, max xyz) and max of h over n
+void launch_tri_bbox(hipStream_t s, const MeshDev& m, double* out);
+void launch_max_f64(hipStream_t s, const double* v, int n, double* out);
+void launch_tri_bin(hipStream_t s, const MeshDev& m, const TriGrid& g, int* cnt, int* rank,
+                    int* start, int* sorted, int* scan_part);
+void launch_surface_lock(hipStream_t s, const Fields& f, int n);
+void launch_surface_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
+                          double scale_num, double totmass, int id_free_surf);
+void launch_contact_nb(hipStream_t s, const Fields& f, int solid, const MeshDev& m,
+                       const TriGrid& g, const int* start, const int* sorted,
+                       const ContactLists& cl, int id_free_surf, Counters* ctr);
+void launch_contact_force(hipStream_t s, const Fields& f, int solid, const MeshDev& m,
+                          const ContactLists& cl, const ContactParams& cp);
+// |cf_i| where ID_i == id, else 0
+void launch_contact_norm(hipStream_t s, const Fields& f, int n, int id, double* out);
 
 // Queue of yielding Johnson-Cook particles whose radial return is solved by
 // k_return (wf_return.hip): particle index, trial von Mises stress, rate x
@@ -111,6 +172,8 @@ struct Counters {
   unsigned long long newton_iters_wave;
   int nb_over_cap;
   int nonfinite;
+  int cont_over_cap;
+  int _pad;
 };
 
 // ---- LDS-tiled pair loops (wf_pair.hip) ----------------------------------

@@ -734,6 +734,56 @@ struct CondM {
   }
 };
 
+// ---------------------------------------------------------------------------
+// S: free-surface normal n_i = sum_j m_j x_ij * N/(M nb_i) and the surface
+// flag |n_i| >= 0.25 h_i with nb_i <= 46 (Geometry.cu:26-88).  Rigid
+// triangles are not particles here, so every list entry counts.
+struct SurfK {
+  static constexpr int NF = 4;  // x y z m
+  struct Prm {
+    int limit;
+    int id_free_surf;
+    double ntot, totmass;
+  };
+  WF_FP64_POLICY
+  struct Own {
+    double x, y, z;
+  };
+  struct Acc {
+    double nx, ny, nz;
+    int cnt;
+  };
+  __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx&) {
+    a.nx = a.ny = a.nz = 0.0;
+    a.cnt = 0;
+    if (!act) return;
+    o.x = f.x[0][i]; o.y = f.x[1][i]; o.z = f.x[2][i];
+  }
+  __device__ static void stage(const Fields& f, int j, double (*L)[kChunk64], int s,
+                               const Ctx&) {
+    L[0][s] = f.x[0][j]; L[1][s] = f.x[1][j]; L[2][s] = f.x[2][j];
+    L[3][s] = f.m[j];
+  }
+  __device__ static void pair(double (*L)[kChunk64], int s, const Own& o, Acc& a, const Prm&) {
+    const double mj = L[3][s];
+    a.nx += mj * (o.x - L[0][s]);
+    a.ny += mj * (o.y - L[1][s]);
+    a.nz += mj * (o.z - L[2][s]);
+    a.cnt++;
+  }
+  __device__ static void finish(const Fields& f, int i, bool act, const Own&, const Acc& a,
+                                const Prm& p) {
+    if (!act) return;
+    const double sc = p.ntot / (p.totmass * (double)a.cnt);
+    const double nx = a.nx * sc, ny = a.ny * sc, nz = a.nz * sc;
+    f.nrm[0][i] = nx;
+    f.nrm[1][i] = ny;
+    f.nrm[2][i] = nz;
+    if (sqrt(nx * nx + ny * ny + nz * nz) >= 0.25 * f.h[i] && a.cnt <= 46 && !f.idlock[i])
+      f.ID[i] = p.id_free_surf;
+  }
+};
+
 static inline int ntiles(int n) { return (n + kTileP - 1) / kTileP; }
 
 void launch_dens_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int n, int solid,
@@ -759,6 +809,12 @@ void launch_cond_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int 
   if (n <= 0) return;
   CondK::Prm p{n, T_inf, h_conv, dt, leap ? 1 : 0, first ? 1 : 0};
   hipLaunchKernelGGL(k_pair_tiled<CondK>, dim3(ntiles(n)), dim3(kTileP), 0, s, f, tl, p);
+}
+void launch_surface_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
+                          double ntot, double totmass, int id_free_surf) {
+  if (solid <= 0) return;
+  SurfK::Prm p{solid, id_free_surf, ntot, totmass};
+  hipLaunchKernelGGL(k_pair_tiled<SurfK>, dim3(ntiles(solid)), dim3(kTileP), 0, s, f, tl, p);
 }
 
 }  // namespace wf

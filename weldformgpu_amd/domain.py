@@ -200,3 +200,60 @@ class Domain:
         _check(self.lib, self.lib.wf_mlp_train(self.h, int(iters), float(lr), C.byref(ns),
                                                C.byref(loss)))
         return ns.value, loss.value
+
+    # -- rigid meshes and contact (Contact.cu, Geometry.cu, Mesh.cu) -----------
+    def add_mesh(self, nodes, elnode, zone_id):
+        """Upload a rigid triangle mesh; returns its index."""
+        nodes = np.ascontiguousarray(nodes, np.float64).reshape(-1, 3)
+        elnode = np.ascontiguousarray(elnode, np.int32).reshape(-1, 3)
+        mi = C.c_int()
+        _check(self.lib, self.lib.wf_add_mesh(self.h, nodes.ctypes.data, len(nodes),
+                                              elnode.ctypes.data, len(elnode), int(zone_id),
+                                              C.byref(mi)))
+        return mi.value
+
+    def set_mesh_motion(self, mesh, v=(0, 0, 0), w=(0, 0, 0), center=(0, 0, 0)):
+        v, w, c = (np.ascontiguousarray(a, np.float64) for a in (v, w, center))
+        _check(self.lib, self.lib.wf_set_mesh_motion(self.h, int(mesh), v.ctypes.data,
+                                                     w.ctypes.data, c.ctypes.data))
+
+    def mesh_info(self, mesh=-1):
+        nn, ne, z = C.c_int(), C.c_int(), C.c_int()
+        _check(self.lib, self.lib.wf_mesh_info(self.h, int(mesh), C.byref(nn), C.byref(ne),
+                                               C.byref(z)))
+        return nn.value, ne.value, z.value
+
+    def get_mesh(self, mesh):
+        nn, ne, _ = self.mesh_info(mesh)
+        out = {"nodes": np.zeros((nn, 3)), "node_v": np.zeros((nn, 3)),
+               "centroids": np.zeros((ne, 3)), "normals": np.zeros((ne, 3)),
+               "pplane": np.zeros(ne)}
+        _check(self.lib, self.lib.wf_get_mesh(
+            self.h, int(mesh), out["nodes"].ctypes.data, out["node_v"].ctypes.data,
+            out["centroids"].ctypes.data, out["normals"].ctypes.data, out["pplane"].ctypes.data))
+        return out
+
+    def CalculateSurface(self):
+        _check(self.lib, self.lib.wf_calc_surface(self.h))
+
+    def CalcContactNb(self):
+        _check(self.lib, self.lib.wf_calc_contact_nb(self.h))
+
+    def CalcContactForces(self):
+        _check(self.lib, self.lib.wf_calc_contact_forces(self.h))
+
+    def MeshUpdate(self, dt):
+        _check(self.lib, self.lib.wf_mesh_update(self.h, float(dt)))
+
+    def contact_neighbors(self):
+        """(counts[n], list[n, 20]) in original particle order, -1 padded."""
+        cnt = np.zeros(self.n, np.int32)
+        lst = np.zeros((self.n, _abi.WF_MAX_CONT_NB), np.int32)
+        _check(self.lib, self.lib.wf_get_contact_neighbors(self.h, cnt.ctypes.data,
+                                                           lst.ctypes.data))
+        return cnt, lst
+
+    def contact_force_sum(self):
+        s = C.c_double()
+        _check(self.lib, self.lib.wf_contact_force_sum(self.h, C.byref(s)))
+        return s.value

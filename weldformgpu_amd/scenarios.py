@@ -25,6 +25,9 @@ class Scenario:
     fields: dict
     bcs: list = field(default_factory=list)
     meta: dict = field(default_factory=dict)
+    # rigid meshes: dicts with nodes (nn,3), elnode (ne,3), zone, and the
+    # rigid motion vel, omega, center
+    meshes: list = field(default_factory=list)
 
     @property
     def n(self):
@@ -40,6 +43,8 @@ def _lib():
     lib.wf_gen_cylinder.argtypes = [P, D, D, D, D, P, I, P]
     lib.wf_assign_zone.restype = I
     lib.wf_assign_zone.argtypes = [P, I, P, P, I, P]
+    lib.wf_gen_plane_mesh.restype = I
+    lib.wf_gen_plane_mesh.argtypes = [P, P, I, P, I, P, I, C.POINTER(I)]
     return lib
 
 
@@ -61,6 +66,23 @@ def gen_box(start, L, r, density):
     m = C.c_double()
     lib.wf_gen_box(V.ctypes.data, L[0], L[1], L[2], r, density, xyz.ctypes.data, n, C.byref(m))
     return xyz, m.value
+
+
+def gen_plane_mesh(p1, p2, dens):
+    """z-normal plate of 2 dens^2 triangles (AxisPlaneMesh): nodes, elnode."""
+    lib = _lib()
+    a = np.ascontiguousarray(p1, np.float64)
+    b = np.ascontiguousarray(p2, np.float64)
+    nn = C.c_int()
+    ne = lib.wf_gen_plane_mesh(a.ctypes.data, b.ctypes.data, int(dens), None, 0, None, 0,
+                               C.byref(nn))
+    if ne < 0:
+        raise ValueError("bad plate mesh arguments")
+    nodes = np.zeros((nn.value, 3))
+    el = np.zeros((ne, 3), np.int32)
+    lib.wf_gen_plane_mesh(a.ctypes.data, b.ctypes.data, int(dens), nodes.ctypes.data, nn.value,
+                          el.ctypes.data, ne, C.byref(nn))
+    return nodes, el
 
 
 def assign_zone(xyz, ID, start, end, zid):
@@ -152,6 +174,40 @@ def c3_johnson_cook(scale=1.0, r=0.15e-3):
     return sc
 
 
+def c4_contact(scale=1.0, r=0.125e-3, dens=64, law=_abi.WF_CONTACT_STIFFNESS, mu=0.2,
+               plate=40e-3, offset=0.0, thermal=True):
+    """Johnson-Cook cylinder (R 15 mm, H 30 mm; scale multiplies R, H and the
+    plate) compressed by a rigid plate of plate x plate with 2 dens^2
+    triangles moving at -2 m/s, starting one h above the top layer; bottom
+    two layers fixed.  Penalty law `law` (stiffness 0.1 E dx or the
+    reference 0.4 * 2 m/dt^2), Coulomb friction mu (static = dynamic).
+    `offset` shifts the plate in x and y by that fraction of its spacing."""
+    sc = c3_johnson_cook(scale=scale, r=r)
+    sc.name = "c4_jc_plate_contact"
+    if not thermal:
+        sc.cfg.thermal = 0
+        sc.cfg.taylor_quinney = 0.0
+    sc.bcs = [b for b in sc.bcs if b[0] != 3]  # the plate drives the top
+    ID = sc.fields["ID"]
+    ID[ID == 3] = 0
+    cfg = sc.cfg
+    cfg.contact = 1
+    cfg.contact_law = law
+    cfg.contact_kappa = 0.1
+    cfg.contact_dx = sc.meta["dx"]
+    cfg.fric_sta = mu
+    cfg.fric_dyn = mu
+    L = plate * scale
+    zt = sc.fields["x"][:, 2].max() + sc.meta["h"]
+    sh = offset * L / dens
+    p1 = np.array([-L / 2 + sh, -L / 2 + sh, zt])
+    nodes, el = gen_plane_mesh(p1, p1 + np.array([L, L, 0.0]), dens)
+    sc.meshes = [{"nodes": nodes, "elnode": el, "zone": 10, "vel": (0.0, 0.0, -2.0),
+                  "omega": (0.0, 0.0, 0.0), "center": (0.0, 0.0, 0.0)}]
+    sc.meta.update({"plate": L, "dens": dens, "plate_z": zt})
+    return sc
+
+
 def c2_thermal_cube(L=46e-3, r=0.5e-3, seed=42):
     """Conduction only: cube of edge L, dx = 2r, rho 2700, cp 900, k 200,
     T = 300 + 500 exp(-|x-c|^2/(2 sigma^2)) + U(-1,1), sigma 5 mm; the first
@@ -191,3 +247,7 @@ def load_domain(dom, sc):
     for k, v in sc.fields.items():
         dom.set(k, v)
     dom.set_bcs(sc.bcs)
+    for m in sc.meshes:
+        mi = dom.add_mesh(m["nodes"], m["elnode"], m.get("zone", 0))
+        dom.set_mesh_motion(mi, m.get("vel", (0, 0, 0)), m.get("omega", (0, 0, 0)),
+                            m.get("center", (0, 0, 0)))
