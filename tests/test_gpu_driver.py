@@ -1,0 +1,135 @@
+"""The host driver end to end on the GPU: deck → domain → Fraser loop → CSV
+snapshots, against the FP64 oracle run of the same setup built here from the
+deck's numbers.  CSV values carry 7 significant digits (%.6e), so fields are
+compared at 2e-6 relative (floor 1e-12 of the field's scale)."""
+import json
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from weldformgpu_amd import _abi, scenarios
+from weldformgpu_amd import build as wbuild
+
+pytestmark = pytest.mark.gpu
+
+DECKS = os.path.join(os.path.dirname(__file__), "golden", "decks")
+HEADER = ("ID, X, Y, Z, posX,posY,posZ,Ux, Uy, Uz, Vx, Vy, Vz, Ax, Ay, Az, rho, p, SigmaEq, "
+          "Pl_Strain, Nb, ContNb, CFx, CFy, CFz, Nx, Ny, Nz, h")
+
+
+def _read_csv(path):
+    with open(path) as f:
+        assert f.readline().rstrip("\n") == HEADER
+    return np.loadtxt(path, delimiter=",", skiprows=1, ndmin=2)
+
+
+def _oracle_from_deck(po, dk):
+    """The setup WeldFormGPU.cu builds from the deck, restated in Python."""
+    c, m, blk = dk["Configuration"], dk["Materials"][0], dk["DomainBlocks"][0]
+    r, rho, E, nu = c["particleRadius"], m["density0"], m["youngsModulus"], m["poissonsRatio"]
+    h = 2 * r * c["hFactor"]
+    K, G = E / (3 * (1 - 2 * nu)), E / (2 * (1 + nu))
+    Cs = math.sqrt(K / rho)
+    xyz, mass = scenarios.gen_cylinder(blk["start"], blk["dim"][0] / 2, blk["dim"][2], r, rho)
+    n = len(xyz)
+    ID = np.zeros(n, np.int32)
+    for z in dk["DomainZones"]:
+        scenarios.assign_zone(xyz, ID, z["start"], z["end"], z["id"])
+    cfg = _abi.default_config()
+    cfg.n_particles = cfg.solid_count = n
+    cfg.h_glob = h
+    cfg.dt = 0.4 * h / (Cs + 10.0)
+    cfg.mat.E, cfg.mat.nu, cfg.mat.sy0 = E, nu, m["yieldStress0"]
+    ext = dk.get("Extensions", {})
+    T0 = 0.0
+    if m["type"] == "Bilinear":
+        cfg.mat.model, cfg.mat.Et = _abi.WF_MAT_BILINEAR, m["const"][0]
+    else:
+        cfg.mat.model = _abi.WF_MAT_JOHNSON_COOK
+        (cfg.mat.A, cfg.mat.B, cfg.mat.n, cfg.mat.C, cfg.mat.eps0, cfg.mat.m, cfg.mat.Tm,
+         cfg.mat.Tt) = m["const"]
+        T0 = ext.get("thermal", {}).get("T0", cfg.mat.Tt)
+        th = min(max((T0 - cfg.mat.Tt) / (cfg.mat.Tm - cfg.mat.Tt), 0.0), 1.0)
+        cfg.mat.sy0 = cfg.mat.A * (1 + cfg.mat.C * math.log(1e-5 / cfg.mat.eps0)) * (
+            1 - th ** cfg.mat.m)
+    if ext.get("returnMap") == "newton":
+        cfg.return_map = _abi.WF_RETURN_NEWTON
+    st = po.State(n)
+    st["x"] = xyz
+    st["xs"] = xyz
+    for k, v in {"rho": rho, "rho0": rho, "h": h, "m": mass, "Cs": Cs, "G": G,
+                 "sigma_y": cfg.mat.sy0, "T": T0}.items():
+        st[k] = np.full(n, v)
+    if "thermal" in ext:
+        t = ext["thermal"]
+        cfg.thermal, cfg.taylor_quinney = 1, t["taylorQuinney"]
+        st["k_T"] = np.full(n, t["k"])
+        st["cp_T"] = np.full(n, t["cp"])
+    st["ID"] = ID
+    bcs = [(b["zoneId"], tuple(b["value"])) for b in dk["BoundaryConditions"]]
+    meshes = []
+    for rb in dk.get("RigidBodies", []):
+        p1 = np.array(rb["start"], float)
+        p2 = p1 + np.array([rb["dim"][0], rb["dim"][1], 0.0])
+        nodes, el = po.gen_plane_mesh(p1, p2, rb["partSide"])
+        vel = [b[1] for b in bcs if b[0] == rb["zoneId"]][0]
+        meshes.append({"nodes": nodes, "elnode": el, "vel": vel})
+    if meshes:
+        cfg.contact = 1
+        ct = dk["Contact"][0]
+        cfg.fric_sta, cfg.fric_dyn = ct["fricCoeffStatic"], ct["fricCoeffDynamic"]
+        st.add_contact(meshes)
+    return st, cfg, bcs
+
+
+@pytest.mark.parametrize("deck", ["compression_mini.json", "contact_mini.json"])
+def test_driver_run_matches_oracle(oracle, tmp_path, deck):
+    path = os.path.join(DECKS, deck)
+    with open(path) as f:
+        dk = json.load(f)
+    print(f"[case] driver {deck}", flush=True)
+    r = subprocess.run([wbuild.DRIVER, path, "--out", str(tmp_path)], capture_output=True,
+                       text=True, timeout=120)
+    print(r.stdout[-2000:])
+    assert r.returncode == 0, r.stderr
+    st, cfg, bcs = _oracle_from_deck(oracle, dk)
+    st.nsearch(cfg)
+    # the loop: steps until Time >= simTime with Time accumulated step by step
+    t, steps = 0.0, 0
+    while t < dk["Configuration"]["simTime"]:
+        t += cfg.dt
+        steps += 1
+    st.step(cfg, bcs, steps)
+    outs = sorted(p for p in os.listdir(tmp_path) if p.startswith("out_"))
+    n_out = int(round(dk["Configuration"]["simTime"] / dk["Configuration"]["outTime"]))
+    assert len(outs) == n_out + 1
+    got = _read_csv(tmp_path / "test.csv")
+    n = st.n
+    ne = sum(len(m) for m in ([st.mesh_arrays["el"]] if st.st.contact else []))
+    assert got.shape[0] == n + ne
+    P = got[:n]
+
+    def close(col, ref, name):
+        ref = np.asarray(ref, float)
+        scale = max(np.abs(ref).max(), 1e-300)
+        err = np.abs(P[:, col] - ref.reshape(P[:, col].shape)) / np.maximum(np.abs(ref.reshape(
+            P[:, col].shape)), 1e-6 * scale)
+        assert err.max() < 2e-6, (name, err.max())
+
+    assert np.array_equal(P[:, 0].astype(int), st["ID"])
+    close(slice(1, 4), st["x"], "x")
+    close(slice(7, 10), st["u"], "u")
+    close(slice(10, 13), st["v"], "v")
+    close(18, st["sigma_eq"], "sigma_eq")
+    close(19, st["pl_strain"], "pl_strain")
+    assert np.array_equal(P[:, 20].astype(int), st["nb_count"])
+    if st.st.contact:
+        close(slice(22, 25), st.contact_arrays["cf"], "contforce")
+        assert np.abs(st.contact_arrays["cf"]).max() > 0
+        R = got[n:]
+        assert np.all(R[:, 0] == dk["RigidBodies"][0]["zoneId"])
+        cen = st.mesh_arrays["c"]
+        assert np.allclose(R[:, 1:4], cen, rtol=2e-6, atol=1e-12)

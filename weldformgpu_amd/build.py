@@ -1,8 +1,10 @@
-"""Build libwfhip.so (gfx950) in-tree with hipcc.
+"""Build libwfhip.so (gfx950) in-tree with hipcc, and the host driver.
 
 Every translation unit is compiled for gfx950 only, with packed-FP32
 instruction selection switched off (-Xclang -target-feature
 -Xclang -packed-fp32-ops), then linked into weldformgpu_amd/libwfhip.so.
+The command-line driver (driver/wf_driver.cpp, host C++ over the C-ABI) is
+built with g++ into weldformgpu_amd/wfdriver, linked to libwfhip.so.
 """
 import concurrent.futures as cf
 import os
@@ -14,6 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libwfhip.so")
+DRIVER = os.path.join(HERE, "wfdriver")
 SOURCES = ["wf_kernels.hip", "wf_pair.hip", "wf_tiles.hip", "wf_nsearch.hip", "wf_return.hip",
            "wf_mlp.hip", "wf_contact.hip", "wf_domain.hip", "wf_geometry.cpp"]
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -48,7 +51,19 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_driver()
     return LIB
+
+
+def build_driver():
+    tmp = DRIVER + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", os.path.join(HERE, "driver", "wf_driver.cpp"),
+           "-o", tmp, f"-L{HERE}", "-lwfhip", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":
