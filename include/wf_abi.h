@@ -328,6 +328,14 @@ int wf_mlp_train(struct wf_domain* d, int iters, float lr, int64_t* samples, flo
  * (3*2*dens^2) when the caps allow; returns the element count.             */
 int wf_gen_plane_mesh(const double p1[3], const double p2[3], int dens, double* nodes,
                       int node_cap, int* elnode, int elem_cap, int* n_nodes);
+/* Nastran small-field triangle mesh: NastranReader::read + ReadFromNastran
+ * (NastranReader.cpp:5-153, Mesh_reader.cu:5-60).  GRID cards (id, x, y, z
+ * in 8-character fields 2 and 4-6, exponent letter optional: "1.5-3") and
+ * CTRIA3 cards (node ids in fields 4-6) -> nodes (3*n_nodes), elnode
+ * (0-based, 3 per triangle); flip reverses each triangle's orientation.
+ * Host only.  Returns the triangle count or a negative WF_E_* code.        */
+int wf_read_nastran(const char* path, int flip, double* nodes, int node_cap, int* elnode,
+                    int elem_cap, int* n_nodes);
 /* Upload a rigid mesh (WeldFormGPU.cu:495-575): nodes xyz interleaved,
  * elnode 3 per triangle (0-based, into this mesh's nodes).  zone_id is the
  * mesh's ID (contact_surf_id).  Centroids, normals (n0n1 x n0n2, unit) and

@@ -81,6 +81,6 @@ def test_bad_decks_fail_with_a_message(tmp_path):
     p.write_text(json.dumps(d))
     assert "not supported" in _dry(str(p), expect_rc=2)
     d = _deck("contact_mini.json")
-    d["RigidBodies"][0]["type"] = "File"
+    d["RigidBodies"][0]["type"] = "Sphere"
     p.write_text(json.dumps(d))
     assert "not supported" in _dry(str(p), expect_rc=2)

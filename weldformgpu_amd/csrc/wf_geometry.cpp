@@ -4,6 +4,11 @@
 // steps (positions by formula for the box, by accumulation for the cylinder)
 // so that counts and coordinates agree bit for bit with a reference run.
 #include <cmath>
+#include <cstdlib>
+#include <fstream>
+#include <map>
+#include <string>
+#include <vector>
 
 #include "../../include/wf_abi.h"
 
@@ -31,6 +36,28 @@ inline void put(double* xyz, int cap, int k, double x, double y, double z) {
     xyz[3 * k + 1] = y;
     xyz[3 * k + 2] = z;
   }
+}
+
+// Nastran small-field real: 8 characters, exponent letter optional
+// ("1.5-3" = 1.5e-3, "-2.5+2" = -250); a leading sign is the number's sign
+double nastran_real(std::string t) {
+  size_t b = t.find_first_not_of(' ');
+  if (b == std::string::npos) return 0.0;
+  t = t.substr(b);
+  t = t.substr(0, t.find_last_not_of(' ') + 1);
+  if (t.find_first_of("eEdD") == std::string::npos) {
+    const size_t k = t.find_first_of("+-", 1);
+    if (k != std::string::npos) t.insert(k, "E");
+  } else {
+    for (char& ch : t)
+      if (ch == 'd' || ch == 'D') ch = 'E';
+  }
+  return std::strtod(t.c_str(), nullptr);
+}
+
+std::string field(const std::string& line, int k) {  // 8-character field k (0-based)
+  const size_t p = 8 * (size_t)k;
+  return p < line.size() ? line.substr(p, 8) : std::string();
 }
 
 }  // namespace
@@ -113,6 +140,48 @@ int wf_assign_zone(const double* xyz, int n, const double start[3], const double
     }
   }
   return hits;
+}
+
+// NastranReader::read + TriMesh_d::ReadFromNastran (NastranReader.cpp:5-153,
+// Mesh_reader.cu:5-60): GRID cards give node id (field 2) and x, y, z
+// (fields 4-6); CTRIA3 cards give three node ids (fields 4-6), mapped to
+// 0-based node order.  flip reverses each triangle (normal -n).  Returns the
+// triangle count, or a negative WF_E_* code.
+int wf_read_nastran(const char* path, int flip, double* nodes, int node_cap, int* elnode,
+                    int elem_cap, int* n_nodes) {
+  if (!path) return WF_E_ARG;
+  std::ifstream in(path);
+  if (!in) return WF_E_ARG;
+  std::vector<double> xyz;
+  std::vector<int> ids;
+  std::vector<int> tri;
+  std::map<int, int> pos;
+  std::string line;
+  while (std::getline(in, line)) {
+    if (!line.empty() && line.back() == '\r') line.pop_back();
+    if (line.compare(0, 4, "GRID") == 0 && line.compare(0, 5, "GRID*") != 0) {
+      const int id = std::atoi(field(line, 1).c_str());
+      pos[id] = (int)ids.size();
+      ids.push_back(id);
+      for (int c = 0; c < 3; c++) xyz.push_back(nastran_real(field(line, 3 + c)));
+    } else if (line.compare(0, 6, "CTRIA3") == 0) {
+      for (int c = 0; c < 3; c++) tri.push_back(std::atoi(field(line, 3 + c).c_str()));
+    }
+  }
+  const int nn = (int)ids.size(), ne = (int)(tri.size() / 3);
+  if (n_nodes) *n_nodes = nn;
+  for (int& t : tri) {
+    auto it = pos.find(t);
+    if (it == pos.end()) return WF_E_ARG;  // element names an unknown node
+    t = it->second;
+  }
+  if (flip)
+    for (int e = 0; e < ne; e++) std::swap(tri[3 * e + 1], tri[3 * e + 2]);
+  if (nodes && node_cap >= nn)
+    for (size_t k = 0; k < xyz.size(); k++) nodes[k] = xyz[k];
+  if (elnode && elem_cap >= ne)
+    for (size_t k = 0; k < tri.size(); k++) elnode[k] = tri[k];
+  return ne;
 }
 
 // z-normal plate: nodes row by row (x fastest) with the spacing accumulated,

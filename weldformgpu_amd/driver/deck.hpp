@@ -6,6 +6,7 @@
 #pragma once
 
 #include <cmath>
+#include <fstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -24,8 +25,8 @@ struct Bc {
   double value[3] = {0, 0, 0}, value_ang[3] = {0, 0, 0};
 };
 struct Rigid {
-  std::string type;
-  double start[3] = {0, 0, 0}, dim[3] = {0, 0, 0};
+  std::string type, file;
+  double start[3] = {0, 0, 0}, dim[3] = {0, 0, 0}, move[3] = {0, 0, 0};
   int dens = 10, zone = 0;
 };
 
@@ -51,6 +52,7 @@ struct Deck {
   bool thermal = false;
   double T0 = -1.0, k_T = 0.0, cp_T = 0.0, taylor_quinney = 0.0, T_inf = 500.0, h_conv = 100.0;
   long max_steps = -1;
+  std::string dir;  // the deck's directory, for relative mesh file names
 };
 
 inline Deck parse_deck(const wfj::Value& j) {
@@ -99,6 +101,8 @@ inline Deck parse_deck(const wfj::Value& j) {
     wfj::read3(rb["dim"], r.dim);
     read(rb["partSide"], r.dens);
     read(rb["zoneId"], r.zone);
+    read(rb["fileName"], r.file);
+    wfj::read3(rb["moveDir"], r.move);
     d.rigid.push_back(r);
   }
   const wfj::Value& ct = j["Contact"][0];
@@ -132,7 +136,8 @@ inline Deck parse_deck(const wfj::Value& j) {
   if (d.dom_type == "Cylinder" && d.grid_cs != "Cartesian")
     throw std::runtime_error("only the Cartesian cylinder lattice is supported");
   for (const Rigid& r : d.rigid)
-    if (r.type != "Plane") throw std::runtime_error("RigidBodies type " + r.type + " is not supported (Plane)");
+    if (r.type != "Plane" && r.type != "File")
+      throw std::runtime_error("RigidBodies type " + r.type + " is not supported (Plane, File)");
   return d;
 }
 
@@ -240,16 +245,30 @@ inline Setup build_setup(const Deck& d) {
   if (s.bc_zone.size() > WF_MAX_BCS) throw std::runtime_error("too many boundary conditions");
   // rigid plates: AxisPlaneMesh(2, false, start, start + (dim.x, dim.y, 0), partSide);
   // a constant-velocity BC on the mesh's zone moves it (WeldFormGPU.cu:597-611)
+  // "File": a Nastran triangle mesh translated by moveDir (WeldFormGPU.cu:546-558;
+  // flipNormals is read there but not passed on, so the file's orientation holds)
   for (const Rigid& r : d.rigid) {
     Mesh me;
     me.zone = r.zone;
-    const double p2[3] = {r.start[0] + r.dim[0], r.start[1] + r.dim[1], r.start[2]};
-    int nn = 0;
-    const int ne = wf_gen_plane_mesh(r.start, p2, r.dens, nullptr, 0, nullptr, 0, &nn);
-    if (ne <= 0) throw std::runtime_error("bad RigidBodies plate");
-    me.nodes.resize(3 * (size_t)nn);
-    me.el.resize(3 * (size_t)ne);
-    wf_gen_plane_mesh(r.start, p2, r.dens, me.nodes.data(), nn, me.el.data(), ne, &nn);
+    int nn = 0, ne = 0;
+    if (r.type == "Plane") {
+      const double p2[3] = {r.start[0] + r.dim[0], r.start[1] + r.dim[1], r.start[2]};
+      ne = wf_gen_plane_mesh(r.start, p2, r.dens, nullptr, 0, nullptr, 0, &nn);
+      if (ne <= 0) throw std::runtime_error("bad RigidBodies plate");
+      me.nodes.resize(3 * (size_t)nn);
+      me.el.resize(3 * (size_t)ne);
+      wf_gen_plane_mesh(r.start, p2, r.dens, me.nodes.data(), nn, me.el.data(), ne, &nn);
+    } else {
+      std::string path = r.file;
+      if (!std::ifstream(path) && !d.dir.empty()) path = d.dir + "/" + r.file;
+      ne = wf_read_nastran(path.c_str(), 0, nullptr, 0, nullptr, 0, &nn);
+      if (ne <= 0) throw std::runtime_error("cannot read the rigid mesh file " + r.file);
+      me.nodes.resize(3 * (size_t)nn);
+      me.el.resize(3 * (size_t)ne);
+      wf_read_nastran(path.c_str(), 0, me.nodes.data(), nn, me.el.data(), ne, &nn);
+      for (int q = 0; q < nn; q++)
+        for (int k = 0; k < 3; k++) me.nodes[3 * q + k] += r.move[k];
+    }
     for (const Bc& b : d.bcs)
       if (b.zone == r.zone && b.value_type == 0)
         for (int k = 0; k < 3; k++) {

@@ -163,6 +163,8 @@ int main(int argc, char** argv) {
   wfd::Setup s;
   try {
     dk = wfd::parse_deck(wfj::parse(ss.str()));
+    const size_t slash = deck_path.find_last_of('/');
+    dk.dir = slash == std::string::npos ? "." : deck_path.substr(0, slash);
     s = wfd::build_setup(dk);
   } catch (const std::exception& e) {
     std::fprintf(stderr, "wfdriver: %s\n", e.what());

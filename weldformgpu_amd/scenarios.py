@@ -45,6 +45,8 @@ def _lib():
     lib.wf_assign_zone.argtypes = [P, I, P, P, I, P]
     lib.wf_gen_plane_mesh.restype = I
     lib.wf_gen_plane_mesh.argtypes = [P, P, I, P, I, P, I, C.POINTER(I)]
+    lib.wf_read_nastran.restype = I
+    lib.wf_read_nastran.argtypes = [C.c_char_p, I, P, I, P, I, C.POINTER(I)]
     return lib
 
 
@@ -82,6 +84,21 @@ def gen_plane_mesh(p1, p2, dens):
     el = np.zeros((ne, 3), np.int32)
     lib.wf_gen_plane_mesh(a.ctypes.data, b.ctypes.data, int(dens), nodes.ctypes.data, nn.value,
                           el.ctypes.data, ne, C.byref(nn))
+    return nodes, el
+
+
+def read_nastran(path, flip=False):
+    """GRID/CTRIA3 triangle mesh of a Nastran small-field file: nodes, elnode."""
+    lib = _lib()
+    nn = C.c_int()
+    bp = str(path).encode()
+    ne = lib.wf_read_nastran(bp, int(flip), None, 0, None, 0, C.byref(nn))
+    if ne < 0:
+        raise ValueError(f"cannot read Nastran mesh {path}")
+    nodes = np.zeros((nn.value, 3))
+    el = np.zeros((ne, 3), np.int32)
+    lib.wf_read_nastran(bp, int(flip), nodes.ctypes.data, nn.value, el.ctypes.data, ne,
+                        C.byref(nn))
     return nodes, el
 
 
