@@ -13,6 +13,11 @@ tensors, pressure + stress + return map + energy, acceleration,
 BC + integration, conduction + leapfrog, neighbour rebuild every 5 steps
 once yielding), all inputs resident in HBM.
 
+--workload c4 runs the rigid-plate contact workload instead: the same JC
+cylinder at dx 0.25 mm (1,331,520 particles) compressed by a rigid 40 x 40 mm
+plate of 2 x 64 x 64 triangles at -2 m/s, penalty stiffness 0.1 E/dx,
+Coulomb friction 0.2, the online-trained surrogate in the mixed run.
+
 Prints ONE JSON line: value = particle-updates/s (particles x steps / s),
 plus the HBM roofline of the dominant kernel (HIP-event durations on the
 library's stream x algorithmic bytes) and the FP64 CPU oracle timed on the
@@ -46,6 +51,8 @@ def cpu_baseline(sc, warm=1, steps=2):
     for k, v in sc.fields.items():
         st[k] = v
     st["xs"] = sc.fields["x"]
+    if sc.meshes:
+        st.add_contact(sc.meshes)
     st.nsearch(sc.cfg)
     st.step(sc.cfg, sc.bcs, warm)
     t0 = time.perf_counter()
@@ -58,6 +65,32 @@ def cpu_baseline(sc, warm=1, steps=2):
                       f"{warm} warm-up step(s), FP64 oracle, OpenMP"}
 
 
+def profiled_traffic(workload, kernel_id):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 summary of
+    this workload (FETCH_SIZE + WRITE_SIZE, each scaled by what the counter
+    reads for a copy kernel of known size), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_jc_mixed_step_kernels_traffic.json")
+    try:
+        with open(path) as fh:
+            k = json.load(fh)["kernels"].get(kernel_id)
+    except (OSError, ValueError, KeyError):
+        return None
+    return None if k is None else k["fetch_bytes"] + k["write_bytes"]
+
+
+def workload_text(sc, n):
+    common = ("mixed precision (FP32 compensated pair loops, FP64 stress/Newton return "
+              "map/integration) with the online-trained MLP Newton guess (BF16 MFMA), "
+              "conduction + Taylor-Quinney 0.9")
+    if sc.meshes:
+        ne = sum(len(m["elnode"]) for m in sc.meshes)
+        return (f"{sc.name}: JC cylinder R15mm H30mm dx0.25mm, {n} particles, rigid plate "
+                f"40x40mm of {ne} triangles at -2 m/s, penalty 0.1 E/dx, Coulomb 0.2, "
+                f"{common}; compared with plain FP64")
+    return (f"{sc.name}: JC thermo-mechanical cylinder R15mm H30mm dx0.3mm, {n} particles, "
+            f"{common}, top at -2 m/s; compared with plain FP64")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +100,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--workload", choices=["c3", "c4"], default="c3")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path is "
@@ -76,8 +110,13 @@ def main():
     from weldformgpu_amd import _abi, scenarios
     from weldformgpu_amd.domain import Domain
 
+    def make():
+        if args.workload == "c4":
+            return scenarios.c4_contact(scale=args.scale)
+        return scenarios.c3_johnson_cook(scale=args.scale)
+
     def run(precision, timing_steps, surrogate):
-        sc = scenarios.c3_johnson_cook(scale=args.scale)
+        sc = make()
         sc.cfg.precision = precision
         sc.cfg.surrogate = surrogate
         dom = Domain(sc.cfg)
@@ -101,7 +140,7 @@ def main():
                 avg = ms / cnt
                 kern[k] = {"avg_ms": avg, "launches": cnt, "alg_bytes": b * sc.n,
                            "GBs": b * sc.n / (avg * 1e-3) / 1e9}
-        for k in ("return", "mlp", "nsearch"):
+        for k in ("return", "mlp", "nsearch", "contact"):
             ms, cnt = dom.kernel_time(k)
             if cnt:
                 kern[k] = {"avg_ms": ms / cnt, "launches": cnt}
@@ -139,7 +178,8 @@ def main():
     dom_k = max((k for k in kern if k in ALG_BYTES_MIXED), key=lambda k: kern[k]["avg_ms"])
     dk = kern[dom_k]
     roof = {"kernel": dom_k, "bound": "hbm", "achieved": dk["GBs"], "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": dk["GBs"] / HBM_PEAK_GBS, "traffic": None,
+            "unit": "GB/s", "frac": dk["GBs"] / HBM_PEAK_GBS,
+            "traffic": profiled_traffic(args.workload, "mixed/" + dom_k),
             "alg_bytes_per_particle": ALG_BYTES_MIXED[dom_k], "avg_ms": dk["avg_ms"]}
 
     cpu = None
@@ -152,11 +192,7 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"{sc.name}: JC thermo-mechanical cylinder R15mm H30mm "
-                               f"dx0.3mm, {n} particles, mixed precision (FP32 compensated "
-                               "pair loops, FP64 stress/Newton return map/integration) with the "
-                               "online-trained MLP Newton guess (BF16 MFMA), conduction + "
-                               "Taylor-Quinney 0.9, top at -2 m/s; compared with plain FP64",
+        "config": {"workload": workload_text(sc, n),
                    "particles": n, "precision": "mixed", "surrogate": True,
                    "parallelism": "single-gpu"},
         "fp64": {"value": rate64, "ms_per_step": r64["secs"] * 1e3 / args.steps,
