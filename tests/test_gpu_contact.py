@@ -133,26 +133,31 @@ def test_contact_forces_single_call(oracle, wflib, law, mu):
 
 @pytest.mark.parametrize("law", [_abi.WF_CONTACT_REFERENCE, _abi.WF_CONTACT_STIFFNESS])
 def test_steps_plate_compression(oracle, wflib, law):
-    sc = _plate_case(law=law, mu=0.2, seed=5)
+    """The plate starts one h above the top layer and closes in at 2 m/s; the
+    fields after each block of steps, and the contact force where the plate
+    touches, match the oracle."""
+    sc = _plate_case(law=law, mu=0.2, sink=0.0, seed=5)
     _case(f"steps law={law}", sc)
     st = to_oracle(oracle, sc)
     st.nsearch(sc.cfg)
-    nsteps = 30
-    st.step(sc.cfg, sc.bcs, nsteps)
+    touched = 0
     with to_gpu(sc, wflib) as d:
-        d.step(nsteps)
-        got = {k: d.get(k) for k in ["u", "v", "sigma", "pl_strain", "T", "contforce", "ID"]}
+        for _ in range(6):
+            st.step(sc.cfg, sc.bcs, 5)
+            d.step(5)
+            cf = st.contact_arrays["cf"]
+            touched += np.count_nonzero(np.linalg.norm(cf, axis=1))
+            assert rel_l2(d.get("contforce"), cf) < STEP_TOL
+            for k in ["u", "v", "sigma", "pl_strain", "T"]:
+                assert rel_l2(d.get(k), st[k]) < STEP_TOL, k
+        assert np.array_equal(d.get("ID"), st["ID"])
         nodes = d.get_mesh(0)["nodes"]
-    assert np.array_equal(got["ID"], st["ID"])
     assert np.max(np.abs(nodes - st.mesh_arrays["x"])) < 1e-14
-    assert np.linalg.norm(st.contact_arrays["cf"]) > 0
-    for k in ["u", "v", "sigma", "pl_strain", "T"]:
-        assert rel_l2(got[k], st[k]) < STEP_TOL, k
-    assert rel_l2(got["contforce"], st.contact_arrays["cf"]) < STEP_TOL
+    assert touched > 0
 
 
 def test_steps_plate_compression_mixed(oracle, wflib):
-    sc = _plate_case(law=_abi.WF_CONTACT_STIFFNESS, mu=0.2)
+    sc = _plate_case(law=_abi.WF_CONTACT_STIFFNESS, mu=0.2, sink=0.0)
     _case("steps mixed", sc)
     st = to_oracle(oracle, sc)
     st.nsearch(sc.cfg)

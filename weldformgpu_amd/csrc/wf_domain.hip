@@ -82,6 +82,7 @@ struct wf_domain {
   double mlp_last_loss = 0.0;
   int64_t mlp_last_samples = 0;
   bool mixed = false;
+  bool uniform_h = true;     // every particle has the same h (FP32 loops skip h_j)
   bool mirror_dirty = true;  // FP32 mirrors out of date with the FP64 state
   bool nb_built = false;
   bool snapshot_set = false;
@@ -538,15 +539,16 @@ int wf_create(const wf_config* cfg, wf_domain** out) {
     if (rc) break;
   }
   d->mixed = cfg->precision == WF_MIXED;
-  {
-    float** f32[] = {&d->f.xr[0], &d->f.xr[1], &d->f.xr[2], &d->f.v32[0], &d->f.v32[1],
-                     &d->f.v32[2], &d->f.h32, &d->f.m32, &d->f.mdj32, &d->f.mdj32_alt,
-                     &d->f.rho32, &d->f.cs32, &d->f.sr32[0], &d->f.sr32[1], &d->f.sr32[2],
-                     &d->f.sr32[3], &d->f.sr32[4], &d->f.sr32[5], &d->f.T32, &d->f.T32_alt,
-                     &d->f.k32};
+  if (d->mixed) {
+    float** f32[] = {&d->f.xr[0], &d->f.xr[1], &d->f.xr[2], &d->f.h32};
     for (float** q : f32)
-      if (!rc && d->mixed) rc = d->alloc_plane_f(q);
-    if (!rc && d->mixed) rc = d->alloc_plane_i(&d->f.cell);
+      if (!rc) rc = d->alloc_plane_f(q);
+    float4** f4[] = {&d->f.v4, &d->f.v4_alt, &d->f.s4a, &d->f.s4b};
+    for (float4** q : f4)
+      if (!rc) rc = d->dev_alloc((void**)q, sizeof(float4) * d->npad);
+    if (!rc) rc = d->dev_alloc((void**)&d->f.tk2, sizeof(float2) * d->npad);
+    if (!rc) rc = d->dev_alloc((void**)&d->f.tk2_alt, sizeof(float2) * d->npad);
+    if (!rc) rc = d->alloc_plane_i(&d->f.cell);
   }
   // return-map queue, surrogate samples and weights
   d->use_queue = cfg->return_map == WF_RETURN_NEWTON && cfg->mat.model == WF_MAT_JOHNSON_COOK;
@@ -691,6 +693,11 @@ int wf_set_field(wf_domain* d, int field, const void* host, size_t bytes) {
   if (field == WF_F_NB_COUNT || field == WF_F_CONT_NB_COUNT)
     return fail(WF_E_ARG, "read-only field");
   if (field == WF_F_ID || field == WF_F_M) d->surf_init = false;
+  if (field == WF_F_H) {
+    const double* hv = (const double*)host;
+    d->uniform_h = true;
+    for (int i = 1; i < d->n && d->uniform_h; i++) d->uniform_h = hv[i] == hv[0];
+  }
   HIPC(hipMemcpyAsync(d->stage, host, bytes, hipMemcpyHostToDevice, d->stream));
   if (elem == 8) {
     Planes6 planes{};
@@ -746,6 +753,7 @@ int wf_fill_field(wf_domain* d, int field, double value) {
   if (field_info(field, &comps, &elem) || elem != 8) return fail(WF_E_ARG, "not a double field");
   for (int c = 0; c < comps; c++) launch_fill_f64(d->stream, *d->plane(field, c).d, value, d->n);
   if (field == WF_F_M) d->surf_init = false;
+  if (field == WF_F_H) d->uniform_h = true;
   if (field == WF_F_X) {
     for (int c = 0; c < 3; c++) launch_fill_f64(d->stream, d->f.xs[c], value, d->n);
     d->snap_gen++;
@@ -909,7 +917,8 @@ int wf_calc_dens_inc(wf_domain* d) {
   NEED_NB(d);
   if (d->mixed) {
     d->refresh_mirrors();
-    launch_dens_mixed(d->stream, d->f, d->tl, d->n, d->solid, d->cfg.dt, false, d->frame.cs32);
+    launch_dens_mixed(d->stream, d->f, d->tl, d->n, d->solid, d->cfg.dt, false, d->frame.cs32,
+                      d->uniform_h);
   } else {
     launch_dens_tiled(d->stream, d->f, d->tl, d->n, d->solid, d->cfg.dt, false);
   }
@@ -928,7 +937,8 @@ int wf_calc_rate_tensors(wf_domain* d) {
   NEED_NB(d);
   if (d->mixed) {
     d->refresh_mirrors();
-    launch_rates_mixed(d->stream, d->f, d->tl, d->solid, d->cfg.quirks, d->frame.cs32);
+    launch_rates_mixed(d->stream, d->f, d->tl, d->solid, d->cfg.quirks, d->frame.cs32,
+                       d->uniform_h);
   } else {
     launch_rates_tiled(d->stream, d->f, d->tl, d->solid, d->cfg.quirks);
   }
@@ -960,7 +970,7 @@ int wf_calc_accel(wf_domain* d) {
   if (d->mixed) {
     d->refresh_mirrors();
     launch_accel_mixed(d->stream, d->f, d->tl, d->solid, d->cfg.alpha, d->cfg.beta,
-                       d->frame.cs32);
+                       d->frame.cs32, d->uniform_h);
   } else {
     launch_accel_tiled(d->stream, d->f, d->tl, d->solid, d->cfg.alpha, d->cfg.beta);
   }
@@ -994,7 +1004,7 @@ int wf_calc_temp_inc(wf_domain* d) {
   if (d->mixed) {
     d->refresh_mirrors();
     launch_cond_mixed(d->stream, d->f, d->tl, d->n, d->cfg.T_inf, d->cfg.h_conv, d->cfg.dt,
-                      false, false, d->frame.cs32);
+                      false, false, d->frame.cs32, d->uniform_h);
   } else {
     launch_cond_tiled(d->stream, d->f, d->tl, d->n, d->cfg.T_inf, d->cfg.h_conv, d->cfg.dt, false,
                       false);
@@ -1008,7 +1018,7 @@ int wf_thermal_calcs(wf_domain* d, double dt) {
   if (d->mixed) {
     d->refresh_mirrors();
     launch_cond_mixed(d->stream, d->f, d->tl, d->n, d->cfg.T_inf, d->cfg.h_conv, dt, false,
-                      false, d->frame.cs32);
+                      false, d->frame.cs32, d->uniform_h);
   } else {
     launch_cond_tiled(d->stream, d->f, d->tl, d->n, d->cfg.T_inf, d->cfg.h_conv, dt, false,
                       false);
@@ -1264,17 +1274,18 @@ int wf_step(wf_domain* d, int n_steps) {
       d->refresh_mirrors();
       const bool mx = d->mixed;
       const float cs32 = d->frame.cs32;
+      const bool uh = d->uniform_h;
       timer_begin(d);
       if (mx) {
-        launch_dens_mixed(d->stream, d->f, d->tl, d->n, d->solid, p.dt, true, cs32);
-        std::swap(d->f.mdj32, d->f.mdj32_alt);
+        launch_dens_mixed(d->stream, d->f, d->tl, d->n, d->solid, p.dt, true, cs32, uh);
+        std::swap(d->f.v4, d->f.v4_alt);
       } else {
         launch_dens_tiled(d->stream, d->f, d->tl, d->n, d->solid, p.dt, true);
       }
       std::swap(d->f.rho, d->f.rho_alt);
       if ((rc = timer_end(d, WF_K_DENS))) return rc;
       timer_begin(d);
-      if (mx) launch_rates_mixed(d->stream, d->f, d->tl, d->solid, p.quirks, cs32);
+      if (mx) launch_rates_mixed(d->stream, d->f, d->tl, d->solid, p.quirks, cs32, uh);
       else launch_rates_tiled(d->stream, d->f, d->tl, d->solid, p.quirks);
       if ((rc = timer_end(d, WF_K_RATES))) return rc;
       timer_begin(d);
@@ -1289,7 +1300,7 @@ int wf_step(wf_domain* d, int n_steps) {
       }
       launch_fold_stats(d->stream, d->bstats, (p.solid + kBlock - 1) / kBlock, d->ctr);
       timer_begin(d);
-      if (mx) launch_accel_mixed(d->stream, d->f, d->tl, d->solid, p.alpha, p.beta, cs32);
+      if (mx) launch_accel_mixed(d->stream, d->f, d->tl, d->solid, p.alpha, p.beta, cs32, uh);
       else launch_accel_tiled(d->stream, d->f, d->tl, d->solid, p.alpha, p.beta);
       if ((rc = timer_end(d, WF_K_ACCEL))) return rc;
       if (d->cfg.contact) {  // SolverFraser.cu:233-257
@@ -1321,8 +1332,8 @@ int wf_step(wf_domain* d, int n_steps) {
       timer_begin(d);
       if (d->mixed) {
         launch_cond_mixed(d->stream, d->f, d->tl, d->n, p.T_inf, p.h_conv, p.dt, true,
-                          d->isfirst_step, d->frame.cs32);
-        std::swap(d->f.T32, d->f.T32_alt);
+                          d->isfirst_step, d->frame.cs32, d->uniform_h);
+        std::swap(d->f.tk2, d->f.tk2_alt);
       } else {
         launch_cond_tiled(d->stream, d->f, d->tl, d->n, p.T_inf, p.h_conv, p.dt, true,
                           d->isfirst_step);

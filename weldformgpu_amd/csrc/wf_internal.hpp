@@ -29,10 +29,13 @@ struct Fields {
   int *ID, *BC_T;
   // FP32 mirrors read by the mixed-precision pair loops (WF_MIXED): position
   // relative to the origin of an integer grid cell (cell packed as
-  // cx | cy << 11 | cz << 22), velocity, h, m, m/rho, rho, Cs, sigma/rho^2,
-  // T, k.  mdj32 and T32 are double-buffered like rho and T.
-  float *xr[3], *v32[3], *h32, *m32, *mdj32, *mdj32_alt, *rho32, *cs32, *sr32[6];
-  float *T32, *T32_alt, *k32;
+  // cx | cy << 11 | cz << 22) and h, per component; the neighbour fields a
+  // pair gathers, packed per particle: v4 = {v, m/rho} (double-buffered like
+  // rho), s4a = sigma/rho^2 {xx, yy, zz, xy}, s4b = {sigma/rho^2 yz, xz, rho,
+  // Cs}, tk2 = {T, k} (double-buffered like T).
+  float *xr[3], *h32;
+  float4 *v4, *v4_alt, *s4a, *s4b;
+  float2 *tk2, *tk2_alt;
   int* cell;
   // return-map surrogate training samples, one slot per particle: features
   // (s_trial/A, 10 eps_p, log10(max(rate,1e-3))/3, theta), target dg/dg_max,
@@ -187,14 +190,15 @@ void launch_cond_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int 
                        double h_conv, double dt, bool leap, bool first);
 
 // mixed precision (FP32 pair loops, compensated sums)
+// uh: every particle has the same h
 void launch_dens_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int n, int solid,
-                       double dt, bool fuse, float cs32);
+                       double dt, bool fuse, float cs32, bool uh);
 void launch_rates_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
-                        int quirks, float cs32);
+                        int quirks, float cs32, bool uh);
 void launch_accel_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
-                        double alpha, double beta, float cs32);
+                        double alpha, double beta, float cs32, bool uh);
 void launch_cond_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int n, double T_inf,
-                       double h_conv, double dt, bool leap, bool first, float cs32);
+                       double h_conv, double dt, bool leap, bool first, float cs32, bool uh);
 // refresh every FP32 mirror from the FP64 state (after uploads and rebuilds)
 void launch_mirror32(hipStream_t s, const Fields& f, int n, const Frame32& fr);
 

@@ -218,14 +218,12 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Block
       f.sigma[3][i] = S.xy;
       f.sigma[4][i] = S.yz;
       f.sigma[5][i] = S.xz;
-      if (MIR) {  // sigma/rho^2 for the FP32 acceleration loop
-        const double ir2 = 1.0 / (f.rho[i] * f.rho[i]);
-        f.sr32[0][i] = (float)((-pi + S.xx) * ir2);
-        f.sr32[1][i] = (float)((-pi + S.yy) * ir2);
-        f.sr32[2][i] = (float)((-pi + S.zz) * ir2);
-        f.sr32[3][i] = (float)(S.xy * ir2);
-        f.sr32[4][i] = (float)(S.yz * ir2);
-        f.sr32[5][i] = (float)(S.xz * ir2);
+      if (MIR) {  // sigma/rho^2, rho and Cs for the FP32 acceleration loop
+        const double rho = f.rho[i], ir2 = 1.0 / (rho * rho);
+        f.s4a[i] = make_float4((float)((-pi + S.xx) * ir2), (float)((-pi + S.yy) * ir2),
+                               (float)((-pi + S.zz) * ir2), (float)(S.xy * ir2));
+        f.s4b[i] = make_float4((float)(S.yz * ir2), (float)(S.xz * ir2), (float)rho,
+                               (float)f.Cs[i]);
       }
       if (ENERGY) {
         double de = f.m[i] / f.rho[i] *
@@ -362,9 +360,8 @@ __global__ __launch_bounds__(kBlock) void k_integrate(Fields f, int n, double dt
     f.xr[0][i] = (float)(f.x[0][i] - (fr.org[0] + cx * fr.cs));
     f.xr[1][i] = (float)(f.x[1][i] - (fr.org[1] + cy * fr.cs));
     f.xr[2][i] = (float)(f.x[2][i] - (fr.org[2] + cz * fr.cs));
-    f.v32[0][i] = (float)v0;
-    f.v32[1][i] = (float)v1;
-    f.v32[2][i] = (float)v2;
+    const float mdj = f.v4[i].w;
+    f.v4[i] = make_float4((float)v0, (float)v1, (float)v2, mdj);
   }
 }
 
@@ -379,22 +376,22 @@ __global__ __launch_bounds__(kBlock) void k_mirror32(Fields f, int n, Frame32 fr
     int lim = a < 2 ? 2047 : 1023;
     c[a] = (int)fmin(fmax(u, 0.0), (double)lim);
     f.xr[a][i] = (float)(f.x[a][i] - (fr.org[a] + c[a] * fr.cs));
-    f.v32[a][i] = (float)f.v[a][i];
   }
   f.cell[i] = pack_cell(c[0], c[1], c[2]);
   const double rho = f.rho[i];
   f.h32[i] = (float)f.h[i];
-  f.m32[i] = (float)f.m[i];
-  const float mdj = (float)(f.m[i] / rho);
-  f.mdj32[i] = mdj;
-  f.mdj32_alt[i] = mdj;
-  f.rho32[i] = (float)rho;
-  f.cs32[i] = (float)f.Cs[i];
+  const float4 v4 = make_float4((float)f.v[0][i], (float)f.v[1][i], (float)f.v[2][i],
+                                (float)(f.m[i] / rho));
+  f.v4[i] = v4;
+  f.v4_alt[i] = v4;
   const double ir2 = 1.0 / (rho * rho);
-  for (int k = 0; k < 6; k++) f.sr32[k][i] = (float)(f.sigma[k][i] * ir2);
-  f.T32[i] = (float)f.T[i];
-  f.T32_alt[i] = (float)f.T[i];
-  f.k32[i] = (float)f.k[i];
+  f.s4a[i] = make_float4((float)(f.sigma[0][i] * ir2), (float)(f.sigma[1][i] * ir2),
+                         (float)(f.sigma[2][i] * ir2), (float)(f.sigma[3][i] * ir2));
+  f.s4b[i] = make_float4((float)(f.sigma[4][i] * ir2), (float)(f.sigma[5][i] * ir2),
+                         (float)rho, (float)f.Cs[i]);
+  const float2 tk = make_float2((float)f.T[i], (float)f.k[i]);
+  f.tk2[i] = tk;
+  f.tk2_alt[i] = tk;
 }
 
 __global__ __launch_bounds__(kBlock) void k_apply_bc(Fields f, int n, int zone, double vx,

@@ -1,8 +1,10 @@
-// wf_pair.hip — LDS-staged FP64 pair loops (gfx950).
+// wf_pair.hip — tiled pair loops (gfx950): FP64 with LDS-staged neighbour
+// fields, FP32 (mixed precision) with LDS-staged positions and gathered
+// neighbour fields.
 //
 // One workgroup per tile of kTileP Morton-consecutive particles, one thread
 // per particle.  The tile's staged set (the particles of every occupied cell
-// adjacent to the tile) streams through LDS in chunks (kChunk64 / kChunk32) with
+// adjacent to the tile) streams through LDS in chunks (kChunk64) with
 // coalesced global loads; each thread walks its neighbour list, whose
 // entries are staged positions in ascending order, so each chunk consumes a
 // prefix of the remaining list.  The per-pair physics is the reference's
@@ -101,7 +103,6 @@ __global__ __launch_bounds__(kTileP) void k_pair_tiled(Fields f, TileLists tl,
 
 // staged particles per LDS chunk: FP64 and FP32 policies
 constexpr int kChunk64 = 256;
-constexpr int kChunk32 = 512;
 
 // FP64 policies need no per-tile context
 struct NoCtx {};
@@ -410,22 +411,26 @@ struct CondK {
 // ===========================================================================
 // Mixed precision (WF_MIXED): the same pair physics in FP32, on FP32 mirrors,
 // with coordinates relative to the tile's base cell (x = xr + (c - c_tile) cs,
-// c an integer grid cell), compensated (Kahan-Neumaier) FP32 sums and FP64
-// results.  Per-particle terms stay FP64.
+// c an integer grid cell), compensated (Kahan) FP32 sums and FP64 results.
+// Per-particle terms stay FP64.
+//
+// The FP32 loops hold only the staged set's tile-frame positions in LDS
+// (kStage32 particles, normally the whole set) and gather the other
+// neighbour fields from packed per-particle float4/float2 mirrors, which stay
+// in L1/L2 because the staged set is spatially compact.  A thread therefore
+// walks its whole neighbour list without waiting at chunk boundaries for the
+// busiest lane of its wavefront (with chunks of a few hundred staged
+// particles a wavefront keeps only ~40-55 % of its lanes busy).
 
-// compensated FP32 accumulator (Kahan 1965): c carries the low-order part
-// lost by each addition
+// sum of FP32 pair terms kept in FP64: with packed FP32 off, an FP64 add
+// issues at the FP32 rate on gfx950, so widening each term costs two
+// operations where compensated (Kahan) FP32 summation costs four, and the
+// sum carries no rounding beyond the terms' own
 struct KSum {
-  float s, c;
-  __device__ __forceinline__ void zero() { s = 0.f; c = 0.f; }
-  __device__ __forceinline__ void add(float y) {
-#pragma clang fp contract(off)
-    const float yc = y - c;
-    const float t = s + yc;
-    c = (t - s) - yc;
-    s = t;
-  }
-  __device__ __forceinline__ double val() const { return (double)s - (double)c; }
+  double s;
+  __device__ __forceinline__ void zero() { s = 0.0; }
+  __device__ __forceinline__ void add(float y) { s += (double)y; }
+  __device__ __forceinline__ double val() const { return s; }
 };
 
 struct HConstF {
@@ -435,13 +440,15 @@ __device__ __forceinline__ HConstF hconst_f(float h) {
   const float hh = h * h;
   return HConstF{h, __builtin_amdgcn_rcpf(h), __builtin_amdgcn_rcpf(hh * hh * (float)kPi)};
 }
-// FP32 cubic-spline gradient (1/r) dW/dr from r^2 (Functions.cu:76-88)
+// FP32 cubic-spline gradient (1/r) dW/dr from r^2 (Functions.cu:76-88);
+// UH: every particle has the same h, so h_j = h_i
+template <bool UH>
 __device__ __forceinline__ float gk_pair_f(float r2, float hj, const HConstF& hc, float& h_) {
   const float rinv = r2 > 0.f ? __builtin_amdgcn_rsqf(r2) : 0.f;
   const float rij = r2 * rinv;
   float inv_h = hc.inv_h, C = hc.C;
   h_ = hc.h;
-  if (hj != hc.h) {
+  if (!UH && hj != hc.h) {
     h_ = 0.5f * (hc.h + hj);
     inv_h = __builtin_amdgcn_rcpf(h_);
     C = __builtin_amdgcn_rcpf(h_ * h_ * h_ * h_ * (float)kPi);
@@ -471,44 +478,118 @@ __device__ __forceinline__ void pos32(const Fields& f, int j, const Ctx32& c, fl
   z = f.xr[2][j] + (float)(cz - c.bz) * c.cs;
 }
 
-struct DensM {
-  static constexpr int CHUNK = kChunk32;
-  static constexpr bool UNROLL = true;
-  static constexpr int NF = 8;  // x y z vx vy vz h m/rho
-  typedef float LT;
-  typedef Ctx32 Ctx;
+// staged tile-frame positions per LDS pass (18 KB); a tile's staged set
+// averages ~1.1k particles on a cubic lattice
+constexpr int kStage32 = 1536;
+
+template <class K>
+__global__ __launch_bounds__(kTileP) void k_pair_gather(Fields f, TileLists tl,
+                                                        typename K::Prm prm) {
+  __shared__ float lx[kStage32], ly[kStage32], lz[kStage32];
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int tid = threadIdx.x;
+  const int i = t * kTileP + tid;
+  const bool act = i < prm.limit;
+  const Ctx32 cx = ctx32(f, t, prm.cs);
+  typename K::Own own;
+  typename K::Acc acc;
+  K::init(f, i, act, own, acc, cx);
+  const int ub = tl.uoff[t], ucnt = tl.uoff[t + 1] - ub;
+  const int* __restrict__ tj = tl.tj + ub;
+  const int cnt = act ? tl.count[i] : 0;
+  if (ucnt <= kStage32) {
+    // the whole staged set fits: one pass, a quad of four slots at a time
+    for (int s = tid; s < ucnt; s += kTileP) pos32(f, tj[s], cx, lx[s], ly[s], lz[s]);
+    __syncthreads();
+    const uint2* __restrict__ q = tl.slotq + i;
+    for (int k = 0; k < cnt; k += 4) {
+      const uint2 w = q[(int64_t)(k >> 2) * tl.npad];
+      const int sl[4] = {(int)(w.x & 0xFFFFu), (int)(w.x >> 16), (int)(w.y & 0xFFFFu),
+                         (int)(w.y >> 16)};
+      const int nk = min(4, cnt - k);
+      typename K::Nb nb[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (u < nk) K::load(f, tj[sl[u]], nb[u]);
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (u < nk) K::pair(lx[sl[u]], ly[sl[u]], lz[sl[u]], nb[u], own, acc, prm);
+    }
+    K::finish(f, i, act, own, acc, prm);
+    return;
+  }
+  SlotReader sr;
+  sr.init(tl, i, cnt);
+  for (int c0 = 0; c0 < ucnt; c0 += kStage32) {
+    const int m = min(kStage32, ucnt - c0);
+    __syncthreads();
+    for (int s = tid; s < m; s += kTileP) pos32(f, tj[c0 + s], cx, lx[s], ly[s], lz[s]);
+    __syncthreads();
+    const int lim = c0 + m;
+    // two neighbours per iteration while both fall in this pass: their
+    // gathers are issued before either pair is evaluated
+    while (sr.k < sr.cnt) {
+      const int s0 = sr.slot();
+      if (s0 >= lim) break;
+      sr.advance();
+      typename K::Nb a;
+      K::load(f, tj[s0], a);
+      if (sr.k < sr.cnt) {
+        const int s1 = sr.slot();
+        if (s1 < lim) {
+          sr.advance();
+          typename K::Nb b;
+          K::load(f, tj[s1], b);
+          K::pair(lx[s0 - c0], ly[s0 - c0], lz[s0 - c0], a, own, acc, prm);
+          K::pair(lx[s1 - c0], ly[s1 - c0], lz[s1 - c0], b, own, acc, prm);
+          continue;
+        }
+      }
+      K::pair(lx[s0 - c0], ly[s0 - c0], lz[s0 - c0], a, own, acc, prm);
+    }
+  }
+  K::finish(f, i, act, own, acc, prm);
+}
+
+// A: density rate (InteractionAlt.cu:166-246), fused density update writes
+// the next {v, m/rho} mirror
+template <bool UH>
+struct DensG {
   struct Prm {
     int limit, n;
     double dt;
     int fuse;
     float cs;
   };
-  __device__ static Ctx ctx(const Fields& f, int t, const Prm& p) { return ctx32(f, t, p.cs); }
   struct Own {
     float x, y, z, vx, vy, vz;
     HConstF hc;
   };
+  struct Nb {
+    float4 v;  // v, m/rho
+    float h;
+  };
   typedef KSum Acc;
-  __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx& c) {
+  __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx32& c) {
     a.zero();
     if (!act) return;
     pos32(f, i, c, o.x, o.y, o.z);
-    o.vx = f.v32[0][i]; o.vy = f.v32[1][i]; o.vz = f.v32[2][i];
+    const float4 v = f.v4[i];
+    o.vx = v.x; o.vy = v.y; o.vz = v.z;
     o.hc = hconst_f(f.h32[i]);
   }
-  __device__ static void stage(const Fields& f, int j, float (*L)[CHUNK], int s,
-                               const Ctx& c) {
-    pos32(f, j, c, L[0][s], L[1][s], L[2][s]);
-    L[3][s] = f.v32[0][j]; L[4][s] = f.v32[1][j]; L[5][s] = f.v32[2][j];
-    L[6][s] = f.h32[j]; L[7][s] = f.mdj32[j];
+  __device__ static void load(const Fields& f, int j, Nb& nb) {
+    nb.v = f.v4[j];
+    nb.h = UH ? 0.f : f.h32[j];
   }
-  __device__ static void pair(float (*L)[CHUNK], int s, const Own& o, Acc& a, const Prm&) {
-    const float xx = o.x - L[0][s], xy = o.y - L[1][s], xz = o.z - L[2][s];
+  __device__ static void pair(float x, float y, float z, const Nb& nb, const Own& o, Acc& a,
+                              const Prm&) {
+    const float xx = o.x - x, xy = o.y - y, xz = o.z - z;
     const float r2 = xx * xx + xy * xy + xz * xz;
     float h_;
-    const float GK = gk_pair_f(r2, L[6][s], o.hc, h_);
-    const float vx = o.vx - L[3][s], vy = o.vy - L[4][s], vz = o.vz - L[5][s];
-    a.add(L[7][s] * (GK * (vx * xx + vy * xy + vz * xz)));
+    const float GK = gk_pair_f<UH>(r2, nb.h, o.hc, h_);
+    const float vx = o.vx - nb.v.x, vy = o.vy - nb.v.y, vz = o.vz - nb.v.z;
+    a.add(nb.v.w * (GK * (vx * xx + vy * xy + vz * xz)));
   }
   __device__ static void finish(const Fields& f, int i, bool act, const Own&, const Acc& a,
                                 const Prm& p) {
@@ -519,53 +600,47 @@ struct DensM {
       if (p.fuse) {
         const double rn = rho + p.dt * drho;
         f.rho_alt[i] = rn;
-        f.rho32[i] = (float)rn;
-        f.mdj32_alt[i] = (float)(f.m[i] / rn);
+        float4 v = f.v4[i];
+        v.w = (float)(f.m[i] / rn);
+        f.v4_alt[i] = v;
       }
     } else if (p.fuse && i < p.n) {
       f.rho_alt[i] = f.rho[i];
-      f.mdj32_alt[i] = f.mdj32[i];
+      f.v4_alt[i] = f.v4[i];
     }
   }
 };
 
-struct RatesM {
-  static constexpr int CHUNK = kChunk32;
-  static constexpr bool UNROLL = true;
-  static constexpr int NF = 8;
-  typedef float LT;
-  typedef Ctx32 Ctx;
+// B: strain and rotation rates (InteractionAlt.cu:262-383)
+template <bool UH>
+struct RatesG {
   struct Prm {
     int limit;
     double xz_sign;
     float cs;
   };
-  __device__ static Ctx ctx(const Fields& f, int t, const Prm& p) { return ctx32(f, t, p.cs); }
-  struct Own {
-    float x, y, z, vx, vy, vz;
-    HConstF hc;
-  };
+  typedef typename DensG<UH>::Own Own;
+  typedef typename DensG<UH>::Nb Nb;
   struct Acc {
     KSum dxx, dyy, dzz, dxy, dyz, dxz, wxy, wyz, wxz;
   };
-  __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx& c) {
+  __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx32& c) {
     a.dxx.zero(); a.dyy.zero(); a.dzz.zero(); a.dxy.zero(); a.dyz.zero(); a.dxz.zero();
     a.wxy.zero(); a.wyz.zero(); a.wxz.zero();
     if (!act) return;
     pos32(f, i, c, o.x, o.y, o.z);
-    o.vx = f.v32[0][i]; o.vy = f.v32[1][i]; o.vz = f.v32[2][i];
+    const float4 v = f.v4[i];
+    o.vx = v.x; o.vy = v.y; o.vz = v.z;
     o.hc = hconst_f(f.h32[i]);
   }
-  __device__ static void stage(const Fields& f, int j, float (*L)[CHUNK], int s,
-                               const Ctx& c) {
-    DensM::stage(f, j, L, s, c);
-  }
-  __device__ static void pair(float (*L)[CHUNK], int s, const Own& o, Acc& a, const Prm&) {
-    const float xx = o.x - L[0][s], xy = o.y - L[1][s], xz = o.z - L[2][s];
+  __device__ static void load(const Fields& f, int j, Nb& nb) { DensG<UH>::load(f, j, nb); }
+  __device__ static void pair(float x, float y, float z, const Nb& nb, const Own& o, Acc& a,
+                              const Prm&) {
+    const float xx = o.x - x, xy = o.y - y, xz = o.z - z;
     const float r2 = xx * xx + xy * xy + xz * xz;
     float h_;
-    const float w = gk_pair_f(r2, L[6][s], o.hc, h_) * L[7][s];
-    const float vx = o.vx - L[3][s], vy = o.vy - L[4][s], vz = o.vz - L[5][s];
+    const float w = gk_pair_f<UH>(r2, nb.h, o.hc, h_) * nb.v.w;
+    const float vx = o.vx - nb.v.x, vy = o.vy - nb.v.y, vz = o.vz - nb.v.z;
     const float ax = w * vx, ay = w * vy, az = w * vz;
     a.dxx.add(ax * xx);
     a.dyy.add(ay * xy);
@@ -590,63 +665,62 @@ struct RatesM {
   }
 };
 
-struct AccelM {
-  static constexpr int CHUNK = kChunk64;
-  static constexpr bool UNROLL = false;
-  static constexpr int NF = 16;  // x y z vx vy vz h m rho Cs (sigma/rho^2)[6]
-  typedef float LT;
-  typedef Ctx32 Ctx;
+// D: acceleration with Monaghan viscosity (InteractionAlt.cu:18-150); the
+// neighbour's mass is (m/rho) rho
+template <bool UH>
+struct AccelG {
   struct Prm {
     int limit;
     double alpha, beta;
     float cs;
   };
-  __device__ static Ctx ctx(const Fields& f, int t, const Prm& p) { return ctx32(f, t, p.cs); }
   struct Own {
     float x, y, z, vx, vy, vz, di, Ci, s0, s1, s2, s3, s4, s5;
     HConstF hc;
   };
+  struct Nb {
+    float4 v, sa, sb;
+    float h;
+  };
   struct Acc {
     KSum ax, ay, az;
   };
-  __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx& c) {
+  __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx32& c) {
     a.ax.zero(); a.ay.zero(); a.az.zero();
     if (!act) return;
     pos32(f, i, c, o.x, o.y, o.z);
-    o.vx = f.v32[0][i]; o.vy = f.v32[1][i]; o.vz = f.v32[2][i];
-    o.di = f.rho32[i]; o.Ci = f.cs32[i];
+    const float4 v = f.v4[i], sa = f.s4a[i], sb = f.s4b[i];
+    o.vx = v.x; o.vy = v.y; o.vz = v.z;
+    o.s0 = sa.x; o.s1 = sa.y; o.s2 = sa.z; o.s3 = sa.w;
+    o.s4 = sb.x; o.s5 = sb.y; o.di = sb.z; o.Ci = sb.w;
     o.hc = hconst_f(f.h32[i]);
-    o.s0 = f.sr32[0][i]; o.s1 = f.sr32[1][i]; o.s2 = f.sr32[2][i];
-    o.s3 = f.sr32[3][i]; o.s4 = f.sr32[4][i]; o.s5 = f.sr32[5][i];
   }
-  __device__ static void stage(const Fields& f, int j, float (*L)[CHUNK], int s,
-                               const Ctx& c) {
-    pos32(f, j, c, L[0][s], L[1][s], L[2][s]);
-    L[3][s] = f.v32[0][j]; L[4][s] = f.v32[1][j]; L[5][s] = f.v32[2][j];
-    L[6][s] = f.h32[j]; L[7][s] = f.m32[j]; L[8][s] = f.rho32[j]; L[9][s] = f.cs32[j];
-    L[10][s] = f.sr32[0][j]; L[11][s] = f.sr32[1][j]; L[12][s] = f.sr32[2][j];
-    L[13][s] = f.sr32[3][j]; L[14][s] = f.sr32[4][j]; L[15][s] = f.sr32[5][j];
+  __device__ static void load(const Fields& f, int j, Nb& nb) {
+    nb.v = f.v4[j];
+    nb.sa = f.s4a[j];
+    nb.sb = f.s4b[j];
+    nb.h = UH ? 0.f : f.h32[j];
   }
-  __device__ static void pair(float (*L)[CHUNK], int s, const Own& o, Acc& a,
+  __device__ static void pair(float x, float y, float z, const Nb& nb, const Own& o, Acc& a,
                               const Prm& p) {
-    const float xx = o.x - L[0][s], xy = o.y - L[1][s], xz = o.z - L[2][s];
+    const float xx = o.x - x, xy = o.y - y, xz = o.z - z;
     const float r2 = xx * xx + xy * xy + xz * xz;
     float h_;
-    const float GK = gk_pair_f(r2, L[6][s], o.hc, h_);
-    const float vx = o.vx - L[3][s], vy = o.vy - L[4][s], vz = o.vz - L[5][s];
+    const float GK = gk_pair_f<UH>(r2, nb.h, o.hc, h_);
+    const float vx = o.vx - nb.v.x, vy = o.vy - nb.v.y, vz = o.vz - nb.v.z;
     const float vdx = vx * xx + vy * xy + vz * xz;
     float PI = 0.f;
     if (vdx < 0.f && (p.alpha != 0.0 || p.beta != 0.0)) {
       const float MU = h_ * vdx * __builtin_amdgcn_rcpf(r2 + 0.01f * h_ * h_);
-      const float Cij = 0.5f * (o.Ci + L[9][s]);
+      const float Cij = 0.5f * (o.Ci + nb.sb.w);
       PI = ((float)p.alpha * Cij * MU + (float)p.beta * MU * MU) *
-           __builtin_amdgcn_rcpf(0.5f * (o.di + L[8][s]));
+           __builtin_amdgcn_rcpf(0.5f * (o.di + nb.sb.z));
     }
-    const float g = GK * L[7][s];
+    const float g = GK * (nb.v.w * nb.sb.z);
     const float gx = g * xx, gy = g * xy, gz = g * xz;
-    const float mxx = o.s0 + L[10][s] + PI, myy = o.s1 + L[11][s] + PI;
-    const float mzz = o.s2 + L[12][s] + PI;
-    const float mxy = o.s3 + L[13][s], myz = o.s4 + L[14][s], mxz = o.s5 + L[15][s];
+    const float mxx = o.s0 + nb.sa.x + PI, myy = o.s1 + nb.sa.y + PI;
+    const float mzz = o.s2 + nb.sa.z + PI;
+    const float mxy = o.s3 + nb.sa.w, myz = o.s4 + nb.sb.x, mxz = o.s5 + nb.sb.y;
     a.ax.add(mxx * gx + mxy * gy + mxz * gz);
     a.ay.add(mxy * gx + myy * gy + myz * gz);
     a.az.add(mxz * gx + myz * gy + mzz * gz);
@@ -660,47 +734,50 @@ struct AccelM {
   }
 };
 
-struct CondM {
-  static constexpr int CHUNK = kChunk32;
-  static constexpr bool UNROLL = true;
-  static constexpr int NF = 7;  // x y z h m/rho T k
-  typedef float LT;
-  typedef Ctx32 Ctx;
+// conduction (Thermal.cu:101-142), fused convection, plastic-work heat and
+// leapfrog; the next {T, k} mirror is written
+template <bool UH>
+struct CondG {
   struct Prm {
     int limit;
     double T_inf, h_conv, dt;
     int leap, first;
     float cs;
   };
-  __device__ static Ctx ctx(const Fields& f, int t, const Prm& p) { return ctx32(f, t, p.cs); }
   struct Own {
     float x, y, z, k, T;
     HConstF hc;
   };
+  struct Nb {
+    float2 tk;
+    float mdj, h;
+  };
   typedef KSum Acc;
-  __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx& c) {
+  __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx32& c) {
     a.zero();
     if (!act) return;
     pos32(f, i, c, o.x, o.y, o.z);
-    o.k = f.k32[i];
-    o.T = f.T32[i];
+    const float2 tk = f.tk2[i];
+    o.T = tk.x;
+    o.k = tk.y;
     o.hc = hconst_f(f.h32[i]);
   }
-  __device__ static void stage(const Fields& f, int j, float (*L)[CHUNK], int s,
-                               const Ctx& c) {
-    pos32(f, j, c, L[0][s], L[1][s], L[2][s]);
-    L[3][s] = f.h32[j]; L[4][s] = f.mdj32[j]; L[5][s] = f.T32[j]; L[6][s] = f.k32[j];
+  __device__ static void load(const Fields& f, int j, Nb& nb) {
+    nb.tk = f.tk2[j];
+    nb.mdj = f.v4[j].w;
+    nb.h = UH ? 0.f : f.h32[j];
   }
-  __device__ static void pair(float (*L)[CHUNK], int s, const Own& o, Acc& a, const Prm&) {
-    const float xx = o.x - L[0][s], xy = o.y - L[1][s], xz = o.z - L[2][s];
+  __device__ static void pair(float x, float y, float z, const Nb& nb, const Own& o, Acc& a,
+                              const Prm&) {
+    const float xx = o.x - x, xy = o.y - y, xz = o.z - z;
     const float r2 = xx * xx + xy * xy + xz * xz;
     float h_;
-    const float GK = gk_pair_f(r2, L[3][s], o.hc, h_);
-    const float kj = L[6][s];
+    const float GK = gk_pair_f<UH>(r2, nb.h, o.hc, h_);
+    const float kj = nb.tk.y;
     const float kh = (kj == o.k) ? 2.f * o.k : 4.f * o.k * kj * __builtin_amdgcn_rcpf(o.k + kj);
-    a.add(L[4][s] * kh * (o.T - L[5][s]) * GK);
+    a.add(nb.mdj * kh * (o.T - nb.tk.x) * GK);
   }
-  __device__ static void finish(const Fields& f, int i, bool act, const Own&, const Acc& a,
+  __device__ static void finish(const Fields& f, int i, bool act, const Own& o, const Acc& a,
                                 const Prm& p) {
     if (!act) return;
     const double rhoi = f.rho[i], cpi = f.cp[i], Ti = f.T[i];
@@ -729,7 +806,7 @@ struct CondM {
         Tn = (tan + ta) / 2.;
       }
       f.T_alt[i] = Tn;
-      f.T32_alt[i] = (float)Tn;
+      f.tk2_alt[i] = make_float2((float)Tn, o.k);
     }
   }
 };
@@ -820,28 +897,39 @@ void launch_surface_tiled(hipStream_t s, const Fields& f, const TileLists& tl, i
 }  // namespace wf
 
 namespace wf {
+// launch policy POL with every particle's h equal (UH) or not
+template <template <bool> class POL, class... A>
+static void launch_gather(hipStream_t s, const Fields& f, const TileLists& tl, int grid, bool uh,
+                          A... args) {
+  if (uh) {
+    const typename POL<true>::Prm p{args...};
+    hipLaunchKernelGGL(k_pair_gather<POL<true>>, dim3(grid), dim3(kTileP), 0, s, f, tl, p);
+  } else {
+    const typename POL<false>::Prm p{args...};
+    hipLaunchKernelGGL(k_pair_gather<POL<false>>, dim3(grid), dim3(kTileP), 0, s, f, tl, p);
+  }
+}
+
 void launch_dens_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int n, int solid,
-                       double dt, bool fuse, float cs32) {
+                       double dt, bool fuse, float cs32, bool uh) {
   if (n <= 0) return;
-  DensM::Prm p{solid, n, dt, fuse ? 1 : 0, cs32};
-  hipLaunchKernelGGL(k_pair_tiled<DensM>, dim3(ntiles(n)), dim3(kTileP), 0, s, f, tl, p);
+  launch_gather<DensG>(s, f, tl, ntiles(n), uh, solid, n, dt, fuse ? 1 : 0, cs32);
 }
 void launch_rates_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
-                        int quirks, float cs32) {
+                        int quirks, float cs32, bool uh) {
   if (solid <= 0) return;
-  RatesM::Prm p{solid, (quirks & WF_Q_ROT_XZ_SIGN) ? -1.0 : 1.0, cs32};
-  hipLaunchKernelGGL(k_pair_tiled<RatesM>, dim3(ntiles(solid)), dim3(kTileP), 0, s, f, tl, p);
+  const double xz = (quirks & WF_Q_ROT_XZ_SIGN) ? -1.0 : 1.0;
+  launch_gather<RatesG>(s, f, tl, ntiles(solid), uh, solid, xz, cs32);
 }
 void launch_accel_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
-                        double alpha, double beta, float cs32) {
+                        double alpha, double beta, float cs32, bool uh) {
   if (solid <= 0) return;
-  AccelM::Prm p{solid, alpha, beta, cs32};
-  hipLaunchKernelGGL(k_pair_tiled<AccelM>, dim3(ntiles(solid)), dim3(kTileP), 0, s, f, tl, p);
+  launch_gather<AccelG>(s, f, tl, ntiles(solid), uh, solid, alpha, beta, cs32);
 }
 void launch_cond_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int n, double T_inf,
-                       double h_conv, double dt, bool leap, bool first, float cs32) {
+                       double h_conv, double dt, bool leap, bool first, float cs32, bool uh) {
   if (n <= 0) return;
-  CondM::Prm p{n, T_inf, h_conv, dt, leap ? 1 : 0, first ? 1 : 0, cs32};
-  hipLaunchKernelGGL(k_pair_tiled<CondM>, dim3(ntiles(n)), dim3(kTileP), 0, s, f, tl, p);
+  launch_gather<CondG>(s, f, tl, ntiles(n), uh, n, T_inf, h_conv, dt, leap ? 1 : 0,
+                       first ? 1 : 0, cs32);
 }
 }  // namespace wf
