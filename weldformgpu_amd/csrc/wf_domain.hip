@@ -543,9 +543,10 @@ int wf_create(const wf_config* cfg, wf_domain** out) {
     float** f32[] = {&d->f.xr[0], &d->f.xr[1], &d->f.xr[2], &d->f.h32};
     for (float** q : f32)
       if (!rc) rc = d->alloc_plane_f(q);
-    float4** f4[] = {&d->f.v4, &d->f.v4_alt, &d->f.s4a, &d->f.s4b};
+    float4** f4[] = {&d->f.v4, &d->f.v4_alt};
     for (float4** q : f4)
       if (!rc) rc = d->dev_alloc((void**)q, sizeof(float4) * d->npad);
+    if (!rc) rc = d->dev_alloc((void**)&d->f.arec, 3 * sizeof(float4) * d->npad);
     if (!rc) rc = d->dev_alloc((void**)&d->f.tk2, sizeof(float2) * d->npad);
     if (!rc) rc = d->dev_alloc((void**)&d->f.tk2_alt, sizeof(float2) * d->npad);
     if (!rc) rc = d->alloc_plane_i(&d->f.cell);

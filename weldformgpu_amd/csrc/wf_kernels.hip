@@ -115,6 +115,7 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Block
     T9 WT = {W.xx, W.yx, W.zx, W.xy, W.yy, W.zy, W.xz, W.yz, W.zz};
     T9 SRT = tmul(S, WT, qp);
     T9 RS = tmul(W, S, qp);
+    if (MIR) f.arec[3 * (size_t)i] = f.v4[i];  // this step's {v, m/rho}
     const double G = f.G[i];
     const double tr3 = 1.0 / 3.0 * (D.xx + D.yy + D.zz);
     const double g2 = 2.0 * G;
@@ -220,10 +221,11 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Block
       f.sigma[5][i] = S.xz;
       if (MIR) {  // sigma/rho^2, rho and Cs for the FP32 acceleration loop
         const double rho = f.rho[i], ir2 = 1.0 / (rho * rho);
-        f.s4a[i] = make_float4((float)((-pi + S.xx) * ir2), (float)((-pi + S.yy) * ir2),
-                               (float)((-pi + S.zz) * ir2), (float)(S.xy * ir2));
-        f.s4b[i] = make_float4((float)(S.yz * ir2), (float)(S.xz * ir2), (float)rho,
-                               (float)f.Cs[i]);
+        f.arec[3 * (size_t)i + 1] =
+            make_float4((float)((-pi + S.xx) * ir2), (float)((-pi + S.yy) * ir2),
+                        (float)((-pi + S.zz) * ir2), (float)(S.xy * ir2));
+        f.arec[3 * (size_t)i + 2] =
+            make_float4((float)(S.yz * ir2), (float)(S.xz * ir2), (float)rho, (float)f.Cs[i]);
       }
       if (ENERGY) {
         double de = f.m[i] / f.rho[i] *
@@ -385,10 +387,13 @@ __global__ __launch_bounds__(kBlock) void k_mirror32(Fields f, int n, Frame32 fr
   f.v4[i] = v4;
   f.v4_alt[i] = v4;
   const double ir2 = 1.0 / (rho * rho);
-  f.s4a[i] = make_float4((float)(f.sigma[0][i] * ir2), (float)(f.sigma[1][i] * ir2),
-                         (float)(f.sigma[2][i] * ir2), (float)(f.sigma[3][i] * ir2));
-  f.s4b[i] = make_float4((float)(f.sigma[4][i] * ir2), (float)(f.sigma[5][i] * ir2),
-                         (float)rho, (float)f.Cs[i]);
+  f.arec[3 * (size_t)i] = v4;
+  f.arec[3 * (size_t)i + 1] =
+      make_float4((float)(f.sigma[0][i] * ir2), (float)(f.sigma[1][i] * ir2),
+                  (float)(f.sigma[2][i] * ir2), (float)(f.sigma[3][i] * ir2));
+  f.arec[3 * (size_t)i + 2] = make_float4((float)(f.sigma[4][i] * ir2),
+                                          (float)(f.sigma[5][i] * ir2), (float)rho,
+                                          (float)f.Cs[i]);
   const float2 tk = make_float2((float)f.T[i], (float)f.k[i]);
   f.tk2[i] = tk;
   f.tk2_alt[i] = tk;

@@ -502,15 +502,31 @@ __global__ __launch_bounds__(kTileP) void k_pair_gather(Fields f, TileLists tl,
     for (int s = tid; s < ucnt; s += kTileP) pos32(f, tj[s], cx, lx[s], ly[s], lz[s]);
     __syncthreads();
     const uint2* __restrict__ q = tl.slotq + i;
+    // the next quad's slots and neighbour indices are fetched before the
+    // current quad's pairs are evaluated
+    uint2 w = cnt > 0 ? q[0] : make_uint2(0u, 0u);
+    int jn[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) jn[u] = u < cnt ? tj[(u & 2) ? (u & 1 ? w.y >> 16 : w.y & 0xFFFFu)
+                                                          : (u & 1 ? w.x >> 16 : w.x & 0xFFFFu)]
+                                                 : 0;
     for (int k = 0; k < cnt; k += 4) {
-      const uint2 w = q[(int64_t)(k >> 2) * tl.npad];
       const int sl[4] = {(int)(w.x & 0xFFFFu), (int)(w.x >> 16), (int)(w.y & 0xFFFFu),
                          (int)(w.y >> 16)};
       const int nk = min(4, cnt - k);
       typename K::Nb nb[4];
 #pragma unroll
       for (int u = 0; u < 4; u++)
-        if (u < nk) K::load(f, tj[sl[u]], nb[u]);
+        if (u < nk) K::load(f, jn[u], nb[u]);
+      const int kn = k + 4;
+      if (kn < cnt) {
+        w = q[(int64_t)(kn >> 2) * tl.npad];
+        const int ns[4] = {(int)(w.x & 0xFFFFu), (int)(w.x >> 16), (int)(w.y & 0xFFFFu),
+                           (int)(w.y >> 16)};
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+          if (kn + u < cnt) jn[u] = tj[ns[u]];
+      }
 #pragma unroll
       for (int u = 0; u < 4; u++)
         if (u < nk) K::pair(lx[sl[u]], ly[sl[u]], lz[sl[u]], nb[u], own, acc, prm);
@@ -689,16 +705,18 @@ struct AccelG {
     a.ax.zero(); a.ay.zero(); a.az.zero();
     if (!act) return;
     pos32(f, i, c, o.x, o.y, o.z);
-    const float4 v = f.v4[i], sa = f.s4a[i], sb = f.s4b[i];
+    const float4* r = f.arec + 3 * (size_t)i;
+    const float4 v = r[0], sa = r[1], sb = r[2];
     o.vx = v.x; o.vy = v.y; o.vz = v.z;
     o.s0 = sa.x; o.s1 = sa.y; o.s2 = sa.z; o.s3 = sa.w;
     o.s4 = sb.x; o.s5 = sb.y; o.di = sb.z; o.Ci = sb.w;
     o.hc = hconst_f(f.h32[i]);
   }
   __device__ static void load(const Fields& f, int j, Nb& nb) {
-    nb.v = f.v4[j];
-    nb.sa = f.s4a[j];
-    nb.sb = f.s4b[j];
+    const float4* r = f.arec + 3 * (size_t)j;
+    nb.v = r[0];
+    nb.sa = r[1];
+    nb.sb = r[2];
     nb.h = UH ? 0.f : f.h32[j];
   }
   __device__ static void pair(float x, float y, float z, const Nb& nb, const Own& o, Acc& a,

@@ -97,10 +97,10 @@ __global__ __launch_bounds__(kBlock) void k_return(Fields f, StepParams p, Block
       const double rho = f.rho[i];
       if (MIR) {
         const double ir2 = 1.0 / (rho * rho);
-        f.s4a[i] = make_float4((float)(sg[0] * ir2), (float)(sg[1] * ir2), (float)(sg[2] * ir2),
-                               (float)(sg[3] * ir2));
-        f.s4b[i] = make_float4((float)(sg[4] * ir2), (float)(sg[5] * ir2), (float)rho,
-                               (float)f.Cs[i]);
+        f.arec[3 * (size_t)i + 1] = make_float4((float)(sg[0] * ir2), (float)(sg[1] * ir2),
+                                                (float)(sg[2] * ir2), (float)(sg[3] * ir2));
+        f.arec[3 * (size_t)i + 2] = make_float4((float)(sg[4] * ir2), (float)(sg[5] * ir2),
+                                                (float)rho, (float)f.Cs[i]);
       }
       const double de = f.m[i] / rho *
                         (S[0] * f.strrate[0][i] + S[1] * f.strrate[1][i] +
