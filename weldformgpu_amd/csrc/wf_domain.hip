@@ -547,8 +547,8 @@ int wf_create(const wf_config* cfg, wf_domain** out) {
     for (float4** q : f4)
       if (!rc) rc = d->dev_alloc((void**)q, sizeof(float4) * d->npad);
     if (!rc) rc = d->dev_alloc((void**)&d->f.arec, 3 * sizeof(float4) * d->npad);
-    if (!rc) rc = d->dev_alloc((void**)&d->f.tk2, sizeof(float2) * d->npad);
-    if (!rc) rc = d->dev_alloc((void**)&d->f.tk2_alt, sizeof(float2) * d->npad);
+    if (!rc) rc = d->dev_alloc((void**)&d->f.tk4, sizeof(float4) * d->npad);
+    if (!rc) rc = d->dev_alloc((void**)&d->f.tk4_alt, sizeof(float4) * d->npad);
     if (!rc) rc = d->alloc_plane_i(&d->f.cell);
   }
   // return-map queue, surrogate samples and weights
@@ -1334,7 +1334,7 @@ int wf_step(wf_domain* d, int n_steps) {
       if (d->mixed) {
         launch_cond_mixed(d->stream, d->f, d->tl, d->n, p.T_inf, p.h_conv, p.dt, true,
                           d->isfirst_step, d->frame.cs32, d->uniform_h);
-        std::swap(d->f.tk2, d->f.tk2_alt);
+        std::swap(d->f.tk4, d->f.tk4_alt);
       } else {
         launch_cond_tiled(d->stream, d->f, d->tl, d->n, p.T_inf, p.h_conv, p.dt, true,
                           d->isfirst_step);

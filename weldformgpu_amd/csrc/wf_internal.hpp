@@ -31,13 +31,14 @@ struct Fields {
   // relative to the origin of an integer grid cell (cell packed as
   // cx | cy << 11 | cz << 22) and h, per component; the neighbour fields a
   // pair gathers, packed per particle: v4 = {v, m/rho} (double-buffered like
-  // rho), tk2 = {T, k} (double-buffered like T), and the acceleration loop's
+  // rho), tk4 = {T, k, m/rho, 0} (double-buffered like T; m/rho refreshed by
+  // the density update), and the acceleration loop's
   // 48-byte record per particle, written by the stress pass: arec[3i] = v4,
   // arec[3i+1] = sigma/rho^2 {xx, yy, zz, xy}, arec[3i+2] = {sigma/rho^2 yz,
   // xz, rho, Cs}, so that a pair reads one record instead of three arrays.
   float *xr[3], *h32;
   float4 *v4, *v4_alt, *arec;
-  float2 *tk2, *tk2_alt;
+  float4 *tk4, *tk4_alt;
   int* cell;
   // return-map surrogate training samples, one slot per particle: features
   // (s_trial/A, 10 eps_p, log10(max(rate,1e-3))/3, theta), target dg/dg_max,

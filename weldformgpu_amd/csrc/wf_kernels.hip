@@ -394,9 +394,9 @@ __global__ __launch_bounds__(kBlock) void k_mirror32(Fields f, int n, Frame32 fr
   f.arec[3 * (size_t)i + 2] = make_float4((float)(f.sigma[4][i] * ir2),
                                           (float)(f.sigma[5][i] * ir2), (float)rho,
                                           (float)f.Cs[i]);
-  const float2 tk = make_float2((float)f.T[i], (float)f.k[i]);
-  f.tk2[i] = tk;
-  f.tk2_alt[i] = tk;
+  const float4 tk = make_float4((float)f.T[i], (float)f.k[i], v4.w, 0.f);
+  f.tk4[i] = tk;
+  f.tk4_alt[i] = tk;
 }
 
 __global__ __launch_bounds__(kBlock) void k_apply_bc(Fields f, int n, int zone, double vx,

@@ -619,6 +619,7 @@ struct DensG {
         float4 v = f.v4[i];
         v.w = (float)(f.m[i] / rn);
         f.v4_alt[i] = v;
+        f.tk4[i].z = v.w;  // m/rho of this step for the conduction loop
       }
     } else if (p.fuse && i < p.n) {
       f.rho_alt[i] = f.rho[i];
@@ -763,26 +764,26 @@ struct CondG {
     float cs;
   };
   struct Own {
-    float x, y, z, k, T;
+    float x, y, z, k, T, mdj;
     HConstF hc;
   };
   struct Nb {
-    float2 tk;
-    float mdj, h;
+    float4 tk;  // T, k, m/rho
+    float h;
   };
   typedef KSum Acc;
   __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx32& c) {
     a.zero();
     if (!act) return;
     pos32(f, i, c, o.x, o.y, o.z);
-    const float2 tk = f.tk2[i];
+    const float4 tk = f.tk4[i];
     o.T = tk.x;
     o.k = tk.y;
+    o.mdj = tk.z;
     o.hc = hconst_f(f.h32[i]);
   }
   __device__ static void load(const Fields& f, int j, Nb& nb) {
-    nb.tk = f.tk2[j];
-    nb.mdj = f.v4[j].w;
+    nb.tk = f.tk4[j];
     nb.h = UH ? 0.f : f.h32[j];
   }
   __device__ static void pair(float x, float y, float z, const Nb& nb, const Own& o, Acc& a,
@@ -793,7 +794,7 @@ struct CondG {
     const float GK = gk_pair_f<UH>(r2, nb.h, o.hc, h_);
     const float kj = nb.tk.y;
     const float kh = (kj == o.k) ? 2.f * o.k : 4.f * o.k * kj * __builtin_amdgcn_rcpf(o.k + kj);
-    a.add(nb.mdj * kh * (o.T - nb.tk.x) * GK);
+    a.add(nb.tk.z * kh * (o.T - nb.tk.x) * GK);
   }
   __device__ static void finish(const Fields& f, int i, bool act, const Own& o, const Acc& a,
                                 const Prm& p) {
@@ -824,7 +825,7 @@ struct CondG {
         Tn = (tan + ta) / 2.;
       }
       f.T_alt[i] = Tn;
-      f.tk2_alt[i] = make_float2((float)Tn, o.k);
+      f.tk4_alt[i] = make_float4((float)Tn, o.k, o.mdj, 0.f);
     }
   }
 };
