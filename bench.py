@@ -69,7 +69,7 @@ def profiled_traffic(workload, kernel_id):
     """HBM bytes per launch of a kernel from the committed rocprofv3 summary of
     this workload (FETCH_SIZE + WRITE_SIZE, each scaled by what the counter
     reads for a copy kernel of known size), or None when there is none."""
-    path = os.path.join(ROOT, "profiles", f"{workload}_jc_mixed_step_kernels_traffic.json")
+    path = os.path.join(ROOT, "profiles", f"{workload}_jc_step_kernels_traffic.json")
     try:
         with open(path) as fh:
             k = json.load(fh)["kernels"].get(kernel_id)
