@@ -340,11 +340,13 @@ int wf_domain::alloc_plane_f(float** p) {
                      (unsigned char*)q, (int64_t)(sizeof(float) * npad));
   return WF_OK;
 }
-// mixed precision: rebuild every FP32 mirror when the FP64 state changed
-// outside the step's own producers (uploads, rebuilds, single-kernel calls)
+// rebuild every FP32 mirror (mixed) or FP64 neighbour record when the FP64
+// state changed outside the step's own producers (uploads, rebuilds,
+// single-kernel calls)
 void wf_domain::refresh_mirrors() {
-  if (!mixed || !mirror_dirty || !nb_built) return;
-  launch_mirror32(stream, f, n, frame);
+  if (!mirror_dirty || !nb_built) return;
+  if (mixed) launch_mirror32(stream, f, n, frame);
+  else launch_records64(stream, f, n);
   mirror_dirty = false;
 }
 int wf_domain::alloc_plane_i(int** p) {
@@ -550,6 +552,11 @@ int wf_create(const wf_config* cfg, wf_domain** out) {
     if (!rc) rc = d->dev_alloc((void**)&d->f.tk4, sizeof(float4) * d->npad);
     if (!rc) rc = d->dev_alloc((void**)&d->f.tk4_alt, sizeof(float4) * d->npad);
     if (!rc) rc = d->alloc_plane_i(&d->f.cell);
+  } else {
+    double4** d4[] = {&d->f.kv, &d->f.kv_alt, &d->f.tk64, &d->f.tk64_alt};
+    for (double4** q : d4)
+      if (!rc) rc = d->dev_alloc((void**)q, sizeof(double4) * d->npad);
+    if (!rc) rc = d->dev_alloc((void**)&d->f.arec64, 3 * sizeof(double4) * d->npad);
   }
   // return-map queue, surrogate samples and weights
   d->use_queue = cfg->return_map == WF_RETURN_NEWTON && cfg->mat.model == WF_MAT_JOHNSON_COOK;
@@ -916,13 +923,12 @@ int wf_get_neighbors(wf_domain* d, int* counts, int* offsets, int* list, int64_t
 int wf_calc_dens_inc(wf_domain* d) {
   if (!d) return fail(WF_E_ARG, "null domain");
   NEED_NB(d);
-  if (d->mixed) {
-    d->refresh_mirrors();
+  d->refresh_mirrors();
+  if (d->mixed)
     launch_dens_mixed(d->stream, d->f, d->tl, d->n, d->solid, d->cfg.dt, false, d->frame.cs32,
                       d->uniform_h);
-  } else {
-    launch_dens_tiled(d->stream, d->f, d->tl, d->n, d->solid, d->cfg.dt, false);
-  }
+  else
+    launch_dens_tiled(d->stream, d->f, d->tl, d->n, d->solid, d->cfg.dt, false, d->uniform_h);
   HIPC(hipGetLastError());
   return WF_OK;
 }
@@ -936,13 +942,12 @@ int wf_update_density(wf_domain* d, double dt) {
 int wf_calc_rate_tensors(wf_domain* d) {
   if (!d) return fail(WF_E_ARG, "null domain");
   NEED_NB(d);
-  if (d->mixed) {
-    d->refresh_mirrors();
+  d->refresh_mirrors();
+  if (d->mixed)
     launch_rates_mixed(d->stream, d->f, d->tl, d->solid, d->cfg.quirks, d->frame.cs32,
                        d->uniform_h);
-  } else {
-    launch_rates_tiled(d->stream, d->f, d->tl, d->solid, d->cfg.quirks);
-  }
+  else
+    launch_rates_tiled(d->stream, d->f, d->tl, d->solid, d->cfg.quirks, d->uniform_h);
   HIPC(hipGetLastError());
   return WF_OK;
 }
@@ -955,11 +960,11 @@ int wf_pressure(wf_domain* d) {
 int wf_stress_strain(wf_domain* d) {
   if (!d) return fail(WF_E_ARG, "null domain");
   const StepParams p = d->params();
-  launch_stress(d->stream, d->f, p, d->bstats, false, true, false,
+  launch_stress(d->stream, d->f, p, d->bstats, false, true, kRecNone,
                 d->use_queue ? &d->rq : nullptr);
   if (d->use_queue)
     launch_return(d->stream, d->f, p, d->bstats, d->rq, d->mlp_w,
-                  d->cfg.surrogate && d->mlp_trained, false, d->step);
+                  d->cfg.surrogate && d->mlp_trained, kRecNone, d->step);
   launch_fold_stats(d->stream, d->bstats, (p.solid + kBlock - 1) / kBlock, d->ctr);
   d->mirror_dirty = true;
   HIPC(hipGetLastError());
@@ -968,13 +973,13 @@ int wf_stress_strain(wf_domain* d) {
 int wf_calc_accel(wf_domain* d) {
   if (!d) return fail(WF_E_ARG, "null domain");
   NEED_NB(d);
-  if (d->mixed) {
-    d->refresh_mirrors();
+  d->refresh_mirrors();
+  if (d->mixed)
     launch_accel_mixed(d->stream, d->f, d->tl, d->solid, d->cfg.alpha, d->cfg.beta,
                        d->frame.cs32, d->uniform_h);
-  } else {
-    launch_accel_tiled(d->stream, d->f, d->tl, d->solid, d->cfg.alpha, d->cfg.beta);
-  }
+  else
+    launch_accel_tiled(d->stream, d->f, d->tl, d->solid, d->cfg.alpha, d->cfg.beta,
+                       d->uniform_h);
   HIPC(hipGetLastError());
   return WF_OK;
 }
@@ -1002,28 +1007,26 @@ int wf_update_vel(wf_domain* d, double dt) {
 int wf_calc_temp_inc(wf_domain* d) {
   if (!d) return fail(WF_E_ARG, "null domain");
   NEED_NB(d);
-  if (d->mixed) {
-    d->refresh_mirrors();
+  d->refresh_mirrors();
+  if (d->mixed)
     launch_cond_mixed(d->stream, d->f, d->tl, d->n, d->cfg.T_inf, d->cfg.h_conv, d->cfg.dt,
                       false, false, d->frame.cs32, d->uniform_h);
-  } else {
+  else
     launch_cond_tiled(d->stream, d->f, d->tl, d->n, d->cfg.T_inf, d->cfg.h_conv, d->cfg.dt, false,
-                      false);
-  }
+                      false, d->uniform_h);
   HIPC(hipGetLastError());
   return WF_OK;
 }
 int wf_thermal_calcs(wf_domain* d, double dt) {
   if (!d) return fail(WF_E_ARG, "null domain");
   NEED_NB(d);
-  if (d->mixed) {
-    d->refresh_mirrors();
+  d->refresh_mirrors();
+  if (d->mixed)
     launch_cond_mixed(d->stream, d->f, d->tl, d->n, d->cfg.T_inf, d->cfg.h_conv, dt, false,
                       false, d->frame.cs32, d->uniform_h);
-  } else {
+  else
     launch_cond_tiled(d->stream, d->f, d->tl, d->n, d->cfg.T_inf, d->cfg.h_conv, dt, false,
-                      false);
-  }
+                      false, d->uniform_h);
   launch_leapfrog(d->stream, d->f, d->n, dt, d->isfirst_step);
   d->mirror_dirty = true;
   d->isfirst_step = false;
@@ -1281,28 +1284,30 @@ int wf_step(wf_domain* d, int n_steps) {
         launch_dens_mixed(d->stream, d->f, d->tl, d->n, d->solid, p.dt, true, cs32, uh);
         std::swap(d->f.v4, d->f.v4_alt);
       } else {
-        launch_dens_tiled(d->stream, d->f, d->tl, d->n, d->solid, p.dt, true);
+        launch_dens_tiled(d->stream, d->f, d->tl, d->n, d->solid, p.dt, true, uh);
+        std::swap(d->f.kv, d->f.kv_alt);
       }
       std::swap(d->f.rho, d->f.rho_alt);
       if ((rc = timer_end(d, WF_K_DENS))) return rc;
       timer_begin(d);
       if (mx) launch_rates_mixed(d->stream, d->f, d->tl, d->solid, p.quirks, cs32, uh);
-      else launch_rates_tiled(d->stream, d->f, d->tl, d->solid, p.quirks);
+      else launch_rates_tiled(d->stream, d->f, d->tl, d->solid, p.quirks, uh);
       if ((rc = timer_end(d, WF_K_RATES))) return rc;
       timer_begin(d);
-      launch_stress(d->stream, d->f, p, d->bstats, true, true, mx,
+      const int rec = mx ? kRec32 : kRec64;
+      launch_stress(d->stream, d->f, p, d->bstats, true, true, rec,
                     d->use_queue ? &d->rq : nullptr);
       if ((rc = timer_end(d, WF_K_STRESS))) return rc;
       if (d->use_queue) {
         timer_begin(d);
         launch_return(d->stream, d->f, p, d->bstats, d->rq, d->mlp_w,
-                      d->cfg.surrogate && d->mlp_trained, mx, d->step);
+                      d->cfg.surrogate && d->mlp_trained, rec, d->step);
         if ((rc = timer_end(d, WF_K_RETURN))) return rc;
       }
       launch_fold_stats(d->stream, d->bstats, (p.solid + kBlock - 1) / kBlock, d->ctr);
       timer_begin(d);
       if (mx) launch_accel_mixed(d->stream, d->f, d->tl, d->solid, p.alpha, p.beta, cs32, uh);
-      else launch_accel_tiled(d->stream, d->f, d->tl, d->solid, p.alpha, p.beta);
+      else launch_accel_tiled(d->stream, d->f, d->tl, d->solid, p.alpha, p.beta, uh);
       if ((rc = timer_end(d, WF_K_ACCEL))) return rc;
       if (d->cfg.contact) {  // SolverFraser.cu:233-257
         timer_begin(d);
@@ -1325,11 +1330,11 @@ int wf_step(wf_domain* d, int n_steps) {
         if (mxp > d->cfg.ns_min_plastic) d->is_yielding = true;
       }
       timer_begin(d);
-      launch_integrate(d->stream, d->f, d->n, d->cfg.dt, d->bcs, mx ? &d->frame : nullptr);
+      launch_integrate(d->stream, d->f, d->n, d->cfg.dt, d->bcs, rec, d->frame);
       if ((rc = timer_end(d, WF_K_INTEGRATE))) return rc;
     }
     if (d->cfg.thermal) {
-      if (d->mixed) d->refresh_mirrors();
+      d->refresh_mirrors();
       timer_begin(d);
       if (d->mixed) {
         launch_cond_mixed(d->stream, d->f, d->tl, d->n, p.T_inf, p.h_conv, p.dt, true,
@@ -1337,7 +1342,8 @@ int wf_step(wf_domain* d, int n_steps) {
         std::swap(d->f.tk4, d->f.tk4_alt);
       } else {
         launch_cond_tiled(d->stream, d->f, d->tl, d->n, p.T_inf, p.h_conv, p.dt, true,
-                          d->isfirst_step);
+                          d->isfirst_step, d->uniform_h);
+        std::swap(d->f.tk64, d->f.tk64_alt);
       }
       std::swap(d->f.T, d->f.T_alt);
       if ((rc = timer_end(d, WF_K_THERMAL))) return rc;

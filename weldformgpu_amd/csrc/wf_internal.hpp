@@ -40,6 +40,11 @@ struct Fields {
   float4 *v4, *v4_alt, *arec;
   float4 *tk4, *tk4_alt;
   int* cell;
+  // the same packed neighbour records in FP64 for the FP64 pair loops
+  // (WF_FP64): kv = {v, m/rho} (double-buffered), arec64[3i..3i+2] = {v, m},
+  // sigma/rho^2 {xx, yy, zz, xy}, {sigma/rho^2 yz, xz, rho, Cs}, and
+  // tk64 = {T, k, m/rho, 0} (double-buffered)
+  double4 *kv, *kv_alt, *arec64, *tk64, *tk64_alt;
   // return-map surrogate training samples, one slot per particle: features
   // (s_trial/A, 10 eps_p, log10(max(rate,1e-3))/3, theta), target dg/dg_max,
   // step of the sample (-1: none)
@@ -50,6 +55,9 @@ struct Fields {
   double *nrm[3], *cf[3];
   int* idlock;
 };
+
+// which packed neighbour records a producer kernel also writes
+enum { kRecNone = 0, kRec32 = 1, kRec64 = 2 };
 
 // ---- rigid triangle meshes (wf_contact.hip) --------------------------------
 struct Counters;
@@ -183,14 +191,15 @@ struct Counters {
 };
 
 // ---- LDS-tiled pair loops (wf_pair.hip) ----------------------------------
+// FP64 pair loops (uh: every particle has the same h)
 void launch_dens_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int n, int solid,
-                       double dt, bool fuse);
+                       double dt, bool fuse, bool uh);
 void launch_rates_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
-                        int quirks);
+                        int quirks, bool uh);
 void launch_accel_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
-                        double alpha, double beta);
+                        double alpha, double beta, bool uh);
 void launch_cond_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int n, double T_inf,
-                       double h_conv, double dt, bool leap, bool first);
+                       double h_conv, double dt, bool leap, bool first, bool uh);
 
 // mixed precision (FP32 pair loops, compensated sums)
 // uh: every particle has the same h
@@ -202,19 +211,21 @@ void launch_accel_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int
                         double alpha, double beta, float cs32, bool uh);
 void launch_cond_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int n, double T_inf,
                        double h_conv, double dt, bool leap, bool first, float cs32, bool uh);
-// refresh every FP32 mirror from the FP64 state (after uploads and rebuilds)
+// refresh every FP32 mirror / FP64 record from the FP64 state (after uploads
+// and rebuilds)
 void launch_mirror32(hipStream_t s, const Fields& f, int n, const Frame32& fr);
+void launch_records64(hipStream_t s, const Fields& f, int n);
 
 // ---- per-particle kernels (wf_kernels.hip) --------------------------------
 void launch_update_density(hipStream_t s, const Fields& f, const StepParams& p, double dt);
 void launch_pressure(hipStream_t s, const Fields& f, const StepParams& p);
 void launch_stress(hipStream_t s, const Fields& f, const StepParams& p, BlockStats* bs,
-                   bool fuse_pressure, bool energy, bool mirror32, const RetQueue* q);
+                   bool fuse_pressure, bool energy, int rec, const RetQueue* q);
 // fold the per-block statistics of the last stress (+ return) pass into ctr
 void launch_fold_stats(hipStream_t s, const BlockStats* bs, int nblocks, Counters* ctr);
 // ---- return map and surrogate (wf_return.hip, wf_mlp.hip) -----------------
 void launch_return(hipStream_t s, const Fields& f, const StepParams& p, BlockStats* bs,
-                   const RetQueue& q, const float* mlp_w, bool use_mlp, bool mirror32,
+                   const RetQueue& q, const float* mlp_w, bool use_mlp, int rec,
                    int64_t step);
 // MFMA inference of the surrogate on given feature rows (test hook)
 void launch_mlp_infer(hipStream_t s, const float* w, const float* X, int n, float* y);
@@ -231,8 +242,9 @@ void launch_apply_bc(hipStream_t s, const Fields& f, int n, int zone, double vx,
                      double vz);
 void launch_update_pos(hipStream_t s, const Fields& f, int n, double dt);
 void launch_update_vel(hipStream_t s, const Fields& f, int n, double dt);
+// rec kRec32: also refresh xr and v4 (fr is the FP32 frame); kRec64: kv
 void launch_integrate(hipStream_t s, const Fields& f, int n, double dt, const BcTable& bc,
-                      const Frame32* fr /* non-null: also refresh xr, v32 */);
+                      int rec, const Frame32& fr);
 void launch_leapfrog(hipStream_t s, const Fields& f, int n, double dt, bool first);
 void launch_int_energy(hipStream_t s, const Fields& f, const StepParams& p);
 void launch_sum(hipStream_t s, const double* v, int n, double* partial, int nblocks,

@@ -86,11 +86,11 @@ __device__ double jc_newton(const MatParams& mp, double s_trial, double G, doubl
 // ---------------------------------------------------------------------------
 // C: Jaumann stress update + return map, StressStrainOne (Mechanical.cu:458-554),
 // optionally with the pressure (FUSE_P), the per-particle internal-energy
-// increment (ENERGY, Energy.cu:5-19 with per-particle indexing) and the FP32
-// sigma/rho^2 mirror (MIR).  With a queue, yielding Johnson-Cook particles
+// increment (ENERGY, Energy.cu:5-19 with per-particle indexing) and the
+// packed FP32 or FP64 neighbour records of the pair loops (REC).  With a queue, yielding Johnson-Cook particles
 // under the Newton return map keep their trial state and are appended to the
 // queue; k_return finishes them.
-template <bool FUSE_P, bool ENERGY, bool MIR>
+template <bool FUSE_P, bool ENERGY, int REC>
 __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, BlockStats* bs,
                                                    RetQueue q, int use_queue) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -115,7 +115,9 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Block
     T9 WT = {W.xx, W.yx, W.zx, W.xy, W.yy, W.zy, W.xz, W.yz, W.zz};
     T9 SRT = tmul(S, WT, qp);
     T9 RS = tmul(W, S, qp);
-    if (MIR) f.arec[3 * (size_t)i] = f.v4[i];  // this step's {v, m/rho}
+    if (REC == kRec32) f.arec[3 * (size_t)i] = f.v4[i];  // this step's {v, m/rho}
+    if (REC == kRec64)
+      f.arec64[3 * (size_t)i] = make_double4(f.v[0][i], f.v[1][i], f.v[2][i], f.m[i]);
     const double G = f.G[i];
     const double tr3 = 1.0 / 3.0 * (D.xx + D.yy + D.zz);
     const double g2 = 2.0 * G;
@@ -219,13 +221,20 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Block
       f.sigma[3][i] = S.xy;
       f.sigma[4][i] = S.yz;
       f.sigma[5][i] = S.xz;
-      if (MIR) {  // sigma/rho^2, rho and Cs for the FP32 acceleration loop
+      if (REC == kRec32) {  // sigma/rho^2, rho and Cs for the FP32 acceleration loop
         const double rho = f.rho[i], ir2 = 1.0 / (rho * rho);
         f.arec[3 * (size_t)i + 1] =
             make_float4((float)((-pi + S.xx) * ir2), (float)((-pi + S.yy) * ir2),
                         (float)((-pi + S.zz) * ir2), (float)(S.xy * ir2));
         f.arec[3 * (size_t)i + 2] =
             make_float4((float)(S.yz * ir2), (float)(S.xz * ir2), (float)rho, (float)f.Cs[i]);
+      }
+      if (REC == kRec64) {  // the same in FP64 for the FP64 acceleration loop
+        const double rho = f.rho[i], ir2 = 1.0 / (rho * rho);
+        f.arec64[3 * (size_t)i + 1] = make_double4(f.sigma[0][i] * ir2, f.sigma[1][i] * ir2,
+                                                   f.sigma[2][i] * ir2, f.sigma[3][i] * ir2);
+        f.arec64[3 * (size_t)i + 2] =
+            make_double4(f.sigma[4][i] * ir2, f.sigma[5][i] * ir2, rho, f.Cs[i]);
       }
       if (ENERGY) {
         double de = f.m[i] / f.rho[i] *
@@ -312,7 +321,7 @@ __global__ __launch_bounds__(256) void k_fold_stats(const BlockStats* bs, int nb
 // ---------------------------------------------------------------------------
 // F: velocity BCs + UpdatePosFraser + UpdateVel + BCs in one pass
 // (SolverFraser.cu:273-281; Mechanical.cu:142-191,572-584)
-template <bool MIR>
+template <int REC>
 __global__ __launch_bounds__(kBlock) void k_integrate(Fields f, int n, double dt, BcTable bc,
                                                      Frame32 fr) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -356,7 +365,8 @@ __global__ __launch_bounds__(kBlock) void k_integrate(Fields f, int n, double dt
   f.a[0][i] = a0;
   f.a[1][i] = a1;
   f.a[2][i] = a2;
-  if (MIR) {
+  if (REC == kRec64) f.kv[i] = make_double4(v0, v1, v2, f.kv[i].w);
+  if (REC == kRec32) {
     int cx, cy, cz;
     unpack_cell(f.cell[i], cx, cy, cz);
     f.xr[0][i] = (float)(f.x[0][i] - (fr.org[0] + cx * fr.cs));
@@ -397,6 +407,24 @@ __global__ __launch_bounds__(kBlock) void k_mirror32(Fields f, int n, Frame32 fr
   const float4 tk = make_float4((float)f.T[i], (float)f.k[i], v4.w, 0.f);
   f.tk4[i] = tk;
   f.tk4_alt[i] = tk;
+}
+
+// every FP64 neighbour record from the FP64 state
+__global__ __launch_bounds__(kBlock) void k_records64(Fields f, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double rho = f.rho[i], mdj = f.m[i] / rho, ir2 = 1.0 / (rho * rho);
+  const double4 kv = make_double4(f.v[0][i], f.v[1][i], f.v[2][i], mdj);
+  f.kv[i] = kv;
+  f.kv_alt[i] = kv;
+  double4* r = f.arec64 + 3 * (size_t)i;
+  r[0] = make_double4(f.v[0][i], f.v[1][i], f.v[2][i], f.m[i]);
+  r[1] = make_double4(f.sigma[0][i] * ir2, f.sigma[1][i] * ir2, f.sigma[2][i] * ir2,
+                      f.sigma[3][i] * ir2);
+  r[2] = make_double4(f.sigma[4][i] * ir2, f.sigma[5][i] * ir2, rho, f.Cs[i]);
+  const double4 tk = make_double4(f.T[i], f.k[i], mdj, 0.0);
+  f.tk64[i] = tk;
+  f.tk64_alt[i] = tk;
 }
 
 __global__ __launch_bounds__(kBlock) void k_apply_bc(Fields f, int n, int zone, double vx,
@@ -503,21 +531,23 @@ void launch_pressure(hipStream_t s, const Fields& f, const StepParams& p) {
   hipLaunchKernelGGL(k_pressure, dim3(grid_for(p.n)), dim3(kBlock), 0, s, f, p.n);
 }
 void launch_stress(hipStream_t s, const Fields& f, const StepParams& p, BlockStats* bs,
-                   bool fuse_pressure, bool energy, bool mirror32, const RetQueue* q) {
+                   bool fuse_pressure, bool energy, int rec, const RetQueue* q) {
   if (p.solid <= 0) return;
   dim3 g(grid_for(p.solid)), b(kBlock);
   RetQueue qq = q ? *q : RetQueue{};
   int uq = q ? 1 : 0;
-  if (mirror32)
-    hipLaunchKernelGGL((k_stress<true, true, true>), g, b, 0, s, f, p, bs, qq, uq);
+  if (rec == kRec32)
+    hipLaunchKernelGGL((k_stress<true, true, kRec32>), g, b, 0, s, f, p, bs, qq, uq);
+  else if (rec == kRec64)
+    hipLaunchKernelGGL((k_stress<true, true, kRec64>), g, b, 0, s, f, p, bs, qq, uq);
   else if (fuse_pressure && energy)
-    hipLaunchKernelGGL((k_stress<true, true, false>), g, b, 0, s, f, p, bs, qq, uq);
+    hipLaunchKernelGGL((k_stress<true, true, kRecNone>), g, b, 0, s, f, p, bs, qq, uq);
   else if (fuse_pressure)
-    hipLaunchKernelGGL((k_stress<true, false, false>), g, b, 0, s, f, p, bs, qq, uq);
+    hipLaunchKernelGGL((k_stress<true, false, kRecNone>), g, b, 0, s, f, p, bs, qq, uq);
   else if (energy)
-    hipLaunchKernelGGL((k_stress<false, true, false>), g, b, 0, s, f, p, bs, qq, uq);
+    hipLaunchKernelGGL((k_stress<false, true, kRecNone>), g, b, 0, s, f, p, bs, qq, uq);
   else
-    hipLaunchKernelGGL((k_stress<false, false, false>), g, b, 0, s, f, p, bs, qq, uq);
+    hipLaunchKernelGGL((k_stress<false, false, kRecNone>), g, b, 0, s, f, p, bs, qq, uq);
 }
 void launch_fold_stats(hipStream_t s, const BlockStats* bs, int nblocks, Counters* ctr) {
   if (nblocks <= 0) return;
@@ -537,14 +567,19 @@ void launch_update_vel(hipStream_t s, const Fields& f, int n, double dt) {
   hipLaunchKernelGGL(k_update_vel, dim3(grid_for(n)), dim3(kBlock), 0, s, f, n, dt);
 }
 void launch_integrate(hipStream_t s, const Fields& f, int n, double dt, const BcTable& bc,
-                      const Frame32* fr) {
+                      int rec, const Frame32& fr) {
   if (n <= 0) return;
-  if (fr)
-    hipLaunchKernelGGL(k_integrate<true>, dim3(grid_for(n)), dim3(kBlock), 0, s, f, n, dt, bc,
-                       *fr);
+  const dim3 g(grid_for(n)), b(kBlock);
+  if (rec == kRec32)
+    hipLaunchKernelGGL(k_integrate<kRec32>, g, b, 0, s, f, n, dt, bc, fr);
+  else if (rec == kRec64)
+    hipLaunchKernelGGL(k_integrate<kRec64>, g, b, 0, s, f, n, dt, bc, fr);
   else
-    hipLaunchKernelGGL(k_integrate<false>, dim3(grid_for(n)), dim3(kBlock), 0, s, f, n, dt, bc,
-                       Frame32{});
+    hipLaunchKernelGGL(k_integrate<kRecNone>, g, b, 0, s, f, n, dt, bc, fr);
+}
+void launch_records64(hipStream_t s, const Fields& f, int n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_records64, dim3(grid_for(n)), dim3(kBlock), 0, s, f, n);
 }
 void launch_mirror32(hipStream_t s, const Fields& f, int n, const Frame32& fr) {
   if (n <= 0) return;

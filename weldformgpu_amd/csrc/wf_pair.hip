@@ -153,20 +153,49 @@ __device__ __forceinline__ double gk_pair(double r2, double hj, const HConst& hc
   return C * rinv * poly;
 }
 
+// gather-kernel traits of a policy: the staged position type, positions per
+// LDS pass, the per-tile context and a neighbour's staged position
+constexpr int kStage64 = 1024;  // 24 KB of FP64 positions
+#define WF_GATHER64                                                                  \
+  typedef double PT;                                                                 \
+  static constexpr int STAGE = kStage64;                                             \
+  typedef NoCtx Ctx;                                                                 \
+  template <class P>                                                                 \
+  __device__ static Ctx ctx(const Fields&, int, const P&) {                          \
+    return Ctx{};                                                                    \
+  }                                                                                  \
+  __device__ static void pos(const Fields& f, int j, const Ctx&, double& x, double& y, \
+                             double& z) {                                            \
+    x = f.x[0][j];                                                                   \
+    y = f.x[1][j];                                                                   \
+    z = f.x[2][j];                                                                   \
+  }
+
 // ---------------------------------------------------------------------------
+// FP64 pair loops (WF_FP64) on the gather kernel below: FP64 positions of the
+// staged set in LDS, the neighbour fields from packed FP64 records.  Each
+// thread sums its pairs in ascending staged order, as the LDS-staged
+// formulation does, so the sums are the same operations in the same order.
+
 // A: density rate drho_i = rho_i sum_j (m_j/rho_j) v_ij . (GK x_ij)
-// (InteractionAlt.cu:166-246) [+ update rho_i + dt drho_i into rho_alt]
-struct DensK {
-  static constexpr int NF = 8;  // x y z vx vy vz h m/rho
+// (InteractionAlt.cu:166-246) [+ update rho_i + dt drho_i into rho_alt and
+// the next {v, m/rho} record]
+template <bool UH>
+struct DensD {
+  WF_GATHER64
+  static constexpr int NBATCH = 4;
   struct Prm {
     int limit, n;
     double dt;
     int fuse;
   };
-  WF_FP64_POLICY
   struct Own {
     double x, y, z, vx, vy, vz, rho;
     HConst hc;
+  };
+  struct Nb {
+    double4 v;  // v, m/rho
+    double h;
   };
   struct Acc {
     double s;
@@ -179,47 +208,54 @@ struct DensK {
     o.rho = f.rho[i];
     o.hc = hconst(f.h[i]);
   }
-  __device__ static void stage(const Fields& f, int j, double (*L)[kChunk64], int s,
-                               const Ctx&) {
-    L[0][s] = f.x[0][j]; L[1][s] = f.x[1][j]; L[2][s] = f.x[2][j];
-    L[3][s] = f.v[0][j]; L[4][s] = f.v[1][j]; L[5][s] = f.v[2][j];
-    L[6][s] = f.h[j]; L[7][s] = f.m[j] / f.rho[j];
+  __device__ static void load(const Fields& f, int j, Nb& nb) {
+    nb.v = f.kv[j];
+    nb.h = UH ? 0.0 : f.h[j];
   }
-  __device__ static void pair(double (*L)[kChunk64], int s, const Own& o, Acc& a, const Prm&) {
-    const double xx = o.x - L[0][s], xy = o.y - L[1][s], xz = o.z - L[2][s];
+  __device__ static void pair(double x, double y, double z, const Nb& nb, const Own& o, Acc& a,
+                              const Prm&) {
+    const double xx = o.x - x, xy = o.y - y, xz = o.z - z;
     const double r2 = xx * xx + xy * xy + xz * xz;
     double rinv, h_;
-    const double GK = gk_pair(r2, L[6][s], o.hc, rinv, h_);
-    const double vx = o.vx - L[3][s], vy = o.vy - L[4][s], vz = o.vz - L[5][s];
-    a.s += L[7][s] * (GK * (vx * xx + vy * xy + vz * xz));
+    const double GK = gk_pair(r2, UH ? o.hc.h : nb.h, o.hc, rinv, h_);
+    const double vx = o.vx - nb.v.x, vy = o.vy - nb.v.y, vz = o.vz - nb.v.z;
+    a.s += nb.v.w * (GK * (vx * xx + vy * xy + vz * xz));
   }
   __device__ static void finish(const Fields& f, int i, bool act, const Own& o, const Acc& a,
                                 const Prm& p) {
     if (act) {
       const double drho = o.rho * a.s;
       f.drho[i] = drho;
-      if (p.fuse) f.rho_alt[i] = o.rho + p.dt * drho;
+      if (p.fuse) {
+        const double rn = o.rho + p.dt * drho;
+        f.rho_alt[i] = rn;
+        const double mdj = f.m[i] / rn;
+        f.kv_alt[i] = make_double4(o.vx, o.vy, o.vz, mdj);
+        f.tk64[i].z = mdj;  // m/rho of this step for the conduction loop
+      }
     } else if (p.fuse && i < p.n) {
       f.rho_alt[i] = f.rho[i];
+      f.kv_alt[i] = f.kv[i];
     }
   }
 };
 
-// ---------------------------------------------------------------------------
 // B: strain rate -1/2 sum (m_j/rho_j) GK (v_ij (x) x_ij + x_ij (x) v_ij) and
 // rotation rate -1/2 sum (m_j/rho_j) GK (v_ij (x) x_ij - x_ij (x) v_ij),
 // upper triangle, xz optionally negated (InteractionAlt.cu:262-383)
-struct RatesK {
-  static constexpr int NF = 8;
+template <bool UH>
+struct RatesD {
+  WF_GATHER64
+  static constexpr int NBATCH = 4;
   struct Prm {
     int limit;
     double xz_sign;
   };
-  WF_FP64_POLICY
   struct Own {
     double x, y, z, vx, vy, vz;
     HConst hc;
   };
+  typedef typename DensD<UH>::Nb Nb;
   struct Acc {
     double dxx, dyy, dzz, dxy, dyz, dxz, wxy, wyz, wxz;
   };
@@ -230,16 +266,14 @@ struct RatesK {
     o.vx = f.v[0][i]; o.vy = f.v[1][i]; o.vz = f.v[2][i];
     o.hc = hconst(f.h[i]);
   }
-  __device__ static void stage(const Fields& f, int j, double (*L)[kChunk64], int s,
-                               const Ctx&) {
-    DensK::stage(f, j, L, s, NoCtx{});
-  }
-  __device__ static void pair(double (*L)[kChunk64], int s, const Own& o, Acc& a, const Prm&) {
-    const double xx = o.x - L[0][s], xy = o.y - L[1][s], xz = o.z - L[2][s];
+  __device__ static void load(const Fields& f, int j, Nb& nb) { DensD<UH>::load(f, j, nb); }
+  __device__ static void pair(double x, double y, double z, const Nb& nb, const Own& o, Acc& a,
+                              const Prm&) {
+    const double xx = o.x - x, xy = o.y - y, xz = o.z - z;
     const double r2 = xx * xx + xy * xy + xz * xz;
     double rinv, h_;
-    const double w = gk_pair(r2, L[6][s], o.hc, rinv, h_) * L[7][s];
-    const double vx = o.vx - L[3][s], vy = o.vy - L[4][s], vz = o.vz - L[5][s];
+    const double w = gk_pair(r2, UH ? o.hc.h : nb.h, o.hc, rinv, h_) * nb.v.w;
+    const double vx = o.vx - nb.v.x, vy = o.vy - nb.v.y, vz = o.vz - nb.v.z;
     const double ax = w * vx, ay = w * vy, az = w * vz;
     a.dxx += ax * xx;
     a.dyy += ay * xy;
@@ -263,21 +297,25 @@ struct RatesK {
   }
 };
 
-// ---------------------------------------------------------------------------
 // D: a_i = sum_j m_j (sigma_i/rho_i^2 + sigma_j/rho_j^2 + Pi_ij I) . (GK x_ij),
 // Pi_ij = (alpha C mu + beta mu^2)/rho_bar for approaching pairs,
-// mu = h v.x/(r^2 + 0.01 h^2) (InteractionAlt.cu:18-150).  sigma/rho^2 is
-// formed once per staged particle.
-struct AccelK {
-  static constexpr int NF = 16;  // x y z vx vy vz h m rho Cs (sigma/rho^2)[6]
+// mu = h v.x/(r^2 + 0.01 h^2) (InteractionAlt.cu:18-150).  sigma_j/rho_j^2
+// comes formed in the neighbour's record.
+template <bool UH>
+struct AccelD {
+  WF_GATHER64
+  static constexpr int NBATCH = 2;
   struct Prm {
     int limit;
     double alpha, beta;
   };
-  WF_FP64_POLICY
   struct Own {
     double x, y, z, vx, vy, vz, di, Ci, s0, s1, s2, s3, s4, s5;
     HConst hc;
+  };
+  struct Nb {
+    double4 v, sa, sb;  // {v, m}, sigma/rho^2 xx yy zz xy, {yz, xz, rho, Cs}
+    double h;
   };
   struct Acc {
     double ax, ay, az;
@@ -293,36 +331,32 @@ struct AccelK {
     o.s0 = f.sigma[0][i] * idi; o.s1 = f.sigma[1][i] * idi; o.s2 = f.sigma[2][i] * idi;
     o.s3 = f.sigma[3][i] * idi; o.s4 = f.sigma[4][i] * idi; o.s5 = f.sigma[5][i] * idi;
   }
-  __device__ static void stage(const Fields& f, int j, double (*L)[kChunk64], int s,
-                               const Ctx&) {
-    L[0][s] = f.x[0][j]; L[1][s] = f.x[1][j]; L[2][s] = f.x[2][j];
-    L[3][s] = f.v[0][j]; L[4][s] = f.v[1][j]; L[5][s] = f.v[2][j];
-    const double rj = f.rho[j];
-    L[6][s] = f.h[j]; L[7][s] = f.m[j]; L[8][s] = rj; L[9][s] = f.Cs[j];
-    const double idj = 1.0 / (rj * rj);
-    L[10][s] = f.sigma[0][j] * idj; L[11][s] = f.sigma[1][j] * idj;
-    L[12][s] = f.sigma[2][j] * idj; L[13][s] = f.sigma[3][j] * idj;
-    L[14][s] = f.sigma[4][j] * idj; L[15][s] = f.sigma[5][j] * idj;
+  __device__ static void load(const Fields& f, int j, Nb& nb) {
+    const double4* r = f.arec64 + 3 * (size_t)j;
+    nb.v = r[0];
+    nb.sa = r[1];
+    nb.sb = r[2];
+    nb.h = UH ? 0.0 : f.h[j];
   }
-  __device__ static void pair(double (*L)[kChunk64], int s, const Own& o, Acc& a,
+  __device__ static void pair(double x, double y, double z, const Nb& nb, const Own& o, Acc& a,
                               const Prm& p) {
-    const double xx = o.x - L[0][s], xy = o.y - L[1][s], xz = o.z - L[2][s];
+    const double xx = o.x - x, xy = o.y - y, xz = o.z - z;
     const double r2 = xx * xx + xy * xy + xz * xz;
     double rinv, h_;
-    const double GK = gk_pair(r2, L[6][s], o.hc, rinv, h_);
-    const double vx = o.vx - L[3][s], vy = o.vy - L[4][s], vz = o.vz - L[5][s];
+    const double GK = gk_pair(r2, UH ? o.hc.h : nb.h, o.hc, rinv, h_);
+    const double vx = o.vx - nb.v.x, vy = o.vy - nb.v.y, vz = o.vz - nb.v.z;
     const double vdx = vx * xx + vy * xy + vz * xz;
     double PI = 0.0;
     if (vdx < 0.0 && (p.alpha != 0.0 || p.beta != 0.0)) {
       const double MU = h_ * vdx / (r2 + 0.01 * h_ * h_);
-      const double Cij = 0.5 * (o.Ci + L[9][s]);
-      PI = (p.alpha * Cij * MU + p.beta * MU * MU) / (0.5 * (o.di + L[8][s]));
+      const double Cij = 0.5 * (o.Ci + nb.sb.w);
+      PI = (p.alpha * Cij * MU + p.beta * MU * MU) / (0.5 * (o.di + nb.sb.z));
     }
-    const double g = GK * L[7][s];
+    const double g = GK * nb.v.w;
     const double gx = g * xx, gy = g * xy, gz = g * xz;
-    const double mxx = o.s0 + L[10][s] + PI, myy = o.s1 + L[11][s] + PI;
-    const double mzz = o.s2 + L[12][s] + PI;
-    const double mxy = o.s3 + L[13][s], myz = o.s4 + L[14][s], mxz = o.s5 + L[15][s];
+    const double mxx = o.s0 + nb.sa.x + PI, myy = o.s1 + nb.sa.y + PI;
+    const double mzz = o.s2 + nb.sa.z + PI;
+    const double mxy = o.s3 + nb.sa.w, myz = o.s4 + nb.sb.x, mxz = o.s5 + nb.sb.y;
     a.ax += mxx * gx + mxy * gy + mxz * gz;
     a.ay += mxy * gx + myy * gy + myz * gz;
     a.az += mxz * gx + myz * gy + mzz * gz;
@@ -336,22 +370,26 @@ struct AccelK {
   }
 };
 
-// ---------------------------------------------------------------------------
 // G: conduction dT_i/dt = 1/(rho_i cp_i) sum_j (m_j/rho_j) 4 k_i k_j/(k_i+k_j)
 // (T_i - T_j) GK  (Thermal.cu:101-142, where x_ij.(GK x_ij)/r^2 = GK), then
 // convection (Thermal.cu:36-64), plastic-work source and FIXED BC; optional
-// fused leapfrog (Thermal.cu:16-31) writing T into T_alt.
-struct CondK {
-  static constexpr int NF = 7;  // x y z h m/rho T k
+// fused leapfrog (Thermal.cu:16-31) writing T into T_alt and the next record.
+template <bool UH>
+struct CondD {
+  WF_GATHER64
+  static constexpr int NBATCH = 4;
   struct Prm {
     int limit;
     double T_inf, h_conv, dt;
     int leap, first;
   };
-  WF_FP64_POLICY
   struct Own {
-    double x, y, z, k, T;
+    double x, y, z, k, T, mdj;
     HConst hc;
+  };
+  struct Nb {
+    double4 tk;  // T, k, m/rho
+    double h;
   };
   struct Acc {
     double q;
@@ -361,21 +399,22 @@ struct CondK {
     if (!act) return;
     o.x = f.x[0][i]; o.y = f.x[1][i]; o.z = f.x[2][i];
     o.k = f.k[i]; o.T = f.T[i];
+    o.mdj = f.tk64[i].z;
     o.hc = hconst(f.h[i]);
   }
-  __device__ static void stage(const Fields& f, int j, double (*L)[kChunk64], int s,
-                               const Ctx&) {
-    L[0][s] = f.x[0][j]; L[1][s] = f.x[1][j]; L[2][s] = f.x[2][j];
-    L[3][s] = f.h[j]; L[4][s] = f.m[j] / f.rho[j]; L[5][s] = f.T[j]; L[6][s] = f.k[j];
+  __device__ static void load(const Fields& f, int j, Nb& nb) {
+    nb.tk = f.tk64[j];
+    nb.h = UH ? 0.0 : f.h[j];
   }
-  __device__ static void pair(double (*L)[kChunk64], int s, const Own& o, Acc& a, const Prm&) {
-    const double xx = o.x - L[0][s], xy = o.y - L[1][s], xz = o.z - L[2][s];
+  __device__ static void pair(double x, double y, double z, const Nb& nb, const Own& o, Acc& a,
+                              const Prm&) {
+    const double xx = o.x - x, xy = o.y - y, xz = o.z - z;
     const double r2 = xx * xx + xy * xy + xz * xz;
     double rinv, h_;
-    const double GK = gk_pair(r2, L[3][s], o.hc, rinv, h_);
-    const double kj = L[6][s];
+    const double GK = gk_pair(r2, UH ? o.hc.h : nb.h, o.hc, rinv, h_);
+    const double kj = nb.tk.y;
     const double kh = (kj == o.k) ? 2.0 * o.k : 4.0 * o.k * kj / (o.k + kj);
-    a.q += L[4][s] * kh * (o.T - L[5][s]) * GK;
+    a.q += nb.tk.z * kh * (o.T - nb.tk.x) * GK;
   }
   __device__ static void finish(const Fields& f, int i, bool act, const Own& o, const Acc& a,
                                 const Prm& p) {
@@ -394,16 +433,19 @@ struct CondK {
     if (bct == WF_BCT_FIXED) acc = 0.0;
     f.dTdt[i] = acc;
     if (p.leap) {
+      double Tn;
       if (p.first) {
         f.Ta[i] = o.T - p.dt / 2.0 * acc;
-        f.T_alt[i] = o.T;
+        Tn = o.T;
       } else {
         double ta = f.Ta[i];
         f.Tb[i] = ta;
         double tan = ta + acc * p.dt;
         f.Ta[i] = tan;
-        f.T_alt[i] = (tan + ta) / 2.;
+        Tn = (tan + ta) / 2.;
       }
+      f.T_alt[i] = Tn;
+      f.tk64_alt[i] = make_double4(Tn, o.k, o.mdj, 0.0);
     }
   }
 };
@@ -481,25 +523,38 @@ __device__ __forceinline__ void pos32(const Fields& f, int j, const Ctx32& c, fl
 // staged tile-frame positions per LDS pass (18 KB); a tile's staged set
 // averages ~1.1k particles on a cubic lattice
 constexpr int kStage32 = 1536;
+#define WF_GATHER32                                                                \
+  typedef float PT;                                                                \
+  static constexpr int STAGE = kStage32;                                           \
+  typedef Ctx32 Ctx;                                                               \
+  template <class P>                                                               \
+  __device__ static Ctx ctx(const Fields& f, int t, const P& p) {                  \
+    return ctx32(f, t, p.cs);                                                      \
+  }                                                                                \
+  __device__ static void pos(const Fields& f, int j, const Ctx& c, float& x, float& y, \
+                             float& z) {                                           \
+    pos32(f, j, c, x, y, z);                                                       \
+  }
 
 template <class K>
 __global__ __launch_bounds__(kTileP) void k_pair_gather(Fields f, TileLists tl,
                                                         typename K::Prm prm) {
-  __shared__ float lx[kStage32], ly[kStage32], lz[kStage32];
+  constexpr int ST = K::STAGE;
+  __shared__ typename K::PT lx[ST], ly[ST], lz[ST];
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
   const int i = t * kTileP + tid;
   const bool act = i < prm.limit;
-  const Ctx32 cx = ctx32(f, t, prm.cs);
+  const typename K::Ctx cx = K::ctx(f, t, prm);
   typename K::Own own;
   typename K::Acc acc;
   K::init(f, i, act, own, acc, cx);
   const int ub = tl.uoff[t], ucnt = tl.uoff[t + 1] - ub;
   const int* __restrict__ tj = tl.tj + ub;
   const int cnt = act ? tl.count[i] : 0;
-  if (ucnt <= kStage32) {
+  if (ucnt <= ST) {
     // the whole staged set fits: one pass, a quad of four slots at a time
-    for (int s = tid; s < ucnt; s += kTileP) pos32(f, tj[s], cx, lx[s], ly[s], lz[s]);
+    for (int s = tid; s < ucnt; s += kTileP) K::pos(f, tj[s], cx, lx[s], ly[s], lz[s]);
     __syncthreads();
     const uint2* __restrict__ q = tl.slotq + i;
     // the next quad's slots and neighbour indices are fetched before the
@@ -514,10 +569,12 @@ __global__ __launch_bounds__(kTileP) void k_pair_gather(Fields f, TileLists tl,
       const int sl[4] = {(int)(w.x & 0xFFFFu), (int)(w.x >> 16), (int)(w.y & 0xFFFFu),
                          (int)(w.y >> 16)};
       const int nk = min(4, cnt - k);
-      typename K::Nb nb[4];
+      // K::NBATCH neighbour records held at once (4, or 2 where registers
+      // would otherwise limit occupancy)
+      constexpr int NB = K::NBATCH;
+      int jc[4];
 #pragma unroll
-      for (int u = 0; u < 4; u++)
-        if (u < nk) K::load(f, jn[u], nb[u]);
+      for (int u = 0; u < 4; u++) jc[u] = jn[u];
       const int kn = k + 4;
       if (kn < cnt) {
         w = q[(int64_t)(kn >> 2) * tl.npad];
@@ -528,18 +585,26 @@ __global__ __launch_bounds__(kTileP) void k_pair_gather(Fields f, TileLists tl,
           if (kn + u < cnt) jn[u] = tj[ns[u]];
       }
 #pragma unroll
-      for (int u = 0; u < 4; u++)
-        if (u < nk) K::pair(lx[sl[u]], ly[sl[u]], lz[sl[u]], nb[u], own, acc, prm);
+      for (int u0 = 0; u0 < 4; u0 += NB) {
+        typename K::Nb nb[NB];
+#pragma unroll
+        for (int u = 0; u < NB; u++)
+          if (u0 + u < nk) K::load(f, jc[u0 + u], nb[u]);
+#pragma unroll
+        for (int u = 0; u < NB; u++)
+          if (u0 + u < nk)
+            K::pair(lx[sl[u0 + u]], ly[sl[u0 + u]], lz[sl[u0 + u]], nb[u], own, acc, prm);
+      }
     }
     K::finish(f, i, act, own, acc, prm);
     return;
   }
   SlotReader sr;
   sr.init(tl, i, cnt);
-  for (int c0 = 0; c0 < ucnt; c0 += kStage32) {
-    const int m = min(kStage32, ucnt - c0);
+  for (int c0 = 0; c0 < ucnt; c0 += ST) {
+    const int m = min(ST, ucnt - c0);
     __syncthreads();
-    for (int s = tid; s < m; s += kTileP) pos32(f, tj[c0 + s], cx, lx[s], ly[s], lz[s]);
+    for (int s = tid; s < m; s += kTileP) K::pos(f, tj[c0 + s], cx, lx[s], ly[s], lz[s]);
     __syncthreads();
     const int lim = c0 + m;
     // two neighbours per iteration while both fall in this pass: their
@@ -571,6 +636,8 @@ __global__ __launch_bounds__(kTileP) void k_pair_gather(Fields f, TileLists tl,
 // the next {v, m/rho} mirror
 template <bool UH>
 struct DensG {
+  WF_GATHER32
+  static constexpr int NBATCH = 4;
   struct Prm {
     int limit, n;
     double dt;
@@ -631,6 +698,8 @@ struct DensG {
 // B: strain and rotation rates (InteractionAlt.cu:262-383)
 template <bool UH>
 struct RatesG {
+  WF_GATHER32
+  static constexpr int NBATCH = 4;
   struct Prm {
     int limit;
     double xz_sign;
@@ -686,6 +755,8 @@ struct RatesG {
 // neighbour's mass is (m/rho) rho
 template <bool UH>
 struct AccelG {
+  WF_GATHER32
+  static constexpr int NBATCH = 4;
   struct Prm {
     int limit;
     double alpha, beta;
@@ -757,6 +828,8 @@ struct AccelG {
 // leapfrog; the next {T, k} mirror is written
 template <bool UH>
 struct CondG {
+  WF_GATHER32
+  static constexpr int NBATCH = 4;
   struct Prm {
     int limit;
     double T_inf, h_conv, dt;
@@ -882,30 +955,6 @@ struct SurfK {
 
 static inline int ntiles(int n) { return (n + kTileP - 1) / kTileP; }
 
-void launch_dens_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int n, int solid,
-                       double dt, bool fuse) {
-  if (n <= 0) return;
-  DensK::Prm p{solid, n, dt, fuse ? 1 : 0};
-  hipLaunchKernelGGL(k_pair_tiled<DensK>, dim3(ntiles(n)), dim3(kTileP), 0, s, f, tl, p);
-}
-void launch_rates_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
-                        int quirks) {
-  if (solid <= 0) return;
-  RatesK::Prm p{solid, (quirks & WF_Q_ROT_XZ_SIGN) ? -1.0 : 1.0};
-  hipLaunchKernelGGL(k_pair_tiled<RatesK>, dim3(ntiles(solid)), dim3(kTileP), 0, s, f, tl, p);
-}
-void launch_accel_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
-                        double alpha, double beta) {
-  if (solid <= 0) return;
-  AccelK::Prm p{solid, alpha, beta};
-  hipLaunchKernelGGL(k_pair_tiled<AccelK>, dim3(ntiles(solid)), dim3(kTileP), 0, s, f, tl, p);
-}
-void launch_cond_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int n, double T_inf,
-                       double h_conv, double dt, bool leap, bool first) {
-  if (n <= 0) return;
-  CondK::Prm p{n, T_inf, h_conv, dt, leap ? 1 : 0, first ? 1 : 0};
-  hipLaunchKernelGGL(k_pair_tiled<CondK>, dim3(ntiles(n)), dim3(kTileP), 0, s, f, tl, p);
-}
 void launch_surface_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
                           double ntot, double totmass, int id_free_surf) {
   if (solid <= 0) return;
@@ -929,6 +978,28 @@ static void launch_gather(hipStream_t s, const Fields& f, const TileLists& tl, i
   }
 }
 
+void launch_dens_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int n, int solid,
+                       double dt, bool fuse, bool uh) {
+  if (n <= 0) return;
+  launch_gather<DensD>(s, f, tl, ntiles(n), uh, solid, n, dt, fuse ? 1 : 0);
+}
+void launch_rates_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
+                        int quirks, bool uh) {
+  if (solid <= 0) return;
+  const double xz = (quirks & WF_Q_ROT_XZ_SIGN) ? -1.0 : 1.0;
+  launch_gather<RatesD>(s, f, tl, ntiles(solid), uh, solid, xz);
+}
+void launch_accel_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
+                        double alpha, double beta, bool uh) {
+  if (solid <= 0) return;
+  launch_gather<AccelD>(s, f, tl, ntiles(solid), uh, solid, alpha, beta);
+}
+void launch_cond_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int n, double T_inf,
+                       double h_conv, double dt, bool leap, bool first, bool uh) {
+  if (n <= 0) return;
+  launch_gather<CondD>(s, f, tl, ntiles(n), uh, n, T_inf, h_conv, dt, leap ? 1 : 0,
+                       first ? 1 : 0);
+}
 void launch_dens_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int n, int solid,
                        double dt, bool fuse, float cs32, bool uh) {
   if (n <= 0) return;

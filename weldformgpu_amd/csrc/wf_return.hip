@@ -41,7 +41,7 @@ __device__ __forceinline__ double jc_newton_q(const MatParams& mp, double s_tria
   return g;
 }
 
-template <bool MIR>
+template <int REC>
 __global__ __launch_bounds__(kBlock) void k_return(Fields f, StepParams p, BlockStats* bs,
                                                    RetQueue q, const float* __restrict__ w,
                                                    int use_mlp, long long step) {
@@ -95,7 +95,13 @@ __global__ __launch_bounds__(kBlock) void k_return(Fields f, StepParams p, Block
 #pragma unroll
       for (int k = 0; k < 6; k++) f.sigma[k][i] = sg[k];
       const double rho = f.rho[i];
-      if (MIR) {
+      if (REC == kRec64) {
+        const double ir2 = 1.0 / (rho * rho);
+        f.arec64[3 * (size_t)i + 1] =
+            make_double4(sg[0] * ir2, sg[1] * ir2, sg[2] * ir2, sg[3] * ir2);
+        f.arec64[3 * (size_t)i + 2] = make_double4(sg[4] * ir2, sg[5] * ir2, rho, f.Cs[i]);
+      }
+      if (REC == kRec32) {
         const double ir2 = 1.0 / (rho * rho);
         f.arec[3 * (size_t)i + 1] = make_float4((float)(sg[0] * ir2), (float)(sg[1] * ir2),
                                                 (float)(sg[2] * ir2), (float)(sg[3] * ir2));
@@ -145,16 +151,20 @@ __global__ __launch_bounds__(kBlock) void k_return(Fields f, StepParams p, Block
 }
 
 void launch_return(hipStream_t s, const Fields& f, const StepParams& p, BlockStats* bs,
-                   const RetQueue& q, const float* mlp_w, bool use_mlp, bool mirror32,
+                   const RetQueue& q, const float* mlp_w, bool use_mlp, int rec,
                    int64_t step) {
   const int blocks = (p.solid + kBlock - 1) / kBlock;
   if (blocks <= 0) return;
-  if (mirror32)
-    hipLaunchKernelGGL(k_return<true>, dim3(blocks), dim3(kBlock), 0, s, f, p, bs, q, mlp_w,
-                       use_mlp ? 1 : 0, (long long)step);
+  const int um = use_mlp ? 1 : 0;
+  if (rec == kRec32)
+    hipLaunchKernelGGL(k_return<kRec32>, dim3(blocks), dim3(kBlock), 0, s, f, p, bs, q, mlp_w,
+                       um, (long long)step);
+  else if (rec == kRec64)
+    hipLaunchKernelGGL(k_return<kRec64>, dim3(blocks), dim3(kBlock), 0, s, f, p, bs, q, mlp_w,
+                       um, (long long)step);
   else
-    hipLaunchKernelGGL(k_return<false>, dim3(blocks), dim3(kBlock), 0, s, f, p, bs, q, mlp_w,
-                       use_mlp ? 1 : 0, (long long)step);
+    hipLaunchKernelGGL(k_return<kRecNone>, dim3(blocks), dim3(kBlock), 0, s, f, p, bs, q, mlp_w,
+                       um, (long long)step);
 }
 
 }  // namespace wf
