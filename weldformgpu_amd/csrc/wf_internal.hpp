@@ -56,6 +56,13 @@ struct Fields {
   int* idlock;
 };
 
+// element c (0..2) of particle i's packed neighbour record (records are
+// contiguous per particle: one record spans one or two cache lines, where
+// three separate arrays would touch three)
+__device__ __forceinline__ size_t rec_at(const Fields&, int i, int c) {
+  return 3 * (size_t)i + c;
+}
+
 // which packed neighbour records a producer kernel also writes
 enum { kRecNone = 0, kRec32 = 1, kRec64 = 2 };
 

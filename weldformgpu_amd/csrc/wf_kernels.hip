@@ -115,9 +115,9 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Block
     T9 WT = {W.xx, W.yx, W.zx, W.xy, W.yy, W.zy, W.xz, W.yz, W.zz};
     T9 SRT = tmul(S, WT, qp);
     T9 RS = tmul(W, S, qp);
-    if (REC == kRec32) f.arec[3 * (size_t)i] = f.v4[i];  // this step's {v, m/rho}
+    if (REC == kRec32) f.arec[rec_at(f, i, 0)] = f.v4[i];  // this step's {v, m/rho}
     if (REC == kRec64)
-      f.arec64[3 * (size_t)i] = make_double4(f.v[0][i], f.v[1][i], f.v[2][i], f.m[i]);
+      f.arec64[rec_at(f, i, 0)] = make_double4(f.v[0][i], f.v[1][i], f.v[2][i], f.m[i]);
     const double G = f.G[i];
     const double tr3 = 1.0 / 3.0 * (D.xx + D.yy + D.zz);
     const double g2 = 2.0 * G;
@@ -223,17 +223,17 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Block
       f.sigma[5][i] = S.xz;
       if (REC == kRec32) {  // sigma/rho^2, rho and Cs for the FP32 acceleration loop
         const double rho = f.rho[i], ir2 = 1.0 / (rho * rho);
-        f.arec[3 * (size_t)i + 1] =
+        f.arec[rec_at(f, i, 1)] =
             make_float4((float)((-pi + S.xx) * ir2), (float)((-pi + S.yy) * ir2),
                         (float)((-pi + S.zz) * ir2), (float)(S.xy * ir2));
-        f.arec[3 * (size_t)i + 2] =
+        f.arec[rec_at(f, i, 2)] =
             make_float4((float)(S.yz * ir2), (float)(S.xz * ir2), (float)rho, (float)f.Cs[i]);
       }
       if (REC == kRec64) {  // the same in FP64 for the FP64 acceleration loop
         const double rho = f.rho[i], ir2 = 1.0 / (rho * rho);
-        f.arec64[3 * (size_t)i + 1] = make_double4(f.sigma[0][i] * ir2, f.sigma[1][i] * ir2,
+        f.arec64[rec_at(f, i, 1)] = make_double4(f.sigma[0][i] * ir2, f.sigma[1][i] * ir2,
                                                    f.sigma[2][i] * ir2, f.sigma[3][i] * ir2);
-        f.arec64[3 * (size_t)i + 2] =
+        f.arec64[rec_at(f, i, 2)] =
             make_double4(f.sigma[4][i] * ir2, f.sigma[5][i] * ir2, rho, f.Cs[i]);
       }
       if (ENERGY) {
@@ -397,11 +397,11 @@ __global__ __launch_bounds__(kBlock) void k_mirror32(Fields f, int n, Frame32 fr
   f.v4[i] = v4;
   f.v4_alt[i] = v4;
   const double ir2 = 1.0 / (rho * rho);
-  f.arec[3 * (size_t)i] = v4;
-  f.arec[3 * (size_t)i + 1] =
+  f.arec[rec_at(f, i, 0)] = v4;
+  f.arec[rec_at(f, i, 1)] =
       make_float4((float)(f.sigma[0][i] * ir2), (float)(f.sigma[1][i] * ir2),
                   (float)(f.sigma[2][i] * ir2), (float)(f.sigma[3][i] * ir2));
-  f.arec[3 * (size_t)i + 2] = make_float4((float)(f.sigma[4][i] * ir2),
+  f.arec[rec_at(f, i, 2)] = make_float4((float)(f.sigma[4][i] * ir2),
                                           (float)(f.sigma[5][i] * ir2), (float)rho,
                                           (float)f.Cs[i]);
   const float4 tk = make_float4((float)f.T[i], (float)f.k[i], v4.w, 0.f);
@@ -417,11 +417,10 @@ __global__ __launch_bounds__(kBlock) void k_records64(Fields f, int n) {
   const double4 kv = make_double4(f.v[0][i], f.v[1][i], f.v[2][i], mdj);
   f.kv[i] = kv;
   f.kv_alt[i] = kv;
-  double4* r = f.arec64 + 3 * (size_t)i;
-  r[0] = make_double4(f.v[0][i], f.v[1][i], f.v[2][i], f.m[i]);
-  r[1] = make_double4(f.sigma[0][i] * ir2, f.sigma[1][i] * ir2, f.sigma[2][i] * ir2,
-                      f.sigma[3][i] * ir2);
-  r[2] = make_double4(f.sigma[4][i] * ir2, f.sigma[5][i] * ir2, rho, f.Cs[i]);
+  f.arec64[rec_at(f, i, 0)] = make_double4(f.v[0][i], f.v[1][i], f.v[2][i], f.m[i]);
+  f.arec64[rec_at(f, i, 1)] = make_double4(f.sigma[0][i] * ir2, f.sigma[1][i] * ir2,
+                                           f.sigma[2][i] * ir2, f.sigma[3][i] * ir2);
+  f.arec64[rec_at(f, i, 2)] = make_double4(f.sigma[4][i] * ir2, f.sigma[5][i] * ir2, rho, f.Cs[i]);
   const double4 tk = make_double4(f.T[i], f.k[i], mdj, 0.0);
   f.tk64[i] = tk;
   f.tk64_alt[i] = tk;

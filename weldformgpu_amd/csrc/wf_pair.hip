@@ -155,7 +155,11 @@ __device__ __forceinline__ double gk_pair(double r2, double hj, const HConst& hc
 
 // gather-kernel traits of a policy: the staged position type, positions per
 // LDS pass, the per-tile context and a neighbour's staged position
-constexpr int kStage64 = 1024;  // 24 KB of FP64 positions
+// staged slots per LDS pass: 768 x 32 B (FP64 position + index) = 24 KB
+#ifndef WF_STAGE64
+#define WF_STAGE64 768
+#endif
+constexpr int kStage64 = WF_STAGE64;
 #define WF_GATHER64                                                                  \
   typedef double PT;                                                                 \
   static constexpr int STAGE = kStage64;                                             \
@@ -332,10 +336,9 @@ struct AccelD {
     o.s3 = f.sigma[3][i] * idi; o.s4 = f.sigma[4][i] * idi; o.s5 = f.sigma[5][i] * idi;
   }
   __device__ static void load(const Fields& f, int j, Nb& nb) {
-    const double4* r = f.arec64 + 3 * (size_t)j;
-    nb.v = r[0];
-    nb.sa = r[1];
-    nb.sb = r[2];
+    nb.v = f.arec64[rec_at(f, j, 0)];
+    nb.sa = f.arec64[rec_at(f, j, 1)];
+    nb.sb = f.arec64[rec_at(f, j, 2)];
     nb.h = UH ? 0.0 : f.h[j];
   }
   __device__ static void pair(double x, double y, double z, const Nb& nb, const Own& o, Acc& a,
@@ -520,9 +523,12 @@ __device__ __forceinline__ void pos32(const Fields& f, int j, const Ctx32& c, fl
   z = f.xr[2][j] + (float)(cz - c.bz) * c.cs;
 }
 
-// staged tile-frame positions per LDS pass (18 KB); a tile's staged set
-// averages ~1.1k particles on a cubic lattice
-constexpr int kStage32 = 1536;
+// staged slots per LDS pass: 1536 x 16 B (tile-frame position + index) =
+// 24 KB; a tile's staged set averages ~1.1k particles on a cubic lattice
+#ifndef WF_STAGE32
+#define WF_STAGE32 1536
+#endif
+constexpr int kStage32 = WF_STAGE32;
 #define WF_GATHER32                                                                \
   typedef float PT;                                                                \
   static constexpr int STAGE = kStage32;                                           \
@@ -536,11 +542,28 @@ constexpr int kStage32 = 1536;
     pos32(f, j, c, x, y, z);                                                       \
   }
 
+// One staged particle in LDS: its position in the tile frame and its
+// particle index, read together by one LDS access per pair (the index
+// addresses the neighbour's packed record in global memory).
+template <class PT>
+struct alignas(16) StagedSlot;
+template <>
+struct alignas(16) StagedSlot<float> {
+  float x, y, z;
+  int j;
+};
+template <>
+struct alignas(16) StagedSlot<double> {
+  double x, y, z;
+  int j, pad;
+};
+
 template <class K>
 __global__ __launch_bounds__(kTileP) void k_pair_gather(Fields f, TileLists tl,
                                                         typename K::Prm prm) {
   constexpr int ST = K::STAGE;
-  __shared__ typename K::PT lx[ST], ly[ST], lz[ST];
+  typedef StagedSlot<typename K::PT> Slot;
+  __shared__ Slot st[ST];
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
   const int i = t * kTileP + tid;
@@ -552,38 +575,41 @@ __global__ __launch_bounds__(kTileP) void k_pair_gather(Fields f, TileLists tl,
   const int ub = tl.uoff[t], ucnt = tl.uoff[t + 1] - ub;
   const int* __restrict__ tj = tl.tj + ub;
   const int cnt = act ? tl.count[i] : 0;
+  auto stage = [&](int c0, int m) {
+    for (int s = tid; s < m; s += kTileP) {
+      Slot v;
+      v.j = tj[c0 + s];
+      K::pos(f, v.j, cx, v.x, v.y, v.z);
+      st[s] = v;
+    }
+  };
+  constexpr int NB = K::NBATCH;  // neighbour records held at once (4, or 2)
   if (ucnt <= ST) {
-    // the whole staged set fits: one pass, a quad of four slots at a time
-    for (int s = tid; s < ucnt; s += kTileP) K::pos(f, tj[s], cx, lx[s], ly[s], lz[s]);
+    // the whole staged set fits: one pass, a quad of four slots at a time;
+    // the slot word two quads ahead is fetched while the current quad's
+    // records are in flight, and the next quad's particle indices come
+    // from LDS
+    stage(0, ucnt);
     __syncthreads();
     const uint2* __restrict__ q = tl.slotq + i;
-    // the next quad's slots and neighbour indices are fetched before the
-    // current quad's pairs are evaluated
-    uint2 w = cnt > 0 ? q[0] : make_uint2(0u, 0u);
+    auto sl_of = [](const uint2 w, int u) {
+      const unsigned h = (u & 2) ? w.y : w.x;
+      return (int)((u & 1) ? h >> 16 : h & 0xFFFFu);
+    };
+    uint2 w0 = cnt > 0 ? q[0] : make_uint2(0u, 0u);
+    uint2 w1 = cnt > 4 ? q[tl.npad] : w0;
     int jn[4];
 #pragma unroll
-    for (int u = 0; u < 4; u++) jn[u] = u < cnt ? tj[(u & 2) ? (u & 1 ? w.y >> 16 : w.y & 0xFFFFu)
-                                                          : (u & 1 ? w.x >> 16 : w.x & 0xFFFFu)]
-                                                 : 0;
+    for (int u = 0; u < 4; u++) jn[u] = u < cnt ? st[sl_of(w0, u)].j : 0;
     for (int k = 0; k < cnt; k += 4) {
-      const int sl[4] = {(int)(w.x & 0xFFFFu), (int)(w.x >> 16), (int)(w.y & 0xFFFFu),
-                         (int)(w.y >> 16)};
       const int nk = min(4, cnt - k);
-      // K::NBATCH neighbour records held at once (4, or 2 where registers
-      // would otherwise limit occupancy)
-      constexpr int NB = K::NBATCH;
       int jc[4];
 #pragma unroll
       for (int u = 0; u < 4; u++) jc[u] = jn[u];
-      const int kn = k + 4;
-      if (kn < cnt) {
-        w = q[(int64_t)(kn >> 2) * tl.npad];
-        const int ns[4] = {(int)(w.x & 0xFFFFu), (int)(w.x >> 16), (int)(w.y & 0xFFFFu),
-                           (int)(w.y >> 16)};
 #pragma unroll
-        for (int u = 0; u < 4; u++)
-          if (kn + u < cnt) jn[u] = tj[ns[u]];
-      }
+      for (int u = 0; u < 4; u++)
+        if (k + 4 + u < cnt) jn[u] = st[sl_of(w1, u)].j;
+      const uint2 w2 = k + 8 < cnt ? q[(int64_t)((k + 8) >> 2) * tl.npad] : w1;
 #pragma unroll
       for (int u0 = 0; u0 < 4; u0 += NB) {
         typename K::Nb nb[NB];
@@ -592,9 +618,13 @@ __global__ __launch_bounds__(kTileP) void k_pair_gather(Fields f, TileLists tl,
           if (u0 + u < nk) K::load(f, jc[u0 + u], nb[u]);
 #pragma unroll
         for (int u = 0; u < NB; u++)
-          if (u0 + u < nk)
-            K::pair(lx[sl[u0 + u]], ly[sl[u0 + u]], lz[sl[u0 + u]], nb[u], own, acc, prm);
+          if (u0 + u < nk) {
+            const Slot& sv = st[sl_of(w0, u0 + u)];
+            K::pair(sv.x, sv.y, sv.z, nb[u], own, acc, prm);
+          }
       }
+      w0 = w1;
+      w1 = w2;
     }
     K::finish(f, i, act, own, acc, prm);
     return;
@@ -604,7 +634,7 @@ __global__ __launch_bounds__(kTileP) void k_pair_gather(Fields f, TileLists tl,
   for (int c0 = 0; c0 < ucnt; c0 += ST) {
     const int m = min(ST, ucnt - c0);
     __syncthreads();
-    for (int s = tid; s < m; s += kTileP) K::pos(f, tj[c0 + s], cx, lx[s], ly[s], lz[s]);
+    stage(c0, m);
     __syncthreads();
     const int lim = c0 + m;
     // two neighbours per iteration while both fall in this pass: their
@@ -613,20 +643,22 @@ __global__ __launch_bounds__(kTileP) void k_pair_gather(Fields f, TileLists tl,
       const int s0 = sr.slot();
       if (s0 >= lim) break;
       sr.advance();
+      const Slot& a0 = st[s0 - c0];
       typename K::Nb a;
-      K::load(f, tj[s0], a);
+      K::load(f, a0.j, a);
       if (sr.k < sr.cnt) {
         const int s1 = sr.slot();
         if (s1 < lim) {
           sr.advance();
+          const Slot& b0 = st[s1 - c0];
           typename K::Nb b;
-          K::load(f, tj[s1], b);
-          K::pair(lx[s0 - c0], ly[s0 - c0], lz[s0 - c0], a, own, acc, prm);
-          K::pair(lx[s1 - c0], ly[s1 - c0], lz[s1 - c0], b, own, acc, prm);
+          K::load(f, b0.j, b);
+          K::pair(a0.x, a0.y, a0.z, a, own, acc, prm);
+          K::pair(b0.x, b0.y, b0.z, b, own, acc, prm);
           continue;
         }
       }
-      K::pair(lx[s0 - c0], ly[s0 - c0], lz[s0 - c0], a, own, acc, prm);
+      K::pair(a0.x, a0.y, a0.z, a, own, acc, prm);
     }
   }
   K::finish(f, i, act, own, acc, prm);
@@ -777,18 +809,17 @@ struct AccelG {
     a.ax.zero(); a.ay.zero(); a.az.zero();
     if (!act) return;
     pos32(f, i, c, o.x, o.y, o.z);
-    const float4* r = f.arec + 3 * (size_t)i;
-    const float4 v = r[0], sa = r[1], sb = r[2];
+    const float4 v = f.arec[rec_at(f, i, 0)], sa = f.arec[rec_at(f, i, 1)],
+                 sb = f.arec[rec_at(f, i, 2)];
     o.vx = v.x; o.vy = v.y; o.vz = v.z;
     o.s0 = sa.x; o.s1 = sa.y; o.s2 = sa.z; o.s3 = sa.w;
     o.s4 = sb.x; o.s5 = sb.y; o.di = sb.z; o.Ci = sb.w;
     o.hc = hconst_f(f.h32[i]);
   }
   __device__ static void load(const Fields& f, int j, Nb& nb) {
-    const float4* r = f.arec + 3 * (size_t)j;
-    nb.v = r[0];
-    nb.sa = r[1];
-    nb.sb = r[2];
+    nb.v = f.arec[rec_at(f, j, 0)];
+    nb.sa = f.arec[rec_at(f, j, 1)];
+    nb.sb = f.arec[rec_at(f, j, 2)];
     nb.h = UH ? 0.f : f.h32[j];
   }
   __device__ static void pair(float x, float y, float z, const Nb& nb, const Own& o, Acc& a,

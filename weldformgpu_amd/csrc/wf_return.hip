@@ -97,15 +97,15 @@ __global__ __launch_bounds__(kBlock) void k_return(Fields f, StepParams p, Block
       const double rho = f.rho[i];
       if (REC == kRec64) {
         const double ir2 = 1.0 / (rho * rho);
-        f.arec64[3 * (size_t)i + 1] =
+        f.arec64[rec_at(f, i, 1)] =
             make_double4(sg[0] * ir2, sg[1] * ir2, sg[2] * ir2, sg[3] * ir2);
-        f.arec64[3 * (size_t)i + 2] = make_double4(sg[4] * ir2, sg[5] * ir2, rho, f.Cs[i]);
+        f.arec64[rec_at(f, i, 2)] = make_double4(sg[4] * ir2, sg[5] * ir2, rho, f.Cs[i]);
       }
       if (REC == kRec32) {
         const double ir2 = 1.0 / (rho * rho);
-        f.arec[3 * (size_t)i + 1] = make_float4((float)(sg[0] * ir2), (float)(sg[1] * ir2),
+        f.arec[rec_at(f, i, 1)] = make_float4((float)(sg[0] * ir2), (float)(sg[1] * ir2),
                                                 (float)(sg[2] * ir2), (float)(sg[3] * ir2));
-        f.arec[3 * (size_t)i + 2] = make_float4((float)(sg[4] * ir2), (float)(sg[5] * ir2),
+        f.arec[rec_at(f, i, 2)] = make_float4((float)(sg[4] * ir2), (float)(sg[5] * ir2),
                                                 (float)rho, (float)f.Cs[i]);
       }
       const double de = f.m[i] / rho *
