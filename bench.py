@@ -14,20 +14,24 @@ BC + integration, conduction + leapfrog, neighbour rebuild every 5 steps
 once yielding), all inputs resident in HBM.
 
 --workload c4 runs the rigid-plate contact workload instead: the same JC
-cylinder at dx 0.25 mm (1,331,520 particles) compressed by a rigid 40 x 40 mm
+cylinder at dx 0.25 mm (1,320,424 particles) compressed by a rigid 40 x 40 mm
 plate of 2 x 64 x 64 triangles at -2 m/s, penalty stiffness 0.1 E/dx,
 Coulomb friction 0.2, the online-trained surrogate in the mixed run.
 
 Prints ONE JSON line: value = particle-updates/s (particles x steps / s),
 plus the HBM roofline of the dominant kernel (HIP-event durations on the
 library's stream x algorithmic bytes) and the FP64 CPU oracle timed on the
-host cores on a bounded sample of the same workload.
+host cores on a bounded sample of the same workload.  error_vs_fp64 compares
+the mixed run's fields with the FP64 run's; error_floor_fp64_perturbed is the
+same comparison between the FP64 run and an FP64 run whose initial positions
+carry 1e-12 m of noise (how far two FP64 runs of the workload drift apart).
 """
 import argparse
 import json
 import os
 import sys
 import time
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -101,6 +105,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--workload", choices=["c3", "c4"], default="c3")
+    ap.add_argument("--no-error-floor", action="store_true",
+                    help="skip the FP64 run with 1e-12 m position noise that measures how "
+                         "far two FP64 runs drift apart on this workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path is "
@@ -115,8 +122,11 @@ def main():
             return scenarios.c4_contact(scale=args.scale)
         return scenarios.c3_johnson_cook(scale=args.scale)
 
-    def run(precision, timing_steps, surrogate):
+    def run(precision, timing_steps, surrogate, noise=0.0):
         sc = make()
+        if noise:
+            rng = np.random.default_rng(1)
+            sc.fields["x"] = sc.fields["x"] + rng.normal(0.0, noise, sc.fields["x"].shape)
         sc.cfg.precision = precision
         sc.cfg.surrogate = surrogate
         dom = Domain(sc.cfg)
@@ -129,6 +139,9 @@ def main():
         t1 = time.perf_counter()
         res = {"sc": sc, "secs": t1 - t0,
                "fields": {k: dom.get(k) for k in ["u", "sigma", "sigma_eq", "pl_strain", "T"]}}
+        if noise:
+            dom.close()
+            return res
         # per-kernel HIP-event durations on the library stream
         dom.set_timing(True)
         dom.step(timing_steps)
@@ -167,12 +180,21 @@ def main():
     print(f"[bench] mixed {ms_step:.3f} ms/step", file=sys.stderr, flush=True)
 
     def rl2(a, b):
-        import numpy as np
         d = np.linalg.norm(b)
         return float(np.linalg.norm(a - b) / d) if d > 0 else float(np.linalg.norm(a - b))
-    err = {k: rl2(rmx["fields"][k], r64["fields"][k]) for k in ["u", "sigma", "sigma_eq",
-                                                                  "pl_strain"]}
-    err["T_rise"] = rl2(rmx["fields"]["T"] - 300.0, r64["fields"]["T"] - 300.0)
+
+    def field_err(r):
+        e = {k: rl2(r["fields"][k], r64["fields"][k]) for k in ["u", "sigma", "sigma_eq",
+                                                                "pl_strain"]}
+        e["T_rise"] = rl2(r["fields"]["T"] - 300.0, r64["fields"]["T"] - 300.0)
+        return e
+    err = field_err(rmx)
+    # the same comparison between two FP64 runs whose initial positions differ
+    # by 1e-12 m noise: the drift the workload itself allows
+    floor = None
+    if not args.no_error_floor:
+        print("[bench] fp64 error floor ...", file=sys.stderr, flush=True)
+        floor = field_err(run(_abi.WF_FP64, 0, 0, noise=1e-12))
     kern = rmx["kern"]
     stats = rmx["stats"]
     dom_k = max((k for k in kern if k in ALG_BYTES_MIXED), key=lambda k: kern[k]["avg_ms"])
@@ -199,6 +221,7 @@ def main():
                  "kernels": r64["kern"]},
         "speedup_vs_fp64": rate / rate64,
         "error_vs_fp64": err,
+        "error_floor_fp64_perturbed": floor,
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": kern,
