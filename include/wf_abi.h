@@ -268,15 +268,16 @@ int wf_set_timing(struct wf_domain* d, int on);
 enum {
   WF_K_DENS = 0,    /* density-rate pair loop + density update      */
   WF_K_RATES,       /* strain/rotation-rate pair loop               */
-  WF_K_STRESS,      /* pressure + stress/plasticity + energy; with the
-                       Newton Johnson-Cook return map, the elastic
-                       predictor and the queue of yielding particles */
+  WF_K_STRESS,      /* pressure + stress/plasticity + energy, with the
+                       Newton Johnson-Cook return map and its
+                       surrogate guess solved in place              */
   WF_K_ACCEL,       /* acceleration pair loop                       */
   WF_K_INTEGRATE,   /* BC + position/velocity update + BC           */
   WF_K_THERMAL,     /* conduction pair loop + leapfrog              */
   WF_K_NSEARCH,     /* neighbour rebuild (all its kernels)          */
   WF_K_MLP,         /* surrogate training                           */
-  WF_K_RETURN,      /* batched Johnson-Cook return map (+ surrogate) */
+  WF_K_RETURN,      /* reserved: the return map runs inside
+                       WF_K_STRESS, so this id reports no launches  */
   WF_K_CONTACT,     /* surface, contact neighbours, forces, mesh move */
   WF_K_COUNT_
 };

@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libwfhip.so")
 DRIVER = os.path.join(HERE, "wfdriver")
-SOURCES = ["wf_kernels.hip", "wf_pair.hip", "wf_tiles.hip", "wf_nsearch.hip", "wf_return.hip",
+SOURCES = ["wf_kernels.hip", "wf_pair.hip", "wf_tiles.hip", "wf_nsearch.hip",
            "wf_mlp.hip", "wf_contact.hip", "wf_domain.hip", "wf_geometry.cpp"]
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 ARCH = os.environ.get("WF_OFFLOAD_ARCH", "gfx950")

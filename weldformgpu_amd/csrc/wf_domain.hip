@@ -65,12 +65,14 @@ struct wf_domain {
   TileLists tl{};
   GridDesc grid{};
   Frame32 frame{};
-  // return-map queue and surrogate
-  RetQueue rq{};
+  // return-map statistics and surrogate
   BlockStats* bstats = nullptr;
   int nblk = 0;  // blocks of the per-particle stress kernel
   int64_t mlp_cap = 0;
-  bool use_queue = false;
+  bool jc_newton = false;  // Johnson-Cook under the Newton return map
+  RetArgs ret_args() const {
+    return RetArgs{mlp_w, (cfg.surrogate && mlp_trained) ? 1 : 0, (long long)step};
+  }
   float *mlp_w = nullptr, *mlp_m = nullptr, *mlp_v = nullptr, *mlp_grad = nullptr;
   float *mlp_part = nullptr, *mlp_loss_part = nullptr, *mlp_loss = nullptr;
   float *mlp_X = nullptr, *mlp_Y = nullptr;
@@ -558,14 +560,9 @@ int wf_create(const wf_config* cfg, wf_domain** out) {
       if (!rc) rc = d->dev_alloc((void**)q, sizeof(double4) * d->npad);
     if (!rc) rc = d->dev_alloc((void**)&d->f.arec64, 3 * sizeof(double4) * d->npad);
   }
-  // return-map queue, surrogate samples and weights
-  d->use_queue = cfg->return_map == WF_RETURN_NEWTON && cfg->mat.model == WF_MAT_JOHNSON_COOK;
-  if (!rc) rc = d->dev_alloc((void**)&d->rq.idx, sizeof(int) * d->npad);
-  if (!rc) rc = d->dev_alloc((void**)&d->rq.strial, sizeof(double) * d->npad);
-  if (!rc) rc = d->dev_alloc((void**)&d->rq.rt, sizeof(double) * d->npad);
-  if (!rc) rc = d->dev_alloc((void**)&d->rq.gmax, sizeof(double) * d->npad);
+  // return-map statistics, surrogate samples and weights
+  d->jc_newton = cfg->return_map == WF_RETURN_NEWTON && cfg->mat.model == WF_MAT_JOHNSON_COOK;
   d->nblk = (int)((d->npad + kBlock - 1) / kBlock);
-  if (!rc) rc = d->dev_alloc((void**)&d->rq.count, sizeof(int) * (d->nblk + 1));
   if (!rc) rc = d->dev_alloc((void**)&d->bstats, sizeof(BlockStats) * (d->nblk + 1));
   for (int k = 0; k < 4 && !rc; k++) rc = d->alloc_plane_f(&d->f.mlx[k]);
   if (!rc) rc = d->alloc_plane_f(&d->f.mly);
@@ -960,11 +957,7 @@ int wf_pressure(wf_domain* d) {
 int wf_stress_strain(wf_domain* d) {
   if (!d) return fail(WF_E_ARG, "null domain");
   const StepParams p = d->params();
-  launch_stress(d->stream, d->f, p, d->bstats, false, true, kRecNone,
-                d->use_queue ? &d->rq : nullptr);
-  if (d->use_queue)
-    launch_return(d->stream, d->f, p, d->bstats, d->rq, d->mlp_w,
-                  d->cfg.surrogate && d->mlp_trained, kRecNone, d->step);
+  launch_stress(d->stream, d->f, p, d->bstats, false, true, kRecNone, d->ret_args());
   launch_fold_stats(d->stream, d->bstats, (p.solid + kBlock - 1) / kBlock, d->ctr);
   d->mirror_dirty = true;
   HIPC(hipGetLastError());
@@ -1295,15 +1288,8 @@ int wf_step(wf_domain* d, int n_steps) {
       if ((rc = timer_end(d, WF_K_RATES))) return rc;
       timer_begin(d);
       const int rec = mx ? kRec32 : kRec64;
-      launch_stress(d->stream, d->f, p, d->bstats, true, true, rec,
-                    d->use_queue ? &d->rq : nullptr);
+      launch_stress(d->stream, d->f, p, d->bstats, true, true, rec, d->ret_args());
       if ((rc = timer_end(d, WF_K_STRESS))) return rc;
-      if (d->use_queue) {
-        timer_begin(d);
-        launch_return(d->stream, d->f, p, d->bstats, d->rq, d->mlp_w,
-                      d->cfg.surrogate && d->mlp_trained, rec, d->step);
-        if ((rc = timer_end(d, WF_K_RETURN))) return rc;
-      }
       launch_fold_stats(d->stream, d->bstats, (p.solid + kBlock - 1) / kBlock, d->ctr);
       timer_begin(d);
       if (mx) launch_accel_mixed(d->stream, d->f, d->tl, d->solid, p.alpha, p.beta, cs32, uh);
@@ -1353,7 +1339,7 @@ int wf_step(wf_domain* d, int n_steps) {
       launch_mesh_move(d->stream, d->mesh, d->motions, p.dt);
       if ((rc = timer_end(d, WF_K_CONTACT))) return rc;
     }
-    if (d->cfg.surrogate && d->use_queue && d->is_yielding && d->cfg.mlp_train_interval > 0 &&
+    if (d->cfg.surrogate && d->jc_newton && d->is_yielding && d->cfg.mlp_train_interval > 0 &&
         (d->step + 1) % d->cfg.mlp_train_interval == 0) {
       timer_begin(d);
       rc = d->mlp_train_round(250, 2e-3f);

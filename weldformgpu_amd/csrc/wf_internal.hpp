@@ -123,18 +123,16 @@ void launch_contact_force(hipStream_t s, const Fields& f, int solid, const MeshD
 // |cf_i| where ID_i == id, else 0
 void launch_contact_norm(hipStream_t s, const Fields& f, int n, int id, double* out);
 
-// Queue of yielding Johnson-Cook particles whose radial return is solved by
-// k_return (wf_return.hip): particle index, trial von Mises stress, rate x
-// thermal factor of the flow stress, and the bracket end dg_max.  The queue
-// is segmented by stress-kernel block: block b owns slots [b*kBlock,
-// b*kBlock + count[b]), compacted in particle order (no atomics).
-struct RetQueue {
-  int* idx;
-  double *strial, *rt, *gmax;
-  int* count;  // per block
+// What the stress kernel needs for the Johnson-Cook Newton return map: the
+// surrogate's weights, whether to use them, and the step number stamped on
+// the logged training samples.
+struct RetArgs {
+  const float* mlp_w;
+  int use_mlp;
+  long long step;
 };
 
-// per-block statistics of the stress/return kernels, folded into Counters
+// per-block statistics of the stress kernel, folded into Counters
 // once per step: max eps_p bits, Newton iterations, Newton calls, sum over
 // wavefronts of the wave's largest iteration count
 struct BlockStats {
@@ -227,13 +225,10 @@ void launch_records64(hipStream_t s, const Fields& f, int n);
 void launch_update_density(hipStream_t s, const Fields& f, const StepParams& p, double dt);
 void launch_pressure(hipStream_t s, const Fields& f, const StepParams& p);
 void launch_stress(hipStream_t s, const Fields& f, const StepParams& p, BlockStats* bs,
-                   bool fuse_pressure, bool energy, int rec, const RetQueue* q);
+                   bool fuse_pressure, bool energy, int rec, const RetArgs& ra);
 // fold the per-block statistics of the last stress (+ return) pass into ctr
 void launch_fold_stats(hipStream_t s, const BlockStats* bs, int nblocks, Counters* ctr);
-// ---- return map and surrogate (wf_return.hip, wf_mlp.hip) -----------------
-void launch_return(hipStream_t s, const Fields& f, const StepParams& p, BlockStats* bs,
-                   const RetQueue& q, const float* mlp_w, bool use_mlp, int rec,
-                   int64_t step);
+// ---- return-map surrogate (wf_mlp.hip) -------------------------------------
 // MFMA inference of the surrogate on given feature rows (test hook)
 void launch_mlp_infer(hipStream_t s, const float* w, const float* X, int n, float* y);
 // one gradient of the mean squared error over n samples (deterministic)

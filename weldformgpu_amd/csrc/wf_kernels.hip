@@ -3,7 +3,7 @@
 // leapfrog, reductions and reorder helpers.  One thread per particle over
 // Morton-sorted SoA planes, so every load and store is coalesced.  The pair
 // loops are in wf_pair.hip.
-#include "wf_device.hpp"
+#include "wf_mlp.hpp"
 
 namespace wf {
 
@@ -52,12 +52,12 @@ __device__ __forceinline__ T9 tmul(const T9& a, const T9& b, bool quirk) {
 }
 
 // Safeguarded Newton for the Johnson-Cook radial return,
-// r(g) = s_trial - 3G g - s_y(ep+g) = 0 on [0, (s_trial - s_y(ep))/(3G)].
-__device__ double jc_newton(const MatParams& mp, double s_trial, double G, double ep,
-                            double rate_f, double therm_f, double g0, double tol, int max_iter,
-                            int* iters) {
-  double sy0 = jc_yield(mp, ep, rate_f, therm_f);
-  double lo = 0.0, hi = (s_trial - sy0) / (3.0 * G);
+// r(g) = s_trial - 3G g - (A + B (ep+g)^n) rt = 0 on [0, hi], from g0, with
+// rt = rate factor x thermal factor and hi = (s_trial - s_y(ep)) / (3G).
+__device__ __forceinline__ double jc_newton(const MatParams& mp, double s_trial, double G,
+                                            double ep, double rt, double hi, double g0,
+                                            double tol, int max_iter, int* iters) {
+  double lo = 0.0;
   if (!(hi > 0.0)) {
     *iters = 0;
     return 0.0;
@@ -65,16 +65,15 @@ __device__ double jc_newton(const MatParams& mp, double s_trial, double G, doubl
   double g = g0;
   if (!(g > lo) || !(g < hi)) g = (g0 >= hi) ? hi : lo;
   int it = 0;
-  const double rt = rate_f * therm_f;
   for (; it < max_iter; it++) {
     // one pow per iterate: s_y = (A + B x^n) rt, H = n B x^n / x rt
     const double x = ep + g;
     const double xn = pow(x, mp.n);
-    double r = s_trial - 3.0 * G * g - (mp.A + mp.B * xn) * rt;
+    const double r = s_trial - 3.0 * G * g - (mp.A + mp.B * xn) * rt;
     if (fabs(r) <= tol * s_trial) break;
     if (r > 0.0) lo = g;
     else hi = g;
-    double H = x > 0.0 ? mp.n * mp.B * (xn / x) * rt : HUGE_VAL;
+    const double H = x > 0.0 ? mp.n * mp.B * (xn / x) * rt : HUGE_VAL;
     double gn = g + r / (3.0 * G + H);
     if (!(gn > lo && gn < hi)) gn = 0.5 * (lo + hi);
     g = gn;
@@ -87,28 +86,37 @@ __device__ double jc_newton(const MatParams& mp, double s_trial, double G, doubl
 // C: Jaumann stress update + return map, StressStrainOne (Mechanical.cu:458-554),
 // optionally with the pressure (FUSE_P), the per-particle internal-energy
 // increment (ENERGY, Energy.cu:5-19 with per-particle indexing) and the
-// packed FP32 or FP64 neighbour records of the pair loops (REC).  With a queue, yielding Johnson-Cook particles
-// under the Newton return map keep their trial state and are appended to the
-// queue; k_return finishes them.
+// packed FP32 or FP64 neighbour records of the pair loops (REC).
+//
+// The Johnson-Cook Newton return map (an extension: the reference re-evaluates
+// the yield stress explicitly, Mechanical.cu:504-508) is solved in place, in
+// three phases per wavefront: every lane forms its elastic predictor; if any
+// lane yields, the whole wavefront evaluates the surrogate MLP on its 64
+// feature rows (BF16 MFMA, wf_mlp.hpp) for an initial plastic increment and
+// the yielding lanes run the FP64 Newton from it (from 0 without the
+// surrogate) to the same residual; then every lane finishes and writes its
+// state once.  The solved increment is logged as a training sample.
 template <bool FUSE_P, bool ENERGY, int REC>
 __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, BlockStats* bs,
-                                                   RetQueue q, int use_queue) {
+                                                   RetArgs ra) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = i < p.solid;
-  bool queued = false;
-  double q_strial = 0.0, q_rt = 0.0, q_gmax = 0.0;
+  const MatParams& mp = p.mat;
+  const double dt = p.dt;
   unsigned long long ep_bits = 0ull, nit = 0ull, ncall = 0ull;
+  T9 S{}, D{};
+  double G = 0.0, sig_trial = 0.0, sigy = 0.0, ep = 0.0, sigeq = 0.0, dep = 0.0, scale = 1.0;
+  double esr = 0.0, T = 0.0, rt = 1.0, gmax = 0.0;
+  bool newton = false;  // a yielding Johnson-Cook particle under the Newton return map
   if (active) {
-    const MatParams& mp = p.mat;
     const bool qp = (p.quirks & WF_Q_TENSOR_PRODUCT) != 0;
-    const double dt = p.dt;
     if (FUSE_P) f.p[i] = 0.0 + (f.Cs[i] * f.Cs[i]) * (f.rho[i] - f.rho0[i]);
-    T9 S, D, W;
     S.xx = f.ss[0][i]; S.yy = f.ss[1][i]; S.zz = f.ss[2][i];
     S.xy = S.yx = f.ss[3][i]; S.yz = S.zy = f.ss[4][i]; S.xz = S.zx = f.ss[5][i];
     D.xx = f.strrate[0][i]; D.yy = f.strrate[1][i]; D.zz = f.strrate[2][i];
     D.xy = D.yx = f.strrate[3][i]; D.yz = D.zy = f.strrate[4][i];
     D.xz = D.zx = f.strrate[5][i];
+    T9 W;
     W.xx = f.rotrate[0][i]; W.yy = f.rotrate[1][i]; W.zz = f.rotrate[2][i];
     W.xy = f.rotrate[3][i]; W.yz = f.rotrate[4][i]; W.xz = f.rotrate[5][i];
     W.yx = -W.xy; W.zy = -W.yz; W.zx = -W.xz;
@@ -118,7 +126,7 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Block
     if (REC == kRec32) f.arec[rec_at(f, i, 0)] = f.v4[i];  // this step's {v, m/rho}
     if (REC == kRec64)
       f.arec64[rec_at(f, i, 0)] = make_double4(f.v[0][i], f.v[1][i], f.v[2][i], f.m[i]);
-    const double G = f.G[i];
+    G = f.G[i];
     const double tr3 = 1.0 / 3.0 * (D.xx + D.yy + D.zz);
     const double g2 = 2.0 * G;
     // S = dt*(2G(D - tr/3 I) + SRT + RS) + S   (Mechanical.cu:490)
@@ -132,19 +140,17 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Block
     S.yz = (D.yz * g2 + SRT.yz + RS.yz) * dt + S.yz;
     S.zy = (D.zy * g2 + SRT.zy + RS.zy) * dt + S.zy;
 
-    const double esr = sqrt(0.5 * ((D.xx - D.yy) * (D.xx - D.yy) + (D.yy - D.zz) * (D.yy - D.zz) +
-                                   (D.zz - D.xx) * (D.zz - D.xx)) +
-                            3.0 * (D.xy * D.xy + D.yz * D.yz + D.zx * D.zx));
+    esr = sqrt(0.5 * ((D.xx - D.yy) * (D.xx - D.yy) + (D.yy - D.zz) * (D.yy - D.zz) +
+                      (D.zz - D.xx) * (D.zz - D.xx)) +
+               3.0 * (D.xy * D.xy + D.yz * D.yz + D.zx * D.zx));
     f.esr[i] = esr;
     const double J2 = 0.5 * (S.xx * S.xx + 2.0 * S.xy * S.yx + 2.0 * S.xz * S.zx +
                              S.yy * S.yy + 2.0 * S.yz * S.zy + S.zz * S.zz);
-    const double sig_trial = sqrt(3.0 * J2);
-    const double T = f.T[i];
-    double sigy = f.sigma_y[i];
-    double ep = f.pl_strain[i];
-    double sigeq = sig_trial;
-    double dep = 0.0;
-    double scale = 1.0;
+    sig_trial = sqrt(3.0 * J2);
+    T = f.T[i];
+    sigy = f.sigma_y[i];
+    ep = f.pl_strain[i];
+    sigeq = sig_trial;
     if (p.return_map == WF_RETURN_REFERENCE) {
       if (sigy < sig_trial) scale = sigy / sig_trial;
       if (mp.model == WF_MAT_HOLLOMON) sigy = hollomon_yield(mp, ep);
@@ -169,18 +175,10 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Block
         if (mp.model == WF_MAT_BILINEAR) {
           g = (sig_trial - sigy) / (3.0 * G + mp.Ep);
           sigy = sigy + mp.Ep * g;
-        } else if (mp.model == WF_MAT_JOHNSON_COOK && use_queue) {
-          queued = true;
-          q_strial = sig_trial;
-          q_rt = rate_f * therm_f;
-          q_gmax = (sig_trial - sigy) / (3.0 * G);
         } else if (mp.model == WF_MAT_JOHNSON_COOK) {
-          int it = 0;
-          g = jc_newton(mp, sig_trial, G, ep, rate_f, therm_f, 0.0, p.newton_tol,
-                        p.newton_max_iter, &it);
-          nit = (unsigned long long)it;
-          ncall = 1ull;
-          sigy = jc_yield(mp, ep + g, rate_f, therm_f);
+          newton = true;  // solved below, with the rest of the wavefront
+          rt = rate_f * therm_f;
+          gmax = (sig_trial - sigy) / (3.0 * G);
         } else {
           double lo = 0.0, hi = (sig_trial - sigy) / (3.0 * G);
           for (int it = 0; it < 200; it++) {
@@ -192,7 +190,7 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Block
           g = 0.5 * (lo + hi);
           sigy = hollomon_yield(mp, ep + g);
         }
-        if (!queued) {
+        if (!newton) {
           scale = (sig_trial - 3.0 * G * g) / sig_trial;
           dep = g;
           ep += g;
@@ -200,6 +198,40 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Block
         }
       }
     }
+  }
+  // wave-uniform: the MFMA needs all 64 lanes
+  if (__ballot(newton) != 0ull) {
+    float f0, f1, f2, f3;
+    mlp_features(newton ? sig_trial : 1.0, mp.A, ep, esr, newton ? jc_theta(mp, T) : 0.0, f0,
+                 f1, f2, f3);
+    double g0 = 0.0;
+    if (ra.use_mlp) {
+      MlpWave mw;
+      mlp_wave_load(ra.mlp_w, mw);
+      const float y = mlp_wave_eval(mw, f0, f1, f2, f3);
+      if (isfinite(y)) g0 = fmin(fmax((double)y, 0.0), 1.0) * gmax;
+    }
+    if (newton) {
+      int it = 0;
+      const double g =
+          jc_newton(mp, sig_trial, G, ep, rt, gmax, g0, p.newton_tol, p.newton_max_iter, &it);
+      nit = (unsigned long long)it;
+      ncall = 1ull;
+      scale = (sig_trial - 3.0 * G * g) / sig_trial;
+      dep = g;
+      sigeq = sig_trial - 3.0 * G * g;
+      // training sample (one slot per particle: deterministic)
+      f.mlx[0][i] = f0;
+      f.mlx[1][i] = f1;
+      f.mlx[2][i] = f2;
+      f.mlx[3][i] = f3;
+      f.mly[i] = gmax > 0.0 ? (float)(g / gmax) : 0.f;
+      f.mlstep[i] = (int)ra.step;
+      ep += g;
+      sigy = (mp.A + mp.B * pow(ep, mp.n)) * rt;
+    }
+  }
+  if (active) {
     if (scale != 1.0) {
       S.xx *= scale; S.yy *= scale; S.zz *= scale;
       S.xy *= scale; S.yz *= scale; S.xz *= scale;
@@ -208,80 +240,58 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Block
     f.ss[3][i] = S.xy; f.ss[4][i] = S.yz; f.ss[5][i] = S.xz;
     f.strain[0][i] += D.xx * dt; f.strain[1][i] += D.yy * dt; f.strain[2][i] += D.zz * dt;
     f.strain[3][i] += D.xy * dt; f.strain[4][i] += D.yz * dt; f.strain[5][i] += D.xz * dt;
-    if (!queued) {
-      f.sigma_y[i] = sigy;
-      f.pl_strain[i] = ep;
-      f.sigma_eq[i] = sigeq;
-      f.qpl[i] = p.taylor_quinney * sigeq * dep / dt;
-      const double pi = f.p[i];
-      // sigma = -p I + S (Mechanical.cu:540); store upper triangle
-      f.sigma[0][i] = -pi + S.xx;
-      f.sigma[1][i] = -pi + S.yy;
-      f.sigma[2][i] = -pi + S.zz;
-      f.sigma[3][i] = S.xy;
-      f.sigma[4][i] = S.yz;
-      f.sigma[5][i] = S.xz;
-      if (REC == kRec32) {  // sigma/rho^2, rho and Cs for the FP32 acceleration loop
-        const double rho = f.rho[i], ir2 = 1.0 / (rho * rho);
-        f.arec[rec_at(f, i, 1)] =
-            make_float4((float)((-pi + S.xx) * ir2), (float)((-pi + S.yy) * ir2),
-                        (float)((-pi + S.zz) * ir2), (float)(S.xy * ir2));
-        f.arec[rec_at(f, i, 2)] =
-            make_float4((float)(S.yz * ir2), (float)(S.xz * ir2), (float)rho, (float)f.Cs[i]);
-      }
-      if (REC == kRec64) {  // the same in FP64 for the FP64 acceleration loop
-        const double rho = f.rho[i], ir2 = 1.0 / (rho * rho);
-        f.arec64[rec_at(f, i, 1)] = make_double4(f.sigma[0][i] * ir2, f.sigma[1][i] * ir2,
-                                                   f.sigma[2][i] * ir2, f.sigma[3][i] * ir2);
-        f.arec64[rec_at(f, i, 2)] =
-            make_double4(f.sigma[4][i] * ir2, f.sigma[5][i] * ir2, rho, f.Cs[i]);
-      }
-      if (ENERGY) {
-        double de = f.m[i] / f.rho[i] *
-                    (S.xx * D.xx + S.yy * D.yy + S.zz * D.zz +
-                     2.0 * (S.xy * D.xy + S.yz * D.yz + S.xz * D.xz));
-        f.eint[i] += de * dt;
-      }
-      ep_bits = (unsigned long long)__double_as_longlong(ep > 0.0 ? ep : 0.0);
+    f.sigma_y[i] = sigy;
+    f.pl_strain[i] = ep;
+    f.sigma_eq[i] = sigeq;
+    f.qpl[i] = p.taylor_quinney * sigeq * dep / dt;
+    const double pi = f.p[i];
+    // sigma = -p I + S (Mechanical.cu:540); store upper triangle
+    const double sg[6] = {-pi + S.xx, -pi + S.yy, -pi + S.zz, S.xy, S.yz, S.xz};
+#pragma unroll
+    for (int k = 0; k < 6; k++) f.sigma[k][i] = sg[k];
+    if (REC == kRec32) {  // sigma/rho^2, rho and Cs for the FP32 acceleration loop
+      const double rho = f.rho[i], ir2 = 1.0 / (rho * rho);
+      f.arec[rec_at(f, i, 1)] = make_float4((float)(sg[0] * ir2), (float)(sg[1] * ir2),
+                                            (float)(sg[2] * ir2), (float)(sg[3] * ir2));
+      f.arec[rec_at(f, i, 2)] =
+          make_float4((float)(sg[4] * ir2), (float)(sg[5] * ir2), (float)rho, (float)f.Cs[i]);
     }
+    if (REC == kRec64) {  // the same in FP64 for the FP64 acceleration loop
+      const double rho = f.rho[i], ir2 = 1.0 / (rho * rho);
+      f.arec64[rec_at(f, i, 1)] =
+          make_double4(sg[0] * ir2, sg[1] * ir2, sg[2] * ir2, sg[3] * ir2);
+      f.arec64[rec_at(f, i, 2)] = make_double4(sg[4] * ir2, sg[5] * ir2, rho, f.Cs[i]);
+    }
+    if (ENERGY) {
+      double de = f.m[i] / f.rho[i] *
+                  (S.xx * D.xx + S.yy * D.yy + S.zz * D.zz +
+                   2.0 * (S.xy * D.xy + S.yz * D.yz + S.xz * D.xz));
+      f.eint[i] += de * dt;
+    }
+    ep_bits = (unsigned long long)__double_as_longlong(ep > 0.0 ? ep : 0.0);
   }
-  // block-segmented queue append (particle order, no atomics) and block stats
-  __shared__ int wcnt[kBlock / 64];
+  // block statistics
   __shared__ unsigned long long wst[kBlock / 64][4];
   const int lane = lane_id(), w = threadIdx.x >> 6;
-  const unsigned long long bal = __ballot(queued);
   const unsigned long long mx = wave_max_u64(ep_bits);
   const unsigned long long sit = wave_sum_u64(nit);
   const unsigned long long sc = wave_sum_u64(ncall);
   const unsigned long long wmax = wave_max_u64(nit);
   if (lane == 0) {
-    wcnt[w] = __popcll(bal);
     wst[w][0] = mx;
     wst[w][1] = sit;
     wst[w][2] = sc;
     wst[w][3] = wmax;
   }
   __syncthreads();
-  int base = 0;
-  for (int u = 0; u < w; u++) base += wcnt[u];
-  if (queued) {
-    const int slot = blockIdx.x * kBlock + base + __popcll(bal & ((1ull << lane) - 1ull));
-    q.idx[slot] = i;
-    q.strial[slot] = q_strial;
-    q.rt[slot] = q_rt;
-    q.gmax[slot] = q_gmax;
-  }
   if (threadIdx.x == 0) {
-    int tot = 0;
     BlockStats b{0ull, 0ull, 0ull, 0ull};
     for (int u = 0; u < kBlock / 64; u++) {
-      tot += wcnt[u];
       b.ep_bits = wst[u][0] > b.ep_bits ? wst[u][0] : b.ep_bits;
       b.iters += wst[u][1];
       b.calls += wst[u][2];
       b.wave_iters += wst[u][3];
     }
-    if (use_queue) q.count[blockIdx.x] = tot;
     bs[blockIdx.x] = b;
   }
 }
@@ -530,23 +540,21 @@ void launch_pressure(hipStream_t s, const Fields& f, const StepParams& p) {
   hipLaunchKernelGGL(k_pressure, dim3(grid_for(p.n)), dim3(kBlock), 0, s, f, p.n);
 }
 void launch_stress(hipStream_t s, const Fields& f, const StepParams& p, BlockStats* bs,
-                   bool fuse_pressure, bool energy, int rec, const RetQueue* q) {
+                   bool fuse_pressure, bool energy, int rec, const RetArgs& ra) {
   if (p.solid <= 0) return;
   dim3 g(grid_for(p.solid)), b(kBlock);
-  RetQueue qq = q ? *q : RetQueue{};
-  int uq = q ? 1 : 0;
   if (rec == kRec32)
-    hipLaunchKernelGGL((k_stress<true, true, kRec32>), g, b, 0, s, f, p, bs, qq, uq);
+    hipLaunchKernelGGL((k_stress<true, true, kRec32>), g, b, 0, s, f, p, bs, ra);
   else if (rec == kRec64)
-    hipLaunchKernelGGL((k_stress<true, true, kRec64>), g, b, 0, s, f, p, bs, qq, uq);
+    hipLaunchKernelGGL((k_stress<true, true, kRec64>), g, b, 0, s, f, p, bs, ra);
   else if (fuse_pressure && energy)
-    hipLaunchKernelGGL((k_stress<true, true, kRecNone>), g, b, 0, s, f, p, bs, qq, uq);
+    hipLaunchKernelGGL((k_stress<true, true, kRecNone>), g, b, 0, s, f, p, bs, ra);
   else if (fuse_pressure)
-    hipLaunchKernelGGL((k_stress<true, false, kRecNone>), g, b, 0, s, f, p, bs, qq, uq);
+    hipLaunchKernelGGL((k_stress<true, false, kRecNone>), g, b, 0, s, f, p, bs, ra);
   else if (energy)
-    hipLaunchKernelGGL((k_stress<false, true, kRecNone>), g, b, 0, s, f, p, bs, qq, uq);
+    hipLaunchKernelGGL((k_stress<false, true, kRecNone>), g, b, 0, s, f, p, bs, ra);
   else
-    hipLaunchKernelGGL((k_stress<false, false, kRecNone>), g, b, 0, s, f, p, bs, qq, uq);
+    hipLaunchKernelGGL((k_stress<false, false, kRecNone>), g, b, 0, s, f, p, bs, ra);
 }
 void launch_fold_stats(hipStream_t s, const BlockStats* bs, int nblocks, Counters* ctr) {
   if (nblocks <= 0) return;
