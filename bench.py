@@ -41,10 +41,10 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per particle-update of each kernel (FP64 plain path):
 # compulsory SoA reads + writes, neighbour data counted once, list excluded
 ALG_BYTES_FP64 = {"dens": 80, "rates": 144, "stress": 384, "accel": 144, "integrate": 172,
-                  "thermal": 120}
+                  "thermal": 120, "cond_dens": 200}
 # mixed (FP32 pair-loop inputs and outputs), same accounting
 ALG_BYTES_MIXED = {"dens": 40, "rates": 72, "stress": 204, "accel": 72, "integrate": 184,
-                   "thermal": 60}
+                   "thermal": 60, "cond_dens": 100}
 
 
 def cpu_baseline(sc, warm=1, steps=2):

@@ -279,6 +279,9 @@ enum {
   WF_K_RETURN,      /* reserved: the return map runs inside
                        WF_K_STRESS, so this id reports no launches  */
   WF_K_CONTACT,     /* surface, contact neighbours, forces, mesh move */
+  WF_K_COND_DENS,   /* conduction pair loop + leapfrog fused with the
+                       next step's density pass + update (inside one
+                       wf_step call, when no rebuild runs between)  */
   WF_K_COUNT_
 };
 /* Accumulated milliseconds and launch count of kernel id k. */

@@ -47,7 +47,7 @@ for _k, _n in enumerate(["ID", "BC_T", "nb_count", "cont_nb_count"]):
     FIELDS[_n] = (64 + _k, 1, "i4")
 
 KERNELS = ["dens", "rates", "stress", "accel", "integrate", "thermal", "nsearch", "mlp", "return",
-           "contact"]
+           "contact", "cond_dens"]
 WF_MLP_PARAMS = 1249
 
 

@@ -1252,7 +1252,20 @@ int wf_contact_force_sum(wf_domain* d, double* sum) {
   return WF_OK;
 }
 
-// Domain_d::MechFraserSolve loop body, SolverFraser.cu:164-399
+// Whether the next step starts with a neighbour rebuild that actually runs
+// (SolverFraser.cu:166; under the stale-snapshot quirk a rebuild from an
+// unchanged snapshot is skipped, see wf_nsearch_build).
+static bool rebuild_next_step(const wf_domain* d) {
+  const int ts_next = d->ts_i + 1 > d->cfg.ns_interval - 1 ? 0 : d->ts_i + 1;
+  if (ts_next != 0 || !d->is_yielding) return false;
+  const bool stale = (d->cfg.quirks & WF_Q_STALE_NS_POS) != 0;
+  return !(stale && d->nb_built && d->built_gen == d->snap_gen);
+}
+
+// Domain_d::MechFraserSolve loop body, SolverFraser.cu:164-399.  Inside one
+// call, a step's conduction pass and the next step's density pass share one
+// traversal of the neighbour lists when no rebuild runs between them (both
+// read the positions after integration; see CondDens in wf_pair.hip).
 int wf_step(wf_domain* d, int n_steps) {
   if (!d) return fail(WF_E_ARG, "null domain");
   if (n_steps < 0) return fail(WF_E_ARG, "negative step count");
@@ -1261,6 +1274,7 @@ int wf_step(wf_domain* d, int n_steps) {
     rc = wf_nsearch_build(d);
     if (rc) return rc;
   }
+  bool dens_done = false;  // this step's density pass ran with the last conduction pass
   for (int s = 0; s < n_steps; s++) {
     StepParams p = d->params();
     if (d->cfg.mechanics) {
@@ -1272,16 +1286,19 @@ int wf_step(wf_domain* d, int n_steps) {
       const bool mx = d->mixed;
       const float cs32 = d->frame.cs32;
       const bool uh = d->uniform_h;
-      timer_begin(d);
-      if (mx) {
-        launch_dens_mixed(d->stream, d->f, d->tl, d->n, d->solid, p.dt, true, cs32, uh);
-        std::swap(d->f.v4, d->f.v4_alt);
-      } else {
-        launch_dens_tiled(d->stream, d->f, d->tl, d->n, d->solid, p.dt, true, uh);
-        std::swap(d->f.kv, d->f.kv_alt);
+      if (!dens_done) {
+        timer_begin(d);
+        if (mx) {
+          launch_dens_mixed(d->stream, d->f, d->tl, d->n, d->solid, p.dt, true, cs32, uh);
+          std::swap(d->f.v4, d->f.v4_alt);
+        } else {
+          launch_dens_tiled(d->stream, d->f, d->tl, d->n, d->solid, p.dt, true, uh);
+          std::swap(d->f.kv, d->f.kv_alt);
+        }
+        std::swap(d->f.rho, d->f.rho_alt);
+        if ((rc = timer_end(d, WF_K_DENS))) return rc;
       }
-      std::swap(d->f.rho, d->f.rho_alt);
-      if ((rc = timer_end(d, WF_K_DENS))) return rc;
+      dens_done = false;
       timer_begin(d);
       if (mx) launch_rates_mixed(d->stream, d->f, d->tl, d->solid, p.quirks, cs32, uh);
       else launch_rates_tiled(d->stream, d->f, d->tl, d->solid, p.quirks, uh);
@@ -1321,18 +1338,31 @@ int wf_step(wf_domain* d, int n_steps) {
     }
     if (d->cfg.thermal) {
       d->refresh_mirrors();
+      const bool fuse = d->cfg.mechanics && s + 1 < n_steps && !rebuild_next_step(d);
       timer_begin(d);
-      if (d->mixed) {
+      if (fuse && d->mixed) {
+        launch_cond_dens_mixed(d->stream, d->f, d->tl, d->n, d->solid, p.T_inf, p.h_conv, p.dt,
+                               true, d->isfirst_step, d->frame.cs32, d->uniform_h);
+        std::swap(d->f.v4, d->f.v4_alt);
+      } else if (fuse) {
+        launch_cond_dens_tiled(d->stream, d->f, d->tl, d->n, d->solid, p.T_inf, p.h_conv,
+                               p.dt, true, d->isfirst_step, d->uniform_h);
+        std::swap(d->f.kv, d->f.kv_alt);
+      } else if (d->mixed) {
         launch_cond_mixed(d->stream, d->f, d->tl, d->n, p.T_inf, p.h_conv, p.dt, true,
                           d->isfirst_step, d->frame.cs32, d->uniform_h);
-        std::swap(d->f.tk4, d->f.tk4_alt);
       } else {
         launch_cond_tiled(d->stream, d->f, d->tl, d->n, p.T_inf, p.h_conv, p.dt, true,
                           d->isfirst_step, d->uniform_h);
-        std::swap(d->f.tk64, d->f.tk64_alt);
       }
+      if (d->mixed) std::swap(d->f.tk4, d->f.tk4_alt);
+      else std::swap(d->f.tk64, d->f.tk64_alt);
       std::swap(d->f.T, d->f.T_alt);
-      if ((rc = timer_end(d, WF_K_THERMAL))) return rc;
+      if (fuse) {
+        std::swap(d->f.rho, d->f.rho_alt);
+        dens_done = true;
+      }
+      if ((rc = timer_end(d, fuse ? WF_K_COND_DENS : WF_K_THERMAL))) return rc;
     }
     if (d->cfg.mechanics && d->cfg.contact && d->n_meshes > 0) {  // SolverFraser.cu:290-318
       timer_begin(d);

@@ -216,6 +216,13 @@ void launch_accel_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int
                         double alpha, double beta, float cs32, bool uh);
 void launch_cond_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int n, double T_inf,
                        double h_conv, double dt, bool leap, bool first, float cs32, bool uh);
+// conduction of this step fused with the density pass (+ update) of the next
+void launch_cond_dens_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int n,
+                            int solid, double T_inf, double h_conv, double dt, bool leap,
+                            bool first, bool uh);
+void launch_cond_dens_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int n,
+                            int solid, double T_inf, double h_conv, double dt, bool leap,
+                            bool first, float cs32, bool uh);
 // refresh every FP32 mirror / FP64 record from the FP64 state (after uploads
 // and rebuilds)
 void launch_mirror32(hipStream_t s, const Fields& f, int n, const Frame32& fr);

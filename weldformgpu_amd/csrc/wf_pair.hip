@@ -192,6 +192,7 @@ struct DensD {
     int limit, n;
     double dt;
     int fuse;
+    int mdj_alt;  // write m/rho into the next conduction record (fused with conduction)
   };
   struct Own {
     double x, y, z, vx, vy, vz, rho;
@@ -235,7 +236,7 @@ struct DensD {
         f.rho_alt[i] = rn;
         const double mdj = f.m[i] / rn;
         f.kv_alt[i] = make_double4(o.vx, o.vy, o.vz, mdj);
-        f.tk64[i].z = mdj;  // m/rho of this step for the conduction loop
+        (p.mdj_alt ? f.tk64_alt : f.tk64)[i].z = mdj;  // m/rho for the conduction loop
       }
     } else if (p.fuse && i < p.n) {
       f.rho_alt[i] = f.rho[i];
@@ -675,6 +676,7 @@ struct DensG {
     double dt;
     int fuse;
     float cs;
+    int mdj_alt;  // write m/rho into the next conduction record (fused with conduction)
   };
   struct Own {
     float x, y, z, vx, vy, vz;
@@ -718,7 +720,7 @@ struct DensG {
         float4 v = f.v4[i];
         v.w = (float)(f.m[i] / rn);
         f.v4_alt[i] = v;
-        f.tk4[i].z = v.w;  // m/rho of this step for the conduction loop
+        (p.mdj_alt ? f.tk4_alt : f.tk4)[i].z = v.w;  // m/rho for the conduction loop
       }
     } else if (p.fuse && i < p.n) {
       f.rho_alt[i] = f.rho[i];
@@ -997,6 +999,65 @@ void launch_surface_tiled(hipStream_t s, const Fields& f, const TileLists& tl, i
 
 namespace wf {
 // launch policy POL with every particle's h equal (UH) or not
+// Conduction of one step fused with the density pass of the next: both read
+// the positions after integration and the same neighbour lists, and neither
+// reads what the other writes (the density pass puts the new m/rho into the
+// next conduction record), so one traversal of the lists serves both.  Each
+// sum runs over the list in the same order as in its own kernel.
+template <class A, class B>
+struct CondDens {
+  typedef typename A::PT PT;
+  static constexpr int STAGE = A::STAGE;
+  static constexpr int NBATCH = A::NBATCH < B::NBATCH ? A::NBATCH : B::NBATCH;
+  typedef typename A::Ctx Ctx;
+  struct Prm {
+    int limit;  // particles of the conduction pass (>= the density pass's)
+    typename A::Prm a;
+    typename B::Prm b;
+  };
+  struct Own {
+    typename A::Own a;
+    typename B::Own b;
+  };
+  struct Nb {
+    typename A::Nb a;
+    typename B::Nb b;
+  };
+  struct Acc {
+    typename A::Acc a;
+    typename B::Acc b;
+  };
+  template <class P>
+  __device__ static Ctx ctx(const Fields& f, int t, const P& p) {
+    return A::ctx(f, t, p.a);
+  }
+  __device__ static void pos(const Fields& f, int j, const Ctx& c, PT& x, PT& y, PT& z) {
+    A::pos(f, j, c, x, y, z);
+  }
+  __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx& c) {
+    A::init(f, i, act, o.a, a.a, c);
+    B::init(f, i, act, o.b, a.b, c);
+  }
+  __device__ static void load(const Fields& f, int j, Nb& nb) {
+    A::load(f, j, nb.a);
+    B::load(f, j, nb.b);
+  }
+  __device__ static void pair(PT x, PT y, PT z, const Nb& nb, const Own& o, Acc& a,
+                              const Prm& p) {
+    A::pair(x, y, z, nb.a, o.a, a.a, p.a);
+    B::pair(x, y, z, nb.b, o.b, a.b, p.b);
+  }
+  __device__ static void finish(const Fields& f, int i, bool act, const Own& o, const Acc& a,
+                                const Prm& p) {
+    A::finish(f, i, act, o.a, a.a, p.a);
+    B::finish(f, i, i < p.b.limit, o.b, a.b, p.b);
+  }
+};
+template <bool UH>
+using CondDensD = CondDens<CondD<UH>, DensD<UH>>;
+template <bool UH>
+using CondDensG = CondDens<CondG<UH>, DensG<UH>>;
+
 template <template <bool> class POL, class... A>
 static void launch_gather(hipStream_t s, const Fields& f, const TileLists& tl, int grid, bool uh,
                           A... args) {
@@ -1052,5 +1113,31 @@ void launch_cond_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int 
   if (n <= 0) return;
   launch_gather<CondG>(s, f, tl, ntiles(n), uh, n, T_inf, h_conv, dt, leap ? 1 : 0,
                        first ? 1 : 0, cs32);
+}
+void launch_cond_dens_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int n,
+                            int solid, double T_inf, double h_conv, double dt, bool leap,
+                            bool first, bool uh) {
+  if (n <= 0) return;
+  auto go = [&](auto tag) {
+    typedef decltype(tag) K;
+    const typename K::Prm p{n, {n, T_inf, h_conv, dt, leap ? 1 : 0, first ? 1 : 0},
+                            {solid, n, dt, 1, 1}};
+    hipLaunchKernelGGL(k_pair_gather<K>, dim3(ntiles(n)), dim3(kTileP), 0, s, f, tl, p);
+  };
+  if (uh) go(CondDensD<true>{});
+  else go(CondDensD<false>{});
+}
+void launch_cond_dens_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int n,
+                            int solid, double T_inf, double h_conv, double dt, bool leap,
+                            bool first, float cs32, bool uh) {
+  if (n <= 0) return;
+  auto go = [&](auto tag) {
+    typedef decltype(tag) K;
+    const typename K::Prm p{n, {n, T_inf, h_conv, dt, leap ? 1 : 0, first ? 1 : 0, cs32},
+                            {solid, n, dt, 1, cs32, 1}};
+    hipLaunchKernelGGL(k_pair_gather<K>, dim3(ntiles(n)), dim3(kTileP), 0, s, f, tl, p);
+  };
+  if (uh) go(CondDensG<true>{});
+  else go(CondDensG<false>{});
 }
 }  // namespace wf
