@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define WF_ABI_VERSION 2
+#define WF_ABI_VERSION 3
 
 /* ---- status codes ------------------------------------------------------ */
 enum {
@@ -105,6 +105,15 @@ typedef struct wf_material_params {
   double A, B, n, C, eps0, m, Tm, Tt; /* Johnson-Cook const[0..7]            */
 } wf_material_params;
 
+/* Neighbour-rebuild policy.  WF_REBUILD_EVERY_N is the reference's: every
+ * ns_interval steps once max eps_p > ns_min_plastic (SolverFraser.cu:166,
+ * 260-271, 395-397).  WF_REBUILD_PLASTIC_ZONE (an extension) builds the
+ * lists with radius 2h + ns_skin from the current positions and rebuilds,
+ * once yielding, only when a particle with eps_p > ep_hot has moved more
+ * than ns_skin / 2 since the last build; pairs beyond 2h contribute nothing
+ * to the pair sums, and the surface pass counts only pairs within 2h. */
+enum { WF_REBUILD_EVERY_N = 0, WF_REBUILD_PLASTIC_ZONE = 1 };
+
 typedef struct wf_config {
   int abi_version;       /* WF_ABI_VERSION                                  */
   int n_particles;       /* particle_count                                  */
@@ -135,7 +144,12 @@ typedef struct wf_config {
   double fric_dyn;       /* Contact[0] fricCoeffDynamic                      */
   int id_free_surf;      /* ID given to free-surface particles (1,
                             SolverFraser.cu:120)                            */
-  int _pad1;
+  int rebuild_policy;    /* WF_REBUILD_*                                    */
+  double fric_heat;      /* fraction of the sliding-friction power that heats
+                            the particle (0 = off; an extension)            */
+  double ns_skin;        /* WF_REBUILD_PLASTIC_ZONE: list radius 2h + skin  */
+  double ep_hot;         /* WF_REBUILD_PLASTIC_ZONE: eps_p of the particles
+                            whose displacement triggers a rebuild (0.05)    */
 } wf_config;
 
 /* Fill *cfg with the reference's defaults: alpha=1, beta=0
@@ -155,6 +169,7 @@ enum {
   WF_F_SIGMA_Y, WF_F_PL_STRAIN, WF_F_SIGMA_EQ, WF_F_EFF_STRAIN_RATE,
   WF_F_INT_ENERGY, WF_F_T, WF_F_TA, WF_F_TB, WF_F_DTDT, WF_F_K_T, WF_F_CP_T,
   WF_F_Q_PLASTIC,
+  WF_F_Q_FRICTION, /* sliding-friction heat power into the particle [W]     */
   /* double[6*n] */
   WF_F_SIGMA = 48, WF_F_SHEARSTRESS, WF_F_STRAIN, WF_F_STRRATE, WF_F_ROTRATE,
   /* int32[n] */

@@ -672,7 +672,9 @@ void or_update_vel(or_state* s, double dt) {
 
 // ThermalSolveKernel (Thermal.cu:101-142, Fraser eq. 3.99) on the mechanical
 // neighbour lists, then CalcConvHeatKernel (Thermal.cu:36-64); plastic work
-// heat q_plastic [W/m^3] enters as a source; BC_T == WF_BCT_FIXED holds T.
+// heat q_plastic [W/m^3] and sliding-friction heat q_friction [W] (an
+// extension: the reference has no friction heat) enter as sources;
+// BC_T == WF_BCT_FIXED holds T.
 void or_calc_temp_inc(or_state* s, const wf_config* c) {
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < s->n; i++) {
@@ -696,6 +698,7 @@ void or_calc_temp_inc(or_state* s, const wf_config* c) {
       acc += 1. / (s->cp_T[i]) * q_conv;
     }
     if (s->q_plastic && s->q_plastic[i] != 0.0) acc += s->q_plastic[i] * (1. / (s->rho[i] * s->cp_T[i]));
+    if (s->q_friction && s->q_friction[i] != 0.0) acc += s->q_friction[i] / (s->m[i] * s->cp_T[i]);
     if (bc == WF_BCT_FIXED) acc = 0.0;
     s->dTdt[i] = acc;
   }
@@ -929,7 +932,9 @@ void or_contact_nb(or_state* s, const wf_config* c) {
   ct->cont_over_cap += over;
 }
 
-// CalcContactForcesWang, Contact.cu:134-321
+// CalcContactForcesWang, Contact.cu:134-321; in the sliding branch the
+// dissipated power fric_heat x mu_d |F_n| |slip| / dt goes to q_friction
+// (an extension).
 void or_contact_forces(or_state* s, const wf_config* c) {
   or_contact* ct = s->contact;
   const or_mesh* m = ct->mesh;
@@ -937,6 +942,7 @@ void or_contact_forces(or_state* s, const wf_config* c) {
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < s->solid; i++) {
     V3 cf{0.0, 0.0, 0.0};
+    double qf = 0.0;  // sliding-friction heat power [W] (an extension)
     const V3 xi = ld3(s->x, i), vi = ld3(s->v, i);
     V3 ai = ld3(s->a, i);
     const double mi = s->m[i], hi = s->h[i];
@@ -983,11 +989,14 @@ void or_contact_forces(or_state* s, const wf_config* c) {
                 c->fric_dyn * lcf * tg.z / ltg};
           cf = sub(cf, fd);
           ai = sub(ai, V3{fd.x / mi, fd.y / mi, fd.z / mi});
+          // |F_friction| times the slip speed |dtg| / dt
+          qf += c->fric_heat * (c->fric_dyn * lcf) * (ltg / kij) / dt;
         }
       }
     }
     st3(s->a, i, ai);
     st3(ct->cf, i, cf);
+    if (s->q_friction) s->q_friction[i] = qf;
   }
 }
 

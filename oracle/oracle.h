@@ -41,7 +41,7 @@ typedef struct or_state {
   double *x, *v, *a, *u, *va, *xs;                 /* 3n; xs = snapshot x_h */
   double *rho, *rho0, *drho, *m, *h, *p, *Cs, *G;  /* n */
   double *sigma_y, *pl_strain, *sigma_eq, *eff_strain_rate, *int_energy;
-  double *T, *Ta, *Tb, *dTdt, *k_T, *cp_T, *q_plastic;
+  double *T, *Ta, *Tb, *dTdt, *k_T, *cp_T, *q_plastic, *q_friction;
   double *sigma, *shearstress, *strain, *strrate, *rotrate; /* 6n */
   int *ID, *BC_T;
   int *nb_count, *nb_offset; /* n, n+1 */

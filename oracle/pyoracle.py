@@ -16,7 +16,8 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 _DBL3 = ["x", "v", "a", "u", "va", "xs"]
 _DBL1 = ["rho", "rho0", "drho", "m", "h", "p", "Cs", "G", "sigma_y", "pl_strain", "sigma_eq",
-         "eff_strain_rate", "int_energy", "T", "Ta", "Tb", "dTdt", "k_T", "cp_T", "q_plastic"]
+         "eff_strain_rate", "int_energy", "T", "Ta", "Tb", "dTdt", "k_T", "cp_T", "q_plastic",
+         "q_friction"]
 _DBL6 = ["sigma", "shearstress", "strain", "strrate", "rotrate"]
 _INT1 = ["ID", "BC_T"]
 

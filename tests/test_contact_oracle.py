@@ -118,6 +118,39 @@ def test_coulomb_friction_branches(oracle, vt, static):
     assert a[2] == pytest.approx(-fn / M, rel=1e-12)
 
 
+@pytest.mark.parametrize("vt,static", [(1.0, True), (10.0, False), (-14.0, False)])
+def test_sliding_friction_heat(oracle, vt, static):
+    """Friction heat (an extension): while sliding, the dissipated power
+    fric_heat x mu_d |F_n| |v_t| enters the particle (the slip over dt is
+    v_t dt, the tangential acceleration being zero before the contact);
+    sticking dissipates nothing.  It then heats the particle at
+    q / (m cp) in the conduction pass (a lone particle: no conduction)."""
+    mu_s, mu_d, beta = 0.5, 0.3, 0.7
+    st = _one(oracle, (0.2, 0.3, 0.9), v=(vt, 0.0, 0.0), fric_sta=mu_s, fric_dyn=mu_d,
+              fric_heat=beta)
+    fn = 0.8 * M * (H - 0.1) / DT ** 2
+    q = st["q_friction"][0]
+    if static:
+        assert q == 0.0
+    else:
+        assert q == pytest.approx(beta * mu_d * fn * abs(vt), rel=1e-12)
+    cp, rho = 900.0, 2700.0
+    st["cp_T"] = [cp]
+    st["k_T"] = [200.0]
+    st["rho"] = [rho]
+    st["T"] = [300.0]
+    cfg = _cfg(fric_sta=mu_s, fric_dyn=mu_d, fric_heat=beta)
+    cfg.thermal = 1
+    st.nsearch(cfg)
+    st.calc_temp_inc(cfg)
+    assert st["dTdt"][0] == pytest.approx(q / (M * cp), rel=1e-12)
+
+
+def test_friction_heat_off_by_default(oracle):
+    st = _one(oracle, (0.2, 0.3, 0.9), v=(10.0, 0.0, 0.0), fric_sta=0.5, fric_dyn=0.3)
+    assert st["q_friction"][0] == 0.0 and st.contact_arrays["cf"][0, 0] != 0.0
+
+
 def test_friction_against_a_moving_plate_uses_node_velocity(oracle):
     """Plate sliding with the particle: no relative slip, no tangential force;
     the element velocity comes from the last mesh move (zero before it)."""

@@ -1,7 +1,8 @@
 """Rigid-mesh contact on the GPU against the FP64 oracle: mesh motion and
 geometry, free-surface detection, contact candidate lists, penalty +
-friction forces (both laws), and whole steps of a plate compressing a
-Johnson-Cook cylinder (FP64 and mixed precision).
+friction forces (both laws), the sliding-friction heat (an extension), and
+whole steps of a plate compressing a Johnson-Cook cylinder (FP64 and mixed
+precision) with that heat feeding the conduction pass.
 
 Tolerances: mesh geometry 1e-14 absolute on O(1e-2) coordinates; surface
 IDs and candidate lists bit-exact; single-call accelerations and contact
@@ -26,7 +27,7 @@ def _case(name, sc):
 
 
 def _plate_case(law=_abi.WF_CONTACT_REFERENCE, mu=0.3, dens=6, sink=0.4, seed=None,
-                scale=0.2, r=0.3e-3, thermal=True):
+                scale=0.2, r=0.3e-3, thermal=True, v_scale=0.3):
     """Small JC cylinder with the plate already `sink` h into the top layer
     (offset in x, y so no particle sits on a triangle edge)."""
     sc = scenarios.c4_contact(scale=scale, r=r, dens=dens, law=law, mu=mu, offset=0.137,
@@ -34,7 +35,7 @@ def _plate_case(law=_abi.WF_CONTACT_REFERENCE, mu=0.3, dens=6, sink=0.4, seed=No
     sc.meshes[0]["nodes"][:, 2] -= sink * sc.meta["h"]
     if seed is not None:
         rng = np.random.default_rng(seed)
-        sc.fields["v"] = rng.uniform(-0.3, 0.3, (sc.n, 3))
+        sc.fields["v"] = rng.uniform(-v_scale, v_scale, (sc.n, 3))
     return sc
 
 
@@ -105,7 +106,10 @@ def test_contact_list_overflow_keeps_smallest_indices(oracle, wflib):
                                     (_abi.WF_CONTACT_REFERENCE, 0.3),
                                     (_abi.WF_CONTACT_STIFFNESS, 0.2)])
 def test_contact_forces_single_call(oracle, wflib, law, mu):
-    sc = _plate_case(law=law, mu=mu, seed=3)
+    # slip over one step must exceed mu x penetration for some particles to
+    # slide: tangential speeds of O(1e3) m/s mix both friction branches
+    sc = _plate_case(law=law, mu=mu, seed=3, v_scale=3000.0)
+    sc.cfg.fric_heat = 0.8
     rng = np.random.default_rng(4)
     sc.fields["a"] = rng.uniform(-1e5, 1e5, (sc.n, 3))
     _case(f"forces law={law} mu={mu}", sc)
@@ -122,8 +126,14 @@ def test_contact_forces_single_call(oracle, wflib, law, mu):
         d.CalcContactNb()
         d.CalcContactForces()
         a, cf = d.get("a"), d.get("contforce")
+        qf = d.get("q_friction")
         fsum = d.contact_force_sum()
     ref_cf = st.contact_arrays["cf"]
+    if mu > 0.0:  # sliding friction heat, an extension
+        assert np.count_nonzero(st["q_friction"]) > 0
+        assert rel_l2(qf, st["q_friction"]) < 1e-12
+    else:
+        assert not np.any(qf)
     assert np.count_nonzero(np.linalg.norm(ref_cf, axis=1)) > 10
     assert rel_l2(a, st["a"]) < 1e-12
     assert rel_l2(cf, ref_cf) < 1e-12
@@ -134,26 +144,32 @@ def test_contact_forces_single_call(oracle, wflib, law, mu):
 @pytest.mark.parametrize("law", [_abi.WF_CONTACT_REFERENCE, _abi.WF_CONTACT_STIFFNESS])
 def test_steps_plate_compression(oracle, wflib, law):
     """The plate starts one h above the top layer and closes in at 2 m/s; the
-    fields after each block of steps, and the contact force where the plate
-    touches, match the oracle."""
+    fields after each block of steps, the contact force where the plate
+    touches and the sliding-friction heat match the oracle (with the
+    reference law's softer static threshold, 0.4 mu delta, surface particles
+    slide in these steps; under the stiffness law they stick)."""
     sc = _plate_case(law=law, mu=0.2, sink=0.0, seed=5)
+    sc.cfg.fric_heat = 0.9
     _case(f"steps law={law}", sc)
     st = to_oracle(oracle, sc)
     st.nsearch(sc.cfg)
-    touched = 0
+    touched = heated = 0
     with to_gpu(sc, wflib) as d:
         for _ in range(6):
             st.step(sc.cfg, sc.bcs, 5)
             d.step(5)
             cf = st.contact_arrays["cf"]
             touched += np.count_nonzero(np.linalg.norm(cf, axis=1))
+            heated += np.count_nonzero(st["q_friction"])
             assert rel_l2(d.get("contforce"), cf) < STEP_TOL
-            for k in ["u", "v", "sigma", "pl_strain", "T"]:
+            for k in ["u", "v", "sigma", "pl_strain", "T", "q_friction"]:
                 assert rel_l2(d.get(k), st[k]) < STEP_TOL, k
         assert np.array_equal(d.get("ID"), st["ID"])
         nodes = d.get_mesh(0)["nodes"]
     assert np.max(np.abs(nodes - st.mesh_arrays["x"])) < 1e-14
+    print(f"  contacts {touched}, sliding-heated {heated}")
     assert touched > 0
+    assert heated > 0 or law == _abi.WF_CONTACT_STIFFNESS
 
 
 def test_steps_plate_compression_mixed(oracle, wflib):

@@ -7,7 +7,7 @@ product can silently run without the native code.
 import ctypes as C
 import os
 
-WF_ABI_VERSION = 2
+WF_ABI_VERSION = 3
 
 WF_OK = 0
 WF_E_ARG = -1
@@ -30,6 +30,7 @@ WF_Q_STALE_NS_POS = 0x4
 WF_Q_ALL = 0x7
 WF_BCT_NONE, WF_BCT_CONVECTION, WF_BCT_FIXED = 0, 1, 2
 WF_CONTACT_REFERENCE, WF_CONTACT_STIFFNESS = 0, 1
+WF_REBUILD_EVERY_N, WF_REBUILD_PLASTIC_ZONE = 0, 1
 WF_MAX_MESHES = 8
 WF_MAX_CONT_NB = 20
 
@@ -39,7 +40,7 @@ for _k, _n in enumerate(["x", "v", "a", "u", "va", "x_snapshot", "normal", "cont
     FIELDS[_n] = (_k, 3, "f8")
 for _k, _n in enumerate(["rho", "rho0", "drho", "m", "h", "p", "Cs", "G", "sigma_y",
                          "pl_strain", "sigma_eq", "eff_strain_rate", "int_energy", "T",
-                         "Ta", "Tb", "dTdt", "k_T", "cp_T", "q_plastic"]):
+                         "Ta", "Tb", "dTdt", "k_T", "cp_T", "q_plastic", "q_friction"]):
     FIELDS[_n] = (16 + _k, 1, "f8")
 for _k, _n in enumerate(["sigma", "shearstress", "strain", "strrate", "rotrate"]):
     FIELDS[_n] = (48 + _k, 6, "f8")
@@ -75,7 +76,8 @@ class wf_config(C.Structure):
                 ("contact", C.c_int), ("contact_law", C.c_int),
                 ("contact_kappa", C.c_double), ("contact_dx", C.c_double),
                 ("fric_sta", C.c_double), ("fric_dyn", C.c_double),
-                ("id_free_surf", C.c_int), ("_pad1", C.c_int)]
+                ("id_free_surf", C.c_int), ("rebuild_policy", C.c_int),
+                ("fric_heat", C.c_double), ("ns_skin", C.c_double), ("ep_hot", C.c_double)]
 
 
 class wf_stats(C.Structure):
@@ -107,6 +109,10 @@ def default_config():
     c.contact_law = WF_CONTACT_REFERENCE
     c.contact_kappa = 0.1
     c.id_free_surf = 1
+    c.rebuild_policy = WF_REBUILD_EVERY_N
+    c.fric_heat = 0.0
+    c.ns_skin = 0.0
+    c.ep_hot = 0.05
     return c
 
 

@@ -144,6 +144,7 @@ __global__ __launch_bounds__(256) void k_contact_force(Fields f, int solid, Mesh
   if (i >= solid) return;
   const int cnt = cl.count[i];
   double cf[3] = {0.0, 0.0, 0.0};
+  double qf = 0.0;  // sliding-friction heat power [W]
   if (cnt > 0) {
     const double dt = cp.dt;
     const double mi = f.m[i], hi = f.h[i];
@@ -199,12 +200,15 @@ __global__ __launch_bounds__(256) void k_contact_force(Fields f, int solid, Mesh
             cf[c] -= fd;
             a[c] -= fd / mi;
           }
+          // dissipated power: |F_friction| x slip speed, |du_t| = |tg| / k
+          qf += cp.fric_heat * (cp.mu_d * lcf) * (ltg / kij) / dt;
         }
       }
     }
     for (int c = 0; c < 3; c++) f.a[c][i] = a[c];
   }
   for (int c = 0; c < 3; c++) f.cf[c][i] = cf[c];
+  f.qfric[i] = qf;
 }
 
 // rigid motion of the nodes: v = v_m + w x (x - c), x += v dt (Mesh.cu:260-276)

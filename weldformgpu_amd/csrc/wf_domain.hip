@@ -151,7 +151,7 @@ int field_info(int field, int* comps, int* elem) {
   if (field >= WF_F_X && field <= WF_F_CONTFORCE) {
     *comps = 3;
     *elem = 8;
-  } else if (field >= WF_F_RHO && field <= WF_F_Q_PLASTIC) {
+  } else if (field >= WF_F_RHO && field <= WF_F_Q_FRICTION) {
     *comps = 1;
     *elem = 8;
   } else if (field >= WF_F_SIGMA && field <= WF_F_ROTRATE) {
@@ -278,6 +278,7 @@ PlaneRef wf_domain::plane(int field, int c) {
     case WF_F_K_T: r.d = &f.k; break;
     case WF_F_CP_T: r.d = &f.cp; break;
     case WF_F_Q_PLASTIC: r.d = &f.qpl; break;
+    case WF_F_Q_FRICTION: r.d = &f.qfric; break;
     case WF_F_SIGMA: r.d = &f.sigma[c]; break;
     case WF_F_SHEARSTRESS: r.d = &f.ss[c]; break;
     case WF_F_STRAIN: r.d = &f.strain[c]; break;
@@ -470,6 +471,7 @@ ContactParams wf_domain::contact_params() const {
   cp.kappa_E_dx = cfg.contact_kappa * cfg.mat.E * cfg.contact_dx;
   cp.mu_s = cfg.fric_sta;
   cp.mu_d = cfg.fric_dyn;
+  cp.fric_heat = cfg.fric_heat;
   cp.law = cfg.contact_law;
   return cp;
 }
@@ -499,6 +501,10 @@ void wf_config_default(wf_config* c) {
   c->contact_law = WF_CONTACT_REFERENCE;
   c->contact_kappa = 0.1;
   c->id_free_surf = 1;  // SolverFraser.cu:120
+  c->rebuild_policy = WF_REBUILD_EVERY_N;
+  c->fric_heat = 0.0;
+  c->ns_skin = 0.0;
+  c->ep_hot = 0.05;
 }
 
 int wf_field_shape(int field, int* components, int* elem_bytes) {
@@ -522,6 +528,10 @@ int wf_create(const wf_config* cfg, wf_domain** out) {
   if (cfg->ns_interval <= 0) return fail(WF_E_ARG, "ns_interval must be positive");
   if (cfg->contact_law != WF_CONTACT_REFERENCE && cfg->contact_law != WF_CONTACT_STIFFNESS)
     return fail(WF_E_ARG, "unknown contact law");
+  if (!(cfg->fric_heat >= 0.0 && cfg->fric_heat <= 1.0))
+    return fail(WF_E_ARG, "fric_heat must lie in [0, 1]");
+  if (cfg->rebuild_policy != WF_REBUILD_EVERY_N)
+    return fail(WF_E_ARG, "unknown rebuild policy");
   if (cfg->contact && cfg->contact_law == WF_CONTACT_STIFFNESS &&
       !(cfg->contact_kappa > 0.0 && cfg->contact_dx > 0.0))
     return fail(WF_E_ARG, "the stiffness contact law needs contact_kappa > 0 and contact_dx > 0");

@@ -24,7 +24,7 @@ struct Fields {
   double *x[3], *v[3], *a[3], *u[3], *va[3], *xs[3];
   double *rho, *rho_alt, *rho0, *drho, *m, *h, *p, *Cs, *G;
   double *sigma_y, *pl_strain, *sigma_eq, *esr, *eint;
-  double *T, *T_alt, *Ta, *Tb, *dTdt, *k, *cp, *qpl;
+  double *T, *T_alt, *Ta, *Tb, *dTdt, *k, *cp, *qpl, *qfric;
   double *sigma[6], *ss[6], *strain[6], *strrate[6], *rotrate[6];
   int *ID, *BC_T;
   // FP32 mirrors read by the mixed-precision pair loops (WF_MIXED): position
@@ -102,6 +102,7 @@ struct ContactLists {
 };
 struct ContactParams {
   double dt, kappa_E_dx, mu_s, mu_d;
+  double fric_heat;  // fraction of the sliding-friction power into the particle
   int law;
 };
 

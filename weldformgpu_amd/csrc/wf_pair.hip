@@ -434,6 +434,8 @@ struct CondD {
     }
     double qp = f.qpl[i];
     if (qp != 0.0) acc += qp * (1. / (rhoi * cpi));
+    const double qf = f.qfric[i];
+    if (qf != 0.0) acc += qf / (f.m[i] * cpi);
     if (bct == WF_BCT_FIXED) acc = 0.0;
     f.dTdt[i] = acc;
     if (p.leap) {
@@ -916,6 +918,8 @@ struct CondG {
     }
     double qp = f.qpl[i];
     if (qp != 0.0) acc += qp * (1. / (rhoi * cpi));
+    const double qf = f.qfric[i];
+    if (qf != 0.0) acc += qf / (f.m[i] * cpi);
     if (bct == WF_BCT_FIXED) acc = 0.0;
     f.dTdt[i] = acc;
     if (p.leap) {

@@ -49,6 +49,7 @@ struct Deck {
   int precision = WF_FP64, return_map = WF_RETURN_REFERENCE, surrogate = 0;
   int contact_law = WF_CONTACT_REFERENCE, quirks = WF_Q_ALL;
   double contact_kappa = 0.1;
+  double fric_heat = 0.0;  // fraction of the sliding-friction power into the particle
   bool thermal = false;
   double T0 = -1.0, k_T = 0.0, cp_T = 0.0, taylor_quinney = 0.0, T_inf = 500.0, h_conv = 100.0;
   long max_steps = -1;
@@ -116,6 +117,7 @@ inline Deck parse_deck(const wfj::Value& j) {
   if (read(ex["contactLaw"], s))
     d.contact_law = s == "stiffness" ? WF_CONTACT_STIFFNESS : WF_CONTACT_REFERENCE;
   read(ex["contactKappa"], d.contact_kappa);
+  read(ex["frictionHeat"], d.fric_heat);
   read(ex["quirks"], d.quirks);
   double ms = -1.0;
   if (read(ex["maxSteps"], ms)) d.max_steps = (long)ms;
@@ -284,6 +286,7 @@ inline Setup build_setup(const Deck& d) {
     c.contact_dx = dx;
     c.fric_sta = d.fric_sta;
     c.fric_dyn = d.fric_dyn;
+    c.fric_heat = d.fric_heat;
   }
   return s;
 }
