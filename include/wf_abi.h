@@ -50,6 +50,9 @@ enum {
 
 /* cap on neighbours per particle: cuNSearchKernels.cuh:6 (CUDA_MAX_NEIGHBORS) */
 #define WF_MAX_NEIGHBORS 70
+/* list capacity under WF_REBUILD_PLASTIC_ZONE (the skin adds the next
+ * lattice shell: 80 neighbours in the bulk of a cubic lattice) */
+#define WF_MAX_NEIGHBORS_SKIN 96
 
 /* ---- material models: src/cuda/Material.cuh:6-8 ------------------------- */
 enum { WF_MAT_BILINEAR = 0, WF_MAT_HOLLOMON = 1, WF_MAT_JOHNSON_COOK = 2 };
@@ -211,7 +214,9 @@ int wf_set_bcs(struct wf_domain* d, int count, const int* zone_ids,
 /* ---- neighbour search: cuNSearch::NeighborhoodSearch::find_neighbors
  *      (src/cuNSearch/src/cuNSearch.cu:111-137).  Radius 2*h_glob, FP32
  *      criterion 0 < |xj-xi|^2 < r^2 (cuNSearchKernels.cu:163-164), cap 70.
- *      Positions come from the snapshot (WF_Q_STALE_NS_POS) or from x. ---- */
+ *      Positions come from the snapshot (WF_Q_STALE_NS_POS) or from x.
+ *      Under WF_REBUILD_PLASTIC_ZONE: radius 2*h_glob + ns_skin, positions
+ *      from x, cap WF_MAX_NEIGHBORS_SKIN. ---- */
 int wf_nsearch_build(struct wf_domain* d);
 /* Total number of stored neighbour entries and particles over the cap. */
 int wf_nsearch_stats(struct wf_domain* d, int64_t* total, int* over_cap);

@@ -799,12 +799,33 @@ double or_mlp_loss_grad(const double* w, const double* X, const double* Y, int n
   return loss;
 }
 
+// cuNSearch find_neighbors (SolverFraser.cu:72,171-175): radius 2 h_glob
+// from the last output snapshot under the stale-position quirk.  Under
+// WF_REBUILD_PLASTIC_ZONE (an extension) the radius is 2 h_glob + ns_skin,
+// the positions are the current ones, and they are kept in xb.
 int or_nsearch_build(or_state* s, const wf_config* c, int* over_cap) {
-  const double* src = (c->quirks & WF_Q_STALE_NS_POS) ? s->xs : s->x;
-  int64_t tot = or_neighbors(src, s->n, 2.0 * c->h_glob, s->nb_count, s->nb_offset, s->nb_list,
+  const bool pz = c->rebuild_policy == WF_REBUILD_PLASTIC_ZONE;
+  const double* src = (!pz && (c->quirks & WF_Q_STALE_NS_POS)) ? s->xs : s->x;
+  const double radius = 2.0 * c->h_glob + (pz ? c->ns_skin : 0.0);
+  int64_t tot = or_neighbors(src, s->n, radius, s->nb_count, s->nb_offset, s->nb_list,
                              s->nb_cap, over_cap);
   if (tot > s->nb_cap) return WF_E_NBCAP;
+  if (pz && s->xb)
+    for (int64_t q = 0; q < 3 * (int64_t)s->n; q++) s->xb[q] = s->x[q];
   return WF_OK;
+}
+
+// WF_REBUILD_PLASTIC_ZONE trigger: some particle with eps_p > ep_hot has
+// moved more than ns_skin / 2 since the last build (squared distances).
+int or_plastic_zone_moved(const or_state* s, const wf_config* c) {
+  double d2max = 0.0;
+  for (int i = 0; i < s->n; i++) {
+    if (!(s->pl_strain[i] > c->ep_hot)) continue;
+    const V3 d = sub(ld3(s->x, i), ld3(s->xb, i));
+    const double d2 = dot(d, d);
+    if (d2 > d2max) d2max = d2;
+  }
+  return 4.0 * d2max > c->ns_skin * c->ns_skin;
 }
 
 // One pass of the MechFraserSolve loop body, SolverFraser.cu:164-399
@@ -898,10 +919,16 @@ void or_calc_surface(or_state* s, const wf_config* c) {
   for (int i = 0; i < s->solid; i++) {
     V3 n{0.0, 0.0, 0.0};
     V3 xi = ld3(s->x, i);
-    const int cnt = s->nb_count[i];
-    for (int k = 0; k < cnt; k++) {
+    // with a list skin only the pairs within the search radius count
+    const double rc2 = c->rebuild_policy == WF_REBUILD_PLASTIC_ZONE
+                           ? 4.0 * c->h_glob * c->h_glob : 0.0;
+    int cnt = 0;
+    for (int k = 0; k < s->nb_count[i]; k++) {
       int j = s->nb_list[s->nb_offset[i] + k];
-      n = add(n, scl(sub(xi, ld3(s->x, j)), s->m[j]));
+      const V3 d = sub(xi, ld3(s->x, j));
+      if (rc2 > 0.0 && !(dot(d, d) < rc2)) continue;
+      n = add(n, scl(d, s->m[j]));
+      cnt++;
     }
     n = scl(n, ntot / (ct->totmass * (double)cnt));
     st3(ct->normal, i, n);
@@ -1003,7 +1030,9 @@ void or_contact_forces(or_state* s, const wf_config* c) {
 int or_step(or_state* s, const wf_config* c, or_run* run, int nbc, const int* bc_zone,
             const double* bc_val) {
   if (c->mechanics) {
-    if (run->ts_i == 0 && run->is_yielding) {
+    const bool pz = c->rebuild_policy == WF_REBUILD_PLASTIC_ZONE;
+    if (pz ? (run->is_yielding && run->rebuild_pending) : (run->ts_i == 0 && run->is_yielding)) {
+      run->rebuild_pending = 0;
       int oc = 0;
       int rc = or_nsearch_build(s, c, &oc);
       if (rc) return rc;
@@ -1034,6 +1063,7 @@ int or_step(or_state* s, const wf_config* c, or_run* run, int nbc, const int* bc
     or_update_vel(s, c->dt);
     for (int b = 0; b < nbc; b++)
       or_apply_bc_vel(s, bc_zone[b], bc_val[3 * b], bc_val[3 * b + 1], bc_val[3 * b + 2]);
+    if (pz && run->is_yielding && s->xb) run->rebuild_pending = or_plastic_zone_moved(s, c);
   }
   if (c->thermal) or_thermal_calcs(s, c, &run->isfirst_step);
   if (c->mechanics) or_int_energy(s, c);

@@ -50,6 +50,8 @@ typedef struct or_state {
   /* Newton statistics of the last or_stress_strain call */
   int64_t newton_iters, newton_calls;
   struct or_contact* contact; /* null: no contact data */
+  double* xb;                 /* 3n: positions at the last build
+                                 (WF_REBUILD_PLASTIC_ZONE; may be null)     */
 } or_state;
 
 /* rigid triangle meshes, all meshes concatenated (TriMesh_d, Mesh.cuh:19-54) */
@@ -81,6 +83,7 @@ typedef struct or_run {
   int ns_builds;
   double max_pl_strain;
   int nb_over_cap;
+  int rebuild_pending;  /* WF_REBUILD_PLASTIC_ZONE: rebuild at the next step */
 } or_run;
 
 /* geometry (Domain.cpp) ----------------------------------------------------*/
@@ -129,6 +132,9 @@ double or_int_energy_sum(const or_state* s);
 
 /* neighbour rebuild from the snapshot (or x) into s->nb_* */
 int or_nsearch_build(or_state* s, const wf_config* c, int* over_cap);
+/* WF_REBUILD_PLASTIC_ZONE: 1 when a particle with eps_p > ep_hot has moved
+ * more than ns_skin / 2 from its position at the last build */
+int or_plastic_zone_moved(const or_state* s, const wf_config* c);
 
 /* Return-map surrogate network (no reference counterpart): FP64 forward and
  * mean-squared-error gradient of the 4 -> 32 -> 32 -> 1 ReLU MLP, parameter

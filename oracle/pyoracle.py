@@ -27,7 +27,7 @@ class or_state(C.Structure):
                 + [(k, C.c_void_p) for k in _DBL3 + _DBL1 + _DBL6 + _INT1]
                 + [("nb_count", C.c_void_p), ("nb_offset", C.c_void_p), ("nb_list", C.c_void_p),
                    ("nb_cap", C.c_int64), ("newton_iters", C.c_int64),
-                   ("newton_calls", C.c_int64), ("contact", C.c_void_p)])
+                   ("newton_calls", C.c_int64), ("contact", C.c_void_p), ("xb", C.c_void_p)])
 
 
 class or_mesh(C.Structure):
@@ -44,7 +44,8 @@ class or_contact(C.Structure):
 class or_run(C.Structure):
     _fields_ = [("time", C.c_double), ("step", C.c_int64), ("ts_i", C.c_int),
                 ("is_yielding", C.c_int), ("isfirst_step", C.c_int), ("ns_builds", C.c_int),
-                ("max_pl_strain", C.c_double), ("nb_over_cap", C.c_int)]
+                ("max_pl_strain", C.c_double), ("nb_over_cap", C.c_int),
+                ("rebuild_pending", C.c_int)]
 
 
 def build():
@@ -90,6 +91,7 @@ def lib():
             "or_int_energy": (None, [S, CF]),
             "or_int_energy_sum": (D, [S]),
             "or_nsearch_build": (I, [S, CF, C.POINTER(I)]),
+            "or_plastic_zone_moved": (I, [S, CF]),
             "or_step": (I, [S, CF, R, I, P, P]),
             "or_gen_plane_mesh": (I, [P, P, I, P, P]),
             "or_mesh_geom": (None, [P]),
@@ -193,6 +195,8 @@ class State:
         for k, a in self.arrays.items():
             setattr(self.st, k, a.ctypes.data)
         self.st.nb_cap = self.nb_cap
+        self.arrays["xb"] = np.zeros((n, 3))  # positions at the last build
+        self.st.xb = self.arrays["xb"].ctypes.data
         self.run = or_run()
         self.run.isfirst_step = 1
 
@@ -212,6 +216,9 @@ class State:
             raise RuntimeError(f"oracle nsearch failed {rc}")
         self.run.nb_over_cap = oc.value
         return oc.value
+
+    def plastic_zone_moved(self, cfg):
+        return bool(lib().or_plastic_zone_moved(self.ref(), C.byref(cfg)))
 
     def neighbor_rows(self):
         c, o, l = self["nb_count"], self["nb_offset"], self["nb_list"]

@@ -72,6 +72,24 @@ def test_contact_deck_setup():
     assert s["T0"] == 300.0
 
 
+def test_extension_options(tmp_path):
+    """Extensions of this library in the deck (the reference ignores the
+    object): sliding-friction heat and the plastic-zone rebuild policy."""
+    d = _deck("contact_mini.json")
+    s = _dry(os.path.join(DECKS, "contact_mini.json"))
+    assert s["fric_heat"] == 0.0 and s["rebuild_policy"] == 0 and s["ep_hot"] == 0.05
+    d["Extensions"]["frictionHeat"] = 0.9
+    d["Extensions"]["rebuild"] = {"policy": "plasticZone", "skin": 2e-5, "epHot": 0.1}
+    p = tmp_path / "ext.json"
+    p.write_text(json.dumps(d))
+    s = _dry(str(p))
+    assert s["fric_heat"] == 0.9 and s["rebuild_policy"] == 1
+    assert s["ns_skin"] == 2e-5 and s["ep_hot"] == 0.1
+    d["Extensions"]["rebuild"]["policy"] = "sometimes"
+    p.write_text(json.dumps(d))
+    assert "plasticZone" in _dry(str(p), expect_rc=2)
+
+
 def test_bad_decks_fail_with_a_message(tmp_path):
     p = tmp_path / "broken.json"
     p.write_text('{"Configuration": {"particleRadius": 0.001,, }}')

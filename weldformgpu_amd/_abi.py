@@ -19,6 +19,7 @@ WF_E_NOMEM = -6
 WF_E_UNSUPPORTED = -7
 
 WF_MAX_NEIGHBORS = 70
+WF_MAX_NEIGHBORS_SKIN = 96
 WF_MAX_BCS = 16
 
 WF_MAT_BILINEAR, WF_MAT_HOLLOMON, WF_MAT_JOHNSON_COOK = 0, 1, 2

@@ -70,6 +70,15 @@ struct wf_domain {
   int nblk = 0;  // blocks of the per-particle stress kernel
   int64_t mlp_cap = 0;
   bool jc_newton = false;  // Johnson-Cook under the Newton return map
+  // WF_REBUILD_PLASTIC_ZONE: positions at the last build, the trigger's
+  // largest squared displacement (device, pinned host copy), and whether
+  // the next step starts with a rebuild
+  double* xb[3] = {nullptr, nullptr, nullptr};
+  unsigned long long *pz_d2 = nullptr, *pz_d2_host = nullptr;
+  bool pz_pending = false;
+  bool plastic_zone() const { return cfg.rebuild_policy == WF_REBUILD_PLASTIC_ZONE; }
+  // with a list skin the surface pass counts only pairs within 2 h_glob
+  double surface_r2cut() const { return plastic_zone() ? 4.0 * cfg.h_glob * cfg.h_glob : 0.0; }
   RetArgs ret_args() const {
     return RetArgs{mlp_w, (cfg.surrogate && mlp_trained) ? 1 : 0, (long long)step};
   }
@@ -530,8 +539,10 @@ int wf_create(const wf_config* cfg, wf_domain** out) {
     return fail(WF_E_ARG, "unknown contact law");
   if (!(cfg->fric_heat >= 0.0 && cfg->fric_heat <= 1.0))
     return fail(WF_E_ARG, "fric_heat must lie in [0, 1]");
-  if (cfg->rebuild_policy != WF_REBUILD_EVERY_N)
+  if (cfg->rebuild_policy != WF_REBUILD_EVERY_N && cfg->rebuild_policy != WF_REBUILD_PLASTIC_ZONE)
     return fail(WF_E_ARG, "unknown rebuild policy");
+  if (cfg->rebuild_policy == WF_REBUILD_PLASTIC_ZONE && !(cfg->ns_skin > 0.0 && cfg->ep_hot >= 0.0))
+    return fail(WF_E_ARG, "the plastic-zone rebuild needs ns_skin > 0 and ep_hot >= 0");
   if (cfg->contact && cfg->contact_law == WF_CONTACT_STIFFNESS &&
       !(cfg->contact_kappa > 0.0 && cfg->contact_dx > 0.0))
     return fail(WF_E_ARG, "the stiffness contact law needs contact_kappa > 0 and contact_dx > 0");
@@ -625,9 +636,15 @@ int wf_create(const wf_config* cfg, wf_domain** out) {
   if (!rc) rc = d->alloc_plane_i(&d->f.idlock);
   if (!rc) rc = d->dev_alloc((void**)&d->cscratch, sizeof(double) * 8);
   if (!rc && hipHostMalloc(&d->cscratch_h, sizeof(double) * 8) != hipSuccess) rc = WF_E_NOMEM;
+  if (cfg->rebuild_policy == WF_REBUILD_PLASTIC_ZONE) {
+    for (int c = 0; c < 3 && !rc; c++) rc = d->alloc_plane_d(&d->xb[c]);
+    if (!rc) rc = d->dev_alloc((void**)&d->pz_d2, sizeof(unsigned long long));
+    if (!rc && hipHostMalloc(&d->pz_d2_host, sizeof(unsigned long long)) != hipSuccess)
+      rc = WF_E_NOMEM;
+  }
   if (!rc) {
     void* q = nullptr;
-    if (hipMalloc(&q, sizeof(int) * d->npad * kMaxNb) != hipSuccess) rc = WF_E_NOMEM;
+    if (hipMalloc(&q, sizeof(int) * d->npad * kListCap) != hipSuccess) rc = WF_E_NOMEM;
     else {
       d->owned.push_back(q);
       d->nb_list = (int*)q;
@@ -685,6 +702,7 @@ int wf_destroy(wf_domain* d) {
   if (d->host_i) hipHostFree(d->host_i);
   if (d->host_f) hipHostFree(d->host_f);
   if (d->cscratch_h) hipHostFree(d->cscratch_h);
+  if (d->pz_d2_host) hipHostFree(d->pz_d2_host);
   for (void* p : d->mesh_owned) hipFree(p);
   ns_free(d->ns);
   if (d->ev0) hipEventDestroy(d->ev0);
@@ -819,7 +837,9 @@ static int timer_end(wf_domain* d, int k) {
 
 int wf_nsearch_build(wf_domain* d) {
   if (!d) return fail(WF_E_ARG, "null domain");
-  const bool stale = (d->cfg.quirks & WF_Q_STALE_NS_POS) != 0;
+  // the plastic-zone policy builds from the current positions with a skin
+  const bool pz = d->plastic_zone();
+  const bool stale = !pz && (d->cfg.quirks & WF_Q_STALE_NS_POS) != 0;
   if (stale && d->nb_built && d->built_gen == d->snap_gen) {
     d->ns_builds++;
     return WF_OK;
@@ -827,9 +847,9 @@ int wf_nsearch_build(wf_domain* d) {
   timer_begin(d);
   const double* src[3];
   for (int c = 0; c < 3; c++) src[c] = stale ? d->f.xs[c] : d->f.x[c];
-  int over = 0;
-  int rc = ns_build(d->stream, &d->ns, src, d->n, d->npad, 2.0 * d->cfg.h_glob, d->gather,
-                    d->nb_count, d->nb_list, d->ctr, &over, d->perm, d->perm_tmp, &d->tl,
+  const double radius = 2.0 * d->cfg.h_glob + (pz ? d->cfg.ns_skin : 0.0);
+  int rc = ns_build(d->stream, &d->ns, src, d->n, d->npad, radius, pz ? kListCap : kMaxNb,
+                    d->gather, d->nb_count, d->nb_list, d->ctr, d->perm, d->perm_tmp, &d->tl,
                     &d->grid);
   if (rc)
     return fail(rc, rc == WF_E_NONFINITE ? "non-finite particle position"
@@ -867,6 +887,11 @@ int wf_nsearch_build(wf_domain* d) {
     }
   }
   std::swap(d->perm, d->perm_tmp);
+  if (pz)
+    for (int c = 0; c < 3; c++)
+      HIPC(hipMemcpyAsync(d->xb[c], d->f.x[c], sizeof(double) * d->n, hipMemcpyDeviceToDevice,
+                          d->stream));
+  d->pz_pending = false;
   HIPC(hipGetLastError());
   d->nb_built = true;
   d->built_gen = d->snap_gen;
@@ -897,7 +922,7 @@ int wf_get_neighbors(wf_domain* d, int* counts, int* offsets, int* list, int64_t
   if (!d || !counts || !offsets) return fail(WF_E_ARG, "null argument");
   if (!d->nb_built) return fail(WF_E_STATE, "neighbour lists not built");
   int n = d->n;
-  std::vector<int> cnt(n), perm(n), lst((size_t)d->npad * kMaxNb);
+  std::vector<int> cnt(n), perm(n), lst((size_t)d->npad * kListCap);
   HIPC(hipMemcpyAsync(cnt.data(), d->nb_count, sizeof(int) * n, hipMemcpyDeviceToHost, d->stream));
   HIPC(hipMemcpyAsync(perm.data(), d->perm, sizeof(int) * n, hipMemcpyDeviceToHost, d->stream));
   HIPC(hipMemcpyAsync(lst.data(), d->nb_list, sizeof(int) * lst.size(), hipMemcpyDeviceToHost,
@@ -1200,7 +1225,7 @@ int wf_calc_surface(wf_domain* d) {
   int rc = d->surface_init();
   if (rc) return rc;
   launch_surface_tiled(d->stream, d->f, d->tl, d->solid, (double)d->n + (double)d->mesh.ne,
-                       d->totmass, d->cfg.id_free_surf);
+                       d->totmass, d->cfg.id_free_surf, d->surface_r2cut());
   HIPC(hipGetLastError());
   return WF_OK;
 }
@@ -1266,6 +1291,7 @@ int wf_contact_force_sum(wf_domain* d, double* sum) {
 // (SolverFraser.cu:166; under the stale-snapshot quirk a rebuild from an
 // unchanged snapshot is skipped, see wf_nsearch_build).
 static bool rebuild_next_step(const wf_domain* d) {
+  if (d->plastic_zone()) return d->is_yielding && d->pz_pending;
   const int ts_next = d->ts_i + 1 > d->cfg.ns_interval - 1 ? 0 : d->ts_i + 1;
   if (ts_next != 0 || !d->is_yielding) return false;
   const bool stale = (d->cfg.quirks & WF_Q_STALE_NS_POS) != 0;
@@ -1288,7 +1314,8 @@ int wf_step(wf_domain* d, int n_steps) {
   for (int s = 0; s < n_steps; s++) {
     StepParams p = d->params();
     if (d->cfg.mechanics) {
-      if (d->ts_i == 0 && d->is_yielding) {
+      if (d->plastic_zone() ? (d->is_yielding && d->pz_pending)
+                            : (d->ts_i == 0 && d->is_yielding)) {
         rc = wf_nsearch_build(d);
         if (rc) return rc;
       }
@@ -1326,7 +1353,7 @@ int wf_step(wf_domain* d, int n_steps) {
         timer_begin(d);
         if ((rc = d->surface_init())) return rc;
         launch_surface_tiled(d->stream, d->f, d->tl, d->solid, (double)d->n + (double)d->mesh.ne,
-                             d->totmass, d->cfg.id_free_surf);
+                             d->totmass, d->cfg.id_free_surf, d->surface_r2cut());
         if (d->n_meshes > 0) {
           if (d->ts_i == 0 && (rc = d->contact_nb())) return rc;
           launch_contact_force(d->stream, d->f, d->solid, d->mesh, d->cl, d->contact_params());
@@ -1345,6 +1372,17 @@ int wf_step(wf_domain* d, int n_steps) {
       timer_begin(d);
       launch_integrate(d->stream, d->f, d->n, d->cfg.dt, d->bcs, rec, d->frame);
       if ((rc = timer_end(d, WF_K_INTEGRATE))) return rc;
+      if (d->plastic_zone() && d->is_yielding) {  // the rebuild trigger, read back each step
+        hipLaunchKernelGGL(k_zero_bytes, dim3(1), dim3(256), 0, d->stream,
+                           (unsigned char*)d->pz_d2, (int64_t)sizeof(unsigned long long));
+        launch_hot_disp(d->stream, d->f, d->xb, d->n, d->cfg.ep_hot, d->pz_d2);
+        HIPC(hipMemcpyAsync(d->pz_d2_host, d->pz_d2, sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, d->stream));
+        HIPC(hipStreamSynchronize(d->stream));
+        double d2;
+        std::memcpy(&d2, d->pz_d2_host, sizeof(d2));
+        d->pz_pending = 4.0 * d2 > d->cfg.ns_skin * d->cfg.ns_skin;
+      }
     }
     if (d->cfg.thermal) {
       d->refresh_mirrors();

@@ -16,7 +16,8 @@
 namespace wf {
 
 constexpr int kBlock = 256;       // threads per block of the per-particle kernels
-constexpr int kMaxNb = WF_MAX_NEIGHBORS;
+constexpr int kMaxNb = WF_MAX_NEIGHBORS;          // the reference's cap
+constexpr int kListCap = WF_MAX_NEIGHBORS_SKIN;  // list storage (either policy)
 
 // Plane pointers of every particle field (device addresses, sorted order).
 // Passed by value to kernels.
@@ -114,8 +115,9 @@ void launch_max_f64(hipStream_t s, const double* v, int n, double* out);
 void launch_tri_bin(hipStream_t s, const MeshDev& m, const TriGrid& g, int* cnt, int* rank,
                     int* start, int* sorted, int* scan_part);
 void launch_surface_lock(hipStream_t s, const Fields& f, int n);
+// r2cut > 0: only pairs with |x_ij|^2 < r2cut count (lists built with a skin)
 void launch_surface_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
-                          double scale_num, double totmass, int id_free_surf);
+                          double scale_num, double totmass, int id_free_surf, double r2cut);
 void launch_contact_nb(hipStream_t s, const Fields& f, int solid, const MeshDev& m,
                        const TriGrid& g, const int* start, const int* sorted,
                        const ContactLists& cl, int id_free_surf, Counters* ctr);
@@ -152,7 +154,8 @@ struct Frame32 {
 };
 
 // Neighbour lists: interleaved, entry k of particle i at list[k*npad + i]
-// (a wavefront reads 64 consecutive ints per k), count[i] <= kMaxNb.
+// (a wavefront reads 64 consecutive ints per k), count[i] <= the build's cap
+// (kMaxNb, or kListCap with a list skin).
 struct NbLists {
   const int* count;
   const int* list;
@@ -235,6 +238,11 @@ void launch_pressure(hipStream_t s, const Fields& f, const StepParams& p);
 void launch_stress(hipStream_t s, const Fields& f, const StepParams& p, BlockStats* bs,
                    bool fuse_pressure, bool energy, int rec, const RetArgs& ra);
 // fold the per-block statistics of the last stress (+ return) pass into ctr
+// largest squared displacement since the last build (x - xb) over the
+// particles with eps_p > ep_hot, as the bits of a non-negative double
+// max-combined into *d2_bits (which the caller clears)
+void launch_hot_disp(hipStream_t s, const Fields& f, const double* const xb[3], int n,
+                     double ep_hot, unsigned long long* d2_bits);
 void launch_fold_stats(hipStream_t s, const BlockStats* bs, int nblocks, Counters* ctr);
 // ---- return-map surrogate (wf_mlp.hip) -------------------------------------
 // MFMA inference of the surrogate on given feature rows (test hook)
@@ -271,7 +279,7 @@ struct NsResult {
 // perm: sorted slot -> original index of the current order; perm_tmp receives
 // the new one.
 int ns_build(hipStream_t s, NsWork** work, const double* const pos_src[3], int n, int64_t npad,
-             double radius, int* gather, int* count, int* list, Counters* ctr, int* over_cap,
+             double radius, int cap, int* gather, int* count, int* list, Counters* ctr,
              int* perm, int* perm_tmp, TileLists* tiles, GridDesc* grid_out);
 void ns_free(NsWork* w);
 // exclusive scan of m ints into out[0..m] (out[m] = total); part needs

@@ -296,6 +296,25 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Block
   }
 }
 
+// Rebuild trigger of WF_REBUILD_PLASTIC_ZONE: the largest squared
+// displacement from the last build over the particles with eps_p > ep_hot.
+// A max of non-negative doubles is the max of their bit patterns, so the
+// result does not depend on the order of the atomics.
+__global__ __launch_bounds__(kBlock) void k_hot_disp(Fields f, const double* __restrict__ xb0,
+                                                     const double* __restrict__ xb1,
+                                                     const double* __restrict__ xb2, int n,
+                                                     double ep_hot,
+                                                     unsigned long long* d2_bits) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long b = 0ull;
+  if (i < n && f.pl_strain[i] > ep_hot) {
+    const double dx = f.x[0][i] - xb0[i], dy = f.x[1][i] - xb1[i], dz = f.x[2][i] - xb2[i];
+    b = (unsigned long long)__double_as_longlong(dx * dx + dy * dy + dz * dz);
+  }
+  b = wave_max_u64(b);
+  if (lane_id() == 0 && b) atomicMax(d2_bits, b);
+}
+
 // one block folds the per-block statistics into the run counters
 __global__ __launch_bounds__(256) void k_fold_stats(const BlockStats* bs, int nb, Counters* ctr) {
   __shared__ unsigned long long sh[4][256];
@@ -555,6 +574,12 @@ void launch_stress(hipStream_t s, const Fields& f, const StepParams& p, BlockSta
     hipLaunchKernelGGL((k_stress<false, true, kRecNone>), g, b, 0, s, f, p, bs, ra);
   else
     hipLaunchKernelGGL((k_stress<false, false, kRecNone>), g, b, 0, s, f, p, bs, ra);
+}
+void launch_hot_disp(hipStream_t s, const Fields& f, const double* const xb[3], int n,
+                     double ep_hot, unsigned long long* d2_bits) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_hot_disp, dim3(grid_for(n)), dim3(kBlock), 0, s, f, xb[0], xb[1], xb[2],
+                     n, ep_hot, d2_bits);
 }
 void launch_fold_stats(hipStream_t s, const BlockStats* bs, int nblocks, Counters* ctr) {
   if (nblocks <= 0) return;

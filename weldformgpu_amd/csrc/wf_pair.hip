@@ -950,6 +950,7 @@ struct SurfK {
     int limit;
     int id_free_surf;
     double ntot, totmass;
+    double r2cut;  // > 0: lists carry a skin; count only pairs within the radius
   };
   WF_FP64_POLICY
   struct Own {
@@ -970,11 +971,14 @@ struct SurfK {
     L[0][s] = f.x[0][j]; L[1][s] = f.x[1][j]; L[2][s] = f.x[2][j];
     L[3][s] = f.m[j];
   }
-  __device__ static void pair(double (*L)[kChunk64], int s, const Own& o, Acc& a, const Prm&) {
+  __device__ static void pair(double (*L)[kChunk64], int s, const Own& o, Acc& a,
+                              const Prm& p) {
+    const double dx = o.x - L[0][s], dy = o.y - L[1][s], dz = o.z - L[2][s];
+    if (p.r2cut > 0.0 && !(dx * dx + dy * dy + dz * dz < p.r2cut)) return;
     const double mj = L[3][s];
-    a.nx += mj * (o.x - L[0][s]);
-    a.ny += mj * (o.y - L[1][s]);
-    a.nz += mj * (o.z - L[2][s]);
+    a.nx += mj * dx;
+    a.ny += mj * dy;
+    a.nz += mj * dz;
     a.cnt++;
   }
   __device__ static void finish(const Fields& f, int i, bool act, const Own&, const Acc& a,
@@ -993,16 +997,15 @@ struct SurfK {
 static inline int ntiles(int n) { return (n + kTileP - 1) / kTileP; }
 
 void launch_surface_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
-                          double ntot, double totmass, int id_free_surf) {
+                          double ntot, double totmass, int id_free_surf, double r2cut) {
   if (solid <= 0) return;
-  SurfK::Prm p{solid, id_free_surf, ntot, totmass};
+  SurfK::Prm p{solid, id_free_surf, ntot, totmass, r2cut};
   hipLaunchKernelGGL(k_pair_tiled<SurfK>, dim3(ntiles(solid)), dim3(kTileP), 0, s, f, tl, p);
 }
 
 }  // namespace wf
 
 namespace wf {
-// launch policy POL with every particle's h equal (UH) or not
 // Conduction of one step fused with the density pass of the next: both read
 // the positions after integration and the same neighbour lists, and neither
 // reads what the other writes (the density pass puts the new m/rho into the
@@ -1062,6 +1065,7 @@ using CondDensD = CondDens<CondD<UH>, DensD<UH>>;
 template <bool UH>
 using CondDensG = CondDens<CondG<UH>, DensG<UH>>;
 
+// launch policy POL with every particle's h equal (UH) or not
 template <template <bool> class POL, class... A>
 static void launch_gather(hipStream_t s, const Fields& f, const TileLists& tl, int grid, bool uh,
                           A... args) {

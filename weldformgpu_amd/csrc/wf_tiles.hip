@@ -155,13 +155,14 @@ __global__ __launch_bounds__(kTileP) void k_tile_cells(const float* __restrict__
 
 // Neighbour lists in staged order (global index + 16-bit slot), and the
 // staged set tile_j.  Same float criterion as the reference search:
-// 0 < d2 < r2, d2 = dx*dx + dy*dy + dz*dz without contraction, cap 70.
+// 0 < d2 < r2, d2 = dx*dx + dy*dy + dz*dz without contraction, at most cap
+// entries (70 as in the reference, 96 with a list skin).
 __global__ __launch_bounds__(kTileP) void k_tile_lists(
     const float* __restrict__ p0, const float* __restrict__ p1, const float* __restrict__ p2,
     int n, GridDesc g, const int* __restrict__ cell_start, const int* __restrict__ tile_keys,
-    const int* __restrict__ tile_nc, const int* __restrict__ tile_uoff, float r2, int64_t npad,
-    int* __restrict__ count, int* __restrict__ list, unsigned short* __restrict__ slot,
-    int* __restrict__ tile_j, int* over_cap) {
+    const int* __restrict__ tile_nc, const int* __restrict__ tile_uoff, float r2, int cap,
+    int64_t npad, int* __restrict__ count, int* __restrict__ list,
+    unsigned short* __restrict__ slot, int* __restrict__ tile_j, int* over_cap) {
 #pragma clang fp contract(off)
   __shared__ int sb[kTileMaxCells], sn[kTileMaxCells], spre[kTileMaxCells];
   __shared__ int scx[kTileMaxCells], scy[kTileMaxCells], scz[kTileMaxCells];
@@ -206,7 +207,7 @@ __global__ __launch_bounds__(kTileP) void k_tile_lists(
         float d2 = __fadd_rn(__fadd_rn(__fmul_rn(ddx, ddx), __fmul_rn(ddy, ddy)),
                              __fmul_rn(ddz, ddz));
         if (d2 < r2 && d2 > 0.0f) {
-          if (c < kMaxNb) {
+          if (c < cap) {
             list[(int64_t)c * npad + i] = j;
             slot[(((int64_t)(c >> 2)) * npad + i) * 4 + (c & 3)] = (unsigned short)(base + (j - b));
           } else {
@@ -216,7 +217,7 @@ __global__ __launch_bounds__(kTileP) void k_tile_lists(
         }
       }
     }
-    count[i] = c < kMaxNb ? c : kMaxNb;
+    count[i] = c < cap ? c : cap;
   }
   unsigned long long o = wave_sum_u64((unsigned long long)over);
   if (lane_id() == 0 && o) atomicAdd(over_cap, (int)o);
@@ -232,11 +233,11 @@ void launch_tile_cells(hipStream_t s, const float* const pf[3], int n, const Gri
 
 void launch_tile_lists(hipStream_t s, const float* const pf[3], int n, const GridDesc& g,
                        const int* cell_start, const int* tile_keys, const int* tile_nc,
-                       const int* tile_uoff, float r2, int64_t npad, int* count, int* list,
-                       unsigned short* slot, int* tile_j, int* over_cap) {
+                       const int* tile_uoff, float r2, int cap, int64_t npad, int* count,
+                       int* list, unsigned short* slot, int* tile_j, int* over_cap) {
   int nt = (n + kTileP - 1) / kTileP;
   hipLaunchKernelGGL(k_tile_lists, dim3(nt), dim3(kTileP), 0, s, pf[0], pf[1], pf[2], n, g,
-                     cell_start, tile_keys, tile_nc, tile_uoff, r2, npad, count, list, slot,
+                     cell_start, tile_keys, tile_nc, tile_uoff, r2, cap, npad, count, list, slot,
                      tile_j, over_cap);
 }
 

@@ -5,11 +5,14 @@
 
 #include <cstdint>
 
+#include "../../include/wf_abi.h"
+
 namespace wf {
 
 constexpr int kTileP = 128;         // particles per tile = threads per pair workgroup
 constexpr int kTileMaxCells = 1024; // occupied neighbour cells per tile
-constexpr int kSlotQuads = (70 + 3) / 4;  // quads of 16-bit slots per particle
+// quads of 16-bit slots per particle (list capacity of either rebuild policy)
+constexpr int kSlotQuads = (WF_MAX_NEIGHBORS_SKIN + 3) / 4;
 
 // Uniform search grid: cell (cx,cy,cz) = floor((p - org) * inv_cs); cell key
 // = 8^3 meta-block index * 512 + Morton code inside the block.
@@ -36,7 +39,7 @@ void launch_tile_cells(hipStream_t s, const float* const pf[3], int n, const Gri
                        int* overflow);
 void launch_tile_lists(hipStream_t s, const float* const pf[3], int n, const GridDesc& g,
                        const int* cell_start, const int* tile_keys, const int* tile_nc,
-                       const int* tile_uoff, float r2, int64_t npad, int* count, int* list,
-                       unsigned short* slot, int* tile_j, int* over_cap);
+                       const int* tile_uoff, float r2, int cap, int64_t npad, int* count,
+                       int* list, unsigned short* slot, int* tile_j, int* over_cap);
 
 }  // namespace wf

@@ -50,6 +50,8 @@ struct Deck {
   int contact_law = WF_CONTACT_REFERENCE, quirks = WF_Q_ALL;
   double contact_kappa = 0.1;
   double fric_heat = 0.0;  // fraction of the sliding-friction power into the particle
+  int rebuild_policy = WF_REBUILD_EVERY_N;
+  double ns_skin = 0.0, ep_hot = 0.05;  // Extensions.rebuild (plastic-zone policy)
   bool thermal = false;
   double T0 = -1.0, k_T = 0.0, cp_T = 0.0, taylor_quinney = 0.0, T_inf = 500.0, h_conv = 100.0;
   long max_steps = -1;
@@ -118,6 +120,16 @@ inline Deck parse_deck(const wfj::Value& j) {
     d.contact_law = s == "stiffness" ? WF_CONTACT_STIFFNESS : WF_CONTACT_REFERENCE;
   read(ex["contactKappa"], d.contact_kappa);
   read(ex["frictionHeat"], d.fric_heat);
+  const wfj::Value& rb = ex["rebuild"];
+  if (!rb.is_null()) {
+    if (read(rb["policy"], s)) {
+      if (s == "plasticZone") d.rebuild_policy = WF_REBUILD_PLASTIC_ZONE;
+      else if (s != "everyN") throw std::runtime_error("Extensions.rebuild.policy " + s +
+                                                       " is not one of everyN, plasticZone");
+    }
+    read(rb["skin"], d.ns_skin);
+    read(rb["epHot"], d.ep_hot);
+  }
   read(ex["quirks"], d.quirks);
   double ms = -1.0;
   if (read(ex["maxSteps"], ms)) d.max_steps = (long)ms;
@@ -279,6 +291,9 @@ inline Setup build_setup(const Deck& d) {
         }
     s.meshes.push_back(me);
   }
+  c.rebuild_policy = d.rebuild_policy;
+  c.ns_skin = d.ns_skin;
+  c.ep_hot = d.ep_hot;
   if (!s.meshes.empty()) {
     c.contact = 1;
     c.contact_law = d.contact_law;

@@ -110,6 +110,9 @@ void print_setup(const wfd::Deck& dk, const wfd::Setup& s) {
               s.n, s.dt, s.h, s.Cs, s.K, s.G, s.mass, c.mat.sy0, s.T0, c.mat.model, c.precision,
               c.return_map, c.thermal, c.contact, c.contact_law, c.fric_sta, c.fric_dyn,
               dk.sim_time, dk.out_time);
+  std::printf(" \"fric_heat\": %.17g, \"rebuild_policy\": %d, \"ns_skin\": %.17g,"
+              " \"ep_hot\": %.17g,",
+              c.fric_heat, c.rebuild_policy, c.ns_skin, c.ep_hot);
   std::printf(" \"zones\": [");
   for (size_t z = 0; z < dk.zones.size(); z++)
     std::printf("%s[%d, %d]", z ? ", " : "", dk.zones[z].id, s.zone_count[z]);
