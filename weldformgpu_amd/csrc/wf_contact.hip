@@ -17,6 +17,8 @@
 // dynamic Coulomb friction from the predicted tangential slip.  Mesh motion,
 // centroids, normals and plane constants follow TriMesh_d::Move /
 // CalcCentroids / CalcNormals / UpdatePlaneCoeff (Mesh.cu:223-276).
+#include <algorithm>
+
 #include "wf_device.hpp"
 
 namespace wf {
@@ -45,17 +47,21 @@ __global__ __launch_bounds__(256) void k_tri_bbox(MeshDev m, double* out) {
   if (threadIdx.x < 6) out[threadIdx.x] = sh[threadIdx.x][0];
 }
 
-__global__ __launch_bounds__(256) void k_max_f64(const double* v, int n, double* out) {
+// max of v[0..n) in two passes: each block reduces a grid-stride share into
+// part[blockIdx.x], then one block reduces the parts (max is exact, so the
+// result does not depend on the split)
+__global__ __launch_bounds__(256) void k_max_f64(const double* v, int64_t n, double* out) {
   __shared__ double sh[256];
   double mx = -1e300;
-  for (int i = threadIdx.x; i < n; i += 256) mx = fmax(mx, v[i]);
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    mx = fmax(mx, v[i]);
   sh[threadIdx.x] = mx;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if (threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = sh[0];
+  if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
 }
 
 // grid cell of a point; false when it lies beyond one cell outside the grid
@@ -278,8 +284,10 @@ void launch_mesh_move(hipStream_t s, const MeshDev& m, const MeshMotions& mo, do
 void launch_tri_bbox(hipStream_t s, const MeshDev& m, double* out) {
   hipLaunchKernelGGL(k_tri_bbox, dim3(1), dim3(256), 0, s, m, out);
 }
-void launch_max_f64(hipStream_t s, const double* v, int n, double* out) {
-  hipLaunchKernelGGL(k_max_f64, dim3(1), dim3(256), 0, s, v, n, out);
+void launch_max_f64(hipStream_t s, const double* v, int n, double* out, double* part) {
+  const int nb = std::max(1, std::min(kMaxParts, (int)(((int64_t)n + 255) / 256)));
+  hipLaunchKernelGGL(k_max_f64, dim3(nb), dim3(256), 0, s, v, (int64_t)n, part);
+  hipLaunchKernelGGL(k_max_f64, dim3(1), dim3(256), 0, s, (const double*)part, (int64_t)nb, out);
 }
 void launch_tri_bin(hipStream_t s, const MeshDev& m, const TriGrid& g, int* cnt, int* rank,
                     int* start, int* sorted, int* scan_part) {

@@ -94,6 +94,8 @@ struct wf_domain {
   int64_t mlp_last_samples = 0;
   bool mixed = false;
   bool uniform_h = true;     // every particle has the same h (FP32 loops skip h_j)
+  bool uniform_m = true;     // every particle has the same mass m_value (the
+  double m_value = 0.0;      // surface pass skips the neighbour-mass gather)
   bool mirror_dirty = true;  // FP32 mirrors out of date with the FP64 state
   bool nb_built = false;
   bool snapshot_set = false;
@@ -117,7 +119,7 @@ struct wf_domain {
   int *tri_cnt = nullptr, *tri_start = nullptr, *tri_rank = nullptr, *tri_sorted = nullptr;
   int* tri_scan_part = nullptr;
   int64_t tri_cells_cap = 0;
-  double* cscratch = nullptr;   // device: centroid bbox + max h
+  double* cscratch = nullptr;   // device: centroid bbox, max h, block partials
   double* cscratch_h = nullptr; // pinned
   // run state (SolverFraser.cu:30-409)
   double time = 0.0;
@@ -424,7 +426,7 @@ int wf_domain::surface_init() {
 int wf_domain::contact_nb() {
   constexpr int64_t kMaxTriCells = 1 << 22;
   launch_tri_bbox(stream, mesh, cscratch);
-  launch_max_f64(stream, f.h, solid, cscratch + 6);
+  launch_max_f64(stream, f.h, solid, cscratch + 6, cscratch + 8);
   HIPC(hipGetLastError());
   HIPC(hipMemcpyAsync(cscratch_h, cscratch, sizeof(double) * 7, hipMemcpyDeviceToHost, stream));
   HIPC(hipStreamSynchronize(stream));
@@ -634,7 +636,7 @@ int wf_create(const wf_config* cfg, wf_domain** out) {
   if (!rc) rc = d->alloc_plane_i(&d->gather);
   if (!rc) rc = d->alloc_plane_i(&d->nb_count);
   if (!rc) rc = d->alloc_plane_i(&d->f.idlock);
-  if (!rc) rc = d->dev_alloc((void**)&d->cscratch, sizeof(double) * 8);
+  if (!rc) rc = d->dev_alloc((void**)&d->cscratch, sizeof(double) * (8 + kMaxParts));
   if (!rc && hipHostMalloc(&d->cscratch_h, sizeof(double) * 8) != hipSuccess) rc = WF_E_NOMEM;
   if (cfg->rebuild_policy == WF_REBUILD_PLASTIC_ZONE) {
     for (int c = 0; c < 3 && !rc; c++) rc = d->alloc_plane_d(&d->xb[c]);
@@ -731,6 +733,12 @@ int wf_set_field(wf_domain* d, int field, const void* host, size_t bytes) {
     d->uniform_h = true;
     for (int i = 1; i < d->n && d->uniform_h; i++) d->uniform_h = hv[i] == hv[0];
   }
+  if (field == WF_F_M) {
+    const double* mv = (const double*)host;
+    d->uniform_m = true;
+    d->m_value = mv[0];
+    for (int i = 1; i < d->n && d->uniform_m; i++) d->uniform_m = mv[i] == mv[0];
+  }
   HIPC(hipMemcpyAsync(d->stage, host, bytes, hipMemcpyHostToDevice, d->stream));
   if (elem == 8) {
     Planes6 planes{};
@@ -785,7 +793,11 @@ int wf_fill_field(wf_domain* d, int field, double value) {
   int comps, elem;
   if (field_info(field, &comps, &elem) || elem != 8) return fail(WF_E_ARG, "not a double field");
   for (int c = 0; c < comps; c++) launch_fill_f64(d->stream, *d->plane(field, c).d, value, d->n);
-  if (field == WF_F_M) d->surf_init = false;
+  if (field == WF_F_M) {
+    d->surf_init = false;
+    d->uniform_m = true;
+    d->m_value = value;
+  }
   if (field == WF_F_H) d->uniform_h = true;
   if (field == WF_F_X) {
     for (int c = 0; c < 3; c++) launch_fill_f64(d->stream, d->f.xs[c], value, d->n);
@@ -1225,7 +1237,8 @@ int wf_calc_surface(wf_domain* d) {
   int rc = d->surface_init();
   if (rc) return rc;
   launch_surface_tiled(d->stream, d->f, d->tl, d->solid, (double)d->n + (double)d->mesh.ne,
-                       d->totmass, d->cfg.id_free_surf, d->surface_r2cut());
+                       d->totmass, d->cfg.id_free_surf, d->surface_r2cut(), d->uniform_m,
+                       d->m_value);
   HIPC(hipGetLastError());
   return WF_OK;
 }
@@ -1353,7 +1366,8 @@ int wf_step(wf_domain* d, int n_steps) {
         timer_begin(d);
         if ((rc = d->surface_init())) return rc;
         launch_surface_tiled(d->stream, d->f, d->tl, d->solid, (double)d->n + (double)d->mesh.ne,
-                             d->totmass, d->cfg.id_free_surf, d->surface_r2cut());
+                             d->totmass, d->cfg.id_free_surf, d->surface_r2cut(), d->uniform_m,
+                             d->m_value);
         if (d->n_meshes > 0) {
           if (d->ts_i == 0 && (rc = d->contact_nb())) return rc;
           launch_contact_force(d->stream, d->f, d->solid, d->mesh, d->cl, d->contact_params());

@@ -111,13 +111,17 @@ void launch_mesh_geom(hipStream_t s, const MeshDev& m);
 void launch_mesh_move(hipStream_t s, const MeshDev& m, const MeshMotions& mo, double dt);
 // centroid bounding box (6 doubles: min xyz, max xyz) and max of h over n
 void launch_tri_bbox(hipStream_t s, const MeshDev& m, double* out);
-void launch_max_f64(hipStream_t s, const double* v, int n, double* out);
+// max of v[0..n) into *out; part holds kMaxParts doubles of block partials
+constexpr int kMaxParts = 1024;
+void launch_max_f64(hipStream_t s, const double* v, int n, double* out, double* part);
 void launch_tri_bin(hipStream_t s, const MeshDev& m, const TriGrid& g, int* cnt, int* rank,
                     int* start, int* sorted, int* scan_part);
 void launch_surface_lock(hipStream_t s, const Fields& f, int n);
-// r2cut > 0: only pairs with |x_ij|^2 < r2cut count (lists built with a skin)
+// r2cut > 0: only pairs with |x_ij|^2 < r2cut count (lists built with a skin);
+// um: every particle's mass is m_value (no neighbour-mass gather)
 void launch_surface_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
-                          double scale_num, double totmass, int id_free_surf, double r2cut);
+                          double scale_num, double totmass, int id_free_surf, double r2cut,
+                          bool um, double m_value);
 void launch_contact_nb(hipStream_t s, const Fields& f, int solid, const MeshDev& m,
                        const TriGrid& g, const int* start, const int* sorted,
                        const ContactLists& cl, int id_free_surf, Counters* ctr);

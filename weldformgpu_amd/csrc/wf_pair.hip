@@ -1,17 +1,17 @@
-// wf_pair.hip — tiled pair loops (gfx950): FP64 with LDS-staged neighbour
-// fields, FP32 (mixed precision) with LDS-staged positions and gathered
-// neighbour fields.
+// wf_pair.hip — the pair loops of the step (gfx950), FP64 and mixed
+// precision, on one kernel template (k_pair_gather).
 //
 // One workgroup per tile of kTileP Morton-consecutive particles, one thread
 // per particle.  The tile's staged set (the particles of every occupied cell
-// adjacent to the tile) streams through LDS in chunks (kChunk64) with
-// coalesced global loads; each thread walks its neighbour list, whose
-// entries are staged positions in ascending order, so each chunk consumes a
-// prefix of the remaining list.  The per-pair physics is the reference's
-// (InteractionAlt.cu, Thermal.cu), with per-particle factors (m/rho,
-// sigma/rho^2, 1/h) formed once instead of once per pair; the driver below
-// is shared, the policies (DensK, RatesK, AccelK, CondK) say what is staged
-// and what a pair adds.
+// adjacent to the tile) is written to LDS as {tile-frame position, particle
+// index} slots; each thread walks its neighbour list (16-bit staged
+// positions in ascending order, four per 8-byte word), reads the neighbour's
+// position and index from LDS and gathers the neighbour's packed record
+// from global memory.  The per-pair physics is the reference's
+// (InteractionAlt.cu, Thermal.cu, Geometry.cu), with per-particle factors
+// (m/rho, sigma/rho^2, 1/h) formed once instead of once per pair; the
+// policies (DensD/DensG, RatesD/RatesG, AccelD/AccelG, CondD/CondG, SurfD,
+// CondDens) say what a neighbour record holds and what a pair adds.
 #include <climits>
 
 #include "wf_device.hpp"
@@ -58,60 +58,8 @@ __device__ __forceinline__ int xcd_tile(int b, int nt) {
   return x * q + min(x, r) + k;
 }
 
-template <class K>
-__global__ __launch_bounds__(kTileP) void k_pair_tiled(Fields f, TileLists tl,
-                                                       typename K::Prm prm) {
-  constexpr int CH = K::CHUNK;
-  __shared__ typename K::LT lds[K::NF][CH];
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
-  const int tid = threadIdx.x;
-  const int i = t * kTileP + tid;
-  const bool act = i < prm.limit;
-  const typename K::Ctx cx = K::ctx(f, t, prm);
-  typename K::Own own;
-  typename K::Acc acc;
-  K::init(f, i, act, own, acc, cx);
-  const int ub = tl.uoff[t], ucnt = tl.uoff[t + 1] - ub;
-  SlotReader sr;
-  sr.init(tl, i, act ? tl.count[i] : 0);
-  for (int c0 = 0; c0 < ucnt; c0 += CH) {
-    const int m = min(CH, ucnt - c0);
-    __syncthreads();
-    for (int s = tid; s < m; s += kTileP) K::stage(f, tl.tj[ub + c0 + s], lds, s, cx);
-    __syncthreads();
-    const int lim = c0 + m;
-    // with K::UNROLL, two neighbours per iteration while both fall in this
-    // chunk, so the LDS reads of the second overlap the arithmetic of the first
-    while (sr.k < sr.cnt) {
-      const int s0 = sr.slot();
-      if (s0 >= lim) break;
-      sr.advance();
-      if (K::UNROLL && sr.k < sr.cnt) {
-        const int s1 = sr.slot();
-        if (s1 < lim) {
-          sr.advance();
-          K::pair(lds, s0 - c0, own, acc, prm);
-          K::pair(lds, s1 - c0, own, acc, prm);
-          continue;
-        }
-      }
-      K::pair(lds, s0 - c0, own, acc, prm);
-    }
-  }
-  K::finish(f, i, act, own, acc, prm);
-}
-
-// staged particles per LDS chunk: FP64 and FP32 policies
-constexpr int kChunk64 = 256;
-
 // FP64 policies need no per-tile context
 struct NoCtx {};
-#define WF_FP64_POLICY                                                           \
-  typedef double LT;                                                             \
-  static constexpr int CHUNK = kChunk64;                                         \
-  static constexpr bool UNROLL = false;                                          \
-  typedef NoCtx Ctx;                                                             \
-  __device__ static Ctx ctx(const Fields&, int, const Prm&) { return Ctx{}; }
 
 // ---------------------------------------------------------------------------
 // Per-pair cubic-spline gradient GK = (1/r) dW/dr (Functions.cu:76-88) from
@@ -944,38 +892,56 @@ struct CondG {
 // S: free-surface normal n_i = sum_j m_j x_ij * N/(M nb_i) and the surface
 // flag |n_i| >= 0.25 h_i with nb_i <= 46 (Geometry.cu:26-88).  Rigid
 // triangles are not particles here, so every list entry counts.
-struct SurfK {
-  static constexpr int NF = 4;  // x y z m
+#ifndef WF_STAGE_SURF
+#define WF_STAGE_SURF 1536
+#endif
+template <bool UM>
+struct SurfD {
+  // FP64 positions; its own staging size (a pair costs little here, so the
+  // chunked path for large staged sets weighs more)
+  typedef double PT;
+  static constexpr int STAGE = WF_STAGE_SURF;
+  typedef NoCtx Ctx;
+  template <class P>
+  __device__ static Ctx ctx(const Fields&, int, const P&) {
+    return Ctx{};
+  }
+  __device__ static void pos(const Fields& f, int j, const Ctx&, double& x, double& y,
+                             double& z) {
+    x = f.x[0][j];
+    y = f.x[1][j];
+    z = f.x[2][j];
+  }
+  static constexpr int NBATCH = 4;
   struct Prm {
     int limit;
     int id_free_surf;
     double ntot, totmass;
     double r2cut;  // > 0: lists carry a skin; count only pairs within the radius
+    double m_uni;  // every particle's mass (UM)
   };
-  WF_FP64_POLICY
   struct Own {
     double x, y, z;
+  };
+  struct Nb {
+    double m;
   };
   struct Acc {
     double nx, ny, nz;
     int cnt;
   };
-  __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx&) {
+  __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const NoCtx&) {
     a.nx = a.ny = a.nz = 0.0;
     a.cnt = 0;
     if (!act) return;
     o.x = f.x[0][i]; o.y = f.x[1][i]; o.z = f.x[2][i];
   }
-  __device__ static void stage(const Fields& f, int j, double (*L)[kChunk64], int s,
-                               const Ctx&) {
-    L[0][s] = f.x[0][j]; L[1][s] = f.x[1][j]; L[2][s] = f.x[2][j];
-    L[3][s] = f.m[j];
-  }
-  __device__ static void pair(double (*L)[kChunk64], int s, const Own& o, Acc& a,
+  __device__ static void load(const Fields& f, int j, Nb& nb) { nb.m = UM ? 0.0 : f.m[j]; }
+  __device__ static void pair(double x, double y, double z, const Nb& nb, const Own& o, Acc& a,
                               const Prm& p) {
-    const double dx = o.x - L[0][s], dy = o.y - L[1][s], dz = o.z - L[2][s];
+    const double dx = o.x - x, dy = o.y - y, dz = o.z - z;
     if (p.r2cut > 0.0 && !(dx * dx + dy * dy + dz * dz < p.r2cut)) return;
-    const double mj = L[3][s];
+    const double mj = UM ? p.m_uni : nb.m;
     a.nx += mj * dx;
     a.ny += mj * dy;
     a.nz += mj * dz;
@@ -997,10 +963,18 @@ struct SurfK {
 static inline int ntiles(int n) { return (n + kTileP - 1) / kTileP; }
 
 void launch_surface_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
-                          double ntot, double totmass, int id_free_surf, double r2cut) {
+                          double ntot, double totmass, int id_free_surf, double r2cut, bool um,
+                          double m_value) {
   if (solid <= 0) return;
-  SurfK::Prm p{solid, id_free_surf, ntot, totmass, r2cut};
-  hipLaunchKernelGGL(k_pair_tiled<SurfK>, dim3(ntiles(solid)), dim3(kTileP), 0, s, f, tl, p);
+  if (um) {
+    const SurfD<true>::Prm p{solid, id_free_surf, ntot, totmass, r2cut, m_value};
+    hipLaunchKernelGGL(k_pair_gather<SurfD<true>>, dim3(ntiles(solid)), dim3(kTileP), 0, s, f,
+                       tl, p);
+  } else {
+    const SurfD<false>::Prm p{solid, id_free_surf, ntot, totmass, r2cut, 0.0};
+    hipLaunchKernelGGL(k_pair_gather<SurfD<false>>, dim3(ntiles(solid)), dim3(kTileP), 0, s, f,
+                       tl, p);
+  }
 }
 
 }  // namespace wf

@@ -9,9 +9,8 @@
 //     both as global indices (int32) and as 16-bit positions in the staged
 //     set, interleaved [k*npad + i] so a wavefront reads 128 contiguous bytes
 //     per list slot.
-// The pair kernels stream the staged set through LDS in chunks of
-// kTileChunk particles with coalesced loads; every neighbour read inside the
-// pair loop is an LDS read.
+// The pair kernels write the staged set to LDS (position + index per slot)
+// and read a neighbour's position from LDS through its slot.
 #include <hip/hip_runtime.h>
 
 #include <climits>
