@@ -526,12 +526,37 @@ __global__ __launch_bounds__(kTileP) void k_pair_gather(Fields f, TileLists tl,
   const int ub = tl.uoff[t], ucnt = tl.uoff[t + 1] - ub;
   const int* __restrict__ tj = tl.tj + ub;
   const int cnt = act ? tl.count[i] : 0;
+#ifndef WF_STAGE_UNROLL
+#define WF_STAGE_UNROLL 8
+#endif
+  // staging: SU slots per thread at a time, their indices, then their
+  // positions, then the LDS writes, so that each thread keeps SU index loads
+  // and then SU position loads in flight instead of one dependent pair
   auto stage = [&](int c0, int m) {
-    for (int s = tid; s < m; s += kTileP) {
-      Slot v;
-      v.j = tj[c0 + s];
-      K::pos(f, v.j, cx, v.x, v.y, v.z);
-      st[s] = v;
+    constexpr int SU = WF_STAGE_UNROLL;
+    for (int s0 = tid; s0 < m; s0 += SU * kTileP) {
+      int jj[SU];
+#pragma unroll
+      for (int u = 0; u < SU; u++) {
+        const int s = s0 + u * kTileP;
+        jj[u] = s < m ? tj[c0 + s] : 0;
+      }
+      typename K::PT x[SU], y[SU], z[SU];
+#pragma unroll
+      for (int u = 0; u < SU; u++)
+        if (s0 + u * kTileP < m) K::pos(f, jj[u], cx, x[u], y[u], z[u]);
+#pragma unroll
+      for (int u = 0; u < SU; u++) {
+        const int s = s0 + u * kTileP;
+        if (s < m) {
+          Slot v;
+          v.x = x[u];
+          v.y = y[u];
+          v.z = z[u];
+          v.j = jj[u];
+          st[s] = v;
+        }
+      }
     }
   };
   constexpr int NB = K::NBATCH;  // neighbour records held at once (4, or 2)
@@ -542,13 +567,17 @@ __global__ __launch_bounds__(kTileP) void k_pair_gather(Fields f, TileLists tl,
     // from LDS
     stage(0, ucnt);
     __syncthreads();
-    const uint2* __restrict__ q = tl.slotq + i;
+    // lanes past the padded particle range read row 0 of the last particle
+    // (their lists are empty); the first two words are loaded
+    // unconditionally, since a select between a load and a constant turns
+    // into a load through a selected pointer with the constant in scratch
+    const uint2* __restrict__ q = tl.slotq + min((int64_t)i, tl.npad - 1);
     auto sl_of = [](const uint2 w, int u) {
       const unsigned h = (u & 2) ? w.y : w.x;
       return (int)((u & 1) ? h >> 16 : h & 0xFFFFu);
     };
-    uint2 w0 = cnt > 0 ? q[0] : make_uint2(0u, 0u);
-    uint2 w1 = cnt > 4 ? q[tl.npad] : w0;
+    uint2 w0 = q[0];
+    uint2 w1 = q[cnt > 4 ? tl.npad : 0];
     int jn[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) jn[u] = u < cnt ? st[sl_of(w0, u)].j : 0;
@@ -560,19 +589,27 @@ __global__ __launch_bounds__(kTileP) void k_pair_gather(Fields f, TileLists tl,
 #pragma unroll
       for (int u = 0; u < 4; u++)
         if (k + 4 + u < cnt) jn[u] = st[sl_of(w1, u)].j;
-      const uint2 w2 = k + 8 < cnt ? q[(int64_t)((k + 8) >> 2) * tl.npad] : w1;
+      // (past the list: the last row again, never read for a slot)
+      const uint2 w2 = q[(int64_t)(min(k + 8, cnt - 1) >> 2) * tl.npad];
 #pragma unroll
       for (int u0 = 0; u0 < 4; u0 += NB) {
         typename K::Nb nb[NB];
 #pragma unroll
         for (int u = 0; u < NB; u++)
           if (u0 + u < nk) K::load(f, jc[u0 + u], nb[u]);
+        // the batch's staged positions are read before any pair is evaluated
+        // (entries past the list read slot 0 and are not evaluated)
+        typename K::PT px[NB], py[NB], pz[NB];
+#pragma unroll
+        for (int u = 0; u < NB; u++) {
+          const Slot& sv = st[u0 + u < nk ? sl_of(w0, u0 + u) : 0];
+          px[u] = sv.x;
+          py[u] = sv.y;
+          pz[u] = sv.z;
+        }
 #pragma unroll
         for (int u = 0; u < NB; u++)
-          if (u0 + u < nk) {
-            const Slot& sv = st[sl_of(w0, u0 + u)];
-            K::pair(sv.x, sv.y, sv.z, nb[u], own, acc, prm);
-          }
+          if (u0 + u < nk) K::pair(px[u], py[u], pz[u], nb[u], own, acc, prm);
       }
       w0 = w1;
       w1 = w2;
