@@ -567,16 +567,14 @@ int wf_create(const wf_config* cfg, wf_domain** out) {
   }
   d->mixed = cfg->precision == WF_MIXED;
   if (d->mixed) {
-    float** f32[] = {&d->f.xr[0], &d->f.xr[1], &d->f.xr[2], &d->f.h32};
-    for (float** q : f32)
-      if (!rc) rc = d->alloc_plane_f(q);
+    if (!rc) rc = d->alloc_plane_f(&d->f.h32);
+    if (!rc) rc = d->dev_alloc((void**)&d->f.p4, sizeof(float4) * d->npad);
     float4** f4[] = {&d->f.v4, &d->f.v4_alt};
     for (float4** q : f4)
       if (!rc) rc = d->dev_alloc((void**)q, sizeof(float4) * d->npad);
     if (!rc) rc = d->dev_alloc((void**)&d->f.arec, 3 * sizeof(float4) * d->npad);
     if (!rc) rc = d->dev_alloc((void**)&d->f.tk4, sizeof(float4) * d->npad);
     if (!rc) rc = d->dev_alloc((void**)&d->f.tk4_alt, sizeof(float4) * d->npad);
-    if (!rc) rc = d->alloc_plane_i(&d->f.cell);
   } else {
     double4** d4[] = {&d->f.kv, &d->f.kv_alt, &d->f.tk64, &d->f.tk64_alt};
     for (double4** q : d4)

@@ -28,19 +28,20 @@ struct Fields {
   double *T, *T_alt, *Ta, *Tb, *dTdt, *k, *cp, *qpl, *qfric;
   double *sigma[6], *ss[6], *strain[6], *strrate[6], *rotrate[6];
   int *ID, *BC_T;
-  // FP32 mirrors read by the mixed-precision pair loops (WF_MIXED): position
-  // relative to the origin of an integer grid cell (cell packed as
-  // cx | cy << 11 | cz << 22) and h, per component; the neighbour fields a
+  // FP32 mirrors read by the mixed-precision pair loops (WF_MIXED): p4 =
+  // {position relative to the origin of an integer grid cell, the cell packed
+  // as cx | cy << 11 | cz << 22 in the bits of w} (one 16-byte load stages a
+  // neighbour) and h; the neighbour fields a
   // pair gathers, packed per particle: v4 = {v, m/rho} (double-buffered like
   // rho), tk4 = {T, k, m/rho, 0} (double-buffered like T; m/rho refreshed by
   // the density update), and the acceleration loop's
   // 48-byte record per particle, written by the stress pass: arec[3i] = v4,
   // arec[3i+1] = sigma/rho^2 {xx, yy, zz, xy}, arec[3i+2] = {sigma/rho^2 yz,
   // xz, rho, Cs}, so that a pair reads one record instead of three arrays.
-  float *xr[3], *h32;
+  float4* p4;
+  float* h32;
   float4 *v4, *v4_alt, *arec;
   float4 *tk4, *tk4_alt;
-  int* cell;
   // the same packed neighbour records in FP64 for the FP64 pair loops
   // (WF_FP64): kv = {v, m/rho} (double-buffered), arec64[3i..3i+2] = {v, m},
   // sigma/rho^2 {xx, yy, zz, xy}, {sigma/rho^2 yz, xz, rho, Cs}, and

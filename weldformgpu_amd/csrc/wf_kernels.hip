@@ -397,10 +397,11 @@ __global__ __launch_bounds__(kBlock) void k_integrate(Fields f, int n, double dt
   if (REC == kRec64) f.kv[i] = make_double4(v0, v1, v2, f.kv[i].w);
   if (REC == kRec32) {
     int cx, cy, cz;
-    unpack_cell(f.cell[i], cx, cy, cz);
-    f.xr[0][i] = (float)(f.x[0][i] - (fr.org[0] + cx * fr.cs));
-    f.xr[1][i] = (float)(f.x[1][i] - (fr.org[1] + cy * fr.cs));
-    f.xr[2][i] = (float)(f.x[2][i] - (fr.org[2] + cz * fr.cs));
+    const float cb = f.p4[i].w;  // the cell kept since the last refresh
+    unpack_cell(__float_as_int(cb), cx, cy, cz);
+    f.p4[i] = make_float4((float)(f.x[0][i] - (fr.org[0] + cx * fr.cs)),
+                          (float)(f.x[1][i] - (fr.org[1] + cy * fr.cs)),
+                          (float)(f.x[2][i] - (fr.org[2] + cz * fr.cs)), cb);
     const float mdj = f.v4[i].w;
     f.v4[i] = make_float4((float)v0, (float)v1, (float)v2, mdj);
   }
@@ -412,13 +413,14 @@ __global__ __launch_bounds__(kBlock) void k_mirror32(Fields f, int n, Frame32 fr
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int c[3];
+  float xr[3];
   for (int a = 0; a < 3; a++) {
     double u = floor((f.x[a][i] - fr.org[a]) / fr.cs);
     int lim = a < 2 ? 2047 : 1023;
     c[a] = (int)fmin(fmax(u, 0.0), (double)lim);
-    f.xr[a][i] = (float)(f.x[a][i] - (fr.org[a] + c[a] * fr.cs));
+    xr[a] = (float)(f.x[a][i] - (fr.org[a] + c[a] * fr.cs));
   }
-  f.cell[i] = pack_cell(c[0], c[1], c[2]);
+  f.p4[i] = make_float4(xr[0], xr[1], xr[2], __int_as_float(pack_cell(c[0], c[1], c[2])));
   const double rho = f.rho[i];
   f.h32[i] = (float)f.h[i];
   const float4 v4 = make_float4((float)f.v[0][i], (float)f.v[1][i], (float)f.v[2][i],

@@ -461,17 +461,18 @@ struct Ctx32 {
 };
 __device__ __forceinline__ Ctx32 ctx32(const Fields& f, int t, float cs) {
   int bx, by, bz;
-  unpack_cell(f.cell[t * kTileP], bx, by, bz);
+  unpack_cell(__float_as_int(f.p4[t * kTileP].w), bx, by, bz);
   return Ctx32{bx, by, bz, cs};
 }
 // position of particle j in the tile frame
 __device__ __forceinline__ void pos32(const Fields& f, int j, const Ctx32& c, float& x, float& y,
                                       float& z) {
   int cx, cy, cz;
-  unpack_cell(f.cell[j], cx, cy, cz);
-  x = f.xr[0][j] + (float)(cx - c.bx) * c.cs;
-  y = f.xr[1][j] + (float)(cy - c.by) * c.cs;
-  z = f.xr[2][j] + (float)(cz - c.bz) * c.cs;
+  const float4 p = f.p4[j];
+  unpack_cell(__float_as_int(p.w), cx, cy, cz);
+  x = p.x + (float)(cx - c.bx) * c.cs;
+  y = p.y + (float)(cy - c.by) * c.cs;
+  z = p.z + (float)(cz - c.bz) * c.cs;
 }
 
 // staged slots per LDS pass: 1536 x 16 B (tile-frame position + index) =
