@@ -394,12 +394,10 @@ int wf_domain::mlp_train_round(int iters, float lr) {
 // Adam iterations on device-resident samples X[s*stride], Y[s*stride]
 int wf_domain::mlp_fit(const float* X, const float* Y, int nsamp, int stride, int iters,
                        float lr) {
-  const int nblocks = std::max(1, std::min(64, (nsamp + 255) / 256));
-  for (int it = 0; it < iters; it++) {
-    launch_mlp_grad(stream, mlp_w, X, Y, nsamp, stride, mlp_part, mlp_loss_part, nblocks,
-                    mlp_grad, mlp_loss);
-    launch_mlp_adam(stream, mlp_w, mlp_m, mlp_v, mlp_grad, ++adam_t, lr);
-  }
+  const int nblocks = mlp_grad_blocks(nsamp);
+  for (int it = 0; it < iters; it++)
+    launch_mlp_train_iter(stream, mlp_w, mlp_m, mlp_v, X, Y, nsamp, stride, mlp_part,
+                          mlp_loss_part, nblocks, ++adam_t, lr, mlp_loss);
   HIPC(hipMemcpyAsync(host_f, mlp_loss, sizeof(float), hipMemcpyDeviceToHost, stream));
   HIPC(hipStreamSynchronize(stream));
   mlp_last_loss = host_f[0];
@@ -594,8 +592,8 @@ int wf_create(const wf_config* cfg, wf_domain** out) {
     if (!rc) rc = d->dev_alloc((void**)&d->mlp_m, P);
     if (!rc) rc = d->dev_alloc((void**)&d->mlp_v, P);
     if (!rc) rc = d->dev_alloc((void**)&d->mlp_grad, P);
-    if (!rc) rc = d->dev_alloc((void**)&d->mlp_part, P * 64);
-    if (!rc) rc = d->dev_alloc((void**)&d->mlp_loss_part, sizeof(float) * 64);
+    if (!rc) rc = d->dev_alloc((void**)&d->mlp_part, P * kMlpMaxBlocks);
+    if (!rc) rc = d->dev_alloc((void**)&d->mlp_loss_part, sizeof(float) * kMlpMaxBlocks);
     if (!rc) rc = d->dev_alloc((void**)&d->mlp_loss, sizeof(float) * 4);
     d->mlp_cap = std::max<int64_t>(d->npad, 65536);
     if (!rc) rc = d->dev_alloc((void**)&d->mlp_X, sizeof(float) * 4 * d->mlp_cap);
@@ -1512,7 +1510,7 @@ int wf_mlp_grad(wf_domain* d, const float* X, const float* Y, int n, float* grad
   if (!d || !X || !Y || !grad) return fail(WF_E_ARG, "null argument");
   int rc = upload_samples(d, X, Y, n);
   if (rc) return rc;
-  const int nblocks = std::max(1, std::min(64, (n + 255) / 256));
+  const int nblocks = mlp_grad_blocks(n);
   launch_mlp_grad(d->stream, d->mlp_w, d->mlp_X, d->mlp_Y, n, 1, d->mlp_part, d->mlp_loss_part,
                   nblocks, d->mlp_grad, d->mlp_loss);
   HIPC(hipGetLastError());

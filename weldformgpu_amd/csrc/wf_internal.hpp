@@ -148,6 +148,7 @@ struct BlockStats {
 };
 
 constexpr int kMlpParams = 1249;  // 4x32 + 32 + 32x32 + 32 + 32 + 1
+constexpr int kMlpMaxBlocks = 256;  // gradient blocks (one 64-sample chunk each)
 constexpr int kMlpOffW1 = 0, kMlpOffB1 = 128, kMlpOffW2 = 160, kMlpOffB2 = 1184,
               kMlpOffW3 = 1216, kMlpOffB3 = 1248;
 
@@ -256,8 +257,13 @@ void launch_mlp_infer(hipStream_t s, const float* w, const float* X, int n, floa
 void launch_mlp_grad(hipStream_t s, const float* w, const float* X, const float* Y, int n,
                      int stride, float* part, float* loss_part, int nblocks, float* grad,
                      float* loss);
-void launch_mlp_adam(hipStream_t s, float* w, float* m, float* v, const float* grad, int t,
-                     float lr);
+// one Adam iteration: gradient block partials over samples s*stride, s < n,
+// then their sum in block order and the Adam step t (both kernels)
+void launch_mlp_train_iter(hipStream_t s, float* w, float* m, float* v, const float* X,
+                           const float* Y, int n, int stride, float* part, float* loss_part,
+                           int nblocks, int t, float lr, float* loss);
+// gradient blocks for n samples: one 64-sample chunk each, at most kMlpMaxBlocks
+int mlp_grad_blocks(int n);
 // gather the logged samples (ascending particle index) into X (n x 4), Y
 void launch_mlp_collect(hipStream_t s, const Fields& f, int n, float* X, float* Y, int* flags,
                         int* scan, int* scan_part, int* count);

@@ -147,13 +147,32 @@ __global__ void k_mlp_reduce(const float* __restrict__ part, const float* __rest
   }
 }
 
-// Adam, Kingma & Ba 2015, Algorithm 1 (beta1 0.9, beta2 0.999, eps 1e-8)
-__global__ void k_mlp_adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
-                           const float* __restrict__ grad, int t, float lr) {
-  const int pidx = blockIdx.x * blockDim.x + threadIdx.x;
+// one training iteration's second half: the block partials summed in block
+// order (as k_mlp_reduce) and the Adam update of the same parameter (Kingma &
+// Ba 2015, Algorithm 1: beta1 0.9, beta2 0.999, eps 1e-8)
+__global__ __launch_bounds__(256) void k_mlp_reduce_adam(const float* __restrict__ part,
+                                                         const float* __restrict__ loss_part,
+                                                         int nblocks, float* __restrict__ w,
+                                                         float* __restrict__ m,
+                                                         float* __restrict__ v, int t, float lr,
+                                                         float* __restrict__ loss) {
+  // one wavefront per parameter (four per block); lane l sums partials
+  // l, l+64, ... in order, then a fixed butterfly adds the lanes, so the
+  // sum does not depend on timing
+  const int pidx = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (pidx == 0) {  // the loss, the same way
+    float l = 0.f;
+    for (int b = lane; b < nblocks; b += 64) l += loss_part[b];
+    for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
+    if (lane == 0) *loss = l;
+  }
   if (pidx >= kMlpParams) return;
+  float g = 0.f;
+  for (int b = lane; b < nblocks; b += 64) g += part[(int64_t)b * kMlpParams + pidx];
+  for (int o = 32; o > 0; o >>= 1) g += __shfl_xor(g, o);
+  if (lane != 0) return;
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-  const float g = grad[pidx];
   const float mn = b1 * m[pidx] + (1.f - b1) * g;
   const float vn = b2 * v[pidx] + (1.f - b2) * g * g;
   m[pidx] = mn;
@@ -162,6 +181,7 @@ __global__ void k_mlp_adam(float* __restrict__ w, float* __restrict__ m, float* 
   const float vhat = vn / (1.f - powf(b2, (float)t));
   w[pidx] -= lr * mhat / (sqrtf(vhat) + eps);
 }
+
 
 // sample collection: flags, exclusive scan (caller), scatter
 __global__ void k_mlp_flags(const int* __restrict__ mlstep, int n, int* __restrict__ flags) {
@@ -193,11 +213,17 @@ void launch_mlp_grad(hipStream_t s, const float* w, const float* X, const float*
                      loss_part, nblocks, grad, loss);
 }
 
-void launch_mlp_adam(hipStream_t s, float* w, float* m, float* v, const float* grad, int t,
-                     float lr) {
-  hipLaunchKernelGGL(k_mlp_adam, dim3((kMlpParams + 255) / 256), dim3(256), 0, s, w, m, v, grad,
-                     t, lr);
+void launch_mlp_train_iter(hipStream_t s, float* w, float* m, float* v, const float* X,
+                           const float* Y, int n, int stride, float* part, float* loss_part,
+                           int nblocks, int t, float lr, float* loss) {
+  const int per_block = (n + nblocks - 1) / nblocks;
+  hipLaunchKernelGGL(k_mlp_grad, dim3(nblocks), dim3(kGradThreads), 0, s, w, X, Y, n, stride,
+                     per_block, part, loss_part);
+  hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((kMlpParams + 3) / 4), dim3(256), 0, s, part,
+                     loss_part, nblocks, w, m, v, t, lr, loss);
 }
+int mlp_grad_blocks(int n) { return std::max(1, std::min(kMlpMaxBlocks, (n + kGradChunk - 1) / kGradChunk)); }
+
 
 void launch_mlp_collect(hipStream_t s, const Fields& f, int n, float* X, float* Y, int* flags,
                         int* scan, int* scan_part, int* count) {
