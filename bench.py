@@ -83,7 +83,7 @@ def profiled_traffic(workload, kernel_id):
 
 
 def workload_text(sc, n):
-    common = ("mixed precision (FP32 compensated pair loops, FP64 stress/Newton return "
+    common = ("mixed precision (FP32 pair loops summed in FP64, FP64 stress/Newton return "
               "map/integration) with the online-trained MLP Newton guess (BF16 MFMA), "
               "conduction + Taylor-Quinney 0.9")
     if sc.meshes:

@@ -1,5 +1,5 @@
-"""Mixed precision (FP32 pair loops with compensated sums on cell-relative
-FP32 coordinates; FP64 stress, return map, integration) against the FP64
+"""Mixed precision (FP32 pair loops on cell-relative FP32 coordinates, each
+FP32 term summed in FP64; FP64 stress, return map, integration) against the FP64
 oracle.
 
 Stated tolerances:

@@ -216,7 +216,7 @@ void launch_accel_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int
 void launch_cond_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int n, double T_inf,
                        double h_conv, double dt, bool leap, bool first, bool uh);
 
-// mixed precision (FP32 pair loops, compensated sums)
+// mixed precision (FP32 pair loops, FP32 terms summed in FP64)
 // uh: every particle has the same h
 void launch_dens_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int n, int solid,
                        double dt, bool fuse, float cs32, bool uh);

@@ -407,7 +407,7 @@ struct CondD {
 // ===========================================================================
 // Mixed precision (WF_MIXED): the same pair physics in FP32, on FP32 mirrors,
 // with coordinates relative to the tile's base cell (x = xr + (c - c_tile) cs,
-// c an integer grid cell), compensated (Kahan) FP32 sums and FP64 results.
+// c an integer grid cell), FP32 pair terms summed in FP64, and FP64 results.
 // Per-particle terms stay FP64.
 //
 // The FP32 loops hold only the staged set's tile-frame positions in LDS
