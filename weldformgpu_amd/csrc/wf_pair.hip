@@ -2,12 +2,16 @@
 // precision, on one kernel template (k_pair_gather).
 //
 // One workgroup per tile of kTileP Morton-consecutive particles, one thread
-// per particle.  The tile's staged set (the particles of every occupied cell
-// adjacent to the tile) is written to LDS as {tile-frame position, particle
-// index} slots; each thread walks its neighbour list (16-bit staged
-// positions in ascending order, four per 8-byte word), reads the neighbour's
-// position and index from LDS and gathers the neighbour's packed record
-// from global memory.  The per-pair physics is the reference's
+// per particle in the FP64 loops and two in the FP32 loops.  The tile's
+// staged set (the particles of every occupied cell adjacent to the tile) is
+// written to LDS as {tile-frame position, particle index} slots; each thread
+// walks its neighbour list (16-bit staged positions in ascending order, four
+// per 8-byte word; with two threads per particle each takes every other
+// word), reads the neighbour's position and index from LDS and gathers the
+// neighbour's packed record from global memory.  The LDS set (24 KB) limits
+// a 128-thread workgroup to 3 wavefronts per SIMD; two threads per particle
+// double that for the FP32 loops, whose register use allows 4-6, while the
+// FP64 loops are limited by their registers to 3-4 either way.  The per-pair physics is the reference's
 // (InteractionAlt.cu, Thermal.cu, Geometry.cu), with per-particle factors
 // (m/rho, sigma/rho^2, 1/h) formed once instead of once per pair; the
 // policies (DensD/DensG, RatesD/RatesG, AccelD/AccelG, CondD/CondG, SurfD,
@@ -45,6 +49,12 @@ struct SlotReader {
       cur = nxt;
       if (k + 4 < cnt) nxt = q[(int64_t)((k >> 2) + 1) * npad + i];
     }
+  }
+  // to the start of the next quad
+  __device__ __forceinline__ void skip_quad() {
+    k = (k | 3) + 1;
+    cur = nxt;
+    if (k + 4 < cnt) nxt = q[(int64_t)((k >> 2) + 1) * npad + i];
   }
 };
 
@@ -108,9 +118,19 @@ __device__ __forceinline__ double gk_pair(double r2, double hj, const HConst& hc
 #define WF_STAGE64 768
 #endif
 constexpr int kStage64 = WF_STAGE64;
+// threads per particle in the pair loops (each takes every LANES-th slot
+// quad); on a 759k-particle cylinder two threads take the FP32 step from
+// 0.672 to 0.591 ms, four are slower (0.678 ms), and the FP64 loops gain nothing
+#ifndef WF_LANES64
+#define WF_LANES64 1
+#endif
+#ifndef WF_LANES32
+#define WF_LANES32 2
+#endif
 #define WF_GATHER64                                                                  \
   typedef double PT;                                                                 \
   static constexpr int STAGE = kStage64;                                             \
+  static constexpr int LANES = WF_LANES64;                                           \
   typedef NoCtx Ctx;                                                                 \
   template <class P>                                                                 \
   __device__ static Ctx ctx(const Fields&, int, const P&) {                          \
@@ -153,6 +173,9 @@ struct DensD {
   struct Acc {
     double s;
   };
+  __device__ static void merge(Acc& a, const Acc& b) {
+    a.s += b.s;
+  }
   __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx&) {
     a.s = 0.0;
     if (!act) return;
@@ -212,6 +235,11 @@ struct RatesD {
   struct Acc {
     double dxx, dyy, dzz, dxy, dyz, dxz, wxy, wyz, wxz;
   };
+  __device__ static void merge(Acc& a, const Acc& b) {
+    a.dxx += b.dxx; a.dyy += b.dyy; a.dzz += b.dzz;
+    a.dxy += b.dxy; a.dyz += b.dyz; a.dxz += b.dxz;
+    a.wxy += b.wxy; a.wyz += b.wyz; a.wxz += b.wxz;
+  }
   __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx&) {
     a = Acc{0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (!act) return;
@@ -273,6 +301,9 @@ struct AccelD {
   struct Acc {
     double ax, ay, az;
   };
+  __device__ static void merge(Acc& a, const Acc& b) {
+    a.ax += b.ax; a.ay += b.ay; a.az += b.az;
+  }
   __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx&) {
     a = Acc{0, 0, 0};
     if (!act) return;
@@ -346,6 +377,9 @@ struct CondD {
   struct Acc {
     double q;
   };
+  __device__ static void merge(Acc& a, const Acc& b) {
+    a.q += b.q;
+  }
   __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx&) {
     a.q = 0.0;
     if (!act) return;
@@ -426,6 +460,7 @@ struct KSum {
   double s;
   __device__ __forceinline__ void zero() { s = 0.0; }
   __device__ __forceinline__ void add(float y) { s += (double)y; }
+  __device__ __forceinline__ void add(const KSum& o) { s += o.s; }
   __device__ __forceinline__ double val() const { return s; }
 };
 
@@ -484,6 +519,7 @@ constexpr int kStage32 = WF_STAGE32;
 #define WF_GATHER32                                                                \
   typedef float PT;                                                                \
   static constexpr int STAGE = kStage32;                                           \
+  static constexpr int LANES = WF_LANES32;                                         \
   typedef Ctx32 Ctx;                                                               \
   template <class P>                                                               \
   __device__ static Ctx ctx(const Fields& f, int t, const P& p) {                  \
@@ -510,15 +546,37 @@ struct alignas(16) StagedSlot<double> {
   int j, pad;
 };
 
+// the value v holds in lane (this lane ^ m)
+template <class T>
+__device__ __forceinline__ T lane_value_xor(const T& v, int m) {
+  static_assert(sizeof(T) % 4 == 0, "whole 32-bit words");
+  constexpr int W = sizeof(T) / 4;
+  int in[W], out[W];
+  __builtin_memcpy(in, &v, sizeof(T));
+#pragma unroll
+  for (int w = 0; w < W; w++) out[w] = __shfl_xor(in[w], m);
+  T r;
+  __builtin_memcpy(&r, out, sizeof(T));
+  return r;
+}
+
+// One workgroup per tile, L = K::LANES consecutive lanes per particle: lanes
+// L p .. L p + L - 1 all hold particle p and deal its slot list out by quads
+// (lane L p + h takes the quads q with q % L == h); their sums are merged
+// (butterfly over the L lanes) before lane L p writes the result.
 template <class K>
-__global__ __launch_bounds__(kTileP) void k_pair_gather(Fields f, TileLists tl,
-                                                        typename K::Prm prm) {
+__global__ __launch_bounds__(4 * kTileP) void k_pair_gather(Fields f, TileLists tl,
+                                                            typename K::Prm prm) {
   constexpr int ST = K::STAGE;
+  constexpr int L = K::LANES;
+  static_assert(L == 1 || L == 2 || L == 4, "1, 2 or 4 lanes per particle");
+  constexpr int NT = L * kTileP;  // threads per tile
   typedef StagedSlot<typename K::PT> Slot;
   __shared__ Slot st[ST];
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
-  const int i = t * kTileP + tid;
+  const int sub = tid & (L - 1);
+  const int i = t * kTileP + tid / L;
   const bool act = i < prm.limit;
   const typename K::Ctx cx = K::ctx(f, t, prm);
   typename K::Own own;
@@ -535,20 +593,20 @@ __global__ __launch_bounds__(kTileP) void k_pair_gather(Fields f, TileLists tl,
   // and then SU position loads in flight instead of one dependent pair
   auto stage = [&](int c0, int m) {
     constexpr int SU = WF_STAGE_UNROLL;
-    for (int s0 = tid; s0 < m; s0 += SU * kTileP) {
+    for (int s0 = tid; s0 < m; s0 += SU * NT) {
       int jj[SU];
 #pragma unroll
       for (int u = 0; u < SU; u++) {
-        const int s = s0 + u * kTileP;
+        const int s = s0 + u * NT;
         jj[u] = s < m ? tj[c0 + s] : 0;
       }
       typename K::PT x[SU], y[SU], z[SU];
 #pragma unroll
       for (int u = 0; u < SU; u++)
-        if (s0 + u * kTileP < m) K::pos(f, jj[u], cx, x[u], y[u], z[u]);
+        if (s0 + u * NT < m) K::pos(f, jj[u], cx, x[u], y[u], z[u]);
 #pragma unroll
       for (int u = 0; u < SU; u++) {
-        const int s = s0 + u * kTileP;
+        const int s = s0 + u * NT;
         if (s < m) {
           Slot v;
           v.x = x[u];
@@ -563,35 +621,39 @@ __global__ __launch_bounds__(kTileP) void k_pair_gather(Fields f, TileLists tl,
   constexpr int NB = K::NBATCH;  // neighbour records held at once (4, or 2)
   if (ucnt <= ST) {
     // the whole staged set fits: one pass, a quad of four slots at a time;
-    // the slot word two quads ahead is fetched while the current quad's
-    // records are in flight, and the next quad's particle indices come
-    // from LDS
+    // the lane's slot word two quads ahead is fetched while the current
+    // quad's records are in flight, and the next quad's particle indices
+    // come from LDS
     stage(0, ucnt);
     __syncthreads();
     // lanes past the padded particle range read row 0 of the last particle
-    // (their lists are empty); the first two words are loaded
-    // unconditionally, since a select between a load and a constant turns
-    // into a load through a selected pointer with the constant in scratch
+    // (their lists are empty); every slot word is loaded unconditionally
+    // from a row clamped into the list, since a select between a load and a
+    // constant turns into a load through a selected pointer with the
+    // constant in scratch
     const uint2* __restrict__ q = tl.slotq + min((int64_t)i, tl.npad - 1);
+    const int klast = max(cnt - 1, 0);
+    auto word = [&](int k) { return q[(int64_t)(min(k, klast) >> 2) * tl.npad]; };
     auto sl_of = [](const uint2 w, int u) {
       const unsigned h = (u & 2) ? w.y : w.x;
       return (int)((u & 1) ? h >> 16 : h & 0xFFFFu);
     };
-    uint2 w0 = q[0];
-    uint2 w1 = q[cnt > 4 ? tl.npad : 0];
+    constexpr int KS = 4 * L;  // slots from one of the lane's quads to its next
+    const int k0 = 4 * sub;
+    uint2 w0 = word(k0);
+    uint2 w1 = word(k0 + KS);
     int jn[4];
 #pragma unroll
-    for (int u = 0; u < 4; u++) jn[u] = u < cnt ? st[sl_of(w0, u)].j : 0;
-    for (int k = 0; k < cnt; k += 4) {
+    for (int u = 0; u < 4; u++) jn[u] = k0 + u < cnt ? st[sl_of(w0, u)].j : 0;
+    for (int k = k0; k < cnt; k += KS) {
       const int nk = min(4, cnt - k);
       int jc[4];
 #pragma unroll
       for (int u = 0; u < 4; u++) jc[u] = jn[u];
 #pragma unroll
       for (int u = 0; u < 4; u++)
-        if (k + 4 + u < cnt) jn[u] = st[sl_of(w1, u)].j;
-      // (past the list: the last row again, never read for a slot)
-      const uint2 w2 = q[(int64_t)(min(k + 8, cnt - 1) >> 2) * tl.npad];
+        if (k + KS + u < cnt) jn[u] = st[sl_of(w1, u)].j;
+      const uint2 w2 = word(k + 2 * KS);
 #pragma unroll
       for (int u0 = 0; u0 < 4; u0 += NB) {
         typename K::Nb nb[NB];
@@ -615,42 +677,54 @@ __global__ __launch_bounds__(kTileP) void k_pair_gather(Fields f, TileLists tl,
       w0 = w1;
       w1 = w2;
     }
-    K::finish(f, i, act, own, acc, prm);
-    return;
-  }
-  SlotReader sr;
-  sr.init(tl, i, cnt);
-  for (int c0 = 0; c0 < ucnt; c0 += ST) {
-    const int m = min(ST, ucnt - c0);
-    __syncthreads();
-    stage(c0, m);
-    __syncthreads();
-    const int lim = c0 + m;
-    // two neighbours per iteration while both fall in this pass: their
-    // gathers are issued before either pair is evaluated
-    while (sr.k < sr.cnt) {
-      const int s0 = sr.slot();
-      if (s0 >= lim) break;
-      sr.advance();
-      const Slot& a0 = st[s0 - c0];
-      typename K::Nb a;
-      K::load(f, a0.j, a);
-      if (sr.k < sr.cnt) {
-        const int s1 = sr.slot();
-        if (s1 < lim) {
-          sr.advance();
-          const Slot& b0 = st[s1 - c0];
-          typename K::Nb b;
-          K::load(f, b0.j, b);
-          K::pair(a0.x, a0.y, a0.z, a, own, acc, prm);
-          K::pair(b0.x, b0.y, b0.z, b, own, acc, prm);
+  } else {
+    SlotReader sr;
+    sr.init(tl, i, cnt);
+    // this lane's slots: those of the quads q with q % L == sub
+    auto mine = [&]() { return ((sr.k >> 2) & (L - 1)) == sub; };
+    for (int c0 = 0; c0 < ucnt; c0 += ST) {
+      const int m = min(ST, ucnt - c0);
+      __syncthreads();
+      stage(c0, m);
+      __syncthreads();
+      const int lim = c0 + m;
+      // two neighbours per iteration while both fall in this pass: their
+      // gathers are issued before either pair is evaluated
+      while (sr.k < sr.cnt) {
+        if (!mine()) {
+          sr.skip_quad();
           continue;
         }
+        const int s0 = sr.slot();
+        if (s0 >= lim) break;
+        sr.advance();
+        const Slot& a0 = st[s0 - c0];
+        typename K::Nb a;
+        K::load(f, a0.j, a);
+        if (sr.k < sr.cnt && mine()) {
+          const int s1 = sr.slot();
+          if (s1 < lim) {
+            sr.advance();
+            const Slot& b0 = st[s1 - c0];
+            typename K::Nb b;
+            K::load(f, b0.j, b);
+            K::pair(a0.x, a0.y, a0.z, a, own, acc, prm);
+            K::pair(b0.x, b0.y, b0.z, b, own, acc, prm);
+            continue;
+          }
+        }
+        K::pair(a0.x, a0.y, a0.z, a, own, acc, prm);
       }
-      K::pair(a0.x, a0.y, a0.z, a, own, acc, prm);
     }
   }
-  K::finish(f, i, act, own, acc, prm);
+  if constexpr (L > 1) {
+#pragma unroll
+    for (int m = 1; m < L; m <<= 1) {
+      const typename K::Acc other = lane_value_xor(acc, m);
+      K::merge(acc, other);
+    }
+  }
+  if (sub == 0) K::finish(f, i, act, own, acc, prm);
 }
 
 // A: density rate (InteractionAlt.cu:166-246), fused density update writes
@@ -675,6 +749,9 @@ struct DensG {
     float h;
   };
   typedef KSum Acc;
+  __device__ static void merge(Acc& a, const Acc& b) {
+    a.add(b);
+  }
   __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx32& c) {
     a.zero();
     if (!act) return;
@@ -732,6 +809,11 @@ struct RatesG {
   struct Acc {
     KSum dxx, dyy, dzz, dxy, dyz, dxz, wxy, wyz, wxz;
   };
+  __device__ static void merge(Acc& a, const Acc& b) {
+    a.dxx.add(b.dxx); a.dyy.add(b.dyy); a.dzz.add(b.dzz);
+    a.dxy.add(b.dxy); a.dyz.add(b.dyz); a.dxz.add(b.dxz);
+    a.wxy.add(b.wxy); a.wyz.add(b.wyz); a.wxz.add(b.wxz);
+  }
   __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx32& c) {
     a.dxx.zero(); a.dyy.zero(); a.dzz.zero(); a.dxy.zero(); a.dyz.zero(); a.dxz.zero();
     a.wxy.zero(); a.wyz.zero(); a.wxz.zero();
@@ -795,6 +877,9 @@ struct AccelG {
   struct Acc {
     KSum ax, ay, az;
   };
+  __device__ static void merge(Acc& a, const Acc& b) {
+    a.ax.add(b.ax); a.ay.add(b.ay); a.az.add(b.az);
+  }
   __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx32& c) {
     a.ax.zero(); a.ay.zero(); a.az.zero();
     if (!act) return;
@@ -866,6 +951,9 @@ struct CondG {
     float h;
   };
   typedef KSum Acc;
+  __device__ static void merge(Acc& a, const Acc& b) {
+    a.add(b);
+  }
   __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx32& c) {
     a.zero();
     if (!act) return;
@@ -939,6 +1027,7 @@ struct SurfD {
   // chunked path for large staged sets weighs more)
   typedef double PT;
   static constexpr int STAGE = WF_STAGE_SURF;
+  static constexpr int LANES = 1;
   typedef NoCtx Ctx;
   template <class P>
   __device__ static Ctx ctx(const Fields&, int, const P&) {
@@ -1006,11 +1095,11 @@ void launch_surface_tiled(hipStream_t s, const Fields& f, const TileLists& tl, i
   if (solid <= 0) return;
   if (um) {
     const SurfD<true>::Prm p{solid, id_free_surf, ntot, totmass, r2cut, m_value};
-    hipLaunchKernelGGL(k_pair_gather<SurfD<true>>, dim3(ntiles(solid)), dim3(kTileP), 0, s, f,
+    hipLaunchKernelGGL(k_pair_gather<SurfD<true>>, dim3(ntiles(solid)), dim3(kTileP * SurfD<true>::LANES), 0, s, f,
                        tl, p);
   } else {
     const SurfD<false>::Prm p{solid, id_free_surf, ntot, totmass, r2cut, 0.0};
-    hipLaunchKernelGGL(k_pair_gather<SurfD<false>>, dim3(ntiles(solid)), dim3(kTileP), 0, s, f,
+    hipLaunchKernelGGL(k_pair_gather<SurfD<false>>, dim3(ntiles(solid)), dim3(kTileP * SurfD<false>::LANES), 0, s, f,
                        tl, p);
   }
 }
@@ -1027,6 +1116,8 @@ template <class A, class B>
 struct CondDens {
   typedef typename A::PT PT;
   static constexpr int STAGE = A::STAGE;
+  static constexpr int LANES = A::LANES;
+  static_assert(A::LANES == B::LANES, "fused passes share the lane split");
   static constexpr int NBATCH = A::NBATCH < B::NBATCH ? A::NBATCH : B::NBATCH;
   typedef typename A::Ctx Ctx;
   struct Prm {
@@ -1066,6 +1157,10 @@ struct CondDens {
     A::pair(x, y, z, nb.a, o.a, a.a, p.a);
     B::pair(x, y, z, nb.b, o.b, a.b, p.b);
   }
+  __device__ static void merge(Acc& a, const Acc& b) {
+    A::merge(a.a, b.a);
+    B::merge(a.b, b.b);
+  }
   __device__ static void finish(const Fields& f, int i, bool act, const Own& o, const Acc& a,
                                 const Prm& p) {
     A::finish(f, i, act, o.a, a.a, p.a);
@@ -1083,10 +1178,10 @@ static void launch_gather(hipStream_t s, const Fields& f, const TileLists& tl, i
                           A... args) {
   if (uh) {
     const typename POL<true>::Prm p{args...};
-    hipLaunchKernelGGL(k_pair_gather<POL<true>>, dim3(grid), dim3(kTileP), 0, s, f, tl, p);
+    hipLaunchKernelGGL(k_pair_gather<POL<true>>, dim3(grid), dim3(kTileP * POL<true>::LANES), 0, s, f, tl, p);
   } else {
     const typename POL<false>::Prm p{args...};
-    hipLaunchKernelGGL(k_pair_gather<POL<false>>, dim3(grid), dim3(kTileP), 0, s, f, tl, p);
+    hipLaunchKernelGGL(k_pair_gather<POL<false>>, dim3(grid), dim3(kTileP * POL<false>::LANES), 0, s, f, tl, p);
   }
 }
 
@@ -1142,7 +1237,7 @@ void launch_cond_dens_tiled(hipStream_t s, const Fields& f, const TileLists& tl,
     typedef decltype(tag) K;
     const typename K::Prm p{n, {n, T_inf, h_conv, dt, leap ? 1 : 0, first ? 1 : 0},
                             {solid, n, dt, 1, 1}};
-    hipLaunchKernelGGL(k_pair_gather<K>, dim3(ntiles(n)), dim3(kTileP), 0, s, f, tl, p);
+    hipLaunchKernelGGL(k_pair_gather<K>, dim3(ntiles(n)), dim3(kTileP * K::LANES), 0, s, f, tl, p);
   };
   if (uh) go(CondDensD<true>{});
   else go(CondDensD<false>{});
@@ -1155,7 +1250,7 @@ void launch_cond_dens_mixed(hipStream_t s, const Fields& f, const TileLists& tl,
     typedef decltype(tag) K;
     const typename K::Prm p{n, {n, T_inf, h_conv, dt, leap ? 1 : 0, first ? 1 : 0, cs32},
                             {solid, n, dt, 1, cs32, 1}};
-    hipLaunchKernelGGL(k_pair_gather<K>, dim3(ntiles(n)), dim3(kTileP), 0, s, f, tl, p);
+    hipLaunchKernelGGL(k_pair_gather<K>, dim3(ntiles(n)), dim3(kTileP * K::LANES), 0, s, f, tl, p);
   };
   if (uh) go(CondDensG<true>{});
   else go(CondDensG<false>{});
