@@ -1027,7 +1027,10 @@ struct SurfD {
   // chunked path for large staged sets weighs more)
   typedef double PT;
   static constexpr int STAGE = WF_STAGE_SURF;
-  static constexpr int LANES = 1;
+#ifndef WF_LANES_SURF
+#define WF_LANES_SURF 2
+#endif
+  static constexpr int LANES = WF_LANES_SURF;
   typedef NoCtx Ctx;
   template <class P>
   __device__ static Ctx ctx(const Fields&, int, const P&) {
@@ -1057,6 +1060,10 @@ struct SurfD {
     double nx, ny, nz;
     int cnt;
   };
+  __device__ static void merge(Acc& a, const Acc& b) {
+    a.nx += b.nx; a.ny += b.ny; a.nz += b.nz;
+    a.cnt += b.cnt;
+  }
   __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const NoCtx&) {
     a.nx = a.ny = a.nz = 0.0;
     a.cnt = 0;
