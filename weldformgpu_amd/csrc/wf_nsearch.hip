@@ -48,6 +48,7 @@ struct NsWork {
   int* tile_nc = nullptr;
   int* tile_ucnt = nullptr;
   int* tile_uoff = nullptr;
+  int* tile_uc = nullptr;  // compacted staged-set sizes
   int* tile_scan = nullptr;
   int* tile_ovf = nullptr;
   int* tile_host = nullptr;  // pinned: [total staged, overflow]
@@ -265,6 +266,7 @@ static int ensure(NsWork* w, int n, int64_t nkeys) {
     hipFree(w->tile_nc);
     hipFree(w->tile_ucnt);
     hipFree(w->tile_uoff);
+    hipFree(w->tile_uc);
     hipFree(w->tile_scan);
     hipFree(w->slot);
     int cap = std::max(nt, 1);
@@ -274,6 +276,7 @@ static int ensure(NsWork* w, int n, int64_t nkeys) {
     if (hipMalloc(&w->tile_nc, sizeof(int) * cap) != hipSuccess) return WF_E_NOMEM;
     if (hipMalloc(&w->tile_ucnt, sizeof(int) * cap) != hipSuccess) return WF_E_NOMEM;
     if (hipMalloc(&w->tile_uoff, sizeof(int) * (cap + 1)) != hipSuccess) return WF_E_NOMEM;
+    if (hipMalloc(&w->tile_uc, sizeof(int) * cap) != hipSuccess) return WF_E_NOMEM;
     if (hipMalloc(&w->tile_scan, sizeof(int) * (grid1(cap, kScanBlock) + 1)) != hipSuccess)
       return WF_E_NOMEM;
     if (hipMalloc(&w->slot, sizeof(unsigned short) * 4 * kSlotQuads * npad) != hipSuccess)
@@ -312,6 +315,7 @@ void ns_free(NsWork* w) {
   hipFree(w->tile_nc);
   hipFree(w->tile_ucnt);
   hipFree(w->tile_uoff);
+  hipFree(w->tile_uc);
   hipFree(w->tile_scan);
   hipFree(w->tile_ovf);
   if (w->tile_host) hipHostFree(w->tile_host);
@@ -410,12 +414,13 @@ int ns_build(hipStream_t s, NsWork** work, const double* const pos_src[3], int n
     w->tj_cap = cap;
   }
   launch_tile_lists(s, pfn, n, g, w->cell_start, w->tile_keys, w->tile_nc, w->tile_uoff, r2,
-                    cap, npad, count, list, w->slot, w->tile_j, &ctr->nb_over_cap);
+                    cap, npad, count, list, w->slot, w->tile_j, w->tile_uc, &ctr->nb_over_cap);
   if (hipGetLastError() != hipSuccess) return WF_E_HIP;
   *grid_out = g;
   tiles->count = count;
   tiles->slotq = (const uint2*)w->slot;
   tiles->uoff = w->tile_uoff;
+  tiles->ucnt = w->tile_uc;
   tiles->tj = w->tile_j;
   tiles->npad = npad;
   return WF_OK;

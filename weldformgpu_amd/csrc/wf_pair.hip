@@ -3,8 +3,8 @@
 //
 // One workgroup per tile of kTileP Morton-consecutive particles, one thread
 // per particle in the FP64 loops and two in the FP32 loops.  The tile's
-// staged set (the particles of every occupied cell adjacent to the tile) is
-// written to LDS as {tile-frame position, particle index} slots; each thread
+// staged set (the particles that are a neighbour of at least one of the
+// tile's particles, in cell order) is written to LDS as {tile-frame position, particle index} slots; each thread
 // walks its neighbour list (16-bit staged positions in ascending order, four
 // per 8-byte word; with two threads per particle each takes every other
 // word), reads the neighbour's position and index from LDS and gathers the
@@ -113,9 +113,11 @@ __device__ __forceinline__ double gk_pair(double r2, double hj, const HConst& hc
 
 // gather-kernel traits of a policy: the staged position type, positions per
 // LDS pass, the per-tile context and a neighbour's staged position
-// staged slots per LDS pass: 768 x 32 B (FP64 position + index) = 24 KB
+// staged slots per LDS pass: 1024 x 32 B (FP64 position + index) = 32 KB;
+// on a cubic lattice with h = 1.2 dx a tile's staged set holds ~740
+// particles (at most ~910), so every tile takes the one-pass path
 #ifndef WF_STAGE64
-#define WF_STAGE64 768
+#define WF_STAGE64 1024
 #endif
 constexpr int kStage64 = WF_STAGE64;
 // threads per particle in the pair loops (each takes every LANES-th slot
@@ -510,10 +512,10 @@ __device__ __forceinline__ void pos32(const Fields& f, int j, const Ctx32& c, fl
   z = p.z + (float)(cz - c.bz) * c.cs;
 }
 
-// staged slots per LDS pass: 1536 x 16 B (tile-frame position + index) =
-// 24 KB; a tile's staged set averages ~1.1k particles on a cubic lattice
+// staged slots per LDS pass: 1024 x 16 B (tile-frame position + index) =
+// 16 KB (768 slots send the larger tiles to the chunked path: 17 % slower)
 #ifndef WF_STAGE32
-#define WF_STAGE32 1536
+#define WF_STAGE32 1024
 #endif
 constexpr int kStage32 = WF_STAGE32;
 #define WF_GATHER32                                                                \
@@ -582,7 +584,7 @@ __global__ __launch_bounds__(4 * kTileP) void k_pair_gather(Fields f, TileLists 
   typename K::Own own;
   typename K::Acc acc;
   K::init(f, i, act, own, acc, cx);
-  const int ub = tl.uoff[t], ucnt = tl.uoff[t + 1] - ub;
+  const int ub = tl.uoff[t], ucnt = tl.ucnt[t];
   const int* __restrict__ tj = tl.tj + ub;
   const int cnt = act ? tl.count[i] : 0;
 #ifndef WF_STAGE_UNROLL
@@ -1019,7 +1021,7 @@ struct CondG {
 // flag |n_i| >= 0.25 h_i with nb_i <= 46 (Geometry.cu:26-88).  Rigid
 // triangles are not particles here, so every list entry counts.
 #ifndef WF_STAGE_SURF
-#define WF_STAGE_SURF 1536
+#define WF_STAGE_SURF 1024
 #endif
 template <bool UM>
 struct SurfD {

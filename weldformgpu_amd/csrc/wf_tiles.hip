@@ -3,8 +3,11 @@
 // A tile is kTileP consecutive particles of the Morton order (one workgroup
 // of the pair kernels).  At each rebuild the tile gets:
 //   - the sorted set of occupied grid cells adjacent to any of its particles
-//     (tile_keys), and the concatenation of those cells' particle ranges
-//     (tile_j, ascending particle index) — the tile's staged set;
+//     (tile_keys); the particles of those cells (in cell order, ascending
+//     particle index within a cell) that are a neighbour of at least one of
+//     the tile's particles form the tile's staged set (tile_j, tile_uc
+//     entries) — on a cubic lattice with h = 1.2 dx about 740 of the ~1,230
+//     particles of the adjacent cells;
 //   - for each of its particles, the neighbour list in ascending index order,
 //     both as global indices (int32) and as 16-bit positions in the staged
 //     set, interleaved [k*npad + i] so a wavefront reads 128 contiguous bytes
@@ -152,19 +155,29 @@ __global__ __launch_bounds__(kTileP) void k_tile_cells(const float* __restrict__
   }
 }
 
+// adjacent-cell sets up to this size are compacted to the particles that
+// some list references (larger ones are staged whole)
+constexpr int kCompactMax = 4096;
+
 // Neighbour lists in staged order (global index + 16-bit slot), and the
-// staged set tile_j.  Same float criterion as the reference search:
-// 0 < d2 < r2, d2 = dx*dx + dy*dy + dz*dz without contraction, at most cap
-// entries (70 as in the reference, 96 with a list skin).
+// staged set tile_j / tile_uc.  Same float criterion as the reference
+// search: 0 < d2 < r2, d2 = dx*dx + dy*dy + dz*dz without contraction, at
+// most cap entries (70 as in the reference, 96 with a list skin).  The
+// compaction keeps the order of the adjacent-cell set, so every list stays
+// in ascending staged order.
 __global__ __launch_bounds__(kTileP) void k_tile_lists(
     const float* __restrict__ p0, const float* __restrict__ p1, const float* __restrict__ p2,
     int n, GridDesc g, const int* __restrict__ cell_start, const int* __restrict__ tile_keys,
     const int* __restrict__ tile_nc, const int* __restrict__ tile_uoff, float r2, int cap,
     int64_t npad, int* __restrict__ count, int* __restrict__ list,
-    unsigned short* __restrict__ slot, int* __restrict__ tile_j, int* over_cap) {
+    unsigned short* __restrict__ slot, int* __restrict__ tile_j, int* __restrict__ tile_uc,
+    int* over_cap) {
 #pragma clang fp contract(off)
   __shared__ int sb[kTileMaxCells], sn[kTileMaxCells], spre[kTileMaxCells];
   __shared__ int scx[kTileMaxCells], scy[kTileMaxCells], scz[kTileMaxCells];
+  __shared__ short sidx[kCompactMax];  // referenced flag, then compacted position (or -1)
+  __shared__ int part[kTileP];
+  __shared__ int s_all, s_used;
   const int t = blockIdx.x;
   const int tid = threadIdx.x;
   const int nc = min(tile_nc[t], kTileMaxCells);
@@ -186,18 +199,22 @@ __global__ __launch_bounds__(kTileP) void k_tile_lists(
       spre[p] = run;
       run += sn[p];
     }
+    s_all = run;
   }
   __syncthreads();
+  const int U = s_all;
+  const bool compact = U <= kCompactMax;
+  if (compact)
+    for (int s = tid; s < U; s += kTileP) sidx[s] = 0;
+  __syncthreads();
   const int uoff = tile_uoff[t];
-  for (int p = 0; p < nc; p++)
-    for (int r = tid; r < sn[p]; r += kTileP) tile_j[uoff + spre[p] + r] = sb[p] + r;
   const int i = t * kTileP + tid;
   int over = 0;
+  int c = 0;
   if (i < n) {
     const float xi = p0[i], yi = p1[i], zi = p2[i];
     int cx, cy, cz;
     cell_of_t(g, xi, yi, zi, cx, cy, cz);
-    int c = 0;
     for (int p = 0; p < nc; p++) {
       if (abs(scx[p] - cx) > 1 || abs(scy[p] - cy) > 1 || abs(scz[p] - cz) > 1) continue;
       const int b = sb[p], e = b + sn[p], base = spre[p];
@@ -209,6 +226,7 @@ __global__ __launch_bounds__(kTileP) void k_tile_lists(
           if (c < cap) {
             list[(int64_t)c * npad + i] = j;
             slot[(((int64_t)(c >> 2)) * npad + i) * 4 + (c & 3)] = (unsigned short)(base + (j - b));
+            if (compact) sidx[base + (j - b)] = 1;
           } else {
             over = 1;
           }
@@ -218,6 +236,64 @@ __global__ __launch_bounds__(kTileP) void k_tile_lists(
     }
     count[i] = c < cap ? c : cap;
   }
+  __syncthreads();
+  int used = U;
+  if (compact) {
+    // exclusive scan of the flags, one contiguous segment per thread
+    const int seg = (U + kTileP - 1) / kTileP;
+    const int s0 = min(tid * seg, U), s1 = min(s0 + seg, U);
+    int sum = 0;
+    for (int s = s0; s < s1; s++) sum += sidx[s];
+    part[tid] = sum;
+    __syncthreads();
+    if (tid == 0) {
+      int run = 0;
+      for (int u = 0; u < kTileP; u++) {
+        const int v = part[u];
+        part[u] = run;
+        run += v;
+      }
+      s_used = run;
+    }
+    __syncthreads();
+    int run = part[tid];
+    for (int s = s0; s < s1; s++) {
+      const int fl = sidx[s];
+      sidx[s] = fl ? run : -1;
+      run += fl;
+    }
+    __syncthreads();
+    used = s_used;
+    // this thread's slots (written above by this thread) to compacted
+    // positions, four quads of four slots in flight at a time
+    const int m = min(c, cap);
+    const int nq = (m + 3) >> 2;
+    uint2* qrow = reinterpret_cast<uint2*>(slot) + i;
+    for (int q0 = 0; q0 < nq; q0 += 4) {
+      uint2 w[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (q0 + u < nq) w[u] = qrow[(int64_t)(q0 + u) * npad];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        if (q0 + u >= nq) continue;
+        const int k = 4 * (q0 + u);  // slots past m keep what they hold
+        unsigned lo = w[u].x, hi = w[u].y;
+        if (k < m) lo = (lo & 0xFFFF0000u) | (unsigned short)sidx[lo & 0xFFFFu];
+        if (k + 1 < m) lo = (lo & 0xFFFFu) | ((unsigned)(unsigned short)sidx[lo >> 16] << 16);
+        if (k + 2 < m) hi = (hi & 0xFFFF0000u) | (unsigned short)sidx[hi & 0xFFFFu];
+        if (k + 3 < m) hi = (hi & 0xFFFFu) | ((unsigned)(unsigned short)sidx[hi >> 16] << 16);
+        qrow[(int64_t)(q0 + u) * npad] = make_uint2(lo, hi);
+      }
+    }
+  }
+  for (int p = 0; p < nc; p++)
+    for (int r = tid; r < sn[p]; r += kTileP) {
+      const int s = spre[p] + r;
+      const int d = compact ? sidx[s] : s;
+      if (d >= 0) tile_j[uoff + d] = sb[p] + r;
+    }
+  if (tid == 0) tile_uc[t] = used;
   unsigned long long o = wave_sum_u64((unsigned long long)over);
   if (lane_id() == 0 && o) atomicAdd(over_cap, (int)o);
 }
@@ -233,11 +309,12 @@ void launch_tile_cells(hipStream_t s, const float* const pf[3], int n, const Gri
 void launch_tile_lists(hipStream_t s, const float* const pf[3], int n, const GridDesc& g,
                        const int* cell_start, const int* tile_keys, const int* tile_nc,
                        const int* tile_uoff, float r2, int cap, int64_t npad, int* count,
-                       int* list, unsigned short* slot, int* tile_j, int* over_cap) {
+                       int* list, unsigned short* slot, int* tile_j, int* tile_uc,
+                       int* over_cap) {
   int nt = (n + kTileP - 1) / kTileP;
   hipLaunchKernelGGL(k_tile_lists, dim3(nt), dim3(kTileP), 0, s, pf[0], pf[1], pf[2], n, g,
                      cell_start, tile_keys, tile_nc, tile_uoff, r2, cap, npad, count, list, slot,
-                     tile_j, over_cap);
+                     tile_j, tile_uc, over_cap);
 }
 
 }  // namespace wf

@@ -29,7 +29,8 @@ struct GridDesc {
 struct TileLists {
   const int* count;
   const uint2* slotq;
-  const int* uoff;  // [ntiles+1] offsets of each tile's staged set in tj
+  const int* uoff;  // [ntiles+1] offsets of each tile's region of tj
+  const int* ucnt;  // [ntiles] size of each tile's staged set (a prefix of its region)
   const int* tj;    // staged particle indices
   int64_t npad;
 };
@@ -40,6 +41,7 @@ void launch_tile_cells(hipStream_t s, const float* const pf[3], int n, const Gri
 void launch_tile_lists(hipStream_t s, const float* const pf[3], int n, const GridDesc& g,
                        const int* cell_start, const int* tile_keys, const int* tile_nc,
                        const int* tile_uoff, float r2, int cap, int64_t npad, int* count,
-                       int* list, unsigned short* slot, int* tile_j, int* over_cap);
+                       int* list, unsigned short* slot, int* tile_j, int* tile_uc,
+                       int* over_cap);
 
 }  // namespace wf
