@@ -810,6 +810,10 @@ int or_nsearch_build(or_state* s, const wf_config* c, int* over_cap) {
   int64_t tot = or_neighbors(src, s->n, radius, s->nb_count, s->nb_offset, s->nb_list,
                              s->nb_cap, over_cap);
   if (tot > s->nb_cap) return WF_E_NBCAP;
+  if (pz && over_cap) {  // skin lists hold up to WF_MAX_NEIGHBORS_SKIN entries
+    *over_cap = 0;
+    for (int i = 0; i < s->n; i++) *over_cap += s->nb_count[i] > WF_MAX_NEIGHBORS_SKIN;
+  }
   if (pz && s->xb)
     for (int64_t q = 0; q < 3 * (int64_t)s->n; q++) s->xb[q] = s->x[q];
   return WF_OK;

@@ -185,7 +185,7 @@ class State:
             self.arrays[k] = np.zeros((n, 6))
         for k in _INT1:
             self.arrays[k] = np.zeros(n, np.int32)
-        self.nb_cap = nb_cap or 80 * n + 16
+        self.nb_cap = nb_cap or 100 * n + 16  # above the 96-entry skin-list cap
         self.arrays["nb_count"] = np.zeros(n, np.int32)
         self.arrays["nb_offset"] = np.zeros(n + 1, np.int32)
         self.arrays["nb_list"] = np.zeros(self.nb_cap, np.int32)

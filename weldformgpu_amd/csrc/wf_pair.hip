@@ -1,17 +1,16 @@
 // wf_pair.hip — the pair loops of the step (gfx950), FP64 and mixed
 // precision, on one kernel template (k_pair_gather).
 //
-// One workgroup per tile of kTileP Morton-consecutive particles, one thread
-// per particle in the FP64 loops and two in the FP32 loops.  The tile's
-// staged set (the particles that are a neighbour of at least one of the
-// tile's particles, in cell order) is written to LDS as {tile-frame position, particle index} slots; each thread
-// walks its neighbour list (16-bit staged positions in ascending order, four
-// per 8-byte word; with two threads per particle each takes every other
-// word), reads the neighbour's position and index from LDS and gathers the
-// neighbour's packed record from global memory.  The LDS set (24 KB) limits
-// a 128-thread workgroup to 3 wavefronts per SIMD; two threads per particle
-// double that for the FP32 loops, whose register use allows 4-6, while the
-// FP64 loops are limited by their registers to 3-4 either way.  The per-pair physics is the reference's
+// One workgroup per tile of kTileP Morton-consecutive particles, two threads
+// per particle.  The tile's staged set (the particles that are a neighbour of
+// at least one of the tile's particles, in cell order) is written to LDS as
+// {tile-frame position, particle index} slots; each thread walks half of its
+// particle's neighbour list (16-bit staged positions in ascending order,
+// four per 8-byte word, every other word), reads the neighbour's position and
+// index from LDS and gathers the neighbour's packed record from global
+// memory.  Two threads per particle give a CU twice the wavefronts for the
+// same LDS; the loops' registers then allow 4-6 (FP32) or 3-4 (FP64)
+// wavefronts per SIMD.  The per-pair physics is the reference's
 // (InteractionAlt.cu, Thermal.cu, Geometry.cu), with per-particle factors
 // (m/rho, sigma/rho^2, 1/h) formed once instead of once per pair; the
 // policies (DensD/DensG, RatesD/RatesG, AccelD/AccelG, CondD/CondG, SurfD,
@@ -122,9 +121,10 @@ __device__ __forceinline__ double gk_pair(double r2, double hj, const HConst& hc
 constexpr int kStage64 = WF_STAGE64;
 // threads per particle in the pair loops (each takes every LANES-th slot
 // quad); on a 759k-particle cylinder two threads take the FP32 step from
-// 0.672 to 0.591 ms, four are slower (0.678 ms), and the FP64 loops gain nothing
+// 0.672 to 0.591 ms (four: 0.678 ms) and, with the 32 KB FP64 stage, the
+// FP64 step from 1.00 to 0.90 ms
 #ifndef WF_LANES64
-#define WF_LANES64 1
+#define WF_LANES64 2
 #endif
 #ifndef WF_LANES32
 #define WF_LANES32 2
@@ -148,8 +148,9 @@ constexpr int kStage64 = WF_STAGE64;
 // ---------------------------------------------------------------------------
 // FP64 pair loops (WF_FP64) on the gather kernel below: FP64 positions of the
 // staged set in LDS, the neighbour fields from packed FP64 records.  Each
-// thread sums its pairs in ascending staged order, as the LDS-staged
-// formulation does, so the sums are the same operations in the same order.
+// thread sums its pairs in ascending staged order and the two threads of a
+// particle add their partial sums, a fixed order, so results are
+// deterministic.
 
 // A: density rate drho_i = rho_i sum_j (m_j/rho_j) v_ij . (GK x_ij)
 // (InteractionAlt.cu:166-246) [+ update rho_i + dt drho_i into rho_alt and
