@@ -173,7 +173,9 @@ enum {
   WF_F_INT_ENERGY, WF_F_T, WF_F_TA, WF_F_TB, WF_F_DTDT, WF_F_K_T, WF_F_CP_T,
   WF_F_Q_PLASTIC,
   WF_F_Q_FRICTION, /* sliding-friction heat power into the particle [W]     */
-  /* double[6*n] */
+  /* double[6*n], flat symmetric order xx yy zz xy yz xz; the rotation rate
+     holds the upper triangle of the antisymmetric W (InteractionAlt.cu:358-365),
+     so its diagonal is always zero: values written there are stored as zero */
   WF_F_SIGMA = 48, WF_F_SHEARSTRESS, WF_F_STRAIN, WF_F_STRRATE, WF_F_ROTRATE,
   /* int32[n] */
   WF_F_ID = 64, WF_F_BC_T,

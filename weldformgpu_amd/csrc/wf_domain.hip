@@ -741,6 +741,13 @@ int wf_set_field(wf_domain* d, int field, const void* host, size_t bytes) {
     for (int c = 0; c < comps; c++) planes.p[c] = *d->plane(field, c).d;
     hipLaunchKernelGGL(k_upload_f64, dim3(gridn(d->n)), dim3(256), 0, d->stream, planes, comps,
                        (const double*)d->stage, d->perm, d->n);
+    // the rotation rate is antisymmetric: its diagonal planes stay zero (the
+    // pair and stress kernels neither write nor read them)
+    if (field == WF_F_ROTRATE)
+      for (int c = 0; c < 3; c++)
+        hipLaunchKernelGGL(k_zero_bytes, dim3(gridn(sizeof(double) * d->npad)), dim3(256), 0,
+                           d->stream, (unsigned char*)d->f.rotrate[c],
+                           (int64_t)sizeof(double) * d->npad);
   } else {
     hipLaunchKernelGGL(k_upload_i32, dim3(gridn(d->n)), dim3(256), 0, d->stream,
                        *d->plane(field, 0).i, (const int*)d->stage, d->perm, d->n);

@@ -117,7 +117,7 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Block
     D.xy = D.yx = f.strrate[3][i]; D.yz = D.zy = f.strrate[4][i];
     D.xz = D.zx = f.strrate[5][i];
     T9 W;
-    W.xx = f.rotrate[0][i]; W.yy = f.rotrate[1][i]; W.zz = f.rotrate[2][i];
+    W.xx = W.yy = W.zz = 0.0;  // antisymmetric: the diagonal planes hold zeros
     W.xy = f.rotrate[3][i]; W.yz = f.rotrate[4][i]; W.xz = f.rotrate[5][i];
     W.yx = -W.xy; W.zy = -W.yz; W.zx = -W.xz;
     T9 WT = {W.xx, W.yx, W.zx, W.xy, W.yy, W.zy, W.xz, W.yz, W.zz};

@@ -275,8 +275,7 @@ struct RatesD {
     f.strrate[0][i] = -a.dxx; f.strrate[1][i] = -a.dyy; f.strrate[2][i] = -a.dzz;
     f.strrate[3][i] = -0.5 * a.dxy; f.strrate[4][i] = -0.5 * a.dyz;
     f.strrate[5][i] = -0.5 * a.dxz;
-    f.rotrate[0][i] = 0.0; f.rotrate[1][i] = 0.0; f.rotrate[2][i] = 0.0;
-    f.rotrate[3][i] = -0.5 * a.wxy; f.rotrate[4][i] = -0.5 * a.wyz;
+    f.rotrate[3][i] = -0.5 * a.wxy; f.rotrate[4][i] = -0.5 * a.wyz;  // diagonal stays 0
     f.rotrate[5][i] = -0.5 * p.xz_sign * a.wxz;
   }
 };
@@ -852,8 +851,7 @@ struct RatesG {
     f.strrate[2][i] = -a.dzz.val();
     f.strrate[3][i] = -0.5 * a.dxy.val(); f.strrate[4][i] = -0.5 * a.dyz.val();
     f.strrate[5][i] = -0.5 * a.dxz.val();
-    f.rotrate[0][i] = 0.0; f.rotrate[1][i] = 0.0; f.rotrate[2][i] = 0.0;
-    f.rotrate[3][i] = -0.5 * a.wxy.val(); f.rotrate[4][i] = -0.5 * a.wyz.val();
+    f.rotrate[3][i] = -0.5 * a.wxy.val(); f.rotrate[4][i] = -0.5 * a.wyz.val();  // diagonal stays 0
     f.rotrate[5][i] = -0.5 * p.xz_sign * a.wxz.val();
   }
 };
