@@ -38,6 +38,7 @@ int fail(int code, const std::string& msg) {
 struct PlaneRef {
   double** d = nullptr;
   int** i = nullptr;
+  float** f = nullptr;  // an FP32 plane behind a double field (mixed stress state)
 };
 
 }  // namespace
@@ -55,6 +56,7 @@ struct wf_domain {
   int* nb_count = nullptr;
   int* nb_list = nullptr;
   double* spare_d = nullptr;  // one plane, reorder scratch
+  float* spare_f = nullptr;   // the same for the FP32 planes (mixed)
   int* spare_i = nullptr;
   void* stage = nullptr;      // transfer staging (6*n doubles)
   Counters* ctr = nullptr;
@@ -199,6 +201,28 @@ __global__ void k_upload_i32(int* plane, const int* stage, const int* perm, int 
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s < n) plane[s] = stage[perm[s]];
 }
+// the same for a field held in FP32 planes (rounded on upload)
+struct Planes6f {
+  float* p[6];
+};
+__global__ void k_upload_f64_f32(Planes6f planes, int comps, const double* stage,
+                                 const int* perm, int n) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int o = perm[s];
+  for (int c = 0; c < comps; c++) planes.p[c][s] = (float)stage[(int64_t)o * comps + c];
+}
+__global__ void k_download_f32_f64(Planes6f planes, int comps, double* stage, const int* perm,
+                                   int n) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int o = perm[s];
+  for (int c = 0; c < comps; c++) stage[(int64_t)o * comps + c] = (double)planes.p[c][s];
+}
+__global__ void k_fill_f32v(float* a, float v, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
 __global__ void k_download_i32(const int* plane, int* stage, const int* perm, int n) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s < n) stage[perm[s]] = plane[s];
@@ -290,11 +314,24 @@ PlaneRef wf_domain::plane(int field, int c) {
     case WF_F_CP_T: r.d = &f.cp; break;
     case WF_F_Q_PLASTIC: r.d = &f.qpl; break;
     case WF_F_Q_FRICTION: r.d = &f.qfric; break;
-    case WF_F_SIGMA: r.d = &f.sigma[c]; break;
-    case WF_F_SHEARSTRESS: r.d = &f.ss[c]; break;
+    // sigma, the deviatoric stress and the rates: FP32 planes in mixed precision
+    case WF_F_SIGMA:
+      if (mixed) r.f = &f.sigma32[c];
+      else r.d = &f.sigma[c];
+      break;
+    case WF_F_SHEARSTRESS:
+      if (mixed) r.f = &f.ss32[c];
+      else r.d = &f.ss[c];
+      break;
     case WF_F_STRAIN: r.d = &f.strain[c]; break;
-    case WF_F_STRRATE: r.d = &f.strrate[c]; break;
-    case WF_F_ROTRATE: r.d = &f.rotrate[c]; break;
+    case WF_F_STRRATE:
+      if (mixed) r.f = &f.strrate32[c];
+      else r.d = &f.strrate[c];
+      break;
+    case WF_F_ROTRATE:
+      if (mixed) r.f = &f.rotrate32[c];
+      else r.d = &f.rotrate[c];
+      break;
     case WF_F_ID: r.i = &f.ID; break;
     case WF_F_BC_T: r.i = &f.BC_T; break;
     case WF_F_NB_COUNT: r.i = &nb_count; break;
@@ -312,7 +349,7 @@ std::vector<PlaneRef> wf_domain::all_planes() {
     if (field_info(fld, &comps, &elem) != WF_OK) continue;
     for (int c = 0; c < comps; c++) {
       PlaneRef r = plane(fld, c);
-      if (r.d || r.i) v.push_back(r);
+      if (r.d || r.i || r.f) v.push_back(r);
     }
   }
   return v;
@@ -559,12 +596,13 @@ int wf_create(const wf_config* cfg, wf_domain** out) {
     return fail(WF_E_HIP, "hipStreamCreate failed");
   }
   int rc = WF_OK;
+  d->mixed = cfg->precision == WF_MIXED;  // before the planes: it selects their precision
   for (PlaneRef r : d->all_planes()) {
-    rc = r.d ? d->alloc_plane_d(r.d) : d->alloc_plane_i(r.i);
+    rc = r.d ? d->alloc_plane_d(r.d) : r.f ? d->alloc_plane_f(r.f) : d->alloc_plane_i(r.i);
     if (rc) break;
   }
-  d->mixed = cfg->precision == WF_MIXED;
   if (d->mixed) {
+    if (!rc) rc = d->alloc_plane_f(&d->spare_f);
     if (!rc) rc = d->alloc_plane_f(&d->f.h32);
     if (!rc) rc = d->dev_alloc((void**)&d->f.p4, sizeof(float4) * d->npad);
     float4** f4[] = {&d->f.v4, &d->f.v4_alt};
@@ -736,7 +774,17 @@ int wf_set_field(wf_domain* d, int field, const void* host, size_t bytes) {
     for (int i = 1; i < d->n && d->uniform_m; i++) d->uniform_m = mv[i] == mv[0];
   }
   HIPC(hipMemcpyAsync(d->stage, host, bytes, hipMemcpyHostToDevice, d->stream));
-  if (elem == 8) {
+  if (elem == 8 && d->plane(field, 0).f) {
+    Planes6f planes{};
+    for (int c = 0; c < comps; c++) planes.p[c] = *d->plane(field, c).f;
+    hipLaunchKernelGGL(k_upload_f64_f32, dim3(gridn(d->n)), dim3(256), 0, d->stream, planes,
+                       comps, (const double*)d->stage, d->perm, d->n);
+    if (field == WF_F_ROTRATE)  // the antisymmetric W's diagonal stays zero
+      for (int c = 0; c < 3; c++)
+        hipLaunchKernelGGL(k_zero_bytes, dim3(gridn(sizeof(float) * d->npad)), dim3(256), 0,
+                           d->stream, (unsigned char*)d->f.rotrate32[c],
+                           (int64_t)sizeof(float) * d->npad);
+  } else if (elem == 8) {
     Planes6 planes{};
     for (int c = 0; c < comps; c++) planes.p[c] = *d->plane(field, c).d;
     hipLaunchKernelGGL(k_upload_f64, dim3(gridn(d->n)), dim3(256), 0, d->stream, planes, comps,
@@ -776,7 +824,12 @@ int wf_get_field(wf_domain* d, int field, void* host, size_t bytes) {
     std::memset(host, 0, bytes);
     return WF_OK;
   }
-  if (elem == 8) {
+  if (elem == 8 && d->plane(field, 0).f) {
+    Planes6f planes{};
+    for (int c = 0; c < comps; c++) planes.p[c] = *d->plane(field, c).f;
+    hipLaunchKernelGGL(k_download_f32_f64, dim3(gridn(d->n)), dim3(256), 0, d->stream, planes,
+                       comps, (double*)d->stage, d->perm, d->n);
+  } else if (elem == 8) {
     Planes6 planes{};
     for (int c = 0; c < comps; c++) planes.p[c] = *d->plane(field, c).d;
     hipLaunchKernelGGL(k_download_f64, dim3(gridn(d->n)), dim3(256), 0, d->stream, planes, comps,
@@ -795,7 +848,23 @@ int wf_fill_field(wf_domain* d, int field, double value) {
   if (!d) return fail(WF_E_ARG, "null domain");
   int comps, elem;
   if (field_info(field, &comps, &elem) || elem != 8) return fail(WF_E_ARG, "not a double field");
-  for (int c = 0; c < comps; c++) launch_fill_f64(d->stream, *d->plane(field, c).d, value, d->n);
+  for (int c = 0; c < comps; c++) {
+    const PlaneRef r = d->plane(field, c);
+    if (r.f)
+      hipLaunchKernelGGL(k_fill_f32v, dim3(gridn(d->n)), dim3(256), 0, d->stream, *r.f,
+                         (float)value, d->n);
+    else
+      launch_fill_f64(d->stream, *r.d, value, d->n);
+  }
+  if (field == WF_F_ROTRATE)  // the antisymmetric W's diagonal stays zero
+    for (int c = 0; c < 3; c++) {
+      const PlaneRef r = d->plane(field, c);
+      if (r.f)
+        hipLaunchKernelGGL(k_fill_f32v, dim3(gridn(d->n)), dim3(256), 0, d->stream, *r.f, 0.f,
+                           d->n);
+      else
+        launch_fill_f64(d->stream, *r.d, 0.0, d->n);
+    }
   if (field == WF_F_M) {
     d->surf_init = false;
     d->uniform_m = true;
@@ -875,6 +944,10 @@ int wf_nsearch_build(wf_domain* d) {
     if (r.d) {
       launch_gather_f64(d->stream, d->spare_d, *r.d, d->gather, d->n);
       std::swap(*r.d, d->spare_d);
+    } else if (r.f) {
+      hipLaunchKernelGGL(k_gather_f32, dim3(gridn(d->n)), dim3(256), 0, d->stream, d->spare_f,
+                         *r.f, d->gather, d->n);
+      std::swap(*r.f, d->spare_f);
     } else {
       launch_gather_i32(d->stream, d->spare_i, *r.i, d->gather, d->n);
       std::swap(*r.i, d->spare_i);
@@ -1007,7 +1080,7 @@ int wf_pressure(wf_domain* d) {
 int wf_stress_strain(wf_domain* d) {
   if (!d) return fail(WF_E_ARG, "null domain");
   const StepParams p = d->params();
-  launch_stress(d->stream, d->f, p, d->bstats, false, true, kRecNone, d->ret_args());
+  launch_stress(d->stream, d->f, p, d->bstats, false, true, kRecNone, d->mixed, d->ret_args());
   launch_fold_stats(d->stream, d->bstats, (p.solid + kBlock - 1) / kBlock, d->ctr);
   d->mirror_dirty = true;
   HIPC(hipGetLastError());
@@ -1358,7 +1431,7 @@ int wf_step(wf_domain* d, int n_steps) {
       if ((rc = timer_end(d, WF_K_RATES))) return rc;
       timer_begin(d);
       const int rec = mx ? kRec32 : kRec64;
-      launch_stress(d->stream, d->f, p, d->bstats, true, true, rec, d->ret_args());
+      launch_stress(d->stream, d->f, p, d->bstats, true, true, rec, mx, d->ret_args());
       if ((rc = timer_end(d, WF_K_STRESS))) return rc;
       launch_fold_stats(d->stream, d->bstats, (p.solid + kBlock - 1) / kBlock, d->ctr);
       timer_begin(d);

@@ -26,7 +26,12 @@ struct Fields {
   double *rho, *rho_alt, *rho0, *drho, *m, *h, *p, *Cs, *G;
   double *sigma_y, *pl_strain, *sigma_eq, *esr, *eint;
   double *T, *T_alt, *Ta, *Tb, *dTdt, *k, *cp, *qpl, *qfric;
+  // FP64 stress state, strain and rates (WF_FP64; in WF_MIXED only strain)
   double *sigma[6], *ss[6], *strain[6], *strrate[6], *rotrate[6];
+  // WF_MIXED keeps sigma, the deviatoric stress and the rates in FP32 (the
+  // stress pass computes in FP64 from them); the FP64 planes above are not
+  // allocated then
+  float *sigma32[6], *ss32[6], *strrate32[6], *rotrate32[6];
   int *ID, *BC_T;
   // FP32 mirrors read by the mixed-precision pair loops (WF_MIXED): p4 =
   // {position relative to the origin of an integer grid cell, the cell packed
@@ -241,8 +246,9 @@ void launch_records64(hipStream_t s, const Fields& f, int n);
 // ---- per-particle kernels (wf_kernels.hip) --------------------------------
 void launch_update_density(hipStream_t s, const Fields& f, const StepParams& p, double dt);
 void launch_pressure(hipStream_t s, const Fields& f, const StepParams& p);
+// s32: the FP32 stress-state planes (mixed precision) instead of the FP64 ones
 void launch_stress(hipStream_t s, const Fields& f, const StepParams& p, BlockStats* bs,
-                   bool fuse_pressure, bool energy, int rec, const RetArgs& ra);
+                   bool fuse_pressure, bool energy, int rec, bool s32, const RetArgs& ra);
 // fold the per-block statistics of the last stress (+ return) pass into ctr
 // largest squared displacement since the last build (x - xb) over the
 // particles with eps_p > ep_hot, as the bits of a non-negative double
@@ -275,7 +281,6 @@ void launch_update_vel(hipStream_t s, const Fields& f, int n, double dt);
 void launch_integrate(hipStream_t s, const Fields& f, int n, double dt, const BcTable& bc,
                       int rec, const Frame32& fr);
 void launch_leapfrog(hipStream_t s, const Fields& f, int n, double dt, bool first);
-void launch_int_energy(hipStream_t s, const Fields& f, const StepParams& p);
 void launch_sum(hipStream_t s, const double* v, int n, double* partial, int nblocks,
                 double* out);
 

@@ -96,7 +96,9 @@ __device__ __forceinline__ double jc_newton(const MatParams& mp, double s_trial,
 // the yielding lanes run the FP64 Newton from it (from 0 without the
 // surrogate) to the same residual; then every lane finishes and writes its
 // state once.  The solved increment is logged as a training sample.
-template <bool FUSE_P, bool ENERGY, int REC>
+// S32: sigma, the deviatoric stress and the rates are stored in FP32
+// (mixed precision); the arithmetic is FP64 either way.
+template <bool FUSE_P, bool ENERGY, int REC, bool S32>
 __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, BlockStats* bs,
                                                    RetArgs ra) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -111,14 +113,16 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Block
   if (active) {
     const bool qp = (p.quirks & WF_Q_TENSOR_PRODUCT) != 0;
     if (FUSE_P) f.p[i] = 0.0 + (f.Cs[i] * f.Cs[i]) * (f.rho[i] - f.rho0[i]);
-    S.xx = f.ss[0][i]; S.yy = f.ss[1][i]; S.zz = f.ss[2][i];
-    S.xy = S.yx = f.ss[3][i]; S.yz = S.zy = f.ss[4][i]; S.xz = S.zx = f.ss[5][i];
-    D.xx = f.strrate[0][i]; D.yy = f.strrate[1][i]; D.zz = f.strrate[2][i];
-    D.xy = D.yx = f.strrate[3][i]; D.yz = D.zy = f.strrate[4][i];
-    D.xz = D.zx = f.strrate[5][i];
+    auto ss = [&](int k) { return S32 ? (double)f.ss32[k][i] : f.ss[k][i]; };
+    auto dd = [&](int k) { return S32 ? (double)f.strrate32[k][i] : f.strrate[k][i]; };
+    auto ww = [&](int k) { return S32 ? (double)f.rotrate32[k][i] : f.rotrate[k][i]; };
+    S.xx = ss(0); S.yy = ss(1); S.zz = ss(2);
+    S.xy = S.yx = ss(3); S.yz = S.zy = ss(4); S.xz = S.zx = ss(5);
+    D.xx = dd(0); D.yy = dd(1); D.zz = dd(2);
+    D.xy = D.yx = dd(3); D.yz = D.zy = dd(4); D.xz = D.zx = dd(5);
     T9 W;
     W.xx = W.yy = W.zz = 0.0;  // antisymmetric: the diagonal planes hold zeros
-    W.xy = f.rotrate[3][i]; W.yz = f.rotrate[4][i]; W.xz = f.rotrate[5][i];
+    W.xy = ww(3); W.yz = ww(4); W.xz = ww(5);
     W.yx = -W.xy; W.zy = -W.yz; W.zx = -W.xz;
     T9 WT = {W.xx, W.yx, W.zx, W.xy, W.yy, W.zy, W.xz, W.yz, W.zz};
     T9 SRT = tmul(S, WT, qp);
@@ -236,8 +240,12 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Block
       S.xx *= scale; S.yy *= scale; S.zz *= scale;
       S.xy *= scale; S.yz *= scale; S.xz *= scale;
     }
-    f.ss[0][i] = S.xx; f.ss[1][i] = S.yy; f.ss[2][i] = S.zz;
-    f.ss[3][i] = S.xy; f.ss[4][i] = S.yz; f.ss[5][i] = S.xz;
+    const double sv[6] = {S.xx, S.yy, S.zz, S.xy, S.yz, S.xz};
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      if (S32) f.ss32[k][i] = (float)sv[k];
+      else f.ss[k][i] = sv[k];
+    }
     f.strain[0][i] += D.xx * dt; f.strain[1][i] += D.yy * dt; f.strain[2][i] += D.zz * dt;
     f.strain[3][i] += D.xy * dt; f.strain[4][i] += D.yz * dt; f.strain[5][i] += D.xz * dt;
     f.sigma_y[i] = sigy;
@@ -248,7 +256,10 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Block
     // sigma = -p I + S (Mechanical.cu:540); store upper triangle
     const double sg[6] = {-pi + S.xx, -pi + S.yy, -pi + S.zz, S.xy, S.yz, S.xz};
 #pragma unroll
-    for (int k = 0; k < 6; k++) f.sigma[k][i] = sg[k];
+    for (int k = 0; k < 6; k++) {
+      if (S32) f.sigma32[k][i] = (float)sg[k];
+      else f.sigma[k][i] = sg[k];
+    }
     if (REC == kRec32) {  // sigma/rho^2, rho and Cs for the FP32 acceleration loop
       const double rho = f.rho[i], ir2 = 1.0 / (rho * rho);
       f.arec[rec_at(f, i, 1)] = make_float4((float)(sg[0] * ir2), (float)(sg[1] * ir2),
@@ -430,10 +441,10 @@ __global__ __launch_bounds__(kBlock) void k_mirror32(Fields f, int n, Frame32 fr
   const double ir2 = 1.0 / (rho * rho);
   f.arec[rec_at(f, i, 0)] = v4;
   f.arec[rec_at(f, i, 1)] =
-      make_float4((float)(f.sigma[0][i] * ir2), (float)(f.sigma[1][i] * ir2),
-                  (float)(f.sigma[2][i] * ir2), (float)(f.sigma[3][i] * ir2));
-  f.arec[rec_at(f, i, 2)] = make_float4((float)(f.sigma[4][i] * ir2),
-                                          (float)(f.sigma[5][i] * ir2), (float)rho,
+      make_float4((float)(f.sigma32[0][i] * ir2), (float)(f.sigma32[1][i] * ir2),
+                  (float)(f.sigma32[2][i] * ir2), (float)(f.sigma32[3][i] * ir2));
+  f.arec[rec_at(f, i, 2)] = make_float4((float)(f.sigma32[4][i] * ir2),
+                                          (float)(f.sigma32[5][i] * ir2), (float)rho,
                                           (float)f.Cs[i]);
   const float4 tk = make_float4((float)f.T[i], (float)f.k[i], v4.w, 0.f);
   f.tk4[i] = tk;
@@ -494,18 +505,6 @@ __global__ __launch_bounds__(kBlock) void k_leapfrog(Fields f, int n, double dt,
   }
 }
 
-// CalcIntEnergy per particle (Energy.cu:5-19, corrected indexing)
-__global__ __launch_bounds__(kBlock) void k_int_energy(Fields f, int n, double dt) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double de = f.m[i] / f.rho[i] *
-              (f.ss[0][i] * f.strrate[0][i] + f.ss[1][i] * f.strrate[1][i] +
-               f.ss[2][i] * f.strrate[2][i] +
-               2.0 * (f.ss[3][i] * f.strrate[3][i] + f.ss[4][i] * f.strrate[4][i] +
-                      f.ss[5][i] * f.strrate[5][i]));
-  f.eint[i] += de * dt;
-}
-
 // Deterministic FP64 sum: fixed-shape block partials, then one block.
 __global__ __launch_bounds__(kBlock) void k_sum_partial(const double* v, int n, double* partial) {
   __shared__ double sh[kBlock / 64];
@@ -560,22 +559,29 @@ void launch_pressure(hipStream_t s, const Fields& f, const StepParams& p) {
   if (p.n <= 0) return;
   hipLaunchKernelGGL(k_pressure, dim3(grid_for(p.n)), dim3(kBlock), 0, s, f, p.n);
 }
-void launch_stress(hipStream_t s, const Fields& f, const StepParams& p, BlockStats* bs,
-                   bool fuse_pressure, bool energy, int rec, const RetArgs& ra) {
-  if (p.solid <= 0) return;
+template <bool S32>
+static void launch_stress_t(hipStream_t s, const Fields& f, const StepParams& p,
+                            BlockStats* bs, bool fuse_pressure, bool energy, int rec,
+                            const RetArgs& ra) {
   dim3 g(grid_for(p.solid)), b(kBlock);
   if (rec == kRec32)
-    hipLaunchKernelGGL((k_stress<true, true, kRec32>), g, b, 0, s, f, p, bs, ra);
+    hipLaunchKernelGGL((k_stress<true, true, kRec32, true>), g, b, 0, s, f, p, bs, ra);
   else if (rec == kRec64)
-    hipLaunchKernelGGL((k_stress<true, true, kRec64>), g, b, 0, s, f, p, bs, ra);
+    hipLaunchKernelGGL((k_stress<true, true, kRec64, false>), g, b, 0, s, f, p, bs, ra);
   else if (fuse_pressure && energy)
-    hipLaunchKernelGGL((k_stress<true, true, kRecNone>), g, b, 0, s, f, p, bs, ra);
+    hipLaunchKernelGGL((k_stress<true, true, kRecNone, S32>), g, b, 0, s, f, p, bs, ra);
   else if (fuse_pressure)
-    hipLaunchKernelGGL((k_stress<true, false, kRecNone>), g, b, 0, s, f, p, bs, ra);
+    hipLaunchKernelGGL((k_stress<true, false, kRecNone, S32>), g, b, 0, s, f, p, bs, ra);
   else if (energy)
-    hipLaunchKernelGGL((k_stress<false, true, kRecNone>), g, b, 0, s, f, p, bs, ra);
+    hipLaunchKernelGGL((k_stress<false, true, kRecNone, S32>), g, b, 0, s, f, p, bs, ra);
   else
-    hipLaunchKernelGGL((k_stress<false, false, kRecNone>), g, b, 0, s, f, p, bs, ra);
+    hipLaunchKernelGGL((k_stress<false, false, kRecNone, S32>), g, b, 0, s, f, p, bs, ra);
+}
+void launch_stress(hipStream_t s, const Fields& f, const StepParams& p, BlockStats* bs,
+                   bool fuse_pressure, bool energy, int rec, bool s32, const RetArgs& ra) {
+  if (p.solid <= 0) return;
+  if (s32) launch_stress_t<true>(s, f, p, bs, fuse_pressure, energy, rec, ra);
+  else launch_stress_t<false>(s, f, p, bs, fuse_pressure, energy, rec, ra);
 }
 void launch_hot_disp(hipStream_t s, const Fields& f, const double* const xb[3], int n,
                      double ep_hot, unsigned long long* d2_bits) {
@@ -622,10 +628,6 @@ void launch_mirror32(hipStream_t s, const Fields& f, int n, const Frame32& fr) {
 void launch_leapfrog(hipStream_t s, const Fields& f, int n, double dt, bool first) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_leapfrog, dim3(grid_for(n)), dim3(kBlock), 0, s, f, n, dt, first ? 1 : 0);
-}
-void launch_int_energy(hipStream_t s, const Fields& f, const StepParams& p) {
-  if (p.n <= 0) return;
-  hipLaunchKernelGGL(k_int_energy, dim3(grid_for(p.n)), dim3(kBlock), 0, s, f, p.n, p.dt);
 }
 void launch_sum(hipStream_t s, const double* v, int n, double* partial, int nblocks,
                 double* out) {

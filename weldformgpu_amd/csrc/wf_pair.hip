@@ -847,12 +847,14 @@ struct RatesG {
   __device__ static void finish(const Fields& f, int i, bool act, const Own&, const Acc& a,
                                 const Prm& p) {
     if (!act) return;
-    f.strrate[0][i] = -a.dxx.val(); f.strrate[1][i] = -a.dyy.val();
-    f.strrate[2][i] = -a.dzz.val();
-    f.strrate[3][i] = -0.5 * a.dxy.val(); f.strrate[4][i] = -0.5 * a.dyz.val();
-    f.strrate[5][i] = -0.5 * a.dxz.val();
-    f.rotrate[3][i] = -0.5 * a.wxy.val(); f.rotrate[4][i] = -0.5 * a.wyz.val();  // diagonal stays 0
-    f.rotrate[5][i] = -0.5 * p.xz_sign * a.wxz.val();
+    f.strrate32[0][i] = (float)-a.dxx.val(); f.strrate32[1][i] = (float)-a.dyy.val();
+    f.strrate32[2][i] = (float)-a.dzz.val();
+    f.strrate32[3][i] = (float)(-0.5 * a.dxy.val());
+    f.strrate32[4][i] = (float)(-0.5 * a.dyz.val());
+    f.strrate32[5][i] = (float)(-0.5 * a.dxz.val());
+    f.rotrate32[3][i] = (float)(-0.5 * a.wxy.val());  // the diagonal stays 0
+    f.rotrate32[4][i] = (float)(-0.5 * a.wyz.val());
+    f.rotrate32[5][i] = (float)(-0.5 * p.xz_sign * a.wxz.val());
   }
 };
 
