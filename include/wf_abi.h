@@ -369,7 +369,9 @@ int wf_read_nastran(const char* path, int flip, double* nodes, int node_cap, int
 int wf_add_mesh(struct wf_domain* d, const double* nodes, int n_nodes, const int* elnode,
                 int n_elems, int zone_id, int* mesh);
 /* Rigid motion of a mesh: node velocity v + w x (x - center) (SetMeshVel,
- * Domain_d.cuh:553-556; TriMesh_d::Move uses w x x, i.e. center 0).       */
+ * Domain_d.cuh:553-556; TriMesh_d::Move uses w x x, i.e. center 0).  The
+ * centre then travels with v (extension: a tool spins about its own moving
+ * axis; the reference's fixed origin is the case v = 0 or w = 0).          */
 int wf_set_mesh_motion(struct wf_domain* d, int mesh, const double v[3], const double w[3],
                        const double center[3]);
 /* Counts of mesh m (nodes, elements, zone id); m = -1: totals over meshes. */

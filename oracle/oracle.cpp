@@ -902,6 +902,10 @@ void or_mesh_move(or_mesh* m, double dt) {
     st3(m->v, q, v);
     st3(m->x, q, add(ld3(m->x, q), scl(v, dt)));
   }
+  // the rotation centre travels with the mesh velocity (extension: the
+  // reference's TriMesh_d::Move rotates about the fixed origin)
+  for (int k = 0; k < m->n_meshes; k++)
+    for (int c = 0; c < 3; c++) m->motion[9 * k + 6 + c] += m->motion[9 * k + c] * dt;
   or_mesh_geom(m);
 }
 

@@ -1333,9 +1333,17 @@ int wf_calc_contact_forces(wf_domain* d) {
   return WF_OK;
 }
 
+// the rotation centre of each mesh travels with its velocity (a tool spins
+// about its own moving axis)
+static void advance_mesh_centres(wf_domain* d, double dt) {
+  for (int k = 0; k < d->n_meshes; k++)
+    for (int c = 0; c < 3; c++) d->motions.m[k].center[c] += d->motions.m[k].vel[c] * dt;
+}
+
 int wf_mesh_update(wf_domain* d, double dt) {
   if (!d) return fail(WF_E_ARG, "null domain");
   launch_mesh_move(d->stream, d->mesh, d->motions, dt);
+  advance_mesh_centres(d, dt);
   HIPC(hipGetLastError());
   return WF_OK;
 }
@@ -1505,6 +1513,7 @@ int wf_step(wf_domain* d, int n_steps) {
     if (d->cfg.mechanics && d->cfg.contact && d->n_meshes > 0) {  // SolverFraser.cu:290-318
       timer_begin(d);
       launch_mesh_move(d->stream, d->mesh, d->motions, p.dt);
+      advance_mesh_centres(d, p.dt);
       if ((rc = timer_end(d, WF_K_CONTACT))) return rc;
     }
     if (d->cfg.surrogate && d->jc_newton && d->is_yielding && d->cfg.mlp_train_interval > 0 &&

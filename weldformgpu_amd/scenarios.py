@@ -225,6 +225,122 @@ def c4_contact(scale=1.0, r=0.125e-3, dens=64, law=_abi.WF_CONTACT_STIFFNESS, mu
     return sc
 
 
+def gen_tool_mesh(center, zs, R_sh, R_pin, L_pin, n_theta=128, n_rs=20, n_zp=24, n_rc=6):
+    """Friction-stir tool surface, axis z through `center` (x, y): the
+    shoulder face (annulus R_pin..R_sh at z = zs), the pin side (radius R_pin,
+    z from zs - L_pin to zs) and the pin tip (disk at zs - L_pin).  Triangle
+    normals (n0n1 x n0n2) point out of the tool, into the workpiece: -z on the
+    shoulder face and tip, radially outward on the pin side.  Every triangle
+    has its own three nodes.  Returns nodes (3 ne, 3), elnode (ne, 3)."""
+    cx, cy = center
+    th = np.linspace(0.0, 2.0 * np.pi, n_theta + 1)[:-1]
+    t0, t1 = th, np.roll(th, -1)
+
+    def ring(r, t, z):
+        return np.stack([cx + r * np.cos(t), cy + r * np.sin(t), np.full_like(t, z)], -1)
+
+    tris = []
+    # shoulder annulus and pin-tip rings: (p00, p01, p10), (p01, p11, p10),
+    # i the inner radius, j the angle -> normal -z
+    def annulus(radii, z):
+        for ri, ro in zip(radii[:-1], radii[1:]):
+            p00, p01 = ring(ri, t0, z), ring(ri, t1, z)
+            p10, p11 = ring(ro, t0, z), ring(ro, t1, z)
+            tris.append(np.stack([p00, p01, p10], 1))
+            tris.append(np.stack([p01, p11, p10], 1))
+    annulus(np.linspace(R_pin, R_sh, n_rs + 1), zs)
+    zb = zs - L_pin
+    # pin side: k the height -> normal e_r
+    for z0, z1 in zip(np.linspace(zb, zs, n_zp + 1)[:-1], np.linspace(zb, zs, n_zp + 1)[1:]):
+        p00, p01 = ring(R_pin, t0, z0), ring(R_pin, t1, z0)
+        p10, p11 = ring(R_pin, t0, z1), ring(R_pin, t1, z1)
+        tris.append(np.stack([p00, p01, p10], 1))
+        tris.append(np.stack([p01, p11, p10], 1))
+    # pin tip: a fan round the axis, then rings out to R_pin -> normal -z
+    radii = np.linspace(0.0, R_pin, n_rc + 1)
+    c = np.tile(np.array([cx, cy, zb]), (n_theta, 1))
+    tris.append(np.stack([c, ring(radii[1], t1, zb), ring(radii[1], t0, zb)], 1))
+    annulus(radii[1:], zb)
+    T = np.concatenate(tris, 0)
+    nodes = T.reshape(-1, 3).copy()
+    el = np.arange(len(nodes), dtype=np.int32).reshape(-1, 3)
+    return nodes, el
+
+
+def c5_friction_stir(scale=1.0, r=0.125e-3, n_theta=128, n_rs=20, n_zp=24, n_rc=6, mu=0.2,
+                     skin=0.1, ep_hot=0.05, thick=1.0, offset=(0.0, 0.0)):
+    """Friction-stir-like plate: 120 x 60 x 6 mm of Johnson-Cook aluminium,
+    dx = 2r, and a rigid tool (shoulder R 9 mm, pin R 3 mm, pin 5 mm long,
+    ~12k triangles at the default resolution; `scale` multiplies the plate
+    and tool dimensions, `thick` the plate thickness) with its shoulder on the top face and its pin
+    plunged, centred at 1/4 of the length and 1/2 of the width, turning at
+    1000 rpm about its axis and travelling at 2 mm/s along x.  Particles
+    inside the pin are left out.  Penalty contact 0.1 E/dx with Coulomb
+    friction mu, all of the sliding-friction power as heat into the plate,
+    Taylor-Quinney 0.9, bottom layer held at 300 K and the bottom two layers
+    at v = 0; lists with a skin of `skin` dx rebuilt when a particle with
+    eps_p > ep_hot has moved half the skin.  `offset` shifts the tool axis by
+    that many spacings in x and y; unequal shifts keep lattice points off the
+    mesh's radial edges (a point on an edge makes the inside-triangle test a
+    matter of rounding)."""
+    rho, E, nu, T0 = 2700.0, 68.9e9, 0.3, 300.0
+    Lx, Ly, Lz = 120e-3 * scale, 60e-3 * scale, 6e-3 * scale * thick
+    R_sh, R_pin, L_pin = 9e-3 * scale, 3e-3 * scale, 5e-3 * scale
+    xyz, mass = gen_box([0.0, 0.0, 0.0], [Lx, Ly, Lz], r, rho)
+    dx = 2.0 * r
+    ztop = xyz[:, 2].max()
+    cx, cy = 0.25 * Lx + offset[0] * dx, 0.5 * Ly + offset[1] * dx
+    rr = np.hypot(xyz[:, 0] - cx, xyz[:, 1] - cy)
+    xyz = xyz[~((rr < R_pin) & (xyz[:, 2] > ztop - L_pin))]
+    n = len(xyz)
+    h = 1.2 * dx
+    K = E / (3. * (1. - 2 * nu))
+    G = E / (2. * (1. + nu))
+    Cs = np.sqrt(K / rho)
+    cfg = _abi.default_config()
+    cfg.n_particles = n
+    cfg.solid_count = n
+    cfg.h_glob = h
+    cfg.dt = 0.4 * h / (Cs + 10.0)
+    mp = cfg.mat
+    mp.E, mp.nu = E, nu
+    for k, v in JC_AL.items():
+        setattr(mp, k, v)
+    cfg.return_map = _abi.WF_RETURN_NEWTON
+    th = min(max((T0 - mp.Tt) / (mp.Tm - mp.Tt), 0.0), 1.0)
+    sy = mp.A * (1.0 + mp.C * np.log(1e-5 / mp.eps0)) * (1.0 - th ** mp.m)
+    f = _mech_fields(xyz, mass, rho, h, Cs, G, sy)
+    big = 10 * Lx
+    assign_zone(xyz, f["ID"], [-big, -big, -big], [big, big, xyz[:, 2].min() + 1.5 * dx], 2)
+    bcs = [(2, (0.0, 0.0, 0.0))]
+    cfg.thermal = 1
+    cfg.taylor_quinney = 0.9
+    f["T"] = np.full(n, T0)
+    f["k_T"] = np.full(n, 200.0)
+    f["cp_T"] = np.full(n, 900.0)
+    bottom = np.isclose(xyz[:, 2], xyz[:, 2].min())
+    f["BC_T"] = np.where(bottom, _abi.WF_BCT_FIXED, _abi.WF_BCT_NONE).astype(np.int32)
+    cfg.contact = 1
+    cfg.contact_law = _abi.WF_CONTACT_STIFFNESS
+    cfg.contact_kappa = 0.1
+    cfg.contact_dx = dx
+    cfg.fric_sta = mu
+    cfg.fric_dyn = mu
+    cfg.fric_heat = 1.0
+    cfg.rebuild_policy = _abi.WF_REBUILD_PLASTIC_ZONE
+    cfg.ns_skin = skin * dx
+    cfg.ep_hot = ep_hot
+    zs = ztop + r  # the shoulder face half a spacing above the top layer
+    nodes, el = gen_tool_mesh((cx, cy), zs, R_sh, R_pin, L_pin, n_theta, n_rs, n_zp, n_rc)
+    omega = 1000.0 * 2.0 * np.pi / 60.0
+    sc = Scenario("c5_friction_stir_plate", cfg, f, bcs,
+                  {"r": r, "dx": dx, "h": h, "Cs": Cs, "mass": mass, "tool_center": (cx, cy),
+                   "shoulder_z": zs, "R_sh": R_sh, "R_pin": R_pin, "L_pin": L_pin})
+    sc.meshes = [{"nodes": nodes, "elnode": el, "zone": 10, "vel": (2e-3, 0.0, 0.0),
+                  "omega": (0.0, 0.0, omega), "center": (cx, cy, zs)}]
+    return sc
+
+
 def c2_thermal_cube(L=46e-3, r=0.5e-3, seed=42):
     """Conduction only: cube of edge L, dx = 2r, rho 2700, cp 900, k 200,
     T = 300 + 500 exp(-|x-c|^2/(2 sigma^2)) + U(-1,1), sigma 5 mm; the first
