@@ -17,6 +17,13 @@ once yielding), all inputs resident in HBM.
 cylinder at dx 0.25 mm (1,320,424 particles) compressed by a rigid 40 x 40 mm
 plate of 2 x 64 x 64 triangles at -2 m/s, penalty stiffness 0.1 E/dx,
 Coulomb friction 0.2, the online-trained surrogate in the mixed run.
+--workload c5 runs the friction-stir-like plate: 120 x 60 x 6 mm of the JC
+aluminium at dx 0.25 mm (2,738,136 particles once the pin's volume is left
+out) under a rigid tool of 12,672 triangles (shoulder R 9 mm, pin R 3 mm x
+5 mm) turning at 1000 rpm and travelling at 2 mm/s, penalty 0.1 E/dx,
+Coulomb 0.2 with all of the sliding-friction power as heat, bottom layer at
+300 K, lists with a 0.4 dx skin rebuilt on the displacement of particles with
+eps_p > 0.05.
 
 Prints ONE JSON line: value = particle-updates/s (particles x steps / s),
 plus the HBM roofline of the dominant kernel (HIP-event durations on the
@@ -86,6 +93,12 @@ def workload_text(sc, n):
     common = ("mixed precision (FP32 pair loops summed in FP64, FP64 stress/Newton return "
               "map/integration) with the online-trained MLP Newton guess (BF16 MFMA), "
               "conduction + Taylor-Quinney 0.9")
+    if sc.name.startswith("c5"):
+        ne = sum(len(m["elnode"]) for m in sc.meshes)
+        return (f"{sc.name}: JC plate 120x60x6mm dx0.25mm, {n} particles, rigid tool of {ne} "
+                f"triangles (shoulder R9mm, pin R3mm x 5mm) at 1000 rpm and 2 mm/s, penalty "
+                f"0.1 E/dx, Coulomb 0.2 with friction heat, plastic-zone list rebuilds, "
+                f"{common}; compared with plain FP64")
     if sc.meshes:
         ne = sum(len(m["elnode"]) for m in sc.meshes)
         return (f"{sc.name}: JC cylinder R15mm H30mm dx0.25mm, {n} particles, rigid plate "
@@ -104,7 +117,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--scale", type=float, default=1.0)
-    ap.add_argument("--workload", choices=["c3", "c4"], default="c3")
+    ap.add_argument("--workload", choices=["c3", "c4", "c5"], default="c3")
     ap.add_argument("--no-error-floor", action="store_true",
                     help="skip the FP64 run with 1e-12 m position noise that measures how "
                          "far two FP64 runs drift apart on this workload")
@@ -120,6 +133,8 @@ def main():
     def make():
         if args.workload == "c4":
             return scenarios.c4_contact(scale=args.scale)
+        if args.workload == "c5":
+            return scenarios.c5_friction_stir(scale=args.scale, offset=(0.137, 0.291))
         return scenarios.c3_johnson_cook(scale=args.scale)
 
     def run(precision, timing_steps, surrogate, noise=0.0):

@@ -268,7 +268,7 @@ def gen_tool_mesh(center, zs, R_sh, R_pin, L_pin, n_theta=128, n_rs=20, n_zp=24,
 
 
 def c5_friction_stir(scale=1.0, r=0.125e-3, n_theta=128, n_rs=20, n_zp=24, n_rc=6, mu=0.2,
-                     skin=0.1, ep_hot=0.05, thick=1.0, offset=(0.0, 0.0)):
+                     skin=0.4, ep_hot=0.05, thick=1.0, offset=(0.0, 0.0)):
     """Friction-stir-like plate: 120 x 60 x 6 mm of Johnson-Cook aluminium,
     dx = 2r, and a rigid tool (shoulder R 9 mm, pin R 3 mm, pin 5 mm long,
     ~12k triangles at the default resolution; `scale` multiplies the plate
@@ -279,7 +279,10 @@ def c5_friction_stir(scale=1.0, r=0.125e-3, n_theta=128, n_rs=20, n_zp=24, n_rc=
     friction mu, all of the sliding-friction power as heat into the plate,
     Taylor-Quinney 0.9, bottom layer held at 300 K and the bottom two layers
     at v = 0; lists with a skin of `skin` dx rebuilt when a particle with
-    eps_p > ep_hot has moved half the skin.  `offset` shifts the tool axis by
+    eps_p > ep_hot has moved half the skin (on the h = 1.2 dx lattice any skin
+    from 0.05 to 0.43 dx lists the same 80 neighbours, since the next lattice
+    shell lies at sqrt(8) dx, so the widest such skin rebuilds least often).
+    `offset` shifts the tool axis by
     that many spacings in x and y; unequal shifts keep lattice points off the
     mesh's radial edges (a point on an edge makes the inside-triangle test a
     matter of rounding)."""
