@@ -54,7 +54,6 @@ struct wf_domain {
   int* perm_tmp = nullptr;
   int* gather = nullptr;
   int* nb_count = nullptr;
-  int* nb_list = nullptr;
   double* spare_d = nullptr;  // one plane, reorder scratch
   float* spare_f = nullptr;   // the same for the FP32 planes (mixed)
   int* spare_i = nullptr;
@@ -680,14 +679,6 @@ int wf_create(const wf_config* cfg, wf_domain** out) {
   }
   if (!rc) {
     void* q = nullptr;
-    if (hipMalloc(&q, sizeof(int) * d->npad * kListCap) != hipSuccess) rc = WF_E_NOMEM;
-    else {
-      d->owned.push_back(q);
-      d->nb_list = (int*)q;
-    }
-  }
-  if (!rc) {
-    void* q = nullptr;
     if (hipMalloc(&q, sizeof(double) * 6 * (int64_t)d->n) != hipSuccess) rc = WF_E_NOMEM;
     else {
       d->owned.push_back(q);
@@ -933,7 +924,7 @@ int wf_nsearch_build(wf_domain* d) {
   for (int c = 0; c < 3; c++) src[c] = stale ? d->f.xs[c] : d->f.x[c];
   const double radius = 2.0 * d->cfg.h_glob + (pz ? d->cfg.ns_skin : 0.0);
   int rc = ns_build(d->stream, &d->ns, src, d->n, d->npad, radius, pz ? kListCap : kMaxNb,
-                    d->gather, d->nb_count, d->nb_list, d->ctr, d->perm, d->perm_tmp, &d->tl,
+                    d->gather, d->nb_count, d->ctr, d->perm, d->perm_tmp, &d->tl,
                     &d->grid);
   if (rc)
     return fail(rc, rc == WF_E_NONFINITE ? "non-finite particle position"
@@ -1010,16 +1001,32 @@ int wf_get_neighbors(wf_domain* d, int* counts, int* offsets, int* list, int64_t
   if (!d || !counts || !offsets) return fail(WF_E_ARG, "null argument");
   if (!d->nb_built) return fail(WF_E_STATE, "neighbour lists not built");
   int n = d->n;
-  std::vector<int> cnt(n), perm(n), lst((size_t)d->npad * kListCap);
+  // rows from the tile lists: slot k of sorted particle s is a position in
+  // its tile's staged set tj[uoff[t] ..]
+  const int nt = (n + kTileP - 1) / kTileP;
+  std::vector<int> cnt(n), perm(n), uoff(nt + 1);
   HIPC(hipMemcpyAsync(cnt.data(), d->nb_count, sizeof(int) * n, hipMemcpyDeviceToHost, d->stream));
   HIPC(hipMemcpyAsync(perm.data(), d->perm, sizeof(int) * n, hipMemcpyDeviceToHost, d->stream));
-  HIPC(hipMemcpyAsync(lst.data(), d->nb_list, sizeof(int) * lst.size(), hipMemcpyDeviceToHost,
+  HIPC(hipMemcpyAsync(uoff.data(), d->tl.uoff, sizeof(int) * (nt + 1), hipMemcpyDeviceToHost,
                       d->stream));
+  HIPC(hipStreamSynchronize(d->stream));
+  std::vector<int> tj(uoff[nt]);
+  std::vector<uint2> slotq((size_t)d->tl.npad * kSlotQuads);
+  HIPC(hipMemcpyAsync(tj.data(), d->tl.tj, sizeof(int) * tj.size(), hipMemcpyDeviceToHost,
+                      d->stream));
+  HIPC(hipMemcpyAsync(slotq.data(), d->tl.slotq, sizeof(uint2) * slotq.size(),
+                      hipMemcpyDeviceToHost, d->stream));
   HIPC(hipStreamSynchronize(d->stream));
   std::vector<std::vector<int>> rows(n);
   for (int s = 0; s < n; s++) {
     auto& row = rows[perm[s]];
-    for (int k = 0; k < cnt[s]; k++) row.push_back(perm[lst[(size_t)k * d->npad + s]]);
+    const int base = uoff[s / kTileP];
+    for (int k = 0; k < cnt[s]; k++) {
+      const uint2 w = slotq[(size_t)(k >> 2) * d->tl.npad + s];
+      const unsigned h = (k & 2) ? w.y : w.x;
+      const int sl = (int)((k & 1) ? h >> 16 : h & 0xFFFFu);
+      row.push_back(perm[tj[base + sl]]);
+    }
     std::sort(row.begin(), row.end());
   }
   int64_t t = 0;

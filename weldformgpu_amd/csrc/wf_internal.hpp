@@ -295,7 +295,7 @@ struct NsResult {
 // perm: sorted slot -> original index of the current order; perm_tmp receives
 // the new one.
 int ns_build(hipStream_t s, NsWork** work, const double* const pos_src[3], int n, int64_t npad,
-             double radius, int cap, int* gather, int* count, int* list, Counters* ctr,
+             double radius, int cap, int* gather, int* count, Counters* ctr,
              int* perm, int* perm_tmp, TileLists* tiles, GridDesc* grid_out);
 void ns_free(NsWork* w);
 // exclusive scan of m ints into out[0..m] (out[m] = total); part needs

@@ -326,7 +326,7 @@ void ns_free(NsWork* w) {
 
 // gather: out slot -> old sorted index; perm (in/out): sorted slot -> original
 int ns_build(hipStream_t s, NsWork** work, const double* const pos_src[3], int n, int64_t npad,
-             double radius, int cap, int* gather, int* count, int* list, Counters* ctr,
+             double radius, int cap, int* gather, int* count, Counters* ctr,
              int* perm, int* perm_tmp, TileLists* tiles, GridDesc* grid_out) {
   if (!*work) *work = new NsWork();
   NsWork* w = *work;
@@ -414,7 +414,7 @@ int ns_build(hipStream_t s, NsWork** work, const double* const pos_src[3], int n
     w->tj_cap = cap;
   }
   launch_tile_lists(s, pfn, n, g, w->cell_start, w->tile_keys, w->tile_nc, w->tile_uoff, r2,
-                    cap, npad, count, list, w->slot, w->tile_j, w->tile_uc, &ctr->nb_over_cap);
+                    cap, npad, count, w->slot, w->tile_j, w->tile_uc, &ctr->nb_over_cap);
   if (hipGetLastError() != hipSuccess) return WF_E_HIP;
   *grid_out = g;
   tiles->count = count;

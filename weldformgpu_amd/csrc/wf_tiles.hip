@@ -8,10 +8,10 @@
 //     the tile's particles form the tile's staged set (tile_j, tile_uc
 //     entries) — on a cubic lattice with h = 1.2 dx about 740 of the ~1,230
 //     particles of the adjacent cells;
-//   - for each of its particles, the neighbour list in ascending index order,
-//     both as global indices (int32) and as 16-bit positions in the staged
-//     set, interleaved [k*npad + i] so a wavefront reads 128 contiguous bytes
-//     per list slot.
+//   - for each of its particles, the neighbour list in ascending staged
+//     order as 16-bit positions in the staged set, four per 8-byte word,
+//     interleaved [(k/4)*npad + i] so a wavefront reads contiguous words (the
+//     particle indices are tile_j of those positions).
 // The pair kernels write the staged set to LDS (position + index per slot)
 // and read a neighbour's position from LDS through its slot.
 #include <hip/hip_runtime.h>
@@ -33,22 +33,6 @@ __device__ __forceinline__ int morton3_t(int x, int y, int z) {
 __device__ __forceinline__ int key_of(const GridDesc& g, int cx, int cy, int cz) {
   int meta = ((cz >> 3) * g.mdim[1] + (cy >> 3)) * g.mdim[0] + (cx >> 3);
   return meta * 512 + morton3_t(cx & 7, cy & 7, cz & 7);
-}
-__device__ __forceinline__ void coords_of(const GridDesc& g, int key, int& cx, int& cy, int& cz) {
-  int meta = key >> 9, m = key & 511;
-  int x = 0, y = 0, z = 0;
-#pragma unroll
-  for (int b = 0; b < 3; b++) {
-    x |= ((m >> (3 * b)) & 1) << b;
-    y |= ((m >> (3 * b + 1)) & 1) << b;
-    z |= ((m >> (3 * b + 2)) & 1) << b;
-  }
-  int mx = meta % g.mdim[0];
-  int my = (meta / g.mdim[0]) % g.mdim[1];
-  int mz = meta / (g.mdim[0] * g.mdim[1]);
-  cx = mx * 8 + x;
-  cy = my * 8 + y;
-  cz = mz * 8 + z;
 }
 __device__ __forceinline__ void cell_of_t(const GridDesc& g, float px, float py, float pz, int& cx,
                                           int& cy, int& cz) {
@@ -158,23 +142,29 @@ __global__ __launch_bounds__(kTileP) void k_tile_cells(const float* __restrict__
 // adjacent-cell sets up to this size are compacted to the particles that
 // some list references (larger ones are staged whole)
 constexpr int kCompactMax = 4096;
+// candidate positions held in LDS at a time by the list builder
+constexpr int kListStage = 1024;
 
-// Neighbour lists in staged order (global index + 16-bit slot), and the
-// staged set tile_j / tile_uc.  Same float criterion as the reference
+// Neighbour lists in staged order (16-bit slots), and the staged set
+// tile_j / tile_uc.  Same float criterion as the reference
 // search: 0 < d2 < r2, d2 = dx*dx + dy*dy + dz*dz without contraction, at
-// most cap entries (70 as in the reference, 96 with a list skin).  The
-// compaction keeps the order of the adjacent-cell set, so every list stays
-// in ascending staged order.
+// most cap entries (70 as in the reference, 96 with a list skin).  Every
+// thread tests every particle of the tile's adjacent-cell set, in set order,
+// from positions staged in LDS: all lanes read the same candidate (an LDS
+// broadcast) and loop the same count.  A candidate outside the cells around
+// the particle's own cell is more than the 1.01 r cell width away, so it
+// never passes the test, and the lists are those of a scan over the 27 cells
+// around each particle.  The compaction keeps the order of the set, so
+// every list stays in ascending staged order.
 __global__ __launch_bounds__(kTileP) void k_tile_lists(
     const float* __restrict__ p0, const float* __restrict__ p1, const float* __restrict__ p2,
     int n, GridDesc g, const int* __restrict__ cell_start, const int* __restrict__ tile_keys,
     const int* __restrict__ tile_nc, const int* __restrict__ tile_uoff, float r2, int cap,
-    int64_t npad, int* __restrict__ count, int* __restrict__ list,
-    unsigned short* __restrict__ slot, int* __restrict__ tile_j, int* __restrict__ tile_uc,
-    int* over_cap) {
+    int64_t npad, int* __restrict__ count, unsigned short* __restrict__ slot,
+    int* __restrict__ tile_j, int* __restrict__ tile_uc, int* over_cap) {
 #pragma clang fp contract(off)
   __shared__ int sb[kTileMaxCells], sn[kTileMaxCells], spre[kTileMaxCells];
-  __shared__ int scx[kTileMaxCells], scy[kTileMaxCells], scz[kTileMaxCells];
+  __shared__ float4 spos[kListStage];  // candidate position, particle index in w
   __shared__ short sidx[kCompactMax];  // referenced flag, then compacted position (or -1)
   __shared__ int part[kTileP];
   __shared__ int s_all, s_used;
@@ -186,11 +176,6 @@ __global__ __launch_bounds__(kTileP) void k_tile_lists(
     int k = keys[p];
     sb[p] = cell_start[k];
     sn[p] = cell_start[k + 1] - cell_start[k];
-    int cx, cy, cz;
-    coords_of(g, k, cx, cy, cz);
-    scx[p] = cx;
-    scy[p] = cy;
-    scz[p] = cz;
   }
   __syncthreads();
   if (tid == 0) {
@@ -209,24 +194,45 @@ __global__ __launch_bounds__(kTileP) void k_tile_lists(
   __syncthreads();
   const int uoff = tile_uoff[t];
   const int i = t * kTileP + tid;
+  const bool act = i < n;
   int over = 0;
   int c = 0;
-  if (i < n) {
-    const float xi = p0[i], yi = p1[i], zi = p2[i];
-    int cx, cy, cz;
-    cell_of_t(g, xi, yi, zi, cx, cy, cz);
+  float xi = 0.f, yi = 0.f, zi = 0.f;
+  if (act) {
+    xi = p0[i];
+    yi = p1[i];
+    zi = p2[i];
+  }
+  for (int c0 = 0; c0 < U; c0 += kListStage) {
+    const int m = min(kListStage, U - c0);
+    __syncthreads();  // the previous chunk is consumed
     for (int p = 0; p < nc; p++) {
-      if (abs(scx[p] - cx) > 1 || abs(scy[p] - cy) > 1 || abs(scz[p] - cz) > 1) continue;
-      const int b = sb[p], e = b + sn[p], base = spre[p];
-      for (int j = b; j < e; j++) {
-        float ddx = p0[j] - xi, ddy = p1[j] - yi, ddz = p2[j] - zi;
-        float d2 = __fadd_rn(__fadd_rn(__fmul_rn(ddx, ddx), __fmul_rn(ddy, ddy)),
-                             __fmul_rn(ddz, ddz));
-        if (d2 < r2 && d2 > 0.0f) {
+      const int lo = max(spre[p], c0), hi = min(spre[p] + sn[p], c0 + m);
+      for (int q = lo + tid; q < hi; q += kTileP) {
+        const int j = sb[p] + (q - spre[p]);
+        spos[q - c0] = make_float4(p0[j], p1[j], p2[j], __int_as_float(j));
+      }
+    }
+    __syncthreads();
+    if (act) {
+      // eight candidates tested into a bit mask, then the (few) hits appended
+      for (int q0 = 0; q0 < m; q0 += 8) {
+        unsigned hit = 0u;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const float4 pj = spos[min(q0 + u, m - 1)];
+          float ddx = pj.x - xi, ddy = pj.y - yi, ddz = pj.z - zi;
+          float d2 = __fadd_rn(__fadd_rn(__fmul_rn(ddx, ddx), __fmul_rn(ddy, ddy)),
+                               __fmul_rn(ddz, ddz));
+          if (q0 + u < m && d2 < r2 && d2 > 0.0f) hit |= 1u << u;
+        }
+        while (hit) {
+          const int q = q0 + __builtin_ctz(hit);
+          hit &= hit - 1u;
           if (c < cap) {
-            list[(int64_t)c * npad + i] = j;
-            slot[(((int64_t)(c >> 2)) * npad + i) * 4 + (c & 3)] = (unsigned short)(base + (j - b));
-            if (compact) sidx[base + (j - b)] = 1;
+            const int sl = c0 + q;
+            slot[(((int64_t)(c >> 2)) * npad + i) * 4 + (c & 3)] = (unsigned short)sl;
+            if (compact) sidx[sl] = 1;
           } else {
             over = 1;
           }
@@ -234,8 +240,8 @@ __global__ __launch_bounds__(kTileP) void k_tile_lists(
         }
       }
     }
-    count[i] = c < cap ? c : cap;
   }
+  if (act) count[i] = c < cap ? c : cap;
   __syncthreads();
   int used = U;
   if (compact) {
@@ -309,11 +315,10 @@ void launch_tile_cells(hipStream_t s, const float* const pf[3], int n, const Gri
 void launch_tile_lists(hipStream_t s, const float* const pf[3], int n, const GridDesc& g,
                        const int* cell_start, const int* tile_keys, const int* tile_nc,
                        const int* tile_uoff, float r2, int cap, int64_t npad, int* count,
-                       int* list, unsigned short* slot, int* tile_j, int* tile_uc,
-                       int* over_cap) {
+                       unsigned short* slot, int* tile_j, int* tile_uc, int* over_cap) {
   int nt = (n + kTileP - 1) / kTileP;
   hipLaunchKernelGGL(k_tile_lists, dim3(nt), dim3(kTileP), 0, s, pf[0], pf[1], pf[2], n, g,
-                     cell_start, tile_keys, tile_nc, tile_uoff, r2, cap, npad, count, list, slot,
+                     cell_start, tile_keys, tile_nc, tile_uoff, r2, cap, npad, count, slot,
                      tile_j, tile_uc, over_cap);
 }
 

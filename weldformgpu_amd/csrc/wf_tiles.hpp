@@ -41,7 +41,6 @@ void launch_tile_cells(hipStream_t s, const float* const pf[3], int n, const Gri
 void launch_tile_lists(hipStream_t s, const float* const pf[3], int n, const GridDesc& g,
                        const int* cell_start, const int* tile_keys, const int* tile_nc,
                        const int* tile_uoff, float r2, int cap, int64_t npad, int* count,
-                       int* list, unsigned short* slot, int* tile_j, int* tile_uc,
-                       int* over_cap);
+                       unsigned short* slot, int* tile_j, int* tile_uc, int* over_cap);
 
 }  // namespace wf
