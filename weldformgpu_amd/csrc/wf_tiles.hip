@@ -48,6 +48,10 @@ constexpr int kSortN = 4096;  // >= kTileP * 27, power of two
 static_assert(kTileP * 27 <= kSortN, "sort buffer too small");
 
 // Per tile: sorted unique occupied neighbour cells and the staged count.
+// The tile's particles are in cell order, so its own cells are the runs of
+// equal keys; only those few cells (typically 10-20) are expanded to their
+// 27 neighbours, and the candidate keys are sorted in the smallest power of
+// two that holds them.
 __global__ __launch_bounds__(kTileP) void k_tile_cells(const float* __restrict__ p0,
                                                        const float* __restrict__ p1,
                                                        const float* __restrict__ p2, int n,
@@ -57,30 +61,58 @@ __global__ __launch_bounds__(kTileP) void k_tile_cells(const float* __restrict__
                                                        int* tile_ucnt, int* overflow) {
   __shared__ int keys[kSortN];
   __shared__ int segcnt[kTileP];
+  __shared__ int own[kTileP];   // the key of each particle's cell
+  __shared__ int wcnt[kTileP / 64 + 1];
   const int t = blockIdx.x;
   const int tid = threadIdx.x;
-  for (int s = tid; s < kSortN; s += kTileP) keys[s] = INT_MAX;
-  __syncthreads();
   const int i = t * kTileP + tid;
+  int cx = 0, cy = 0, cz = 0, kown = INT_MAX;
   if (i < n) {
-    int cx, cy, cz;
     cell_of_t(g, p0[i], p1[i], p2[i], cx, cy, cz);
-    int q = 0;
-    for (int dz = -1; dz <= 1; dz++)
-      for (int dy = -1; dy <= 1; dy++)
-        for (int dx = -1; dx <= 1; dx++, q++) {
-          int x = cx + dx, y = cy + dy, z = cz + dz;
-          if (x < 0 || y < 0 || z < 0 || x >= g.dim[0] || y >= g.dim[1] || z >= g.dim[2]) continue;
-          int k = key_of(g, x, y, z);
-          if (cell_start[k + 1] > cell_start[k]) keys[tid * 27 + q] = k;
-        }
+    kown = key_of(g, cx, cy, cz);
+  }
+  own[tid] = kown;
+  __syncthreads();
+  // first particle of each run of equal cells (keys ascend along the tile)
+  const bool first = kown != INT_MAX && (tid == 0 || own[tid - 1] != kown);
+  const unsigned long long bal = __ballot(first);
+  const int lane = lane_id(), w = tid >> 6;
+  if (lane == 0) wcnt[w] = __popcll(bal);
+  __syncthreads();
+  int wbase = 0;
+  for (int u = 0; u < w; u++) wbase += wcnt[u];
+  int nown = 0;
+  for (int u = 0; u < kTileP / 64; u++) nown += wcnt[u];
+  const int rank = wbase + __popcll(bal & ((1ull << lane) - 1ull));
+  // the coordinates of the tile's unique cells, in key order
+  __shared__ int ocx[kTileP], ocy[kTileP], ocz[kTileP];
+  if (first) {
+    ocx[rank] = cx;
+    ocy[rank] = cy;
+    ocz[rank] = cz;
   }
   __syncthreads();
-  // bitonic sort of kSortN keys
-  for (int size = 2; size <= kSortN; size <<= 1) {
+  const int total = 27 * nown;
+  int S = 2;
+  while (S < total) S <<= 1;
+  for (int q = tid; q < S; q += kTileP) {
+    int k = INT_MAX;
+    if (q < total) {
+      const int o = q / 27, r = q - 27 * o;
+      const int x = ocx[o] + (r % 3) - 1, y = ocy[o] + ((r / 3) % 3) - 1, z = ocz[o] + r / 9 - 1;
+      if (x >= 0 && y >= 0 && z >= 0 && x < g.dim[0] && y < g.dim[1] && z < g.dim[2]) {
+        const int kk = key_of(g, x, y, z);
+        if (cell_start[kk + 1] > cell_start[kk]) k = kk;
+      }
+    }
+    keys[q] = k;
+  }
+  __syncthreads();
+  // bitonic sort of the S candidate keys
+  for (int size = 2; size <= S; size <<= 1) {
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int s = tid; s < kSortN / 2; s += kTileP) {
-        int lo = 2 * s - (s & (stride - 1));
+      for (int s2 = tid; s2 < S / 2; s2 += kTileP) {
+        int lo = 2 * s2 - (s2 & (stride - 1));
         int hi = lo + stride;
         bool up = ((lo & size) == 0);
         int a = keys[lo], b = keys[hi];
@@ -92,10 +124,11 @@ __global__ __launch_bounds__(kTileP) void k_tile_cells(const float* __restrict__
       __syncthreads();
     }
   }
-  // unique: each thread owns a segment of kSortN/kTileP entries
-  constexpr int seg = kSortN / kTileP;
+  // unique: each thread owns a segment of ceil(S/kTileP) entries
+  const int seg = (S + kTileP - 1) / kTileP;
+  const int sb0 = min(tid * seg, S), sb1 = min(sb0 + seg, S);
   int c = 0;
-  for (int s = tid * seg; s < tid * seg + seg; s++) {
+  for (int s = sb0; s < sb1; s++) {
     int k = keys[s];
     if (k != INT_MAX && (s == 0 || keys[s - 1] != k)) c++;
   }
@@ -114,7 +147,7 @@ __global__ __launch_bounds__(kTileP) void k_tile_cells(const float* __restrict__
   __syncthreads();
   int pos = segcnt[tid];
   int* out = tile_keys + (int64_t)t * kTileMaxCells;
-  for (int s = tid * seg; s < tid * seg + seg; s++) {
+  for (int s = sb0; s < sb1; s++) {
     int k = keys[s];
     if (k != INT_MAX && (s == 0 || keys[s - 1] != k)) {
       if (pos < kTileMaxCells) out[pos] = k;
