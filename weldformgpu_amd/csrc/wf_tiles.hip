@@ -189,23 +189,27 @@ constexpr int kListStage = 1024;
 // never passes the test, and the lists are those of a scan over the 27 cells
 // around each particle.  The compaction keeps the order of the set, so
 // every list stays in ascending staged order.
-__global__ __launch_bounds__(kTileP) void k_tile_lists(
+constexpr int kListThreads = 2 * kTileP;  // two threads per particle
+
+__global__ __launch_bounds__(kListThreads) void k_tile_lists(
     const float* __restrict__ p0, const float* __restrict__ p1, const float* __restrict__ p2,
     int n, GridDesc g, const int* __restrict__ cell_start, const int* __restrict__ tile_keys,
     const int* __restrict__ tile_nc, const int* __restrict__ tile_uoff, float r2, int cap,
     int64_t npad, int* __restrict__ count, unsigned short* __restrict__ slot,
     int* __restrict__ tile_j, int* __restrict__ tile_uc, int* over_cap) {
 #pragma clang fp contract(off)
+  constexpr int NT = kListThreads;
   __shared__ int sb[kTileMaxCells], sn[kTileMaxCells], spre[kTileMaxCells];
   __shared__ float4 spos[kListStage];  // candidate position, particle index in w
   __shared__ short sidx[kCompactMax];  // referenced flag, then compacted position (or -1)
-  __shared__ int part[kTileP];
+  __shared__ int part[NT];
   __shared__ int s_all, s_used;
   const int t = blockIdx.x;
   const int tid = threadIdx.x;
+  const int sub = tid & 1;  // lanes 2p and 2p+1 hold particle p
   const int nc = min(tile_nc[t], kTileMaxCells);
   const int* keys = tile_keys + (int64_t)t * kTileMaxCells;
-  for (int p = tid; p < nc; p += kTileP) {
+  for (int p = tid; p < nc; p += NT) {
     int k = keys[p];
     sb[p] = cell_start[k];
     sn[p] = cell_start[k + 1] - cell_start[k];
@@ -223,13 +227,12 @@ __global__ __launch_bounds__(kTileP) void k_tile_lists(
   const int U = s_all;
   const bool compact = U <= kCompactMax;
   if (compact)
-    for (int s = tid; s < U; s += kTileP) sidx[s] = 0;
+    for (int s = tid; s < U; s += NT) sidx[s] = 0;
   __syncthreads();
   const int uoff = tile_uoff[t];
-  const int i = t * kTileP + tid;
+  const int i = t * kTileP + (tid >> 1);
   const bool act = i < n;
-  int over = 0;
-  int c = 0;
+  int c = 0;  // the particle's list length so far (the same in both lanes)
   float xi = 0.f, yi = 0.f, zi = 0.f;
   if (act) {
     xi = p0[i];
@@ -241,45 +244,51 @@ __global__ __launch_bounds__(kTileP) void k_tile_lists(
     __syncthreads();  // the previous chunk is consumed
     for (int p = 0; p < nc; p++) {
       const int lo = max(spre[p], c0), hi = min(spre[p] + sn[p], c0 + m);
-      for (int q = lo + tid; q < hi; q += kTileP) {
+      for (int q = lo + tid; q < hi; q += NT) {
         const int j = sb[p] + (q - spre[p]);
         spos[q - c0] = make_float4(p0[j], p1[j], p2[j], __int_as_float(j));
       }
     }
     __syncthreads();
     if (act) {
-      // eight candidates tested into a bit mask, then the (few) hits appended
-      for (int q0 = 0; q0 < m; q0 += 8) {
-        unsigned hit = 0u;
+      // sixteen candidates at a time: each lane tests eight into a bit mask,
+      // the two masks are swapped, and each lane appends its own hits at the
+      // positions the combined mask gives them, in candidate order
+      for (int q0 = 0; q0 < m; q0 += 16) {
+        const int qb = q0 + 8 * sub;
+        unsigned mine = 0u;
 #pragma unroll
         for (int u = 0; u < 8; u++) {
-          const float4 pj = spos[min(q0 + u, m - 1)];
+          const float4 pj = spos[min(qb + u, m - 1)];
           float ddx = pj.x - xi, ddy = pj.y - yi, ddz = pj.z - zi;
           float d2 = __fadd_rn(__fadd_rn(__fmul_rn(ddx, ddx), __fmul_rn(ddy, ddy)),
                                __fmul_rn(ddz, ddz));
-          if (q0 + u < m && d2 < r2 && d2 > 0.0f) hit |= 1u << u;
+          if (qb + u < m && d2 < r2 && d2 > 0.0f) mine |= 1u << u;
         }
-        while (hit) {
-          const int q = q0 + __builtin_ctz(hit);
-          hit &= hit - 1u;
-          if (c < cap) {
+        const unsigned other = (unsigned)__shfl_xor((int)mine, 1);
+        const int nlo = __popc(sub ? other : mine);  // hits among the first eight
+        int k = c + (sub ? nlo : 0);
+        unsigned h = mine;
+        while (h) {
+          const int q = qb + __builtin_ctz(h);
+          h &= h - 1u;
+          if (k < cap) {
             const int sl = c0 + q;
-            slot[(((int64_t)(c >> 2)) * npad + i) * 4 + (c & 3)] = (unsigned short)sl;
+            slot[(((int64_t)(k >> 2)) * npad + i) * 4 + (k & 3)] = (unsigned short)sl;
             if (compact) sidx[sl] = 1;
-          } else {
-            over = 1;
           }
-          c++;
+          k++;
         }
+        c += __popc(mine) + __popc(other);
       }
     }
   }
-  if (act) count[i] = c < cap ? c : cap;
-  __syncthreads();
+  if (act && sub == 0) count[i] = c < cap ? c : cap;
+  __syncthreads();  // every slot write (either lane's) is visible
   int used = U;
   if (compact) {
     // exclusive scan of the flags, one contiguous segment per thread
-    const int seg = (U + kTileP - 1) / kTileP;
+    const int seg = (U + NT - 1) / NT;
     const int s0 = min(tid * seg, U), s1 = min(s0 + seg, U);
     int sum = 0;
     for (int s = s0; s < s1; s++) sum += sidx[s];
@@ -287,7 +296,7 @@ __global__ __launch_bounds__(kTileP) void k_tile_lists(
     __syncthreads();
     if (tid == 0) {
       int run = 0;
-      for (int u = 0; u < kTileP; u++) {
+      for (int u = 0; u < NT; u++) {
         const int v = part[u];
         part[u] = run;
         run += v;
@@ -303,36 +312,38 @@ __global__ __launch_bounds__(kTileP) void k_tile_lists(
     }
     __syncthreads();
     used = s_used;
-    // this thread's slots (written above by this thread) to compacted
-    // positions, four quads of four slots in flight at a time
-    const int m = min(c, cap);
+    // the particle's slots to compacted positions; lane `sub` takes the
+    // quads q with q % 2 == sub, four of them in flight at a time
+    const int m = act ? min(c, cap) : 0;
     const int nq = (m + 3) >> 2;
     uint2* qrow = reinterpret_cast<uint2*>(slot) + i;
-    for (int q0 = 0; q0 < nq; q0 += 4) {
+    for (int q0 = sub; q0 < nq; q0 += 8) {
       uint2 w[4];
 #pragma unroll
       for (int u = 0; u < 4; u++)
-        if (q0 + u < nq) w[u] = qrow[(int64_t)(q0 + u) * npad];
+        if (q0 + 2 * u < nq) w[u] = qrow[(int64_t)(q0 + 2 * u) * npad];
 #pragma unroll
       for (int u = 0; u < 4; u++) {
-        if (q0 + u >= nq) continue;
-        const int k = 4 * (q0 + u);  // slots past m keep what they hold
+        const int qq = q0 + 2 * u;
+        if (qq >= nq) continue;
+        const int k = 4 * qq;  // slots past m keep what they hold
         unsigned lo = w[u].x, hi = w[u].y;
         if (k < m) lo = (lo & 0xFFFF0000u) | (unsigned short)sidx[lo & 0xFFFFu];
         if (k + 1 < m) lo = (lo & 0xFFFFu) | ((unsigned)(unsigned short)sidx[lo >> 16] << 16);
         if (k + 2 < m) hi = (hi & 0xFFFF0000u) | (unsigned short)sidx[hi & 0xFFFFu];
         if (k + 3 < m) hi = (hi & 0xFFFFu) | ((unsigned)(unsigned short)sidx[hi >> 16] << 16);
-        qrow[(int64_t)(q0 + u) * npad] = make_uint2(lo, hi);
+        qrow[(int64_t)qq * npad] = make_uint2(lo, hi);
       }
     }
   }
   for (int p = 0; p < nc; p++)
-    for (int r = tid; r < sn[p]; r += kTileP) {
+    for (int r = tid; r < sn[p]; r += NT) {
       const int s = spre[p] + r;
       const int d = compact ? sidx[s] : s;
       if (d >= 0) tile_j[uoff + d] = sb[p] + r;
     }
   if (tid == 0) tile_uc[t] = used;
+  const int over = act && sub == 0 && c > cap;  // one count per particle over the cap
   unsigned long long o = wave_sum_u64((unsigned long long)over);
   if (lane_id() == 0 && o) atomicAdd(over_cap, (int)o);
 }
@@ -350,7 +361,7 @@ void launch_tile_lists(hipStream_t s, const float* const pf[3], int n, const Gri
                        const int* tile_uoff, float r2, int cap, int64_t npad, int* count,
                        unsigned short* slot, int* tile_j, int* tile_uc, int* over_cap) {
   int nt = (n + kTileP - 1) / kTileP;
-  hipLaunchKernelGGL(k_tile_lists, dim3(nt), dim3(kTileP), 0, s, pf[0], pf[1], pf[2], n, g,
+  hipLaunchKernelGGL(k_tile_lists, dim3(nt), dim3(kListThreads), 0, s, pf[0], pf[1], pf[2], n, g,
                      cell_start, tile_keys, tile_nc, tile_uoff, r2, cap, npad, count, slot,
                      tile_j, tile_uc, over_cap);
 }
