@@ -107,6 +107,9 @@ __global__ __launch_bounds__(256) void k_contact_nb(Fields f, int solid, MeshDev
                                                    const int* __restrict__ sorted,
                                                    ContactLists cl, int id_free_surf,
                                                    Counters* ctr) {
+  // the thread's candidates, kept sorted in LDS while they are collected
+  __shared__ int L[kMaxContNb * 256];
+  const int tl = threadIdx.x;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= solid) return;
   int c = 0, over = 0;
@@ -126,19 +129,20 @@ __global__ __launch_bounds__(256) void k_contact_nb(Fields f, int solid, MeshDev
             if (!(sqrt(ex * ex + ey * ey + ez * ez) < r)) continue;
             if (c == kMaxContNb) {
               over++;
-              if (e > cl.list[(int64_t)(c - 1) * cl.npad + i]) continue;
+              if (e > L[(c - 1) * 256 + tl]) continue;
               c--;  // drop the largest index
             }
             int p = c;
-            while (p > 0 && cl.list[(int64_t)(p - 1) * cl.npad + i] > e) {
-              cl.list[(int64_t)p * cl.npad + i] = cl.list[(int64_t)(p - 1) * cl.npad + i];
+            while (p > 0 && L[(p - 1) * 256 + tl] > e) {
+              L[p * 256 + tl] = L[(p - 1) * 256 + tl];
               p--;
             }
-            cl.list[(int64_t)p * cl.npad + i] = e;
+            L[p * 256 + tl] = e;
             c++;
           }
         }
   }
+  for (int k = 0; k < c; k++) cl.list[(int64_t)k * cl.npad + i] = L[k * 256 + tl];
   cl.count[i] = c;
   if (over) atomicAdd(&ctr->cont_over_cap, 1);
 }
