@@ -5,7 +5,8 @@ whole steps of a plate compressing a Johnson-Cook cylinder (FP64 and mixed
 precision) with that heat feeding the conduction pass.
 
 Tolerances: mesh geometry 1e-14 absolute on O(1e-2) coordinates; surface
-IDs and candidate lists bit-exact; single-call accelerations and contact
+IDs and candidate lists bit-exact (in both precisions; the mixed surface pass
+sums FP32 terms, its normals within 1e-6); single-call accelerations and contact
 forces relative L2 <= 1e-12; FP64 steps relative L2 <= 1e-9 (the pair-loop
 step tolerance); mixed-precision steps <= 1e-4 against the FP64 oracle.
 """
@@ -82,6 +83,22 @@ def test_surface_ids_and_contact_lists(oracle, wflib):
     assert np.array_equal(cnt, st.contact_arrays["cnb_count"])
     assert np.array_equal(lst, st.contact_arrays["cnb_list"])
     assert cnt.max() > 0 and s["cont_over_cap"] == st.ct.cont_over_cap == 0
+
+
+def test_surface_ids_mixed_precision(oracle, wflib):
+    sc = _plate_case()
+    sc.cfg.precision = _abi.WF_MIXED
+    _case("surface mixed", sc)
+    st = to_oracle(oracle, sc)
+    st.nsearch(sc.cfg)
+    st.calc_surface(sc.cfg)
+    with to_gpu(sc, wflib) as d:
+        d.nsearch()
+        d.CalculateSurface()
+        ID, nrm = d.get("ID"), d.get("normal")
+    assert np.array_equal(ID, st["ID"])
+    assert (ID == 1).sum() > 0
+    assert rel_l2(nrm, st.contact_arrays["normal"]) < 1e-6
 
 
 def test_contact_list_overflow_keeps_smallest_indices(oracle, wflib):

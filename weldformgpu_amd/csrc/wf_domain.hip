@@ -80,6 +80,18 @@ struct wf_domain {
   bool plastic_zone() const { return cfg.rebuild_policy == WF_REBUILD_PLASTIC_ZONE; }
   // with a list skin the surface pass counts only pairs within 2 h_glob
   double surface_r2cut() const { return plastic_zone() ? 4.0 * cfg.h_glob * cfg.h_glob : 0.0; }
+  // the free-surface pass in the domain's precision
+  void launch_surface() {
+    const double ntot = (double)n + (double)mesh.ne;
+    if (mixed) {
+      refresh_mirrors();
+      launch_surface_mixed(stream, f, tl, solid, ntot, totmass, cfg.id_free_surf,
+                           surface_r2cut(), uniform_m, m_value, frame.cs32);
+    } else {
+      launch_surface_tiled(stream, f, tl, solid, ntot, totmass, cfg.id_free_surf,
+                           surface_r2cut(), uniform_m, m_value);
+    }
+  }
   RetArgs ret_args() const {
     return RetArgs{mlp_w, (cfg.surrogate && mlp_trained) ? 1 : 0, (long long)step};
   }
@@ -1319,9 +1331,7 @@ int wf_calc_surface(wf_domain* d) {
   NEED_NB(d);
   int rc = d->surface_init();
   if (rc) return rc;
-  launch_surface_tiled(d->stream, d->f, d->tl, d->solid, (double)d->n + (double)d->mesh.ne,
-                       d->totmass, d->cfg.id_free_surf, d->surface_r2cut(), d->uniform_m,
-                       d->m_value);
+  d->launch_surface();
   HIPC(hipGetLastError());
   return WF_OK;
 }
@@ -1456,9 +1466,7 @@ int wf_step(wf_domain* d, int n_steps) {
       if (d->cfg.contact) {  // SolverFraser.cu:233-257
         timer_begin(d);
         if ((rc = d->surface_init())) return rc;
-        launch_surface_tiled(d->stream, d->f, d->tl, d->solid, (double)d->n + (double)d->mesh.ne,
-                             d->totmass, d->cfg.id_free_surf, d->surface_r2cut(), d->uniform_m,
-                             d->m_value);
+        d->launch_surface();
         if (d->n_meshes > 0) {
           if (d->ts_i == 0 && (rc = d->contact_nb())) return rc;
           launch_contact_force(d->stream, d->f, d->solid, d->mesh, d->cl, d->contact_params());

@@ -128,6 +128,10 @@ void launch_surface_lock(hipStream_t s, const Fields& f, int n);
 void launch_surface_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
                           double scale_num, double totmass, int id_free_surf, double r2cut,
                           bool um, double m_value);
+// mixed precision: FP32 tile-frame positions (cs32 the FP32 cell size)
+void launch_surface_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
+                          double scale_num, double totmass, int id_free_surf, double r2cut,
+                          bool um, double m_value, float cs32);
 void launch_contact_nb(hipStream_t s, const Fields& f, int solid, const MeshDev& m,
                        const TriGrid& g, const int* start, const int* sorted,
                        const ContactLists& cl, int id_free_surf, Counters* ctr);

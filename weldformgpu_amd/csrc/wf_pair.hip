@@ -1099,6 +1099,83 @@ struct SurfD {
 
 static inline int ntiles(int n) { return (n + kTileP - 1) / kTileP; }
 
+// the same in mixed precision: FP32 tile-frame positions, each FP32 term
+// m_j x_ij summed in FP64 and the flag decided in FP64 from those sums (a
+// flag can differ from the FP64 pass only for |n| within ~1e-7 of 0.25 h)
+template <bool UM>
+struct SurfG {
+  WF_GATHER32
+  static constexpr int NBATCH = 4;
+  struct Prm {
+    int limit;
+    int id_free_surf;
+    double ntot, totmass;
+    float r2cut;  // > 0: lists carry a skin; count only pairs within the radius
+    float m_uni;  // every particle's mass (UM)
+    float cs;
+  };
+  struct Own {
+    float x, y, z;
+  };
+  struct Nb {
+    float m;
+  };
+  struct Acc {
+    KSum nx, ny, nz;
+    int cnt;
+  };
+  __device__ static void merge(Acc& a, const Acc& b) {
+    a.nx.add(b.nx); a.ny.add(b.ny); a.nz.add(b.nz);
+    a.cnt += b.cnt;
+  }
+  __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx32& c) {
+    a.nx.zero(); a.ny.zero(); a.nz.zero();
+    a.cnt = 0;
+    if (!act) return;
+    pos32(f, i, c, o.x, o.y, o.z);
+  }
+  __device__ static void load(const Fields& f, int j, Nb& nb) {
+    nb.m = UM ? 0.f : (float)f.m[j];
+  }
+  __device__ static void pair(float x, float y, float z, const Nb& nb, const Own& o, Acc& a,
+                              const Prm& p) {
+    const float dx = o.x - x, dy = o.y - y, dz = o.z - z;
+    if (p.r2cut > 0.f && !(dx * dx + dy * dy + dz * dz < p.r2cut)) return;
+    const float mj = UM ? p.m_uni : nb.m;
+    a.nx.add(mj * dx);
+    a.ny.add(mj * dy);
+    a.nz.add(mj * dz);
+    a.cnt++;
+  }
+  __device__ static void finish(const Fields& f, int i, bool act, const Own&, const Acc& a,
+                                const Prm& p) {
+    if (!act) return;
+    const double sc = p.ntot / (p.totmass * (double)a.cnt);
+    const double nx = a.nx.val() * sc, ny = a.ny.val() * sc, nz = a.nz.val() * sc;
+    f.nrm[0][i] = nx;
+    f.nrm[1][i] = ny;
+    f.nrm[2][i] = nz;
+    if (sqrt(nx * nx + ny * ny + nz * nz) >= 0.25 * f.h[i] && a.cnt <= 46 && !f.idlock[i])
+      f.ID[i] = p.id_free_surf;
+  }
+};
+
+void launch_surface_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
+                          double ntot, double totmass, int id_free_surf, double r2cut, bool um,
+                          double m_value, float cs32) {
+  if (solid <= 0) return;
+  const float r2 = (float)r2cut;
+  if (um) {
+    const SurfG<true>::Prm p{solid, id_free_surf, ntot, totmass, r2, (float)m_value, cs32};
+    hipLaunchKernelGGL(k_pair_gather<SurfG<true>>, dim3(ntiles(solid)),
+                       dim3(kTileP * SurfG<true>::LANES), 0, s, f, tl, p);
+  } else {
+    const SurfG<false>::Prm p{solid, id_free_surf, ntot, totmass, r2, 0.f, cs32};
+    hipLaunchKernelGGL(k_pair_gather<SurfG<false>>, dim3(ntiles(solid)),
+                       dim3(kTileP * SurfG<false>::LANES), 0, s, f, tl, p);
+  }
+}
+
 void launch_surface_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
                           double ntot, double totmass, int id_free_surf, double r2cut, bool um,
                           double m_value) {
