@@ -90,6 +90,25 @@ def test_extension_options(tmp_path):
     assert "plasticZone" in _dry(str(p), expect_rc=2)
 
 
+def test_rotating_rigid_body(tmp_path):
+    """A rigid body turning about its own axis: the angular velocity comes
+    from the valueAng of its zone's boundary condition (SetMeshVel) and the
+    centre from RigidBodies[].rotationCenter (an extension; the reference
+    turns bodies about the origin)."""
+    d = _deck("contact_mini.json")
+    s = _dry(os.path.join(DECKS, "contact_mini.json"))
+    assert s["mesh_rotation"] == [[0, 0, 0, 0, 0, 0]]
+    zone = d["RigidBodies"][0]["zoneId"]
+    for b in d["BoundaryConditions"]:
+        if b["zoneId"] == zone:
+            b["valueAng"] = [0.0, 0.0, 104.72]
+    d["RigidBodies"][0]["rotationCenter"] = [0.001, -0.002, 0.0]
+    p = tmp_path / "rot.json"
+    p.write_text(json.dumps(d))
+    s = _dry(str(p))
+    assert s["mesh_rotation"] == [[0, 0, 104.72, 0.001, -0.002, 0.0]]
+
+
 def test_bad_decks_fail_with_a_message(tmp_path):
     p = tmp_path / "broken.json"
     p.write_text('{"Configuration": {"particleRadius": 0.001,, }}')

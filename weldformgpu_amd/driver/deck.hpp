@@ -27,6 +27,9 @@ struct Bc {
 struct Rigid {
   std::string type, file;
   double start[3] = {0, 0, 0}, dim[3] = {0, 0, 0}, move[3] = {0, 0, 0};
+  // rotation centre of the body's angular velocity (an extension the
+  // reference ignores; without it the body turns about the origin)
+  double center[3] = {0, 0, 0};
   int dens = 10, zone = 0;
 };
 
@@ -106,6 +109,7 @@ inline Deck parse_deck(const wfj::Value& j) {
     read(rb["zoneId"], r.zone);
     read(rb["fileName"], r.file);
     wfj::read3(rb["moveDir"], r.move);
+    wfj::read3(rb["rotationCenter"], r.center);
     d.rigid.push_back(r);
   }
   const wfj::Value& ct = j["Contact"][0];
@@ -159,7 +163,7 @@ struct Mesh {
   std::vector<double> nodes;
   std::vector<int> el;
   int zone = 0;
-  double vel[3] = {0, 0, 0}, omega[3] = {0, 0, 0};
+  double vel[3] = {0, 0, 0}, omega[3] = {0, 0, 0}, center[3] = {0, 0, 0};
 };
 
 struct Setup {
@@ -264,6 +268,7 @@ inline Setup build_setup(const Deck& d) {
   for (const Rigid& r : d.rigid) {
     Mesh me;
     me.zone = r.zone;
+    for (int k = 0; k < 3; k++) me.center[k] = r.center[k];
     int nn = 0, ne = 0;
     if (r.type == "Plane") {
       const double p2[3] = {r.start[0] + r.dim[0], r.start[1] + r.dim[1], r.start[2]};

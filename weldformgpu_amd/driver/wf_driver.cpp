@@ -125,6 +125,11 @@ void print_setup(const wfd::Deck& dk, const wfd::Setup& s) {
     std::printf("%s[%d, %zu, %zu, %.17g, %.17g, %.17g]", m ? ", " : "", s.meshes[m].zone,
                 s.meshes[m].nodes.size() / 3, s.meshes[m].el.size() / 3, s.meshes[m].vel[0],
                 s.meshes[m].vel[1], s.meshes[m].vel[2]);
+  std::printf("], \"mesh_rotation\": [");  // angular velocity and rotation centre
+  for (size_t m = 0; m < s.meshes.size(); m++)
+    std::printf("%s[%.17g, %.17g, %.17g, %.17g, %.17g, %.17g]", m ? ", " : "",
+                s.meshes[m].omega[0], s.meshes[m].omega[1], s.meshes[m].omega[2],
+                s.meshes[m].center[0], s.meshes[m].center[1], s.meshes[m].center[2]);
   std::printf("]}\n");
 }
 
@@ -201,7 +206,7 @@ int main(int argc, char** argv) {
     int mi = 0;
     const int nn = (int)(me.nodes.size() / 3), ne = (int)(me.el.size() / 3);
     check(wf_add_mesh(d, me.nodes.data(), nn, me.el.data(), ne, me.zone, &mi), "wf_add_mesh");
-    check(wf_set_mesh_motion(d, mi, me.vel, me.omega, nullptr), "wf_set_mesh_motion");
+    check(wf_set_mesh_motion(d, mi, me.vel, me.omega, me.center), "wf_set_mesh_motion");
     MeshRows r;
     r.zone = me.zone;
     r.el = me.el;
