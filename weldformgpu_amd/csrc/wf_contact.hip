@@ -18,6 +18,7 @@
 // centroids, normals and plane constants follow TriMesh_d::Move /
 // CalcCentroids / CalcNormals / UpdatePlaneCoeff (Mesh.cu:223-276).
 #include <algorithm>
+#include <climits>
 
 #include "wf_device.hpp"
 
@@ -102,49 +103,122 @@ __global__ void k_tri_place(MeshDev m, const int* start, const int* rank, int* s
 // within 2 h_i (Contact.cu:40-55), ascending element index; candidates past
 // kMaxContNb are counted and the smallest indices kept (the reference writes
 // slot 20 before its overflow check)
-__global__ __launch_bounds__(256) void k_contact_nb(Fields f, int solid, MeshDev m, TriGrid g,
-                                                   const int* __restrict__ start,
-                                                   const int* __restrict__ sorted,
-                                                   ContactLists cl, int id_free_surf,
-                                                   Counters* ctr) {
-  // the thread's candidates, kept sorted in LDS while they are collected
-  __shared__ int L[kMaxContNb * 256];
-  const int tl = threadIdx.x;
+// Contact candidates in two passes.  Pass 1 clears every particle's count and
+// appends the particles that need candidates (free-surface ID, within a cell
+// of the triangle grid) to `act` (act[0] the length, the indices from
+// act[1]), in no particular order: pass 2 writes only each particle's own row.
+__global__ __launch_bounds__(256) void k_contact_active(Fields f, int solid, TriGrid g,
+                                                       ContactLists cl, int id_free_surf,
+                                                       int* __restrict__ act) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= solid) return;
-  int c = 0, over = 0;
-  int cx, cy, cz;
-  if (f.ID[i] == id_free_surf && tri_cell(g, f.x[0][i], f.x[1][i], f.x[2][i], cx, cy, cz)) {
-    const double xi = f.x[0][i], yi = f.x[1][i], zi = f.x[2][i], r = 2. * f.h[i];
-    for (int dz = -1; dz <= 1; dz++)
-      for (int dy = -1; dy <= 1; dy++)
-        for (int dx = -1; dx <= 1; dx++) {
-          const int x = cx + dx, y = cy + dy, z = cz + dz;
-          if (x < 0 || y < 0 || z < 0 || x >= g.dim[0] || y >= g.dim[1] || z >= g.dim[2])
-            continue;
-          const int cell = (z * g.dim[1] + y) * g.dim[0] + x;
-          for (int s = start[cell]; s < start[cell + 1]; s++) {
-            const int e = sorted[s];
-            const double ex = xi - m.c[0][e], ey = yi - m.c[1][e], ez = zi - m.c[2][e];
-            if (!(sqrt(ex * ex + ey * ey + ez * ez) < r)) continue;
-            if (c == kMaxContNb) {
-              over++;
-              if (e > L[(c - 1) * 256 + tl]) continue;
-              c--;  // drop the largest index
-            }
-            int p = c;
-            while (p > 0 && L[(p - 1) * 256 + tl] > e) {
-              L[p * 256 + tl] = L[(p - 1) * 256 + tl];
-              p--;
-            }
-            L[p * 256 + tl] = e;
-            c++;
-          }
-        }
+  bool need = false;
+  if (i < solid) {
+    int cx, cy, cz;
+    need = f.ID[i] == id_free_surf && tri_cell(g, f.x[0][i], f.x[1][i], f.x[2][i], cx, cy, cz);
+    cl.count[i] = 0;
   }
-  for (int k = 0; k < c; k++) cl.list[(int64_t)k * cl.npad + i] = L[k * 256 + tl];
-  cl.count[i] = c;
-  if (over) atomicAdd(&ctr->cont_over_cap, 1);
+  const unsigned long long b = __ballot(need);
+  if (!b) return;
+  const int lane = lane_id();
+  const int lead = __ffsll((long long)b) - 1;
+  int base = 0;
+  if (lane == lead) base = atomicAdd(act, __popcll(b));
+  base = __shfl(base, lead);
+  if (need) act[1 + base + __popcll(b & ((1ull << lane) - 1ull))] = i;
+}
+
+// Orders this wave's LDS writes before its following LDS reads (the wave's
+// own operations only; other waves of the block are not waited for).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Pass 2 (CalcContactNb, Contact.cu:29-62): one wave per listed particle.
+// Lanes 0..26 read the 3x3x3 cells around the particle's cell; the wave then
+// tests their triangles 64 at a time (lane k of a batch takes the k-th
+// triangle of the cells' concatenation) against sqrt(d2) < 2 h in FP64, and
+// merges the hits into the particle's list, kept as the kMaxContNb smallest
+// element indices in ascending order (lanes 0..L-1 hold it in a register).
+// A merged value's position is its rank in the union of the list and the
+// batch's hits (indices are distinct: a triangle lies in one cell).  A
+// particle with more than kMaxContNb hits is counted once in cont_over_cap.
+constexpr int kCnbWaves = 4;
+__global__ __launch_bounds__(64 * kCnbWaves) void k_contact_nb(Fields f, MeshDev m, TriGrid g,
+                                                              const int* __restrict__ start,
+                                                              const int* __restrict__ sorted,
+                                                              ContactLists cl,
+                                                              const int* __restrict__ act,
+                                                              Counters* ctr) {
+  __shared__ int s_pre[kCnbWaves][28];  // first flat position of each cell
+  __shared__ int s_beg[kCnbWaves][27];  // its first entry in `sorted`
+  __shared__ int s_top[kCnbWaves][kMaxContNb];
+  const int w = threadIdx.x >> 6, lane = lane_id();
+  const int nact = act[0];
+  const int nw = gridDim.x * kCnbWaves;
+  for (int a = blockIdx.x * kCnbWaves + w; a < nact; a += nw) {
+    const int i = act[1 + a];
+    const double xi = f.x[0][i], yi = f.x[1][i], zi = f.x[2][i], r = 2. * f.h[i];
+    int cx, cy, cz;
+    tri_cell(g, xi, yi, zi, cx, cy, cz);  // inside (pass 1 listed it)
+    int beg = 0, cnt = 0;
+    if (lane < 27) {
+      const int x = cx + lane % 3 - 1, y = cy + (lane / 3) % 3 - 1, z = cz + lane / 9 - 1;
+      if (x >= 0 && y >= 0 && z >= 0 && x < g.dim[0] && y < g.dim[1] && z < g.dim[2]) {
+        const int cell = (z * g.dim[1] + y) * g.dim[0] + x;
+        beg = start[cell];
+        cnt = start[cell + 1] - beg;
+      }
+    }
+    int incl = cnt;  // inclusive prefix over lanes 0..26 (lanes above 26 add 0)
+    for (int d = 1; d < 32; d <<= 1) {
+      const int o = __shfl_up(incl, d);
+      if (lane >= d) incl += o;
+    }
+    if (lane < 27) {
+      s_pre[w][lane + 1] = incl;
+      s_beg[w][lane] = beg;
+    }
+    if (lane == 0) s_pre[w][0] = 0;
+    const int total = __shfl(incl, 26);
+    wave_lds_sync();
+    int L = 0, hits = 0;
+    int tq = INT_MAX;  // lane q < L: the list's q-th entry
+    for (int base = 0; base < total; base += 64) {
+      const int k = base + lane;
+      int e = INT_MAX;
+      bool hit = false;
+      if (k < total) {
+        int sg = 0;
+        for (int q = 1; q < 27; q++) sg += k >= s_pre[w][q];
+        e = sorted[s_beg[w][sg] + (k - s_pre[w][sg])];
+        const double ex = xi - m.c[0][e], ey = yi - m.c[1][e], ez = zi - m.c[2][e];
+        hit = sqrt(ex * ex + ey * ey + ez * ez) < r;
+      }
+      const unsigned long long hm = __ballot(hit);
+      if (!hm) continue;
+      int below_e = 0, below_t = 0;  // hits smaller than my hit / my list entry
+      for (unsigned long long mm = hm; mm; mm &= mm - 1ull) {
+        const int v = __shfl(e, __ffsll((long long)mm) - 1);
+        below_e += v < e;
+        below_t += v < tq;
+      }
+      for (int q = 0; q < L; q++) below_e += __shfl(tq, q) < e;
+      if (hit && below_e < kMaxContNb) s_top[w][below_e] = e;
+      if (lane < L && lane + below_t < kMaxContNb) s_top[w][lane + below_t] = tq;
+      wave_lds_sync();
+      const int nh = __popcll(hm);
+      L = min(L + nh, kMaxContNb);
+      hits += nh;
+      tq = lane < L ? s_top[w][lane] : INT_MAX;
+      wave_lds_sync();
+    }
+    if (lane < L) cl.list[(int64_t)lane * cl.npad + i] = tq;
+    if (lane == 0) {
+      cl.count[i] = L;
+      if (hits > kMaxContNb) atomicAdd(&ctr->cont_over_cap, 1);
+    }
+  }
 }
 
 // penalty contact + Coulomb friction (Contact.cu:134-321)
@@ -305,10 +379,14 @@ void launch_surface_lock(hipStream_t s, const Fields& f, int n) {
 }
 void launch_contact_nb(hipStream_t s, const Fields& f, int solid, const MeshDev& m,
                        const TriGrid& g, const int* start, const int* sorted,
-                       const ContactLists& cl, int id_free_surf, Counters* ctr) {
-  if (solid <= 0) return;
-  hipLaunchKernelGGL(k_contact_nb, dim3(gblk(solid)), dim3(256), 0, s, f, solid, m, g, start,
-                     sorted, cl, id_free_surf, ctr);
+                       const ContactLists& cl, int id_free_surf, int* act, Counters* ctr) {
+  if (solid <= 0) return;  // act[0] was cleared by the caller
+  hipLaunchKernelGGL(k_contact_active, dim3(gblk(solid)), dim3(256), 0, s, f, solid, g, cl,
+                     id_free_surf, act);
+  // enough waves to fill the chip; each loops over the listed particles
+  const int blocks = std::max(1, std::min(2048, gblk(solid)));
+  hipLaunchKernelGGL(k_contact_nb, dim3(blocks), dim3(64 * kCnbWaves), 0, s, f, m, g, start,
+                     sorted, cl, (const int*)act, ctr);
 }
 void launch_contact_force(hipStream_t s, const Fields& f, int solid, const MeshDev& m,
                           const ContactLists& cl, const ContactParams& cp) {

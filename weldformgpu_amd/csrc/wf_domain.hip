@@ -126,6 +126,7 @@ struct wf_domain {
   int mesh_zone[kMaxMeshes] = {0};
   std::vector<void*> mesh_owned;
   ContactLists cl{};
+  int* cnb_act = nullptr;  // particles needing candidates: length, then indices
   bool cont_built = false;
   bool surf_init = false;  // not_write_surf_ID and total mass taken
   double totmass = 0.0;
@@ -516,7 +517,9 @@ int wf_domain::contact_nb() {
   hipLaunchKernelGGL(k_fill_i32v, dim3(gridn(cells)), dim3(256), 0, stream, tri_cnt, 0,
                      (int)cells);
   launch_tri_bin(stream, mesh, g, tri_cnt, tri_rank, tri_start, tri_sorted, tri_scan_part);
-  launch_contact_nb(stream, f, solid, mesh, g, tri_start, tri_sorted, cl, cfg.id_free_surf, ctr);
+  hipLaunchKernelGGL(k_fill_i32v, dim3(1), dim3(64), 0, stream, cnb_act, 0, 1);
+  launch_contact_nb(stream, f, solid, mesh, g, tri_start, tri_sorted, cl, cfg.id_free_surf,
+                    cnb_act, ctr);
   HIPC(hipGetLastError());
   cont_built = true;
   return WF_OK;
@@ -1245,7 +1248,10 @@ int wf_add_mesh(wf_domain* d, const double* nodes, int nn, const int* el, int ne
   if (!d->cl.count) {
     int rc = d->alloc_plane_i(&d->cl.count);
     if (!rc) rc = d->dev_alloc((void**)&d->cl.list, sizeof(int) * d->npad * kMaxContNb);
+    if (!rc) rc = d->dev_alloc((void**)&d->cnb_act, sizeof(int) * (d->npad + 1));
     if (rc) {
+      // all three or none (what was allocated is freed with the domain)
+      d->cl.count = d->cl.list = d->cnb_act = nullptr;
       for (void* q : fresh) hipFree(q);
       return fail(rc, "contact list allocation failed");
     }

@@ -134,7 +134,7 @@ void launch_surface_mixed(hipStream_t s, const Fields& f, const TileLists& tl, i
                           bool um, double m_value, float cs32);
 void launch_contact_nb(hipStream_t s, const Fields& f, int solid, const MeshDev& m,
                        const TriGrid& g, const int* start, const int* sorted,
-                       const ContactLists& cl, int id_free_surf, Counters* ctr);
+                       const ContactLists& cl, int id_free_surf, int* act, Counters* ctr);
 void launch_contact_force(hipStream_t s, const Fields& f, int solid, const MeshDev& m,
                           const ContactLists& cl, const ContactParams& cp);
 // |cf_i| where ID_i == id, else 0
