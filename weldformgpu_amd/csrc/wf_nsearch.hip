@@ -35,7 +35,6 @@ struct NsWork {
   int* key = nullptr;
   int* rank = nullptr;
   int* slot_orig = nullptr;  // key-sorted: original index (for ordering)
-  int* slot_old = nullptr;   // key-sorted: old sorted index
   int* cell_count = nullptr;
   int* cell_start = nullptr;  // key_cap + 1
   int* scan_part = nullptr;
@@ -98,12 +97,25 @@ __global__ __launch_bounds__(256) void k_ns_minmax_part(const float* p0, const f
   if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
-__global__ void k_ns_minmax_final(const float* part, int nb, float* mm) {
-  int c = threadIdx.x;
-  if (c >= 6) return;
-  float v = c < 3 ? INFINITY : -INFINITY;
-  for (int b = 0; b < nb; b++) v = c < 3 ? fminf(v, part[b * 6 + c]) : fmaxf(v, part[b * 6 + c]);
-  mm[c] = v;
+__global__ __launch_bounds__(256) void k_ns_minmax_final(const float* part, int nb, float* mm) {
+  __shared__ float sh[6][256];
+  float v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  for (int b = threadIdx.x; b < nb; b += 256)
+    for (int c = 0; c < 3; c++) {
+      v[c] = fminf(v[c], part[b * 6 + c]);
+      v[3 + c] = fmaxf(v[3 + c], part[b * 6 + 3 + c]);
+    }
+  for (int c = 0; c < 6; c++) sh[c][threadIdx.x] = v[c];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st)
+      for (int c = 0; c < 3; c++) {
+        sh[c][threadIdx.x] = fminf(sh[c][threadIdx.x], sh[c][threadIdx.x + st]);
+        sh[3 + c][threadIdx.x] = fmaxf(sh[3 + c][threadIdx.x], sh[3 + c][threadIdx.x + st]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) mm[threadIdx.x] = sh[threadIdx.x][0];
 }
 
 using Grid = GridDesc;
@@ -134,15 +146,39 @@ __global__ void k_ns_fill_i32(int* a, int v, int64_t n) {
   if (i < n) a[i] = v;
 }
 
+// Cell key of each particle and a rank unique within its cell.  The
+// particles are still in the last rebuild's cell order, so neighbouring lanes
+// mostly share a cell: each run of equal keys within a wave (active lanes are
+// a prefix of the wave) takes its ranks with one atomic by its first lane.
+// Which rank a particle gets does not matter: k_ns_order orders each cell by
+// original index.
 __global__ void k_ns_keys(const float* p0, const float* p1, const float* p2, int n, Grid g,
                           int* key, int* rank, int* cell_count) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int cx, cy, cz;
-  cell_of(g, p0[i], p1[i], p2[i], cx, cy, cz);
-  int k = cell_key(g, cx, cy, cz);
-  key[i] = k;
-  rank[i] = atomicAdd(&cell_count[k], 1);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = i < n;
+  int k = -1;
+  if (act) {
+    int cx, cy, cz;
+    cell_of(g, p0[i], p1[i], p2[i], cx, cy, cz);
+    k = cell_key(g, cx, cy, cz);
+    key[i] = k;
+  }
+  const int lane = lane_id();
+  const int kp = __shfl_up(k, 1);
+  const bool head = act && (lane == 0 || kp != k);
+  const unsigned long long H = __ballot(head);
+  const unsigned long long A = __ballot(act);
+  const unsigned long long upto = lane == 63 ? ~0ull : (2ull << lane) - 1ull;
+  const unsigned long long hb = H & upto;
+  const int h = hb ? 63 - __clzll((long long)hb) : 0;  // my run's first lane
+  int base = 0;
+  if (head) {
+    const unsigned long long above = lane == 63 ? 0ull : H >> (lane + 1);
+    const int next = above ? lane + 1 + __builtin_ctzll(above) : __popcll(A);
+    base = atomicAdd(&cell_count[k], next - lane);
+  }
+  base = __shfl(base, h);
+  if (act) rank[i] = base + (lane - h);
 }
 
 // exclusive scan over int (length m), three phases
@@ -186,31 +222,27 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_apply(const int* in, int64_
 }
 
 __global__ void k_ns_place(const int* key, const int* rank, const int* cell_start,
-                           const int* perm, int n, int* slot_orig, int* slot_old) {
+                           const int* perm, int n, int* slot_orig) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  int s = cell_start[key[i]] + rank[i];
-  slot_orig[s] = perm[i];
-  slot_old[s] = i;
+  slot_orig[cell_start[key[i]] + rank[i]] = perm[i];
 }
 
-// order each cell's particles by original index (one thread per cell)
-__global__ void k_ns_cell_sort(const int* cell_start, int64_t ncell, int* slot_orig,
-                               int* slot_old) {
-  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell) return;
-  int b = cell_start[c], e = cell_start[c + 1];
-  for (int s = b + 1; s < e; s++) {
-    int ko = slot_orig[s], kd = slot_old[s];
-    int t = s - 1;
-    while (t >= b && slot_orig[t] > ko) {
-      slot_orig[t + 1] = slot_orig[t];
-      slot_old[t + 1] = slot_old[t];
-      t--;
-    }
-    slot_orig[t + 1] = ko;
-    slot_old[t + 1] = kd;
-  }
+// Final slot of each particle: its cell's start plus the number of particles
+// of the cell with a smaller original index, so a cell lists its particles by
+// original index and the order is deterministic.  Writes the new permutation
+// (slot -> original index) and the gather map (slot -> current slot).
+__global__ void k_ns_order(const int* key, const int* cell_start, const int* perm,
+                           const int* slot_orig, int n, int* perm_new, int* gather) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int k = key[i];
+  const int b = cell_start[k], e = cell_start[k + 1];
+  const int o = perm[i];
+  int r = 0;
+  for (int t = b; t < e; t++) r += slot_orig[t] < o;
+  perm_new[b + r] = o;
+  gather[b + r] = i;
 }
 
 __global__ void k_ns_gather_f32(float* dst, const float* src, const int* idx, int n) {
@@ -237,7 +269,6 @@ static int ensure(NsWork* w, int n, int64_t nkeys) {
     hipFree(w->key);
     hipFree(w->rank);
     hipFree(w->slot_orig);
-    hipFree(w->slot_old);
     int cap = std::max(n, 64);
     for (int c = 0; c < 3; c++) {
       if (hipMalloc(&w->pf[c], sizeof(float) * cap) != hipSuccess) return WF_E_NOMEM;
@@ -246,7 +277,6 @@ static int ensure(NsWork* w, int n, int64_t nkeys) {
     if (hipMalloc(&w->key, sizeof(int) * cap) != hipSuccess) return WF_E_NOMEM;
     if (hipMalloc(&w->rank, sizeof(int) * cap) != hipSuccess) return WF_E_NOMEM;
     if (hipMalloc(&w->slot_orig, sizeof(int) * cap) != hipSuccess) return WF_E_NOMEM;
-    if (hipMalloc(&w->slot_old, sizeof(int) * cap) != hipSuccess) return WF_E_NOMEM;
     w->n_cap = cap;
   }
   if (nkeys > w->key_cap) {
@@ -304,7 +334,6 @@ void ns_free(NsWork* w) {
   hipFree(w->key);
   hipFree(w->rank);
   hipFree(w->slot_orig);
-  hipFree(w->slot_old);
   hipFree(w->cell_count);
   hipFree(w->cell_start);
   hipFree(w->scan_part);
@@ -339,7 +368,7 @@ int ns_build(hipStream_t s, NsWork** work, const double* const pos_src[3], int n
   int nbm = std::min(kMmBlocks, std::max(1, grid1(n, 256)));
   hipLaunchKernelGGL(k_ns_minmax_part, dim3(nbm), dim3(256), 0, s, w->pf[0], w->pf[1], w->pf[2],
                      n, w->mm_part);
-  hipLaunchKernelGGL(k_ns_minmax_final, dim3(1), dim3(64), 0, s, w->mm_part, nbm, w->mm);
+  hipLaunchKernelGGL(k_ns_minmax_final, dim3(1), dim3(256), 0, s, w->mm_part, nbm, w->mm);
   if (hipMemcpyAsync(w->mm_host, w->mm, 6 * sizeof(float), hipMemcpyDeviceToHost, s) != hipSuccess)
     return WF_E_HIP;
   if (hipStreamSynchronize(s) != hipSuccess) return WF_E_HIP;
@@ -372,16 +401,10 @@ int ns_build(hipStream_t s, NsWork** work, const double* const pos_src[3], int n
   hipLaunchKernelGGL(k_scan_apply, dim3(nsb), dim3(kScanBlock), 0, s, w->cell_count, nkeys,
                      w->scan_part, w->cell_start);
   hipLaunchKernelGGL(k_ns_place, dim3(grid1(n, 256)), dim3(256), 0, s, w->key, w->rank,
-                     w->cell_start, perm, n, w->slot_orig, w->slot_old);
-  hipLaunchKernelGGL(k_ns_cell_sort, dim3(grid1(nkeys, 256)), dim3(256), 0, s, w->cell_start,
-                     nkeys, w->slot_orig, w->slot_old);
+                     w->cell_start, perm, n, w->slot_orig);
   // new permutation and gather map
-  if (hipMemcpyAsync(perm_tmp, w->slot_orig, sizeof(int) * n, hipMemcpyDeviceToDevice, s) !=
-      hipSuccess)
-    return WF_E_HIP;
-  if (hipMemcpyAsync(gather, w->slot_old, sizeof(int) * n, hipMemcpyDeviceToDevice, s) !=
-      hipSuccess)
-    return WF_E_HIP;
+  hipLaunchKernelGGL(k_ns_order, dim3(grid1(n, 256)), dim3(256), 0, s, w->key, w->cell_start,
+                     perm, w->slot_orig, n, perm_tmp, gather);
   for (int c = 0; c < 3; c++)
     hipLaunchKernelGGL(k_ns_gather_f32, dim3(grid1(n, 256)), dim3(256), 0, s, w->pfn[c], w->pf[c],
                        gather, n);

@@ -191,6 +191,42 @@ constexpr int kListStage = 1024;
 // every list stays in ascending staged order.
 constexpr int kListThreads = 2 * kTileP;  // two threads per particle
 
+// Exclusive prefix of v over the kListThreads threads of the block (wave
+// prefixes by shuffles, then the wave totals through LDS); `total` receives
+// the block sum.  Every thread of the block must call it.
+__device__ __forceinline__ int block_excl_scan(int v, int* wsum, int& total) {
+  constexpr int NW = kListThreads / 64;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int incl = v;
+  for (int d = 1; d < 64; d <<= 1) {
+    const int o = __shfl_up(incl, d);
+    if (lane >= d) incl += o;
+  }
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  int off = 0;
+  total = 0;
+  for (int u = 0; u < NW; u++) {
+    const int t = wsum[u];
+    off += u < w ? t : 0;
+    total += t;
+  }
+  __syncthreads();  // wsum may be written again
+  return off + incl - v;
+}
+
+// The cell holding staged position q: the last p with spre[p] <= q (an empty
+// cell shares its prefix with the next one, so the cell found is never empty).
+__device__ __forceinline__ int cell_of(const int* spre, int nc, int q) {
+  int lo = 0, hi = nc - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (spre[mid] <= q) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 __global__ __launch_bounds__(kListThreads) void k_tile_lists(
     const float* __restrict__ p0, const float* __restrict__ p1, const float* __restrict__ p2,
     int n, GridDesc g, const int* __restrict__ cell_start, const int* __restrict__ tile_keys,
@@ -202,29 +238,29 @@ __global__ __launch_bounds__(kListThreads) void k_tile_lists(
   __shared__ int sb[kTileMaxCells], sn[kTileMaxCells], spre[kTileMaxCells];
   __shared__ float4 spos[kListStage];  // candidate position, particle index in w
   __shared__ short sidx[kCompactMax];  // referenced flag, then compacted position (or -1)
-  __shared__ int part[NT];
-  __shared__ int s_all, s_used;
+  __shared__ int wsum[NT / 64];
   const int t = blockIdx.x;
   const int tid = threadIdx.x;
   const int sub = tid & 1;  // lanes 2p and 2p+1 hold particle p
   const int nc = min(tile_nc[t], kTileMaxCells);
   const int* keys = tile_keys + (int64_t)t * kTileMaxCells;
-  for (int p = tid; p < nc; p += NT) {
-    int k = keys[p];
-    sb[p] = cell_start[k];
-    sn[p] = cell_start[k + 1] - cell_start[k];
+  // cell ranges and their prefix: thread tid takes cells [tid*K, tid*K+K)
+  const int K = (nc + NT - 1) / NT;  // at most kTileMaxCells / NT
+  const int p0c = min(tid * K, nc), p1c = min(p0c + K, nc);
+  int mine = 0;
+  for (int p = p0c; p < p1c; p++) {
+    const int k = keys[p];
+    const int b = cell_start[k];
+    sb[p] = b;
+    sn[p] = cell_start[k + 1] - b;
+    mine += sn[p];
   }
-  __syncthreads();
-  if (tid == 0) {
-    int run = 0;
-    for (int p = 0; p < nc; p++) {
-      spre[p] = run;
-      run += sn[p];
-    }
-    s_all = run;
+  int U;
+  int run0 = block_excl_scan(mine, wsum, U);
+  for (int p = p0c; p < p1c; p++) {
+    spre[p] = run0;
+    run0 += sn[p];
   }
-  __syncthreads();
-  const int U = s_all;
   const bool compact = U <= kCompactMax;
   if (compact)
     for (int s = tid; s < U; s += NT) sidx[s] = 0;
@@ -241,13 +277,11 @@ __global__ __launch_bounds__(kListThreads) void k_tile_lists(
   }
   for (int c0 = 0; c0 < U; c0 += kListStage) {
     const int m = min(kListStage, U - c0);
-    __syncthreads();  // the previous chunk is consumed
-    for (int p = 0; p < nc; p++) {
-      const int lo = max(spre[p], c0), hi = min(spre[p] + sn[p], c0 + m);
-      for (int q = lo + tid; q < hi; q += NT) {
-        const int j = sb[p] + (q - spre[p]);
-        spos[q - c0] = make_float4(p0[j], p1[j], p2[j], __int_as_float(j));
-      }
+    __syncthreads();  // the previous chunk is consumed (and spre is complete)
+    for (int q = tid; q < m; q += NT) {
+      const int p = cell_of(spre, nc, c0 + q);
+      const int j = sb[p] + (c0 + q - spre[p]);
+      spos[q] = make_float4(p0[j], p1[j], p2[j], __int_as_float(j));
     }
     __syncthreads();
     if (act) {
@@ -292,26 +326,13 @@ __global__ __launch_bounds__(kListThreads) void k_tile_lists(
     const int s0 = min(tid * seg, U), s1 = min(s0 + seg, U);
     int sum = 0;
     for (int s = s0; s < s1; s++) sum += sidx[s];
-    part[tid] = sum;
-    __syncthreads();
-    if (tid == 0) {
-      int run = 0;
-      for (int u = 0; u < NT; u++) {
-        const int v = part[u];
-        part[u] = run;
-        run += v;
-      }
-      s_used = run;
-    }
-    __syncthreads();
-    int run = part[tid];
+    int run = block_excl_scan(sum, wsum, used);
     for (int s = s0; s < s1; s++) {
       const int fl = sidx[s];
       sidx[s] = fl ? run : -1;
       run += fl;
     }
     __syncthreads();
-    used = s_used;
     // the particle's slots to compacted positions; lane `sub` takes the
     // quads q with q % 2 == sub, four of them in flight at a time
     const int m = act ? min(c, cap) : 0;
@@ -336,12 +357,13 @@ __global__ __launch_bounds__(kListThreads) void k_tile_lists(
       }
     }
   }
-  for (int p = 0; p < nc; p++)
-    for (int r = tid; r < sn[p]; r += NT) {
-      const int s = spre[p] + r;
-      const int d = compact ? sidx[s] : s;
-      if (d >= 0) tile_j[uoff + d] = sb[p] + r;
+  for (int s = tid; s < U; s += NT) {
+    const int d = compact ? sidx[s] : s;
+    if (d >= 0) {
+      const int p = cell_of(spre, nc, s);
+      tile_j[uoff + d] = sb[p] + (s - spre[p]);
     }
+  }
   if (tid == 0) tile_uc[t] = used;
   const int over = act && sub == 0 && c > cap;  // one count per particle over the cap
   unsigned long long o = wave_sum_u64((unsigned long long)over);
