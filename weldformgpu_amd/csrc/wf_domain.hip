@@ -43,6 +43,8 @@ struct PlaneRef {
 
 }  // namespace
 
+constexpr int kGatherBatch = 8;  // planes per reorder gather launch
+
 struct wf_domain {
   wf_config cfg;
   int n = 0, solid = 0;
@@ -54,8 +56,8 @@ struct wf_domain {
   int* perm_tmp = nullptr;
   int* gather = nullptr;
   int* nb_count = nullptr;
-  double* spare_d = nullptr;  // one plane, reorder scratch
-  float* spare_f = nullptr;   // the same for the FP32 planes (mixed)
+  double* spare_d[kGatherBatch] = {};  // reorder scratch planes
+  float* spare_f[kGatherBatch] = {};   // the same for the FP32 planes
   int* spare_i = nullptr;
   void* stage = nullptr;      // transfer staging (6*n doubles)
   Counters* ctr = nullptr;
@@ -126,6 +128,7 @@ struct wf_domain {
   int mesh_zone[kMaxMeshes] = {0};
   std::vector<void*> mesh_owned;
   ContactLists cl{};
+  int* cl_list_alt = nullptr;  // the candidate rows of the next order (reorder)
   int* cnb_act = nullptr;  // particles needing candidates: length, then indices
   bool cont_built = false;
   bool surf_init = false;  // not_write_surf_ID and total mass taken
@@ -372,9 +375,37 @@ __global__ void k_fill_i32v(int* a, int v, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = v;
 }
-__global__ void k_gather_f32(float* dst, const float* src, const int* idx, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = src[idx[i]];
+template <class T>
+struct GatherSet {
+  const T* src[kGatherBatch];
+  T* dst[kGatherBatch];
+  int count;
+};
+// up to kGatherBatch planes into the new order in one pass: the map is read
+// once and a particle's loads from the planes are independent
+template <class T>
+__global__ void k_gather_batch(GatherSet<T> g, const int* __restrict__ idx, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int j = idx[i];
+  T v[kGatherBatch];
+#pragma unroll
+  for (int q = 0; q < kGatherBatch; q++)
+    if (q < g.count) v[q] = g.src[q][j];
+#pragma unroll
+  for (int q = 0; q < kGatherBatch; q++)
+    if (q < g.count) g.dst[q][i] = v[q];
+}
+// the contact candidate rows into the new order, only the entries below each
+// particle's count (the rest are never read)
+__global__ void k_gather_cont_rows(int* __restrict__ dst, const int* __restrict__ src,
+                                   const int* __restrict__ cnt, const int* __restrict__ idx,
+                                   int n, int64_t npad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int j = idx[i];
+  const int c = cnt[j];
+  for (int k = 0; k < c; k++) dst[k * npad + i] = src[k * npad + j];
 }
 // splitmix64, for the deterministic initial weights
 uint64_t splitmix64(uint64_t& x) {
@@ -537,6 +568,23 @@ ContactParams wf_domain::contact_params() const {
 }
 
 // ============================================================================
+// the planes into the new order, kGatherBatch per launch; each gathered plane
+// swaps places with a scratch plane
+template <class T>
+static void gather_planes(wf_domain* d, const std::vector<T**>& planes, T** spares) {
+  for (size_t b = 0; b < planes.size(); b += kGatherBatch) {
+    GatherSet<T> g{};
+    g.count = (int)std::min<size_t>(kGatherBatch, planes.size() - b);
+    for (int q = 0; q < g.count; q++) {
+      g.src[q] = *planes[b + q];
+      g.dst[q] = spares[q];
+    }
+    hipLaunchKernelGGL(k_gather_batch<T>, dim3(gridn(d->n)), dim3(256), 0, d->stream, g,
+                       d->gather, d->n);
+    for (int q = 0; q < g.count; q++) std::swap(*planes[b + q], spares[q]);
+  }
+}
+
 extern "C" {
 
 void wf_config_default(wf_config* c) {
@@ -616,7 +664,6 @@ int wf_create(const wf_config* cfg, wf_domain** out) {
     if (rc) break;
   }
   if (d->mixed) {
-    if (!rc) rc = d->alloc_plane_f(&d->spare_f);
     if (!rc) rc = d->alloc_plane_f(&d->f.h32);
     if (!rc) rc = d->dev_alloc((void**)&d->f.p4, sizeof(float4) * d->npad);
     float4** f4[] = {&d->f.v4, &d->f.v4_alt};
@@ -677,7 +724,8 @@ int wf_create(const wf_config* cfg, wf_domain** out) {
   }
   if (!rc) rc = d->alloc_plane_d(&d->f.rho_alt);
   if (!rc) rc = d->alloc_plane_d(&d->f.T_alt);
-  if (!rc) rc = d->alloc_plane_d(&d->spare_d);
+  for (int q = 0; q < kGatherBatch && !rc; q++) rc = d->alloc_plane_d(&d->spare_d[q]);
+  for (int q = 0; q < kGatherBatch && !rc; q++) rc = d->alloc_plane_f(&d->spare_f[q]);
   if (!rc) rc = d->alloc_plane_i(&d->spare_i);
   if (!rc) rc = d->alloc_plane_i(&d->perm);
   if (!rc) rc = d->alloc_plane_i(&d->perm_tmp);
@@ -946,39 +994,32 @@ int wf_nsearch_build(wf_domain* d) {
                     : rc == WF_E_NBCAP   ? "a neighbour tile exceeds its cell/slot budget"
                                          : "neighbour search failed");
   // physically reorder every field into the new order
+  std::vector<double**> pd;
+  std::vector<float**> pf;
   for (PlaneRef r : d->all_planes()) {
     if (r.d) {
-      launch_gather_f64(d->stream, d->spare_d, *r.d, d->gather, d->n);
-      std::swap(*r.d, d->spare_d);
+      pd.push_back(r.d);
     } else if (r.f) {
-      hipLaunchKernelGGL(k_gather_f32, dim3(gridn(d->n)), dim3(256), 0, d->stream, d->spare_f,
-                         *r.f, d->gather, d->n);
-      std::swap(*r.f, d->spare_f);
+      pf.push_back(r.f);
     } else {
       launch_gather_i32(d->stream, d->spare_i, *r.i, d->gather, d->n);
       std::swap(*r.i, d->spare_i);
     }
   }
-  for (int k = 0; k < 5; k++) {
-    float*& pl = k < 4 ? d->f.mlx[k] : d->f.mly;
-    float* tmp = (float*)d->spare_d;  // a double plane holds a float plane
-    hipLaunchKernelGGL(k_gather_f32, dim3(gridn(d->n)), dim3(256), 0, d->stream, tmp, pl,
-                       d->gather, d->n);
-    HIPC(hipMemcpyAsync(pl, tmp, sizeof(float) * d->n, hipMemcpyDeviceToDevice, d->stream));
-  }
+  for (int k = 0; k < 4; k++) pf.push_back(&d->f.mlx[k]);
+  pf.push_back(&d->f.mly);
+  gather_planes(d, pd, d->spare_d);
+  gather_planes(d, pf, d->spare_f);
   launch_gather_i32(d->stream, d->spare_i, d->f.mlstep, d->gather, d->n);
   std::swap(d->f.mlstep, d->spare_i);
   launch_gather_i32(d->stream, d->spare_i, d->f.idlock, d->gather, d->n);
   std::swap(d->f.idlock, d->spare_i);
   if (d->cl.count) {  // contact candidates follow their particles
+    hipLaunchKernelGGL(k_gather_cont_rows, dim3(gridn(d->n)), dim3(256), 0, d->stream,
+                       d->cl_list_alt, d->cl.list, d->cl.count, d->gather, d->n, d->npad);
+    std::swap(d->cl.list, d->cl_list_alt);
     launch_gather_i32(d->stream, d->spare_i, d->cl.count, d->gather, d->n);
     std::swap(d->cl.count, d->spare_i);
-    for (int k = 0; k < kMaxContNb; k++) {
-      int* row = d->cl.list + (int64_t)k * d->npad;
-      launch_gather_i32(d->stream, d->spare_i, row, d->gather, d->n);
-      HIPC(hipMemcpyAsync(row, d->spare_i, sizeof(int) * d->n, hipMemcpyDeviceToDevice,
-                          d->stream));
-    }
   }
   std::swap(d->perm, d->perm_tmp);
   if (pz)
@@ -1248,10 +1289,11 @@ int wf_add_mesh(wf_domain* d, const double* nodes, int nn, const int* el, int ne
   if (!d->cl.count) {
     int rc = d->alloc_plane_i(&d->cl.count);
     if (!rc) rc = d->dev_alloc((void**)&d->cl.list, sizeof(int) * d->npad * kMaxContNb);
+    if (!rc) rc = d->dev_alloc((void**)&d->cl_list_alt, sizeof(int) * d->npad * kMaxContNb);
     if (!rc) rc = d->dev_alloc((void**)&d->cnb_act, sizeof(int) * (d->npad + 1));
     if (rc) {
-      // all three or none (what was allocated is freed with the domain)
-      d->cl.count = d->cl.list = d->cnb_act = nullptr;
+      // all or none (what was allocated is freed with the domain)
+      d->cl.count = d->cl.list = d->cl_list_alt = d->cnb_act = nullptr;
       for (void* q : fresh) hipFree(q);
       return fail(rc, "contact list allocation failed");
     }
@@ -1398,9 +1440,9 @@ int wf_get_contact_neighbors(wf_domain* d, int* counts, int* list) {
 
 int wf_contact_force_sum(wf_domain* d, double* sum) {
   if (!d || !sum) return fail(WF_E_ARG, "null argument");
-  launch_contact_norm(d->stream, d->f, d->n, d->cfg.id_free_surf, d->spare_d);
+  launch_contact_norm(d->stream, d->f, d->n, d->cfg.id_free_surf, d->spare_d[0]);
   const int nb = std::min(1024, std::max(1, gridn(d->n)));
-  launch_sum(d->stream, d->spare_d, d->n, d->sum_part, nb, d->sum_out);
+  launch_sum(d->stream, d->spare_d[0], d->n, d->sum_part, nb, d->sum_out);
   HIPC(hipGetLastError());
   HIPC(hipMemcpyAsync(sum, d->sum_out, sizeof(double), hipMemcpyDeviceToHost, d->stream));
   HIPC(hipStreamSynchronize(d->stream));
