@@ -176,7 +176,7 @@ __global__ __launch_bounds__(kTileP) void k_tile_cells(const float* __restrict__
 // some list references (larger ones are staged whole)
 constexpr int kCompactMax = 4096;
 // candidate positions held in LDS at a time by the list builder
-constexpr int kListStage = 1024;
+constexpr int kListStage = 512;
 
 // Neighbour lists in staged order (16-bit slots), and the staged set
 // tile_j / tile_uc.  Same float criterion as the reference
