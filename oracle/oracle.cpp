@@ -4,6 +4,7 @@
 // -O2 -ffp-contract=off so every FP64 expression rounds in source order, the
 // way the reference's --fmad=false build does (CMakeLists.txt:29).  Each
 // function cites the reference lines it restates.
+#include <omp.h>
 #include "oracle.h"
 
 #include <algorithm>
@@ -1084,5 +1085,8 @@ int or_step(or_state* s, const wf_config* c, or_run* run, int nbc, const int* bc
   if (run->ts_i > (c->ns_interval - 1)) run->ts_i = 0;
   return WF_OK;
 }
+
+void or_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int or_get_threads(void) { return omp_get_max_threads(); }
 
 }  // extern "C"

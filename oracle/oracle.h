@@ -152,6 +152,11 @@ void or_calc_surface(or_state* s, const wf_config* c);  /* CalculateSurface     
 void or_contact_nb(or_state* s, const wf_config* c);    /* CalcContactNb (brute force) */
 void or_contact_forces(or_state* s, const wf_config* c);/* CalcContactForcesWang     */
 
+/* OpenMP threads of the oracle's particle loops (the CPU baseline is timed
+ * on one thread and on all of them); or_get_threads returns the current cap */
+void or_set_threads(int n);
+int or_get_threads(void);
+
 /* one Fraser step (SolverFraser.cu:164-399) */
 int or_step(or_state* s, const wf_config* c, or_run* run, int nbc,
             const int* bc_zone, const double* bc_val);

@@ -101,6 +101,8 @@ def lib():
             "or_contact_forces": (None, [S, CF]),
             "or_mlp_forward": (D, [P, P]),
             "or_mlp_loss_grad": (D, [P, P, P, I, P]),
+            "or_set_threads": (None, [I]),
+            "or_get_threads": (I, []),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -328,6 +330,15 @@ class State:
 
     def int_energy_sum(self):
         return lib().or_int_energy_sum(self.ref())
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's particle loops."""
+    lib().or_set_threads(int(n))
+
+
+def get_threads():
+    return lib().or_get_threads()
 
 
 def kernel(q, h):

@@ -1,11 +1,13 @@
 """The host driver end to end on the GPU: deck → domain → Fraser loop → CSV
 snapshots, against the FP64 oracle run of the same setup built here from the
-deck's numbers.  CSV values carry 7 significant digits (%.6e, rounding <=
+deck's numbers (a rigid body of type File is read here by an independent
+parse of the Nastran fixed fields).  CSV values carry 7 significant digits (%.6e, rounding <=
 5e-7 relative), so each field is compared at relative L2 <= 1e-6 and
 pointwise to 1e-5 of the field's largest magnitude."""
 import json
 import math
 import os
+import re
 import subprocess
 
 import numpy as np
@@ -27,6 +29,29 @@ def _read_csv(path):
     return np.loadtxt(path, delimiter=",", skiprows=1, ndmin=2)
 
 
+def _nas_real(t):
+    """An 8-character Nastran real: "1.5-3", "2.5E-01", "-1.00D0", "0.0"."""
+    t = t.strip()
+    m = re.fullmatch(r"([+-]?[0-9.]+)(?:[EeDd]?([+-][0-9]+))?", t)
+    assert m, t
+    return float(m.group(1)) * 10.0 ** int(m.group(2) or 0)
+
+
+def _read_nas(path):
+    """GRID (fields 2, 4-6) and CTRIA3 (fields 4-6) of a small-field file."""
+    grid, tri = [], []
+    with open(path) as f:
+        for line in f:
+            if line.startswith("GRID"):
+                grid.append((int(line[8:16]),
+                             [_nas_real(line[24 + 8 * c:32 + 8 * c]) for c in range(3)]))
+            elif line.startswith("CTRIA3"):
+                tri.append([int(line[24 + 8 * c:32 + 8 * c]) for c in range(3)])
+    pos = {gid: k for k, (gid, _) in enumerate(grid)}
+    return (np.array([g[1] for g in grid], float),
+            np.array([[pos[t] for t in row] for row in tri], np.int32))
+
+
 def _oracle_from_deck(po, dk):
     """The setup WeldFormGPU.cu builds from the deck, restated in Python."""
     c, m, blk = dk["Configuration"], dk["Materials"][0], dk["DomainBlocks"][0]
@@ -34,7 +59,11 @@ def _oracle_from_deck(po, dk):
     h = 2 * r * c["hFactor"]
     K, G = E / (3 * (1 - 2 * nu)), E / (2 * (1 + nu))
     Cs = math.sqrt(K / rho)
-    xyz, mass = scenarios.gen_cylinder(blk["start"], blk["dim"][0] / 2, blk["dim"][2], r, rho)
+    if blk["type"] == "Box":
+        xyz, mass = scenarios.gen_box(blk["start"], blk["dim"], r, rho)
+    else:
+        xyz, mass = scenarios.gen_cylinder(blk["start"], blk["dim"][0] / 2, blk["dim"][2], r,
+                                           rho)
     n = len(xyz)
     ID = np.zeros(n, np.int32)
     for z in dk["DomainZones"]:
@@ -73,9 +102,13 @@ def _oracle_from_deck(po, dk):
     bcs = [(b["zoneId"], tuple(b["value"])) for b in dk["BoundaryConditions"]]
     meshes = []
     for rb in dk.get("RigidBodies", []):
-        p1 = np.array(rb["start"], float)
-        p2 = p1 + np.array([rb["dim"][0], rb["dim"][1], 0.0])
-        nodes, el = po.gen_plane_mesh(p1, p2, rb["partSide"])
+        if rb["type"] == "File":
+            nodes, el = _read_nas(os.path.join(DECKS, rb["fileName"]))
+            nodes = nodes + np.array(rb["moveDir"], float)
+        else:
+            p1 = np.array(rb["start"], float)
+            p2 = p1 + np.array([rb["dim"][0], rb["dim"][1], 0.0])
+            nodes, el = po.gen_plane_mesh(p1, p2, rb["partSide"])
         vel = [b[1] for b in bcs if b[0] == rb["zoneId"]][0]
         meshes.append({"nodes": nodes, "elnode": el, "vel": vel})
     if meshes:
@@ -86,7 +119,8 @@ def _oracle_from_deck(po, dk):
     return st, cfg, bcs
 
 
-@pytest.mark.parametrize("deck", ["compression_mini.json", "contact_mini.json"])
+@pytest.mark.parametrize("deck", ["compression_mini.json", "contact_mini.json",
+                                  "contact_file_press.json"])
 def test_driver_run_matches_oracle(oracle, tmp_path, deck):
     path = os.path.join(DECKS, deck)
     with open(path) as f:

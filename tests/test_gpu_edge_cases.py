@@ -109,3 +109,34 @@ def test_edge_case_kernels_and_steps(oracle, wflib, case, precision, ktol, stol)
         d.step(5)
         for k in ["x", "v", "rho", "sigma"]:
             _check(d.get(k), st[k], stol, k)
+
+
+@pytest.mark.parametrize("nz, fits", [(2400, True), (2600, False)])
+def test_mixed_grid_limit_of_the_packed_cell(oracle, wflib, nz, fits):
+    """Mixed precision keeps each particle's search cell packed in 11/11/10
+    bits, so its grid holds at most 1024 cells of 1.01 x 2h along z.  A line
+    of 2,600 particles dx apart spans 2600 / 2.424 + 3 = 1075 cells: the
+    search refuses it with WF_E_UNSUPPORTED and a message, where FP64 builds
+    it; 2,400 particles (993 cells) build in both precisions, with the
+    oracle's lists."""
+    from weldformgpu_amd.domain import WfError
+    sc = randomize(small_c1(0.4), seed=57)
+    dx = 2 * sc.meta["r"]
+    x = np.zeros((nz, 3))
+    x[:, 2] = dx * np.arange(nz)
+    sc = _subset(sc, np.arange(nz) % sc.n, x)
+    print(f"[case] packed-cell limit nz={nz}", flush=True)
+    cnt, off, lst, _ = oracle.neighbors(x, 2 * sc.cfg.h_glob)
+    assert cnt.max() == 4  # +-1 and +-2 dx along the line
+    for prec in (_abi.WF_FP64, _abi.WF_MIXED):
+        sc.cfg.precision = prec
+        with to_gpu(sc, wflib) as d:
+            if prec == _abi.WF_MIXED and not fits:
+                with pytest.raises(WfError) as ei:
+                    d.nsearch()
+                assert ei.value.code == _abi.WF_E_UNSUPPORTED
+                assert "11/11/10 bits" in str(ei.value)
+                continue
+            d.nsearch()
+            gc, go, gl = d.neighbors()
+            assert np.array_equal(gc, cnt) and np.array_equal(gl, lst)

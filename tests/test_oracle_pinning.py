@@ -462,3 +462,169 @@ def test_oracle_c1_steps_hold_bcs(oracle):
     assert np.all(v[sc.fields["ID"] == 3] == [0, 0, -1.0])
     assert np.all(v[sc.fields["ID"] == 2] == 0.0)
     assert st["u"][sc.fields["ID"] == 3][:, 2].max() < 0
+
+
+# ---------------------------------------------------------------------------
+# Hollomon law (Material.cuh:51-59, 192-201): eps0 = sy0/E, eps1 = (sy0/K)^(1/m);
+# sigma_y = sy0 on the plateau (eps_p + eps0 <= eps1), K (eps_p + eps0)^m above.
+# With m = 1/2 and m = 1/3 the power law is a square or cube root, so the
+# expected values below come from sqrt/cbrt, not from pow.
+HOL = dict(E=68.9e9, sy0=1e8, K_h=1e9, m_h=0.5)
+
+
+def _hol_params(**kw):
+    return _abi.wf_material_params(model=_abi.WF_MAT_HOLLOMON, **dict(HOL, **kw))
+
+
+def test_hollomon_yield_closed_form(oracle):
+    mp = _hol_params()
+    eps0 = HOL["sy0"] / HOL["E"]
+    eps1 = (HOL["sy0"] / HOL["K_h"]) ** 2  # m = 1/2: (sy0/K)^2 = 0.01
+    assert eps1 == pytest.approx(0.01, rel=1e-15)
+    y = lambda ep: oracle.lib().or_hollomon_yield(mp, ep)
+    # plateau, up to just below the knee eps_p = eps1 - eps0
+    for ep in (0.0, 1e-4, 0.5 * (eps1 - eps0), (eps1 - eps0) * (1 - 1e-9)):
+        assert y(ep) == HOL["sy0"]
+    # power branch: K sqrt(eps_p + eps0)
+    for ep in (0.009, 0.015, 0.1, 0.7):
+        assert y(ep) == pytest.approx(1e9 * math.sqrt(ep + eps0), rel=1e-14)
+    # continuous at the knee, increasing after it
+    knee = eps1 - eps0
+    assert y(knee * (1 + 1e-9)) == pytest.approx(HOL["sy0"], rel=1e-8)
+    assert y(0.02) < y(0.03) < y(0.3)
+    # m = 1/3: cube roots
+    mp3 = _hol_params(K_h=5e8, m_h=1.0 / 3.0)
+    for ep in (0.05, 0.2, 0.9):
+        assert oracle.lib().or_hollomon_yield(mp3, ep) == pytest.approx(
+            5e8 * np.cbrt(ep + eps0), rel=1e-14)
+    assert oracle.lib().or_hollomon_yield(mp3, 0.0) == HOL["sy0"]  # (0.2)^3 = 8e-3 > eps0
+
+
+def test_hollomon_reference_return_map(oracle):
+    """Reference return map (Mechanical.cu:504-535) with Hollomon: the trial
+    deviator is scaled with the OLD sigma_y, sigma_y is re-evaluated at the
+    OLD eps_p, dep = (s_trial - sigma_y)/(3G) and sigma_eq = the new sigma_y."""
+    S6 = np.array([3e8, -1e8, -2e8, 1e8, 0.5e8, -0.8e8])
+    G, ep, sy_old = 26.5e9, 0.02, 1.3e8
+    st, _ = _one(oracle, S6, np.zeros(6), np.zeros(6), _abi.WF_MAT_HOLLOMON, sy_old, G=G,
+                 ep=ep, mat=HOL)
+    s_trial = _vm(S6)
+    assert s_trial > sy_old
+    sy_new = 1e9 * math.sqrt(ep + HOL["sy0"] / HOL["E"])
+    assert _vm(st["shearstress"][0]) == pytest.approx(sy_old, rel=1e-13)
+    assert st["sigma_y"][0] == pytest.approx(sy_new, rel=1e-14)
+    dep = (s_trial - sy_new) / (3 * G)
+    assert st["pl_strain"][0] == pytest.approx(ep + dep, rel=1e-13)
+    assert st["sigma_eq"][0] == pytest.approx(sy_new, rel=1e-14)
+    # elastic particle: nothing changes but sigma_y (re-evaluated at eps_p)
+    st2, _ = _one(oracle, S6 * 0.2, np.zeros(6), np.zeros(6), _abi.WF_MAT_HOLLOMON, 2e8, G=G,
+                  ep=ep, mat=HOL)
+    assert st2["pl_strain"][0] == ep
+    assert np.allclose(st2["shearstress"][0], S6 * 0.2, rtol=1e-15)
+    assert st2["sigma_y"][0] == pytest.approx(sy_new, rel=1e-14)
+
+
+@pytest.mark.parametrize("ep", [0.0, 0.02, 0.2])
+def test_hollomon_consistent_return_closed_form(oracle, ep):
+    """Consistent radial return: s_t - 3 G dg = K sqrt(ep + dg + eps0).  With
+    u = sqrt(ep + dg + eps0) that is 3G u^2 + K u - (s_t + 3G (ep + eps0)) = 0,
+    a quadratic with one positive root; a root short of the knee (u^2 below
+    eps1) means the return ends on the plateau, dg = (s_t - sy0)/(3G).
+    ep = 0 ends on the plateau, 0.02 and 0.2 on the power branch."""
+    S6 = np.array([3e8, -1e8, -2e8, 1e8, 0.5e8, -0.8e8])
+    G = 26.5e9
+    st, _ = _one(oracle, S6, np.zeros(6), np.zeros(6), _abi.WF_MAT_HOLLOMON, 1e8, G=G, ep=ep,
+                 rm=_abi.WF_RETURN_NEWTON, mat=HOL)
+    K, eps0, s_t = HOL["K_h"], HOL["sy0"] / HOL["E"], _vm(S6)
+    c = s_t + 3 * G * (ep + eps0)
+    u = (-K + math.sqrt(K * K + 12 * G * c)) / (6 * G)
+    if u * u > 0.01:  # the return ends past the knee, on the power branch
+        dg, sy = u * u - ep - eps0, K * u
+    else:  # it ends on the plateau: perfect plasticity at sy0
+        dg, sy = (s_t - HOL["sy0"]) / (3 * G), HOL["sy0"]
+        assert ep + dg + eps0 <= 0.01
+    assert dg > 0
+    assert st["pl_strain"][0] == pytest.approx(ep + dg, rel=1e-10)
+    assert st["sigma_y"][0] == pytest.approx(sy, rel=1e-10)
+    assert st["sigma_eq"][0] == pytest.approx(s_t - 3 * G * dg, rel=1e-10)
+    assert _vm(st["shearstress"][0]) == pytest.approx(sy, rel=1e-10)
+
+
+# ---------------------------------------------------------------------------
+# equation of state, case 0 (Functions.cu:207-283): p = P0 + Cs^2 (rho - rho0)
+# with P0 = 0 (never uploaded by the reference driver); DensitySolid inverts it.
+def test_eos_round_trip_and_bulk_modulus(oracle):
+    rng = np.random.default_rng(12)
+    n = 64
+    st = oracle.State(n)
+    rho0 = rng.uniform(2000.0, 8000.0, n)
+    Cs = rng.uniform(1000.0, 6000.0, n)
+    rho = rho0 * (1 + rng.uniform(-0.02, 0.02, n))
+    st["rho"], st["rho0"], st["Cs"] = rho, rho0, Cs
+    st.pressure()
+    p = st["p"].copy()
+    # DensitySolid case 0 (Functions.cu:259-283): (p - P0)/Cs^2 + rho0 = rho
+    assert np.allclose(p / (Cs * Cs) + rho0, rho, rtol=1e-14, atol=0.0)
+    # sign: compression (rho > rho0) gives positive pressure
+    assert np.all(np.sign(p) == np.sign(rho - rho0))
+    # with Cs^2 = K/rho0 (the driver's Cs, WeldFormGPU.cu:253-267), the EOS is
+    # the small-strain bulk law p = K (rho/rho0 - 1) = -K dV/V to first order
+    E, nu, r0 = 68.9e9, 0.3, 2700.0
+    K = E / (3 * (1 - 2 * nu))
+    st1 = oracle.State(1)
+    st1["rho0"] = r0
+    st1["Cs"] = math.sqrt(K / r0)
+    for vol in (1e-4, -3e-4):
+        st1["rho"] = r0 / (1 + vol)
+        st1.pressure()
+        assert st1["p"][0] == pytest.approx(-K * vol, rel=2 * abs(vol))
+
+
+# ---------------------------------------------------------------------------
+# convection BC (CalcConvHeatKernel, Thermal.cu:36-64): one particle of
+# volume V = m/rho with exposed face A = V^(2/3) (a dx^2 face for a dx^3
+# particle) cools by Newton's law m cp dT/dt = h A (T_inf - T).
+def _convecting_particle(oracle, T0, dx=1e-3, rho=2700.0, cp=900.0, h_conv=100.0,
+                         T_inf=300.0, dt=0.1):
+    st = oracle.State(1)
+    st["x"] = [[0.0, 0.0, 0.0]]
+    st["h"] = 1.2 * dx
+    st["rho"], st["rho0"] = rho, rho
+    st["m"] = rho * dx ** 3
+    st["T"] = T0
+    st["k_T"] = 200.0
+    st["cp_T"] = cp
+    st["BC_T"] = _abi.WF_BCT_CONVECTION
+    cfg = _abi.default_config()
+    cfg.n_particles = cfg.solid_count = 1
+    cfg.h_glob = 1.2 * dx
+    cfg.mechanics, cfg.thermal = 0, 1
+    cfg.dt = dt
+    cfg.h_conv, cfg.T_inf = h_conv, T_inf
+    st.nsearch(cfg)
+    assert st["nb_count"][0] == 0
+    return st, cfg
+
+
+def test_convection_newton_cooling_rate(oracle):
+    dx, rho, cp, h, Tinf = 1e-3, 2700.0, 900.0, 100.0, 300.0
+    for T0 in (500.0, 250.0):
+        st, cfg = _convecting_particle(oracle, T0, dx, rho, cp, h, Tinf)
+        st.calc_temp_inc(cfg)
+        m = rho * dx ** 3
+        # exponent 0.666666666 instead of 2/3 changes (1e-9)^(2/3) by 1.4e-8
+        assert st["dTdt"][0] == pytest.approx(h * dx * dx * (Tinf - T0) / (m * cp), rel=1e-7)
+
+
+def test_convection_exponential_decay(oracle):
+    """Leapfrog over 200 steps follows T_inf + (T0 - T_inf) exp(-t/tau),
+    tau = m cp/(h A) = 24.3 s, to the scheme's O(dt^2) accuracy."""
+    dx, rho, cp, h, Tinf, T0, dt = 1e-3, 2700.0, 900.0, 100.0, 300.0, 500.0, 0.1
+    st, cfg = _convecting_particle(oracle, T0, dx, rho, cp, h, Tinf, dt)
+    nst = 200
+    st.step(cfg, [], nst)
+    tau = rho * dx ** 3 * cp / (h * dx * dx)
+    t = (nst - 1) * dt  # the first leapfrog step does not advance T
+    exp = Tinf + (T0 - Tinf) * math.exp(-t / tau)
+    assert st["T"][0] - Tinf == pytest.approx(exp - Tinf, rel=1e-3)
+    assert st["T"][0] < T0

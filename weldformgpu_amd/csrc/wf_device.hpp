@@ -46,7 +46,8 @@ __device__ __forceinline__ double hollomon_yield(const MatParams& mp, double ep)
   return mp.sy0;
 }
 
-// packed integer cell of the FP32 frame: cx | cy << 11 | cz << 22
+// packed integer cell of the FP32 frame: cx | cy << 11 | cz << 22 (a grid
+// of at most kPackedCells cells, wf_tiles.hpp)
 __device__ __forceinline__ int pack_cell(int cx, int cy, int cz) {
   return cx | (cy << 11) | (cz << 22);
 }

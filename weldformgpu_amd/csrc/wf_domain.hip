@@ -986,13 +986,17 @@ int wf_nsearch_build(wf_domain* d) {
   const double* src[3];
   for (int c = 0; c < 3; c++) src[c] = stale ? d->f.xs[c] : d->f.x[c];
   const double radius = 2.0 * d->cfg.h_glob + (pz ? d->cfg.ns_skin : 0.0);
+  // mixed precision keeps each particle's cell packed in 11/11/10 bits
   int rc = ns_build(d->stream, &d->ns, src, d->n, d->npad, radius, pz ? kListCap : kMaxNb,
                     d->gather, d->nb_count, d->ctr, d->perm, d->perm_tmp, &d->tl,
-                    &d->grid);
+                    &d->grid, d->mixed ? kPackedCells : nullptr);
   if (rc)
-    return fail(rc, rc == WF_E_NONFINITE ? "non-finite particle position"
-                    : rc == WF_E_NBCAP   ? "a neighbour tile exceeds its cell/slot budget"
-                                         : "neighbour search failed");
+    return fail(rc, rc == WF_E_NONFINITE  ? "non-finite particle position"
+                    : rc == WF_E_NBCAP    ? "a neighbour tile exceeds its cell/slot budget"
+                    : rc == WF_E_UNSUPPORTED
+                        ? "mixed precision packs a particle's search cell in 11/11/10 bits: "
+                          "the domain spans more than 2048 x 2048 x 1024 cells of 2h"
+                        : "neighbour search failed");
   // physically reorder every field into the new order
   std::vector<double**> pd;
   std::vector<float**> pf;
@@ -1714,3 +1718,5 @@ int wf_kernel_time(wf_domain* d, int k, double* total_ms, int64_t* launches) {
 }
 
 }  // extern "C"
+
+int wf::set_last_error(int code, const char* msg) { return fail(code, msg ? msg : ""); }

@@ -274,6 +274,8 @@ void launch_mlp_train_iter(hipStream_t s, float* w, float* m, float* v, const fl
                            int nblocks, int t, float lr, float* loss);
 // gradient blocks for n samples: one 64-sample chunk each, at most kMlpMaxBlocks
 int mlp_grad_blocks(int n);
+// record msg as the calling thread's wf_last_error() text; returns code
+int set_last_error(int code, const char* msg);
 // gather the logged samples (ascending particle index) into X (n x 4), Y
 void launch_mlp_collect(hipStream_t s, const Fields& f, int n, float* X, float* Y, int* flags,
                         int* scan, int* scan_part, int* count);
@@ -298,9 +300,12 @@ struct NsResult {
 // lists in list/count (new order).  Returns WF_OK or an error code.
 // perm: sorted slot -> original index of the current order; perm_tmp receives
 // the new one.
+// max_dim (may be null): the largest grid, in cells per axis, the caller can
+// represent; a larger one returns WF_E_UNSUPPORTED before anything changes
 int ns_build(hipStream_t s, NsWork** work, const double* const pos_src[3], int n, int64_t npad,
              double radius, int cap, int* gather, int* count, Counters* ctr,
-             int* perm, int* perm_tmp, TileLists* tiles, GridDesc* grid_out);
+             int* perm, int* perm_tmp, TileLists* tiles, GridDesc* grid_out,
+             const int* max_dim);
 void ns_free(NsWork* w);
 // exclusive scan of m ints into out[0..m] (out[m] = total); part needs
 // ceil(m/1024)+1 ints

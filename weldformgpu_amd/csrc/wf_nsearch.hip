@@ -120,27 +120,6 @@ __global__ __launch_bounds__(256) void k_ns_minmax_final(const float* part, int 
 
 using Grid = GridDesc;
 
-__device__ __forceinline__ int morton3(int x, int y, int z) {
-  int m = 0;
-#pragma unroll
-  for (int b = 0; b < 3; b++)
-    m |= (((x >> b) & 1) << (3 * b)) | (((y >> b) & 1) << (3 * b + 1)) | (((z >> b) & 1) << (3 * b + 2));
-  return m;
-}
-__device__ __forceinline__ int cell_key(const Grid& g, int cx, int cy, int cz) {
-  int meta = ((cz >> 3) * g.mdim[1] + (cy >> 3)) * g.mdim[0] + (cx >> 3);
-  return meta * 512 + morton3(cx & 7, cy & 7, cz & 7);
-}
-__device__ __forceinline__ void cell_of(const Grid& g, float px, float py, float pz, int& cx,
-                                        int& cy, int& cz) {
-  cx = (int)floor(((double)px - g.org[0]) * g.inv_cs);
-  cy = (int)floor(((double)py - g.org[1]) * g.inv_cs);
-  cz = (int)floor(((double)pz - g.org[2]) * g.inv_cs);
-  cx = min(max(cx, 0), g.dim[0] - 1);
-  cy = min(max(cy, 0), g.dim[1] - 1);
-  cz = min(max(cz, 0), g.dim[2] - 1);
-}
-
 __global__ void k_ns_fill_i32(int* a, int v, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = v;
@@ -356,7 +335,8 @@ void ns_free(NsWork* w) {
 // gather: out slot -> old sorted index; perm (in/out): sorted slot -> original
 int ns_build(hipStream_t s, NsWork** work, const double* const pos_src[3], int n, int64_t npad,
              double radius, int cap, int* gather, int* count, Counters* ctr,
-             int* perm, int* perm_tmp, TileLists* tiles, GridDesc* grid_out) {
+             int* perm, int* perm_tmp, TileLists* tiles, GridDesc* grid_out,
+             const int* max_dim) {
   if (!*work) *work = new NsWork();
   NsWork* w = *work;
   int rc = ensure(w, n, 0);
@@ -383,6 +363,7 @@ int ns_build(hipStream_t s, NsWork** work, const double* const pos_src[3], int n
     double ext = (hi - g.org[c]) / cs;
     if (ext > 1.0e6) return WF_E_ARG;  // domain too sparse for a uniform grid
     g.dim[c] = (int)std::floor(ext) + 2;
+    if (max_dim && g.dim[c] > max_dim[c]) return WF_E_UNSUPPORTED;
     g.mdim[c] = (g.dim[c] + 7) / 8;
     nkeys *= g.mdim[c];
   }

@@ -16,6 +16,7 @@
 // policies (DensD/DensG, RatesD/RatesG, AccelD/AccelG, CondD/CondG, SurfD,
 // CondDens) say what a neighbour record holds and what a pair adds.
 #include <climits>
+#include <type_traits>
 
 #include "wf_device.hpp"
 #include "wf_tiles.hpp"
@@ -562,13 +563,24 @@ __device__ __forceinline__ T lane_value_xor(const T& v, int m) {
   return r;
 }
 
+// waves per SIMD a policy asks the compiler to leave room for (its register
+// budget: 512 / waves VGPRs); 1, the default, sets no limit
+template <class K, class = void>
+struct waves_of {
+  static constexpr int v = 1;
+};
+template <class K>
+struct waves_of<K, std::void_t<decltype(K::WAVES)>> {
+  static constexpr int v = K::WAVES;
+};
+
 // One workgroup per tile, L = K::LANES consecutive lanes per particle: lanes
 // L p .. L p + L - 1 all hold particle p and deal its slot list out by quads
 // (lane L p + h takes the quads q with q % L == h); their sums are merged
 // (butterfly over the L lanes) before lane L p writes the result.
 template <class K>
-__global__ __launch_bounds__(4 * kTileP) void k_pair_gather(Fields f, TileLists tl,
-                                                            typename K::Prm prm) {
+__global__ __launch_bounds__(4 * kTileP) __attribute__((amdgpu_waves_per_eu(waves_of<K>::v)))
+void k_pair_gather(Fields f, TileLists tl, typename K::Prm prm) {
   constexpr int ST = K::STAGE;
   constexpr int L = K::LANES;
   static_assert(L == 1 || L == 2 || L == 4, "1, 2 or 4 lanes per particle");
@@ -860,10 +872,14 @@ struct RatesG {
 
 // D: acceleration with Monaghan viscosity (InteractionAlt.cu:18-150); the
 // neighbour's mass is (m/rho) rho
+#ifndef WF_ACCEL32_WAVES
+#define WF_ACCEL32_WAVES 1
+#endif
 template <bool UH>
 struct AccelG {
   WF_GATHER32
   static constexpr int NBATCH = 4;
+  static constexpr int WAVES = WF_ACCEL32_WAVES;
   struct Prm {
     int limit;
     double alpha, beta;

@@ -23,27 +23,6 @@
 
 namespace wf {
 
-__device__ __forceinline__ int morton3_t(int x, int y, int z) {
-  int m = 0;
-#pragma unroll
-  for (int b = 0; b < 3; b++)
-    m |= (((x >> b) & 1) << (3 * b)) | (((y >> b) & 1) << (3 * b + 1)) | (((z >> b) & 1) << (3 * b + 2));
-  return m;
-}
-__device__ __forceinline__ int key_of(const GridDesc& g, int cx, int cy, int cz) {
-  int meta = ((cz >> 3) * g.mdim[1] + (cy >> 3)) * g.mdim[0] + (cx >> 3);
-  return meta * 512 + morton3_t(cx & 7, cy & 7, cz & 7);
-}
-__device__ __forceinline__ void cell_of_t(const GridDesc& g, float px, float py, float pz, int& cx,
-                                          int& cy, int& cz) {
-  cx = (int)floor(((double)px - g.org[0]) * g.inv_cs);
-  cy = (int)floor(((double)py - g.org[1]) * g.inv_cs);
-  cz = (int)floor(((double)pz - g.org[2]) * g.inv_cs);
-  cx = min(max(cx, 0), g.dim[0] - 1);
-  cy = min(max(cy, 0), g.dim[1] - 1);
-  cz = min(max(cz, 0), g.dim[2] - 1);
-}
-
 constexpr int kSortN = 4096;  // >= kTileP * 27, power of two
 static_assert(kTileP * 27 <= kSortN, "sort buffer too small");
 
@@ -68,8 +47,8 @@ __global__ __launch_bounds__(kTileP) void k_tile_cells(const float* __restrict__
   const int i = t * kTileP + tid;
   int cx = 0, cy = 0, cz = 0, kown = INT_MAX;
   if (i < n) {
-    cell_of_t(g, p0[i], p1[i], p2[i], cx, cy, cz);
-    kown = key_of(g, cx, cy, cz);
+    cell_of(g, p0[i], p1[i], p2[i], cx, cy, cz);
+    kown = cell_key(g, cx, cy, cz);
   }
   own[tid] = kown;
   __syncthreads();
@@ -101,7 +80,7 @@ __global__ __launch_bounds__(kTileP) void k_tile_cells(const float* __restrict__
       const int o = q / 27, r = q - 27 * o;
       const int x = ocx[o] + (r % 3) - 1, y = ocy[o] + ((r / 3) % 3) - 1, z = ocz[o] + r / 9 - 1;
       if (x >= 0 && y >= 0 && z >= 0 && x < g.dim[0] && y < g.dim[1] && z < g.dim[2]) {
-        const int kk = key_of(g, x, y, z);
+        const int kk = cell_key(g, x, y, z);
         if (cell_start[kk + 1] > cell_start[kk]) k = kk;
       }
     }
@@ -217,7 +196,7 @@ __device__ __forceinline__ int block_excl_scan(int v, int* wsum, int& total) {
 
 // The cell holding staged position q: the last p with spre[p] <= q (an empty
 // cell shares its prefix with the next one, so the cell found is never empty).
-__device__ __forceinline__ int cell_of(const int* spre, int nc, int q) {
+__device__ __forceinline__ int staged_cell(const int* spre, int nc, int q) {
   int lo = 0, hi = nc - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -279,7 +258,7 @@ __global__ __launch_bounds__(kListThreads) void k_tile_lists(
     const int m = min(kListStage, U - c0);
     __syncthreads();  // the previous chunk is consumed (and spre is complete)
     for (int q = tid; q < m; q += NT) {
-      const int p = cell_of(spre, nc, c0 + q);
+      const int p = staged_cell(spre, nc, c0 + q);
       const int j = sb[p] + (c0 + q - spre[p]);
       spos[q] = make_float4(p0[j], p1[j], p2[j], __int_as_float(j));
     }
@@ -360,7 +339,7 @@ __global__ __launch_bounds__(kListThreads) void k_tile_lists(
   for (int s = tid; s < U; s += NT) {
     const int d = compact ? sidx[s] : s;
     if (d >= 0) {
-      const int p = cell_of(spre, nc, s);
+      const int p = staged_cell(spre, nc, s);
       tile_j[uoff + d] = sb[p] + (s - spre[p]);
     }
   }

@@ -22,6 +22,37 @@ struct GridDesc {
   int dim[3];
   int mdim[3];
 };
+// mixed precision packs a particle's cell in 11/11/10 bits (pack_cell,
+// wf_device.hpp), so its grids hold at most these cells per axis
+constexpr int kPackedCells[3] = {2048, 2048, 1024};
+
+// The cell and key functions below are the one definition the search's
+// counting sort (wf_nsearch.hip) and the tile builder (wf_tiles.hip) share:
+// the lists are right only if both bin a position into the same cell and
+// order cells by the same key (the 8^3 meta-block layout of
+// cuNSearchKernels.cu:10-21, the bits interleaved x, y, z from bit 0).
+__device__ __forceinline__ int morton3(int x, int y, int z) {
+  int m = 0;
+#pragma unroll
+  for (int b = 0; b < 3; b++)
+    m |= (((x >> b) & 1) << (3 * b)) | (((y >> b) & 1) << (3 * b + 1)) |
+         (((z >> b) & 1) << (3 * b + 2));
+  return m;
+}
+__device__ __forceinline__ int cell_key(const GridDesc& g, int cx, int cy, int cz) {
+  int meta = ((cz >> 3) * g.mdim[1] + (cy >> 3)) * g.mdim[0] + (cx >> 3);
+  return meta * 512 + morton3(cx & 7, cy & 7, cz & 7);
+}
+// cell of an FP32 position (FP64 arithmetic), clamped into the grid
+__device__ __forceinline__ void cell_of(const GridDesc& g, float px, float py, float pz, int& cx,
+                                        int& cy, int& cz) {
+  cx = (int)floor(((double)px - g.org[0]) * g.inv_cs);
+  cy = (int)floor(((double)py - g.org[1]) * g.inv_cs);
+  cz = (int)floor(((double)pz - g.org[2]) * g.inv_cs);
+  cx = min(max(cx, 0), g.dim[0] - 1);
+  cy = min(max(cy, 0), g.dim[1] - 1);
+  cz = min(max(cz, 0), g.dim[2] - 1);
+}
 
 // What the pair kernels read.  slotq holds four 16-bit staged-set positions
 // per 8 bytes: entry k of particle i is 16-bit lane (k&3) of
