@@ -336,6 +336,22 @@ int wf_mlp_fit(struct wf_domain* d, const float* X, const float* Y, int n, int i
 /* one online round on the logged samples */
 int wf_mlp_train(struct wf_domain* d, int iters, float lr, int64_t* samples, float* loss);
 
+/* wf_mlp_grad on device arrays, allocating nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  params (WF_MLP_PARAMS floats), X (n x 4), Y (n), grad
+ * (WF_MLP_PARAMS) and loss (1 float) are device pointers on the current
+ * device; stream is a hipStream_t (NULL: the null stream).  The call only
+ * enqueues; results are ready when the stream has reached it.  The gradient's
+ * per-block partial sums (at most 256 blocks) live in a fixed device array of
+ * the library (1.22 MiB per GPU, held until the process ends) instead of an
+ * allocation; n must lie in 1..WF_MLP_NOALLOC_MAX_SAMPLES (WF_E_ARG
+ * otherwise), and calls that overlap on one GPU race on that array.  Results
+ * are bit-identical to wf_mlp_grad's on the same inputs. */
+#define WF_MLP_NOALLOC_MAX_SAMPLES (1 << 24)
+int wf_mlp_grad_noalloc(const float* params, const float* X, const float* Y, int n,
+                        float* grad, float* loss, void* stream);
+
 /* ---- rigid triangle meshes and penalty contact ---------------------------
  * The reference stores each rigid mesh as a device TriMesh_d (Mesh.cuh:19-54)
  * and adds one zero-mass "particle" per triangle centroid to the SPH set

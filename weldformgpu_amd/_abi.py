@@ -161,6 +161,7 @@ def _proto(lib):
         "wf_mlp_grad": (I, [P, VP, VP, I, VP, C.POINTER(C.c_float)]),
         "wf_mlp_fit": (I, [P, VP, VP, I, I, C.c_float, C.POINTER(C.c_float)]),
         "wf_mlp_train": (I, [P, I, C.c_float, C.POINTER(I64), C.POINTER(C.c_float)]),
+        "wf_mlp_grad_noalloc": (I, [P, P, P, I, P, P, P]),
         "wf_gen_plane_mesh": (I, [VP, VP, I, VP, I, VP, I, C.POINTER(I)]),
         "wf_read_nastran": (I, [C.c_char_p, I, VP, I, VP, I, C.POINTER(I)]),
         "wf_add_mesh": (I, [P, VP, I, VP, I, I, C.POINTER(I)]),
@@ -194,7 +195,7 @@ def declared_symbols():
             "wf_apply_bc_vel", "wf_update_pos_fraser", "wf_update_vel", "wf_calc_temp_inc",
             "wf_thermal_calcs", "wf_int_energy", "wf_step", "wf_get_stats", "wf_set_timing",
             "wf_kernel_time", "wf_mlp_get_params", "wf_mlp_set_params", "wf_mlp_reset_adam",
-            "wf_mlp_infer", "wf_mlp_grad", "wf_mlp_fit", "wf_mlp_train",
+            "wf_mlp_infer", "wf_mlp_grad", "wf_mlp_fit", "wf_mlp_train", "wf_mlp_grad_noalloc",
             "wf_gen_plane_mesh", "wf_read_nastran", "wf_add_mesh", "wf_set_mesh_motion", "wf_mesh_info",
             "wf_get_mesh", "wf_calc_surface", "wf_calc_contact_nb", "wf_calc_contact_forces",
             "wf_mesh_update", "wf_get_contact_neighbors", "wf_contact_force_sum", "wf_gen_box",

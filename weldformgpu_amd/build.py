@@ -18,7 +18,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libwfhip.so")
 DRIVER = os.path.join(HERE, "wfdriver")
 SOURCES = ["wf_kernels.hip", "wf_pair.hip", "wf_tiles.hip", "wf_nsearch.hip",
-           "wf_mlp.hip", "wf_contact.hip", "wf_domain.hip", "wf_geometry.cpp"]
+           "wf_mlp.hip", "wf_contact.hip", "wf_domain.hip", "wf_geometry.cpp", "wf_noalloc.hip"]
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 ARCH = os.environ.get("WF_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
