@@ -143,9 +143,11 @@ def workload_text(sc, n):
               "conduction + Taylor-Quinney 0.9")
     if sc.name.startswith("c5"):
         ne = sum(len(m["elnode"]) for m in sc.meshes)
+        how = ("local re-lists near the plastic zone"
+               if sc.cfg.rebuild_policy == 2 else "plastic-zone list rebuilds")
         return (f"{sc.name}: JC plate 120x60x6mm dx0.25mm, {n} particles, rigid tool of {ne} "
                 f"triangles (shoulder R9mm, pin R3mm x 5mm) at 1000 rpm and 2 mm/s, penalty "
-                f"0.1 E/dx, Coulomb 0.2 with friction heat, plastic-zone list rebuilds, "
+                f"0.1 E/dx, Coulomb 0.2 with friction heat, {how}, "
                 f"{common}; compared with plain FP64")
     if sc.meshes:
         ne = sum(len(m["elnode"]) for m in sc.meshes)
@@ -157,6 +159,34 @@ def workload_text(sc, n):
 
 
 DECK_STEPS = {"c3": 3000, "c4": 3000, "c5": 20000}
+
+
+def calibration_copy(nbytes):
+    """One device-to-device copy of a cold buffer of nbytes (hipMemcpy's copy
+    kernel, which reads the source once and writes the destination once),
+    after a fill of a second buffer of the same size has pushed the source
+    out of L2 and the MALL: a plain copy kernel of known size for
+    calibrating rocprofv3's FETCH_SIZE and WRITE_SIZE counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    bufs = [C.c_void_p() for _ in range(3)]
+    for b in bufs:
+        if hip.hipMalloc(C.byref(b), nbytes) != 0:
+            raise RuntimeError("hipMalloc failed for the calibration copy")
+    src, dst, evict = bufs
+    ok = (hip.hipMemset(src, 1, nbytes) == 0 and hip.hipMemset(evict, 2, nbytes) == 0 and
+          hip.hipDeviceSynchronize() == 0 and hip.hipMemcpy(dst, src, nbytes, 3) == 0 and
+          hip.hipDeviceSynchronize() == 0)
+    for b in bufs:
+        hip.hipFree(b)
+    if not ok:
+        raise RuntimeError("calibration copy failed")
+    return {"bytes_read": nbytes, "bytes_written": nbytes,
+            "kernel": "hipMemcpy device-to-device copy kernel"}
 
 
 def main():
@@ -176,11 +206,21 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--workload", choices=["c3", "c4", "c5"], default="c3")
+    ap.add_argument("--rebuild", choices=["global", "local"], default="local",
+                    help="c5: on the plastic-zone trigger, rebuild every tile's lists with a "
+                         "reorder (global) or re-list in place the tiles near the plastic zone "
+                         "(local)")
     ap.add_argument("--no-error-floor", action="store_true",
                     help="skip the FP64 run with 1e-12 m position noise that measures how "
                          "far two FP64 runs drift apart on this workload")
     ap.add_argument("--no-plain-mixed", action="store_true",
                     help="skip the mixed-precision run without the surrogate")
+    ap.add_argument("--calibration-copy-mib", type=int, default=0,
+                    help="after the runs, one device-to-device copy of a cold buffer of this "
+                         "many MiB (read once, written once; a second buffer of the same size "
+                         "is filled first so that the source is out of L2 and the MALL): a "
+                         "plain copy kernel of known size for calibrating rocprofv3's "
+                         "FETCH_SIZE and WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path is "
@@ -198,7 +238,10 @@ def main():
         if args.workload == "c4":
             return scenarios.c4_contact(scale=args.scale)
         if args.workload == "c5":
-            return scenarios.c5_friction_stir(scale=args.scale, offset=(0.137, 0.291))
+            sc = scenarios.c5_friction_stir(scale=args.scale, offset=(0.137, 0.291))
+            if args.rebuild == "local":
+                sc.cfg.rebuild_policy = _abi.WF_REBUILD_PLASTIC_ZONE_LOCAL
+            return sc
         return scenarios.c3_johnson_cook(scale=args.scale)
 
     def run(precision, surrogate, noise=0.0, timing_steps=0):
@@ -259,25 +302,28 @@ def main():
         dom.close()
         return res
 
-    def charged(r, interval):
+    def charged(r):
         """Window time with its training rounds replaced by the rounds' cost
-        amortized over the training interval: a 20-step window that happens
-        to end on a training step pays 1/10 of a round, not a whole one."""
+        amortized over the training interval in force at the window's end
+        (the cadence backs off while the network keeps fitting): a 20-step
+        window that happens to end on a training step pays 20/interval of a
+        round, not a whole one."""
         n_train = r["s1"]["mlp_trainings"] - r["s0"]["mlp_trainings"]
         t_train = r.get("train_ms", 0.0) * 1e-3
+        interval = max(1, r["s1"]["mlp_interval"])
         amort = t_train * K / interval if r.get("train_ms") is not None else 0.0
-        return r["secs"] - n_train * t_train + amort, n_train
+        return r["secs"] - n_train * t_train + amort, n_train, interval
 
     r64 = run(_abi.WF_FP64, 0, timing_steps=args.timing_steps)
     sc = r64["sc"]
     n = sc.n
-    interval = sc.cfg.mlp_train_interval
     rmx = run(_abi.WF_MIXED, 1, timing_steps=args.timing_steps)
-    secs_mx, n_train_window = charged(rmx, interval)
+    secs_mx, n_train_window, interval = charged(rmx)
     ms_step = secs_mx * 1e3 / K
     rate = n * K / secs_mx
     rate64 = n * K / r64["secs"]
-    rplain = None if args.no_plain_mixed else run(_abi.WF_MIXED, 0)
+    rplain = None if args.no_plain_mixed else run(_abi.WF_MIXED, 0,
+                                                  timing_steps=args.timing_steps)
 
     def rl2(a, b):
         d = np.linalg.norm(b)
@@ -332,16 +378,24 @@ def main():
         "surrogate": {
             "train_round_ms": rmx.get("train_ms"),
             "train_interval_steps": interval,
+            "train_interval_base": sc.cfg.mlp_train_interval,
+            "train_interval_max": sc.cfg.mlp_interval_max,
             "train_ms_per_step_amortized": rmx.get("train_ms", 0.0) / interval,
+            "train_ms_per_step_run_average": rmx.get("train_ms", 0.0) *
+            rmx["s1"]["mlp_trainings"] / max(1, rmx["s1"]["step"]),
             "trainings_in_window": n_train_window,
             "window_ms_per_step_measured": rmx["secs"] * 1e3 / K,
+            "mlp_start_loss": rmx["s1"]["mlp_start_loss"],
         },
         "run": {"newton_calls": stats["newton_calls"], "newton_iters": stats["newton_iters"],
                 "newton_iters_per_call": stats["newton_iters"] / max(1, stats["newton_calls"]),
                 "fp64_plain_newton_iters_per_call": r64["s1"]["newton_iters"] /
                 max(1, r64["s1"]["newton_calls"]),
                 "mlp_trainings": stats["mlp_trainings"], "mlp_loss": stats["mlp_loss"],
-                "ns_builds": stats["ns_builds"], "is_yielding": stats["is_yielding"]},
+                "ns_builds": stats["ns_builds"], "is_yielding": stats["is_yielding"],
+                "ns_local_builds": stats["ns_local_builds"],
+                "ns_local_tiles": stats["ns_local_tiles"],
+                "newton_iters_wave": stats["newton_iters_wave"]},
     }
     if rplain is not None:
         sp = rplain["s1"]
@@ -350,11 +404,14 @@ def main():
             "speedup_vs_fp64": (n * K / rplain["secs"]) / rate64,
             "newton_iters_per_call": sp["newton_iters"] / max(1, sp["newton_calls"]),
             "error_vs_fp64": field_err(rplain),
-            "surrogate_net_ms_per_step": ms_step - rplain["secs"] * 1e3 / K}
+            "surrogate_net_ms_per_step": ms_step - rplain["secs"] * 1e3 / K,
+            "stress_kernel_ms": rplain["kern"].get("stress", {}).get("avg_ms")}
     if args.workload == "c4":
         out["note"] = ("on this workload two FP64 runs 1e-12 m apart already differ by ~9e-4 "
                        "in eps_p and T - T0 after 3,000 steps (error_floor_fp64_perturbed), so "
                        "the 1e-4 target is below the workload's own floor for those fields")
+    if args.calibration_copy_mib:
+        out["calibration_copy"] = calibration_copy(args.calibration_copy_mib << 20)
     print(json.dumps(out))
 
 

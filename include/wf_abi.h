@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define WF_ABI_VERSION 3
+#define WF_ABI_VERSION 4
 
 /* ---- status codes ------------------------------------------------------ */
 enum {
@@ -115,7 +115,14 @@ typedef struct wf_material_params {
  * once yielding, only when a particle with eps_p > ep_hot has moved more
  * than ns_skin / 2 since the last build; pairs beyond 2h contribute nothing
  * to the pair sums, and the surface pass counts only pairs within 2h. */
-enum { WF_REBUILD_EVERY_N = 0, WF_REBUILD_PLASTIC_ZONE = 1 };
+enum {
+  WF_REBUILD_EVERY_N = 0,
+  WF_REBUILD_PLASTIC_ZONE = 1,
+  WF_REBUILD_PLASTIC_ZONE_LOCAL = 2 /* the same trigger; a rebuild re-lists
+    in place only the tiles within two search cells of a particle with
+    eps_p > ep_hot (no reorder), and a full rebuild follows every
+    ns_global_every re-lists or when a tile outgrows its staging region */
+};
 
 typedef struct wf_config {
   int abi_version;       /* WF_ABI_VERSION                                  */
@@ -153,6 +160,20 @@ typedef struct wf_config {
   double ns_skin;        /* WF_REBUILD_PLASTIC_ZONE: list radius 2h + skin  */
   double ep_hot;         /* WF_REBUILD_PLASTIC_ZONE: eps_p of the particles
                             whose displacement triggers a rebuild (0.05)    */
+  /* surrogate training (extension): each online round runs mlp_train_iters
+   * Adam steps at rate mlp_lr.  Rounds are due every mlp_train_interval steps
+   * while yielding; with mlp_interval_max > 0 the interval doubles, up to
+   * mlp_interval_max, after a round whose starting loss (the network before
+   * the round, on the round's samples) is at most mlp_refit_ratio times the
+   * previous round's final loss, and returns to mlp_train_interval after a
+   * round where it is not. */
+  int mlp_train_iters;   /* Adam steps per round (250)                       */
+  int mlp_interval_max;  /* cap of the adaptive interval (0 = fixed cadence) */
+  double mlp_lr;         /* Adam learning rate (2e-3)                        */
+  double mlp_refit_ratio;/* starting/previous loss ratio that still fits (2) */
+  int ns_global_every;   /* WF_REBUILD_PLASTIC_ZONE_LOCAL: a full rebuild
+                            (reorder) after this many re-lists (0: 32)      */
+  int _pad1;
 } wf_config;
 
 /* Fill *cfg with the reference's defaults: alpha=1, beta=0
@@ -280,6 +301,12 @@ typedef struct wf_stats {
   int cont_over_cap;       /* contact candidates beyond WF_MAX_CONT_NB      */
   double mlp_loss;         /* training loss of the last surrogate round     */
   int64_t mlp_samples;     /* logged samples available at that round        */
+  int mlp_interval;        /* current training interval in steps            */
+  int _pad0;
+  double mlp_start_loss;   /* loss of the network at the start of that round */
+  int ns_local_builds;     /* local re-lists (WF_REBUILD_PLASTIC_ZONE_LOCAL) */
+  int _pad1;
+  int64_t ns_local_tiles;  /* tiles re-listed by them, in total             */
 } wf_stats;
 int wf_get_stats(struct wf_domain* d, wf_stats* out);
 

@@ -74,10 +74,12 @@ def test_contact_deck_setup():
 
 def test_extension_options(tmp_path):
     """Extensions of this library in the deck (the reference ignores the
-    object): sliding-friction heat and the plastic-zone rebuild policy."""
+    object): sliding-friction heat, the plastic-zone rebuild policies and the
+    surrogate's training settings."""
     d = _deck("contact_mini.json")
     s = _dry(os.path.join(DECKS, "contact_mini.json"))
     assert s["fric_heat"] == 0.0 and s["rebuild_policy"] == 0 and s["ep_hot"] == 0.05
+    assert s["ns_global_every"] == 32 and s["mlp"] == [200, 250, 2e-3, 0, 2.0]
     d["Extensions"]["frictionHeat"] = 0.9
     d["Extensions"]["rebuild"] = {"policy": "plasticZone", "skin": 2e-5, "epHot": 0.1}
     p = tmp_path / "ext.json"
@@ -85,9 +87,18 @@ def test_extension_options(tmp_path):
     s = _dry(str(p))
     assert s["fric_heat"] == 0.9 and s["rebuild_policy"] == 1
     assert s["ns_skin"] == 2e-5 and s["ep_hot"] == 0.1
+    d["Extensions"]["rebuild"] = {"policy": "plasticZoneLocal", "skin": 2e-5, "epHot": 0.1,
+                                  "globalEvery": 12}
+    d["Extensions"]["surrogateTraining"] = {"interval": 100, "iterations": 80,
+                                            "learningRate": 1e-3, "intervalMax": 1600,
+                                            "refitRatio": 1.5}
+    p.write_text(json.dumps(d))
+    s = _dry(str(p))
+    assert s["rebuild_policy"] == 2 and s["ns_global_every"] == 12
+    assert s["mlp"] == [100, 80, 1e-3, 1600, 1.5]
     d["Extensions"]["rebuild"]["policy"] = "sometimes"
     p.write_text(json.dumps(d))
-    assert "plasticZone" in _dry(str(p), expect_rc=2)
+    assert "plasticZoneLocal" in _dry(str(p), expect_rc=2)
 
 
 def test_rotating_rigid_body(tmp_path):

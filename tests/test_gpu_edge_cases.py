@@ -140,3 +140,73 @@ def test_mixed_grid_limit_of_the_packed_cell(oracle, wflib, nz, fits):
             d.nsearch()
             gc, go, gl = d.neighbors()
             assert np.array_equal(gc, cnt) and np.array_equal(gl, lst)
+
+
+def _uncompacted_sizes(x, radius, tile=128):
+    """Particles in the occupied cells adjacent to each tile's cells (cell =
+    1.01 radius on FP32 positions, tiles of 128 in the search's cell order):
+    the set the list builder stages before it compacts."""
+    xf = x.astype(np.float32).astype(np.float64)
+    cs = float(np.float32(radius)) * 1.01
+    org = xf.min(0) - cs
+    c = np.floor((xf - org) / cs).astype(np.int64)
+    mdim = (c.max(0) + 2 + 7) // 8
+    m = np.zeros(len(xf), np.int64)
+    for b in range(3):
+        for a in range(3):
+            m |= (((c[:, a] & 7) >> b) & 1) << (3 * b + a)
+    key = (((c[:, 2] >> 3) * mdim[1] + (c[:, 1] >> 3)) * mdim[0] + (c[:, 0] >> 3)) * 512 + m
+    order = np.lexsort((np.arange(len(xf)), key))
+    cells = {}
+    for cc in map(tuple, c):
+        cells[cc] = cells.get(cc, 0) + 1
+    sizes = []
+    for t in range(0, len(order), tile):
+        own = {tuple(v) for v in c[order[t:t + tile]]}
+        adj = {(a + i, b + j, d + k) for a, b, d in own
+               for i in (-1, 0, 1) for j in (-1, 0, 1) for k in (-1, 0, 1)}
+        sizes.append(sum(cells.get(q, 0) for q in adj))
+    return np.array(sizes)
+
+
+@pytest.mark.parametrize("precision,tol", [(_abi.WF_FP64, 1e-12), (_abi.WF_MIXED, 2e-5)])
+def test_uncompacted_staged_sets_over_4096(oracle, wflib, precision, tol):
+    """6,000 particles at random in a 2.4 mm cube with 2h = 1.2 mm: every
+    tile's adjacent cells hold more than the 4,096 particles the list builder
+    compacts, so the lists keep raw positions into the whole set and the pair
+    loops take the chunked path over it.  Every particle has far more than
+    70 candidates: each list holds 70 true neighbours.  The pair kernels are
+    compared with the oracle run on the GPU's own lists."""
+    rng = np.random.default_rng(58)
+    sc = randomize(small_c1(0.4), seed=58, v_scale=0.5, s_scale=100e6)
+    n = 6000
+    x = rng.uniform(0.0, 2.4e-3, (n, 3))
+    sc = _subset(sc, np.arange(n) % sc.n, x)
+    sc.cfg.precision = precision
+    radius = 2 * sc.cfg.h_glob
+    sizes = _uncompacted_sizes(x, radius)
+    print(f"[case] uncompacted sets: {len(sizes)} tiles, {sizes.min()}..{sizes.max()} "
+          f"particles, precision {precision}", flush=True)
+    assert sizes.min() > 4096
+    st = to_oracle(oracle, sc)
+    with to_gpu(sc, wflib) as d:
+        d.nsearch()
+        gc, go, gl = d.neighbors()
+        _, over = d.nsearch_stats()
+        assert over == n and np.all(gc == _abi.WF_MAX_NEIGHBORS)
+        cnt, off, lst, _ = oracle.neighbors(x, radius)
+        for i in range(0, n, 97):
+            assert set(gl[go[i]:go[i] + gc[i]]) <= set(lst[off[i]:off[i] + cnt[i]])
+        st["nb_count"] = gc
+        st["nb_offset"] = go
+        st["nb_list"][:len(gl)] = gl
+        for call, name, fields in (("CalcDensInc", "calc_dens_inc", ["drho"]),
+                                   ("CalcRateTensors", "calc_rate_tensors",
+                                    ["strrate", "rotrate"]),
+                                   ("CalcAccel", "calc_accel", ["a"])):
+            getattr(d, call)()
+            getattr(st, name)(sc.cfg)
+            for k in fields:
+                e = rel_l2(d.get(k), st[k])
+                print(f"  {k}: {e:.2e}")
+                assert e < tol, k

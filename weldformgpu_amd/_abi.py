@@ -7,7 +7,7 @@ product can silently run without the native code.
 import ctypes as C
 import os
 
-WF_ABI_VERSION = 3
+WF_ABI_VERSION = 4
 
 WF_OK = 0
 WF_E_ARG = -1
@@ -31,7 +31,7 @@ WF_Q_STALE_NS_POS = 0x4
 WF_Q_ALL = 0x7
 WF_BCT_NONE, WF_BCT_CONVECTION, WF_BCT_FIXED = 0, 1, 2
 WF_CONTACT_REFERENCE, WF_CONTACT_STIFFNESS = 0, 1
-WF_REBUILD_EVERY_N, WF_REBUILD_PLASTIC_ZONE = 0, 1
+WF_REBUILD_EVERY_N, WF_REBUILD_PLASTIC_ZONE, WF_REBUILD_PLASTIC_ZONE_LOCAL = 0, 1, 2
 WF_MAX_MESHES = 8
 WF_MAX_CONT_NB = 20
 
@@ -78,7 +78,10 @@ class wf_config(C.Structure):
                 ("contact_kappa", C.c_double), ("contact_dx", C.c_double),
                 ("fric_sta", C.c_double), ("fric_dyn", C.c_double),
                 ("id_free_surf", C.c_int), ("rebuild_policy", C.c_int),
-                ("fric_heat", C.c_double), ("ns_skin", C.c_double), ("ep_hot", C.c_double)]
+                ("fric_heat", C.c_double), ("ns_skin", C.c_double), ("ep_hot", C.c_double),
+                ("mlp_train_iters", C.c_int), ("mlp_interval_max", C.c_int),
+                ("mlp_lr", C.c_double), ("mlp_refit_ratio", C.c_double),
+                ("ns_global_every", C.c_int), ("_pad1", C.c_int)]
 
 
 class wf_stats(C.Structure):
@@ -86,7 +89,9 @@ class wf_stats(C.Structure):
                 ("ns_builds", C.c_int), ("newton_iters", C.c_int64),
                 ("newton_calls", C.c_int64), ("newton_iters_wave", C.c_int64),
                 ("mlp_trainings", C.c_int), ("nb_over_cap", C.c_int), ("cont_over_cap", C.c_int),
-                ("mlp_loss", C.c_double), ("mlp_samples", C.c_int64)]
+                ("mlp_loss", C.c_double), ("mlp_samples", C.c_int64),
+                ("mlp_interval", C.c_int), ("_pad0", C.c_int), ("mlp_start_loss", C.c_double),
+                ("ns_local_builds", C.c_int), ("_pad1", C.c_int), ("ns_local_tiles", C.c_int64)]
 
 
 def default_config():
@@ -106,6 +111,11 @@ def default_config():
     c.newton_tol = 1e-11
     c.surrogate = 0
     c.mlp_train_interval = 200
+    c.mlp_train_iters = 250
+    c.mlp_interval_max = 0
+    c.mlp_lr = 2e-3
+    c.mlp_refit_ratio = 2.0
+    c.ns_global_every = 32
     c.contact = 0
     c.contact_law = WF_CONTACT_REFERENCE
     c.contact_kappa = 0.1

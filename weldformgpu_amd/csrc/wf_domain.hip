@@ -79,7 +79,15 @@ struct wf_domain {
   double* xb[3] = {nullptr, nullptr, nullptr};
   unsigned long long *pz_d2 = nullptr, *pz_d2_host = nullptr;
   bool pz_pending = false;
-  bool plastic_zone() const { return cfg.rebuild_policy == WF_REBUILD_PLASTIC_ZONE; }
+  bool plastic_zone() const {
+    return cfg.rebuild_policy == WF_REBUILD_PLASTIC_ZONE ||
+           cfg.rebuild_policy == WF_REBUILD_PLASTIC_ZONE_LOCAL;
+  }
+  // WF_REBUILD_PLASTIC_ZONE_LOCAL: re-lists since the last reorder, and the
+  // totals of re-lists and re-listed tiles
+  int local_since_reorder = 0;
+  int ns_local_builds = 0;
+  int64_t ns_local_tiles = 0;
   // with a list skin the surface pass counts only pairs within 2 h_glob
   double surface_r2cut() const { return plastic_zone() ? 4.0 * cfg.h_glob * cfg.h_glob : 0.0; }
   // the free-surface pass in the domain's precision
@@ -106,7 +114,10 @@ struct wf_domain {
   int adam_t = 0;
   bool mlp_trained = false;
   double mlp_last_loss = 0.0;
+  double mlp_start_loss = 0.0;  // loss of the network on a round's samples before its first step
   int64_t mlp_last_samples = 0;
+  int mlp_interval = 0;         // current training interval in steps (0: not started)
+  int64_t mlp_next_train = 0;   // the step count at which the next round is due
   bool mixed = false;
   bool uniform_h = true;     // every particle has the same h (FP32 loops skip h_j)
   bool uniform_m = true;     // every particle has the same mass m_value (the
@@ -149,6 +160,10 @@ struct wf_domain {
   // timing
   bool timing = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // before yield: the step's max eps_p, copied to ctr_host behind the stress
+  // pass and read when the step's later passes are queued
+  hipEvent_t ev_yield = nullptr;
+  bool yield_pending = false;
   double kt_ms[WF_K_COUNT_] = {0};
   int64_t kt_n[WF_K_COUNT_] = {0};
 
@@ -160,6 +175,7 @@ struct wf_domain {
   int alloc_plane_f(float** p);
   void refresh_mirrors();
   int mlp_train_round(int iters, float lr);
+  int relist_local();
   int mlp_fit(const float* X, const float* Y, int n, int stride, int iters, float lr);
   int dev_alloc(void** p, size_t bytes);
   int surface_init();
@@ -475,12 +491,18 @@ int wf_domain::mlp_train_round(int iters, float lr) {
 int wf_domain::mlp_fit(const float* X, const float* Y, int nsamp, int stride, int iters,
                        float lr) {
   const int nblocks = mlp_grad_blocks(nsamp);
-  for (int it = 0; it < iters; it++)
+  for (int it = 0; it < iters; it++) {
     launch_mlp_train_iter(stream, mlp_w, mlp_m, mlp_v, X, Y, nsamp, stride, mlp_part,
                           mlp_loss_part, nblocks, ++adam_t, lr, mlp_loss);
-  HIPC(hipMemcpyAsync(host_f, mlp_loss, sizeof(float), hipMemcpyDeviceToHost, stream));
+    // the first iteration's loss is that of the weights the round started from
+    if (it == 0)
+      HIPC(hipMemcpyAsync(mlp_loss + 1, mlp_loss, sizeof(float), hipMemcpyDeviceToDevice,
+                          stream));
+  }
+  HIPC(hipMemcpyAsync(host_f, mlp_loss, 2 * sizeof(float), hipMemcpyDeviceToHost, stream));
   HIPC(hipStreamSynchronize(stream));
   mlp_last_loss = host_f[0];
+  mlp_start_loss = iters > 0 ? host_f[1] : host_f[0];
   return WF_OK;
 }
 
@@ -605,6 +627,10 @@ void wf_config_default(wf_config* c) {
   c->newton_max_iter = 50;
   c->newton_tol = 1e-11;
   c->mlp_train_interval = 200;
+  c->mlp_train_iters = 250;
+  c->mlp_lr = 2e-3;
+  c->mlp_interval_max = 0;
+  c->mlp_refit_ratio = 2.0;
   c->contact = 0;
   c->contact_law = WF_CONTACT_REFERENCE;
   c->contact_kappa = 0.1;
@@ -613,6 +639,7 @@ void wf_config_default(wf_config* c) {
   c->fric_heat = 0.0;
   c->ns_skin = 0.0;
   c->ep_hot = 0.05;
+  c->ns_global_every = 32;
 }
 
 int wf_field_shape(int field, int* components, int* elem_bytes) {
@@ -638,10 +665,16 @@ int wf_create(const wf_config* cfg, wf_domain** out) {
     return fail(WF_E_ARG, "unknown contact law");
   if (!(cfg->fric_heat >= 0.0 && cfg->fric_heat <= 1.0))
     return fail(WF_E_ARG, "fric_heat must lie in [0, 1]");
-  if (cfg->rebuild_policy != WF_REBUILD_EVERY_N && cfg->rebuild_policy != WF_REBUILD_PLASTIC_ZONE)
+  if (cfg->rebuild_policy != WF_REBUILD_EVERY_N && cfg->rebuild_policy != WF_REBUILD_PLASTIC_ZONE &&
+      cfg->rebuild_policy != WF_REBUILD_PLASTIC_ZONE_LOCAL)
     return fail(WF_E_ARG, "unknown rebuild policy");
-  if (cfg->rebuild_policy == WF_REBUILD_PLASTIC_ZONE && !(cfg->ns_skin > 0.0 && cfg->ep_hot >= 0.0))
+  if (cfg->rebuild_policy != WF_REBUILD_EVERY_N && !(cfg->ns_skin > 0.0 && cfg->ep_hot >= 0.0))
     return fail(WF_E_ARG, "the plastic-zone rebuild needs ns_skin > 0 and ep_hot >= 0");
+  if (cfg->rebuild_policy == WF_REBUILD_PLASTIC_ZONE_LOCAL && cfg->ns_skin > 2.0 * cfg->h_glob)
+    return fail(WF_E_ARG, "the local re-list reaches two cells: ns_skin must not exceed 2 h_glob");
+  if (cfg->mlp_train_iters < 0 || !(cfg->mlp_lr > 0.0) || cfg->mlp_interval_max < 0 ||
+      !(cfg->mlp_refit_ratio >= 0.0))
+    return fail(WF_E_ARG, "surrogate training: iters >= 0, lr > 0, interval cap >= 0, ratio >= 0");
   if (cfg->contact && cfg->contact_law == WF_CONTACT_STIFFNESS &&
       !(cfg->contact_kappa > 0.0 && cfg->contact_dx > 0.0))
     return fail(WF_E_ARG, "the stiffness contact law needs contact_kappa > 0 and contact_dx > 0");
@@ -734,7 +767,7 @@ int wf_create(const wf_config* cfg, wf_domain** out) {
   if (!rc) rc = d->alloc_plane_i(&d->f.idlock);
   if (!rc) rc = d->dev_alloc((void**)&d->cscratch, sizeof(double) * (8 + kMaxParts));
   if (!rc && hipHostMalloc(&d->cscratch_h, sizeof(double) * 8) != hipSuccess) rc = WF_E_NOMEM;
-  if (cfg->rebuild_policy == WF_REBUILD_PLASTIC_ZONE) {
+  if (d->plastic_zone()) {
     for (int c = 0; c < 3 && !rc; c++) rc = d->alloc_plane_d(&d->xb[c]);
     if (!rc) rc = d->dev_alloc((void**)&d->pz_d2, sizeof(unsigned long long));
     if (!rc && hipHostMalloc(&d->pz_d2_host, sizeof(unsigned long long)) != hipSuccess)
@@ -768,6 +801,8 @@ int wf_create(const wf_config* cfg, wf_domain** out) {
       d->sum_out = d->sum_part + 1024;
     }
   }
+  if (!rc && hipEventCreateWithFlags(&d->ev_yield, hipEventDisableTiming) != hipSuccess)
+    rc = WF_E_HIP;
   if (!rc && (hipEventCreate(&d->ev0) != hipSuccess || hipEventCreate(&d->ev1) != hipSuccess))
     rc = WF_E_HIP;
   if (rc) {
@@ -797,6 +832,7 @@ int wf_destroy(wf_domain* d) {
   ns_free(d->ns);
   if (d->ev0) hipEventDestroy(d->ev0);
   if (d->ev1) hipEventDestroy(d->ev1);
+  if (d->ev_yield) hipEventDestroy(d->ev_yield);
   if (d->stream) hipStreamDestroy(d->stream);
   delete d;
   return WF_OK;
@@ -973,6 +1009,36 @@ static int timer_end(wf_domain* d, int k) {
   return WF_OK;
 }
 
+// WF_REBUILD_PLASTIC_ZONE_LOCAL: instead of a rebuild (cell sort, reorder of
+// every field, every tile's lists), re-list in place the tiles within two
+// search cells of a particle with eps_p > ep_hot, from a cell sort of the
+// current positions that only indexes the particles.  Their build
+// positions restart.  Every hot particle lies in such a tile, so after a
+// re-list every hot particle's displacement counts from its new lists, and
+// a tile left alone is farther than 2h + skin from any hot particle.
+// Returns WF_E_STATE when a full rebuild is due instead: under the other
+// policies, before the first build, after ns_global_every re-lists (the
+// tiles drift out of their cells as the particles stay in place), or when a
+// re-listed tile outgrows the staging region its last rebuild gave it.
+int wf_domain::relist_local() {
+  if (cfg.rebuild_policy != WF_REBUILD_PLASTIC_ZONE_LOCAL || !nb_built) return WF_E_STATE;
+  const int every = cfg.ns_global_every > 0 ? cfg.ns_global_every : 32;
+  if (local_since_reorder >= every) return WF_E_STATE;
+  timer_begin(this);
+  const double* xs[3] = {f.x[0], f.x[1], f.x[2]};
+  const double radius = 2.0 * cfg.h_glob + cfg.ns_skin;
+  int tiles = 0;
+  int rc = ns_relist_local(stream, ns, xs, f.pl_strain, cfg.ep_hot, n, npad, grid, radius,
+                           kListCap, perm, nb_count, ctr, xb, &tiles);
+  if (rc == WF_E_NBCAP) return WF_E_STATE;  // a tile outgrew its region: full rebuild
+  if (rc) return fail(rc, "local neighbour re-list failed");
+  pz_pending = false;
+  local_since_reorder++;
+  ns_local_builds++;
+  ns_local_tiles += tiles;
+  return timer_end(this, WF_K_NSEARCH);
+}
+
 int wf_nsearch_build(wf_domain* d) {
   if (!d) return fail(WF_E_ARG, "null domain");
   // the plastic-zone policy builds from the current positions with a skin
@@ -1031,6 +1097,7 @@ int wf_nsearch_build(wf_domain* d) {
       HIPC(hipMemcpyAsync(d->xb[c], d->f.x[c], sizeof(double) * d->n, hipMemcpyDeviceToDevice,
                           d->stream));
   d->pz_pending = false;
+  d->local_since_reorder = 0;
   HIPC(hipGetLastError());
   d->nb_built = true;
   d->built_gen = d->snap_gen;
@@ -1456,12 +1523,36 @@ int wf_contact_force_sum(wf_domain* d, double* sum) {
 // Whether the next step starts with a neighbour rebuild that actually runs
 // (SolverFraser.cu:166; under the stale-snapshot quirk a rebuild from an
 // unchanged snapshot is skipped, see wf_nsearch_build).
-static bool rebuild_next_step(const wf_domain* d) {
-  if (d->plastic_zone()) return d->is_yielding && d->pz_pending;
+// Before yield each step's max eps_p is copied back behind its stress pass;
+// the host reads it (SolverFraser.cu:260-271) only when it must, by then
+// with the step's later passes queued, so the device does not wait for it.
+static int resolve_yield(wf_domain* d) {
+  if (!d->yield_pending) return WF_OK;
+  HIPC(hipEventSynchronize(d->ev_yield));
+  d->yield_pending = false;
+  double mxp;
+  const unsigned long long b = d->ctr_host->max_pl_strain_bits;
+  std::memcpy(&mxp, &b, sizeof(mxp));
+  if (mxp > d->cfg.ns_min_plastic) d->is_yielding = true;
+  return WF_OK;
+}
+
+// whether the next step starts with a rebuild that changes the lists; the
+// yield read-back is waited for only when the answer depends on it
+static int rebuild_next_step(wf_domain* d, bool* out) {
+  *out = false;
+  if (d->plastic_zone()) {
+    *out = d->is_yielding && d->pz_pending;
+    return WF_OK;
+  }
   const int ts_next = d->ts_i + 1 > d->cfg.ns_interval - 1 ? 0 : d->ts_i + 1;
-  if (ts_next != 0 || !d->is_yielding) return false;
+  if (ts_next != 0) return WF_OK;
   const bool stale = (d->cfg.quirks & WF_Q_STALE_NS_POS) != 0;
-  return !(stale && d->nb_built && d->built_gen == d->snap_gen);
+  if (stale && d->nb_built && d->built_gen == d->snap_gen) return WF_OK;  // a no-op rebuild
+  int rc = resolve_yield(d);
+  if (rc) return rc;
+  *out = d->is_yielding;
+  return WF_OK;
 }
 
 // Domain_d::MechFraserSolve loop body, SolverFraser.cu:164-399.  Inside one
@@ -1482,7 +1573,8 @@ int wf_step(wf_domain* d, int n_steps) {
     if (d->cfg.mechanics) {
       if (d->plastic_zone() ? (d->is_yielding && d->pz_pending)
                             : (d->ts_i == 0 && d->is_yielding)) {
-        rc = wf_nsearch_build(d);
+        rc = d->relist_local();  // WF_E_STATE: a full rebuild is due
+        if (rc == WF_E_STATE) rc = wf_nsearch_build(d);
         if (rc) return rc;
       }
       d->refresh_mirrors();
@@ -1528,15 +1620,13 @@ int wf_step(wf_domain* d, int n_steps) {
       if (!d->is_yielding) {  // SolverFraser.cu:260-271, on the device-side max
         HIPC(hipMemcpyAsync(d->ctr_host, d->ctr, sizeof(Counters), hipMemcpyDeviceToHost,
                             d->stream));
-        HIPC(hipStreamSynchronize(d->stream));
-        double mxp;
-        unsigned long long b = d->ctr_host->max_pl_strain_bits;
-        std::memcpy(&mxp, &b, sizeof(mxp));
-        if (mxp > d->cfg.ns_min_plastic) d->is_yielding = true;
+        HIPC(hipEventRecord(d->ev_yield, d->stream));
+        d->yield_pending = true;
       }
       timer_begin(d);
       launch_integrate(d->stream, d->f, d->n, d->cfg.dt, d->bcs, rec, d->frame);
       if ((rc = timer_end(d, WF_K_INTEGRATE))) return rc;
+      if (d->plastic_zone() && (rc = resolve_yield(d))) return rc;
       if (d->plastic_zone() && d->is_yielding) {  // the rebuild trigger, read back each step
         hipLaunchKernelGGL(k_zero_bytes, dim3(1), dim3(256), 0, d->stream,
                            (unsigned char*)d->pz_d2, (int64_t)sizeof(unsigned long long));
@@ -1551,7 +1641,10 @@ int wf_step(wf_domain* d, int n_steps) {
     }
     if (d->cfg.thermal) {
       d->refresh_mirrors();
-      const bool fuse = d->cfg.mechanics && s + 1 < n_steps && !rebuild_next_step(d);
+      bool rebuild_next = false;
+      if (d->cfg.mechanics && s + 1 < n_steps && (rc = rebuild_next_step(d, &rebuild_next)))
+        return rc;
+      const bool fuse = d->cfg.mechanics && s + 1 < n_steps && !rebuild_next;
       timer_begin(d);
       if (fuse && d->mixed) {
         launch_cond_dens_mixed(d->stream, d->f, d->tl, d->n, d->solid, p.T_inf, p.h_conv, p.dt,
@@ -1583,12 +1676,34 @@ int wf_step(wf_domain* d, int n_steps) {
       advance_mesh_centres(d, p.dt);
       if ((rc = timer_end(d, WF_K_CONTACT))) return rc;
     }
-    if (d->cfg.surrogate && d->jc_newton && d->is_yielding && d->cfg.mlp_train_interval > 0 &&
-        (d->step + 1) % d->cfg.mlp_train_interval == 0) {
-      timer_begin(d);
-      rc = d->mlp_train_round(250, 2e-3f);
-      if (rc) return rc;
-      if ((rc = timer_end(d, WF_K_MLP))) return rc;
+    if ((rc = resolve_yield(d))) return rc;  // the step's later passes are queued
+    if (d->cfg.surrogate && d->jc_newton && d->cfg.mlp_train_interval > 0) {
+      // rounds are due every mlp_interval steps, taken while yielding; the
+      // interval stays mlp_train_interval, or with mlp_interval_max > 0 it
+      // doubles (up to that cap) after a round that finds the network still
+      // fitting the new samples: its starting loss within mlp_refit_ratio of
+      // the previous round's final loss.  A round that does not resets it.
+      if (d->mlp_interval == 0) {
+        d->mlp_interval = d->cfg.mlp_train_interval;
+        d->mlp_next_train = d->mlp_interval;
+      }
+      if (d->step + 1 >= d->mlp_next_train) {
+        if (d->is_yielding) {
+          const int done = d->mlp_trainings;
+          const double prev = d->mlp_last_loss;
+          timer_begin(d);
+          rc = d->mlp_train_round(d->cfg.mlp_train_iters, (float)d->cfg.mlp_lr);
+          if (rc) return rc;
+          if ((rc = timer_end(d, WF_K_MLP))) return rc;
+          if (d->cfg.mlp_interval_max > 0 && d->mlp_trainings > done) {
+            const bool fits = done > 0 && d->mlp_start_loss <= d->cfg.mlp_refit_ratio * prev;
+            d->mlp_interval = fits ? std::min(2 * d->mlp_interval, std::max(
+                                                  d->cfg.mlp_interval_max, d->cfg.mlp_train_interval))
+                                   : d->cfg.mlp_train_interval;
+          }
+        }
+        d->mlp_next_train = d->step + 1 + d->mlp_interval;
+      }
     }
     d->isfirst_step = false;
     d->time += d->cfg.dt;
@@ -1617,6 +1732,10 @@ int wf_get_stats(wf_domain* d, wf_stats* out) {
   out->cont_over_cap = d->ctr_host->cont_over_cap;
   out->mlp_loss = d->mlp_last_loss;
   out->mlp_samples = d->mlp_last_samples;
+  out->mlp_interval = d->mlp_interval > 0 ? d->mlp_interval : d->cfg.mlp_train_interval;
+  out->mlp_start_loss = d->mlp_start_loss;
+  out->ns_local_builds = d->ns_local_builds;
+  out->ns_local_tiles = d->ns_local_tiles;
   return WF_OK;
 }
 

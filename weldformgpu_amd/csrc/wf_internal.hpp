@@ -300,6 +300,14 @@ struct NsResult {
 // lists in list/count (new order).  Returns WF_OK or an error code.
 // perm: sorted slot -> original index of the current order; perm_tmp receives
 // the new one.
+// Re-list, without reordering, the tiles within two cells of a particle with
+// eps_p > ep_hot, on the grid g of the last ns_build, and set their
+// particles' build positions xb to x; *relisted gets the tile count.
+// WF_E_NBCAP: a tile outgrew its staging region (a full build is needed).
+int ns_relist_local(hipStream_t s, NsWork* w, const double* const x[3], const double* pl_strain,
+                    double ep_hot, int n, int64_t npad, const GridDesc& g, double radius, int cap,
+                    const int* perm, int* count, Counters* ctr, double* const xb[3],
+                    int* relisted);
 // max_dim (may be null): the largest grid, in cells per axis, the caller can
 // represent; a larger one returns WF_E_UNSUPPORTED before anything changes
 int ns_build(hipStream_t s, NsWork** work, const double* const pos_src[3], int n, int64_t npad,

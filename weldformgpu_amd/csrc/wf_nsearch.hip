@@ -54,6 +54,16 @@ struct NsWork {
   int64_t tj_cap = 0;
   int* tile_j = nullptr;
   unsigned short* slot = nullptr;
+  int* tile_over = nullptr;  // particles over the list cap, per tile
+  // re-listing without a reorder: the cell sort of the current positions
+  // (sorted position -> storage index), the cells near the plastic zone,
+  // the tiles they touch and the list of those tiles
+  int* lgather = nullptr;
+  int* lperm = nullptr;
+  int* cell_flag = nullptr;
+  int* tile_aff = nullptr;
+  int* tile_pos = nullptr;
+  int* tile_list = nullptr;
 };
 
 constexpr int kScanBlock = 1024;
@@ -229,6 +239,69 @@ __global__ void k_ns_gather_f32(float* dst, const float* src, const int* idx, in
   if (i < n) dst[i] = src[idx[i]];
 }
 
+// *out = sum of v[0..n) (one block, a fixed order: deterministic)
+__global__ __launch_bounds__(256) void k_sum_tiles(const int* __restrict__ v, int n, int* out) {
+  __shared__ int sh[256];
+  int a = 0;
+  for (int i = threadIdx.x; i < n; i += 256) a += v[i];
+  sh[threadIdx.x] = a;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sh[0];
+}
+
+// Flag the cells within two cells (on each axis) of a particle with
+// eps_p > ep_hot: with cells of 1.01 r, r = 2h + skin, every particle that
+// such a particle can reach before it has moved skin/2 lies in them.
+__global__ void k_hot_cells(const float* p0, const float* p1, const float* p2, int n, Grid g,
+                            const double* __restrict__ pl_strain, double ep_hot, int* flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !(pl_strain[i] > ep_hot)) return;
+  int cx, cy, cz;
+  cell_of(g, p0[i], p1[i], p2[i], cx, cy, cz);
+  for (int z = max(cz - 2, 0); z <= min(cz + 2, g.dim[2] - 1); z++)
+    for (int y = max(cy - 2, 0); y <= min(cy + 2, g.dim[1] - 1); y++)
+      for (int x = max(cx - 2, 0); x <= min(cx + 2, g.dim[0] - 1); x++)
+        flag[cell_key(g, x, y, z)] = 1;
+}
+
+// aff[t] = 1 when a particle of tile t lies in a flagged cell
+__global__ __launch_bounds__(kTileP) void k_tiles_touched(const float* p0, const float* p1,
+                                                           const float* p2, int n, Grid g,
+                                                           const int* __restrict__ flag,
+                                                           int* aff) {
+  const int t = blockIdx.x, i = t * kTileP + threadIdx.x;
+  int hit = 0;
+  if (i < n) {
+    int cx, cy, cz;
+    cell_of(g, p0[i], p1[i], p2[i], cx, cy, cz);
+    hit = flag[cell_key(g, cx, cy, cz)];
+  }
+  hit = __syncthreads_or(hit);
+  if (threadIdx.x == 0) aff[t] = hit ? 1 : 0;
+}
+
+// list of the flagged tiles, ascending
+__global__ void k_tile_compact(const int* aff, const int* pos, int nt, int* list) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nt && aff[t]) list[pos[t]] = t;
+}
+
+// the build positions of the re-listed tiles' particles
+__global__ __launch_bounds__(kTileP) void k_tiles_xb(const double* x0, const double* x1,
+                                                     const double* x2, int n,
+                                                     const int* __restrict__ list, double* b0,
+                                                     double* b1, double* b2) {
+  const int i = list[blockIdx.x] * kTileP + threadIdx.x;
+  if (i >= n) return;
+  b0[i] = x0[i];
+  b1[i] = x1[i];
+  b2[i] = x2[i];
+}
+
 static int grid1(int64_t n, int b) { return (int)((n + b - 1) / b); }
 
 void launch_scan_exclusive(hipStream_t s, const int* in, int64_t m, int* part, int* out) {
@@ -256,13 +329,19 @@ static int ensure(NsWork* w, int n, int64_t nkeys) {
     if (hipMalloc(&w->key, sizeof(int) * cap) != hipSuccess) return WF_E_NOMEM;
     if (hipMalloc(&w->rank, sizeof(int) * cap) != hipSuccess) return WF_E_NOMEM;
     if (hipMalloc(&w->slot_orig, sizeof(int) * cap) != hipSuccess) return WF_E_NOMEM;
+    hipFree(w->lgather);
+    hipFree(w->lperm);
+    if (hipMalloc(&w->lgather, sizeof(int) * cap) != hipSuccess) return WF_E_NOMEM;
+    if (hipMalloc(&w->lperm, sizeof(int) * cap) != hipSuccess) return WF_E_NOMEM;
     w->n_cap = cap;
   }
   if (nkeys > w->key_cap) {
     hipFree(w->cell_count);
     hipFree(w->cell_start);
     hipFree(w->scan_part);
+    hipFree(w->cell_flag);
     int64_t cap = nkeys + nkeys / 4 + 1024;
+    if (hipMalloc(&w->cell_flag, sizeof(int) * cap) != hipSuccess) return WF_E_NOMEM;
     if (hipMalloc(&w->cell_count, sizeof(int) * cap) != hipSuccess) return WF_E_NOMEM;
     if (hipMalloc(&w->cell_start, sizeof(int) * (cap + 1)) != hipSuccess) return WF_E_NOMEM;
     if (hipMalloc(&w->scan_part, sizeof(int) * (grid1(cap, kScanBlock) + 1)) != hipSuccess)
@@ -278,6 +357,10 @@ static int ensure(NsWork* w, int n, int64_t nkeys) {
     hipFree(w->tile_uc);
     hipFree(w->tile_scan);
     hipFree(w->slot);
+    hipFree(w->tile_over);
+    hipFree(w->tile_aff);
+    hipFree(w->tile_pos);
+    hipFree(w->tile_list);
     int cap = std::max(nt, 1);
     int64_t npad = ((int64_t)cap * kTileP);
     if (hipMalloc(&w->tile_keys, sizeof(int) * (int64_t)cap * kTileMaxCells) != hipSuccess)
@@ -290,6 +373,10 @@ static int ensure(NsWork* w, int n, int64_t nkeys) {
       return WF_E_NOMEM;
     if (hipMalloc(&w->slot, sizeof(unsigned short) * 4 * kSlotQuads * npad) != hipSuccess)
       return WF_E_NOMEM;
+    if (hipMalloc(&w->tile_over, sizeof(int) * cap) != hipSuccess) return WF_E_NOMEM;
+    if (hipMalloc(&w->tile_aff, sizeof(int) * cap) != hipSuccess) return WF_E_NOMEM;
+    if (hipMalloc(&w->tile_pos, sizeof(int) * (cap + 1)) != hipSuccess) return WF_E_NOMEM;
+    if (hipMalloc(&w->tile_list, sizeof(int) * cap) != hipSuccess) return WF_E_NOMEM;
     w->nt_cap = cap;
   }
   if (!w->tile_ovf) {
@@ -328,6 +415,13 @@ void ns_free(NsWork* w) {
   hipFree(w->tile_ovf);
   if (w->tile_host) hipHostFree(w->tile_host);
   hipFree(w->tile_j);
+  hipFree(w->tile_over);
+  hipFree(w->lgather);
+  hipFree(w->lperm);
+  hipFree(w->cell_flag);
+  hipFree(w->tile_aff);
+  hipFree(w->tile_pos);
+  hipFree(w->tile_list);
   hipFree(w->slot);
   delete w;
 }
@@ -394,8 +488,8 @@ int ns_build(hipStream_t s, NsWork** work, const double* const pos_src[3], int n
   // tiles of the new order: occupied neighbour cells and staged counts
   const int nt = (n + kTileP - 1) / kTileP;
   const float* pfn[3] = {w->pfn[0], w->pfn[1], w->pfn[2]};
-  launch_tile_cells(s, pfn, n, g, w->cell_start, w->tile_keys, w->tile_nc, w->tile_ucnt,
-                    w->tile_ovf);
+  launch_tile_cells(s, pfn, n, g, w->cell_start, nullptr, nt, false, w->tile_keys, w->tile_nc,
+                    w->tile_ucnt, w->tile_ovf);
   int ntb = grid1(nt, kScanBlock);
   hipLaunchKernelGGL(k_scan_reduce, dim3(ntb), dim3(kScanBlock), 0, s, w->tile_ucnt,
                      (int64_t)nt, w->tile_scan);
@@ -417,8 +511,10 @@ int ns_build(hipStream_t s, NsWork** work, const double* const pos_src[3], int n
     if (hipMalloc(&w->tile_j, sizeof(int) * cap) != hipSuccess) return WF_E_NOMEM;
     w->tj_cap = cap;
   }
-  launch_tile_lists(s, pfn, n, g, w->cell_start, w->tile_keys, w->tile_nc, w->tile_uoff, r2,
-                    cap, npad, count, w->slot, w->tile_j, w->tile_uc, &ctr->nb_over_cap);
+  launch_tile_lists(s, pfn, nullptr, nullptr, nullptr, nt, n, g, w->cell_start, w->tile_keys,
+                    w->tile_nc, w->tile_uoff, r2, cap, npad, count, w->slot, w->tile_j,
+                    w->tile_uc, w->tile_over, w->tile_ovf);
+  hipLaunchKernelGGL(k_sum_tiles, dim3(1), dim3(256), 0, s, w->tile_over, nt, &ctr->nb_over_cap);
   if (hipGetLastError() != hipSuccess) return WF_E_HIP;
   *grid_out = g;
   tiles->count = count;
@@ -427,6 +523,70 @@ int ns_build(hipStream_t s, NsWork** work, const double* const pos_src[3], int n
   tiles->ucnt = w->tile_uc;
   tiles->tj = w->tile_j;
   tiles->npad = npad;
+  return WF_OK;
+}
+
+int ns_relist_local(hipStream_t s, NsWork* w, const double* const x[3], const double* pl_strain,
+                    double ep_hot, int n, int64_t npad, const GridDesc& g, double radius, int cap,
+                    const int* perm, int* count, Counters* ctr, double* const xb[3],
+                    int* relisted) {
+  *relisted = 0;
+  if (!w || w->n_cap < n) return WF_E_STATE;
+  const int64_t nkeys = (int64_t)g.mdim[0] * g.mdim[1] * g.mdim[2] * 512;
+  if (nkeys > w->key_cap) return WF_E_STATE;
+  const float rf = (float)radius;
+  const float r2 = rf * rf;
+  const int nt = (n + kTileP - 1) / kTileP;
+  // the cell sort of the current positions, on the last reorder's grid;
+  // the particles stay where they are (the sorted order only indexes them)
+  hipLaunchKernelGGL(k_ns_to_float, dim3(grid1(n, 256)), dim3(256), 0, s, x[0], x[1], x[2],
+                     w->pf[0], w->pf[1], w->pf[2], n);
+  hipLaunchKernelGGL(k_ns_fill_i32, dim3(grid1(nkeys + 1, 256)), dim3(256), 0, s, w->cell_count, 0,
+                     nkeys + 1);
+  hipLaunchKernelGGL(k_ns_fill_i32, dim3(grid1(nkeys + 1, 256)), dim3(256), 0, s, w->cell_flag, 0,
+                     nkeys + 1);
+  hipLaunchKernelGGL(k_ns_keys, dim3(grid1(n, 256)), dim3(256), 0, s, w->pf[0], w->pf[1],
+                     w->pf[2], n, g, w->key, w->rank, w->cell_count);
+  launch_scan_exclusive(s, w->cell_count, nkeys, w->scan_part, w->cell_start);
+  hipLaunchKernelGGL(k_ns_place, dim3(grid1(n, 256)), dim3(256), 0, s, w->key, w->rank,
+                     w->cell_start, perm, n, w->slot_orig);
+  hipLaunchKernelGGL(k_ns_order, dim3(grid1(n, 256)), dim3(256), 0, s, w->key, w->cell_start,
+                     perm, w->slot_orig, n, w->lperm, w->lgather);
+  for (int c = 0; c < 3; c++)
+    hipLaunchKernelGGL(k_ns_gather_f32, dim3(grid1(n, 256)), dim3(256), 0, s, w->pfn[c], w->pf[c],
+                       w->lgather, n);
+  // the tiles within reach of the plastic zone
+  hipLaunchKernelGGL(k_hot_cells, dim3(grid1(n, 256)), dim3(256), 0, s, w->pf[0], w->pf[1],
+                     w->pf[2], n, g, pl_strain, ep_hot, w->cell_flag);
+  hipLaunchKernelGGL(k_tiles_touched, dim3(nt), dim3(kTileP), 0, s, w->pf[0], w->pf[1], w->pf[2],
+                     n, g, w->cell_flag, w->tile_aff);
+  launch_scan_exclusive(s, w->tile_aff, nt, w->tile_scan, w->tile_pos);
+  hipLaunchKernelGGL(k_tile_compact, dim3(grid1(nt, 256)), dim3(256), 0, s, w->tile_aff,
+                     w->tile_pos, nt, w->tile_list);
+  if (hipMemcpyAsync(w->tile_host, w->tile_pos + nt, sizeof(int), hipMemcpyDeviceToHost, s) !=
+      hipSuccess)
+    return WF_E_HIP;
+  if (hipStreamSynchronize(s) != hipSuccess) return WF_E_HIP;
+  const int ntl = w->tile_host[0];
+  *relisted = ntl;
+  if (ntl == 0) return WF_OK;
+  hipLaunchKernelGGL(k_ns_fill_i32, dim3(1), dim3(64), 0, s, w->tile_ovf, 0, (int64_t)1);
+  const float* pfn[3] = {w->pfn[0], w->pfn[1], w->pfn[2]};
+  const float* own[3] = {w->pf[0], w->pf[1], w->pf[2]};
+  launch_tile_cells(s, own, n, g, w->cell_start, w->tile_list, ntl, true, w->tile_keys,
+                    w->tile_nc, w->tile_ucnt, w->tile_ovf);
+  launch_tile_lists(s, pfn, own, w->lgather, w->tile_list, ntl, n, g, w->cell_start,
+                    w->tile_keys, w->tile_nc, w->tile_uoff, r2, cap, npad, count, w->slot,
+                    w->tile_j, w->tile_uc, w->tile_over, w->tile_ovf);
+  if (hipMemcpyAsync(w->tile_host + 1, w->tile_ovf, sizeof(int), hipMemcpyDeviceToHost, s) !=
+      hipSuccess)
+    return WF_E_HIP;
+  if (hipStreamSynchronize(s) != hipSuccess) return WF_E_HIP;
+  if (w->tile_host[1] != 0) return WF_E_NBCAP;  // a tile outgrew its region: reorder
+  hipLaunchKernelGGL(k_sum_tiles, dim3(1), dim3(256), 0, s, w->tile_over, nt, &ctr->nb_over_cap);
+  hipLaunchKernelGGL(k_tiles_xb, dim3(ntl), dim3(kTileP), 0, s, x[0], x[1], x[2], n,
+                     w->tile_list, xb[0], xb[1], xb[2]);
+  if (hipGetLastError() != hipSuccess) return WF_E_HIP;
   return WF_OK;
 }
 

@@ -16,7 +16,6 @@
 // policies (DensD/DensG, RatesD/RatesG, AccelD/AccelG, CondD/CondG, SurfD,
 // CondDens) say what a neighbour record holds and what a pair adds.
 #include <climits>
-#include <type_traits>
 
 #include "wf_device.hpp"
 #include "wf_tiles.hpp"
@@ -285,10 +284,14 @@ struct RatesD {
 // Pi_ij = (alpha C mu + beta mu^2)/rho_bar for approaching pairs,
 // mu = h v.x/(r^2 + 0.01 h^2) (InteractionAlt.cu:18-150).  sigma_j/rho_j^2
 // comes formed in the neighbour's record.
+// neighbour records (96 B each) a lane holds in flight in the FP64 accel loop
+#ifndef WF_ACCEL64_NBATCH
+#define WF_ACCEL64_NBATCH 2
+#endif
 template <bool UH>
 struct AccelD {
   WF_GATHER64
-  static constexpr int NBATCH = 2;
+  static constexpr int NBATCH = WF_ACCEL64_NBATCH;
   struct Prm {
     int limit;
     double alpha, beta;
@@ -334,9 +337,14 @@ struct AccelD {
     const double vdx = vx * xx + vy * xy + vz * xz;
     double PI = 0.0;
     if (vdx < 0.0 && (p.alpha != 0.0 || p.beta != 0.0)) {
-      const double MU = h_ * vdx / (r2 + 0.01 * h_ * h_);
+      // (alpha C mu + beta mu^2)/rho_bar with mu = h vdx/e, e = r^2 + 0.01 h^2,
+      // as 2 h vdx (alpha C + beta mu)/(e (rho_i + rho_j)): one division
+      // where the reference's form takes two (the beta term keeps its own)
+      const double e = r2 + 0.01 * h_ * h_;
       const double Cij = 0.5 * (o.Ci + nb.sb.w);
-      PI = (p.alpha * Cij * MU + p.beta * MU * MU) / (0.5 * (o.di + nb.sb.z));
+      double coef = p.alpha * Cij;
+      if (p.beta != 0.0) coef += p.beta * (h_ * vdx / e);
+      PI = 2.0 * h_ * vdx * coef / (e * (o.di + nb.sb.z));
     }
     const double g = GK * nb.v.w;
     const double gx = g * xx, gy = g * xy, gz = g * xz;
@@ -563,24 +571,13 @@ __device__ __forceinline__ T lane_value_xor(const T& v, int m) {
   return r;
 }
 
-// waves per SIMD a policy asks the compiler to leave room for (its register
-// budget: 512 / waves VGPRs); 1, the default, sets no limit
-template <class K, class = void>
-struct waves_of {
-  static constexpr int v = 1;
-};
-template <class K>
-struct waves_of<K, std::void_t<decltype(K::WAVES)>> {
-  static constexpr int v = K::WAVES;
-};
-
 // One workgroup per tile, L = K::LANES consecutive lanes per particle: lanes
 // L p .. L p + L - 1 all hold particle p and deal its slot list out by quads
 // (lane L p + h takes the quads q with q % L == h); their sums are merged
 // (butterfly over the L lanes) before lane L p writes the result.
 template <class K>
-__global__ __launch_bounds__(4 * kTileP) __attribute__((amdgpu_waves_per_eu(waves_of<K>::v)))
-void k_pair_gather(Fields f, TileLists tl, typename K::Prm prm) {
+__global__ __launch_bounds__(4 * kTileP) void k_pair_gather(Fields f, TileLists tl,
+                                                            typename K::Prm prm) {
   constexpr int ST = K::STAGE;
   constexpr int L = K::LANES;
   static_assert(L == 1 || L == 2 || L == 4, "1, 2 or 4 lanes per particle");
@@ -872,17 +869,13 @@ struct RatesG {
 
 // D: acceleration with Monaghan viscosity (InteractionAlt.cu:18-150); the
 // neighbour's mass is (m/rho) rho
-#ifndef WF_ACCEL32_WAVES
-#define WF_ACCEL32_WAVES 1
-#endif
 template <bool UH>
 struct AccelG {
   WF_GATHER32
   static constexpr int NBATCH = 4;
-  static constexpr int WAVES = WF_ACCEL32_WAVES;
   struct Prm {
     int limit;
-    double alpha, beta;
+    float alpha32, beta32;
     float cs;
   };
   struct Own {
@@ -924,13 +917,16 @@ struct AccelG {
     const float GK = gk_pair_f<UH>(r2, nb.h, o.hc, h_);
     const float vx = o.vx - nb.v.x, vy = o.vy - nb.v.y, vz = o.vz - nb.v.z;
     const float vdx = vx * xx + vy * xy + vz * xz;
-    float PI = 0.f;
-    if (vdx < 0.f && (p.alpha != 0.0 || p.beta != 0.0)) {
-      const float MU = h_ * vdx * __builtin_amdgcn_rcpf(r2 + 0.01f * h_ * h_);
-      const float Cij = 0.5f * (o.Ci + nb.sb.w);
-      PI = ((float)p.alpha * Cij * MU + (float)p.beta * MU * MU) *
-           __builtin_amdgcn_rcpf(0.5f * (o.di + nb.sb.z));
-    }
+    // Monaghan viscosity on approaching pairs only: (alpha C mu + beta mu^2)
+    // / rho_bar with mu = h v/e, v = min(v_ij.x_ij, 0), e = r^2 + 0.01 h^2,
+    // as 2 h v (alpha C + beta mu) / (e (rho_i + rho_j)): one reciprocal and
+    // no branch (receding pairs get v = 0)
+    const float vn = fminf(vdx, 0.f);
+    const float rs = o.di + nb.sb.z;
+    const float inv = __builtin_amdgcn_rcpf((r2 + 0.01f * h_ * h_) * rs);
+    float coef = p.alpha32 * (0.5f * (o.Ci + nb.sb.w));
+    if (p.beta32 != 0.f) coef += p.beta32 * (h_ * vn * rs * inv);
+    const float PI = 2.f * h_ * vn * coef * inv;
     const float g = GK * (nb.v.w * nb.sb.z);
     const float gx = g * xx, gy = g * xy, gz = g * xz;
     const float mxx = o.s0 + nb.sa.x + PI, myy = o.s1 + nb.sa.y + PI;
@@ -1324,7 +1320,7 @@ void launch_rates_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int
 void launch_accel_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
                         double alpha, double beta, float cs32, bool uh) {
   if (solid <= 0) return;
-  launch_gather<AccelG>(s, f, tl, ntiles(solid), uh, solid, alpha, beta, cs32);
+  launch_gather<AccelG>(s, f, tl, ntiles(solid), uh, solid, (float)alpha, (float)beta, cs32);
 }
 void launch_cond_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int n, double T_inf,
                        double h_conv, double dt, bool leap, bool first, float cs32, bool uh) {

@@ -26,23 +26,43 @@ namespace wf {
 constexpr int kSortN = 4096;  // >= kTileP * 27, power of two
 static_assert(kTileP * 27 <= kSortN, "sort buffer too small");
 
+// The cell coordinates of a cell key (the inverse of cell_key).
+__device__ __forceinline__ void key_cell(const GridDesc& g, int key, int& cx, int& cy, int& cz) {
+  const int meta = key >> 9, m = key & 511;
+  int x = 0, y = 0, z = 0;
+#pragma unroll
+  for (int b = 0; b < 3; b++) {
+    x |= ((m >> (3 * b)) & 1) << b;
+    y |= ((m >> (3 * b + 1)) & 1) << b;
+    z |= ((m >> (3 * b + 2)) & 1) << b;
+  }
+  const int mx = meta % g.mdim[0], myz = meta / g.mdim[0];
+  cx = 8 * mx + x;
+  cy = 8 * (myz % g.mdim[1]) + y;
+  cz = 8 * (myz / g.mdim[1]) + z;
+}
+
 // Per tile: sorted unique occupied neighbour cells and the staged count.
-// The tile's particles are in cell order, so its own cells are the runs of
-// equal keys; only those few cells (typically 10-20) are expanded to their
-// 27 neighbours, and the candidate keys are sorted in the smallest power of
-// two that holds them.
+// After a reorder the tile's particles are in cell order, so its own cells
+// are the runs of equal keys; a tile re-listed without a reorder
+// (SORT_OWN) first sorts its particles' keys.  Only the tile's few distinct
+// cells (typically 10-20) are expanded to their 27 neighbours, and the
+// candidate keys are sorted in the smallest power of two that holds them.
+// tiles: the tiles to build (null: block b builds tile b).
+template <bool SORT_OWN>
 __global__ __launch_bounds__(kTileP) void k_tile_cells(const float* __restrict__ p0,
                                                        const float* __restrict__ p1,
                                                        const float* __restrict__ p2, int n,
                                                        GridDesc g,
                                                        const int* __restrict__ cell_start,
+                                                       const int* __restrict__ tiles,
                                                        int* tile_keys, int* tile_nc,
                                                        int* tile_ucnt, int* overflow) {
   __shared__ int keys[kSortN];
   __shared__ int segcnt[kTileP];
   __shared__ int own[kTileP];   // the key of each particle's cell
   __shared__ int wcnt[kTileP / 64 + 1];
-  const int t = blockIdx.x;
+  const int t = tiles ? tiles[blockIdx.x] : blockIdx.x;
   const int tid = threadIdx.x;
   const int i = t * kTileP + tid;
   int cx = 0, cy = 0, cz = 0, kown = INT_MAX;
@@ -52,6 +72,25 @@ __global__ __launch_bounds__(kTileP) void k_tile_cells(const float* __restrict__
   }
   own[tid] = kown;
   __syncthreads();
+  if constexpr (SORT_OWN) {
+    for (int size = 2; size <= kTileP; size <<= 1) {
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        if (tid < kTileP / 2) {
+          const int lo = 2 * tid - (tid & (stride - 1));
+          const int hi = lo + stride;
+          const bool up = (lo & size) == 0;
+          const int a = own[lo], b = own[hi];
+          if ((a > b) == up) {
+            own[lo] = b;
+            own[hi] = a;
+          }
+        }
+        __syncthreads();
+      }
+    }
+    kown = own[tid];
+    if (kown != INT_MAX) key_cell(g, kown, cx, cy, cz);
+  }
   // first particle of each run of equal cells (keys ascend along the tile)
   const bool first = kown != INT_MAX && (tid == 0 || own[tid - 1] != kown);
   const unsigned long long bal = __ballot(first);
@@ -206,19 +245,28 @@ __device__ __forceinline__ int staged_cell(const int* spre, int nc, int q) {
   return lo;
 }
 
+// p0..p2: the positions of the cell-sorted order (candidates); o0..o2: the
+// positions of the tiles' own particles by storage index, and map[s] the
+// storage index at sorted position s (both null right after a reorder,
+// where the two orders are one).  tiles: the tiles to build (null: all).
+// With map, each tile's staged set must fit the region it was given at the
+// last reorder; a tile whose set does not raises *overflow.
 __global__ __launch_bounds__(kListThreads) void k_tile_lists(
     const float* __restrict__ p0, const float* __restrict__ p1, const float* __restrict__ p2,
+    const float* __restrict__ o0, const float* __restrict__ o1, const float* __restrict__ o2,
+    const int* __restrict__ map, const int* __restrict__ tiles,
     int n, GridDesc g, const int* __restrict__ cell_start, const int* __restrict__ tile_keys,
     const int* __restrict__ tile_nc, const int* __restrict__ tile_uoff, float r2, int cap,
     int64_t npad, int* __restrict__ count, unsigned short* __restrict__ slot,
-    int* __restrict__ tile_j, int* __restrict__ tile_uc, int* over_cap) {
+    int* __restrict__ tile_j, int* __restrict__ tile_uc, int* __restrict__ tile_over,
+    int* overflow) {
 #pragma clang fp contract(off)
   constexpr int NT = kListThreads;
   __shared__ int sb[kTileMaxCells], sn[kTileMaxCells], spre[kTileMaxCells];
   __shared__ float4 spos[kListStage];  // candidate position, particle index in w
   __shared__ short sidx[kCompactMax];  // referenced flag, then compacted position (or -1)
   __shared__ int wsum[NT / 64];
-  const int t = blockIdx.x;
+  const int t = tiles ? tiles[blockIdx.x] : blockIdx.x;
   const int tid = threadIdx.x;
   const int sub = tid & 1;  // lanes 2p and 2p+1 hold particle p
   const int nc = min(tile_nc[t], kTileMaxCells);
@@ -250,9 +298,9 @@ __global__ __launch_bounds__(kListThreads) void k_tile_lists(
   int c = 0;  // the particle's list length so far (the same in both lanes)
   float xi = 0.f, yi = 0.f, zi = 0.f;
   if (act) {
-    xi = p0[i];
-    yi = p1[i];
-    zi = p2[i];
+    xi = o0 ? o0[i] : p0[i];
+    yi = o1 ? o1[i] : p1[i];
+    zi = o2 ? o2[i] : p2[i];
   }
   for (int c0 = 0; c0 < U; c0 += kListStage) {
     const int m = min(kListStage, U - c0);
@@ -336,35 +384,58 @@ __global__ __launch_bounds__(kListThreads) void k_tile_lists(
       }
     }
   }
-  for (int s = tid; s < U; s += NT) {
-    const int d = compact ? sidx[s] : s;
-    if (d >= 0) {
-      const int p = staged_cell(spre, nc, s);
-      tile_j[uoff + d] = sb[p] + (s - spre[p]);
+  // the region this tile was given at the last reorder
+  const int room = map ? tile_uoff[t + 1] - uoff : INT_MAX;
+  if (used > room) {
+    if (tid == 0) atomicAdd(overflow, 1);
+  } else {
+    for (int s = tid; s < U; s += NT) {
+      const int d = compact ? sidx[s] : s;
+      if (d >= 0) {
+        const int p = staged_cell(spre, nc, s);
+        const int js = sb[p] + (s - spre[p]);
+        tile_j[uoff + d] = map ? map[js] : js;
+      }
     }
   }
   if (tid == 0) tile_uc[t] = used;
-  const int over = act && sub == 0 && c > cap;  // one count per particle over the cap
-  unsigned long long o = wave_sum_u64((unsigned long long)over);
-  if (lane_id() == 0 && o) atomicAdd(over_cap, (int)o);
+  // particles of the tile over the cap, one count each
+  const int over = act && sub == 0 && c > cap;
+  const unsigned long long o = wave_sum_u64((unsigned long long)over);
+  if (lane_id() == 0) wsum[threadIdx.x >> 6] = (int)o;
+  __syncthreads();
+  if (tid == 0) {
+    int tot = 0;
+    for (int u = 0; u < NT / 64; u++) tot += wsum[u];
+    tile_over[t] = tot;
+  }
 }
 
 void launch_tile_cells(hipStream_t s, const float* const pf[3], int n, const GridDesc& g,
-                       const int* cell_start, int* tile_keys, int* tile_nc, int* tile_ucnt,
-                       int* overflow) {
-  int nt = (n + kTileP - 1) / kTileP;
-  hipLaunchKernelGGL(k_tile_cells, dim3(nt), dim3(kTileP), 0, s, pf[0], pf[1], pf[2], n, g,
-                     cell_start, tile_keys, tile_nc, tile_ucnt, overflow);
+                       const int* cell_start, const int* tiles, int ntiles, bool sort_own,
+                       int* tile_keys, int* tile_nc, int* tile_ucnt, int* overflow) {
+  if (ntiles <= 0) return;
+  if (sort_own)
+    hipLaunchKernelGGL(k_tile_cells<true>, dim3(ntiles), dim3(kTileP), 0, s, pf[0], pf[1], pf[2],
+                       n, g, cell_start, tiles, tile_keys, tile_nc, tile_ucnt, overflow);
+  else
+    hipLaunchKernelGGL(k_tile_cells<false>, dim3(ntiles), dim3(kTileP), 0, s, pf[0], pf[1],
+                       pf[2], n, g, cell_start, tiles, tile_keys, tile_nc, tile_ucnt, overflow);
 }
 
-void launch_tile_lists(hipStream_t s, const float* const pf[3], int n, const GridDesc& g,
+void launch_tile_lists(hipStream_t s, const float* const pf[3], const float* const own[3],
+                       const int* map, const int* tiles, int ntiles, int n, const GridDesc& g,
                        const int* cell_start, const int* tile_keys, const int* tile_nc,
                        const int* tile_uoff, float r2, int cap, int64_t npad, int* count,
-                       unsigned short* slot, int* tile_j, int* tile_uc, int* over_cap) {
-  int nt = (n + kTileP - 1) / kTileP;
-  hipLaunchKernelGGL(k_tile_lists, dim3(nt), dim3(kListThreads), 0, s, pf[0], pf[1], pf[2], n, g,
-                     cell_start, tile_keys, tile_nc, tile_uoff, r2, cap, npad, count, slot,
-                     tile_j, tile_uc, over_cap);
+                       unsigned short* slot, int* tile_j, int* tile_uc, int* tile_over,
+                       int* overflow) {
+  if (ntiles <= 0) return;
+  const float* o0 = own ? own[0] : nullptr;
+  const float* o1 = own ? own[1] : nullptr;
+  const float* o2 = own ? own[2] : nullptr;
+  hipLaunchKernelGGL(k_tile_lists, dim3(ntiles), dim3(kListThreads), 0, s, pf[0], pf[1], pf[2],
+                     o0, o1, o2, map, tiles, n, g, cell_start, tile_keys, tile_nc, tile_uoff, r2,
+                     cap, npad, count, slot, tile_j, tile_uc, tile_over, overflow);
 }
 
 }  // namespace wf

@@ -66,12 +66,20 @@ struct TileLists {
   int64_t npad;
 };
 
+// tiles/ntiles: the tiles to build (tiles null: tiles 0..ntiles-1);
+// sort_own: the tiles' particles are not in cell order (no reorder since the
+// last build)
 void launch_tile_cells(hipStream_t s, const float* const pf[3], int n, const GridDesc& g,
-                       const int* cell_start, int* tile_keys, int* tile_nc, int* tile_ucnt,
-                       int* overflow);
-void launch_tile_lists(hipStream_t s, const float* const pf[3], int n, const GridDesc& g,
+                       const int* cell_start, const int* tiles, int ntiles, bool sort_own,
+                       int* tile_keys, int* tile_nc, int* tile_ucnt, int* overflow);
+// pf: cell-sorted positions; own (may be null): positions by storage index
+// and map (may be null) the storage index of each sorted position, for a
+// build without a reorder; tile_over[t] gets the tile's particles over cap
+void launch_tile_lists(hipStream_t s, const float* const pf[3], const float* const own[3],
+                       const int* map, const int* tiles, int ntiles, int n, const GridDesc& g,
                        const int* cell_start, const int* tile_keys, const int* tile_nc,
                        const int* tile_uoff, float r2, int cap, int64_t npad, int* count,
-                       unsigned short* slot, int* tile_j, int* tile_uc, int* over_cap);
+                       unsigned short* slot, int* tile_j, int* tile_uc, int* tile_over,
+                       int* overflow);
 
 }  // namespace wf

@@ -55,6 +55,10 @@ struct Deck {
   double fric_heat = 0.0;  // fraction of the sliding-friction power into the particle
   int rebuild_policy = WF_REBUILD_EVERY_N;
   double ns_skin = 0.0, ep_hot = 0.05;  // Extensions.rebuild (plastic-zone policy)
+  int ns_global_every = 32;             // Extensions.rebuild.globalEvery (local re-lists)
+  // Extensions.surrogateTraining: cadence and Adam settings of the online rounds
+  int mlp_train_interval = 200, mlp_train_iters = 250, mlp_interval_max = 0;
+  double mlp_lr = 2e-3, mlp_refit_ratio = 2.0;
   bool thermal = false;
   double T0 = -1.0, k_T = 0.0, cp_T = 0.0, taylor_quinney = 0.0, T_inf = 500.0, h_conv = 100.0;
   long max_steps = -1;
@@ -128,11 +132,22 @@ inline Deck parse_deck(const wfj::Value& j) {
   if (!rb.is_null()) {
     if (read(rb["policy"], s)) {
       if (s == "plasticZone") d.rebuild_policy = WF_REBUILD_PLASTIC_ZONE;
-      else if (s != "everyN") throw std::runtime_error("Extensions.rebuild.policy " + s +
-                                                       " is not one of everyN, plasticZone");
+      else if (s == "plasticZoneLocal") d.rebuild_policy = WF_REBUILD_PLASTIC_ZONE_LOCAL;
+      else if (s != "everyN")
+        throw std::runtime_error("Extensions.rebuild.policy " + s +
+                                 " is not one of everyN, plasticZone, plasticZoneLocal");
     }
     read(rb["skin"], d.ns_skin);
     read(rb["epHot"], d.ep_hot);
+    read(rb["globalEvery"], d.ns_global_every);
+  }
+  const wfj::Value& tr = ex["surrogateTraining"];
+  if (!tr.is_null()) {
+    read(tr["interval"], d.mlp_train_interval);
+    read(tr["iterations"], d.mlp_train_iters);
+    read(tr["learningRate"], d.mlp_lr);
+    read(tr["intervalMax"], d.mlp_interval_max);
+    read(tr["refitRatio"], d.mlp_refit_ratio);
   }
   read(ex["quirks"], d.quirks);
   double ms = -1.0;
@@ -297,6 +312,12 @@ inline Setup build_setup(const Deck& d) {
     s.meshes.push_back(me);
   }
   c.rebuild_policy = d.rebuild_policy;
+  c.ns_global_every = d.ns_global_every;
+  c.mlp_train_interval = d.mlp_train_interval;
+  c.mlp_train_iters = d.mlp_train_iters;
+  c.mlp_lr = d.mlp_lr;
+  c.mlp_interval_max = d.mlp_interval_max;
+  c.mlp_refit_ratio = d.mlp_refit_ratio;
   c.ns_skin = d.ns_skin;
   c.ep_hot = d.ep_hot;
   if (!s.meshes.empty()) {

@@ -172,6 +172,15 @@ JC_AL = {"model": _abi.WF_MAT_JOHNSON_COOK, "A": 324e6, "B": 114e6, "n": 0.42, "
          "eps0": 1.0, "m": 1.34, "Tm": 925.0, "Tt": 300.0}
 
 
+def _adaptive_training(cfg, interval=200, interval_max=3200):
+    """Surrogate rounds every `interval` steps while yielding, the interval
+    doubling up to `interval_max` while the network still fits the new
+    samples (a round's starting loss within 2x of the last final loss)."""
+    cfg.mlp_train_interval = interval
+    cfg.mlp_interval_max = interval_max
+    cfg.mlp_refit_ratio = 2.0
+
+
 def c3_johnson_cook(scale=1.0, r=0.15e-3):
     """Coupled thermo-mechanical Johnson-Cook compression: R 15 mm, H 30 mm,
     dx 0.3 mm, aluminium-like JC, Taylor-Quinney 0.9, top at -2 m/s,
@@ -183,6 +192,7 @@ def c3_johnson_cook(scale=1.0, r=0.15e-3):
                                        "taylor_quinney": 0.9},
                               name="c3_jc_thermomech_cylinder")
     sc.cfg.return_map = _abi.WF_RETURN_NEWTON
+    _adaptive_training(sc.cfg)
     # initial flow stress at eps_p = 0, rate 0 (-> 1e-5), T0: Material.cuh:209-219
     mp = sc.cfg.mat
     th = min(max((T0 - mp.Tt) / (mp.Tm - mp.Tt), 0.0), 1.0)
@@ -310,6 +320,7 @@ def c5_friction_stir(scale=1.0, r=0.125e-3, n_theta=128, n_rs=20, n_zp=24, n_rc=
     for k, v in JC_AL.items():
         setattr(mp, k, v)
     cfg.return_map = _abi.WF_RETURN_NEWTON
+    _adaptive_training(cfg)
     th = min(max((T0 - mp.Tt) / (mp.Tm - mp.Tt), 0.0), 1.0)
     sy = mp.A * (1.0 + mp.C * np.log(1e-5 / mp.eps0)) * (1.0 - th ** mp.m)
     f = _mech_fields(xyz, mass, rho, h, Cs, G, sy)
