@@ -23,12 +23,26 @@ namespace wf {
 
 typedef short bf16x4_t __attribute__((ext_vector_type(4)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
 
-// round-to-nearest-even FP32 -> BF16 bit pattern
-__device__ __forceinline__ short bf16_bits(float x) {
-  unsigned u = __float_as_uint(x);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (short)(u >> 16);
+// round-to-nearest-even FP32 -> BF16, two values per v_cvt_pk_bf16_f32
+__device__ __forceinline__ unsigned bf16_pair(float lo, float hi) {
+  const f32x2_t v = {lo, hi};
+  const bf16x2_t b = __builtin_convertvector(v, bf16x2_t);
+  unsigned u;
+  __builtin_memcpy(&u, &b, 4);
+  return u;
+}
+// four values as an MFMA operand
+__device__ __forceinline__ bf16x4_t bf16_quad(float a, float b, float c, float d) {
+  const unsigned lo = bf16_pair(a, b), hi = bf16_pair(c, d);
+  bf16x4_t q;
+  q[0] = (short)(lo & 0xFFFFu);
+  q[1] = (short)(lo >> 16);
+  q[2] = (short)(hi & 0xFFFFu);
+  q[3] = (short)(hi >> 16);
+  return q;
 }
 
 struct MlpWave {
@@ -41,16 +55,18 @@ struct MlpWave {
 
 __device__ __forceinline__ void mlp_wave_load(const float* __restrict__ w, MlpWave& mw) {
   const int l = lane_id(), half = l >> 5, c = l & 31;
+  float v[4];
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     const int k = 4 * half + j;
-    const float v = k < 4 ? w[kMlpOffW1 + k * 32 + c] : (k == 4 ? w[kMlpOffB1 + c] : 0.f);
-    mw.a1[j] = bf16_bits(v);
+    v[j] = k < 4 ? w[kMlpOffW1 + k * 32 + c] : (k == 4 ? w[kMlpOffB1 + c] : 0.f);
   }
+  mw.a1 = bf16_quad(v[0], v[1], v[2], v[3]);
 #pragma unroll
-  for (int kb = 0; kb < 4; kb++)
-#pragma unroll
-    for (int j = 0; j < 4; j++) mw.a2[kb][j] = bf16_bits(w[kMlpOffW2 + (8 * kb + 4 * half + j) * 32 + c]);
+  for (int kb = 0; kb < 4; kb++) {
+    const float* r = w + kMlpOffW2 + (8 * kb + 4 * half) * 32 + c;
+    mw.a2[kb] = bf16_quad(r[0], r[32], r[64], r[96]);
+  }
 #pragma unroll
   for (int r = 0; r < 16; r++) {
     const int row = 8 * (r >> 2) + 4 * half + (r & 3);
@@ -65,7 +81,7 @@ __device__ __forceinline__ void mlp_wave_load(const float* __restrict__ w, MlpWa
 __device__ __forceinline__ float mlp_wave_eval(const MlpWave& mw, float f0, float f1, float f2,
                                                float f3) {
   const int half = lane_id() >> 5;
-  const bf16x4_t one = {bf16_bits(1.f), 0, 0, 0};
+  const bf16x4_t one = {(short)0x3F80, 0, 0, 0};  // BF16 1.0
   float out0 = 0.f, out1 = 0.f;
 #pragma unroll
   for (int pass = 0; pass < 2; pass++) {
@@ -76,7 +92,7 @@ __device__ __forceinline__ float mlp_wave_eval(const MlpWave& mw, float f0, floa
       g2 = __shfl_xor(f2, 32, 64);
       g3 = __shfl_xor(f3, 32, 64);
     }
-    bf16x4_t b1 = {bf16_bits(g0), bf16_bits(g1), bf16_bits(g2), bf16_bits(g3)};
+    bf16x4_t b1 = bf16_quad(g0, g1, g2, g3);
     if (half) b1 = one;
     f32x16_t zero;
 #pragma unroll
@@ -87,9 +103,8 @@ __device__ __forceinline__ float mlp_wave_eval(const MlpWave& mw, float f0, floa
     for (int r = 0; r < 16; r++) d2[r] = mw.b2r[r];
 #pragma unroll
     for (int kb = 0; kb < 4; kb++) {
-      const bf16x4_t b2 = {bf16_bits(fmaxf(d1[4 * kb], 0.f)), bf16_bits(fmaxf(d1[4 * kb + 1], 0.f)),
-                           bf16_bits(fmaxf(d1[4 * kb + 2], 0.f)),
-                           bf16_bits(fmaxf(d1[4 * kb + 3], 0.f))};
+      const bf16x4_t b2 = bf16_quad(fmaxf(d1[4 * kb], 0.f), fmaxf(d1[4 * kb + 1], 0.f),
+                                    fmaxf(d1[4 * kb + 2], 0.f), fmaxf(d1[4 * kb + 3], 0.f));
       d2 = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(mw.a2[kb], b2, d2, 0, 0, 0);
     }
     float part = 0.f;
@@ -106,7 +121,7 @@ __device__ __forceinline__ float mlp_wave_eval(const MlpWave& mw, float f0, floa
 __device__ __forceinline__ void mlp_features(double s_trial, double A, double ep, double rate,
                                              double theta, float& f0, float& f1, float& f2,
                                              float& f3) {
-  f0 = (float)(s_trial / A);
+  f0 = (float)s_trial * __builtin_amdgcn_rcpf((float)A);
   f1 = (float)(10.0 * ep);
   f2 = log10f(fmaxf((float)rate, 1e-3f)) * (1.f / 3.f);
   f3 = (float)theta;
