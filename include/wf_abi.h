@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define WF_ABI_VERSION 4
+#define WF_ABI_VERSION 5
 
 /* ---- status codes ------------------------------------------------------ */
 enum {
@@ -173,7 +173,10 @@ typedef struct wf_config {
   double mlp_refit_ratio;/* starting/previous loss ratio that still fits (2) */
   int ns_global_every;   /* WF_REBUILD_PLASTIC_ZONE_LOCAL: a full rebuild
                             (reorder) after this many re-lists (0: 32)      */
-  int _pad1;
+  int axisymm;           /* 1: the axisymmetric-slice acceleration clamp
+                            after the acceleration pass (dom_bid_type
+                            AxiSymm_3D, SolverFraser.cu:229-231; set by a
+                            cylindrical-grid slice, WeldFormGPU.cu:366-375) */
 } wf_config;
 
 /* Fill *cfg with the reference's defaults: alpha=1, beta=0
@@ -236,10 +239,16 @@ int wf_set_bcs(struct wf_domain* d, int count, const int* zone_ids,
 
 /* ---- neighbour search: cuNSearch::NeighborhoodSearch::find_neighbors
  *      (src/cuNSearch/src/cuNSearch.cu:111-137).  Radius 2*h_glob, FP32
- *      criterion 0 < |xj-xi|^2 < r^2 (cuNSearchKernels.cu:163-164), cap 70.
+ *      criterion 0 < |xj-xi|^2 < r^2 (cuNSearchKernels.cu:163-164).
  *      Positions come from the snapshot (WF_Q_STALE_NS_POS) or from x.
  *      Under WF_REBUILD_PLASTIC_ZONE: radius 2*h_glob + ns_skin, positions
- *      from x, cap WF_MAX_NEIGHBORS_SKIN. ---- */
+ *      from x.  A list holds up to WF_MAX_NEIGHBORS_SKIN neighbours under
+ *      either policy; particles with more than the policy's cap (70, the
+ *      reference's CUDA_MAX_NEIGHBORS; WF_MAX_NEIGHBORS_SKIN for skin lists)
+ *      are counted in nb_over_cap.  The reference stops a list at its 70th
+ *      hit in an order that its atomic cell insertion leaves undefined
+ *      (cuNSearchKernels.cu:169-173), so no fixed rule reproduces its set;
+ *      here every neighbour up to the storage limit is kept. ---- */
 int wf_nsearch_build(struct wf_domain* d);
 /* Total number of stored neighbour entries and particles over the cap. */
 int wf_nsearch_stats(struct wf_domain* d, int64_t* total, int* over_cap);
@@ -263,6 +272,11 @@ int wf_pressure(struct wf_domain* d);
 int wf_stress_strain(struct wf_domain* d);
 /* CalcAccelKernel, InteractionAlt.cu:5-150 (KernelType 0, XSPH 0)          */
 int wf_calc_accel(struct wf_domain* d);
+/* ApplyAxiSymmBCKernel, Domain_d.cu:665-695: a particle with in-plane
+ * acceleration |a_xy| > 1e-3, x > h/2 and a_x > 1e-2 gets a_x = a_y = 0 (the
+ * projection the reference computes there is overwritten by those zeros;
+ * its printf output is not reproduced).                                    */
+int wf_apply_axisymm_bc(struct wf_domain* d);
 /* ApplyBCVelKernel, Mechanical.cu:572-589                                  */
 int wf_apply_bc_vel(struct wf_domain* d, int zone_id, double vx, double vy,
                     double vz);
@@ -461,6 +475,20 @@ int wf_gen_box(const double start[3], double Lx, double Ly, double Lz, double r,
 /* Domain::AddCylinderLength (Domain.cpp:490-574, calcHalfPartCount :473-488) */
 int wf_gen_cylinder(const double start[3], double R, double Lz, double r,
                     double density, double* xyz, int cap, double* mass_out);
+/* Domain::AddCylUniformLength (Domain.cpp:1071-1193), the cylindrical grid:
+ * layers z = r, 3r, ... while z <= Lz (the domain's start point is not
+ * used); on each, rings of radius ri = r_i, r_i + 2r, ... < R, with
+ * ceil(ang ri / 2r) particles at angles k ang / count (one particle when
+ * ri == 0), x = ri cos, y = ri sin.  ID (when non-null) gets tag, or, on a
+ * slice (ang < 2 pi), 2 for the first `rows` and 3 for the last `rows`
+ * angles of a ring; 4 for a particle within r/10 of the axis in both x and
+ * y.  *mass_out = pi R^2 Lz density / count (the whole cylinder's volume,
+ * also for a slice).  Full grid: ang = 2 pi, rows = 1, r_i = 0; the slice
+ * the reference builds when sliceAngle is not exactly 2 pi: ang = pi/16,
+ * rows = 1, r_i = R/2 (WeldFormGPU.cu:366-375).                            */
+int wf_gen_cylinder_polar(int tag, double R, double Lz, double r, double density,
+                          double ang, int rows, double r_i, double* xyz, int* ID,
+                          int cap, double* mass_out);
 /* Domain::AssignZone (Domain.cpp:989-1005): ID[a] = id for particles inside
  * the closed box [start,end]; returns how many.                             */
 int wf_assign_zone(const double* xyz, int n, const double start[3],

@@ -237,6 +237,51 @@ int or_gen_cylinder(const double V[3], double Rxy, double Lz, double r, double* 
   return count;
 }
 
+// Domain::AddCylUniformLength, Domain.cpp:1071-1193 (Dimension 3): radial
+// rings per layer, z0 = r, the start point unused; rows/ang mark slice edges
+int or_gen_cylinder_polar(int tag, double Rxy, double Lz, double r, double density, double ang,
+                          int rows, double r_i, double* xyz, int* ID, int cap,
+                          double* mass_out) {
+  int count = 0;
+  const double z0 = r;
+  double zp = z0;
+  while (zp <= (z0 + Lz - r)) {
+    for (double ri = r_i; ri < Rxy; ri += 2. * r) {
+      int tgcount;
+      double dalpha;
+      if (ri == 0.) {
+        tgcount = 1;
+        dalpha = 0.;
+      } else {
+        tgcount = (int)ceil((ang * ri) / (2. * r));
+        dalpha = ang / (tgcount);
+      }
+      for (int alphai = 0; alphai < tgcount; alphai++) {
+        int id = tag;
+        if (ang < 2.0 * M_PI) {
+          if (alphai < rows) id = 2;
+          else if (alphai > tgcount - 1 - rows) id = 3;
+        }
+        const double xp = ri * cos(alphai * dalpha);
+        const double yp = ri * sin(alphai * dalpha);
+        if ((fabs(xp) < r / 10) && (fabs(yp) < r / 10)) id = 4;
+        if (count < cap) {
+          if (xyz) {
+            xyz[3 * count] = xp;
+            xyz[3 * count + 1] = yp;
+            xyz[3 * count + 2] = zp;
+          }
+          if (ID) ID[count] = id;
+        }
+        count++;
+      }
+    }
+    zp += 2.0 * r;
+  }
+  if (mass_out && count > 0) *mass_out = M_PI * Rxy * Rxy * Lz * density / count;
+  return count;
+}
+
 // Domain::AssignZone, Domain.cpp:989-1005
 int or_assign_zone(const double* xyz, int n, const double start[3], const double end[3], int id,
                    int* ID) {
@@ -820,6 +865,21 @@ int or_nsearch_build(or_state* s, const wf_config* c, int* over_cap) {
   return WF_OK;
 }
 
+// Domain_d::ApplyAxiSymmBC, Domain_d.cu:665-689, over every particle: the
+// projection of a_xy on the radial direction that it computes is then
+// overwritten by a_x = a_y = 0, so only the condition matters (printf output
+// omitted)
+void or_apply_axisymm_bc(or_state* s) {
+  for (int i = 0; i < s->n; i++) {
+    const double ax = s->a[3 * i], ay = s->a[3 * i + 1];
+    const double abs_a = sqrt(ax * ax + ay * ay);
+    if (abs_a > 1.0e-3 && s->x[3 * i] > s->h[i] / 2.0 && ax > 1.0e-2) {
+      s->a[3 * i] = 0.0;
+      s->a[3 * i + 1] = 0.0;
+    }
+  }
+}
+
 // WF_REBUILD_PLASTIC_ZONE trigger: some particle with eps_p > ep_hot has
 // moved more than ns_skin / 2 since the last build (squared distances).
 int or_plastic_zone_moved(const or_state* s, const wf_config* c) {
@@ -833,8 +893,7 @@ int or_plastic_zone_moved(const or_state* s, const wf_config* c) {
   return 4.0 * d2max > c->ns_skin * c->ns_skin;
 }
 
-// One pass of the MechFraserSolve loop body, SolverFraser.cu:164-399
-// (contact and AxiSymm branches excluded).
+// One pass of the MechFraserSolve loop body, SolverFraser.cu:164-399.
 // ---- rigid-mesh contact ----------------------------------------------------
 namespace {
 // vector_math.h:811-814
@@ -1054,6 +1113,7 @@ int or_step(or_state* s, const wf_config* c, or_run* run, int nbc, const int* bc
     or_pressure(s);
     or_stress_strain(s, c);
     or_calc_accel(s, c);
+    if (c->axisymm) or_apply_axisymm_bc(s);  // SolverFraser.cu:229-231
     if (c->contact && s->contact) {  // SolverFraser.cu:233-257
       or_calc_surface(s, c);
       if (s->contact->mesh && s->contact->mesh->ne > 0) {

@@ -92,6 +92,9 @@ int or_gen_box(const double V[3], double Lx, double Ly, double Lz, double r,
 int or_gen_cylinder(const double V[3], double Rxy, double Lz, double r,
                     double* xyz, int cap, double density, double* mass_out);
 int or_half_part_count(double r, double R, int xinc);
+int or_gen_cylinder_polar(int tag, double Rxy, double Lz, double r, double density,
+                          double ang, int rows, double r_i, double* xyz, int* ID,
+                          int cap, double* mass_out);
 int or_assign_zone(const double* xyz, int n, const double start[3],
                    const double end[3], int id, int* ID);
 
@@ -122,6 +125,7 @@ void or_calc_rate_tensors(or_state* s, const wf_config* c);
 void or_pressure(or_state* s);
 void or_stress_strain(or_state* s, const wf_config* c);
 void or_calc_accel(or_state* s, const wf_config* c);
+void or_apply_axisymm_bc(or_state* s);
 void or_apply_bc_vel(or_state* s, int zone, double vx, double vy, double vz);
 void or_update_pos_fraser(or_state* s, double dt);
 void or_update_vel(or_state* s, double dt);

@@ -66,6 +66,7 @@ def lib():
         sig = {
             "or_gen_box": (I, [P, D, D, D, D, P, I, D, P]),
             "or_gen_cylinder": (I, [P, D, D, D, P, I, D, P]),
+            "or_gen_cylinder_polar": (I, [I, D, D, D, D, D, I, D, P, P, I, P]),
             "or_half_part_count": (I, [D, D, I]),
             "or_assign_zone": (I, [P, I, P, P, I, P]),
             "or_neighbors": (I64, [P, I, D, P, P, P, I64, P]),
@@ -83,6 +84,7 @@ def lib():
             "or_pressure": (None, [S]),
             "or_stress_strain": (None, [S, CF]),
             "or_calc_accel": (None, [S, CF]),
+            "or_apply_axisymm_bc": (None, [S]),
             "or_apply_bc_vel": (None, [S, I, D, D, D]),
             "or_update_pos_fraser": (None, [S, D]),
             "or_update_vel": (None, [S, D]),
@@ -133,6 +135,17 @@ def gen_cylinder(V, R, Lz, r, density=1.0):
     mass = C.c_double(0.0)
     lib().or_gen_cylinder(_ptr(V), R, Lz, r, _ptr(xyz), n, density, C.byref(mass))
     return xyz, mass.value
+
+
+def gen_cylinder_polar(R, Lz, r, density=1.0, ang=2.0 * np.pi, rows=1, r_i=0.0, tag=0):
+    """Positions (n, 3), IDs (n,) and particle mass of the cylindrical grid."""
+    n = lib().or_gen_cylinder_polar(tag, R, Lz, r, density, ang, rows, r_i, None, None, 0, None)
+    xyz = np.zeros((n, 3))
+    ID = np.zeros(n, dtype=np.int32)
+    mass = C.c_double(0.0)
+    lib().or_gen_cylinder_polar(tag, R, Lz, r, density, ang, rows, r_i, _ptr(xyz), _ptr(ID), n,
+                                C.byref(mass))
+    return xyz, ID, mass.value
 
 
 def assign_zone(xyz, ID, start, end, zid):
@@ -307,6 +320,9 @@ class State:
 
     def calc_accel(self, cfg):
         lib().or_calc_accel(self.ref(), C.byref(cfg))
+
+    def apply_axisymm_bc(self):
+        lib().or_apply_axisymm_bc(self.ref())
 
     def apply_bc_vel(self, zone, v):
         lib().or_apply_bc_vel(self.ref(), zone, v[0], v[1], v[2])

@@ -59,17 +59,29 @@ def _oracle_from_deck(po, dk):
     h = 2 * r * c["hFactor"]
     K, G = E / (3 * (1 - 2 * nu)), E / (2 * (1 + nu))
     Cs = math.sqrt(K / rho)
+    axisymm = 0
+    ID = None
     if blk["type"] == "Box":
         xyz, mass = scenarios.gen_box(blk["start"], blk["dim"], r, rho)
-    else:
+    elif blk.get("gridCoordSys", "Cartesian") == "Cartesian":
         xyz, mass = scenarios.gen_cylinder(blk["start"], blk["dim"][0] / 2, blk["dim"][2], r,
                                            rho)
+    else:  # WeldFormGPU.cu:366-375, sliceAngle default 2.0000001 pi (:311)
+        L = blk["dim"]
+        if blk.get("sliceAngle", 2.0000001 * math.pi) == 2.0 * math.pi:
+            xyz, ID, mass = po.gen_cylinder_polar(L[0] / 2, L[2], r, rho)
+        else:
+            xyz, ID, mass = po.gen_cylinder_polar(L[0] / 2, L[2], r, rho, math.pi / 16, 1,
+                                                  L[0] / 4)
+            axisymm = 1
     n = len(xyz)
-    ID = np.zeros(n, np.int32)
+    if ID is None:
+        ID = np.zeros(n, np.int32)
     for z in dk["DomainZones"]:
         scenarios.assign_zone(xyz, ID, z["start"], z["end"], z["id"])
     cfg = _abi.default_config()
     cfg.n_particles = cfg.solid_count = n
+    cfg.axisymm = axisymm
     cfg.h_glob = h
     cfg.dt = 0.4 * h / (Cs + 10.0)
     cfg.mat.E, cfg.mat.nu, cfg.mat.sy0 = E, nu, m["yieldStress0"]
@@ -120,7 +132,8 @@ def _oracle_from_deck(po, dk):
 
 
 @pytest.mark.parametrize("deck", ["compression_mini.json", "contact_mini.json",
-                                  "contact_file_press.json"])
+                                  "contact_file_press.json", "cylindrical_slice_mini.json",
+                                  "cylindrical_full_mini.json"])
 def test_driver_run_matches_oracle(oracle, tmp_path, deck):
     path = os.path.join(DECKS, deck)
     with open(path) as f:
@@ -132,15 +145,23 @@ def test_driver_run_matches_oracle(oracle, tmp_path, deck):
     assert r.returncode == 0, r.stderr
     st, cfg, bcs = _oracle_from_deck(oracle, dk)
     st.nsearch(cfg)
-    # the loop: steps until Time >= simTime with Time accumulated step by step
-    t, steps = 0.0, 0
+    # the loop: steps until Time >= simTime with Time accumulated step by step,
+    # a snapshot file whenever Time reaches the next output time, named by
+    # Time in seconds to six decimals (SolverFraser.cu:144,330), so outputs
+    # less than a microsecond apart share one name
+    t, steps, t_out = 0.0, 0, dk["Configuration"]["outTime"]
+    names, n_out = {"out_0.000000.csv"}, 0
     while t < dk["Configuration"]["simTime"]:
         t += cfg.dt
         steps += 1
+        if t >= t_out:
+            names.add(f"out_{t:.6f}.csv")
+            n_out += 1
+            t_out += dk["Configuration"]["outTime"]
     st.step(cfg, bcs, steps)
     outs = sorted(p for p in os.listdir(tmp_path) if p.startswith("out_"))
-    n_out = int(round(dk["Configuration"]["simTime"] / dk["Configuration"]["outTime"]))
-    assert len(outs) == n_out + 1
+    assert n_out == int(round(dk["Configuration"]["simTime"] / dk["Configuration"]["outTime"]))
+    assert outs == sorted(names)
     got = _read_csv(tmp_path / "test.csv")
     n = st.n
     ne = sum(len(m) for m in ([st.mesh_arrays["el"]] if st.st.contact else []))

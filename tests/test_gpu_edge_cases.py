@@ -175,8 +175,9 @@ def test_uncompacted_staged_sets_over_4096(oracle, wflib, precision, tol):
     tile's adjacent cells hold more than the 4,096 particles the list builder
     compacts, so the lists keep raw positions into the whole set and the pair
     loops take the chunked path over it.  Every particle has far more than
-    70 candidates: each list holds 70 true neighbours.  The pair kernels are
-    compared with the oracle run on the GPU's own lists."""
+    96 candidates: each list holds 96 true neighbours (the storage limit),
+    and every particle counts as over the reference's cap of 70.  The pair
+    kernels are compared with the oracle run on the GPU's own lists."""
     rng = np.random.default_rng(58)
     sc = randomize(small_c1(0.4), seed=58, v_scale=0.5, s_scale=100e6)
     n = 6000
@@ -193,7 +194,7 @@ def test_uncompacted_staged_sets_over_4096(oracle, wflib, precision, tol):
         d.nsearch()
         gc, go, gl = d.neighbors()
         _, over = d.nsearch_stats()
-        assert over == n and np.all(gc == _abi.WF_MAX_NEIGHBORS)
+        assert over == n and np.all(gc == _abi.WF_MAX_NEIGHBORS_SKIN)
         cnt, off, lst, _ = oracle.neighbors(x, radius)
         for i in range(0, n, 97):
             assert set(gl[go[i]:go[i] + gc[i]]) <= set(lst[off[i]:off[i] + cnt[i]])

@@ -7,7 +7,7 @@ product can silently run without the native code.
 import ctypes as C
 import os
 
-WF_ABI_VERSION = 4
+WF_ABI_VERSION = 5
 
 WF_OK = 0
 WF_E_ARG = -1
@@ -81,7 +81,7 @@ class wf_config(C.Structure):
                 ("fric_heat", C.c_double), ("ns_skin", C.c_double), ("ep_hot", C.c_double),
                 ("mlp_train_iters", C.c_int), ("mlp_interval_max", C.c_int),
                 ("mlp_lr", C.c_double), ("mlp_refit_ratio", C.c_double),
-                ("ns_global_every", C.c_int), ("_pad1", C.c_int)]
+                ("ns_global_every", C.c_int), ("axisymm", C.c_int)]
 
 
 class wf_stats(C.Structure):
@@ -154,6 +154,7 @@ def _proto(lib):
         "wf_pressure": (I, [P]),
         "wf_stress_strain": (I, [P]),
         "wf_calc_accel": (I, [P]),
+        "wf_apply_axisymm_bc": (I, [P]),
         "wf_apply_bc_vel": (I, [P, I, D, D, D]),
         "wf_update_pos_fraser": (I, [P, D]),
         "wf_update_vel": (I, [P, D]),
@@ -186,6 +187,7 @@ def _proto(lib):
         "wf_contact_force_sum": (I, [P, C.POINTER(D)]),
         "wf_gen_box": (I, [VP, D, D, D, D, D, VP, I, VP]),
         "wf_gen_cylinder": (I, [VP, D, D, D, D, VP, I, VP]),
+        "wf_gen_cylinder_polar": (I, [I, D, D, D, D, D, I, D, VP, VP, I, VP]),
         "wf_assign_zone": (I, [VP, I, VP, VP, I, VP]),
     }
     for name, (res, args) in sig.items():
@@ -202,14 +204,14 @@ def declared_symbols():
             "wf_snapshot_positions", "wf_set_bcs", "wf_nsearch_build", "wf_nsearch_stats",
             "wf_get_neighbors", "wf_calc_dens_inc", "wf_update_density",
             "wf_calc_rate_tensors", "wf_pressure", "wf_stress_strain", "wf_calc_accel",
-            "wf_apply_bc_vel", "wf_update_pos_fraser", "wf_update_vel", "wf_calc_temp_inc",
+            "wf_apply_axisymm_bc", "wf_apply_bc_vel", "wf_update_pos_fraser", "wf_update_vel", "wf_calc_temp_inc",
             "wf_thermal_calcs", "wf_int_energy", "wf_step", "wf_get_stats", "wf_set_timing",
             "wf_kernel_time", "wf_mlp_get_params", "wf_mlp_set_params", "wf_mlp_reset_adam",
             "wf_mlp_infer", "wf_mlp_grad", "wf_mlp_fit", "wf_mlp_train", "wf_mlp_grad_noalloc",
             "wf_gen_plane_mesh", "wf_read_nastran", "wf_add_mesh", "wf_set_mesh_motion", "wf_mesh_info",
             "wf_get_mesh", "wf_calc_surface", "wf_calc_contact_nb", "wf_calc_contact_forces",
             "wf_mesh_update", "wf_get_contact_neighbors", "wf_contact_force_sum", "wf_gen_box",
-            "wf_gen_cylinder", "wf_assign_zone"]
+            "wf_gen_cylinder", "wf_gen_cylinder_polar", "wf_assign_zone"]
 
 
 def load(path=None):

@@ -702,7 +702,7 @@ int wf_create(const wf_config* cfg, wf_domain** out) {
     float4** f4[] = {&d->f.v4, &d->f.v4_alt};
     for (float4** q : f4)
       if (!rc) rc = d->dev_alloc((void**)q, sizeof(float4) * d->npad);
-    if (!rc) rc = d->dev_alloc((void**)&d->f.arec, 3 * sizeof(float4) * d->npad);
+    if (!rc) rc = d->dev_alloc((void**)&d->f.arec, 2 * sizeof(float4) * d->npad);
     if (!rc) rc = d->dev_alloc((void**)&d->f.tk4, sizeof(float4) * d->npad);
     if (!rc) rc = d->dev_alloc((void**)&d->f.tk4_alt, sizeof(float4) * d->npad);
   } else {
@@ -1233,6 +1233,12 @@ int wf_calc_accel(wf_domain* d) {
   HIPC(hipGetLastError());
   return WF_OK;
 }
+int wf_apply_axisymm_bc(wf_domain* d) {
+  if (!d) return fail(WF_E_ARG, "null domain");
+  launch_axisymm_bc(d->stream, d->f, d->n);
+  HIPC(hipGetLastError());
+  return WF_OK;
+}
 int wf_apply_bc_vel(wf_domain* d, int zone, double vx, double vy, double vz) {
   if (!d) return fail(WF_E_ARG, "null domain");
   launch_apply_bc(d->stream, d->f, d->n, zone, vx, vy, vz);
@@ -1606,6 +1612,7 @@ int wf_step(wf_domain* d, int n_steps) {
       timer_begin(d);
       if (mx) launch_accel_mixed(d->stream, d->f, d->tl, d->solid, p.alpha, p.beta, cs32, uh);
       else launch_accel_tiled(d->stream, d->f, d->tl, d->solid, p.alpha, p.beta, uh);
+      if (d->cfg.axisymm) launch_axisymm_bc(d->stream, d->f, d->n);  // SolverFraser.cu:229-231
       if ((rc = timer_end(d, WF_K_ACCEL))) return rc;
       if (d->cfg.contact) {  // SolverFraser.cu:233-257
         timer_begin(d);

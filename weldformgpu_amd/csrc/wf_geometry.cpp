@@ -125,6 +125,37 @@ int wf_gen_cylinder(const double V[3], double R, double Lz, double r, double den
   return count;
 }
 
+int wf_gen_cylinder_polar(int tag, double R, double Lz, double r, double density, double ang,
+                          int rows, double r_i, double* xyz, int* ID, int cap, double* mass_out) {
+  if (!(r > 0.0) || !(ang > 0.0) || rows < 0) return WF_E_ARG;
+  const bool slice = ang < 2.0 * M_PI;
+  int count = 0;
+  // the radius advances by accumulation and the layer height too, as the
+  // reference's loops do, so ring and layer counts agree at the ends
+  for (double zp = r; zp <= (r + Lz - r); zp += 2.0 * r) {
+    for (double ri = r_i; ri < R; ri += 2. * r) {
+      int ring = 1;
+      double da = 0.0;
+      if (ri != 0.) {
+        ring = (int)std::ceil((ang * ri) / (2. * r));
+        da = ang / ring;
+      }
+      for (int k = 0; k < ring; k++) {
+        int id = tag;
+        if (slice && k < rows) id = 2;
+        else if (slice && k > ring - 1 - rows) id = 3;
+        const double xp = ri * std::cos(k * da), yp = ri * std::sin(k * da);
+        if (std::fabs(xp) < r / 10 && std::fabs(yp) < r / 10) id = 4;  // on the axis
+        put(xyz, cap, count, xp, yp, zp);
+        if (ID && count < cap) ID[count] = id;
+        count++;
+      }
+    }
+  }
+  if (mass_out && count > 0) *mass_out = M_PI * R * R * Lz * density / count;
+  return count;
+}
+
 int wf_assign_zone(const double* xyz, int n, const double start[3], const double end[3], int id,
                    int* ID) {
   if (!xyz || !ID || !start || !end || n < 0) return WF_E_ARG;

@@ -40,9 +40,10 @@ struct Fields {
   // pair gathers, packed per particle: v4 = {v, m/rho} (double-buffered like
   // rho), tk4 = {T, k, m/rho, 0} (double-buffered like T; m/rho refreshed by
   // the density update), and the acceleration loop's
-  // 48-byte record per particle, written by the stress pass: arec[3i] = v4,
-  // arec[3i+1] = sigma/rho^2 {xx, yy, zz, xy}, arec[3i+2] = {sigma/rho^2 yz,
-  // xz, rho, Cs}, so that a pair reads one record instead of three arrays.
+  // 32-byte record per particle, written by the stress pass: arec[2i] =
+  // sigma/rho^2 {xx, yy, zz, xy}, arec[2i+1] = {sigma/rho^2 yz, xz, rho, Cs}
+  // (one aligned half of a 64-byte line; the neighbour's v4 is staged in LDS
+  // with its position).
   float4* p4;
   float* h32;
   float4 *v4, *v4_alt, *arec;
@@ -69,6 +70,8 @@ struct Fields {
 __device__ __forceinline__ size_t rec_at(const Fields&, int i, int c) {
   return 3 * (size_t)i + c;
 }
+// element c (0..1) of particle i's FP32 acceleration record
+__device__ __forceinline__ size_t rec32_at(int i, int c) { return 2 * (size_t)i + c; }
 
 // which packed neighbour records a producer kernel also writes
 enum { kRecNone = 0, kRec32 = 1, kRec64 = 2 };
@@ -279,6 +282,7 @@ int set_last_error(int code, const char* msg);
 // gather the logged samples (ascending particle index) into X (n x 4), Y
 void launch_mlp_collect(hipStream_t s, const Fields& f, int n, float* X, float* Y, int* flags,
                         int* scan, int* scan_part, int* count);
+void launch_axisymm_bc(hipStream_t s, const Fields& f, int n);
 void launch_apply_bc(hipStream_t s, const Fields& f, int n, int zone, double vx, double vy,
                      double vz);
 void launch_update_pos(hipStream_t s, const Fields& f, int n, double dt);

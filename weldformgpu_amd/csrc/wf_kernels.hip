@@ -127,7 +127,6 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Block
     T9 WT = {W.xx, W.yx, W.zx, W.xy, W.yy, W.zy, W.xz, W.yz, W.zz};
     T9 SRT = tmul(S, WT, qp);
     T9 RS = tmul(W, S, qp);
-    if (REC == kRec32) f.arec[rec_at(f, i, 0)] = f.v4[i];  // this step's {v, m/rho}
     if (REC == kRec64)
       f.arec64[rec_at(f, i, 0)] = make_double4(f.v[0][i], f.v[1][i], f.v[2][i], f.m[i]);
     G = f.G[i];
@@ -262,9 +261,9 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Block
     }
     if (REC == kRec32) {  // sigma/rho^2, rho and Cs for the FP32 acceleration loop
       const double rho = f.rho[i], ir2 = 1.0 / (rho * rho);
-      f.arec[rec_at(f, i, 1)] = make_float4((float)(sg[0] * ir2), (float)(sg[1] * ir2),
-                                            (float)(sg[2] * ir2), (float)(sg[3] * ir2));
-      f.arec[rec_at(f, i, 2)] =
+      f.arec[rec32_at(i, 0)] = make_float4((float)(sg[0] * ir2), (float)(sg[1] * ir2),
+                                           (float)(sg[2] * ir2), (float)(sg[3] * ir2));
+      f.arec[rec32_at(i, 1)] =
           make_float4((float)(sg[4] * ir2), (float)(sg[5] * ir2), (float)rho, (float)f.Cs[i]);
     }
     if (REC == kRec64) {  // the same in FP64 for the FP64 acceleration loop
@@ -439,11 +438,10 @@ __global__ __launch_bounds__(kBlock) void k_mirror32(Fields f, int n, Frame32 fr
   f.v4[i] = v4;
   f.v4_alt[i] = v4;
   const double ir2 = 1.0 / (rho * rho);
-  f.arec[rec_at(f, i, 0)] = v4;
-  f.arec[rec_at(f, i, 1)] =
+  f.arec[rec32_at(i, 0)] =
       make_float4((float)(f.sigma32[0][i] * ir2), (float)(f.sigma32[1][i] * ir2),
                   (float)(f.sigma32[2][i] * ir2), (float)(f.sigma32[3][i] * ir2));
-  f.arec[rec_at(f, i, 2)] = make_float4((float)(f.sigma32[4][i] * ir2),
+  f.arec[rec32_at(i, 1)] = make_float4((float)(f.sigma32[4][i] * ir2),
                                           (float)(f.sigma32[5][i] * ir2), (float)rho,
                                           (float)f.Cs[i]);
   const float4 tk = make_float4((float)f.T[i], (float)f.k[i], v4.w, 0.f);
@@ -475,6 +473,19 @@ __global__ __launch_bounds__(kBlock) void k_apply_bc(Fields f, int n, int zone, 
   f.a[0][i] = 0.0; f.a[1][i] = 0.0; f.a[2][i] = 0.0;
   f.v[0][i] = vx; f.v[1][i] = vy; f.v[2][i] = vz;
   f.va[0][i] = vx; f.va[1][i] = vy; f.va[2][i] = vz;
+}
+
+// ApplyAxiSymmBC (Domain_d.cu:665-689): the radial projection computed there
+// is overwritten by zeros, so a particle meeting the condition loses its
+// in-plane acceleration
+__global__ __launch_bounds__(kBlock) void k_axisymm_bc(Fields f, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double ax = f.a[0][i], ay = f.a[1][i];
+  if (sqrt(ax * ax + ay * ay) > 1.0e-3 && f.x[0][i] > f.h[i] / 2.0 && ax > 1.0e-2) {
+    f.a[0][i] = 0.0;
+    f.a[1][i] = 0.0;
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_update_pos(Fields f, int n, double dt) {
@@ -597,6 +608,10 @@ void launch_apply_bc(hipStream_t s, const Fields& f, int n, int zone, double vx,
                      double vz) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_apply_bc, dim3(grid_for(n)), dim3(kBlock), 0, s, f, n, zone, vx, vy, vz);
+}
+void launch_axisymm_bc(hipStream_t s, const Fields& f, int n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_axisymm_bc, dim3(grid_for(n)), dim3(kBlock), 0, s, f, n);
 }
 void launch_update_pos(hipStream_t s, const Fields& f, int n, double dt) {
   if (n <= 0) return;

@@ -512,7 +512,7 @@ int ns_build(hipStream_t s, NsWork** work, const double* const pos_src[3], int n
     w->tj_cap = cap;
   }
   launch_tile_lists(s, pfn, nullptr, nullptr, nullptr, nt, n, g, w->cell_start, w->tile_keys,
-                    w->tile_nc, w->tile_uoff, r2, cap, npad, count, w->slot, w->tile_j,
+                    w->tile_nc, w->tile_uoff, r2, kListCap, cap, npad, count, w->slot, w->tile_j,
                     w->tile_uc, w->tile_over, w->tile_ovf);
   hipLaunchKernelGGL(k_sum_tiles, dim3(1), dim3(256), 0, s, w->tile_over, nt, &ctr->nb_over_cap);
   if (hipGetLastError() != hipSuccess) return WF_E_HIP;
@@ -576,7 +576,8 @@ int ns_relist_local(hipStream_t s, NsWork* w, const double* const x[3], const do
   launch_tile_cells(s, own, n, g, w->cell_start, w->tile_list, ntl, true, w->tile_keys,
                     w->tile_nc, w->tile_ucnt, w->tile_ovf);
   launch_tile_lists(s, pfn, own, w->lgather, w->tile_list, ntl, n, g, w->cell_start,
-                    w->tile_keys, w->tile_nc, w->tile_uoff, r2, cap, npad, count, w->slot,
+                    w->tile_keys, w->tile_nc, w->tile_uoff, r2, kListCap, cap, npad, count,
+                    w->slot,
                     w->tile_j, w->tile_uc, w->tile_over, w->tile_ovf);
   if (hipMemcpyAsync(w->tile_host + 1, w->tile_ovf, sizeof(int), hipMemcpyDeviceToHost, s) !=
       hipSuccess)

@@ -16,6 +16,7 @@
 // policies (DensD/DensG, RatesD/RatesG, AccelD/AccelG, CondD/CondG, SurfD,
 // CondDens) say what a neighbour record holds and what a pair adds.
 #include <climits>
+#include <type_traits>
 
 #include "wf_device.hpp"
 #include "wf_tiles.hpp"
@@ -557,6 +558,32 @@ struct alignas(16) StagedSlot<double> {
   int j, pad;
 };
 
+// A pair functor may stage one more 16-byte value per staged particle in LDS
+// beside its position (typedef Extra, Extra extra(f, j) reads it when the set
+// is staged, put(nb, e) hands it to the neighbour record): a field every pair
+// needs then costs an LDS read instead of a global gather per pair.
+template <class K, class = void>
+struct ExtraOf {
+  static constexpr bool on = false;
+  typedef int type;
+};
+template <class K>
+struct ExtraOf<K, std::void_t<typename K::Extra>> {
+  static constexpr bool on = true;
+  typedef typename K::Extra type;
+};
+
+// waves per SIMD a pair functor asks the register allocator to leave room
+// for (MIN_WAVES; 1 = no request)
+template <class K, class = void>
+struct WavesOf {
+  static constexpr int value = 1;
+};
+template <class K>
+struct WavesOf<K, std::void_t<decltype(K::MIN_WAVES)>> {
+  static constexpr int value = K::MIN_WAVES;
+};
+
 // the value v holds in lane (this lane ^ m)
 template <class T>
 __device__ __forceinline__ T lane_value_xor(const T& v, int m) {
@@ -576,14 +603,17 @@ __device__ __forceinline__ T lane_value_xor(const T& v, int m) {
 // (lane L p + h takes the quads q with q % L == h); their sums are merged
 // (butterfly over the L lanes) before lane L p writes the result.
 template <class K>
-__global__ __launch_bounds__(4 * kTileP) void k_pair_gather(Fields f, TileLists tl,
-                                                            typename K::Prm prm) {
+__global__ __launch_bounds__(4 * kTileP) __attribute__((amdgpu_waves_per_eu(WavesOf<K>::value)))
+void k_pair_gather(Fields f, TileLists tl, typename K::Prm prm) {
   constexpr int ST = K::STAGE;
   constexpr int L = K::LANES;
   static_assert(L == 1 || L == 2 || L == 4, "1, 2 or 4 lanes per particle");
   constexpr int NT = L * kTileP;  // threads per tile
   typedef StagedSlot<typename K::PT> Slot;
   __shared__ Slot st[ST];
+  constexpr bool XS = ExtraOf<K>::on;
+  typedef typename ExtraOf<K>::type XT;
+  __shared__ XT sx[XS ? ST : 1];
   const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
   const int sub = tid & (L - 1);
@@ -615,6 +645,12 @@ __global__ __launch_bounds__(4 * kTileP) void k_pair_gather(Fields f, TileLists 
 #pragma unroll
       for (int u = 0; u < SU; u++)
         if (s0 + u * NT < m) K::pos(f, jj[u], cx, x[u], y[u], z[u]);
+      XT e[XS ? SU : 1];
+      if constexpr (XS) {
+#pragma unroll
+        for (int u = 0; u < SU; u++)
+          if (s0 + u * NT < m) e[u] = K::extra(f, jj[u]);
+      }
 #pragma unroll
       for (int u = 0; u < SU; u++) {
         const int s = s0 + u * NT;
@@ -625,6 +661,7 @@ __global__ __launch_bounds__(4 * kTileP) void k_pair_gather(Fields f, TileLists 
           v.z = z[u];
           v.j = jj[u];
           st[s] = v;
+          if constexpr (XS) sx[s] = e[u];
         }
       }
     }
@@ -674,16 +711,22 @@ __global__ __launch_bounds__(4 * kTileP) void k_pair_gather(Fields f, TileLists 
         // the batch's staged positions are read before any pair is evaluated
         // (entries past the list read slot 0 and are not evaluated)
         typename K::PT px[NB], py[NB], pz[NB];
+        int sl[NB];
 #pragma unroll
         for (int u = 0; u < NB; u++) {
-          const Slot& sv = st[u0 + u < nk ? sl_of(w0, u0 + u) : 0];
+          sl[u] = u0 + u < nk ? sl_of(w0, u0 + u) : 0;
+          const Slot& sv = st[sl[u]];
           px[u] = sv.x;
           py[u] = sv.y;
           pz[u] = sv.z;
         }
+        // a staged extra value is read just before its pair
 #pragma unroll
         for (int u = 0; u < NB; u++)
-          if (u0 + u < nk) K::pair(px[u], py[u], pz[u], nb[u], own, acc, prm);
+          if (u0 + u < nk) {
+            if constexpr (XS) K::put(nb[u], sx[sl[u]]);
+            K::pair(px[u], py[u], pz[u], nb[u], own, acc, prm);
+          }
       }
       w0 = w1;
       w1 = w2;
@@ -712,6 +755,7 @@ __global__ __launch_bounds__(4 * kTileP) void k_pair_gather(Fields f, TileLists 
         const Slot& a0 = st[s0 - c0];
         typename K::Nb a;
         K::load(f, a0.j, a);
+        if constexpr (XS) K::put(a, sx[s0 - c0]);
         if (sr.k < sr.cnt && mine()) {
           const int s1 = sr.slot();
           if (s1 < lim) {
@@ -719,6 +763,7 @@ __global__ __launch_bounds__(4 * kTileP) void k_pair_gather(Fields f, TileLists 
             const Slot& b0 = st[s1 - c0];
             typename K::Nb b;
             K::load(f, b0.j, b);
+            if constexpr (XS) K::put(b, sx[s1 - c0]);
             K::pair(a0.x, a0.y, a0.z, a, own, acc, prm);
             K::pair(b0.x, b0.y, b0.z, b, own, acc, prm);
             continue;
@@ -870,9 +915,16 @@ struct RatesG {
 // D: acceleration with Monaghan viscosity (InteractionAlt.cu:18-150); the
 // neighbour's mass is (m/rho) rho
 template <bool UH>
+#ifndef WF_ACCEL32_NBATCH
+#define WF_ACCEL32_NBATCH 4
+#endif
+#ifndef WF_ACCEL32_WAVES
+#define WF_ACCEL32_WAVES 1
+#endif
 struct AccelG {
   WF_GATHER32
-  static constexpr int NBATCH = 4;
+  static constexpr int NBATCH = WF_ACCEL32_NBATCH;
+  static constexpr int MIN_WAVES = WF_ACCEL32_WAVES;
   struct Prm {
     int limit;
     float alpha32, beta32;
@@ -896,17 +948,20 @@ struct AccelG {
     a.ax.zero(); a.ay.zero(); a.az.zero();
     if (!act) return;
     pos32(f, i, c, o.x, o.y, o.z);
-    const float4 v = f.arec[rec_at(f, i, 0)], sa = f.arec[rec_at(f, i, 1)],
-                 sb = f.arec[rec_at(f, i, 2)];
+    const float4 v = f.v4[i], sa = f.arec[rec32_at(i, 0)], sb = f.arec[rec32_at(i, 1)];
     o.vx = v.x; o.vy = v.y; o.vz = v.z;
     o.s0 = sa.x; o.s1 = sa.y; o.s2 = sa.z; o.s3 = sa.w;
     o.s4 = sb.x; o.s5 = sb.y; o.di = sb.z; o.Ci = sb.w;
     o.hc = hconst_f(f.h32[i]);
   }
+  // the neighbour's {v, m/rho} is staged in LDS with its position, so a pair
+  // gathers only the 32-byte stress record
+  typedef float4 Extra;
+  __device__ static Extra extra(const Fields& f, int j) { return f.v4[j]; }
+  __device__ static void put(Nb& nb, const Extra& e) { nb.v = e; }
   __device__ static void load(const Fields& f, int j, Nb& nb) {
-    nb.v = f.arec[rec_at(f, j, 0)];
-    nb.sa = f.arec[rec_at(f, j, 1)];
-    nb.sb = f.arec[rec_at(f, j, 2)];
+    nb.sa = f.arec[rec32_at(j, 0)];
+    nb.sb = f.arec[rec32_at(j, 1)];
     nb.h = UH ? 0.f : f.h32[j];
   }
   __device__ static void pair(float x, float y, float z, const Nb& nb, const Own& o, Acc& a,
@@ -979,10 +1034,12 @@ struct CondG {
     o.mdj = tk.z;
     o.hc = hconst_f(f.h32[i]);
   }
-  __device__ static void load(const Fields& f, int j, Nb& nb) {
-    nb.tk = f.tk4[j];
-    nb.h = UH ? 0.f : f.h32[j];
-  }
+  // the neighbour's {T, k, m/rho} is staged in LDS with its position (the
+  // pass writes only the next record, tk4_alt)
+  typedef float4 Extra;
+  __device__ static Extra extra(const Fields& f, int j) { return f.tk4[j]; }
+  __device__ static void put(Nb& nb, const Extra& e) { nb.tk = e; }
+  __device__ static void load(const Fields& f, int j, Nb& nb) { nb.h = UH ? 0.f : f.h32[j]; }
   __device__ static void pair(float x, float y, float z, const Nb& nb, const Own& o, Acc& a,
                               const Prm&) {
     const float xx = o.x - x, xy = o.y - y, xz = o.z - z;
@@ -1206,13 +1263,29 @@ void launch_surface_tiled(hipStream_t s, const Fields& f, const TileLists& tl, i
 }  // namespace wf
 
 namespace wf {
+// neighbour record of a fused pass, and the first pass's staged extra value
+// (if it has one) forwarded to the fused functor
+template <class A, class B>
+struct CondDensNb {
+  typename A::Nb a;
+  typename B::Nb b;
+};
+template <class A, class Nb, bool = ExtraOf<A>::on>
+struct FusedExtra {};
+template <class A, class Nb>
+struct FusedExtra<A, Nb, true> {
+  typedef typename A::Extra Extra;
+  __device__ static Extra extra(const Fields& f, int j) { return A::extra(f, j); }
+  __device__ static void put(Nb& nb, const Extra& e) { A::put(nb.a, e); }
+};
+
 // Conduction of one step fused with the density pass of the next: both read
 // the positions after integration and the same neighbour lists, and neither
 // reads what the other writes (the density pass puts the new m/rho into the
 // next conduction record), so one traversal of the lists serves both.  Each
 // sum runs over the list in the same order as in its own kernel.
 template <class A, class B>
-struct CondDens {
+struct CondDens : FusedExtra<A, CondDensNb<A, B>> {
   typedef typename A::PT PT;
   static constexpr int STAGE = A::STAGE;
   static constexpr int LANES = A::LANES;
@@ -1228,10 +1301,7 @@ struct CondDens {
     typename A::Own a;
     typename B::Own b;
   };
-  struct Nb {
-    typename A::Nb a;
-    typename B::Nb b;
-  };
+  typedef CondDensNb<A, B> Nb;
   struct Acc {
     typename A::Acc a;
     typename B::Acc b;

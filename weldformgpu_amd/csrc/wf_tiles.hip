@@ -257,7 +257,7 @@ __global__ __launch_bounds__(kListThreads) void k_tile_lists(
     const int* __restrict__ map, const int* __restrict__ tiles,
     int n, GridDesc g, const int* __restrict__ cell_start, const int* __restrict__ tile_keys,
     const int* __restrict__ tile_nc, const int* __restrict__ tile_uoff, float r2, int cap,
-    int64_t npad, int* __restrict__ count, unsigned short* __restrict__ slot,
+    int rcap, int64_t npad, int* __restrict__ count, unsigned short* __restrict__ slot,
     int* __restrict__ tile_j, int* __restrict__ tile_uc, int* __restrict__ tile_over,
     int* overflow) {
 #pragma clang fp contract(off)
@@ -399,8 +399,8 @@ __global__ __launch_bounds__(kListThreads) void k_tile_lists(
     }
   }
   if (tid == 0) tile_uc[t] = used;
-  // particles of the tile over the cap, one count each
-  const int over = act && sub == 0 && c > cap;
+  // particles of the tile over the reported cap, one count each
+  const int over = act && sub == 0 && c > rcap;
   const unsigned long long o = wave_sum_u64((unsigned long long)over);
   if (lane_id() == 0) wsum[threadIdx.x >> 6] = (int)o;
   __syncthreads();
@@ -426,16 +426,16 @@ void launch_tile_cells(hipStream_t s, const float* const pf[3], int n, const Gri
 void launch_tile_lists(hipStream_t s, const float* const pf[3], const float* const own[3],
                        const int* map, const int* tiles, int ntiles, int n, const GridDesc& g,
                        const int* cell_start, const int* tile_keys, const int* tile_nc,
-                       const int* tile_uoff, float r2, int cap, int64_t npad, int* count,
-                       unsigned short* slot, int* tile_j, int* tile_uc, int* tile_over,
-                       int* overflow) {
+                       const int* tile_uoff, float r2, int cap, int rcap, int64_t npad,
+                       int* count, unsigned short* slot, int* tile_j, int* tile_uc,
+                       int* tile_over, int* overflow) {
   if (ntiles <= 0) return;
   const float* o0 = own ? own[0] : nullptr;
   const float* o1 = own ? own[1] : nullptr;
   const float* o2 = own ? own[2] : nullptr;
   hipLaunchKernelGGL(k_tile_lists, dim3(ntiles), dim3(kListThreads), 0, s, pf[0], pf[1], pf[2],
                      o0, o1, o2, map, tiles, n, g, cell_start, tile_keys, tile_nc, tile_uoff, r2,
-                     cap, npad, count, slot, tile_j, tile_uc, tile_over, overflow);
+                     cap, rcap, npad, count, slot, tile_j, tile_uc, tile_over, overflow);
 }
 
 }  // namespace wf

@@ -121,6 +121,9 @@ class Domain:
     def CalcAccel(self):
         _check(self.lib, self.lib.wf_calc_accel(self.h))
 
+    def ApplyAxiSymmBC(self):
+        _check(self.lib, self.lib.wf_apply_axisymm_bc(self.h))
+
     def ApplyBCVel(self, zone, v):
         _check(self.lib, self.lib.wf_apply_bc_vel(self.h, zone, v[0], v[1], v[2]))
 

@@ -43,6 +43,7 @@ struct Deck {
   std::vector<double> c;
   // DomainBlocks[0] (:313-400)
   std::string dom_type = "Box", grid_cs = "Cartesian";
+  double slice_ang = 2.0000001 * M_PI;  // sliceAngle default (:311)
   double start[3] = {0, 0, 0}, L[3] = {0, 0, 0};
   std::vector<Zone> zones;      // DomainZones (:406-421)
   std::vector<Bc> bcs;          // BoundaryConditions (:457-485)
@@ -85,6 +86,7 @@ inline Deck parse_deck(const wfj::Value& j) {
   const wfj::Value& blk = j["DomainBlocks"][0];
   read(blk["type"], d.dom_type);
   read(blk["gridCoordSys"], d.grid_cs);
+  read(blk["sliceAngle"], d.slice_ang);
   wfj::read3(blk["start"], d.start);
   wfj::read3(blk["dim"], d.L);
   for (const wfj::Value& z : j["DomainZones"].arr) {
@@ -166,8 +168,9 @@ inline Deck parse_deck(const wfj::Value& j) {
   if (!(d.rho > 0.0 && d.E > 0.0)) throw std::runtime_error("Materials[0] needs density0 and youngsModulus");
   if (d.dom_type != "Box" && d.dom_type != "Cylinder")
     throw std::runtime_error("DomainBlocks[0].type " + d.dom_type + " is not supported (Box, Cylinder)");
-  if (d.dom_type == "Cylinder" && d.grid_cs != "Cartesian")
-    throw std::runtime_error("only the Cartesian cylinder lattice is supported");
+  if (d.dom_type == "Cylinder" && d.grid_cs != "Cartesian" && d.grid_cs != "Cylindrical")
+    throw std::runtime_error("DomainBlocks[0].gridCoordSys " + d.grid_cs +
+                             " is not one of Cartesian, Cylindrical");
   for (const Rigid& r : d.rigid)
     if (r.type != "Plane" && r.type != "File")
       throw std::runtime_error("RigidBodies type " + r.type + " is not supported (Plane, File)");
@@ -211,24 +214,43 @@ inline Setup build_setup(const Deck& d) {
   s.G = d.E / (2. * (1. + d.nu));
   s.Cs = std::sqrt(s.K / d.rho);
   s.dt = 0.4 * s.h / (s.Cs + 10.0);  // VMAX = 10, WeldFormGPU.cu:62,906
-  // lattice (Domain::AddBoxLength / AddCylinderLength)
+  // lattice (Domain::AddBoxLength / AddCylinderLength / AddCylUniformLength,
+  // chosen at WeldFormGPU.cu:341-375)
   int n = 0;
+  bool axisymm = false;
   if (d.dom_type == "Box") {
     n = wf_gen_box(d.start, d.L[0], d.L[1], d.L[2], d.r, d.rho, nullptr, 0, nullptr);
     s.x.resize(3 * (size_t)n);
     wf_gen_box(d.start, d.L[0], d.L[1], d.L[2], d.r, d.rho, s.x.data(), n, &s.mass);
-  } else {
+    s.ID.assign(n, 0);
+  } else if (d.grid_cs == "Cartesian") {
     n = wf_gen_cylinder(d.start, d.L[0] / 2., d.L[2], d.r, d.rho, nullptr, 0, nullptr);
     s.x.resize(3 * (size_t)n);
     wf_gen_cylinder(d.start, d.L[0] / 2., d.L[2], d.r, d.rho, s.x.data(), n, &s.mass);
+    s.ID.assign(n, 0);
+  } else {
+    // the whole circle only for a sliceAngle of exactly 2 pi; otherwise a
+    // pi/16 slice of the outer half of the radius, edge rows marked 2 and 3,
+    // and the axisymmetric acceleration clamp in the step loop
+    const bool full = d.slice_ang == 2.0 * M_PI;
+    const double ang = full ? 2.0 * M_PI : M_PI / 16.0, r_i = full ? 0.0 : d.L[0] / 4.0;
+    axisymm = !full;
+    n = wf_gen_cylinder_polar(0, d.L[0] / 2.0, d.L[2], d.r, d.rho, ang, 1, r_i, nullptr, nullptr,
+                              0, nullptr);
+    if (n > 0) {
+      s.x.resize(3 * (size_t)n);
+      s.ID.assign(n, 0);
+      wf_gen_cylinder_polar(0, d.L[0] / 2.0, d.L[2], d.r, d.rho, ang, 1, r_i, s.x.data(),
+                            s.ID.data(), n, &s.mass);
+    }
   }
   if (n <= 0) throw std::runtime_error("the domain block holds no particles");
   s.n = n;
-  s.ID.assign(n, 0);
   for (const Zone& z : d.zones)
     s.zone_count.push_back(wf_assign_zone(s.x.data(), n, z.start, z.end, z.id, s.ID.data()));
   c.n_particles = n;
   c.solid_count = n;
+  c.axisymm = axisymm ? 1 : 0;
   c.dt = s.dt;
   c.alpha = d.alpha;
   c.beta = d.beta;

@@ -111,8 +111,8 @@ void print_setup(const wfd::Deck& dk, const wfd::Setup& s) {
               c.return_map, c.thermal, c.contact, c.contact_law, c.fric_sta, c.fric_dyn,
               dk.sim_time, dk.out_time);
   std::printf(" \"fric_heat\": %.17g, \"rebuild_policy\": %d, \"ns_skin\": %.17g,"
-              " \"ep_hot\": %.17g, \"ns_global_every\": %d,",
-              c.fric_heat, c.rebuild_policy, c.ns_skin, c.ep_hot, c.ns_global_every);
+              " \"ep_hot\": %.17g, \"ns_global_every\": %d, \"axisymm\": %d,",
+              c.fric_heat, c.rebuild_policy, c.ns_skin, c.ep_hot, c.ns_global_every, c.axisymm);
   std::printf(" \"mlp\": [%d, %d, %.17g, %d, %.17g],", c.mlp_train_interval, c.mlp_train_iters,
               c.mlp_lr, c.mlp_interval_max, c.mlp_refit_ratio);
   std::printf(" \"zones\": [");

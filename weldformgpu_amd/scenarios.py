@@ -41,6 +41,8 @@ def _lib():
     lib.wf_gen_box.argtypes = [P, D, D, D, D, D, P, I, P]
     lib.wf_gen_cylinder.restype = I
     lib.wf_gen_cylinder.argtypes = [P, D, D, D, D, P, I, P]
+    lib.wf_gen_cylinder_polar.restype = I
+    lib.wf_gen_cylinder_polar.argtypes = [I, D, D, D, D, D, I, D, P, P, I, P]
     lib.wf_assign_zone.restype = I
     lib.wf_assign_zone.argtypes = [P, I, P, P, I, P]
     lib.wf_gen_plane_mesh.restype = I
@@ -58,6 +60,20 @@ def gen_cylinder(start, R, Lz, r, density):
     m = C.c_double()
     lib.wf_gen_cylinder(V.ctypes.data, R, Lz, r, density, xyz.ctypes.data, n, C.byref(m))
     return xyz, m.value
+
+
+def gen_cylinder_polar(R, Lz, r, density, ang=2.0 * np.pi, rows=1, r_i=0.0, tag=0):
+    """The cylindrical grid (Domain::AddCylUniformLength): positions, IDs, mass."""
+    lib = _lib()
+    n = lib.wf_gen_cylinder_polar(tag, R, Lz, r, density, ang, rows, r_i, None, None, 0, None)
+    if n < 0:
+        raise ValueError(f"wf_gen_cylinder_polar: error {n}")
+    xyz = np.zeros((n, 3))
+    ID = np.zeros(n, np.int32)
+    m = C.c_double()
+    lib.wf_gen_cylinder_polar(tag, R, Lz, r, density, ang, rows, r_i, xyz.ctypes.data,
+                              ID.ctypes.data, n, C.byref(m))
+    return xyz, ID, m.value
 
 
 def gen_box(start, L, r, density):
@@ -121,10 +137,25 @@ def _mech_fields(xyz, mass, rho, h, Cs, G, sy0):
 
 
 def cylinder_compression(R, H, r, *, rho, E, nu, material, v_top, hfac=1.2, thermal=None,
-                         name="cylinder", layers=2):
+                         name="cylinder", layers=2, slice_angle=None):
     """Cylinder on the z axis, bottom `layers` layers held at v=0 (zone 2),
-    top `layers` layers driven at (0,0,v_top) (zone 3)."""
-    xyz, mass = gen_cylinder([0.0, 0.0, 0.0], R, H, r, rho)
+    top `layers` layers driven at (0,0,v_top) (zone 3).
+
+    slice_angle None: the Cartesian lattice (AddCylinderLength).  Otherwise
+    the cylindrical grid a deck with gridCoordSys "Cylindrical" and this
+    sliceAngle gets (WeldFormGPU.cu:366-375): the whole circle for exactly
+    2 pi, else a pi/16 slice of the outer half of the radius with the
+    axisymmetric acceleration clamp; the grid's IDs (2, 3 on the slice edges,
+    4 on the axis) stand where no zone overrides them."""
+    ID0 = None
+    axisymm = 0
+    if slice_angle is None:
+        xyz, mass = gen_cylinder([0.0, 0.0, 0.0], R, H, r, rho)
+    elif slice_angle == 2.0 * np.pi:
+        xyz, ID0, mass = gen_cylinder_polar(R, H, r, rho)
+    else:
+        xyz, ID0, mass = gen_cylinder_polar(R, H, r, rho, np.pi / 16.0, 1, R / 2.0)
+        axisymm = 1
     n = len(xyz)
     dx = 2.0 * r
     h = dx * hfac
@@ -134,6 +165,7 @@ def cylinder_compression(R, H, r, *, rho, E, nu, material, v_top, hfac=1.2, ther
     cfg = _abi.default_config()
     cfg.n_particles = n
     cfg.solid_count = n
+    cfg.axisymm = axisymm
     cfg.h_glob = h
     cfg.dt = 0.4 * h / (Cs + 10.0)
     mp = cfg.mat
@@ -142,6 +174,8 @@ def cylinder_compression(R, H, r, *, rho, E, nu, material, v_top, hfac=1.2, ther
         setattr(mp, k, v)
     sy0 = material.get("sy0", 0.0)
     f = _mech_fields(xyz, mass, rho, h, Cs, G, sy0)
+    if ID0 is not None:
+        f["ID"][:] = ID0
     zmin, zmax = xyz[:, 2].min(), xyz[:, 2].max()
     big = 10 * R
     lay = (layers - 0.5) * dx
