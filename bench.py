@@ -29,8 +29,10 @@ Coulomb 0.2 with all of the sliding-friction power as heat, bottom layer at
 eps_p > 0.05.
 
 Prints ONE JSON line: value = particle-updates/s (particles x steps / s),
-plus the HBM roofline of the dominant kernel (HIP-event durations on the
-library's stream x algorithmic bytes) and the FP64 CPU oracle timed on the
+plus the HBM roofline of the dominant kernel (algorithmic bytes over its
+average launch: 20 launches back to back between two HIP events on the
+library's stream; the per-kernel table brackets each launch of the timed
+steps with its own event pair) and the FP64 CPU oracle timed on the
 host cores on a bounded sample of the same workload (on all the threads
 the process may use, and on one).  error_vs_fp64 compares the mixed run's
 fields with the FP64 run's at the end of the window (the deck's last step);
@@ -284,6 +286,13 @@ def main():
                 if cnt:
                     kern[k] = {"avg_ms": ms / cnt, "launches": cnt}
             dom.set_timing(False)
+            # the pair passes again, launched back to back without events in
+            # between: the kernels' own durations, as a kernel trace reports them
+            for k in ("rates", "accel"):
+                if k in kern:
+                    avg = dom.time_pass(k, 20)
+                    kern[k]["avg_ms_back_to_back"] = avg
+                    kern[k]["GBs_back_to_back"] = kern[k]["alg_bytes"] / (avg * 1e-3) / 1e9
             res["kern"] = kern
             # known-size copy kernel for HBM-counter calibration: the x -> x_h
             # snapshot copies 3 planes of n doubles (24 n bytes read, 24 n written)
@@ -344,10 +353,15 @@ def main():
     stats = rmx["s1"]
     dom_k = max((k for k in kern if k in ALG_BYTES_MIXED), key=lambda k: kern[k]["avg_ms"])
     dk = kern[dom_k]
-    roof = {"kernel": dom_k, "bound": "hbm", "achieved": dk["GBs"], "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": dk["GBs"] / HBM_PEAK_GBS,
+    # the dominant kernel's average launch: back to back where the pass can be
+    # repeated on its own (rates, accel), else its event-bracketed average
+    avg_ms = dk.get("avg_ms_back_to_back", dk["avg_ms"])
+    achieved = dk["alg_bytes"] / (avg_ms * 1e-3) / 1e9
+    roof = {"kernel": dom_k, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
             "traffic": profiled_traffic(args.workload, "mixed/" + dom_k),
-            "alg_bytes_per_particle": ALG_BYTES_MIXED[dom_k], "avg_ms": dk["avg_ms"]}
+            "alg_bytes_per_particle": ALG_BYTES_MIXED[dom_k], "avg_ms": avg_ms,
+            "avg_ms_event_bracketed": dk["avg_ms"]}
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -347,8 +347,16 @@ enum {
                        wf_step call, when no rebuild runs between)  */
   WF_K_COUNT_
 };
-/* Accumulated milliseconds and launch count of kernel id k. */
+/* Accumulated milliseconds and launch count of kernel id k.  Each timed
+ * kernel is bracketed by its own event pair and waited for, so the figures
+ * include the dispatch and the end-of-kernel cache release. */
 int wf_kernel_time(struct wf_domain* d, int k, double* total_ms, int64_t* launches);
+/* A measurement aid with no counterpart in the reference: launch the pair
+ * pass k (WF_K_RATES or WF_K_ACCEL, which only read the state their inputs
+ * come from) reps times back to back between two events on the handle's
+ * stream, and return the elapsed time over reps: the kernel's own duration,
+ * as a kernel trace of the same launches reports it. */
+int wf_time_pass(struct wf_domain* d, int k, int reps, double* ms_per_launch);
 
 /* ---- return-map surrogate ----------------------------------------------------
  * No counterpart in the reference (an extension of the Johnson-Cook return

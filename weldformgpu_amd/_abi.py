@@ -165,6 +165,7 @@ def _proto(lib):
         "wf_get_stats": (I, [P, C.POINTER(wf_stats)]),
         "wf_set_timing": (I, [P, I]),
         "wf_kernel_time": (I, [P, I, C.POINTER(D), C.POINTER(I64)]),
+        "wf_time_pass": (I, [P, I, I, C.POINTER(D)]),
         "wf_mlp_get_params": (I, [P, VP, I]),
         "wf_mlp_set_params": (I, [P, VP, I]),
         "wf_mlp_reset_adam": (I, [P]),
@@ -199,19 +200,20 @@ def _proto(lib):
 
 def declared_symbols():
     """Names of every function include/wf_abi.h declares."""
-    return ["wf_config_default", "wf_field_shape", "wf_create", "wf_destroy",
-            "wf_last_error", "wf_sync", "wf_set_field", "wf_get_field", "wf_fill_field",
-            "wf_snapshot_positions", "wf_set_bcs", "wf_nsearch_build", "wf_nsearch_stats",
-            "wf_get_neighbors", "wf_calc_dens_inc", "wf_update_density",
-            "wf_calc_rate_tensors", "wf_pressure", "wf_stress_strain", "wf_calc_accel",
-            "wf_apply_axisymm_bc", "wf_apply_bc_vel", "wf_update_pos_fraser", "wf_update_vel", "wf_calc_temp_inc",
-            "wf_thermal_calcs", "wf_int_energy", "wf_step", "wf_get_stats", "wf_set_timing",
-            "wf_kernel_time", "wf_mlp_get_params", "wf_mlp_set_params", "wf_mlp_reset_adam",
+    return ["wf_config_default", "wf_field_shape", "wf_create", "wf_destroy", "wf_last_error",
+            "wf_sync", "wf_set_field", "wf_get_field", "wf_fill_field", "wf_snapshot_positions",
+            "wf_set_bcs", "wf_nsearch_build", "wf_nsearch_stats", "wf_get_neighbors",
+            "wf_calc_dens_inc", "wf_update_density", "wf_calc_rate_tensors", "wf_pressure",
+            "wf_stress_strain", "wf_calc_accel", "wf_apply_axisymm_bc", "wf_apply_bc_vel",
+            "wf_update_pos_fraser", "wf_update_vel", "wf_calc_temp_inc", "wf_thermal_calcs",
+            "wf_int_energy", "wf_step", "wf_get_stats", "wf_set_timing", "wf_kernel_time",
+            "wf_time_pass", "wf_mlp_get_params", "wf_mlp_set_params", "wf_mlp_reset_adam",
             "wf_mlp_infer", "wf_mlp_grad", "wf_mlp_fit", "wf_mlp_train", "wf_mlp_grad_noalloc",
-            "wf_gen_plane_mesh", "wf_read_nastran", "wf_add_mesh", "wf_set_mesh_motion", "wf_mesh_info",
-            "wf_get_mesh", "wf_calc_surface", "wf_calc_contact_nb", "wf_calc_contact_forces",
-            "wf_mesh_update", "wf_get_contact_neighbors", "wf_contact_force_sum", "wf_gen_box",
-            "wf_gen_cylinder", "wf_gen_cylinder_polar", "wf_assign_zone"]
+            "wf_gen_plane_mesh", "wf_read_nastran", "wf_add_mesh", "wf_set_mesh_motion",
+            "wf_mesh_info", "wf_get_mesh", "wf_calc_surface", "wf_calc_contact_nb",
+            "wf_calc_contact_forces", "wf_mesh_update", "wf_get_contact_neighbors",
+            "wf_contact_force_sum", "wf_gen_box", "wf_gen_cylinder", "wf_gen_cylinder_polar",
+            "wf_assign_zone"]
 
 
 def load(path=None):

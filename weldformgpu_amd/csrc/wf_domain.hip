@@ -1843,6 +1843,35 @@ int wf_kernel_time(wf_domain* d, int k, double* total_ms, int64_t* launches) {
   return WF_OK;
 }
 
+int wf_time_pass(wf_domain* d, int k, int reps, double* ms_per_launch) {
+  if (!d || !ms_per_launch || reps < 1 || (k != WF_K_RATES && k != WF_K_ACCEL))
+    return fail(WF_E_ARG, "wf_time_pass: WF_K_RATES or WF_K_ACCEL, reps >= 1");
+  NEED_NB(d);
+  d->refresh_mirrors();
+  const bool mx = d->mixed, uh = d->uniform_h;
+  const float cs32 = d->frame.cs32;
+  auto launch = [&]() {
+    if (k == WF_K_RATES) {
+      if (mx) launch_rates_mixed(d->stream, d->f, d->tl, d->solid, d->cfg.quirks, cs32, uh);
+      else launch_rates_tiled(d->stream, d->f, d->tl, d->solid, d->cfg.quirks, uh);
+    } else {
+      if (mx) launch_accel_mixed(d->stream, d->f, d->tl, d->solid, d->cfg.alpha, d->cfg.beta,
+                                 cs32, uh);
+      else launch_accel_tiled(d->stream, d->f, d->tl, d->solid, d->cfg.alpha, d->cfg.beta, uh);
+    }
+  };
+  launch();  // warm
+  HIPC(hipEventRecord(d->ev0, d->stream));
+  for (int r = 0; r < reps; r++) launch();
+  HIPC(hipEventRecord(d->ev1, d->stream));
+  HIPC(hipEventSynchronize(d->ev1));
+  float ms = 0.f;
+  HIPC(hipEventElapsedTime(&ms, d->ev0, d->ev1));
+  HIPC(hipGetLastError());
+  *ms_per_launch = (double)ms / reps;
+  return WF_OK;
+}
+
 }  // extern "C"
 
 int wf::set_last_error(int code, const char* msg) { return fail(code, msg ? msg : ""); }

@@ -879,7 +879,11 @@ struct RatesG {
     o.vx = v.x; o.vy = v.y; o.vz = v.z;
     o.hc = hconst_f(f.h32[i]);
   }
-  __device__ static void load(const Fields& f, int j, Nb& nb) { DensG<UH>::load(f, j, nb); }
+  // the neighbour's {v, m/rho} is staged in LDS with its position
+  typedef float4 Extra;
+  __device__ static Extra extra(const Fields& f, int j) { return f.v4[j]; }
+  __device__ static void put(Nb& nb, const Extra& e) { nb.v = e; }
+  __device__ static void load(const Fields& f, int j, Nb& nb) { nb.h = UH ? 0.f : f.h32[j]; }
   __device__ static void pair(float x, float y, float z, const Nb& nb, const Own& o, Acc& a,
                               const Prm&) {
     const float xx = o.x - x, xy = o.y - y, xz = o.z - z;

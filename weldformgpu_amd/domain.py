@@ -156,6 +156,14 @@ class Domain:
     def set_timing(self, on):
         _check(self.lib, self.lib.wf_set_timing(self.h, 1 if on else 0))
 
+    def time_pass(self, name, reps=20):
+        """Milliseconds per launch of the rates or accel pass, launched reps
+        times back to back (no event or wait between launches)."""
+        k = _abi.KERNELS.index(name)
+        ms = C.c_double()
+        _check(self.lib, self.lib.wf_time_pass(self.h, k, int(reps), C.byref(ms)))
+        return ms.value
+
     def kernel_time(self, name):
         k = _abi.KERNELS.index(name)
         ms, n = C.c_double(), C.c_int64()
