@@ -348,8 +348,8 @@ enum {
   WF_K_COUNT_
 };
 /* Accumulated milliseconds and launch count of kernel id k.  Each timed
- * kernel is bracketed by its own event pair and waited for, so the figures
- * include the dispatch and the end-of-kernel cache release. */
+ * kernel launch is bracketed by its own event pair (events without the
+ * system-scope fence; the host does not wait per launch), read here. */
 int wf_kernel_time(struct wf_domain* d, int k, double* total_ms, int64_t* launches);
 /* A measurement aid with no counterpart in the reference: launch the pair
  * pass k (WF_K_RATES or WF_K_ACCEL, which only read the state their inputs

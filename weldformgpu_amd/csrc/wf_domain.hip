@@ -157,9 +157,16 @@ struct wf_domain {
   bool isfirst_step = true;
   int ns_builds = 0;
   int mlp_trainings = 0;
-  // timing
+  // timing: ev0/ev1 bracket wf_time_pass; each timed kernel launch gets an
+  // event pair of the pool (no system-scope fence, no wait per launch), read
+  // when the pool is full or the times are asked for
   bool timing = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  static constexpr int kTimerPool = 256;
+  hipEvent_t tev[2][kTimerPool] = {};
+  int tev_k[kTimerPool] = {0};
+  int tev_n = 0;
+  bool tev_pass = false;
   // before yield: the step's max eps_p, copied to ctr_host behind the stress
   // pass and read when the step's later passes are queued
   hipEvent_t ev_yield = nullptr;
@@ -656,6 +663,10 @@ int wf_create(const wf_config* cfg, wf_domain** out) {
   if (!cfg || !out) return fail(WF_E_ARG, "null argument");
   if (cfg->abi_version != WF_ABI_VERSION) return fail(WF_E_ARG, "abi_version mismatch");
   if (cfg->n_particles <= 0) return fail(WF_E_ARG, "n_particles must be positive");
+  // the pair loops address per-particle records (up to 96 B) with 32-bit
+  // byte offsets
+  if (cfg->n_particles > kMaxParticles)
+    return fail(WF_E_ARG, "n_particles exceeds 40,000,000");
   if (cfg->solid_count < 0 || cfg->solid_count > cfg->n_particles)
     return fail(WF_E_ARG, "solid_count out of range");
   if (cfg->precision != WF_FP64 && cfg->precision != WF_MIXED)
@@ -805,6 +816,10 @@ int wf_create(const wf_config* cfg, wf_domain** out) {
     rc = WF_E_HIP;
   if (!rc && (hipEventCreate(&d->ev0) != hipSuccess || hipEventCreate(&d->ev1) != hipSuccess))
     rc = WF_E_HIP;
+  for (int b = 0; b < 2 && !rc; b++)
+    for (int e = 0; e < wf_domain::kTimerPool && !rc; e++)
+      if (hipEventCreateWithFlags(&d->tev[b][e], hipEventDisableSystemFence) != hipSuccess)
+        rc = WF_E_HIP;
   if (rc) {
     wf_destroy(d);
     return fail(rc, "device allocation failed");
@@ -832,6 +847,9 @@ int wf_destroy(wf_domain* d) {
   ns_free(d->ns);
   if (d->ev0) hipEventDestroy(d->ev0);
   if (d->ev1) hipEventDestroy(d->ev1);
+  for (int b = 0; b < 2; b++)
+    for (int e = 0; e < wf_domain::kTimerPool; e++)
+      if (d->tev[b][e]) hipEventDestroy(d->tev[b][e]);
   if (d->ev_yield) hipEventDestroy(d->ev_yield);
   if (d->stream) hipStreamDestroy(d->stream);
   delete d;
@@ -995,17 +1013,39 @@ int wf_set_bcs(wf_domain* d, int count, const int* zone_ids, const double* value
   return WF_OK;
 }
 
-static void timer_begin(wf_domain* d) {
-  if (d->timing) hipEventRecord(d->ev0, d->stream);
+// add the recorded event pairs to the kernel totals (waits for the last)
+static int timer_resolve(wf_domain* d) {
+  if (d->tev_n == 0) return WF_OK;
+  HIPC(hipEventSynchronize(d->tev[1][d->tev_n - 1]));
+  for (int e = 0; e < d->tev_n; e++) {
+    float ms = 0.f;
+    HIPC(hipEventElapsedTime(&ms, d->tev[0][e], d->tev[1][e]));
+    d->kt_ms[d->tev_k[e]] += ms;
+    d->kt_n[d->tev_k[e]] += 1;
+  }
+  d->tev_n = 0;
+  return WF_OK;
+}
+// pass: the region is one pair pass, timed by its own dispatch timestamps;
+// otherwise by events recorded before and after the region's launches
+static void timer_begin(wf_domain* d, bool pass = false) {
+  if (!d->timing) return;
+  if (d->tev_n == wf_domain::kTimerPool) timer_resolve(d);
+  d->tev_pass = pass;
+  if (pass) g_next_launch = LaunchEvents{d->tev[0][d->tev_n], d->tev[1][d->tev_n]};
+  else hipEventRecord(d->tev[0][d->tev_n], d->stream);
 }
 static int timer_end(wf_domain* d, int k) {
   if (!d->timing) return WF_OK;
-  HIPC(hipEventRecord(d->ev1, d->stream));
-  HIPC(hipEventSynchronize(d->ev1));
-  float ms = 0.f;
-  HIPC(hipEventElapsedTime(&ms, d->ev0, d->ev1));
-  d->kt_ms[k] += ms;
-  d->kt_n[k] += 1;
+  if (d->tev_pass) {
+    // nothing was launched (an empty pass): nothing to count
+    const bool launched = g_next_launch.start == nullptr;
+    g_next_launch = LaunchEvents{nullptr, nullptr};
+    if (!launched) return WF_OK;
+  } else {
+    HIPC(hipEventRecord(d->tev[1][d->tev_n], d->stream));
+  }
+  d->tev_k[d->tev_n++] = k;
   return WF_OK;
 }
 
@@ -1588,7 +1628,7 @@ int wf_step(wf_domain* d, int n_steps) {
       const float cs32 = d->frame.cs32;
       const bool uh = d->uniform_h;
       if (!dens_done) {
-        timer_begin(d);
+        timer_begin(d, true);
         if (mx) {
           launch_dens_mixed(d->stream, d->f, d->tl, d->n, d->solid, p.dt, true, cs32, uh);
           std::swap(d->f.v4, d->f.v4_alt);
@@ -1600,7 +1640,7 @@ int wf_step(wf_domain* d, int n_steps) {
         if ((rc = timer_end(d, WF_K_DENS))) return rc;
       }
       dens_done = false;
-      timer_begin(d);
+      timer_begin(d, true);
       if (mx) launch_rates_mixed(d->stream, d->f, d->tl, d->solid, p.quirks, cs32, uh);
       else launch_rates_tiled(d->stream, d->f, d->tl, d->solid, p.quirks, uh);
       if ((rc = timer_end(d, WF_K_RATES))) return rc;
@@ -1609,7 +1649,7 @@ int wf_step(wf_domain* d, int n_steps) {
       launch_stress(d->stream, d->f, p, d->bstats, true, true, rec, mx, d->ret_args());
       if ((rc = timer_end(d, WF_K_STRESS))) return rc;
       launch_fold_stats(d->stream, d->bstats, (p.solid + kBlock - 1) / kBlock, d->ctr);
-      timer_begin(d);
+      timer_begin(d, true);
       if (mx) launch_accel_mixed(d->stream, d->f, d->tl, d->solid, p.alpha, p.beta, cs32, uh);
       else launch_accel_tiled(d->stream, d->f, d->tl, d->solid, p.alpha, p.beta, uh);
       if (d->cfg.axisymm) launch_axisymm_bc(d->stream, d->f, d->n);  // SolverFraser.cu:229-231
@@ -1652,7 +1692,7 @@ int wf_step(wf_domain* d, int n_steps) {
       if (d->cfg.mechanics && s + 1 < n_steps && (rc = rebuild_next_step(d, &rebuild_next)))
         return rc;
       const bool fuse = d->cfg.mechanics && s + 1 < n_steps && !rebuild_next;
-      timer_begin(d);
+      timer_begin(d, true);
       if (fuse && d->mixed) {
         launch_cond_dens_mixed(d->stream, d->f, d->tl, d->n, d->solid, p.T_inf, p.h_conv, p.dt,
                                true, d->isfirst_step, d->frame.cs32, d->uniform_h);
@@ -1828,6 +1868,7 @@ int wf_mlp_train(wf_domain* d, int iters, float lr, int64_t* samples, float* los
 int wf_set_timing(wf_domain* d, int on) {
   if (!d) return fail(WF_E_ARG, "null domain");
   HIPC(hipStreamSynchronize(d->stream));
+  d->tev_n = 0;
   d->timing = on != 0;
   for (int k = 0; k < WF_K_COUNT_; k++) {
     d->kt_ms[k] = 0.0;
@@ -1838,6 +1879,7 @@ int wf_set_timing(wf_domain* d, int on) {
 
 int wf_kernel_time(wf_domain* d, int k, double* total_ms, int64_t* launches) {
   if (!d || k < 0 || k >= WF_K_COUNT_) return fail(WF_E_ARG, "bad argument");
+  if (int rc = timer_resolve(d)) return rc;
   if (total_ms) *total_ms = d->kt_ms[k];
   if (launches) *launches = d->kt_n[k];
   return WF_OK;

@@ -64,6 +64,10 @@ struct Fields {
   int* idlock;
 };
 
+// the most particles a domain holds: the pair loops address the per-particle
+// records (at most 96 B each) with 32-bit byte offsets
+constexpr int kMaxParticles = 40000000;
+
 // element c (0..2) of particle i's packed neighbour record (records are
 // contiguous per particle: one record spans one or two cache lines, where
 // three separate arrays would touch three)
@@ -218,6 +222,16 @@ struct Counters {
 };
 
 // ---- LDS-tiled pair loops (wf_pair.hip) ----------------------------------
+// Timing aid: when start is set, the next pair-pass launch (dens, rates,
+// accel, cond, fused cond + dens) records its own start and end in these two
+// events (the kernel's dispatch timestamps, as a kernel trace reports them)
+// and clears them.  One process-wide pair, for a host thread timing one
+// domain at a time.
+struct LaunchEvents {
+  hipEvent_t start, stop;
+};
+extern LaunchEvents g_next_launch;
+
 // FP64 pair loops (uh: every particle has the same h)
 void launch_dens_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int n, int solid,
                        double dt, bool fuse, bool uh);

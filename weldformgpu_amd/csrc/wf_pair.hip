@@ -15,11 +15,28 @@
 // (m/rho, sigma/rho^2, 1/h) formed once instead of once per pair; the
 // policies (DensD/DensG, RatesD/RatesG, AccelD/AccelG, CondD/CondG, SurfD,
 // CondDens) say what a neighbour record holds and what a pair adds.
+#include <hip/hip_ext.h>
+
 #include <climits>
 #include <type_traits>
 
 #include "wf_device.hpp"
 #include "wf_tiles.hpp"
+
+// FP32 partial sums added to the FP64 sums after every pair instead of
+// every batch (1: the per-term widening of the pair terms)
+#ifndef WF_PART_PER_PAIR
+#define WF_PART_PER_PAIR 0
+#endif
+// straight-line batches (drop(), below) in the FP32 rates and FP64
+// acceleration loops: off, measured slower there (759k-particle cylinder:
+// rates 0.072 vs 0.059 ms, FP64 acceleration 0.346 vs 0.321 ms)
+#ifndef WF_DROP_RATES32
+#define WF_DROP_RATES32 0
+#endif
+#ifndef WF_DROP_ACCEL64
+#define WF_DROP_ACCEL64 0
+#endif
 
 namespace wf {
 
@@ -68,6 +85,16 @@ __device__ __forceinline__ int xcd_tile(int b, int nt) {
   return x * q + min(x, r) + k;
 }
 
+// Gather of particle j's record from an array of S-byte records (+ off
+// bytes) through a 32-bit byte offset: the load then takes the array base
+// from scalar registers and the offset costs one shift (a 64-bit index costs
+// three instructions).  Domains hold at most kMaxParticles particles, so
+// j * S stays below 2^32.
+template <class T, unsigned S = sizeof(T)>
+__device__ __forceinline__ T gat(const void* base, int j, unsigned off = 0) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (unsigned)j * S + off);
+}
+
 // FP64 policies need no per-tile context
 struct NoCtx {};
 
@@ -75,14 +102,22 @@ struct NoCtx {};
 // Per-pair cubic-spline gradient GK = (1/r) dW/dr (Functions.cu:76-88) from
 // r^2: one refined reciprocal square root instead of sqrt + three IEEE
 // divisions.  With hj == hi (the solver sets one h for all particles,
-// Domain_d::Set_h) the thread's own 1/h and 1/(pi h^4) are reused, so the
-// common case needs no division at all.  Differences from the reference's
-// expression order are a few ulp.
+// Domain_d::Set_h) the thread's own 1/h and 3/(pi h^4) are reused, so the
+// common case needs no division at all.
+//
+// The two branches of dW/dq (q < 1: -3q + 9/4 q^2; 1 <= q < 2: -3/4 (2-q)^2;
+// else 0) are one branch-free expression: with u = clamp(1 - q, 0, 1) and
+// t = clamp(1 - q/2, 0, 1),  dW/dq = 3 (u^2 - t^2) = 3 (u - t)(u + t)
+// (for q < 1, 3[(1-q)^2 - (1-q/2)^2] = -3q + 9/4 q^2; for 1 <= q < 2, u = 0
+// and -3 t^2 = -3/4 (2-q)^2; for q >= 2 both are 0).  Each clamp folds into
+// the instruction that produces it, so the polynomial costs five operations
+// and no divergent branch.  Differences from the reference's expression
+// order are a few ulp.
 struct HConst {
-  double h, inv_h, C;
+  double h, inv_h, C3;  // h, 1/h, 3/(pi h^4)
 };
 __device__ __forceinline__ HConst hconst(double h) {
-  return HConst{h, 1.0 / h, 1.0 / (h * h * h * h * kPi)};
+  return HConst{h, 1.0 / h, 3.0 / (h * h * h * h * kPi)};
 }
 __device__ __forceinline__ double rsqrt_f64(double x) {
   double y = __builtin_amdgcn_rsq(x);
@@ -91,24 +126,32 @@ __device__ __forceinline__ double rsqrt_f64(double x) {
   y = y * (1.5 - hx * y * y);
   return y;
 }
+// 1/x to within an ulp: the hardware estimate and two Newton steps
+__device__ __forceinline__ double rcp_f64(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, y, 1.0);
+  y = __builtin_fma(y, e, y);
+  e = __builtin_fma(-x, y, 1.0);
+  return __builtin_fma(y, e, y);
+}
+__device__ __forceinline__ double clamp01(double x) { return fmin(fmax(x, 0.0), 1.0); }
+__device__ __forceinline__ float clamp01f(float x) { return fminf(fmaxf(x, 0.f), 1.f); }
 __device__ __forceinline__ double gk_pair(double r2, double hj, const HConst& hc, double& rinv,
                                           double& h_) {
   // coincident particles (never in a list built with d2 > 0, but possible
-  // after motion) contribute nothing instead of 0 * inf
-  rinv = r2 > 0.0 ? rsqrt_f64(r2) : 0.0;
+  // after motion) give q = 0, where dW/dq = 0, and contribute nothing
+  rinv = rsqrt_f64(fmax(r2, 1e-300));
   const double rij = r2 * rinv;
-  double inv_h = hc.inv_h, C = hc.C;
+  double inv_h = hc.inv_h, C3 = hc.C3;
   h_ = hc.h;
   if (hj != hc.h) {
     h_ = (hc.h + hj) / 2.0;
     inv_h = 1.0 / h_;
-    C = 1.0 / (h_ * h_ * h_ * h_ * kPi);
+    C3 = 3.0 / (h_ * h_ * h_ * h_ * kPi);
   }
   const double q = rij * inv_h;
-  const double t = 2.0 - q;
-  const double poly = q < 1.0 ? (-3.0 * q + (9.0 / 4.0) * q * q)
-                              : (q < 2.0 ? (-3.0 / 4.0) * t * t : 0.0);
-  return C * rinv * poly;
+  const double u = clamp01(1.0 - q), t = clamp01(1.0 - 0.5 * q);
+  return C3 * rinv * ((u - t) * (u + t));
 }
 
 // gather-kernel traits of a policy: the staged position type, positions per
@@ -174,6 +217,7 @@ struct DensD {
     double4 v;  // v, m/rho
     double h;
   };
+  __device__ static void drop(Nb& nb) { nb.v.w = 0.0; }
   struct Acc {
     double s;
   };
@@ -189,8 +233,8 @@ struct DensD {
     o.hc = hconst(f.h[i]);
   }
   __device__ static void load(const Fields& f, int j, Nb& nb) {
-    nb.v = f.kv[j];
-    nb.h = UH ? 0.0 : f.h[j];
+    nb.v = gat<double4>(f.kv, j);
+    nb.h = UH ? 0.0 : gat<double>(f.h, j);
   }
   __device__ static void pair(double x, double y, double z, const Nb& nb, const Own& o, Acc& a,
                               const Prm&) {
@@ -236,6 +280,7 @@ struct RatesD {
     HConst hc;
   };
   typedef typename DensD<UH>::Nb Nb;
+  __device__ static void drop(Nb& nb) { DensD<UH>::drop(nb); }
   struct Acc {
     double dxx, dyy, dzz, dxy, dyz, dxz, wxy, wyz, wxz;
   };
@@ -289,7 +334,8 @@ struct RatesD {
 #ifndef WF_ACCEL64_NBATCH
 #define WF_ACCEL64_NBATCH 2
 #endif
-template <bool UH>
+// BV: the beta (quadratic) viscosity term is on
+template <bool UH, bool BV = false>
 struct AccelD {
   WF_GATHER64
   static constexpr int NBATCH = WF_ACCEL64_NBATCH;
@@ -305,6 +351,9 @@ struct AccelD {
     double4 v, sa, sb;  // {v, m}, sigma/rho^2 xx yy zz xy, {yz, xz, rho, Cs}
     double h;
   };
+#if WF_DROP_ACCEL64
+  __device__ static void drop(Nb& nb) { nb.v.w = 0.0; }
+#endif
   struct Acc {
     double ax, ay, az;
   };
@@ -323,10 +372,10 @@ struct AccelD {
     o.s3 = f.sigma[3][i] * idi; o.s4 = f.sigma[4][i] * idi; o.s5 = f.sigma[5][i] * idi;
   }
   __device__ static void load(const Fields& f, int j, Nb& nb) {
-    nb.v = f.arec64[rec_at(f, j, 0)];
-    nb.sa = f.arec64[rec_at(f, j, 1)];
-    nb.sb = f.arec64[rec_at(f, j, 2)];
-    nb.h = UH ? 0.0 : f.h[j];
+    nb.v = gat<double4, 96>(f.arec64, j);  // rec_at(f, j, 0..2)
+    nb.sa = gat<double4, 96>(f.arec64, j, 32);
+    nb.sb = gat<double4, 96>(f.arec64, j, 64);
+    nb.h = UH ? 0.0 : gat<double>(f.h, j);
   }
   __device__ static void pair(double x, double y, double z, const Nb& nb, const Own& o, Acc& a,
                               const Prm& p) {
@@ -336,16 +385,19 @@ struct AccelD {
     const double GK = gk_pair(r2, UH ? o.hc.h : nb.h, o.hc, rinv, h_);
     const double vx = o.vx - nb.v.x, vy = o.vy - nb.v.y, vz = o.vz - nb.v.z;
     const double vdx = vx * xx + vy * xy + vz * xz;
-    double PI = 0.0;
-    if (vdx < 0.0 && (p.alpha != 0.0 || p.beta != 0.0)) {
-      // (alpha C mu + beta mu^2)/rho_bar with mu = h vdx/e, e = r^2 + 0.01 h^2,
-      // as 2 h vdx (alpha C + beta mu)/(e (rho_i + rho_j)): one division
-      // where the reference's form takes two (the beta term keeps its own)
-      const double e = r2 + 0.01 * h_ * h_;
-      const double Cij = 0.5 * (o.Ci + nb.sb.w);
-      double coef = p.alpha * Cij;
-      if (p.beta != 0.0) coef += p.beta * (h_ * vdx / e);
-      PI = 2.0 * h_ * vdx * coef / (e * (o.di + nb.sb.z));
+    // Monaghan viscosity on approaching pairs only: (alpha C mu + beta mu^2)
+    // / rho_bar with mu = h v/e, v = min(v_ij.x_ij, 0), e = r^2 + 0.01 h^2,
+    // as 2 h v (alpha C + beta mu) / (e (rho_i + rho_j)): one refined
+    // reciprocal and no branch (receding pairs get v = 0)
+    const double vn = fmin(vdx, 0.0);
+    const double rs = o.di + nb.sb.z;
+    const double inv = rcp_f64((r2 + 0.01 * h_ * h_) * rs);
+    double PI;
+    if constexpr (BV) {
+      const double coef = (0.5 * p.alpha) * (o.Ci + nb.sb.w) + p.beta * (h_ * vn * rs * inv);
+      PI = 2.0 * h_ * vn * coef * inv;
+    } else {
+      PI = vn * ((h_ * p.alpha) * (o.Ci + nb.sb.w)) * inv;
     }
     const double g = GK * nb.v.w;
     const double gx = g * xx, gy = g * xy, gz = g * xz;
@@ -386,6 +438,7 @@ struct CondD {
     double4 tk;  // T, k, m/rho
     double h;
   };
+  __device__ static void drop(Nb& nb) { nb.tk.z = 0.0; }
   struct Acc {
     double q;
   };
@@ -401,8 +454,8 @@ struct CondD {
     o.hc = hconst(f.h[i]);
   }
   __device__ static void load(const Fields& f, int j, Nb& nb) {
-    nb.tk = f.tk64[j];
-    nb.h = UH ? 0.0 : f.h[j];
+    nb.tk = gat<double4>(f.tk64, j);
+    nb.h = UH ? 0.0 : gat<double>(f.h, j);
   }
   __device__ static void pair(double x, double y, double z, const Nb& nb, const Own& o, Acc& a,
                               const Prm&) {
@@ -477,29 +530,28 @@ struct KSum {
 };
 
 struct HConstF {
-  float h, inv_h, C;
+  float h, inv_h, C3;  // h, 1/h, 3/(pi h^4)
 };
 __device__ __forceinline__ HConstF hconst_f(float h) {
   const float hh = h * h;
-  return HConstF{h, __builtin_amdgcn_rcpf(h), __builtin_amdgcn_rcpf(hh * hh * (float)kPi)};
+  return HConstF{h, __builtin_amdgcn_rcpf(h), 3.f * __builtin_amdgcn_rcpf(hh * hh * (float)kPi)};
 }
-// FP32 cubic-spline gradient (1/r) dW/dr from r^2 (Functions.cu:76-88);
-// UH: every particle has the same h, so h_j = h_i
+// FP32 cubic-spline gradient (1/r) dW/dr from r^2 (Functions.cu:76-88), the
+// branch-free form of gk_pair; UH: every particle has the same h, so h_j = h_i
 template <bool UH>
 __device__ __forceinline__ float gk_pair_f(float r2, float hj, const HConstF& hc, float& h_) {
-  const float rinv = r2 > 0.f ? __builtin_amdgcn_rsqf(r2) : 0.f;
+  const float rinv = __builtin_amdgcn_rsqf(fmaxf(r2, 1e-30f));
   const float rij = r2 * rinv;
-  float inv_h = hc.inv_h, C = hc.C;
+  float inv_h = hc.inv_h, C3 = hc.C3;
   h_ = hc.h;
   if (!UH && hj != hc.h) {
     h_ = 0.5f * (hc.h + hj);
     inv_h = __builtin_amdgcn_rcpf(h_);
-    C = __builtin_amdgcn_rcpf(h_ * h_ * h_ * h_ * (float)kPi);
+    C3 = 3.f * __builtin_amdgcn_rcpf(h_ * h_ * h_ * h_ * (float)kPi);
   }
   const float q = rij * inv_h;
-  const float t = 2.f - q;
-  const float poly = q < 1.f ? (-3.f * q + 2.25f * q * q) : (q < 2.f ? -0.75f * t * t : 0.f);
-  return C * rinv * poly;
+  const float u = clamp01f(1.f - q), t = clamp01f(1.f - 0.5f * q);
+  return C3 * rinv * ((u - t) * (u + t));
 }
 
 struct Ctx32 {
@@ -515,7 +567,7 @@ __device__ __forceinline__ Ctx32 ctx32(const Fields& f, int t, float cs) {
 __device__ __forceinline__ void pos32(const Fields& f, int j, const Ctx32& c, float& x, float& y,
                                       float& z) {
   int cx, cy, cz;
-  const float4 p = f.p4[j];
+  const float4 p = gat<float4>(f.p4, j);
   unpack_cell(__float_as_int(p.w), cx, cy, cz);
   x = p.x + (float)(cx - c.bx) * c.cs;
   y = p.y + (float)(cy - c.by) * c.cs;
@@ -571,6 +623,44 @@ template <class K>
 struct ExtraOf<K, std::void_t<typename K::Extra>> {
   static constexpr bool on = true;
   typedef typename K::Extra type;
+};
+
+// A pair functor may sum a batch's pair terms in a narrower partial sum
+// (typedef Part, pzero(part), flush(acc, part)): pair() then adds into the
+// partial, which is added to the functor's sums once per batch of up to
+// NBATCH pairs.  Without a Part, pair() adds into the sums directly.
+template <class K, class = void>
+struct PartOf {
+  static constexpr bool on = false;
+};
+template <class K>
+struct PartOf<K, std::void_t<typename K::Part>> {
+  static constexpr bool on = true;
+};
+
+// A pair functor whose every term scales with one neighbour weight (m_j or
+// m_j/rho_j) may say how to zero it (drop(nb)): the batch then evaluates
+// all of its pairs in straight-line code, entries past the list with a zero
+// weight (adding exact zeros), so that the compiler keeps every record load
+// ahead of the first pair and each pair waits only for its own record.
+template <class K, class = void>
+struct DropOf {
+  static constexpr bool on = false;
+};
+template <class K>
+struct DropOf<K, std::void_t<decltype(K::drop(std::declval<typename K::Nb&>()))>> {
+  static constexpr bool on = true;
+};
+
+// FLUSH_EACH: the partial sum goes to the FP64 sums after every pair (the
+// per-term widening), not after every batch
+template <class K, class = void>
+struct FlushEachOf {
+  static constexpr bool value = WF_PART_PER_PAIR;
+};
+template <class K>
+struct FlushEachOf<K, std::void_t<decltype(K::FLUSH_EACH)>> {
+  static constexpr bool value = K::FLUSH_EACH;
 };
 
 // waves per SIMD a pair functor asks the register allocator to leave room
@@ -681,7 +771,9 @@ void k_pair_gather(Fields f, TileLists tl, typename K::Prm prm) {
     // constant in scratch
     const uint2* __restrict__ q = tl.slotq + min((int64_t)i, tl.npad - 1);
     const int klast = max(cnt - 1, 0);
-    auto word = [&](int k) { return q[(int64_t)(min(k, klast) >> 2) * tl.npad]; };
+    // 32-bit word index: rows x npad stays below 2^32 (kMaxParticles)
+    const unsigned np = (unsigned)tl.npad;
+    auto word = [&](int k) { return q[(unsigned)(min(k, klast) >> 2) * np]; };
     auto sl_of = [](const uint2 w, int u) {
       const unsigned h = (u & 2) ? w.y : w.x;
       return (int)((u & 1) ? h >> 16 : h & 0xFFFFu);
@@ -704,10 +796,13 @@ void k_pair_gather(Fields f, TileLists tl, typename K::Prm prm) {
       const uint2 w2 = word(k + 2 * KS);
 #pragma unroll
       for (int u0 = 0; u0 < 4; u0 += NB) {
+        // every record of the batch is loaded, also for entries past the
+        // list: their index is the particle index an earlier entry held (or
+        // particle 0), so the load stays in bounds, and straight-line loads
+        // let each pair wait only for its own record instead of all of them
         typename K::Nb nb[NB];
 #pragma unroll
-        for (int u = 0; u < NB; u++)
-          if (u0 + u < nk) K::load(f, jc[u0 + u], nb[u]);
+        for (int u = 0; u < NB; u++) K::load(f, jc[u0 + u], nb[u]);
         // the batch's staged positions are read before any pair is evaluated
         // (entries past the list read slot 0 and are not evaluated)
         typename K::PT px[NB], py[NB], pz[NB];
@@ -721,12 +816,32 @@ void k_pair_gather(Fields f, TileLists tl, typename K::Prm prm) {
           pz[u] = sv.z;
         }
         // a staged extra value is read just before its pair
+        auto batch = [&](auto& sum) {
 #pragma unroll
-        for (int u = 0; u < NB; u++)
-          if (u0 + u < nk) {
-            if constexpr (XS) K::put(nb[u], sx[sl[u]]);
-            K::pair(px[u], py[u], pz[u], nb[u], own, acc, prm);
+          for (int u = 0; u < NB; u++) {
+            if constexpr (DropOf<K>::on) {
+              if constexpr (XS) K::put(nb[u], sx[sl[u]]);
+              if (u0 + u >= nk) K::drop(nb[u]);
+              K::pair(px[u], py[u], pz[u], nb[u], own, sum, prm);
+            } else if (u0 + u == 0 || u0 + u < nk) {
+              // (the quad's first entry is always in the list: k < cnt)
+              if constexpr (XS) K::put(nb[u], sx[sl[u]]);
+              K::pair(px[u], py[u], pz[u], nb[u], own, sum, prm);
+            }
+            if constexpr (PartOf<K>::on && FlushEachOf<K>::value) {
+              K::flush(acc, sum);
+              K::pzero(sum);
+            }
           }
+        };
+        if constexpr (PartOf<K>::on) {
+          typename K::Part part;
+          K::pzero(part);
+          batch(part);
+          K::flush(acc, part);
+        } else {
+          batch(acc);
+        }
       }
       w0 = w1;
       w1 = w2;
@@ -756,6 +871,17 @@ void k_pair_gather(Fields f, TileLists tl, typename K::Prm prm) {
         typename K::Nb a;
         K::load(f, a0.j, a);
         if constexpr (XS) K::put(a, sx[s0 - c0]);
+        // one or two pairs, summed like a batch of the one-pass loop
+        auto eval = [&](auto&& body) {
+          if constexpr (PartOf<K>::on) {
+            typename K::Part part;
+            K::pzero(part);
+            body(part);
+            K::flush(acc, part);
+          } else {
+            body(acc);
+          }
+        };
         if (sr.k < sr.cnt && mine()) {
           const int s1 = sr.slot();
           if (s1 < lim) {
@@ -764,12 +890,14 @@ void k_pair_gather(Fields f, TileLists tl, typename K::Prm prm) {
             typename K::Nb b;
             K::load(f, b0.j, b);
             if constexpr (XS) K::put(b, sx[s1 - c0]);
-            K::pair(a0.x, a0.y, a0.z, a, own, acc, prm);
-            K::pair(b0.x, b0.y, b0.z, b, own, acc, prm);
+            eval([&](auto& sum) {
+              K::pair(a0.x, a0.y, a0.z, a, own, sum, prm);
+              K::pair(b0.x, b0.y, b0.z, b, own, sum, prm);
+            });
             continue;
           }
         }
-        K::pair(a0.x, a0.y, a0.z, a, own, acc, prm);
+        eval([&](auto& sum) { K::pair(a0.x, a0.y, a0.z, a, own, sum, prm); });
       }
     }
   }
@@ -804,7 +932,11 @@ struct DensG {
     float4 v;  // v, m/rho
     float h;
   };
+  __device__ static void drop(Nb& nb) { nb.v.w = 0.f; }
   typedef KSum Acc;
+  typedef float Part;  // a batch's terms in FP32, then added to the FP64 sum
+  __device__ static void pzero(Part& q) { q = 0.f; }
+  __device__ static void flush(Acc& a, const Part& q) { a.add(q); }
   __device__ static void merge(Acc& a, const Acc& b) {
     a.add(b);
   }
@@ -817,17 +949,17 @@ struct DensG {
     o.hc = hconst_f(f.h32[i]);
   }
   __device__ static void load(const Fields& f, int j, Nb& nb) {
-    nb.v = f.v4[j];
-    nb.h = UH ? 0.f : f.h32[j];
+    nb.v = gat<float4>(f.v4, j);
+    nb.h = UH ? 0.f : gat<float>(f.h32, j);
   }
-  __device__ static void pair(float x, float y, float z, const Nb& nb, const Own& o, Acc& a,
+  __device__ static void pair(float x, float y, float z, const Nb& nb, const Own& o, Part& a,
                               const Prm&) {
     const float xx = o.x - x, xy = o.y - y, xz = o.z - z;
     const float r2 = xx * xx + xy * xy + xz * xz;
     float h_;
     const float GK = gk_pair_f<UH>(r2, nb.h, o.hc, h_);
     const float vx = o.vx - nb.v.x, vy = o.vy - nb.v.y, vz = o.vz - nb.v.z;
-    a.add(nb.v.w * (GK * (vx * xx + vy * xy + vz * xz)));
+    a = __builtin_fmaf(nb.v.w, GK * (vx * xx + vy * xy + vz * xz), a);
   }
   __device__ static void finish(const Fields& f, int i, bool act, const Own&, const Acc& a,
                                 const Prm& p) {
@@ -850,11 +982,24 @@ struct DensG {
   }
 };
 
-// B: strain and rotation rates (InteractionAlt.cu:262-383)
+// B: strain and rotation rates (InteractionAlt.cu:262-383).  With
+// a = w v_ij (w = (m_j/rho_j) GK) the rates are sums of the nine products
+// a_k x_l: the diagonal takes a_x x_x, a_y x_y, a_z x_z, each off-diagonal
+// pair (k, l) the sum (strain) and the difference (rotation) of a_k x_l and
+// a_l x_k.  The loop sums the nine products (one FMA each) and forms the
+// sums and differences once per particle.
+// the rates loop widens each of its nine products to FP64 (flush after every
+// pair): summing four pairs in FP32 first is faster (0.061 vs 0.078 ms on a
+// 759k-particle cylinder) but doubles the plastic-strain and temperature
+// error of a 3,000-step Johnson-Cook run against FP64 (5.6e-5 vs 2.8e-5)
+#ifndef WF_RATES_FLUSH_EACH
+#define WF_RATES_FLUSH_EACH 1
+#endif
 template <bool UH>
 struct RatesG {
   WF_GATHER32
   static constexpr int NBATCH = 4;
+  static constexpr bool FLUSH_EACH = WF_RATES_FLUSH_EACH;
   struct Prm {
     int limit;
     double xz_sign;
@@ -862,17 +1007,30 @@ struct RatesG {
   };
   typedef typename DensG<UH>::Own Own;
   typedef typename DensG<UH>::Nb Nb;
+#if WF_DROP_RATES32
+  __device__ static void drop(Nb& nb) { DensG<UH>::drop(nb); }
+#endif
+  // products a_k x_l: xx, yy, zz, then (x y, y x), (y z, z y), (x z, z x)
   struct Acc {
-    KSum dxx, dyy, dzz, dxy, dyz, dxz, wxy, wyz, wxz;
+    KSum xx, yy, zz, xy, yx, yz, zy, xz, zx;
   };
+  struct Part {
+    float xx, yy, zz, xy, yx, yz, zy, xz, zx;
+  };
+  __device__ static void pzero(Part& q) { q = Part{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}; }
+  __device__ static void flush(Acc& a, const Part& q) {
+    a.xx.add(q.xx); a.yy.add(q.yy); a.zz.add(q.zz);
+    a.xy.add(q.xy); a.yx.add(q.yx); a.yz.add(q.yz);
+    a.zy.add(q.zy); a.xz.add(q.xz); a.zx.add(q.zx);
+  }
   __device__ static void merge(Acc& a, const Acc& b) {
-    a.dxx.add(b.dxx); a.dyy.add(b.dyy); a.dzz.add(b.dzz);
-    a.dxy.add(b.dxy); a.dyz.add(b.dyz); a.dxz.add(b.dxz);
-    a.wxy.add(b.wxy); a.wyz.add(b.wyz); a.wxz.add(b.wxz);
+    a.xx.add(b.xx); a.yy.add(b.yy); a.zz.add(b.zz);
+    a.xy.add(b.xy); a.yx.add(b.yx); a.yz.add(b.yz);
+    a.zy.add(b.zy); a.xz.add(b.xz); a.zx.add(b.zx);
   }
   __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx32& c) {
-    a.dxx.zero(); a.dyy.zero(); a.dzz.zero(); a.dxy.zero(); a.dyz.zero(); a.dxz.zero();
-    a.wxy.zero(); a.wyz.zero(); a.wxz.zero();
+    a.xx.zero(); a.yy.zero(); a.zz.zero(); a.xy.zero(); a.yx.zero(); a.yz.zero();
+    a.zy.zero(); a.xz.zero(); a.zx.zero();
     if (!act) return;
     pos32(f, i, c, o.x, o.y, o.z);
     const float4 v = f.v4[i];
@@ -881,10 +1039,12 @@ struct RatesG {
   }
   // the neighbour's {v, m/rho} is staged in LDS with its position
   typedef float4 Extra;
-  __device__ static Extra extra(const Fields& f, int j) { return f.v4[j]; }
+  __device__ static Extra extra(const Fields& f, int j) { return gat<float4>(f.v4, j); }
   __device__ static void put(Nb& nb, const Extra& e) { nb.v = e; }
-  __device__ static void load(const Fields& f, int j, Nb& nb) { nb.h = UH ? 0.f : f.h32[j]; }
-  __device__ static void pair(float x, float y, float z, const Nb& nb, const Own& o, Acc& a,
+  __device__ static void load(const Fields& f, int j, Nb& nb) {
+    nb.h = UH ? 0.f : gat<float>(f.h32, j);
+  }
+  __device__ static void pair(float x, float y, float z, const Nb& nb, const Own& o, Part& a,
                               const Prm&) {
     const float xx = o.x - x, xy = o.y - y, xz = o.z - z;
     const float r2 = xx * xx + xy * xy + xz * xz;
@@ -892,39 +1052,41 @@ struct RatesG {
     const float w = gk_pair_f<UH>(r2, nb.h, o.hc, h_) * nb.v.w;
     const float vx = o.vx - nb.v.x, vy = o.vy - nb.v.y, vz = o.vz - nb.v.z;
     const float ax = w * vx, ay = w * vy, az = w * vz;
-    a.dxx.add(ax * xx);
-    a.dyy.add(ay * xy);
-    a.dzz.add(az * xz);
-    a.dxy.add(ax * xy + ay * xx);
-    a.dyz.add(ay * xz + az * xy);
-    a.dxz.add(ax * xz + az * xx);
-    a.wxy.add(ax * xy - ay * xx);
-    a.wyz.add(ay * xz - az * xy);
-    a.wxz.add(ax * xz - az * xx);
+    a.xx = __builtin_fmaf(ax, xx, a.xx);
+    a.yy = __builtin_fmaf(ay, xy, a.yy);
+    a.zz = __builtin_fmaf(az, xz, a.zz);
+    a.xy = __builtin_fmaf(ax, xy, a.xy);
+    a.yx = __builtin_fmaf(ay, xx, a.yx);
+    a.yz = __builtin_fmaf(ay, xz, a.yz);
+    a.zy = __builtin_fmaf(az, xy, a.zy);
+    a.xz = __builtin_fmaf(ax, xz, a.xz);
+    a.zx = __builtin_fmaf(az, xx, a.zx);
   }
   __device__ static void finish(const Fields& f, int i, bool act, const Own&, const Acc& a,
                                 const Prm& p) {
     if (!act) return;
-    f.strrate32[0][i] = (float)-a.dxx.val(); f.strrate32[1][i] = (float)-a.dyy.val();
-    f.strrate32[2][i] = (float)-a.dzz.val();
-    f.strrate32[3][i] = (float)(-0.5 * a.dxy.val());
-    f.strrate32[4][i] = (float)(-0.5 * a.dyz.val());
-    f.strrate32[5][i] = (float)(-0.5 * a.dxz.val());
-    f.rotrate32[3][i] = (float)(-0.5 * a.wxy.val());  // the diagonal stays 0
-    f.rotrate32[4][i] = (float)(-0.5 * a.wyz.val());
-    f.rotrate32[5][i] = (float)(-0.5 * p.xz_sign * a.wxz.val());
+    const double xy = a.xy.val(), yx = a.yx.val(), yz = a.yz.val(), zy = a.zy.val();
+    const double xz = a.xz.val(), zx = a.zx.val();
+    f.strrate32[0][i] = (float)-a.xx.val(); f.strrate32[1][i] = (float)-a.yy.val();
+    f.strrate32[2][i] = (float)-a.zz.val();
+    f.strrate32[3][i] = (float)(-0.5 * (xy + yx));
+    f.strrate32[4][i] = (float)(-0.5 * (yz + zy));
+    f.strrate32[5][i] = (float)(-0.5 * (xz + zx));
+    f.rotrate32[3][i] = (float)(-0.5 * (xy - yx));  // the diagonal stays 0
+    f.rotrate32[4][i] = (float)(-0.5 * (yz - zy));
+    f.rotrate32[5][i] = (float)(-0.5 * p.xz_sign * (xz - zx));
   }
 };
 
-// D: acceleration with Monaghan viscosity (InteractionAlt.cu:18-150); the
-// neighbour's mass is (m/rho) rho
-template <bool UH>
+// D: acceleration with Monaghan viscosity (InteractionAlt.cu:18-150)
 #ifndef WF_ACCEL32_NBATCH
 #define WF_ACCEL32_NBATCH 4
 #endif
 #ifndef WF_ACCEL32_WAVES
 #define WF_ACCEL32_WAVES 1
 #endif
+// BV: the beta (quadratic) viscosity term is on
+template <bool UH, bool BV = false>
 struct AccelG {
   WF_GATHER32
   static constexpr int NBATCH = WF_ACCEL32_NBATCH;
@@ -939,12 +1101,21 @@ struct AccelG {
     HConstF hc;
   };
   struct Nb {
-    float4 v, sa, sb;
+    float4 v, sa, sb;  // {v, m}, sigma/rho^2 {xx, yy, zz, xy}, {yz, xz, rho, Cs}
     float h;
   };
+  __device__ static void drop(Nb& nb) { nb.v.w = 0.f; }
   struct Acc {
     KSum ax, ay, az;
   };
+  // a batch's pair terms summed in FP32, then added to the FP64 sums
+  struct Part {
+    float ax, ay, az;
+  };
+  __device__ static void pzero(Part& q) { q = Part{0.f, 0.f, 0.f}; }
+  __device__ static void flush(Acc& a, const Part& q) {
+    a.ax.add(q.ax); a.ay.add(q.ay); a.az.add(q.az);
+  }
   __device__ static void merge(Acc& a, const Acc& b) {
     a.ax.add(b.ax); a.ay.add(b.ay); a.az.add(b.az);
   }
@@ -958,17 +1129,22 @@ struct AccelG {
     o.s4 = sb.x; o.s5 = sb.y; o.di = sb.z; o.Ci = sb.w;
     o.hc = hconst_f(f.h32[i]);
   }
-  // the neighbour's {v, m/rho} is staged in LDS with its position, so a pair
-  // gathers only the 32-byte stress record
+  // the neighbour's {v, m} (m = (m/rho) rho, formed once per staged
+  // particle) is staged in LDS with its position, so a pair gathers only the
+  // 32-byte stress record
   typedef float4 Extra;
-  __device__ static Extra extra(const Fields& f, int j) { return f.v4[j]; }
+  __device__ static Extra extra(const Fields& f, int j) {
+    float4 v = gat<float4>(f.v4, j);
+    v.w *= gat<float4, 32>(f.arec, j, 16).z;  // rec32_at(j, 1)
+    return v;
+  }
   __device__ static void put(Nb& nb, const Extra& e) { nb.v = e; }
   __device__ static void load(const Fields& f, int j, Nb& nb) {
-    nb.sa = f.arec[rec32_at(j, 0)];
-    nb.sb = f.arec[rec32_at(j, 1)];
-    nb.h = UH ? 0.f : f.h32[j];
+    nb.sa = gat<float4, 32>(f.arec, j);  // rec32_at(j, 0..1)
+    nb.sb = gat<float4, 32>(f.arec, j, 16);
+    nb.h = UH ? 0.f : gat<float>(f.h32, j);
   }
-  __device__ static void pair(float x, float y, float z, const Nb& nb, const Own& o, Acc& a,
+  __device__ static void pair(float x, float y, float z, const Nb& nb, const Own& o, Part& a,
                               const Prm& p) {
     const float xx = o.x - x, xy = o.y - y, xz = o.z - z;
     const float r2 = xx * xx + xy * xy + xz * xz;
@@ -978,22 +1154,28 @@ struct AccelG {
     const float vdx = vx * xx + vy * xy + vz * xz;
     // Monaghan viscosity on approaching pairs only: (alpha C mu + beta mu^2)
     // / rho_bar with mu = h v/e, v = min(v_ij.x_ij, 0), e = r^2 + 0.01 h^2,
-    // as 2 h v (alpha C + beta mu) / (e (rho_i + rho_j)): one reciprocal and
-    // no branch (receding pairs get v = 0)
+    // as 2 h v (alpha C + beta mu) / (e (rho_i + rho_j)) = v h alpha (C_i +
+    // C_j) / (e (rho_i + rho_j)) without beta: one reciprocal and no branch
+    // (receding pairs get v = 0)
     const float vn = fminf(vdx, 0.f);
     const float rs = o.di + nb.sb.z;
     const float inv = __builtin_amdgcn_rcpf((r2 + 0.01f * h_ * h_) * rs);
-    float coef = p.alpha32 * (0.5f * (o.Ci + nb.sb.w));
-    if (p.beta32 != 0.f) coef += p.beta32 * (h_ * vn * rs * inv);
-    const float PI = 2.f * h_ * vn * coef * inv;
-    const float g = GK * (nb.v.w * nb.sb.z);
+    float PI;
+    if constexpr (BV) {
+      const float coef =
+          (0.5f * p.alpha32) * (o.Ci + nb.sb.w) + p.beta32 * (h_ * vn * rs * inv);
+      PI = 2.f * h_ * vn * coef * inv;
+    } else {
+      PI = vn * ((h_ * p.alpha32) * (o.Ci + nb.sb.w)) * inv;
+    }
+    const float g = GK * nb.v.w;
     const float gx = g * xx, gy = g * xy, gz = g * xz;
     const float mxx = o.s0 + nb.sa.x + PI, myy = o.s1 + nb.sa.y + PI;
     const float mzz = o.s2 + nb.sa.z + PI;
     const float mxy = o.s3 + nb.sa.w, myz = o.s4 + nb.sb.x, mxz = o.s5 + nb.sb.y;
-    a.ax.add(mxx * gx + mxy * gy + mxz * gz);
-    a.ay.add(mxy * gx + myy * gy + myz * gz);
-    a.az.add(mxz * gx + myz * gy + mzz * gz);
+    a.ax = __builtin_fmaf(mxz, gz, __builtin_fmaf(mxy, gy, __builtin_fmaf(mxx, gx, a.ax)));
+    a.ay = __builtin_fmaf(myz, gz, __builtin_fmaf(myy, gy, __builtin_fmaf(mxy, gx, a.ay)));
+    a.az = __builtin_fmaf(mzz, gz, __builtin_fmaf(myz, gy, __builtin_fmaf(mxz, gx, a.az)));
   }
   __device__ static void finish(const Fields& f, int i, bool act, const Own&, const Acc& a,
                                 const Prm&) {
@@ -1024,7 +1206,11 @@ struct CondG {
     float4 tk;  // T, k, m/rho
     float h;
   };
+  __device__ static void drop(Nb& nb) { nb.tk.z = 0.f; }
   typedef KSum Acc;
+  typedef float Part;  // a batch's terms in FP32, then added to the FP64 sum
+  __device__ static void pzero(Part& q) { q = 0.f; }
+  __device__ static void flush(Acc& a, const Part& q) { a.add(q); }
   __device__ static void merge(Acc& a, const Acc& b) {
     a.add(b);
   }
@@ -1041,10 +1227,12 @@ struct CondG {
   // the neighbour's {T, k, m/rho} is staged in LDS with its position (the
   // pass writes only the next record, tk4_alt)
   typedef float4 Extra;
-  __device__ static Extra extra(const Fields& f, int j) { return f.tk4[j]; }
+  __device__ static Extra extra(const Fields& f, int j) { return gat<float4>(f.tk4, j); }
   __device__ static void put(Nb& nb, const Extra& e) { nb.tk = e; }
-  __device__ static void load(const Fields& f, int j, Nb& nb) { nb.h = UH ? 0.f : f.h32[j]; }
-  __device__ static void pair(float x, float y, float z, const Nb& nb, const Own& o, Acc& a,
+  __device__ static void load(const Fields& f, int j, Nb& nb) {
+    nb.h = UH ? 0.f : gat<float>(f.h32, j);
+  }
+  __device__ static void pair(float x, float y, float z, const Nb& nb, const Own& o, Part& a,
                               const Prm&) {
     const float xx = o.x - x, xy = o.y - y, xz = o.z - z;
     const float r2 = xx * xx + xy * xy + xz * xz;
@@ -1052,7 +1240,7 @@ struct CondG {
     const float GK = gk_pair_f<UH>(r2, nb.h, o.hc, h_);
     const float kj = nb.tk.y;
     const float kh = (kj == o.k) ? 2.f * o.k : 4.f * o.k * kj * __builtin_amdgcn_rcpf(o.k + kj);
-    a.add(nb.tk.z * kh * (o.T - nb.tk.x) * GK);
+    a = __builtin_fmaf(nb.tk.z * kh, (o.T - nb.tk.x) * GK, a);
   }
   __device__ static void finish(const Fields& f, int i, bool act, const Own& o, const Acc& a,
                                 const Prm& p) {
@@ -1288,8 +1476,39 @@ struct FusedExtra<A, Nb, true> {
 // reads what the other writes (the density pass puts the new m/rho into the
 // next conduction record), so one traversal of the lists serves both.  Each
 // sum runs over the list in the same order as in its own kernel.
+// the fused pass sums a batch in partial sums when both passes do
+template <class A, class B, bool = PartOf<A>::on && PartOf<B>::on>
+struct FusedPart {};
 template <class A, class B>
-struct CondDens : FusedExtra<A, CondDensNb<A, B>> {
+struct FusedPart<A, B, true> {
+  struct Part {
+    typename A::Part a;
+    typename B::Part b;
+  };
+  __device__ static void pzero(Part& q) {
+    A::pzero(q.a);
+    B::pzero(q.b);
+  }
+  template <class Acc>
+  __device__ static void flush(Acc& s, const Part& q) {
+    A::flush(s.a, q.a);
+    B::flush(s.b, q.b);
+  }
+};
+
+// and evaluates a batch in straight-line code when both passes can
+template <class A, class B, bool = DropOf<A>::on && DropOf<B>::on>
+struct FusedDrop {};
+template <class A, class B>
+struct FusedDrop<A, B, true> {
+  __device__ static void drop(CondDensNb<A, B>& nb) {
+    A::drop(nb.a);
+    B::drop(nb.b);
+  }
+};
+
+template <class A, class B>
+struct CondDens : FusedExtra<A, CondDensNb<A, B>>, FusedPart<A, B>, FusedDrop<A, B> {
   typedef typename A::PT PT;
   static constexpr int STAGE = A::STAGE;
   static constexpr int LANES = A::LANES;
@@ -1325,7 +1544,9 @@ struct CondDens : FusedExtra<A, CondDensNb<A, B>> {
     A::load(f, j, nb.a);
     B::load(f, j, nb.b);
   }
-  __device__ static void pair(PT x, PT y, PT z, const Nb& nb, const Own& o, Acc& a,
+  // S: the sums (Acc) or the batch's partial sums (Part)
+  template <class S>
+  __device__ static void pair(PT x, PT y, PT z, const Nb& nb, const Own& o, S& a,
                               const Prm& p) {
     A::pair(x, y, z, nb.a, o.a, a.a, p.a);
     B::pair(x, y, z, nb.b, o.b, a.b, p.b);
@@ -1345,17 +1566,34 @@ using CondDensD = CondDens<CondD<UH>, DensD<UH>>;
 template <bool UH>
 using CondDensG = CondDens<CondG<UH>, DensG<UH>>;
 
+// the acceleration loops with the beta viscosity term
+template <bool UH>
+using AccelDB = AccelD<UH, true>;
+template <bool UH>
+using AccelGB = AccelG<UH, true>;
+
+LaunchEvents g_next_launch = {nullptr, nullptr};
+
+// one pair pass: grid tiles of kTileP particles, K::LANES threads each (with
+// its dispatch timestamps in g_next_launch's events when those are set)
+template <class K>
+static void launch_pair(hipStream_t s, int grid, const Fields& f, const TileLists& tl,
+                        const typename K::Prm& p) {
+  if (g_next_launch.start) {
+    hipExtLaunchKernelGGL(k_pair_gather<K>, dim3(grid), dim3(kTileP * K::LANES), 0, s,
+                          g_next_launch.start, g_next_launch.stop, 0, f, tl, p);
+    g_next_launch = LaunchEvents{nullptr, nullptr};
+  } else {
+    hipLaunchKernelGGL(k_pair_gather<K>, dim3(grid), dim3(kTileP * K::LANES), 0, s, f, tl, p);
+  }
+}
+
 // launch policy POL with every particle's h equal (UH) or not
 template <template <bool> class POL, class... A>
 static void launch_gather(hipStream_t s, const Fields& f, const TileLists& tl, int grid, bool uh,
                           A... args) {
-  if (uh) {
-    const typename POL<true>::Prm p{args...};
-    hipLaunchKernelGGL(k_pair_gather<POL<true>>, dim3(grid), dim3(kTileP * POL<true>::LANES), 0, s, f, tl, p);
-  } else {
-    const typename POL<false>::Prm p{args...};
-    hipLaunchKernelGGL(k_pair_gather<POL<false>>, dim3(grid), dim3(kTileP * POL<false>::LANES), 0, s, f, tl, p);
-  }
+  if (uh) launch_pair<POL<true>>(s, grid, f, tl, typename POL<true>::Prm{args...});
+  else launch_pair<POL<false>>(s, grid, f, tl, typename POL<false>::Prm{args...});
 }
 
 void launch_dens_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int n, int solid,
@@ -1372,7 +1610,11 @@ void launch_rates_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int
 void launch_accel_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
                         double alpha, double beta, bool uh) {
   if (solid <= 0) return;
-  launch_gather<AccelD>(s, f, tl, ntiles(solid), uh, solid, alpha, beta);
+  if (beta != 0.0) {
+    launch_gather<AccelDB>(s, f, tl, ntiles(solid), uh, solid, alpha, beta);
+  } else {
+    launch_gather<AccelD>(s, f, tl, ntiles(solid), uh, solid, alpha, beta);
+  }
 }
 void launch_cond_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int n, double T_inf,
                        double h_conv, double dt, bool leap, bool first, bool uh) {
@@ -1394,7 +1636,10 @@ void launch_rates_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int
 void launch_accel_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
                         double alpha, double beta, float cs32, bool uh) {
   if (solid <= 0) return;
-  launch_gather<AccelG>(s, f, tl, ntiles(solid), uh, solid, (float)alpha, (float)beta, cs32);
+  if (beta != 0.0)
+    launch_gather<AccelGB>(s, f, tl, ntiles(solid), uh, solid, (float)alpha, (float)beta, cs32);
+  else
+    launch_gather<AccelG>(s, f, tl, ntiles(solid), uh, solid, (float)alpha, (float)beta, cs32);
 }
 void launch_cond_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int n, double T_inf,
                        double h_conv, double dt, bool leap, bool first, float cs32, bool uh) {
@@ -1410,7 +1655,7 @@ void launch_cond_dens_tiled(hipStream_t s, const Fields& f, const TileLists& tl,
     typedef decltype(tag) K;
     const typename K::Prm p{n, {n, T_inf, h_conv, dt, leap ? 1 : 0, first ? 1 : 0},
                             {solid, n, dt, 1, 1}};
-    hipLaunchKernelGGL(k_pair_gather<K>, dim3(ntiles(n)), dim3(kTileP * K::LANES), 0, s, f, tl, p);
+    launch_pair<K>(s, ntiles(n), f, tl, p);
   };
   if (uh) go(CondDensD<true>{});
   else go(CondDensD<false>{});
@@ -1423,7 +1668,7 @@ void launch_cond_dens_mixed(hipStream_t s, const Fields& f, const TileLists& tl,
     typedef decltype(tag) K;
     const typename K::Prm p{n, {n, T_inf, h_conv, dt, leap ? 1 : 0, first ? 1 : 0, cs32},
                             {solid, n, dt, 1, cs32, 1}};
-    hipLaunchKernelGGL(k_pair_gather<K>, dim3(ntiles(n)), dim3(kTileP * K::LANES), 0, s, f, tl, p);
+    launch_pair<K>(s, ntiles(n), f, tl, p);
   };
   if (uh) go(CondDensG<true>{});
   else go(CondDensG<false>{});
