@@ -102,10 +102,21 @@ struct wf_domain {
                            surface_r2cut(), uniform_m, m_value);
     }
   }
-  RetArgs ret_args() const {
-    return RetArgs{mlp_w, (cfg.surrogate && mlp_trained) ? 1 : 0, (long long)step};
+  // the stress pass's return-map arguments; the packed weights are brought
+  // up to date (queued on the stream ahead of the pass) when they are used
+  RetArgs ret_args() {
+    const bool use = cfg.surrogate && mlp_trained;
+    if (use) pack_mlp();
+    return RetArgs{mlp_pack, use ? 1 : 0, (long long)step};
+  }
+  void pack_mlp() {
+    if (!mlp_pack_stale) return;
+    launch_mlp_pack(stream, mlp_w, mlp_pack);
+    mlp_pack_stale = false;
   }
   float *mlp_w = nullptr, *mlp_m = nullptr, *mlp_v = nullptr, *mlp_grad = nullptr;
+  void* mlp_pack = nullptr;     // per-lane MFMA operands of mlp_w
+  bool mlp_pack_stale = true;   // mlp_w changed since the last pack
   float *mlp_part = nullptr, *mlp_loss_part = nullptr, *mlp_loss = nullptr;
   float *mlp_X = nullptr, *mlp_Y = nullptr;
   int *mlp_flags = nullptr, *mlp_scan = nullptr, *mlp_scan_part = nullptr, *mlp_count = nullptr;
@@ -501,6 +512,7 @@ int wf_domain::mlp_fit(const float* X, const float* Y, int nsamp, int stride, in
   for (int it = 0; it < iters; it++) {
     launch_mlp_train_iter(stream, mlp_w, mlp_m, mlp_v, X, Y, nsamp, stride, mlp_part,
                           mlp_loss_part, nblocks, ++adam_t, lr, mlp_loss);
+    mlp_pack_stale = true;
     // the first iteration's loss is that of the weights the round started from
     if (it == 0)
       HIPC(hipMemcpyAsync(mlp_loss + 1, mlp_loss, sizeof(float), hipMemcpyDeviceToDevice,
@@ -732,6 +744,7 @@ int wf_create(const wf_config* cfg, wf_domain** out) {
   if (!rc) {
     const size_t P = sizeof(float) * kMlpParams;
     rc = d->dev_alloc((void**)&d->mlp_w, P);
+    if (!rc) rc = d->dev_alloc(&d->mlp_pack, (size_t)16 * 64 * kMlpPackWords);
     if (!rc) rc = d->dev_alloc((void**)&d->mlp_m, P);
     if (!rc) rc = d->dev_alloc((void**)&d->mlp_v, P);
     if (!rc) rc = d->dev_alloc((void**)&d->mlp_grad, P);
@@ -1798,6 +1811,7 @@ int wf_mlp_set_params(wf_domain* d, const float* in, int n) {
   HIPC(hipMemcpyAsync(d->mlp_w, in, sizeof(float) * n, hipMemcpyHostToDevice, d->stream));
   HIPC(hipStreamSynchronize(d->stream));
   d->mlp_trained = true;
+  d->mlp_pack_stale = true;
   return WF_OK;
 }
 
@@ -1821,7 +1835,8 @@ int wf_mlp_infer(wf_domain* d, const float* X, int n, float* y) {
   if (!d || !X || !y) return fail(WF_E_ARG, "null argument");
   int rc = upload_samples(d, X, nullptr, n);
   if (rc) return rc;
-  launch_mlp_infer(d->stream, d->mlp_w, d->mlp_X, n, d->mlp_Y);
+  d->pack_mlp();
+  launch_mlp_infer(d->stream, d->mlp_pack, d->mlp_X, n, d->mlp_Y);
   HIPC(hipGetLastError());
   HIPC(hipMemcpyAsync(y, d->mlp_Y, sizeof(float) * n, hipMemcpyDeviceToHost, d->stream));
   HIPC(hipStreamSynchronize(d->stream));

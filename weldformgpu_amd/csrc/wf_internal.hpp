@@ -151,7 +151,7 @@ void launch_contact_norm(hipStream_t s, const Fields& f, int n, int id, double* 
 // surrogate's weights, whether to use them, and the step number stamped on
 // the logged training samples.
 struct RetArgs {
-  const float* mlp_w;
+  const void* mlp_pack;  // the weights as per-lane MFMA operands (launch_mlp_pack)
   int use_mlp;
   long long step;
 };
@@ -279,7 +279,12 @@ void launch_hot_disp(hipStream_t s, const Fields& f, const double* const xb[3], 
 void launch_fold_stats(hipStream_t s, const BlockStats* bs, int nblocks, Counters* ctr);
 // ---- return-map surrogate (wf_mlp.hip) -------------------------------------
 // MFMA inference of the surrogate on given feature rows (test hook)
-void launch_mlp_infer(hipStream_t s, const float* w, const float* X, int n, float* y);
+// the surrogate's per-lane MFMA operands, kMlpPackWords 16-byte words a lane
+// (64 lanes), written from the weights after every weight update
+constexpr int kMlpPackWords = 11;
+void launch_mlp_pack(hipStream_t s, const float* w, void* pack);
+// inference on n explicit feature rows from the packed weights
+void launch_mlp_infer(hipStream_t s, const void* pack, const float* X, int n, float* y);
 // one gradient of the mean squared error over n samples (deterministic)
 void launch_mlp_grad(hipStream_t s, const float* w, const float* X, const float* Y, int n,
                      int stride, float* part, float* loss_part, int nblocks, float* grad,

@@ -210,7 +210,7 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Block
     double g0 = 0.0;
     if (ra.use_mlp) {
       MlpWave mw;
-      mlp_wave_load(ra.mlp_w, mw);
+      mlp_wave_load_packed((const uint4*)ra.mlp_pack, mw);
       const float y = mlp_wave_eval(mw, f0, f1, f2, f3);
       if (isfinite(y)) g0 = fmin(fmax((double)y, 0.0), 1.0) * gmax;
     }

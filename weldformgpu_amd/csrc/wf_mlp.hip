@@ -15,12 +15,36 @@ constexpr int kGradThreads = 256;
 constexpr int kGradChunk = 64;
 constexpr int kParamsPerThread = (kMlpParams + kGradThreads - 1) / kGradThreads;
 
-// MFMA inference on explicit rows (n x 4): the test hook of mlp_wave_eval
-__global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ w,
+// the per-lane operands of the 64 lanes of a wavefront (one wavefront)
+__global__ __launch_bounds__(64) void k_mlp_pack(const float* __restrict__ w,
+                                                 uint4* __restrict__ pack) {
+  MlpWave mw;
+  mlp_wave_load(w, mw);
+  MlpLanePack lp;
+  lp.a1 = mw.a1;
+#pragma unroll
+  for (int kb = 0; kb < 4; kb++) lp.a2[kb] = mw.a2[kb];
+  lp.b3 = mw.b3;
+  lp.pad = 0.f;
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    lp.b2r[r] = mw.b2r[r];
+    lp.w3r[r] = mw.w3r[r];
+  }
+  uint4 q[kMlpPackWords];
+  __builtin_memcpy(q, &lp, sizeof(lp));
+  const int l = lane_id();
+#pragma unroll
+  for (int k = 0; k < kMlpPackWords; k++) pack[64 * k + l] = q[k];
+}
+
+// MFMA inference on explicit rows (n x 4): the test hook of mlp_wave_eval,
+// loading the weights as the stress kernel does
+__global__ __launch_bounds__(256) void k_mlp_infer(const uint4* __restrict__ pack,
                                                    const float* __restrict__ X, int n,
                                                    float* __restrict__ y) {
   MlpWave mw;
-  mlp_wave_load(w, mw);
+  mlp_wave_load_packed(pack, mw);
   const int lane = lane_id();
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int nwaves = (gridDim.x * blockDim.x) >> 6;
@@ -197,10 +221,14 @@ __global__ void k_mlp_scatter(Fields f, int n, const int* __restrict__ pos, floa
   Y[s] = f.mly[i];
 }
 
-void launch_mlp_infer(hipStream_t s, const float* w, const float* X, int n, float* y) {
+void launch_mlp_pack(hipStream_t s, const float* w, void* pack) {
+  hipLaunchKernelGGL(k_mlp_pack, dim3(1), dim3(64), 0, s, w, (uint4*)pack);
+}
+
+void launch_mlp_infer(hipStream_t s, const void* pack, const float* X, int n, float* y) {
   if (n <= 0) return;
   const int blocks = std::min(1024, (n + 255) / 256);
-  hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, s, w, X, n, y);
+  hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), 0, s, (const uint4*)pack, X, n, y);
 }
 
 void launch_mlp_grad(hipStream_t s, const float* w, const float* X, const float* Y, int n,

@@ -76,6 +76,38 @@ __device__ __forceinline__ void mlp_wave_load(const float* __restrict__ w, MlpWa
   mw.b3 = w[kMlpOffB3];
 }
 
+// mlp_wave_load's registers of one lane, stored after each weight update so
+// that a wavefront loads its operands as kMlpPackWords coalesced 16-byte
+// words (word q of lane l at pack[64 q + l]) instead of 53 scattered weights
+// a lane
+struct MlpLanePack {
+  bf16x4_t a1;
+  bf16x4_t a2[4];
+  float b3, pad;
+  float b2r[16];
+  float w3r[16];
+};
+static_assert(sizeof(MlpLanePack) == 16 * kMlpPackWords, "16-byte words");
+
+__device__ __forceinline__ void mlp_wave_load_packed(const uint4* __restrict__ pack,
+                                                     MlpWave& mw) {
+  const int l = lane_id();
+  uint4 q[kMlpPackWords];
+#pragma unroll
+  for (int k = 0; k < kMlpPackWords; k++) q[k] = pack[64 * k + l];
+  MlpLanePack lp;
+  __builtin_memcpy(&lp, q, sizeof(lp));
+  mw.a1 = lp.a1;
+#pragma unroll
+  for (int kb = 0; kb < 4; kb++) mw.a2[kb] = lp.a2[kb];
+#pragma unroll
+  for (int r = 0; r < 16; r++) {
+    mw.b2r[r] = lp.b2r[r];
+    mw.w3r[r] = lp.w3r[r];
+  }
+  mw.b3 = lp.b3;
+}
+
 // Every lane passes the features of its own sample; every lane gets its own
 // sample's output.  All 64 lanes must execute this.
 __device__ __forceinline__ float mlp_wave_eval(const MlpWave& mw, float f0, float f1, float f2,
