@@ -15,6 +15,7 @@ import numpy as np
 import pytest
 
 from weldformgpu_amd import _abi
+from weldformgpu_amd.domain import WfError
 
 from harness import randomize, rel_l2, small_c1, to_gpu, to_oracle
 
@@ -211,3 +212,29 @@ def test_uncompacted_staged_sets_over_4096(oracle, wflib, precision, tol):
                 e = rel_l2(d.get(k), st[k])
                 print(f"  {k}: {e:.2e}")
                 assert e < tol, k
+
+
+@pytest.mark.parametrize("precision", [_abi.WF_FP64, _abi.WF_MIXED])
+def test_packed_cell_limit(wflib, precision):
+    """Mixed precision keeps each particle's search cell packed in 11/11/10
+    bits, so its grid holds at most 2048 x 2048 x 1024 cells of 1.01 (2h):
+    a line of particles longer than 1024 cells along z is refused with
+    WF_E_UNSUPPORTED and a message naming the limit; FP64 builds its lists."""
+    sc = randomize(small_c1(0.4), seed=55)
+    dx = 2 * sc.meta["r"]
+    cells = 2 * sc.cfg.h_glob * 1.01 / dx   # a search cell in units of dx
+    nz = int(1100 * cells)                  # > 1024 cells along z
+    idx = np.arange(nz) % sc.n
+    x = np.zeros((nz, 3))
+    x[:, 2] = np.arange(nz) * dx
+    sc = _subset(sc, idx, x)
+    sc.cfg.precision = precision
+    print(f"[case] packed cell limit prec={precision} n={sc.n}", flush=True)
+    with to_gpu(sc, wflib) as d:
+        if precision == _abi.WF_MIXED:
+            with pytest.raises(WfError, match="2048 x 2048 x 1024"):
+                d.nsearch()
+        else:
+            d.nsearch()
+            total, over = d.nsearch_stats()
+            assert over == 0 and total == 2 * (nz - 1) + 2 * (nz - 2)

@@ -796,13 +796,16 @@ void k_pair_gather(Fields f, TileLists tl, typename K::Prm prm) {
       const uint2 w2 = word(k + 2 * KS);
 #pragma unroll
       for (int u0 = 0; u0 < 4; u0 += NB) {
-        // every record of the batch is loaded, also for entries past the
-        // list: their index is the particle index an earlier entry held (or
-        // particle 0), so the load stays in bounds, and straight-line loads
-        // let each pair wait only for its own record instead of all of them
+        // with straight-line batches (drop()) every record of the batch is
+        // loaded, also for entries past the list: their index is the
+        // particle index an earlier entry held (or particle 0), so the load
+        // stays in bounds, and each pair waits only for its own record;
+        // otherwise only the list's entries are loaded (the FP64
+        // acceleration loop is bound by its gathers)
         typename K::Nb nb[NB];
 #pragma unroll
-        for (int u = 0; u < NB; u++) K::load(f, jc[u0 + u], nb[u]);
+        for (int u = 0; u < NB; u++)
+          if (DropOf<K>::on || u0 + u == 0 || u0 + u < nk) K::load(f, jc[u0 + u], nb[u]);
         // the batch's staged positions are read before any pair is evaluated
         // (entries past the list read slot 0 and are not evaluated)
         typename K::PT px[NB], py[NB], pz[NB];
