@@ -59,10 +59,13 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per particle-update of each kernel (FP64 plain path):
 # compulsory SoA reads + writes, neighbour data counted once, list excluded
 ALG_BYTES_FP64 = {"dens": 80, "rates": 144, "stress": 384, "accel": 144, "integrate": 172,
-                  "thermal": 120, "cond_dens": 200}
+                  "thermal": 120, "cond_dens": 200,
+                  # the acceleration pass integrating its particles: both
+                  # kernels' bytes less the acceleration's read back (24 B)
+                  "accel_integrate": 144 + 172 - 24}
 # mixed (FP32 pair-loop inputs and outputs), same accounting
 ALG_BYTES_MIXED = {"dens": 40, "rates": 72, "stress": 204, "accel": 72, "integrate": 184,
-                   "thermal": 60, "cond_dens": 100}
+                   "thermal": 60, "cond_dens": 100, "accel_integrate": 72 + 184 - 12}
 
 
 def _cpu_info():

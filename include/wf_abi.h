@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define WF_ABI_VERSION 5
+#define WF_ABI_VERSION 6
 
 /* ---- status codes ------------------------------------------------------ */
 enum {
@@ -177,6 +177,11 @@ typedef struct wf_config {
                             after the acceleration pass (dom_bid_type
                             AxiSymm_3D, SolverFraser.cu:229-231; set by a
                             cylindrical-grid slice, WeldFormGPU.cu:366-375) */
+  int fuse_integrate;    /* 1 (default): in contact-free, non-axisymmetric
+                            mixed-precision steps with every particle solid,
+                            the acceleration pass integrates each particle
+                            (BC + position/velocity update) as soon as its
+                            acceleration is summed; 0: a separate pass     */
 } wf_config;
 
 /* Fill *cfg with the reference's defaults: alpha=1, beta=0
@@ -345,6 +350,8 @@ enum {
   WF_K_COND_DENS,   /* conduction pair loop + leapfrog fused with the
                        next step's density pass + update (inside one
                        wf_step call, when no rebuild runs between)  */
+  WF_K_ACCEL_INTEGRATE, /* acceleration pair loop with each particle's BC +
+                           position/velocity update (contact-free steps)  */
   WF_K_COUNT_
 };
 /* Accumulated milliseconds and launch count of kernel id k.  Each timed

@@ -80,3 +80,30 @@ def test_fused_steps_with_rebuilds_match_oracle(oracle, wflib):
         for k in FIELDS:
             assert rel_l2(d.get(k), st[k]) < 1e-9, k
         assert np.all(np.isfinite(d.get("T")))
+
+
+@pytest.mark.parametrize("stale", [True, False])
+def test_accel_integrate_fusion_is_bit_identical(wflib, stale):
+    """Mixed precision: the acceleration pass that integrates each particle as
+    soon as its sum is done (cfg.fuse_integrate, the default) against the
+    separate integration pass (fuse_integrate = 0).  Both round the same
+    arithmetic in the same order (integrate_particle, no FMA contraction), so
+    every field is bit-identical; only the kernel ids the timing reports
+    differ."""
+    runs = {}
+    for fuse in (1, 0):
+        sc = _case(_abi.WF_MIXED, stale)
+        sc.cfg.fuse_integrate = fuse
+        print(f"[case] accel+integrate fusion {fuse} stale={stale}", flush=True)
+        with to_gpu(sc, wflib) as d:
+            d.set_timing(True)
+            d.step(NSTEPS)
+            runs[fuse] = ({k: d.get(k) for k in FIELDS + ["x", "a"]},
+                          d.kernel_time("accel_integrate")[1], d.kernel_time("integrate")[1],
+                          d.stats())
+    (a, fa, ia, sa), (b, fb, ib, sb) = runs[1], runs[0]
+    assert fa == NSTEPS and ia == 0
+    assert fb == 0 and ib == NSTEPS
+    assert sa["newton_calls"] > 0 and sa["ns_builds"] == sb["ns_builds"]
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k

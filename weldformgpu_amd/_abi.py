@@ -7,7 +7,7 @@ product can silently run without the native code.
 import ctypes as C
 import os
 
-WF_ABI_VERSION = 5
+WF_ABI_VERSION = 6
 
 WF_OK = 0
 WF_E_ARG = -1
@@ -49,7 +49,7 @@ for _k, _n in enumerate(["ID", "BC_T", "nb_count", "cont_nb_count"]):
     FIELDS[_n] = (64 + _k, 1, "i4")
 
 KERNELS = ["dens", "rates", "stress", "accel", "integrate", "thermal", "nsearch", "mlp", "return",
-           "contact", "cond_dens"]
+           "contact", "cond_dens", "accel_integrate"]
 WF_MLP_PARAMS = 1249
 
 
@@ -81,7 +81,8 @@ class wf_config(C.Structure):
                 ("fric_heat", C.c_double), ("ns_skin", C.c_double), ("ep_hot", C.c_double),
                 ("mlp_train_iters", C.c_int), ("mlp_interval_max", C.c_int),
                 ("mlp_lr", C.c_double), ("mlp_refit_ratio", C.c_double),
-                ("ns_global_every", C.c_int), ("axisymm", C.c_int)]
+                ("ns_global_every", C.c_int), ("axisymm", C.c_int),
+                ("fuse_integrate", C.c_int)]
 
 
 class wf_stats(C.Structure):
@@ -116,6 +117,7 @@ def default_config():
     c.mlp_lr = 2e-3
     c.mlp_refit_ratio = 2.0
     c.ns_global_every = 32
+    c.fuse_integrate = 1
     c.contact = 0
     c.contact_law = WF_CONTACT_REFERENCE
     c.contact_kappa = 0.1

@@ -57,6 +57,70 @@ __device__ __forceinline__ void unpack_cell(int c, int& cx, int& cy, int& cz) {
   cz = (c >> 22) & 1023;
 }
 
+// ---- integration of one particle -------------------------------------------
+// BC, position and velocity update of particle i from its acceleration a:
+// ApplyBCVelKernel (Mechanical.cu:572-589) before and after
+// UpdatePosFraserKernel (:179-191, u = v dt + a dt^2/2) and UpdateVelKernel
+// (:142-152), plus the record the next passes read (REC)
+template <int REC>
+__device__ __forceinline__ void integrate_particle(const Fields& f, int i, double a0, double a1,
+                                                   double a2, double dt, const BcTable& bc,
+                                                   const Frame32& fr, const IntegrateOut& o) {
+  // no FMA contraction: the separate integration pass and the acceleration
+  // pass that integrates its own particles round identically (and as the
+  // FP64 oracle does)
+#pragma clang fp contract(off)
+  const int id = f.ID[i];
+  double v0 = f.v[0][i], v1 = f.v[1][i], v2 = f.v[2][i];
+  int hit = -1;
+  for (int b = 0; b < bc.count; b++)
+    if (id == bc.zone[b]) hit = b;
+  if (hit >= 0) {
+    a0 = a1 = a2 = 0.0;
+    v0 = bc.val[hit][0];
+    v1 = bc.val[hit][1];
+    v2 = bc.val[hit][2];
+  }
+  const double u0 = v0 * dt + ((a0 * 0.5) * dt) * dt;
+  const double u1 = v1 * dt + ((a1 * 0.5) * dt) * dt;
+  const double u2 = v2 * dt + ((a2 * 0.5) * dt) * dt;
+  const double x0 = f.x[0][i] + u0, x1 = f.x[1][i] + u1, x2 = f.x[2][i] + u2;
+  o.x[0][i] = x0;
+  o.x[1][i] = x1;
+  o.x[2][i] = x2;
+  f.u[0][i] += u0;
+  f.u[1][i] += u1;
+  f.u[2][i] += u2;
+  v0 += a0 * dt;
+  v1 += a1 * dt;
+  v2 += a2 * dt;
+  if (hit >= 0) {
+    a0 = a1 = a2 = 0.0;
+    v0 = bc.val[hit][0];
+    v1 = bc.val[hit][1];
+    v2 = bc.val[hit][2];
+    f.va[0][i] = v0;
+    f.va[1][i] = v1;
+    f.va[2][i] = v2;
+  }
+  f.v[0][i] = v0;
+  f.v[1][i] = v1;
+  f.v[2][i] = v2;
+  f.a[0][i] = a0;
+  f.a[1][i] = a1;
+  f.a[2][i] = a2;
+  if (REC == kRec64) f.kv[i] = make_double4(v0, v1, v2, f.kv[i].w);
+  if (REC == kRec32) {
+    int cx, cy, cz;
+    const float cb = f.p4[i].w;  // the cell kept since the last refresh
+    unpack_cell(__float_as_int(cb), cx, cy, cz);
+    o.p4[i] = make_float4((float)(x0 - (fr.org[0] + cx * fr.cs)),
+                          (float)(x1 - (fr.org[1] + cy * fr.cs)),
+                          (float)(x2 - (fr.org[2] + cz * fr.cs)), cb);
+    o.v4[i] = make_float4((float)v0, (float)v1, (float)v2, f.v4[i].w);
+  }
+}
+
 // ---- wavefront (64-lane) reductions ---------------------------------------
 __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
 #pragma unroll

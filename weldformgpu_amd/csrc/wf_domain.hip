@@ -168,6 +168,9 @@ struct wf_domain {
   bool isfirst_step = true;
   int ns_builds = 0;
   int mlp_trainings = 0;
+  // the spare plane the mixed acceleration pass writes the integrated
+  // position mirror to (the velocity mirror goes to v4_alt)
+  float4* p4_spare = nullptr;
   // timing: ev0/ev1 bracket wf_time_pass; each timed kernel launch gets an
   // event pair of the pool (no system-scope fence, no wait per launch), read
   // when the pool is full or the times are asked for
@@ -659,6 +662,7 @@ void wf_config_default(wf_config* c) {
   c->ns_skin = 0.0;
   c->ep_hot = 0.05;
   c->ns_global_every = 32;
+  c->fuse_integrate = 1;
 }
 
 int wf_field_shape(int field, int* components, int* elem_bytes) {
@@ -722,6 +726,7 @@ int wf_create(const wf_config* cfg, wf_domain** out) {
   if (d->mixed) {
     if (!rc) rc = d->alloc_plane_f(&d->f.h32);
     if (!rc) rc = d->dev_alloc((void**)&d->f.p4, sizeof(float4) * d->npad);
+    if (!rc) rc = d->dev_alloc((void**)&d->p4_spare, sizeof(float4) * d->npad);
     float4** f4[] = {&d->f.v4, &d->f.v4_alt};
     for (float4** q : f4)
       if (!rc) rc = d->dev_alloc((void**)q, sizeof(float4) * d->npad);
@@ -1662,11 +1667,33 @@ int wf_step(wf_domain* d, int n_steps) {
       launch_stress(d->stream, d->f, p, d->bstats, true, true, rec, mx, d->ret_args());
       if ((rc = timer_end(d, WF_K_STRESS))) return rc;
       launch_fold_stats(d->stream, d->bstats, (p.solid + kBlock - 1) / kBlock, d->ctr);
+      // Without contact forces or the axisymmetric clamp between them, the
+      // mixed acceleration pass integrates each particle as soon as its sum
+      // is done (the reference's BC + UpdatePosFraser + UpdateVel + BC,
+      // SolverFraser.cu:273-281): the new FP32 mirrors go to spare planes,
+      // swapped in after the pass.  (The FP64 pass, at lower occupancy, was
+      // slower fused: 0.396 against 0.332 + 0.032 ms on a 759k-particle
+      // cylinder.)
+      const bool fuse_int = mx && d->cfg.fuse_integrate && !d->cfg.axisymm &&
+                            !d->cfg.contact && d->solid == d->n;
       timer_begin(d, true);
-      if (mx) launch_accel_mixed(d->stream, d->f, d->tl, d->solid, p.alpha, p.beta, cs32, uh);
-      else launch_accel_tiled(d->stream, d->f, d->tl, d->solid, p.alpha, p.beta, uh);
-      if (d->cfg.axisymm) launch_axisymm_bc(d->stream, d->f, d->n);  // SolverFraser.cu:229-231
-      if ((rc = timer_end(d, WF_K_ACCEL))) return rc;
+      if (fuse_int) {
+        // the FP32 pass reads no FP64 position, so x is updated in place
+        IntegrateOut out{};
+        for (int c = 0; c < 3; c++) out.x[c] = d->f.x[c];
+        out.p4 = d->p4_spare;
+        out.v4 = d->f.v4_alt;
+        launch_accel_integrate_mixed(d->stream, d->f, d->tl, d->solid, p.alpha, p.beta, cs32, uh,
+                                     d->cfg.dt, d->bcs, d->frame, out);
+        std::swap(d->f.p4, d->p4_spare);
+        std::swap(d->f.v4, d->f.v4_alt);
+        if ((rc = timer_end(d, WF_K_ACCEL_INTEGRATE))) return rc;
+      } else {
+        if (mx) launch_accel_mixed(d->stream, d->f, d->tl, d->solid, p.alpha, p.beta, cs32, uh);
+        else launch_accel_tiled(d->stream, d->f, d->tl, d->solid, p.alpha, p.beta, uh);
+        if (d->cfg.axisymm) launch_axisymm_bc(d->stream, d->f, d->n);  // SolverFraser.cu:229-231
+        if ((rc = timer_end(d, WF_K_ACCEL))) return rc;
+      }
       if (d->cfg.contact) {  // SolverFraser.cu:233-257
         timer_begin(d);
         if ((rc = d->surface_init())) return rc;
@@ -1683,9 +1710,11 @@ int wf_step(wf_domain* d, int n_steps) {
         HIPC(hipEventRecord(d->ev_yield, d->stream));
         d->yield_pending = true;
       }
-      timer_begin(d);
-      launch_integrate(d->stream, d->f, d->n, d->cfg.dt, d->bcs, rec, d->frame);
-      if ((rc = timer_end(d, WF_K_INTEGRATE))) return rc;
+      if (!fuse_int) {
+        timer_begin(d);
+        launch_integrate(d->stream, d->f, d->n, d->cfg.dt, d->bcs, rec, d->frame);
+        if ((rc = timer_end(d, WF_K_INTEGRATE))) return rc;
+      }
       if (d->plastic_zone() && (rc = resolve_yield(d))) return rc;
       if (d->plastic_zone() && d->is_yielding) {  // the rebuild trigger, read back each step
         hipLaunchKernelGGL(k_zero_bytes, dim3(1), dim3(256), 0, d->stream,

@@ -209,6 +209,15 @@ struct BcTable {
   double val[WF_MAX_BCS][3];
 };
 
+// Where the integration writes what other threads of the same kernel may
+// still read: the new positions (x) and FP32 mirrors (p4, v4).  The
+// integration kernel writes them in place; the acceleration pass that
+// integrates its own particles writes the spare planes, swapped in after.
+struct IntegrateOut {
+  double* x[3];
+  float4 *p4, *v4;
+};
+
 // device-side counters written by the kernels (one allocation)
 struct Counters {
   unsigned long long max_pl_strain_bits;  // max eps_p as ordered bits (eps_p >= 0)
@@ -241,6 +250,14 @@ void launch_accel_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int
                         double alpha, double beta, bool uh);
 void launch_cond_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int n, double T_inf,
                        double h_conv, double dt, bool leap, bool first, bool uh);
+
+// the mixed acceleration pass with each particle's BC + position / velocity
+// update done as soon as its acceleration is summed (contact-free,
+// non-axisymmetric steps with every particle solid); the new positions and
+// FP32 mirrors go to out's planes (IntegrateOut, above)
+void launch_accel_integrate_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
+                                  double alpha, double beta, float cs32, bool uh, double dt,
+                                  const BcTable& bc, const Frame32& fr, const IntegrateOut& out);
 
 // mixed precision (FP32 pair loops, FP32 terms summed in FP64)
 // uh: every particle has the same h

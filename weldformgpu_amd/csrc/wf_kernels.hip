@@ -365,56 +365,9 @@ __global__ __launch_bounds__(kBlock) void k_integrate(Fields f, int n, double dt
                                                      Frame32 fr) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int id = f.ID[i];
-  double v0 = f.v[0][i], v1 = f.v[1][i], v2 = f.v[2][i];
-  double a0 = f.a[0][i], a1 = f.a[1][i], a2 = f.a[2][i];
-  int hit = -1;
-  for (int b = 0; b < bc.count; b++)
-    if (id == bc.zone[b]) hit = b;
-  if (hit >= 0) {
-    a0 = a1 = a2 = 0.0;
-    v0 = bc.val[hit][0];
-    v1 = bc.val[hit][1];
-    v2 = bc.val[hit][2];
-  }
-  double u0 = v0 * dt + ((a0 * 0.5) * dt) * dt;
-  double u1 = v1 * dt + ((a1 * 0.5) * dt) * dt;
-  double u2 = v2 * dt + ((a2 * 0.5) * dt) * dt;
-  f.x[0][i] += u0;
-  f.x[1][i] += u1;
-  f.x[2][i] += u2;
-  f.u[0][i] += u0;
-  f.u[1][i] += u1;
-  f.u[2][i] += u2;
-  v0 += a0 * dt;
-  v1 += a1 * dt;
-  v2 += a2 * dt;
-  if (hit >= 0) {
-    a0 = a1 = a2 = 0.0;
-    v0 = bc.val[hit][0];
-    v1 = bc.val[hit][1];
-    v2 = bc.val[hit][2];
-    f.va[0][i] = v0;
-    f.va[1][i] = v1;
-    f.va[2][i] = v2;
-  }
-  f.v[0][i] = v0;
-  f.v[1][i] = v1;
-  f.v[2][i] = v2;
-  f.a[0][i] = a0;
-  f.a[1][i] = a1;
-  f.a[2][i] = a2;
-  if (REC == kRec64) f.kv[i] = make_double4(v0, v1, v2, f.kv[i].w);
-  if (REC == kRec32) {
-    int cx, cy, cz;
-    const float cb = f.p4[i].w;  // the cell kept since the last refresh
-    unpack_cell(__float_as_int(cb), cx, cy, cz);
-    f.p4[i] = make_float4((float)(f.x[0][i] - (fr.org[0] + cx * fr.cs)),
-                          (float)(f.x[1][i] - (fr.org[1] + cy * fr.cs)),
-                          (float)(f.x[2][i] - (fr.org[2] + cz * fr.cs)), cb);
-    const float mdj = f.v4[i].w;
-    f.v4[i] = make_float4((float)v0, (float)v1, (float)v2, mdj);
-  }
+  // in place: each thread reads and writes only its own particle
+  const IntegrateOut o{{f.x[0], f.x[1], f.x[2]}, f.p4, f.v4};
+  integrate_particle<REC>(f, i, f.a[0][i], f.a[1][i], f.a[2][i], dt, bc, fr, o);
 }
 
 // every FP32 mirror from the FP64 state; the integer cell of each particle is

@@ -95,6 +95,18 @@ __device__ __forceinline__ T gat(const void* base, int j, unsigned off = 0) {
   return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (unsigned)j * S + off);
 }
 
+// The mixed acceleration pass may integrate each particle as soon as its sum
+// is done (the step's BC + position/velocity update, integrate_particle):
+// other tiles still read the old FP32 mirrors, so the new ones go to the
+// spare planes in out, swapped in by the caller after the pass.
+struct IntegratePrm {
+  int on;
+  double dt;
+  BcTable bc;
+  Frame32 fr;
+  IntegrateOut out;
+};
+
 // FP64 policies need no per-tile context
 struct NoCtx {};
 
@@ -1098,6 +1110,7 @@ struct AccelG {
     int limit;
     float alpha32, beta32;
     float cs;
+    IntegratePrm ip;  // integrate each particle after its sum (ip.on)
   };
   struct Own {
     float x, y, z, vx, vy, vz, di, Ci, s0, s1, s2, s3, s4, s5;
@@ -1181,8 +1194,13 @@ struct AccelG {
     a.az = __builtin_fmaf(mzz, gz, __builtin_fmaf(myz, gy, __builtin_fmaf(mxz, gx, a.az)));
   }
   __device__ static void finish(const Fields& f, int i, bool act, const Own&, const Acc& a,
-                                const Prm&) {
+                                const Prm& p) {
     if (!act) return;
+    if (p.ip.on) {
+      integrate_particle<kRec32>(f, i, a.ax.val(), a.ay.val(), a.az.val(), p.ip.dt, p.ip.bc,
+                                 p.ip.fr, p.ip.out);
+      return;
+    }
     f.a[0][i] = a.ax.val();
     f.a[1][i] = a.ay.val();
     f.a[2][i] = a.az.val();
@@ -1618,6 +1636,19 @@ void launch_accel_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int
   } else {
     launch_gather<AccelD>(s, f, tl, ntiles(solid), uh, solid, alpha, beta);
   }
+}
+void launch_accel_integrate_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
+                                  double alpha, double beta, float cs32, bool uh, double dt,
+                                  const BcTable& bc, const Frame32& fr, const IntegrateOut& out) {
+  if (solid <= 0) return;
+  const IntegratePrm ip{1, dt, bc, fr, out};
+  auto go = [&](auto tag) {
+    typedef decltype(tag) K;
+    launch_pair<K>(s, ntiles(solid), f, tl,
+                   typename K::Prm{solid, (float)alpha, (float)beta, cs32, ip});
+  };
+  if (beta != 0.0) uh ? go(AccelG<true, true>{}) : go(AccelG<false, true>{});
+  else uh ? go(AccelG<true, false>{}) : go(AccelG<false, false>{});
 }
 void launch_cond_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int n, double T_inf,
                        double h_conv, double dt, bool leap, bool first, bool uh) {
