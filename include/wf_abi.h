@@ -182,6 +182,11 @@ typedef struct wf_config {
                             the acceleration pass integrates each particle
                             (BC + position/velocity update) as soon as its
                             acceleration is summed; 0: a separate pass     */
+  int ns_compact_max;    /* a tile whose adjacent cells hold at most this
+                            many candidates gets its staged set compacted to
+                            the particles its lists reference; larger sets
+                            are staged whole (0: 4096, the list builder's
+                            LDS flag array; larger values count as 4096)   */
 } wf_config;
 
 /* Fill *cfg with the reference's defaults: alpha=1, beta=0

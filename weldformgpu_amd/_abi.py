@@ -82,7 +82,7 @@ class wf_config(C.Structure):
                 ("mlp_train_iters", C.c_int), ("mlp_interval_max", C.c_int),
                 ("mlp_lr", C.c_double), ("mlp_refit_ratio", C.c_double),
                 ("ns_global_every", C.c_int), ("axisymm", C.c_int),
-                ("fuse_integrate", C.c_int)]
+                ("fuse_integrate", C.c_int), ("ns_compact_max", C.c_int)]
 
 
 class wf_stats(C.Structure):

@@ -683,6 +683,7 @@ int wf_create(const wf_config* cfg, wf_domain** out) {
   // byte offsets
   if (cfg->n_particles > kMaxParticles)
     return fail(WF_E_ARG, "n_particles exceeds 40,000,000");
+  if (cfg->ns_compact_max < 0) return fail(WF_E_ARG, "ns_compact_max must be >= 0");
   if (cfg->solid_count < 0 || cfg->solid_count > cfg->n_particles)
     return fail(WF_E_ARG, "solid_count out of range");
   if (cfg->precision != WF_FP64 && cfg->precision != WF_MIXED)
@@ -1110,6 +1111,7 @@ int wf_nsearch_build(wf_domain* d) {
   const double* src[3];
   for (int c = 0; c < 3; c++) src[c] = stale ? d->f.xs[c] : d->f.x[c];
   const double radius = 2.0 * d->cfg.h_glob + (pz ? d->cfg.ns_skin : 0.0);
+  ns_set_compact_max(&d->ns, d->cfg.ns_compact_max);
   // mixed precision keeps each particle's cell packed in 11/11/10 bits
   int rc = ns_build(d->stream, &d->ns, src, d->n, d->npad, radius, pz ? kListCap : kMaxNb,
                     d->gather, d->nb_count, d->ctr, d->perm, d->perm_tmp, &d->tl,

@@ -355,6 +355,8 @@ int ns_build(hipStream_t s, NsWork** work, const double* const pos_src[3], int n
              int* perm, int* perm_tmp, TileLists* tiles, GridDesc* grid_out,
              const int* max_dim);
 void ns_free(NsWork* w);
+// the list builder's compaction limit (launch_tile_lists; 0: 4096)
+void ns_set_compact_max(NsWork** work, int compact_max);
 // exclusive scan of m ints into out[0..m] (out[m] = total); part needs
 // ceil(m/1024)+1 ints
 void launch_scan_exclusive(hipStream_t s, const int* in, int64_t m, int* part, int* out);

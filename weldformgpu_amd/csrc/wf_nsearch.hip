@@ -28,6 +28,7 @@
 namespace wf {
 
 struct NsWork {
+  int compact_max = 0;  // launch_tile_lists' compaction limit (0: its default)
   int n_cap = 0;
   int64_t key_cap = 0;
   float* pf[3] = {nullptr, nullptr, nullptr};   // float positions, old order
@@ -391,6 +392,11 @@ static int ensure(NsWork* w, int n, int64_t nkeys) {
   return WF_OK;
 }
 
+void ns_set_compact_max(NsWork** work, int compact_max) {
+  if (!*work) *work = new NsWork();
+  (*work)->compact_max = compact_max;
+}
+
 void ns_free(NsWork* w) {
   if (!w) return;
   for (int c = 0; c < 3; c++) {
@@ -513,7 +519,7 @@ int ns_build(hipStream_t s, NsWork** work, const double* const pos_src[3], int n
   }
   launch_tile_lists(s, pfn, nullptr, nullptr, nullptr, nt, n, g, w->cell_start, w->tile_keys,
                     w->tile_nc, w->tile_uoff, r2, kListCap, cap, npad, count, w->slot, w->tile_j,
-                    w->tile_uc, w->tile_over, w->tile_ovf);
+                    w->tile_uc, w->tile_over, w->tile_ovf, w->compact_max);
   hipLaunchKernelGGL(k_sum_tiles, dim3(1), dim3(256), 0, s, w->tile_over, nt, &ctr->nb_over_cap);
   if (hipGetLastError() != hipSuccess) return WF_E_HIP;
   *grid_out = g;
@@ -578,7 +584,7 @@ int ns_relist_local(hipStream_t s, NsWork* w, const double* const x[3], const do
   launch_tile_lists(s, pfn, own, w->lgather, w->tile_list, ntl, n, g, w->cell_start,
                     w->tile_keys, w->tile_nc, w->tile_uoff, r2, kListCap, cap, npad, count,
                     w->slot,
-                    w->tile_j, w->tile_uc, w->tile_over, w->tile_ovf);
+                    w->tile_j, w->tile_uc, w->tile_over, w->tile_ovf, w->compact_max);
   if (hipMemcpyAsync(w->tile_host + 1, w->tile_ovf, sizeof(int), hipMemcpyDeviceToHost, s) !=
       hipSuccess)
     return WF_E_HIP;

@@ -16,6 +16,7 @@
 // and read a neighbour's position from LDS through its slot.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 
 #include "wf_device.hpp"
@@ -190,8 +191,9 @@ __global__ __launch_bounds__(kTileP) void k_tile_cells(const float* __restrict__
   }
 }
 
-// adjacent-cell sets up to this size are compacted to the particles that
-// some list references (larger ones are staged whole)
+// adjacent-cell sets up to compact_max (at most this, the LDS flag array)
+// are compacted to the particles that some list references (larger ones are
+// staged whole)
 constexpr int kCompactMax = 4096;
 // candidate positions held in LDS at a time by the list builder
 constexpr int kListStage = 512;
@@ -259,7 +261,7 @@ __global__ __launch_bounds__(kListThreads) void k_tile_lists(
     const int* __restrict__ tile_nc, const int* __restrict__ tile_uoff, float r2, int cap,
     int rcap, int64_t npad, int* __restrict__ count, unsigned short* __restrict__ slot,
     int* __restrict__ tile_j, int* __restrict__ tile_uc, int* __restrict__ tile_over,
-    int* overflow) {
+    int* overflow, int compact_max) {
 #pragma clang fp contract(off)
   constexpr int NT = kListThreads;
   __shared__ int sb[kTileMaxCells], sn[kTileMaxCells], spre[kTileMaxCells];
@@ -288,7 +290,7 @@ __global__ __launch_bounds__(kListThreads) void k_tile_lists(
     spre[p] = run0;
     run0 += sn[p];
   }
-  const bool compact = U <= kCompactMax;
+  const bool compact = U <= compact_max;
   if (compact)
     for (int s = tid; s < U; s += NT) sidx[s] = 0;
   __syncthreads();
@@ -428,14 +430,16 @@ void launch_tile_lists(hipStream_t s, const float* const pf[3], const float* con
                        const int* cell_start, const int* tile_keys, const int* tile_nc,
                        const int* tile_uoff, float r2, int cap, int rcap, int64_t npad,
                        int* count, unsigned short* slot, int* tile_j, int* tile_uc,
-                       int* tile_over, int* overflow) {
+                       int* tile_over, int* overflow, int compact_max) {
   if (ntiles <= 0) return;
+  compact_max = compact_max > 0 ? std::min(compact_max, kCompactMax) : kCompactMax;
   const float* o0 = own ? own[0] : nullptr;
   const float* o1 = own ? own[1] : nullptr;
   const float* o2 = own ? own[2] : nullptr;
   hipLaunchKernelGGL(k_tile_lists, dim3(ntiles), dim3(kListThreads), 0, s, pf[0], pf[1], pf[2],
                      o0, o1, o2, map, tiles, n, g, cell_start, tile_keys, tile_nc, tile_uoff, r2,
-                     cap, rcap, npad, count, slot, tile_j, tile_uc, tile_over, overflow);
+                     cap, rcap, npad, count, slot, tile_j, tile_uc, tile_over, overflow,
+                     compact_max);
 }
 
 }  // namespace wf

@@ -75,12 +75,14 @@ void launch_tile_cells(hipStream_t s, const float* const pf[3], int n, const Gri
 // pf: cell-sorted positions; own (may be null): positions by storage index
 // and map (may be null) the storage index of each sorted position, for a
 // build without a reorder; a list holds at most cap entries, and tile_over[t]
-// gets the number of the tile's particles with more than rcap neighbours
+// gets the number of the tile's particles with more than rcap neighbours;
+// a tile's adjacent-cell set is compacted to the particles its lists
+// reference when it holds at most compact_max (0: 4096) candidates
 void launch_tile_lists(hipStream_t s, const float* const pf[3], const float* const own[3],
                        const int* map, const int* tiles, int ntiles, int n, const GridDesc& g,
                        const int* cell_start, const int* tile_keys, const int* tile_nc,
                        const int* tile_uoff, float r2, int cap, int rcap, int64_t npad,
                        int* count, unsigned short* slot, int* tile_j, int* tile_uc,
-                       int* tile_over, int* overflow);
+                       int* tile_over, int* overflow, int compact_max);
 
 }  // namespace wf
