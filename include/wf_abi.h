@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define WF_ABI_VERSION 6
+#define WF_ABI_VERSION 7
 
 /* ---- status codes ------------------------------------------------------ */
 enum {

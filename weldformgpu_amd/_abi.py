@@ -7,7 +7,7 @@ product can silently run without the native code.
 import ctypes as C
 import os
 
-WF_ABI_VERSION = 6
+WF_ABI_VERSION = 7
 
 WF_OK = 0
 WF_E_ARG = -1
