@@ -107,3 +107,35 @@ def test_accel_integrate_fusion_is_bit_identical(wflib, stale):
     assert sa["newton_calls"] > 0 and sa["ns_builds"] == sb["ns_builds"]
     for k in a:
         assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("prec", [_abi.WF_FP64, _abi.WF_MIXED])
+def test_fused_call_with_plastic_zone_trigger(wflib, prec):
+    """Under the plastic-zone trigger the fused pass is queued before the host
+    has read whether the next step starts with a rebuild.  Its density half
+    reads the trigger from device memory and writes nothing when a rebuild is
+    due; the host reads the same value behind the pass and runs the density
+    pass after the rebuild.  A 12-step call against 12 single-step calls
+    (skin 0.01 dx: the oracle rebuilds at steps 4, 6 and 9 of this case):
+    the same rebuilds and fields (FP64 1e-13, mixed 1e-6), a fused pass in
+    every step but the last, and a separate density pass in the first step
+    and after each rebuild."""
+    def case():
+        sc = _case(prec, stale=False)
+        sc.cfg.rebuild_policy = _abi.WF_REBUILD_PLASTIC_ZONE
+        sc.cfg.ns_skin = 0.01 * 2 * sc.meta["r"]
+        sc.cfg.ep_hot = 1e-6
+        return sc
+    print(f"[case] fusion under the plastic-zone trigger prec={prec}", flush=True)
+    a, fused, dens, sa = _run(case(), wflib, single=False)
+    b, fused1, dens1, sb = _run(case(), wflib, single=True)
+    rebuilds = sa["ns_builds"] - 1  # the initial build aside
+    print(f"  rebuilds {rebuilds}, fused {fused}, density passes {dens}")
+    assert sa["ns_builds"] == sb["ns_builds"] and rebuilds >= 2
+    assert fused1 == 0 and dens1 == NSTEPS
+    assert fused == NSTEPS - 1 and dens == 1 + rebuilds
+    tol = 1e-13 if prec == _abi.WF_FP64 else 1e-6
+    for k in FIELDS:
+        err = rel_l2(a[k], b[k])
+        print(f"  {k}: {err:.2e}")
+        assert err <= tol, k

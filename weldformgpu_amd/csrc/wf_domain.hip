@@ -185,6 +185,11 @@ struct wf_domain {
   // pass and read when the step's later passes are queued
   hipEvent_t ev_yield = nullptr;
   bool yield_pending = false;
+  // yielding, under the plastic-zone trigger: the step's largest squared hot
+  // displacement, copied to pz_d2_host behind the hot-displacement pass and
+  // read once the step's conduction pass is queued
+  hipEvent_t ev_pz = nullptr;
+  bool pz_wait = false;
   double kt_ms[WF_K_COUNT_] = {0};
   int64_t kt_n[WF_K_COUNT_] = {0};
 
@@ -833,6 +838,8 @@ int wf_create(const wf_config* cfg, wf_domain** out) {
   }
   if (!rc && hipEventCreateWithFlags(&d->ev_yield, hipEventDisableTiming) != hipSuccess)
     rc = WF_E_HIP;
+  if (!rc && hipEventCreateWithFlags(&d->ev_pz, hipEventDisableTiming) != hipSuccess)
+    rc = WF_E_HIP;
   if (!rc && (hipEventCreate(&d->ev0) != hipSuccess || hipEventCreate(&d->ev1) != hipSuccess))
     rc = WF_E_HIP;
   for (int b = 0; b < 2 && !rc; b++)
@@ -870,6 +877,7 @@ int wf_destroy(wf_domain* d) {
     for (int e = 0; e < wf_domain::kTimerPool; e++)
       if (d->tev[b][e]) hipEventDestroy(d->tev[b][e]);
   if (d->ev_yield) hipEventDestroy(d->ev_yield);
+  if (d->ev_pz) hipEventDestroy(d->ev_pz);
   if (d->stream) hipStreamDestroy(d->stream);
   delete d;
   return WF_OK;
@@ -1603,11 +1611,27 @@ static int resolve_yield(wf_domain* d) {
   return WF_OK;
 }
 
+// The plastic-zone trigger (SolverFraser.cu:166 under the skin policy): a
+// rebuild starts the next step when a hot particle moved more than skin/2
+// since the last build.  The host reads the step's largest squared
+// displacement once the conduction pass behind it is queued.
+static int resolve_pz(wf_domain* d) {
+  if (!d->pz_wait) return WF_OK;
+  HIPC(hipEventSynchronize(d->ev_pz));
+  d->pz_wait = false;
+  double d2;
+  std::memcpy(&d2, d->pz_d2_host, sizeof(d2));
+  d->pz_pending = 4.0 * d2 > d->cfg.ns_skin * d->cfg.ns_skin;
+  return WF_OK;
+}
+
 // whether the next step starts with a rebuild that changes the lists; the
 // yield read-back is waited for only when the answer depends on it
 static int rebuild_next_step(wf_domain* d, bool* out) {
   *out = false;
   if (d->plastic_zone()) {
+    int rc = resolve_pz(d);
+    if (rc) return rc;
     *out = d->is_yielding && d->pz_pending;
     return WF_OK;
   }
@@ -1637,6 +1661,7 @@ int wf_step(wf_domain* d, int n_steps) {
   for (int s = 0; s < n_steps; s++) {
     StepParams p = d->params();
     if (d->cfg.mechanics) {
+      if ((rc = resolve_pz(d))) return rc;
       if (d->plastic_zone() ? (d->is_yielding && d->pz_pending)
                             : (d->ts_i == 0 && d->is_yielding)) {
         rc = d->relist_local();  // WF_E_STATE: a full rebuild is due
@@ -1724,27 +1749,31 @@ int wf_step(wf_domain* d, int n_steps) {
         launch_hot_disp(d->stream, d->f, d->xb, d->n, d->cfg.ep_hot, d->pz_d2);
         HIPC(hipMemcpyAsync(d->pz_d2_host, d->pz_d2, sizeof(unsigned long long),
                             hipMemcpyDeviceToHost, d->stream));
-        HIPC(hipStreamSynchronize(d->stream));
-        double d2;
-        std::memcpy(&d2, d->pz_d2_host, sizeof(d2));
-        d->pz_pending = 4.0 * d2 > d->cfg.ns_skin * d->cfg.ns_skin;
+        HIPC(hipEventRecord(d->ev_pz, d->stream));
+        d->pz_wait = true;
       }
     }
     if (d->cfg.thermal) {
       d->refresh_mirrors();
+      // With the trigger's answer still on its way, the fused pass is queued
+      // anyway and takes the answer from device memory: when a rebuild is
+      // due its density half writes nothing, and the host, reading the same
+      // value behind it, runs the density pass after the rebuild as usual.
+      const bool pz_open = d->pz_wait;
       bool rebuild_next = false;
-      if (d->cfg.mechanics && s + 1 < n_steps && (rc = rebuild_next_step(d, &rebuild_next)))
+      if (d->cfg.mechanics && s + 1 < n_steps && !pz_open &&
+          (rc = rebuild_next_step(d, &rebuild_next)))
         return rc;
       const bool fuse = d->cfg.mechanics && s + 1 < n_steps && !rebuild_next;
+      const unsigned long long* pz_d2 = pz_open ? d->pz_d2 : nullptr;
+      const double skin2 = d->cfg.ns_skin * d->cfg.ns_skin;
       timer_begin(d, true);
       if (fuse && d->mixed) {
         launch_cond_dens_mixed(d->stream, d->f, d->tl, d->n, d->solid, p.T_inf, p.h_conv, p.dt,
-                               true, d->isfirst_step, d->frame.cs32, d->uniform_h);
-        std::swap(d->f.v4, d->f.v4_alt);
+                               true, d->isfirst_step, d->frame.cs32, d->uniform_h, pz_d2, skin2);
       } else if (fuse) {
         launch_cond_dens_tiled(d->stream, d->f, d->tl, d->n, d->solid, p.T_inf, p.h_conv,
-                               p.dt, true, d->isfirst_step, d->uniform_h);
-        std::swap(d->f.kv, d->f.kv_alt);
+                               p.dt, true, d->isfirst_step, d->uniform_h, pz_d2, skin2);
       } else if (d->mixed) {
         launch_cond_mixed(d->stream, d->f, d->tl, d->n, p.T_inf, p.h_conv, p.dt, true,
                           d->isfirst_step, d->frame.cs32, d->uniform_h);
@@ -1755,11 +1784,15 @@ int wf_step(wf_domain* d, int n_steps) {
       if (d->mixed) std::swap(d->f.tk4, d->f.tk4_alt);
       else std::swap(d->f.tk64, d->f.tk64_alt);
       std::swap(d->f.T, d->f.T_alt);
-      if (fuse) {
+      if ((rc = timer_end(d, fuse ? WF_K_COND_DENS : WF_K_THERMAL))) return rc;
+      if ((rc = resolve_pz(d))) return rc;  // the conduction pass is queued
+      // whether the density half wrote the next step's density
+      if (fuse && !(pz_open && d->is_yielding && d->pz_pending)) {
+        if (d->mixed) std::swap(d->f.v4, d->f.v4_alt);
+        else std::swap(d->f.kv, d->f.kv_alt);
         std::swap(d->f.rho, d->f.rho_alt);
         dens_done = true;
       }
-      if ((rc = timer_end(d, fuse ? WF_K_COND_DENS : WF_K_THERMAL))) return rc;
     }
     if (d->cfg.mechanics && d->cfg.contact && d->n_meshes > 0) {  // SolverFraser.cu:290-318
       timer_begin(d);
@@ -1767,6 +1800,7 @@ int wf_step(wf_domain* d, int n_steps) {
       advance_mesh_centres(d, p.dt);
       if ((rc = timer_end(d, WF_K_CONTACT))) return rc;
     }
+    if ((rc = resolve_pz(d))) return rc;
     if ((rc = resolve_yield(d))) return rc;  // the step's later passes are queued
     if (d->cfg.surrogate && d->jc_newton && d->cfg.mlp_train_interval > 0) {
       // rounds are due every mlp_interval steps, taken while yielding; the

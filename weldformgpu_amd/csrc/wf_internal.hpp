@@ -269,13 +269,18 @@ void launch_accel_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int
                         double alpha, double beta, float cs32, bool uh);
 void launch_cond_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int n, double T_inf,
                        double h_conv, double dt, bool leap, bool first, float cs32, bool uh);
-// conduction of this step fused with the density pass (+ update) of the next
+// conduction of this step fused with the density pass (+ update) of the
+// next; with pz_d2 (device, FP64 bits of the plastic-zone trigger's largest
+// squared hot displacement) the density half writes nothing when
+// 4 * d2 > pz_skin2, the host's own test for a rebuild at the next step
 void launch_cond_dens_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int n,
                             int solid, double T_inf, double h_conv, double dt, bool leap,
-                            bool first, bool uh);
+                            bool first, bool uh, const unsigned long long* pz_d2,
+                            double pz_skin2);
 void launch_cond_dens_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int n,
                             int solid, double T_inf, double h_conv, double dt, bool leap,
-                            bool first, float cs32, bool uh);
+                            bool first, float cs32, bool uh, const unsigned long long* pz_d2,
+                            double pz_skin2);
 // refresh every FP32 mirror / FP64 record from the FP64 state (after uploads
 // and rebuilds)
 void launch_mirror32(hipStream_t s, const Fields& f, int n, const Frame32& fr);

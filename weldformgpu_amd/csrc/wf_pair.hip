@@ -1540,6 +1540,11 @@ struct CondDens : FusedExtra<A, CondDensNb<A, B>>, FusedPart<A, B>, FusedDrop<A,
     int limit;  // particles of the conduction pass (>= the density pass's)
     typename A::Prm a;
     typename B::Prm b;
+    // (may be null) the plastic-zone trigger's largest squared hot
+    // displacement, as FP64 bits, and skin^2: when 4 d2 > skin^2 a rebuild
+    // starts the next step, and the density half writes nothing
+    const unsigned long long* pz_d2;
+    double pz_skin2;
   };
   struct Own {
     typename A::Own a;
@@ -1579,6 +1584,7 @@ struct CondDens : FusedExtra<A, CondDensNb<A, B>>, FusedPart<A, B>, FusedDrop<A,
   __device__ static void finish(const Fields& f, int i, bool act, const Own& o, const Acc& a,
                                 const Prm& p) {
     A::finish(f, i, act, o.a, a.a, p.a);
+    if (p.pz_d2 && 4.0 * __longlong_as_double((long long)*p.pz_d2) > p.pz_skin2) return;
     B::finish(f, i, i < p.b.limit, o.b, a.b, p.b);
   }
 };
@@ -1683,12 +1689,13 @@ void launch_cond_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int 
 }
 void launch_cond_dens_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int n,
                             int solid, double T_inf, double h_conv, double dt, bool leap,
-                            bool first, bool uh) {
+                            bool first, bool uh, const unsigned long long* pz_d2,
+                            double pz_skin2) {
   if (n <= 0) return;
   auto go = [&](auto tag) {
     typedef decltype(tag) K;
     const typename K::Prm p{n, {n, T_inf, h_conv, dt, leap ? 1 : 0, first ? 1 : 0},
-                            {solid, n, dt, 1, 1}};
+                            {solid, n, dt, 1, 1}, pz_d2, pz_skin2};
     launch_pair<K>(s, ntiles(n), f, tl, p);
   };
   if (uh) go(CondDensD<true>{});
@@ -1696,12 +1703,13 @@ void launch_cond_dens_tiled(hipStream_t s, const Fields& f, const TileLists& tl,
 }
 void launch_cond_dens_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int n,
                             int solid, double T_inf, double h_conv, double dt, bool leap,
-                            bool first, float cs32, bool uh) {
+                            bool first, float cs32, bool uh, const unsigned long long* pz_d2,
+                            double pz_skin2) {
   if (n <= 0) return;
   auto go = [&](auto tag) {
     typedef decltype(tag) K;
     const typename K::Prm p{n, {n, T_inf, h_conv, dt, leap ? 1 : 0, first ? 1 : 0, cs32},
-                            {solid, n, dt, 1, cs32, 1}};
+                            {solid, n, dt, 1, cs32, 1}, pz_d2, pz_skin2};
     launch_pair<K>(s, ntiles(n), f, tl, p);
   };
   if (uh) go(CondDensG<true>{});
