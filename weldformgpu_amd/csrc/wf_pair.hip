@@ -37,6 +37,14 @@
 #ifndef WF_DROP_ACCEL64
 #define WF_DROP_ACCEL64 0
 #endif
+// cooperative record loads (coload_rec) in the FP64 acceleration, density
+// and conduction loops (759k-particle cylinder, yielding: acceleration 0.333
+// -> 0.291 ms, density 0.151 -> 0.143, conduction 0.137 -> 0.128); the FP64
+// rates loop (0.145 -> 0.149) and the FP32 acceleration loop (0.118 ->
+// 0.120) were slower with them and load per lane
+#ifndef WF_COLOAD64
+#define WF_COLOAD64 1
+#endif
 
 namespace wf {
 
@@ -93,6 +101,42 @@ __device__ __forceinline__ int xcd_tile(int b, int nt) {
 template <class T, unsigned S = sizeof(T)>
 __device__ __forceinline__ T gat(const void* base, int j, unsigned off = 0) {
   return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (unsigned)j * S + off);
+}
+
+// lanes 2p and 2p+1 swap a 32-bit word (DPP quad permutation [1, 0, 3, 2])
+__device__ __forceinline__ unsigned swap_lane_pair(unsigned v) {
+  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);
+}
+
+// Cooperative record loads for two lanes per particle: lanes 2p and 2p+1
+// each hold one neighbour j, and both load both records, lane s reading the
+// 16-byte chunks c with c % 2 == s ^ r of record r (r = 0: lane 2p's
+// neighbour, r = 1: lane 2p+1's).  A load instruction then touches one record
+// per lane pair instead of one per lane: the address unit's cost per gather
+// grows with the distinct cache lines it touches, so this halves that cost
+// for the same bytes.  Each lane keeps the half of its own record that it
+// read and swaps the other half with its partner.  C: chunks per record of
+// rec_bytes = 16 C at base.
+template <int C>
+__device__ __forceinline__ void coload_rec(const void* base, int j, int sub, uint4 (&out)[C]) {
+  static_assert(C % 2 == 0, "whole chunk pairs");
+  const int jo = (int)swap_lane_pair((unsigned)j);
+  const int j0 = sub ? jo : j, j1 = sub ? j : jo;
+  uint4 a[C / 2], b[C / 2];
+#pragma unroll
+  for (int c = 0; c < C / 2; c++) a[c] = gat<uint4, 16 * C>(base, j0, 16u * (2 * c + sub));
+#pragma unroll
+  for (int c = 0; c < C / 2; c++) b[c] = gat<uint4, 16 * C>(base, j1, 16u * (2 * c + (sub ^ 1)));
+  // lane 2p read the even chunks of its record (a) and the odd ones of its
+  // partner's (b); lane 2p+1 the odd chunks of lane 2p's record (a) and the
+  // even ones of its own (b)
+#pragma unroll
+  for (int c = 0; c < C / 2; c++) {
+    const uint4 send = sub ? a[c] : b[c];
+    out[2 * c] = sub ? b[c] : a[c];
+    out[2 * c + 1] = make_uint4(swap_lane_pair(send.x), swap_lane_pair(send.y),
+                                swap_lane_pair(send.z), swap_lane_pair(send.w));
+  }
 }
 
 // The mixed acceleration pass may integrate each particle as soon as its sum
@@ -248,6 +292,15 @@ struct DensD {
     nb.v = gat<double4>(f.kv, j);
     nb.h = UH ? 0.0 : gat<double>(f.h, j);
   }
+#if WF_COLOAD64
+  static constexpr bool CO = true;
+  __device__ static void coload(const Fields& f, int j, int sub, Nb& nb) {
+    uint4 c[2];
+    coload_rec<2>(f.kv, j, sub, c);
+    __builtin_memcpy(&nb.v, c, 32);
+    nb.h = UH ? 0.0 : gat<double>(f.h, j);
+  }
+#endif
   __device__ static void pair(double x, double y, double z, const Nb& nb, const Own& o, Acc& a,
                               const Prm&) {
     const double xx = o.x - x, xy = o.y - y, xz = o.z - z;
@@ -389,6 +442,17 @@ struct AccelD {
     nb.sb = gat<double4, 96>(f.arec64, j, 64);
     nb.h = UH ? 0.0 : gat<double>(f.h, j);
   }
+#if WF_COLOAD64
+  static constexpr bool CO = true;
+  __device__ static void coload(const Fields& f, int j, int sub, Nb& nb) {
+    uint4 c[6];
+    coload_rec<6>(f.arec64, j, sub, c);
+    __builtin_memcpy(&nb.v, &c[0], 32);
+    __builtin_memcpy(&nb.sa, &c[2], 32);
+    __builtin_memcpy(&nb.sb, &c[4], 32);
+    nb.h = UH ? 0.0 : gat<double>(f.h, j);
+  }
+#endif
   __device__ static void pair(double x, double y, double z, const Nb& nb, const Own& o, Acc& a,
                               const Prm& p) {
     const double xx = o.x - x, xy = o.y - y, xz = o.z - z;
@@ -469,6 +533,15 @@ struct CondD {
     nb.tk = gat<double4>(f.tk64, j);
     nb.h = UH ? 0.0 : gat<double>(f.h, j);
   }
+#if WF_COLOAD64
+  static constexpr bool CO = true;
+  __device__ static void coload(const Fields& f, int j, int sub, Nb& nb) {
+    uint4 c[2];
+    coload_rec<2>(f.tk64, j, sub, c);
+    __builtin_memcpy(&nb.tk, c, 32);
+    nb.h = UH ? 0.0 : gat<double>(f.h, j);
+  }
+#endif
   __device__ static void pair(double x, double y, double z, const Nb& nb, const Own& o, Acc& a,
                               const Prm&) {
     const double xx = o.x - x, xy = o.y - y, xz = o.z - z;
@@ -700,6 +773,18 @@ __device__ __forceinline__ T lane_value_xor(const T& v, int m) {
   return r;
 }
 
+// A pair functor may load its neighbour records cooperatively (static
+// constexpr bool CO = true, coload(f, j, sub, nb)); with two lanes per
+// particle both lanes then run the same iterations of the list walk.
+template <class K, class = void>
+struct CoOf {
+  static constexpr bool on = false;
+};
+template <class K>
+struct CoOf<K, std::void_t<decltype(K::CO)>> {
+  static constexpr bool on = K::CO;
+};
+
 // One workgroup per tile, L = K::LANES consecutive lanes per particle: lanes
 // L p .. L p + L - 1 all hold particle p and deal its slot list out by quads
 // (lane L p + h takes the quads q with q % L == h); their sums are merged
@@ -792,12 +877,16 @@ void k_pair_gather(Fields f, TileLists tl, typename K::Prm prm) {
     };
     constexpr int KS = 4 * L;  // slots from one of the lane's quads to its next
     const int k0 = 4 * sub;
+    // with cooperative loads both lanes of a particle take the same number
+    // of steps (a lane past its list loads for its partner and adds nothing)
+    constexpr bool CO = CoOf<K>::on && L == 2;
+    const int kend = CO ? (cnt + KS - 1) / KS * KS + k0 : cnt;
     uint2 w0 = word(k0);
     uint2 w1 = word(k0 + KS);
     int jn[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) jn[u] = k0 + u < cnt ? st[sl_of(w0, u)].j : 0;
-    for (int k = k0; k < cnt; k += KS) {
+    for (int k = k0; k < kend; k += KS) {
       const int nk = min(4, cnt - k);
       int jc[4];
 #pragma unroll
@@ -816,8 +905,10 @@ void k_pair_gather(Fields f, TileLists tl, typename K::Prm prm) {
         // acceleration loop is bound by its gathers)
         typename K::Nb nb[NB];
 #pragma unroll
-        for (int u = 0; u < NB; u++)
-          if (DropOf<K>::on || u0 + u == 0 || u0 + u < nk) K::load(f, jc[u0 + u], nb[u]);
+        for (int u = 0; u < NB; u++) {
+          if constexpr (CO) K::coload(f, jc[u0 + u], sub, nb[u]);
+          else if (DropOf<K>::on || u0 + u == 0 || u0 + u < nk) K::load(f, jc[u0 + u], nb[u]);
+        }
         // the batch's staged positions are read before any pair is evaluated
         // (entries past the list read slot 0 and are not evaluated)
         typename K::PT px[NB], py[NB], pz[NB];
@@ -838,8 +929,9 @@ void k_pair_gather(Fields f, TileLists tl, typename K::Prm prm) {
               if constexpr (XS) K::put(nb[u], sx[sl[u]]);
               if (u0 + u >= nk) K::drop(nb[u]);
               K::pair(px[u], py[u], pz[u], nb[u], own, sum, prm);
-            } else if (u0 + u == 0 || u0 + u < nk) {
-              // (the quad's first entry is always in the list: k < cnt)
+            } else if ((!CO && u0 + u == 0) || u0 + u < nk) {
+              // (the quad's first entry is in the list unless the lane only
+              // loads for its partner: k < cnt)
               if constexpr (XS) K::put(nb[u], sx[sl[u]]);
               K::pair(px[u], py[u], pz[u], nb[u], own, sum, prm);
             }
@@ -1160,6 +1252,7 @@ struct AccelG {
     nb.sb = gat<float4, 32>(f.arec, j, 16);
     nb.h = UH ? 0.f : gat<float>(f.h32, j);
   }
+
   __device__ static void pair(float x, float y, float z, const Nb& nb, const Own& o, Part& a,
                               const Prm& p) {
     const float xx = o.x - x, xy = o.y - y, xz = o.z - z;
@@ -1569,6 +1662,13 @@ struct CondDens : FusedExtra<A, CondDensNb<A, B>>, FusedPart<A, B>, FusedDrop<A,
   __device__ static void load(const Fields& f, int j, Nb& nb) {
     A::load(f, j, nb.a);
     B::load(f, j, nb.b);
+  }
+  static constexpr bool CO = CoOf<A>::on && CoOf<B>::on;
+  __device__ static void coload(const Fields& f, int j, int sub, Nb& nb) {
+    if constexpr (CO) {
+      A::coload(f, j, sub, nb.a);
+      B::coload(f, j, sub, nb.b);
+    }
   }
   // S: the sums (Acc) or the batch's partial sums (Part)
   template <class S>
