@@ -1094,19 +1094,14 @@ struct DensG {
 // a_k x_l: the diagonal takes a_x x_x, a_y x_y, a_z x_z, each off-diagonal
 // pair (k, l) the sum (strain) and the difference (rotation) of a_k x_l and
 // a_l x_k.  The loop sums the nine products (one FMA each) and forms the
-// sums and differences once per particle.
-// the rates loop widens each of its nine products to FP64 (flush after every
-// pair): summing four pairs in FP32 first is faster (0.061 vs 0.078 ms on a
-// 759k-particle cylinder) but doubles the plastic-strain and temperature
-// error of a 3,000-step Johnson-Cook run against FP64 (5.6e-5 vs 2.8e-5)
-#ifndef WF_RATES_FLUSH_EACH
-#define WF_RATES_FLUSH_EACH 1
-#endif
+// sums and differences once per particle.  Summing four pairs' products in
+// FP32 before adding them to FP64 was as fast but doubled the plastic-strain
+// and temperature error of a 3,000-step Johnson-Cook run against FP64
+// (5.6e-5 vs 2.8e-5).
 template <bool UH>
 struct RatesG {
   WF_GATHER32
   static constexpr int NBATCH = 4;
-  static constexpr bool FLUSH_EACH = WF_RATES_FLUSH_EACH;
   struct Prm {
     int limit;
     double xz_sign;
@@ -1117,33 +1112,6 @@ struct RatesG {
 #if WF_DROP_RATES32
   __device__ static void drop(Nb& nb) { DensG<UH>::drop(nb); }
 #endif
-  // products a_k x_l: xx, yy, zz, then (x y, y x), (y z, z y), (x z, z x)
-  struct Acc {
-    KSum xx, yy, zz, xy, yx, yz, zy, xz, zx;
-  };
-  struct Part {
-    float xx, yy, zz, xy, yx, yz, zy, xz, zx;
-  };
-  __device__ static void pzero(Part& q) { q = Part{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f}; }
-  __device__ static void flush(Acc& a, const Part& q) {
-    a.xx.add(q.xx); a.yy.add(q.yy); a.zz.add(q.zz);
-    a.xy.add(q.xy); a.yx.add(q.yx); a.yz.add(q.yz);
-    a.zy.add(q.zy); a.xz.add(q.xz); a.zx.add(q.zx);
-  }
-  __device__ static void merge(Acc& a, const Acc& b) {
-    a.xx.add(b.xx); a.yy.add(b.yy); a.zz.add(b.zz);
-    a.xy.add(b.xy); a.yx.add(b.yx); a.yz.add(b.yz);
-    a.zy.add(b.zy); a.xz.add(b.xz); a.zx.add(b.zx);
-  }
-  __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx32& c) {
-    a.xx.zero(); a.yy.zero(); a.zz.zero(); a.xy.zero(); a.yx.zero(); a.yz.zero();
-    a.zy.zero(); a.xz.zero(); a.zx.zero();
-    if (!act) return;
-    pos32(f, i, c, o.x, o.y, o.z);
-    const float4 v = f.v4[i];
-    o.vx = v.x; o.vy = v.y; o.vz = v.z;
-    o.hc = hconst_f(f.h32[i]);
-  }
   // the neighbour's {v, m/rho} is staged in LDS with its position
   typedef float4 Extra;
   __device__ static Extra extra(const Fields& f, int j) { return gat<float4>(f.v4, j); }
@@ -1151,31 +1119,55 @@ struct RatesG {
   __device__ static void load(const Fields& f, int j, Nb& nb) {
     nb.h = UH ? 0.f : gat<float>(f.h32, j);
   }
-  __device__ static void pair(float x, float y, float z, const Nb& nb, const Own& o, Part& a,
+  // products a_k x_l: xx, yy, zz, then (x y, y x), (y z, z y), (x z, z x)
+  // each product of two FP32 factors is exact in FP64 and added by one FP64
+  // FMA: 6 conversions and 9 FMAs a pair.  Widening each FP32 product and
+  // adding it costs 27 operations, and this loop is bound by its vector ALU:
+  // on a 759k-particle cylinder 0.082 -> 0.066 ms, with the same error
+  // against the FP64 rates (4.7e-7 relative L2; the FP32 factors set it)
+  struct Acc {
+    double xx, yy, zz, xy, yx, yz, zy, xz, zx;
+  };
+  __device__ static void merge(Acc& a, const Acc& b) {
+    a.xx += b.xx; a.yy += b.yy; a.zz += b.zz;
+    a.xy += b.xy; a.yx += b.yx; a.yz += b.yz;
+    a.zy += b.zy; a.xz += b.xz; a.zx += b.zx;
+  }
+  __device__ static void init(const Fields& f, int i, bool act, Own& o, Acc& a, const Ctx32& c) {
+    a = Acc{0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (!act) return;
+    pos32(f, i, c, o.x, o.y, o.z);
+    const float4 v = f.v4[i];
+    o.vx = v.x; o.vy = v.y; o.vz = v.z;
+    o.hc = hconst_f(f.h32[i]);
+  }
+  __device__ static void pair(float x, float y, float z, const Nb& nb, const Own& o, Acc& a,
                               const Prm&) {
     const float xx = o.x - x, xy = o.y - y, xz = o.z - z;
     const float r2 = xx * xx + xy * xy + xz * xz;
     float h_;
     const float w = gk_pair_f<UH>(r2, nb.h, o.hc, h_) * nb.v.w;
     const float vx = o.vx - nb.v.x, vy = o.vy - nb.v.y, vz = o.vz - nb.v.z;
-    const float ax = w * vx, ay = w * vy, az = w * vz;
-    a.xx = __builtin_fmaf(ax, xx, a.xx);
-    a.yy = __builtin_fmaf(ay, xy, a.yy);
-    a.zz = __builtin_fmaf(az, xz, a.zz);
-    a.xy = __builtin_fmaf(ax, xy, a.xy);
-    a.yx = __builtin_fmaf(ay, xx, a.yx);
-    a.yz = __builtin_fmaf(ay, xz, a.yz);
-    a.zy = __builtin_fmaf(az, xy, a.zy);
-    a.xz = __builtin_fmaf(ax, xz, a.xz);
-    a.zx = __builtin_fmaf(az, xx, a.zx);
+    const double ax = w * vx, ay = w * vy, az = w * vz;
+    const double dx = xx, dy = xy, dz = xz;
+    a.xx = __builtin_fma(ax, dx, a.xx);
+    a.yy = __builtin_fma(ay, dy, a.yy);
+    a.zz = __builtin_fma(az, dz, a.zz);
+    a.xy = __builtin_fma(ax, dy, a.xy);
+    a.yx = __builtin_fma(ay, dx, a.yx);
+    a.yz = __builtin_fma(ay, dz, a.yz);
+    a.zy = __builtin_fma(az, dy, a.zy);
+    a.xz = __builtin_fma(ax, dz, a.xz);
+    a.zx = __builtin_fma(az, dx, a.zx);
   }
+  __device__ static double val(double v) { return v; }
   __device__ static void finish(const Fields& f, int i, bool act, const Own&, const Acc& a,
                                 const Prm& p) {
     if (!act) return;
-    const double xy = a.xy.val(), yx = a.yx.val(), yz = a.yz.val(), zy = a.zy.val();
-    const double xz = a.xz.val(), zx = a.zx.val();
-    f.strrate32[0][i] = (float)-a.xx.val(); f.strrate32[1][i] = (float)-a.yy.val();
-    f.strrate32[2][i] = (float)-a.zz.val();
+    const double xy = val(a.xy), yx = val(a.yx), yz = val(a.yz), zy = val(a.zy);
+    const double xz = val(a.xz), zx = val(a.zx);
+    f.strrate32[0][i] = (float)-val(a.xx); f.strrate32[1][i] = (float)-val(a.yy);
+    f.strrate32[2][i] = (float)-val(a.zz);
     f.strrate32[3][i] = (float)(-0.5 * (xy + yx));
     f.strrate32[4][i] = (float)(-0.5 * (yz + zy));
     f.strrate32[5][i] = (float)(-0.5 * (xz + zx));
