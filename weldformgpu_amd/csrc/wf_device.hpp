@@ -58,23 +58,44 @@ __device__ __forceinline__ void unpack_cell(int c, int& cx, int& cy, int& cz) {
 }
 
 // ---- integration of one particle -------------------------------------------
+// The state of particle i that integrate_particle reads (integrate_load);
+// the acceleration pass that integrates its own particles loads it before
+// its pair loop, so that the loads are in flight while the pairs are summed.
+struct IntegrateIn {
+  double x[3], v[3];
+  int id;
+  float cell, vw;  // the FP32 mirrors' kept cell and m/rho (kRec32)
+};
+template <int REC>
+__device__ __forceinline__ void integrate_load(const Fields& f, int i, IntegrateIn& in) {
+  in.id = f.ID[i];
+  for (int c = 0; c < 3; c++) {
+    in.x[c] = f.x[c][i];
+    in.v[c] = f.v[c][i];
+  }
+  if (REC == kRec32) {
+    in.cell = f.p4[i].w;
+    in.vw = f.v4[i].w;
+  }
+}
+
 // BC, position and velocity update of particle i from its acceleration a:
 // ApplyBCVelKernel (Mechanical.cu:572-589) before and after
 // UpdatePosFraserKernel (:179-191, u = v dt + a dt^2/2) and UpdateVelKernel
-// (:142-152), plus the record the next passes read (REC)
+// (:142-152), plus the record the next passes read (REC), from the state in
 template <int REC>
-__device__ __forceinline__ void integrate_particle(const Fields& f, int i, double a0, double a1,
-                                                   double a2, double dt, const BcTable& bc,
-                                                   const Frame32& fr, const IntegrateOut& o) {
+__device__ __forceinline__ void integrate_apply(const Fields& f, int i, double a0, double a1,
+                                                double a2, double dt, const BcTable& bc,
+                                                const Frame32& fr, const IntegrateOut& o,
+                                                const IntegrateIn& in) {
   // no FMA contraction: the separate integration pass and the acceleration
   // pass that integrates its own particles round identically (and as the
   // FP64 oracle does)
 #pragma clang fp contract(off)
-  const int id = f.ID[i];
-  double v0 = f.v[0][i], v1 = f.v[1][i], v2 = f.v[2][i];
+  double v0 = in.v[0], v1 = in.v[1], v2 = in.v[2];
   int hit = -1;
   for (int b = 0; b < bc.count; b++)
-    if (id == bc.zone[b]) hit = b;
+    if (in.id == bc.zone[b]) hit = b;
   if (hit >= 0) {
     a0 = a1 = a2 = 0.0;
     v0 = bc.val[hit][0];
@@ -84,7 +105,7 @@ __device__ __forceinline__ void integrate_particle(const Fields& f, int i, doubl
   const double u0 = v0 * dt + ((a0 * 0.5) * dt) * dt;
   const double u1 = v1 * dt + ((a1 * 0.5) * dt) * dt;
   const double u2 = v2 * dt + ((a2 * 0.5) * dt) * dt;
-  const double x0 = f.x[0][i] + u0, x1 = f.x[1][i] + u1, x2 = f.x[2][i] + u2;
+  const double x0 = in.x[0] + u0, x1 = in.x[1] + u1, x2 = in.x[2] + u2;
   o.x[0][i] = x0;
   o.x[1][i] = x1;
   o.x[2][i] = x2;
@@ -112,13 +133,21 @@ __device__ __forceinline__ void integrate_particle(const Fields& f, int i, doubl
   if (REC == kRec64) f.kv[i] = make_double4(v0, v1, v2, f.kv[i].w);
   if (REC == kRec32) {
     int cx, cy, cz;
-    const float cb = f.p4[i].w;  // the cell kept since the last refresh
-    unpack_cell(__float_as_int(cb), cx, cy, cz);
+    unpack_cell(__float_as_int(in.cell), cx, cy, cz);  // the cell kept since the last refresh
     o.p4[i] = make_float4((float)(x0 - (fr.org[0] + cx * fr.cs)),
                           (float)(x1 - (fr.org[1] + cy * fr.cs)),
-                          (float)(x2 - (fr.org[2] + cz * fr.cs)), cb);
-    o.v4[i] = make_float4((float)v0, (float)v1, (float)v2, f.v4[i].w);
+                          (float)(x2 - (fr.org[2] + cz * fr.cs)), in.cell);
+    o.v4[i] = make_float4((float)v0, (float)v1, (float)v2, in.vw);
   }
+}
+
+template <int REC>
+__device__ __forceinline__ void integrate_particle(const Fields& f, int i, double a0, double a1,
+                                                   double a2, double dt, const BcTable& bc,
+                                                   const Frame32& fr, const IntegrateOut& o) {
+  IntegrateIn in;
+  integrate_load<REC>(f, i, in);
+  integrate_apply<REC>(f, i, a0, a1, a2, dt, bc, fr, o, in);
 }
 
 // ---- wavefront (64-lane) reductions ---------------------------------------

@@ -140,11 +140,10 @@ __device__ __forceinline__ void coload_rec(const void* base, int j, int sub, uin
 }
 
 // The mixed acceleration pass may integrate each particle as soon as its sum
-// is done (the step's BC + position/velocity update, integrate_particle):
+// is done (the step's BC + position/velocity update, integrate_apply):
 // other tiles still read the old FP32 mirrors, so the new ones go to the
 // spare planes in out, swapped in by the caller after the pass.
 struct IntegratePrm {
-  int on;
   double dt;
   BcTable bc;
   Frame32 fr;
@@ -1184,8 +1183,10 @@ struct RatesG {
 #ifndef WF_ACCEL32_WAVES
 #define WF_ACCEL32_WAVES 1
 #endif
-// BV: the beta (quadratic) viscosity term is on
-template <bool UH, bool BV = false>
+// BV: the beta (quadratic) viscosity term is on; IT: each particle is
+// integrated as soon as its sum is done (IntegratePrm), its integration
+// inputs loaded before the pair loop
+template <bool UH, bool BV = false, bool IT = false>
 struct AccelG {
   WF_GATHER32
   static constexpr int NBATCH = WF_ACCEL32_NBATCH;
@@ -1194,11 +1195,13 @@ struct AccelG {
     int limit;
     float alpha32, beta32;
     float cs;
-    IntegratePrm ip;  // integrate each particle after its sum (ip.on)
+    IntegratePrm ip;  // integrate each particle after its sum (IT)
   };
+  struct NoIn {};
   struct Own {
     float x, y, z, vx, vy, vz, di, Ci, s0, s1, s2, s3, s4, s5;
     HConstF hc;
+    std::conditional_t<IT, IntegrateIn, NoIn> in;
   };
   struct Nb {
     float4 v, sa, sb;  // {v, m}, sigma/rho^2 {xx, yy, zz, xy}, {yz, xz, rho, Cs}
@@ -1228,6 +1231,7 @@ struct AccelG {
     o.s0 = sa.x; o.s1 = sa.y; o.s2 = sa.z; o.s3 = sa.w;
     o.s4 = sb.x; o.s5 = sb.y; o.di = sb.z; o.Ci = sb.w;
     o.hc = hconst_f(f.h32[i]);
+    if constexpr (IT) integrate_load<kRec32>(f, i, o.in);
   }
   // the neighbour's {v, m} (m = (m/rho) rho, formed once per staged
   // particle) is staged in LDS with its position, so a pair gathers only the
@@ -1278,12 +1282,12 @@ struct AccelG {
     a.ay = __builtin_fmaf(myz, gz, __builtin_fmaf(myy, gy, __builtin_fmaf(mxy, gx, a.ay)));
     a.az = __builtin_fmaf(mzz, gz, __builtin_fmaf(myz, gy, __builtin_fmaf(mxz, gx, a.az)));
   }
-  __device__ static void finish(const Fields& f, int i, bool act, const Own&, const Acc& a,
+  __device__ static void finish(const Fields& f, int i, bool act, const Own& o, const Acc& a,
                                 const Prm& p) {
     if (!act) return;
-    if (p.ip.on) {
-      integrate_particle<kRec32>(f, i, a.ax.val(), a.ay.val(), a.az.val(), p.ip.dt, p.ip.bc,
-                                 p.ip.fr, p.ip.out);
+    if constexpr (IT) {
+      integrate_apply<kRec32>(f, i, a.ax.val(), a.ay.val(), a.az.val(), p.ip.dt, p.ip.bc,
+                              p.ip.fr, p.ip.out, o.in);
       return;
     }
     f.a[0][i] = a.ax.val();
@@ -1739,14 +1743,14 @@ void launch_accel_integrate_mixed(hipStream_t s, const Fields& f, const TileList
                                   double alpha, double beta, float cs32, bool uh, double dt,
                                   const BcTable& bc, const Frame32& fr, const IntegrateOut& out) {
   if (solid <= 0) return;
-  const IntegratePrm ip{1, dt, bc, fr, out};
+  const IntegratePrm ip{dt, bc, fr, out};
   auto go = [&](auto tag) {
     typedef decltype(tag) K;
     launch_pair<K>(s, ntiles(solid), f, tl,
                    typename K::Prm{solid, (float)alpha, (float)beta, cs32, ip});
   };
-  if (beta != 0.0) uh ? go(AccelG<true, true>{}) : go(AccelG<false, true>{});
-  else uh ? go(AccelG<true, false>{}) : go(AccelG<false, false>{});
+  if (beta != 0.0) uh ? go(AccelG<true, true, true>{}) : go(AccelG<false, true, true>{});
+  else uh ? go(AccelG<true, false, true>{}) : go(AccelG<false, false, true>{});
 }
 void launch_cond_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int n, double T_inf,
                        double h_conv, double dt, bool leap, bool first, bool uh) {
