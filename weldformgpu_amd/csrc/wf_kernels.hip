@@ -54,9 +54,11 @@ __device__ __forceinline__ T9 tmul(const T9& a, const T9& b, bool quirk) {
 // Safeguarded Newton for the Johnson-Cook radial return,
 // r(g) = s_trial - 3G g - (A + B (ep+g)^n) rt = 0 on [0, hi], from g0, with
 // rt = rate factor x thermal factor and hi = (s_trial - s_y(ep)) / (3G).
+// On convergence *xn holds (ep+g)^n of the returned g (the final yield
+// stress needs it), else it is left alone.
 __device__ __forceinline__ double jc_newton(const MatParams& mp, double s_trial, double G,
                                             double ep, double rt, double hi, double g0,
-                                            double tol, int max_iter, int* iters) {
+                                            double tol, int max_iter, int* iters, double* xn_out) {
   double lo = 0.0;
   if (!(hi > 0.0)) {
     *iters = 0;
@@ -70,7 +72,10 @@ __device__ __forceinline__ double jc_newton(const MatParams& mp, double s_trial,
     const double x = ep + g;
     const double xn = pow(x, mp.n);
     const double r = s_trial - 3.0 * G * g - (mp.A + mp.B * xn) * rt;
-    if (fabs(r) <= tol * s_trial) break;
+    if (fabs(r) <= tol * s_trial) {
+      *xn_out = xn;
+      break;
+    }
     if (r > 0.0) lo = g;
     else hi = g;
     const double H = x > 0.0 ? mp.n * mp.B * (xn / x) * rt : HUGE_VAL;
@@ -216,8 +221,9 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Block
     }
     if (newton) {
       int it = 0;
-      const double g =
-          jc_newton(mp, sig_trial, G, ep, rt, gmax, g0, p.newton_tol, p.newton_max_iter, &it);
+      double xn = -1.0;  // (ep+g)^n >= 0 once the Newton has converged
+      const double g = jc_newton(mp, sig_trial, G, ep, rt, gmax, g0, p.newton_tol,
+                                 p.newton_max_iter, &it, &xn);
       nit = (unsigned long long)it;
       ncall = 1ull;
       scale = (sig_trial - 3.0 * G * g) / sig_trial;
@@ -230,8 +236,8 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Block
       f.mlx[3][i] = f3;
       f.mly[i] = gmax > 0.0 ? (float)(g / gmax) : 0.f;
       f.mlstep[i] = (int)ra.step;
-      ep += g;
-      sigy = (mp.A + mp.B * pow(ep, mp.n)) * rt;
+      ep += g;  // the same rounding as the Newton's ep + g
+      sigy = (mp.A + mp.B * (xn >= 0.0 ? xn : pow(ep, mp.n))) * rt;
     }
   }
   if (active) {
