@@ -23,11 +23,6 @@
 #include "wf_device.hpp"
 #include "wf_tiles.hpp"
 
-// FP32 partial sums added to the FP64 sums after every pair instead of
-// every batch (1: the per-term widening of the pair terms)
-#ifndef WF_PART_PER_PAIR
-#define WF_PART_PER_PAIR 0
-#endif
 // straight-line batches (drop(), below) in the FP32 rates and FP64
 // acceleration loops: off, measured slower there (759k-particle cylinder:
 // rates 0.072 vs 0.059 ms, FP64 acceleration 0.346 vs 0.321 ms)
@@ -736,17 +731,6 @@ struct DropOf<K, std::void_t<decltype(K::drop(std::declval<typename K::Nb&>()))>
   static constexpr bool on = true;
 };
 
-// FLUSH_EACH: the partial sum goes to the FP64 sums after every pair (the
-// per-term widening), not after every batch
-template <class K, class = void>
-struct FlushEachOf {
-  static constexpr bool value = WF_PART_PER_PAIR;
-};
-template <class K>
-struct FlushEachOf<K, std::void_t<decltype(K::FLUSH_EACH)>> {
-  static constexpr bool value = K::FLUSH_EACH;
-};
-
 // waves per SIMD a pair functor asks the register allocator to leave room
 // for (MIN_WAVES; 1 = no request)
 template <class K, class = void>
@@ -933,10 +917,6 @@ void k_pair_gather(Fields f, TileLists tl, typename K::Prm prm) {
               // loads for its partner: k < cnt)
               if constexpr (XS) K::put(nb[u], sx[sl[u]]);
               K::pair(px[u], py[u], pz[u], nb[u], own, sum, prm);
-            }
-            if constexpr (PartOf<K>::on && FlushEachOf<K>::value) {
-              K::flush(acc, sum);
-              K::pzero(sum);
             }
           }
         };
