@@ -230,8 +230,9 @@ def main():
                          "FETCH_SIZE and WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path is "
-              "single-GPU in this build)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the step "
+              "runs on a single GPU; sharding by space is designed, not built)",
+              file=sys.stderr)
         sys.exit(2)
 
     from weldformgpu_amd import _abi, scenarios
