@@ -171,4 +171,47 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
 }
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
+// Fold a stress pass's per-block statistics into the run counters with the
+// whole workgroup (blockDim.x a multiple of 64, at most 1,024): four entries
+// in flight per thread, then wavefront and block reductions.  sh: LDS for
+// 4 x blockDim.x / 64 values, free for the call.  Integer max and sums, so
+// the result does not depend on the order.
+__device__ __forceinline__ void block_fold_stats(const StatsFold& fd, unsigned long long* sh) {
+  const int nt = blockDim.x, nw = nt >> 6;
+  unsigned long long m = 0ull, it = 0ull, ca = 0ull, wi = 0ull;
+#pragma unroll 4
+  for (int b = threadIdx.x; b < fd.nb; b += nt) {
+    const BlockStats x = fd.bs[b];
+    m = x.ep_bits > m ? x.ep_bits : m;
+    it += x.iters;
+    ca += x.calls;
+    wi += x.wave_iters;
+  }
+  m = wave_max_u64(m);
+  it = wave_sum_u64(it);
+  ca = wave_sum_u64(ca);
+  wi = wave_sum_u64(wi);
+  const int w = threadIdx.x >> 6;
+  if (lane_id() == 0) {
+    sh[w] = m;
+    sh[nw + w] = it;
+    sh[2 * nw + w] = ca;
+    sh[3 * nw + w] = wi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int u = 1; u < nw; u++) {
+      m = sh[u] > m ? sh[u] : m;
+      it += sh[nw + u];
+      ca += sh[2 * nw + u];
+      wi += sh[3 * nw + u];
+    }
+    Counters* c = fd.ctr;
+    if (m > c->max_pl_strain_bits) c->max_pl_strain_bits = m;
+    c->newton_iters += it;
+    c->newton_calls += ca;
+    c->newton_iters_wave += wi;
+  }
+}
+
 }  // namespace wf

@@ -1693,7 +1693,8 @@ int wf_step(wf_domain* d, int n_steps) {
       const int rec = mx ? kRec32 : kRec64;
       launch_stress(d->stream, d->f, p, d->bstats, true, true, rec, mx, d->ret_args());
       if ((rc = timer_end(d, WF_K_STRESS))) return rc;
-      launch_fold_stats(d->stream, d->bstats, (p.solid + kBlock - 1) / kBlock, d->ctr);
+      // the acceleration pass's first workgroup folds the stress statistics
+      const StatsFold fold{d->bstats, (p.solid + kBlock - 1) / kBlock, d->ctr};
       // Without contact forces or the axisymmetric clamp between them, the
       // mixed acceleration pass integrates each particle as soon as its sum
       // is done (the reference's BC + UpdatePosFraser + UpdateVel + BC,
@@ -1711,13 +1712,14 @@ int wf_step(wf_domain* d, int n_steps) {
         out.p4 = d->p4_spare;
         out.v4 = d->f.v4_alt;
         launch_accel_integrate_mixed(d->stream, d->f, d->tl, d->solid, p.alpha, p.beta, cs32, uh,
-                                     d->cfg.dt, d->bcs, d->frame, out);
+                                     d->cfg.dt, d->bcs, d->frame, out, fold);
         std::swap(d->f.p4, d->p4_spare);
         std::swap(d->f.v4, d->f.v4_alt);
         if ((rc = timer_end(d, WF_K_ACCEL_INTEGRATE))) return rc;
       } else {
-        if (mx) launch_accel_mixed(d->stream, d->f, d->tl, d->solid, p.alpha, p.beta, cs32, uh);
-        else launch_accel_tiled(d->stream, d->f, d->tl, d->solid, p.alpha, p.beta, uh);
+        if (mx)
+          launch_accel_mixed(d->stream, d->f, d->tl, d->solid, p.alpha, p.beta, cs32, uh, fold);
+        else launch_accel_tiled(d->stream, d->f, d->tl, d->solid, p.alpha, p.beta, uh, fold);
         if (d->cfg.axisymm) launch_axisymm_bc(d->stream, d->f, d->n);  // SolverFraser.cu:229-231
         if ((rc = timer_end(d, WF_K_ACCEL))) return rc;
       }

@@ -162,6 +162,13 @@ struct RetArgs {
 struct BlockStats {
   unsigned long long ep_bits, iters, calls, wave_iters;
 };
+// the fold of a stress pass's nb BlockStats into ctr, done by the first
+// workgroup of the acceleration pass that follows it (nb = 0: none)
+struct StatsFold {
+  const BlockStats* bs;
+  int nb;
+  Counters* ctr;
+};
 
 constexpr int kMlpParams = 1249;  // 4x32 + 32 + 32x32 + 32 + 32 + 1
 constexpr int kMlpMaxBlocks = 256;  // gradient blocks (one 64-sample chunk each)
@@ -247,7 +254,7 @@ void launch_dens_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int 
 void launch_rates_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
                         int quirks, bool uh);
 void launch_accel_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
-                        double alpha, double beta, bool uh);
+                        double alpha, double beta, bool uh, const StatsFold& fold = {});
 void launch_cond_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int n, double T_inf,
                        double h_conv, double dt, bool leap, bool first, bool uh);
 
@@ -257,7 +264,8 @@ void launch_cond_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int 
 // FP32 mirrors go to out's planes (IntegrateOut, above)
 void launch_accel_integrate_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
                                   double alpha, double beta, float cs32, bool uh, double dt,
-                                  const BcTable& bc, const Frame32& fr, const IntegrateOut& out);
+                                  const BcTable& bc, const Frame32& fr, const IntegrateOut& out,
+                                  const StatsFold& fold = {});
 
 // mixed precision (FP32 pair loops, FP32 terms summed in FP64)
 // uh: every particle has the same h
@@ -266,7 +274,7 @@ void launch_dens_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int 
 void launch_rates_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
                         int quirks, float cs32, bool uh);
 void launch_accel_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
-                        double alpha, double beta, float cs32, bool uh);
+                        double alpha, double beta, float cs32, bool uh, const StatsFold& fold = {});
 void launch_cond_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int n, double T_inf,
                        double h_conv, double dt, bool leap, bool first, float cs32, bool uh);
 // conduction of this step fused with the density pass (+ update) of the

@@ -331,47 +331,15 @@ __global__ __launch_bounds__(kBlock) void k_hot_disp(Fields f, const double* __r
   if (lane_id() == 0 && b) atomicMax(d2_bits, b);
 }
 
-// one block folds the per-block statistics into the run counters: 1,024
-// threads, four entries in flight each, then wavefront and block reductions
-// (one 256-thread block reading one entry at a time took 20 us for the 10.7k
-// blocks of a 2.7M-particle step)
+// one block folds the per-block statistics into the run counters (used
+// where no acceleration pass follows the stress pass to do it); one
+// 256-thread block reading one entry at a time took 20 us for the 10.7k
+// blocks of a 2.7M-particle step
 constexpr int kFoldThreads = 1024;
 __global__ __launch_bounds__(kFoldThreads) void k_fold_stats(const BlockStats* bs, int nb,
                                                              Counters* ctr) {
-  __shared__ unsigned long long sh[4][kFoldThreads / 64];
-  unsigned long long m = 0ull, it = 0ull, ca = 0ull, wi = 0ull;
-#pragma unroll 4
-  for (int b = threadIdx.x; b < nb; b += kFoldThreads) {
-    const BlockStats x = bs[b];
-    m = x.ep_bits > m ? x.ep_bits : m;
-    it += x.iters;
-    ca += x.calls;
-    wi += x.wave_iters;
-  }
-  m = wave_max_u64(m);
-  it = wave_sum_u64(it);
-  ca = wave_sum_u64(ca);
-  wi = wave_sum_u64(wi);
-  const int w = threadIdx.x >> 6;
-  if (lane_id() == 0) {
-    sh[0][w] = m;
-    sh[1][w] = it;
-    sh[2][w] = ca;
-    sh[3][w] = wi;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int u = 1; u < kFoldThreads / 64; u++) {
-      m = sh[0][u] > m ? sh[0][u] : m;
-      it += sh[1][u];
-      ca += sh[2][u];
-      wi += sh[3][u];
-    }
-    if (m > ctr->max_pl_strain_bits) ctr->max_pl_strain_bits = m;
-    ctr->newton_iters += it;
-    ctr->newton_calls += ca;
-    ctr->newton_iters_wave += wi;
-  }
+  __shared__ unsigned long long sh[4 * kFoldThreads / 64];
+  block_fold_stats(StatsFold{bs, nb, ctr}, sh);
 }
 
 // ---------------------------------------------------------------------------

@@ -401,6 +401,7 @@ struct AccelD {
   struct Prm {
     int limit;
     double alpha, beta;
+    StatsFold fold;  // the stress pass's statistics, folded by workgroup 0
   };
   struct Own {
     double x, y, z, vx, vy, vz, di, Ci, s0, s1, s2, s3, s4, s5;
@@ -768,6 +769,18 @@ struct CoOf<K, std::void_t<decltype(K::CO)>> {
   static constexpr bool on = K::CO;
 };
 
+// A pass whose Prm has a StatsFold (fold) folds the preceding stress pass's
+// block statistics in its first workgroup, after that workgroup's tile, so
+// that the fold overlaps the other workgroups instead of taking a launch.
+template <class K, class = void>
+struct FoldOf {
+  static constexpr bool on = false;
+};
+template <class K>
+struct FoldOf<K, std::void_t<decltype(std::declval<typename K::Prm&>().fold)>> {
+  static constexpr bool on = true;
+};
+
 // One workgroup per tile, L = K::LANES consecutive lanes per particle: lanes
 // L p .. L p + L - 1 all hold particle p and deal its slot list out by quads
 // (lane L p + h takes the quads q with q % L == h); their sums are merged
@@ -995,6 +1008,12 @@ void k_pair_gather(Fields f, TileLists tl, typename K::Prm prm) {
     }
   }
   if (sub == 0) K::finish(f, i, act, own, acc, prm);
+  if constexpr (FoldOf<K>::on) {
+    if (blockIdx.x == 0 && prm.fold.nb > 0) {
+      __syncthreads();  // the staged set is read no more: its LDS is scratch
+      block_fold_stats(prm.fold, reinterpret_cast<unsigned long long*>(st));
+    }
+  }
 }
 
 // A: density rate (InteractionAlt.cu:166-246), fused density update writes
@@ -1176,6 +1195,7 @@ struct AccelG {
     float alpha32, beta32;
     float cs;
     IntegratePrm ip;  // integrate each particle after its sum (IT)
+    StatsFold fold;   // the stress pass's statistics, folded by workgroup 0
   };
   struct NoIn {};
   struct Own {
@@ -1711,23 +1731,24 @@ void launch_rates_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int
   launch_gather<RatesD>(s, f, tl, ntiles(solid), uh, solid, xz);
 }
 void launch_accel_tiled(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
-                        double alpha, double beta, bool uh) {
+                        double alpha, double beta, bool uh, const StatsFold& fold) {
   if (solid <= 0) return;
   if (beta != 0.0) {
-    launch_gather<AccelDB>(s, f, tl, ntiles(solid), uh, solid, alpha, beta);
+    launch_gather<AccelDB>(s, f, tl, ntiles(solid), uh, solid, alpha, beta, fold);
   } else {
-    launch_gather<AccelD>(s, f, tl, ntiles(solid), uh, solid, alpha, beta);
+    launch_gather<AccelD>(s, f, tl, ntiles(solid), uh, solid, alpha, beta, fold);
   }
 }
 void launch_accel_integrate_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
                                   double alpha, double beta, float cs32, bool uh, double dt,
-                                  const BcTable& bc, const Frame32& fr, const IntegrateOut& out) {
+                                  const BcTable& bc, const Frame32& fr, const IntegrateOut& out,
+                                  const StatsFold& fold) {
   if (solid <= 0) return;
   const IntegratePrm ip{dt, bc, fr, out};
   auto go = [&](auto tag) {
     typedef decltype(tag) K;
     launch_pair<K>(s, ntiles(solid), f, tl,
-                   typename K::Prm{solid, (float)alpha, (float)beta, cs32, ip});
+                   typename K::Prm{solid, (float)alpha, (float)beta, cs32, ip, fold});
   };
   if (beta != 0.0) uh ? go(AccelG<true, true, true>{}) : go(AccelG<false, true, true>{});
   else uh ? go(AccelG<true, false, true>{}) : go(AccelG<false, false, true>{});
@@ -1750,12 +1771,14 @@ void launch_rates_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int
   launch_gather<RatesG>(s, f, tl, ntiles(solid), uh, solid, xz, cs32);
 }
 void launch_accel_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int solid,
-                        double alpha, double beta, float cs32, bool uh) {
+                        double alpha, double beta, float cs32, bool uh, const StatsFold& fold) {
   if (solid <= 0) return;
   if (beta != 0.0)
-    launch_gather<AccelGB>(s, f, tl, ntiles(solid), uh, solid, (float)alpha, (float)beta, cs32);
+    launch_gather<AccelGB>(s, f, tl, ntiles(solid), uh, solid, (float)alpha, (float)beta, cs32,
+                           IntegratePrm{}, fold);
   else
-    launch_gather<AccelG>(s, f, tl, ntiles(solid), uh, solid, (float)alpha, (float)beta, cs32);
+    launch_gather<AccelG>(s, f, tl, ntiles(solid), uh, solid, (float)alpha, (float)beta, cs32,
+                          IntegratePrm{}, fold);
 }
 void launch_cond_mixed(hipStream_t s, const Fields& f, const TileLists& tl, int n, double T_inf,
                        double h_conv, double dt, bool leap, bool first, float cs32, bool uh) {
