@@ -114,10 +114,20 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Block
   T9 S{}, D{};
   double G = 0.0, sig_trial = 0.0, sigy = 0.0, ep = 0.0, sigeq = 0.0, dep = 0.0, scale = 1.0;
   double esr = 0.0, T = 0.0, rt = 1.0, gmax = 0.0;
+  // the particle's density, sound speed and pressure, read once (the
+  // stores below could alias them, so the compiler would read them again)
+  double rho_i = 0.0, cs_i = 0.0, p_i = 0.0;
   bool newton = false;  // a yielding Johnson-Cook particle under the Newton return map
   if (active) {
     const bool qp = (p.quirks & WF_Q_TENSOR_PRODUCT) != 0;
-    if (FUSE_P) f.p[i] = 0.0 + (f.Cs[i] * f.Cs[i]) * (f.rho[i] - f.rho0[i]);
+    rho_i = f.rho[i];
+    cs_i = f.Cs[i];
+    if (FUSE_P) {
+      p_i = 0.0 + (cs_i * cs_i) * (rho_i - f.rho0[i]);
+      f.p[i] = p_i;
+    } else {
+      p_i = f.p[i];
+    }
     auto ss = [&](int k) { return S32 ? (double)f.ss32[k][i] : f.ss[k][i]; };
     auto dd = [&](int k) { return S32 ? (double)f.strrate32[k][i] : f.strrate[k][i]; };
     auto ww = [&](int k) { return S32 ? (double)f.rotrate32[k][i] : f.rotrate[k][i]; };
@@ -257,7 +267,7 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Block
     f.pl_strain[i] = ep;
     f.sigma_eq[i] = sigeq;
     f.qpl[i] = p.taylor_quinney * sigeq * dep / dt;
-    const double pi = f.p[i];
+    const double pi = p_i;
     // sigma = -p I + S (Mechanical.cu:540); store upper triangle
     const double sg[6] = {-pi + S.xx, -pi + S.yy, -pi + S.zz, S.xy, S.yz, S.xz};
 #pragma unroll
@@ -266,20 +276,20 @@ __global__ __launch_bounds__(kBlock) void k_stress(Fields f, StepParams p, Block
       else f.sigma[k][i] = sg[k];
     }
     if (REC == kRec32) {  // sigma/rho^2, rho and Cs for the FP32 acceleration loop
-      const double rho = f.rho[i], ir2 = 1.0 / (rho * rho);
+      const double rho = rho_i, ir2 = 1.0 / (rho * rho);
       f.arec[rec32_at(i, 0)] = make_float4((float)(sg[0] * ir2), (float)(sg[1] * ir2),
                                            (float)(sg[2] * ir2), (float)(sg[3] * ir2));
       f.arec[rec32_at(i, 1)] =
-          make_float4((float)(sg[4] * ir2), (float)(sg[5] * ir2), (float)rho, (float)f.Cs[i]);
+          make_float4((float)(sg[4] * ir2), (float)(sg[5] * ir2), (float)rho, (float)cs_i);
     }
     if (REC == kRec64) {  // the same in FP64 for the FP64 acceleration loop
-      const double rho = f.rho[i], ir2 = 1.0 / (rho * rho);
+      const double rho = rho_i, ir2 = 1.0 / (rho * rho);
       f.arec64[rec_at(f, i, 1)] =
           make_double4(sg[0] * ir2, sg[1] * ir2, sg[2] * ir2, sg[3] * ir2);
-      f.arec64[rec_at(f, i, 2)] = make_double4(sg[4] * ir2, sg[5] * ir2, rho, f.Cs[i]);
+      f.arec64[rec_at(f, i, 2)] = make_double4(sg[4] * ir2, sg[5] * ir2, rho, cs_i);
     }
     if (ENERGY) {
-      double de = f.m[i] / f.rho[i] *
+      double de = f.m[i] / rho_i *
                   (S.xx * D.xx + S.yy * D.yy + S.zz * D.zz +
                    2.0 * (S.xy * D.xy + S.yz * D.yz + S.xz * D.xz));
       f.eint[i] += de * dt;
